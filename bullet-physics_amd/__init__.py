@@ -1,0 +1,285 @@
+"""MI355X-native collision hot path of bullet-physics (broadphase + convex narrowphase).
+
+Host-side mirror of the reference's interfaces, over the C ABI in
+include/bullet_collision_hip.h:
+
+* :class:`BroadPhase` -- the ``BroadPhase`` trait
+  (crates/bullet-collision/src/broadphase/mod.rs:94-130): proxies with
+  lowest-free-slot reuse, ``compute_pairs`` returning the sorted ``(i, j)``
+  overlap set.
+* :func:`collide` / :func:`collide_seeded` / :func:`collide_batch` --
+  ``narrowphase::collide`` / ``collide_seeded`` (narrowphase/mod.rs:25-286):
+  ``None`` for no contact, else a :class:`ContactManifold` of at most four
+  :class:`ContactPoint` (normal A -> B, position = witness on B,
+  penetration > 0 overlapping).
+* :class:`CollisionWorld` -- the world's per-step detection pass
+  (world/step.rs:349-416, 445-854) with bodies resident on the GPU.
+
+Everything runs through the HIP library; importing works without a GPU, but
+any call needs one (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib, scenes  # noqa: F401
+from ._lib import BchBodies, BchError
+
+SPHERE, BOX, CAPSULE, HULL, PLANE = 0, 1, 2, 3, 4
+
+
+def _check(ctx, rc):
+    if rc != _lib.BCH_OK:
+        msg = _lib.load().bch_last_error(ctx).decode() if ctx else ""
+        raise BchError(rc, msg)
+
+
+class _Context:
+    """One HIP context (device + stream); one per host thread."""
+
+    def __init__(self, device: int = 0):
+        self.L = _lib.load()
+        h = C.c_void_p()
+        rc = self.L.bch_create(device, C.byref(h))
+        if rc != _lib.BCH_OK:
+            raise BchError(rc, "bch_create failed (no HIP device?)")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.bch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context() -> _Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = _Context(0)
+    return _default_ctx
+
+
+# ---------------------------------------------------------------- contacts --
+@dataclass
+class ContactPoint:
+    """manifold.rs:12-39 (position, normal A->B, penetration)."""
+    position: np.ndarray
+    normal: np.ndarray
+    penetration: float
+
+
+@dataclass
+class ContactManifold:
+    points: list
+
+    def flipped(self) -> "ContactManifold":
+        return ContactManifold([ContactPoint(p.position, -p.normal, p.penetration) for p in self.points])
+
+
+class Bodies:
+    """Body arrays kept alive for a C call (see scenes.py for the layout)."""
+
+    def __init__(self, scene: dict):
+        self.pos = np.ascontiguousarray(scene["position"], np.float32).reshape(-1, 3)
+        n = self.pos.shape[0]
+        self.quat = np.ascontiguousarray(scene["orientation"], np.float32).reshape(n, 4)
+        self.type = np.ascontiguousarray(scene["shape_type"], np.uint8).reshape(n)
+        self.params = np.ascontiguousarray(scene["shape_params"], np.float32).reshape(n, 4)
+        self.hull_first = np.ascontiguousarray(scene.get("hull_first", np.zeros(n)), np.uint32).reshape(n)
+        self.hull_count = np.ascontiguousarray(scene.get("hull_count", np.zeros(n)), np.uint32).reshape(n)
+        self.hull_vertices = np.ascontiguousarray(scene.get("hull_vertices", np.zeros((0, 3))),
+                                                  np.float32).reshape(-1, 3)
+        b = BchBodies()
+        b.n = n
+        b.position = self.pos.ctypes.data
+        b.orientation = self.quat.ctypes.data
+        b.shape_type = self.type.ctypes.data
+        b.shape_params = self.params.ctypes.data
+        b.hull_first = self.hull_first.ctypes.data
+        b.hull_count = self.hull_count.ctypes.data
+        b.hull_vertices = self.hull_vertices.ctypes.data if self.hull_vertices.shape[0] else None
+        b.n_hull_vertices = self.hull_vertices.shape[0]
+        self.c = b
+
+
+# -------------------------------------------------------------- broadphase --
+class BroadPhase:
+    """The BroadPhase trait (broadphase/mod.rs:94-130) on the GPU."""
+
+    def __init__(self, ctx: _Context | None = None):
+        self.ctx = ctx or _Context(0)
+        self.L = self.ctx.L
+
+    def create_proxy(self, aabb_min, aabb_max) -> int:
+        mn = np.ascontiguousarray(aabb_min, np.float32)
+        mx = np.ascontiguousarray(aabb_max, np.float32)
+        out = C.c_uint32()
+        _check(self.ctx.h, self.L.bch_bp_create_proxy(self.ctx.h, mn.ctypes.data, mx.ctypes.data, C.byref(out)))
+        return out.value
+
+    def set_aabb(self, proxy: int, aabb_min, aabb_max) -> None:
+        mn = np.ascontiguousarray(aabb_min, np.float32)
+        mx = np.ascontiguousarray(aabb_max, np.float32)
+        _check(self.ctx.h, self.L.bch_bp_set_aabb(self.ctx.h, proxy, mn.ctypes.data, mx.ctypes.data))
+
+    def set_aabbs(self, proxies, mins, maxs) -> None:
+        p = np.ascontiguousarray(proxies, np.uint32)
+        mn = np.ascontiguousarray(mins, np.float32)
+        mx = np.ascontiguousarray(maxs, np.float32)
+        _check(self.ctx.h, self.L.bch_bp_set_aabbs(self.ctx.h, p.shape[0], p.ctypes.data, mn.ctypes.data,
+                                                    mx.ctypes.data))
+
+    def destroy_proxy(self, proxy: int) -> None:
+        _check(self.ctx.h, self.L.bch_bp_destroy_proxy(self.ctx.h, proxy))
+
+    def proxy_aabb(self, proxy: int):
+        mn = np.zeros(3, np.float32)
+        mx = np.zeros(3, np.float32)
+        live = C.c_int()
+        _check(self.ctx.h, self.L.bch_bp_proxy_aabb(self.ctx.h, proxy, mn.ctypes.data, mx.ctypes.data,
+                                                     C.byref(live)))
+        return (mn, mx) if live.value else None
+
+    def compute_pairs(self) -> np.ndarray:
+        """Sorted (i, j), i < j, overlapping live proxies: uint32[P, 2]."""
+        cnt = C.c_uint64()
+        rc = self.L.bch_bp_compute_pairs(self.ctx.h, None, 0, C.byref(cnt))
+        if rc not in (_lib.BCH_OK, _lib.BCH_OVERFLOW):
+            _check(self.ctx.h, rc)
+        out = np.zeros((max(cnt.value, 1), 2), np.uint32)
+        _check(self.ctx.h, self.L.bch_bp_compute_pairs(self.ctx.h, out.ctypes.data, out.shape[0], C.byref(cnt)))
+        return out[:cnt.value]
+
+    def aabb_candidates(self, aabb_min, aabb_max) -> np.ndarray:
+        mn = np.ascontiguousarray(aabb_min, np.float32)
+        mx = np.ascontiguousarray(aabb_max, np.float32)
+        cnt = C.c_uint64()
+        rc = self.L.bch_bp_aabb_candidates(self.ctx.h, mn.ctypes.data, mx.ctypes.data, None, 0, C.byref(cnt))
+        if rc not in (_lib.BCH_OK, _lib.BCH_OVERFLOW):
+            _check(self.ctx.h, rc)
+        out = np.zeros(max(cnt.value, 1), np.uint32)
+        _check(self.ctx.h, self.L.bch_bp_aabb_candidates(self.ctx.h, mn.ctypes.data, mx.ctypes.data,
+                                                          out.ctypes.data, out.shape[0], C.byref(cnt)))
+        return out[:cnt.value]
+
+
+# ------------------------------------------------------------- narrowphase --
+def collide_batch(scene: dict, pairs, seeds=None, ctx: _Context | None = None):
+    """collide_seeded for each (i, j): returns (count u8[P], points f32[P, 4, 7])."""
+    ctx = ctx or default_context()
+    b = Bodies(scene)
+    pr = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+    P = pr.shape[0]
+    cnt = np.zeros(max(P, 1), np.uint8)
+    pts = np.zeros((max(P, 1), 4, 7), np.float32)
+    sd = None if seeds is None else np.ascontiguousarray(seeds, np.float32).reshape(P, 3)
+    _check(ctx.h, ctx.L.bch_collide_batch(ctx.h, C.byref(b.c), pr.ctypes.data, P,
+                                          None if sd is None else sd.ctypes.data, cnt.ctypes.data, pts.ctypes.data))
+    return cnt[:P], pts[:P]
+
+
+def _shape_scene(shapes_poses):
+    """Tiny scene from [(type, params, hull_pts|None, pos, quat), ...]."""
+    n = len(shapes_poses)
+    sc = scenes._empty(n)
+    verts = []
+    off = 0
+    for i, (t, prm, hp, pos, quat) in enumerate(shapes_poses):
+        sc["shape_type"][i] = t
+        prm = list(prm) + [0.0] * (4 - len(prm))
+        sc["shape_params"][i] = prm[:4]
+        sc["position"][i] = pos
+        sc["orientation"][i] = quat
+        if t == HULL:
+            hp = np.asarray(hp, np.float32).reshape(-1, 3)
+            sc["hull_first"][i] = off
+            sc["hull_count"][i] = hp.shape[0]
+            verts.append(hp)
+            off += hp.shape[0]
+    if verts:
+        sc["hull_vertices"] = np.concatenate(verts)
+    return sc
+
+
+def collide_seeded(shape_a, pa, qa, shape_b, pb, qb, seed=None, ctx=None):
+    """narrowphase::collide_seeded for one pair; shapes are (type, params[, hull_points])."""
+    def unpack(s):
+        return (s[0], s[1], s[2] if len(s) > 2 else None)
+    ta, pra, ha = unpack(shape_a)
+    tb, prb, hb = unpack(shape_b)
+    sc = _shape_scene([(ta, pra, ha, pa, qa), (tb, prb, hb, pb, qb)])
+    cnt, pts = collide_batch(sc, np.array([[0, 1]], np.uint32),
+                             None if seed is None else np.asarray(seed, np.float32).reshape(1, 3), ctx)
+    if cnt[0] == 0:
+        return None
+    return ContactManifold([ContactPoint(pts[0, k, :3].copy(), pts[0, k, 3:6].copy(), float(pts[0, k, 6]))
+                            for k in range(cnt[0])])
+
+
+def collide(shape_a, pa, qa, shape_b, pb, qb, ctx=None):
+    """narrowphase::collide (mod.rs:25-47): collide_seeded with seed None."""
+    return collide_seeded(shape_a, pa, qa, shape_b, pb, qb, None, ctx)
+
+
+# ------------------------------------------------------------------- world --
+class CollisionWorld:
+    """Bodies resident in HBM; one detection pass per :meth:`collide`."""
+
+    def __init__(self, scene: dict, ctx: _Context | None = None, batched: bool | None = None):
+        self.ctx = ctx or _Context(0)
+        self.L = self.ctx.L
+        self._bodies = Bodies(scene)
+        self.n = self._bodies.pos.shape[0]
+        wo = scene.get("world_offsets") if batched in (None, True) else None
+        if wo is not None:
+            self._wo = np.ascontiguousarray(wo, np.uint32)
+            _check(self.ctx.h, self.L.bch_world_set_bodies(self.ctx.h, C.byref(self._bodies.c),
+                                                            self._wo.shape[0] - 1, self._wo.ctypes.data))
+        else:
+            _check(self.ctx.h, self.L.bch_world_set_bodies(self.ctx.h, C.byref(self._bodies.c), 1, None))
+        self.n_pairs = 0
+        self.n_contacts = 0
+
+    def set_poses(self, position, orientation) -> None:
+        p = np.ascontiguousarray(position, np.float32)
+        q = np.ascontiguousarray(orientation, np.float32)
+        _check(self.ctx.h, self.L.bch_world_set_poses(self.ctx.h, p.ctypes.data, q.ctypes.data))
+
+    def set_poses_device(self, pos_ptr: int, quat_ptr: int) -> None:
+        _check(self.ctx.h, self.L.bch_world_set_poses_device(self.ctx.h, pos_ptr, quat_ptr))
+
+    def collide(self, aabb_margin: float = 0.02):
+        """Run one pass; returns (n_pairs, n_contacts)."""
+        P = C.c_uint64()
+        Cc = C.c_uint64()
+        _check(self.ctx.h, self.L.bch_world_collide(self.ctx.h, aabb_margin, C.byref(P), C.byref(Cc)))
+        self.n_pairs, self.n_contacts = P.value, Cc.value
+        return self.n_pairs, self.n_contacts
+
+    def results(self):
+        """(pairs u32[P, 2], contact_offsets u32[P+1], contacts f32[C, 7]) of the last pass."""
+        pairs = np.zeros((max(self.n_pairs, 1), 2), np.uint32)
+        offs = np.zeros(self.n_pairs + 1, np.uint32)
+        cont = np.zeros((max(self.n_contacts, 1), 7), np.float32)
+        _check(self.ctx.h, self.L.bch_world_results(self.ctx.h, pairs.ctypes.data, offs.ctypes.data,
+                                                     cont.ctypes.data))
+        return pairs[:self.n_pairs], offs, cont[:self.n_contacts]
+
+    def timings(self):
+        out = np.zeros(7, np.float32)
+        _check(self.ctx.h, self.L.bch_last_timings(self.ctx.h, out.ctypes.data))
+        return out
+
+    def synchronize(self):
+        _check(self.ctx.h, self.L.bch_synchronize(self.ctx.h))

@@ -1,0 +1,72 @@
+"""ctypes binding of libbullet_collision_hip.so (include/bullet_collision_hip.h).
+
+The library is built in-tree (build.py).  Loading fails loudly when it is
+missing: there is no CPU fallback on this path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbullet_collision_hip.so")
+
+BCH_OK, BCH_OVERFLOW, BCH_INVALID_ARG, BCH_HIP_ERROR, BCH_NO_DEVICE = 0, 1, 2, 3, 4
+
+# Every symbol include/bullet_collision_hip.h declares.
+EXPORTS = [
+    "bch_create", "bch_destroy", "bch_last_error", "bch_synchronize", "bch_last_timings",
+    "bch_bp_create_proxy", "bch_bp_set_aabb", "bch_bp_destroy_proxy", "bch_bp_proxy_aabb", "bch_bp_set_aabbs",
+    "bch_bp_compute_pairs", "bch_bp_aabb_candidates", "bch_collide_batch",
+    "bch_world_set_bodies", "bch_world_set_poses", "bch_world_set_poses_device", "bch_world_collide",
+    "bch_world_results", "bch_world_results_device",
+]
+
+
+class BchBodies(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("position", C.c_void_p), ("orientation", C.c_void_p),
+                ("shape_type", C.c_void_p), ("shape_params", C.c_void_p), ("hull_first", C.c_void_p),
+                ("hull_count", C.c_void_p), ("hull_vertices", C.c_void_p), ("n_hull_vertices", C.c_uint32)]
+
+
+class BchError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bullet_collision_hip error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load the in-tree HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built: run python bullet-physics_amd/build.py (needs hipcc)")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, u64, f32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float
+    L.bch_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.bch_destroy.argtypes = [vp]
+    L.bch_destroy.restype = None
+    L.bch_last_error.argtypes = [vp]
+    L.bch_last_error.restype = C.c_char_p
+    L.bch_synchronize.argtypes = [vp]
+    L.bch_last_timings.argtypes = [vp, vp]
+    L.bch_bp_create_proxy.argtypes = [vp, vp, vp, C.POINTER(u32)]
+    L.bch_bp_set_aabb.argtypes = [vp, u32, vp, vp]
+    L.bch_bp_destroy_proxy.argtypes = [vp, u32]
+    L.bch_bp_proxy_aabb.argtypes = [vp, u32, vp, vp, C.POINTER(C.c_int)]
+    L.bch_bp_set_aabbs.argtypes = [vp, u32, vp, vp, vp]
+    L.bch_bp_compute_pairs.argtypes = [vp, vp, u64, C.POINTER(u64)]
+    L.bch_bp_aabb_candidates.argtypes = [vp, vp, vp, vp, u64, C.POINTER(u64)]
+    L.bch_collide_batch.argtypes = [vp, C.POINTER(BchBodies), vp, u64, vp, vp, vp]
+    L.bch_world_set_bodies.argtypes = [vp, C.POINTER(BchBodies), u32, vp]
+    L.bch_world_set_poses.argtypes = [vp, vp, vp]
+    L.bch_world_set_poses_device.argtypes = [vp, vp, vp]
+    L.bch_world_collide.argtypes = [vp, f32, C.POINTER(u64), C.POINTER(u64)]
+    L.bch_world_results.argtypes = [vp, vp, vp, vp]
+    L.bch_world_results_device.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
+    _lib = L
+    return L

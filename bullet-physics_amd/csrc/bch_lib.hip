@@ -1,0 +1,684 @@
+// bch_lib.hip -- context, pass orchestration and the C ABI
+// (include/bullet_collision_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/bullet_collision_hip.h"
+#include "bch_kernels.hpp"
+
+using namespace bch;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  bool ensure(size_t need) {
+    if (need <= bytes) return true;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t nb = std::max<size_t>(need + need / 4, 256);
+    if (hipMalloc(&p, nb) != hipSuccess) {
+      p = nullptr;
+      return false;
+    }
+    bytes = nb;
+    return true;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+constexpr int kBlock = 256;
+constexpr int kEpaGrid = 1024;  // blocks of 64 lanes, one EPA store per lane
+inline uint32_t blocks(uint64_t n, int b = kBlock) { return (uint32_t)((n + b - 1) / b); }
+
+}  // namespace
+
+struct bch_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  hipEvent_t ev[8] = {};
+  float timings[7] = {0, 0, 0, 0, 0, 0, 0};
+
+  // world-resident bodies
+  uint32_t n = 0, n_worlds = 1, max_world = 0;
+  bool batched = false;
+  DevBuf bodies, hull, world_offsets, world_of, staging_a, staging_b, staging_c, staging_d, staging_e, staging_f;
+
+  // pass buffers
+  DevBuf aabb, flags, big_ids, counters, keys, keys_alt, vals, vals_alt, nodes, parent, refit, node_first, wend;
+  DevBuf leaf_counts, leaf_offsets, slots, pair_keys, pair_keys_alt, big_pairs, pairs, cls, cls_alt, perm,
+      perm_alt, ccount, coffsets, cslots, contacts, epa_q, epa_v, epa_f, epa_e, tmp;
+  uint32_t K = 16;
+  uint64_t P = 0, C = 0;
+  Counters host_cnt{};
+
+  // proxy API state (host mirror of the broadphase slots)
+  std::vector<float> proxy_box;  // 6 per slot
+  std::vector<uint8_t> proxy_live;
+};
+
+namespace {
+
+int fail(bch_context* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                          \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail(ctx, BCH_HIP_ERROR, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+#define ENSURE(ctx, buf, bytes)                                                      \
+  do {                                                                               \
+    if (!(buf).ensure(bytes)) return fail(ctx, BCH_HIP_ERROR, "device allocation failed"); \
+  } while (0)
+
+template <class F>
+int with_tmp(bch_context* c, F f) {
+  size_t need = 0;
+  if (f(nullptr, need) != hipSuccess) return fail(c, BCH_HIP_ERROR, "rocprim size query failed");
+  ENSURE(c, c->tmp, need);
+  size_t have = c->tmp.bytes;
+  if (f(c->tmp.p, have) != hipSuccess) return fail(c, BCH_HIP_ERROR, "rocprim call failed");
+  return BCH_OK;
+}
+
+int bits_for(uint64_t v) {
+  int b = 1;
+  while ((1ull << b) <= v) ++b;
+  return b;
+}
+
+// Broadphase over the AABBs already in c->aabb (n boxes).  Leaves the sorted
+// pair keys in c->pair_keys and c->P set.
+int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const uint32_t* d_world_offsets,
+                   bool given_boxes) {
+  hipStream_t s = c->stream;
+  Counters* cnt = c->counters.as<Counters>();
+  if (given_boxes) {
+    hipLaunchKernelGGL(k_aabb_given, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->aabb.as<float4>(),
+                       c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(), cnt);
+  }
+  hipLaunchKernelGGL(k_morton, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->aabb.as<float4>(), c->flags.as<uint8_t>(),
+                     d_world_of, cnt, c->keys.as<uint64_t>(), c->vals.as<uint32_t>());
+  int wbits = d_world_of ? bits_for(c->n_worlds) : 0;
+  int end_bit = std::min(64, 31 + wbits);
+  // all-ones keys of excluded bodies still sort last within end_bit bits
+  int rc = with_tmp(c, [&](void* t, size_t& sz) {
+    return rocprim::radix_sort_pairs(t, sz, c->keys.as<uint64_t>(), c->keys_alt.as<uint64_t>(),
+                                     c->vals.as<uint32_t>(), c->vals_alt.as<uint32_t>(), (size_t)n, 0, end_bit, s);
+  });
+  if (rc) return rc;
+  const uint64_t* skeys = c->keys_alt.as<uint64_t>();
+  const uint32_t* order = c->vals_alt.as<uint32_t>();
+  uint32_t* nfirst = d_world_of ? c->node_first.as<uint32_t>() : nullptr;
+  hipLaunchKernelGGL(k_leaves, dim3(blocks(n)), dim3(kBlock), 0, s, n, cnt, skeys, order, c->aabb.as<float4>(),
+                     c->nodes.as<Node>(), nfirst);
+  hipLaunchKernelGGL(k_build, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, skeys, c->nodes.as<Node>(),
+                     c->parent.as<int>(), c->refit.as<uint32_t>(), nfirst);
+  hipLaunchKernelGGL(k_refit, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, c->nodes.as<Node>(), c->parent.as<int>(),
+                     c->refit.as<uint32_t>());
+  uint32_t* wend = nullptr;
+  if (d_world_of) {
+    wend = c->wend.as<uint32_t>();
+    hipLaunchKernelGGL(k_world_end, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, skeys, wend);
+  }
+  (void)hipEventRecord(c->ev[2], s);
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    ENSURE(c, c->slots, (size_t)n * c->K * sizeof(uint64_t));
+    hipLaunchKernelGGL(k_traverse, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, c->nodes.as<Node>(), nfirst, wend,
+                       c->K, c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), cnt);
+    (void)hipEventRecord(c->ev[3], s);
+    HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    if (c->host_cnt.max_leaf <= c->K) break;
+    c->K = std::max<uint32_t>(c->K * 2, (c->host_cnt.max_leaf + 7) & ~7u);
+  }
+  if (c->host_cnt.max_leaf > c->K) return fail(c, BCH_HIP_ERROR, "per-leaf pair slots did not converge");
+  uint32_t nreg = c->host_cnt.n_reg;
+  // prefix offsets of the per-leaf counts
+  rc = with_tmp(c, [&](void* t, size_t& sz) {
+    return rocprim::exclusive_scan(t, sz, c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), 0u,
+                                   (size_t)nreg + 1, rocprim::plus<uint32_t>(), s);
+  });
+  if (rc) return rc;
+  uint32_t preg = 0;
+  HIP_TRY(c, hipMemcpyAsync(&preg, c->leaf_offsets.as<uint32_t>() + nreg, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  // big bodies (planes): brute force against their world
+  uint32_t nbig = c->host_cnt.n_big;
+  uint64_t big_cap = 0;
+  if (nbig) {
+    big_cap = std::max<uint64_t>((uint64_t)nbig * (c->batched ? c->max_world : n), 1024);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      ENSURE(c, c->big_pairs, big_cap * sizeof(uint64_t));
+      hipLaunchKernelGGL(k_reset_big, dim3(1), dim3(64), 0, s, cnt);
+      uint32_t gy = std::min<uint32_t>(nbig, 65535u);
+      uint32_t span = c->batched ? c->max_world : n;
+      uint32_t gx = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(span), 1024u));
+      hipLaunchKernelGGL(k_big, dim3(gx, gy), dim3(kBlock), 0, s, cnt, c->big_ids.as<uint32_t>(), c->aabb.as<float4>(),
+                         c->flags.as<uint8_t>(), c->bodies.as<BodyRec>(), d_world_offsets, n,
+                         c->big_pairs.as<uint64_t>(), (uint32_t)std::min<uint64_t>(big_cap, 0xffffffffu), cnt);
+      HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+      HIP_TRY(c, hipStreamSynchronize(s));
+      if (c->host_cnt.n_big_pairs <= big_cap) break;
+      big_cap = c->host_cnt.n_big_pairs;
+    }
+  }
+  uint64_t pbig = c->host_cnt.n_big_pairs;
+  c->P = (uint64_t)preg + pbig;
+  ENSURE(c, c->pair_keys, std::max<uint64_t>(c->P, 1) * sizeof(uint64_t));
+  ENSURE(c, c->pair_keys_alt, std::max<uint64_t>(c->P, 1) * sizeof(uint64_t));
+  hipLaunchKernelGGL(k_compact_slots, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, c->K, c->slots.as<uint64_t>(),
+                     c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), c->pair_keys.as<uint64_t>());
+  if (pbig)
+    HIP_TRY(c, hipMemcpyAsync(c->pair_keys.as<uint64_t>() + preg, c->big_pairs.p, pbig * sizeof(uint64_t),
+                              hipMemcpyDeviceToDevice, s));
+  if (c->P > 1) {
+    int eb = 32 + bits_for(n);
+    rc = with_tmp(c, [&](void* t, size_t& sz) {
+      return rocprim::radix_sort_keys(t, sz, c->pair_keys.as<uint64_t>(), c->pair_keys_alt.as<uint64_t>(),
+                                      (size_t)c->P, 0, eb, s);
+    });
+    if (rc) return rc;
+    std::swap(c->pair_keys, c->pair_keys_alt);
+  }
+  return BCH_OK;
+}
+
+int ensure_pass_buffers(bch_context* c, uint32_t n) {
+  ENSURE(c, c->aabb, (size_t)n * 2 * sizeof(float4));
+  ENSURE(c, c->flags, (size_t)n);
+  ENSURE(c, c->big_ids, (size_t)n * sizeof(uint32_t));
+  ENSURE(c, c->counters, sizeof(Counters));
+  ENSURE(c, c->keys, (size_t)n * sizeof(uint64_t));
+  ENSURE(c, c->keys_alt, (size_t)n * sizeof(uint64_t));
+  ENSURE(c, c->vals, (size_t)n * sizeof(uint32_t));
+  ENSURE(c, c->vals_alt, (size_t)n * sizeof(uint32_t));
+  ENSURE(c, c->nodes, (size_t)(2 * n + 1) * sizeof(Node));
+  ENSURE(c, c->parent, (size_t)(2 * n + 1) * sizeof(int));
+  ENSURE(c, c->refit, (size_t)(n + 1) * sizeof(uint32_t));
+  ENSURE(c, c->node_first, (size_t)(2 * n + 1) * sizeof(uint32_t));
+  ENSURE(c, c->wend, (size_t)(n + 1) * sizeof(uint32_t));
+  ENSURE(c, c->leaf_counts, (size_t)(n + 1) * sizeof(uint32_t));
+  ENSURE(c, c->leaf_offsets, (size_t)(n + 1) * sizeof(uint32_t));
+  return BCH_OK;
+}
+
+// Narrowphase over c->pair_keys (sorted, c->P pairs) with the world bodies.
+// Fills c->pairs (2P), c->coffsets (P+1), c->contacts (7C), c->C.
+int run_narrowphase(bch_context* c, const float* d_seeds) {
+  hipStream_t s = c->stream;
+  Counters* cnt = c->counters.as<Counters>();
+  uint64_t P = c->P;
+  size_t Pm = std::max<uint64_t>(P, 1);
+  ENSURE(c, c->pairs, Pm * 2 * sizeof(uint32_t));
+  ENSURE(c, c->cls, Pm * sizeof(uint32_t));
+  ENSURE(c, c->cls_alt, Pm * sizeof(uint32_t));
+  ENSURE(c, c->perm, Pm * sizeof(uint32_t));
+  ENSURE(c, c->perm_alt, Pm * sizeof(uint32_t));
+  ENSURE(c, c->ccount, (Pm + 1) * sizeof(uint32_t));
+  ENSURE(c, c->coffsets, (Pm + 1) * sizeof(uint32_t));
+  ENSURE(c, c->cslots, Pm * 8 * sizeof(float4));
+  ENSURE(c, c->epa_q, Pm * 4 * sizeof(float4));
+  ENSURE(c, c->epa_v, (size_t)kEpaGrid * 64 * BCH_EPA_MAX_VERTS * sizeof(float4));
+  ENSURE(c, c->epa_f, (size_t)kEpaGrid * 64 * BCH_EPA_MAX_FACES * sizeof(EpaFace));
+  ENSURE(c, c->epa_e, (size_t)kEpaGrid * 64 * BCH_EPA_MAX_EDGES * sizeof(uint32_t));
+  hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
+  if (P) {
+    hipLaunchKernelGGL(k_split_pairs, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->pair_keys.as<uint64_t>(),
+                       c->bodies.as<BodyRec>(), c->pairs.as<uint32_t>(), c->cls.as<uint32_t>(), c->perm.as<uint32_t>());
+    int rc = with_tmp(c, [&](void* t, size_t& sz) {
+      return rocprim::radix_sort_pairs(t, sz, c->cls.as<uint32_t>(), c->cls_alt.as<uint32_t>(), c->perm.as<uint32_t>(),
+                                       c->perm_alt.as<uint32_t>(), (size_t)P, 0, 3, s);
+    });
+    if (rc) return rc;
+    (void)hipEventRecord(c->ev[4], s);
+    hipLaunchKernelGGL(k_narrow, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->perm_alt.as<uint32_t>(),
+                       c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_seeds,
+                       c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->epa_q.as<float4>(), cnt);
+    (void)hipEventRecord(c->ev[5], s);
+    hipLaunchKernelGGL(k_epa, dim3(kEpaGrid), dim3(64), 0, s, cnt, c->epa_q.as<float4>(), c->pairs.as<uint32_t>(),
+                       c->bodies.as<BodyRec>(), c->hull.as<float4>(), c->ccount.as<uint32_t>(), c->cslots.as<float4>(),
+                       c->epa_v.as<float4>(), c->epa_f.as<EpaFace>(), c->epa_e.as<uint32_t>(), cnt);
+    (void)hipEventRecord(c->ev[6], s);
+  } else {
+    (void)hipEventRecord(c->ev[4], s);
+    (void)hipEventRecord(c->ev[5], s);
+    (void)hipEventRecord(c->ev[6], s);
+  }
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, 1u, c->ccount.as<uint32_t>() + P, 0u);
+  int rc = with_tmp(c, [&](void* t, size_t& sz) {
+    return rocprim::exclusive_scan(t, sz, c->ccount.as<uint32_t>(), c->coffsets.as<uint32_t>(), 0u, (size_t)P + 1,
+                                   rocprim::plus<uint32_t>(), s);
+  });
+  if (rc) return rc;
+  uint32_t total = 0;
+  HIP_TRY(c, hipMemcpyAsync(&total, c->coffsets.as<uint32_t>() + P, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  c->C = total;
+  ENSURE(c, c->contacts, std::max<uint64_t>(c->C, 1) * 7 * sizeof(float));
+  if (P)
+    hipLaunchKernelGGL(k_gather, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->ccount.as<uint32_t>(),
+                       c->coffsets.as<uint32_t>(), c->cslots.as<float4>(), c->contacts.as<float>());
+  (void)hipEventRecord(c->ev[7], s);
+  HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  if (c->host_cnt.errors & 2u) return fail(c, BCH_HIP_ERROR, "EPA polytope store overflow");
+  return BCH_OK;
+}
+
+int upload_bodies(bch_context* c, const bch_bodies* b, const uint32_t* world_of_host) {
+  hipStream_t s = c->stream;
+  uint32_t n = b->n;
+  ENSURE(c, c->bodies, (size_t)std::max<uint32_t>(n, 1) * sizeof(BodyRec));
+  ENSURE(c, c->staging_a, (size_t)std::max<uint32_t>(n, 1) * 3 * sizeof(float));
+  ENSURE(c, c->staging_b, (size_t)std::max<uint32_t>(n, 1) * 4 * sizeof(float));
+  ENSURE(c, c->staging_c, (size_t)std::max<uint32_t>(n, 1));
+  ENSURE(c, c->staging_d, (size_t)std::max<uint32_t>(n, 1) * 4 * sizeof(float));
+  ENSURE(c, c->staging_e, (size_t)std::max<uint32_t>(n, 1) * 2 * sizeof(uint32_t));
+  uint32_t nv = b->hull_vertices ? b->n_hull_vertices : 0;
+  ENSURE(c, c->hull, (size_t)std::max<uint32_t>(nv, 1) * sizeof(float4));
+  ENSURE(c, c->staging_f, (size_t)std::max<uint32_t>(nv, 1) * 3 * sizeof(float));
+  if (n) {
+    HIP_TRY(c, hipMemcpyAsync(c->staging_a.p, b->position, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(c->staging_b.p, b->orientation, (size_t)n * 4 * sizeof(float), hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(c->staging_c.p, b->shape_type, (size_t)n, hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(c->staging_d.p, b->shape_params, (size_t)n * 4 * sizeof(float), hipMemcpyHostToDevice, s));
+  }
+  uint32_t* hf = nullptr;
+  uint32_t* hc = nullptr;
+  if (nv && n) {
+    if (!b->hull_first || !b->hull_count) return fail(c, BCH_INVALID_ARG, "hull vertices without hull_first/count");
+    hf = c->staging_e.as<uint32_t>();
+    hc = hf + n;
+    HIP_TRY(c, hipMemcpyAsync(hf, b->hull_first, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(hc, b->hull_count, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(c->staging_f.p, b->hull_vertices, (size_t)nv * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_xyz_to_float4, dim3(blocks(nv)), dim3(kBlock), 0, s, nv, c->staging_f.as<float>(),
+                       c->hull.as<float4>());
+  }
+  uint32_t* wo = nullptr;
+  if (world_of_host) {
+    ENSURE(c, c->world_of, (size_t)std::max<uint32_t>(n, 1) * sizeof(uint32_t));
+    HIP_TRY(c, hipMemcpyAsync(c->world_of.p, world_of_host, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    wo = c->world_of.as<uint32_t>();
+  }
+  if (n)
+    hipLaunchKernelGGL(k_pack_bodies, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->staging_a.as<float>(),
+                       c->staging_b.as<float>(), c->staging_c.as<uint8_t>(), c->staging_d.as<float>(), hf, hc,
+                       c->hull.as<float4>(), wo, c->bodies.as<BodyRec>());
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return BCH_OK;
+}
+
+int validate_bodies(bch_context* c, const bch_bodies* b) {
+  if (!b) return fail(c, BCH_INVALID_ARG, "bodies is NULL");
+  if (b->n && (!b->position || !b->orientation || !b->shape_type || !b->shape_params))
+    return fail(c, BCH_INVALID_ARG, "body arrays must be non-NULL");
+  if (b->n > 0x7fffffffu) return fail(c, BCH_INVALID_ARG, "too many bodies");
+  for (uint32_t i = 0; i < b->n; ++i) {
+    uint8_t t = b->shape_type[i];
+    if (t > BCH_PLANE) return fail(c, BCH_INVALID_ARG, "unknown shape type at body " + std::to_string(i));
+    if (t == BCH_HULL) {
+      if (!b->hull_first || !b->hull_count || !b->hull_vertices)
+        return fail(c, BCH_INVALID_ARG, "hull body without hull arrays");
+      uint64_t end = (uint64_t)b->hull_first[i] + b->hull_count[i];
+      if (end > b->n_hull_vertices) return fail(c, BCH_INVALID_ARG, "hull range out of bounds at body " + std::to_string(i));
+    }
+  }
+  return BCH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bch_create(int device, bch_context** out) {
+  if (!out) return BCH_INVALID_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) return BCH_NO_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return BCH_NO_DEVICE;
+  bch_context* c = new bch_context();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return BCH_HIP_ERROR;
+  }
+  for (auto& e : c->ev) (void)hipEventCreate(&e);
+  *out = c;
+  return BCH_OK;
+}
+
+void bch_destroy(bch_context* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  DevBuf* all[] = {&c->bodies, &c->hull, &c->world_offsets, &c->world_of, &c->staging_a, &c->staging_b,
+                   &c->staging_c, &c->staging_d, &c->staging_e, &c->staging_f, &c->aabb, &c->flags, &c->big_ids,
+                   &c->counters, &c->keys, &c->keys_alt, &c->vals, &c->vals_alt, &c->nodes, &c->parent, &c->refit,
+                   &c->node_first, &c->wend, &c->leaf_counts, &c->leaf_offsets, &c->slots, &c->pair_keys,
+                   &c->pair_keys_alt, &c->big_pairs, &c->pairs, &c->cls, &c->cls_alt, &c->perm, &c->perm_alt,
+                   &c->ccount, &c->coffsets, &c->cslots, &c->contacts, &c->epa_q, &c->epa_v, &c->epa_f, &c->epa_e,
+                   &c->tmp};
+  for (DevBuf* b : all) b->release();
+  for (auto& e : c->ev) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* bch_last_error(const bch_context* c) { return c ? c->err.c_str() : "no context"; }
+
+int bch_synchronize(bch_context* c) {
+  if (!c) return BCH_INVALID_ARG;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return BCH_OK;
+}
+
+int bch_last_timings(bch_context* c, float* out_ms) {
+  if (!c || !out_ms) return BCH_INVALID_ARG;
+  for (int i = 0; i < 7; ++i) out_ms[i] = c->timings[i];
+  return BCH_OK;
+}
+
+// ------------------------------------------------------------ proxies --
+int bch_bp_create_proxy(bch_context* c, const float* mn, const float* mx, uint32_t* out) {
+  if (!c || !mn || !mx || !out) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  size_t slot = c->proxy_live.size();
+  for (size_t i = 0; i < c->proxy_live.size(); ++i)
+    if (!c->proxy_live[i]) {
+      slot = i;
+      break;
+    }
+  if (slot == c->proxy_live.size()) {
+    c->proxy_live.push_back(0);
+    c->proxy_box.resize(c->proxy_box.size() + 6);
+  }
+  c->proxy_live[slot] = 1;
+  memcpy(&c->proxy_box[6 * slot], mn, 3 * sizeof(float));
+  memcpy(&c->proxy_box[6 * slot + 3], mx, 3 * sizeof(float));
+  *out = (uint32_t)slot;
+  return BCH_OK;
+}
+
+int bch_bp_set_aabb(bch_context* c, uint32_t proxy, const float* mn, const float* mx) {
+  if (!c || !mn || !mx) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  if (proxy >= c->proxy_live.size() || !c->proxy_live[proxy]) return BCH_OK;
+  memcpy(&c->proxy_box[6 * proxy], mn, 3 * sizeof(float));
+  memcpy(&c->proxy_box[6 * proxy + 3], mx, 3 * sizeof(float));
+  return BCH_OK;
+}
+
+int bch_bp_set_aabbs(bch_context* c, uint32_t count, const uint32_t* proxies, const float* mins, const float* maxs) {
+  if (!c || (count && (!proxies || !mins || !maxs))) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  for (uint32_t k = 0; k < count; ++k) bch_bp_set_aabb(c, proxies[k], mins + 3 * k, maxs + 3 * k);
+  return BCH_OK;
+}
+
+int bch_bp_destroy_proxy(bch_context* c, uint32_t proxy) {
+  if (!c) return BCH_INVALID_ARG;
+  if (proxy < c->proxy_live.size()) c->proxy_live[proxy] = 0;
+  return BCH_OK;
+}
+
+int bch_bp_proxy_aabb(bch_context* c, uint32_t proxy, float* mn, float* mx, int* live) {
+  if (!c || !live) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  *live = proxy < c->proxy_live.size() && c->proxy_live[proxy];
+  if (*live) {
+    if (mn) memcpy(mn, &c->proxy_box[6 * proxy], 3 * sizeof(float));
+    if (mx) memcpy(mx, &c->proxy_box[6 * proxy + 3], 3 * sizeof(float));
+  }
+  return BCH_OK;
+}
+
+int bch_bp_compute_pairs(bch_context* c, uint32_t* out_pairs, uint64_t capacity, uint64_t* out_count) {
+  if (!c || !out_count) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  HIP_TRY(c, hipSetDevice(c->device));
+  uint32_t n = (uint32_t)c->proxy_live.size();
+  *out_count = 0;
+  if (n == 0) return BCH_OK;
+  int rc = ensure_pass_buffers(c, n);
+  if (rc) return rc;
+  std::vector<float4> boxes(2 * (size_t)n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* b = &c->proxy_box[6 * i];
+    if (c->proxy_live[i]) {
+      boxes[2 * i] = make_float4(b[0], b[1], b[2], 0.0f);
+      boxes[2 * i + 1] = make_float4(b[3], b[4], b[5], 0.0f);
+    } else {
+      boxes[2 * i] = boxes[2 * i + 1] = make_float4(NAN, NAN, NAN, 0.0f);
+    }
+  }
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, c->counters.as<Counters>());
+  HIP_TRY(c, hipMemcpyAsync(c->aabb.p, boxes.data(), boxes.size() * sizeof(float4), hipMemcpyHostToDevice, s));
+  bool saved = c->batched;
+  c->batched = false;
+  rc = run_broadphase(c, n, nullptr, nullptr, true);
+  c->batched = saved;
+  if (rc) return rc;
+  *out_count = c->P;
+  if (c->P > capacity || (!out_pairs && c->P)) return BCH_OVERFLOW;
+  if (c->P) {
+    std::vector<uint64_t> keys(c->P);
+    HIP_TRY(c, hipMemcpyAsync(keys.data(), c->pair_keys.p, c->P * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    for (uint64_t k = 0; k < c->P; ++k) {
+      out_pairs[2 * k] = (uint32_t)(keys[k] >> 32);
+      out_pairs[2 * k + 1] = (uint32_t)keys[k];
+    }
+  }
+  return BCH_OK;
+}
+
+int bch_bp_aabb_candidates(bch_context* c, const float* mn, const float* mx, uint32_t* out, uint64_t capacity,
+                           uint64_t* out_count) {
+  if (!c || !mn || !mx || !out_count) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  // Host scan over the proxy arena: the exact query of
+  // broadphase/mod.rs:135-146 (one box against every live proxy).
+  uint64_t k = 0;
+  Box6 q{V3{mn[0], mn[1], mn[2]}, V3{mx[0], mx[1], mx[2]}};
+  for (size_t i = 0; i < c->proxy_live.size(); ++i) {
+    if (!c->proxy_live[i]) continue;
+    const float* b = &c->proxy_box[6 * i];
+    Box6 o{V3{b[0], b[1], b[2]}, V3{b[3], b[4], b[5]}};
+    if (overlaps(o, q)) {
+      if (out && k < capacity) out[k] = (uint32_t)i;
+      ++k;
+    }
+  }
+  *out_count = k;
+  return k > capacity ? BCH_OVERFLOW : BCH_OK;
+}
+
+// ------------------------------------------------------- collide_batch --
+int bch_collide_batch(bch_context* c, const bch_bodies* bodies, const uint32_t* pairs, uint64_t n_pairs,
+                      const float* seeds, uint8_t* out_count, float* out_points) {
+  if (!c) return BCH_INVALID_ARG;
+  int rc = validate_bodies(c, bodies);
+  if (rc) return rc;
+  if (n_pairs && (!pairs || !out_count || !out_points)) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  for (uint64_t k = 0; k < 2 * n_pairs; ++k)
+    if (pairs[k] >= bodies->n) return fail(c, BCH_INVALID_ARG, "pair index out of range");
+  HIP_TRY(c, hipSetDevice(c->device));
+  rc = upload_bodies(c, bodies, nullptr);
+  if (rc) return rc;
+  c->n = bodies->n;
+  c->batched = false;
+  if (!n_pairs) return BCH_OK;
+  hipStream_t s = c->stream;
+  ENSURE(c, c->counters, sizeof(Counters));
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, c->counters.as<Counters>());
+  // Keys in caller order: the narrowphase consumes them unsorted.
+  c->P = n_pairs;
+  ENSURE(c, c->pair_keys, n_pairs * sizeof(uint64_t));
+  std::vector<uint64_t> keys(n_pairs);
+  for (uint64_t k = 0; k < n_pairs; ++k) keys[k] = ((uint64_t)pairs[2 * k] << 32) | pairs[2 * k + 1];
+  HIP_TRY(c, hipMemcpyAsync(c->pair_keys.p, keys.data(), n_pairs * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  float* d_seeds = nullptr;
+  if (seeds) {
+    ENSURE(c, c->staging_a, std::max<size_t>(c->staging_a.bytes, n_pairs * 3 * sizeof(float)));
+    d_seeds = c->staging_a.as<float>();
+    HIP_TRY(c, hipMemcpyAsync(d_seeds, seeds, n_pairs * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+  }
+  rc = run_narrowphase(c, d_seeds);
+  if (rc) return rc;
+  if (c->host_cnt.errors & 1u) return fail(c, BCH_HIP_ERROR, "box-box clip polygon overflow");
+  std::vector<uint32_t> cnt(n_pairs);
+  std::vector<float4> slots(n_pairs * 8);
+  HIP_TRY(c, hipMemcpyAsync(cnt.data(), c->ccount.p, n_pairs * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(slots.data(), c->cslots.p, n_pairs * 8 * sizeof(float4), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  for (uint64_t k = 0; k < n_pairs; ++k) {
+    out_count[k] = (uint8_t)cnt[k];
+    for (uint32_t q = 0; q < 4; ++q) {
+      float* o = out_points + 28 * k + 7 * q;
+      if (q < cnt[k]) {
+        float4 a = slots[8 * k + 2 * q], b = slots[8 * k + 2 * q + 1];
+        o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = b.x; o[4] = b.y; o[5] = b.z; o[6] = a.w;
+      } else {
+        for (int z = 0; z < 7; ++z) o[z] = 0.0f;
+      }
+    }
+  }
+  return BCH_OK;
+}
+
+// ------------------------------------------------------ world pipeline --
+int bch_world_set_bodies(bch_context* c, const bch_bodies* bodies, uint32_t n_worlds, const uint32_t* world_offsets) {
+  if (!c) return BCH_INVALID_ARG;
+  int rc = validate_bodies(c, bodies);
+  if (rc) return rc;
+  HIP_TRY(c, hipSetDevice(c->device));
+  std::vector<uint32_t> world_of;
+  uint32_t max_world = bodies->n;
+  if (world_offsets) {
+    max_world = 0;
+    if (n_worlds == 0 || world_offsets[0] != 0 || world_offsets[n_worlds] != bodies->n)
+      return fail(c, BCH_INVALID_ARG, "world_offsets must start at 0 and end at n");
+    world_of.resize(bodies->n);
+    for (uint32_t w = 0; w < n_worlds; ++w) {
+      if (world_offsets[w + 1] < world_offsets[w]) return fail(c, BCH_INVALID_ARG, "world_offsets not monotone");
+      max_world = std::max<uint32_t>(max_world, world_offsets[w + 1] - world_offsets[w]);
+      for (uint32_t i = world_offsets[w]; i < world_offsets[w + 1]; ++i) world_of[i] = w;
+    }
+  } else if (n_worlds > 1) {
+    return fail(c, BCH_INVALID_ARG, "n_worlds > 1 needs world_offsets");
+  }
+  rc = upload_bodies(c, bodies, world_offsets ? world_of.data() : nullptr);
+  if (rc) return rc;
+  c->n = bodies->n;
+  c->n_worlds = world_offsets ? n_worlds : 1;
+  c->batched = world_offsets != nullptr;
+  c->max_world = max_world;
+  if (world_offsets) {
+    ENSURE(c, c->world_offsets, (size_t)(n_worlds + 1) * sizeof(uint32_t));
+    HIP_TRY(c, hipMemcpy(c->world_offsets.p, world_offsets, (size_t)(n_worlds + 1) * sizeof(uint32_t),
+                         hipMemcpyHostToDevice));
+  }
+  return ensure_pass_buffers(c, std::max<uint32_t>(c->n, 1));
+}
+
+int bch_world_set_poses(bch_context* c, const float* position, const float* orientation) {
+  if (!c || (c->n && (!position || !orientation))) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  if (!c->n) return BCH_OK;
+  HIP_TRY(c, hipMemcpyAsync(c->staging_a.p, position, (size_t)c->n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(c->staging_b.p, orientation, (size_t)c->n * 4 * sizeof(float), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_pack_poses, dim3(blocks(c->n)), dim3(kBlock), 0, s, c->n, c->staging_a.as<float>(),
+                     c->staging_b.as<float>(), c->bodies.as<BodyRec>());
+  HIP_TRY(c, hipGetLastError());
+  return BCH_OK;
+}
+
+int bch_world_set_poses_device(bch_context* c, const float* d_position, const float* d_orientation) {
+  if (!c || (c->n && (!d_position || !d_orientation))) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (!c->n) return BCH_OK;
+  hipLaunchKernelGGL(k_pack_poses, dim3(blocks(c->n)), dim3(kBlock), 0, c->stream, c->n, d_position, d_orientation,
+                     c->bodies.as<BodyRec>());
+  HIP_TRY(c, hipGetLastError());
+  return BCH_OK;
+}
+
+int bch_world_collide(bch_context* c, float aabb_margin, uint64_t* out_pairs, uint64_t* out_contacts) {
+  if (!c) return BCH_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  uint32_t n = c->n;
+  hipStream_t s = c->stream;
+  c->P = 0;
+  c->C = 0;
+  int rc = ensure_pass_buffers(c, std::max<uint32_t>(n, 1));
+  if (rc) return rc;
+  (void)hipEventRecord(c->ev[0], s);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, c->counters.as<Counters>());
+  if (n) {
+    hipLaunchKernelGGL(k_aabb, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->bodies.as<BodyRec>(), c->hull.as<float4>(),
+                       aabb_margin, c->aabb.as<float4>(), c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(),
+                       c->counters.as<Counters>());
+    (void)hipEventRecord(c->ev[1], s);
+    rc = run_broadphase(c, n, c->batched ? c->world_of.as<uint32_t>() : nullptr,
+                        c->batched ? c->world_offsets.as<uint32_t>() : nullptr, false);
+    if (rc) return rc;
+  } else {
+    for (int k = 1; k <= 3; ++k) (void)hipEventRecord(c->ev[k], s);
+  }
+  rc = run_narrowphase(c, nullptr);
+  if (rc) return rc;
+  HIP_TRY(c, hipGetLastError());
+  for (int k = 0; k < 7; ++k) (void)hipEventElapsedTime(&c->timings[k], c->ev[k], c->ev[k + 1]);
+  if (c->host_cnt.errors & 1u) return fail(c, BCH_HIP_ERROR, "box-box clip polygon overflow");
+  if (out_pairs) *out_pairs = c->P;
+  if (out_contacts) *out_contacts = c->C;
+  return BCH_OK;
+}
+
+int bch_world_results(bch_context* c, uint32_t* pairs, uint32_t* offsets, float* contacts) {
+  if (!c) return BCH_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  if (pairs && c->P)
+    HIP_TRY(c, hipMemcpyAsync(pairs, c->pairs.p, c->P * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (offsets)
+    HIP_TRY(c, hipMemcpyAsync(offsets, c->coffsets.p, (c->P + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (contacts && c->C)
+    HIP_TRY(c, hipMemcpyAsync(contacts, c->contacts.p, c->C * 7 * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return BCH_OK;
+}
+
+int bch_world_results_device(bch_context* c, const uint32_t** d_pairs, const uint32_t** d_offsets,
+                             const float** d_contacts) {
+  if (!c) return BCH_INVALID_ARG;
+  if (d_pairs) *d_pairs = c->pairs.as<uint32_t>();
+  if (d_offsets) *d_offsets = c->coffsets.as<uint32_t>();
+  if (d_contacts) *d_contacts = c->contacts.as<float>();
+  return BCH_OK;
+}
+
+}  // extern "C"

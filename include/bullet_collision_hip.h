@@ -1,0 +1,139 @@
+/* bullet_collision_hip.h -- C ABI of the MI355X (gfx950) collision hot path:
+ * broadphase (overlapping AABB pairs) and convex narrowphase (contact
+ * manifolds of at most four points) for the bullet-collision crate of
+ * CrazyboyQCD/bullet-physics.
+ *
+ * Plain C types only: float / uint32 / uint8 arrays and sizes.  Every entry
+ * point returns a BCH_* status; on failure bch_last_error() describes it.
+ * Host pointers are caller-owned and only read/written during the call.
+ * One context per host thread (the reference is single-threaded:
+ * `&mut self` everywhere); calls on one context are stream-ordered.
+ *
+ * Conventions (crates/bullet-collision/src/narrowphase/mod.rs:21-23):
+ *   - a pair (i, j) always has i < j; pair lists are sorted lexicographically
+ *     (broadphase/mod.rs:103-118, identical to BruteForceBroadPhase);
+ *   - for pair (i, j) body i is shape A and body j is shape B: the contact
+ *     normal points from A to B, `position` is the witness point on B,
+ *     penetration > 0 is overlap and < 0 a separation inside the contact band;
+ *   - a contact is 7 floats: position xyz, normal xyz, penetration.
+ *
+ * Shapes (crates/bullet-collision/src/shapes):
+ *   BCH_SPHERE  params = radius
+ *   BCH_BOX     params = half extents x, y, z
+ *   BCH_CAPSULE params = radius, half height (segment along local Y)
+ *   BCH_HULL    vertices hull_vertices[hull_first[i] .. + hull_count[i]]
+ *   BCH_PLANE   params = normal x, y, z (normalised on upload), constant
+ * Orientation quaternions are (x, y, z, w).
+ */
+#ifndef BULLET_COLLISION_HIP_H
+#define BULLET_COLLISION_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BCH_OK 0
+#define BCH_OVERFLOW 1       /* output capacity too small: *count holds the size needed */
+#define BCH_INVALID_ARG 2
+#define BCH_HIP_ERROR 3
+#define BCH_NO_DEVICE 4
+
+#define BCH_SPHERE 0
+#define BCH_BOX 1
+#define BCH_CAPSULE 2
+#define BCH_HULL 3
+#define BCH_PLANE 4
+
+typedef struct bch_context bch_context;
+
+/* Bodies in structure-of-arrays form (host or device pointers, as the entry
+ * point states).  hull_* may be NULL when no body is a hull. */
+typedef struct bch_bodies {
+  uint32_t n;
+  const float* position;      /* [3n] */
+  const float* orientation;   /* [4n] x, y, z, w */
+  const uint8_t* shape_type;  /* [n] BCH_SPHERE .. BCH_PLANE */
+  const float* shape_params;  /* [4n] */
+  const uint32_t* hull_first; /* [n] first vertex of a hull body */
+  const uint32_t* hull_count; /* [n] vertex count of a hull body */
+  const float* hull_vertices; /* [3 * n_hull_vertices] local-space xyz */
+  uint32_t n_hull_vertices;
+} bch_bodies;
+
+/* ------------------------------------------------------------- context -- */
+/* No reference counterpart: the reference has no device.  Creates a context
+ * on HIP device `device` with its own stream. */
+int bch_create(int device, bch_context** out);
+void bch_destroy(bch_context* ctx);
+const char* bch_last_error(const bch_context* ctx);
+int bch_synchronize(bch_context* ctx);
+/* Device times (ms) of the last bch_world_collide pass, 7 floats:
+ * [0] world AABBs, [1] tree build (Morton codes, sort, nodes, refit),
+ * [2] tree traversal (pair search), [3] pair list (planes, compaction,
+ * sort), [4] narrowphase, [5] EPA, [6] contact compaction. */
+int bch_last_timings(bch_context* ctx, float* out_ms);
+
+/* ------------------------------------------ BroadPhase trait (proxies) -- */
+/* broadphase/mod.rs:94-130.  Slots follow the reference: create_proxy reuses
+ * the lowest free slot (brute_force.rs:26-34); set_aabb / destroy on a dead
+ * slot are no-ops (axis_sweep3.rs:424-426). */
+int bch_bp_create_proxy(bch_context* ctx, const float aabb_min[3], const float aabb_max[3], uint32_t* out_proxy);
+int bch_bp_set_aabb(bch_context* ctx, uint32_t proxy, const float aabb_min[3], const float aabb_max[3]);
+int bch_bp_destroy_proxy(bch_context* ctx, uint32_t proxy);
+/* *live = 0 for a dead or unknown slot (BroadPhase::proxy_aabb -> None). */
+int bch_bp_proxy_aabb(bch_context* ctx, uint32_t proxy, float aabb_min[3], float aabb_max[3], int* live);
+/* Batched set_aabb for `count` proxies (the world's per-step AABB refresh,
+ * world/step.rs:349-363). */
+int bch_bp_set_aabbs(bch_context* ctx, uint32_t count, const uint32_t* proxies, const float* mins,
+                     const float* maxs);
+/* compute_pairs_into (broadphase/mod.rs:103-118): every live (i, j), i < j,
+ * whose stored AABBs overlap (closed intervals), sorted; out_pairs[2k] = i,
+ * out_pairs[2k+1] = j.  BCH_OVERFLOW when capacity < count. */
+int bch_bp_compute_pairs(bch_context* ctx, uint32_t* out_pairs, uint64_t capacity, uint64_t* out_count);
+/* aabb_candidates (broadphase/mod.rs:126-129): live proxies overlapping the
+ * query box, ascending. */
+int bch_bp_aabb_candidates(bch_context* ctx, const float aabb_min[3], const float aabb_max[3], uint32_t* out,
+                           uint64_t capacity, uint64_t* out_count);
+
+/* ------------------------------------------------- narrowphase (batch) -- */
+/* collide_seeded (narrowphase/mod.rs:63-286) for every pair: shape A =
+ * pairs[2k], shape B = pairs[2k+1], seeds[3k..] an optional GJK warm-start
+ * direction (NULL = None).  out_count[k] = number of contact points (0 =
+ * None); out_points[28k + 7c ..] = contact c.  Host pointers. */
+int bch_collide_batch(bch_context* ctx, const bch_bodies* bodies, const uint32_t* pairs, uint64_t n_pairs,
+                      const float* seeds, uint8_t* out_count, float* out_points);
+
+/* ---------------------------------------------- world-resident pipeline -- */
+/* The per-step detection pass of PhysicsWorld (world/step.rs:349-416 +
+ * 445-854 for rigid pairs): world AABBs padded by `aabb_margin`, the sorted
+ * overlapping pair set, collide_seeded(body i, body j, None) per pair.
+ * Bodies stay resident in device memory between passes. */
+
+/* Upload (copy) all bodies from host memory; a batch of independent worlds
+ * passes world_offsets[n_worlds + 1] (bodies of world w are
+ * world_offsets[w] .. world_offsets[w+1]; pairs never cross worlds), or NULL
+ * and n_worlds = 1 for one world. */
+int bch_world_set_bodies(bch_context* ctx, const bch_bodies* bodies, uint32_t n_worlds,
+                         const uint32_t* world_offsets);
+/* Per-step pose refresh from host memory (positions [3n], orientations [4n]). */
+int bch_world_set_poses(bch_context* ctx, const float* position, const float* orientation);
+/* Per-step pose refresh from device memory on the same device. */
+int bch_world_set_poses_device(bch_context* ctx, const float* d_position, const float* d_orientation);
+/* Run one pass on the context stream; returns after enqueueing plus one
+ * small device->host read of the pair count.  *out_pairs / *out_contacts
+ * receive the counts. */
+int bch_world_collide(bch_context* ctx, float aabb_margin, uint64_t* out_pairs, uint64_t* out_contacts);
+/* Copy the last pass's results to host memory: pairs [2P], per-pair contact
+ * offsets [P+1] (contacts of pair k are offsets[k] .. offsets[k+1]) and
+ * contacts [7C].  Any pointer may be NULL to skip that array. */
+int bch_world_results(bch_context* ctx, uint32_t* pairs, uint32_t* contact_offsets, float* contacts);
+/* Device pointers of the last pass's results (valid until the next pass). */
+int bch_world_results_device(bch_context* ctx, const uint32_t** d_pairs, const uint32_t** d_contact_offsets,
+                             const float** d_contacts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

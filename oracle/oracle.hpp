@@ -1,0 +1,1349 @@
+// oracle.hpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of the bullet-collision crate's broadphase + convex
+// narrowphase (CrazyboyQCD/bullet-physics), templated on float / double so the
+// same text gives the f32 build (the parity oracle) and an f64 build (for the
+// error-vs-f64 figures).  Every function cites the reference file:line it
+// follows; statement order, operand order and branch structure are kept so an
+// f32 build with -ffp-contract=off reproduces the reference arithmetic except
+// where glam's internal op order is unknown (quaternion rotate, matrix-vector
+// product: the order used is the one written in `qrot` / `mmul` below).
+//
+// Only tests/, the smoke check and bench.py's CPU-baseline leg may use this
+// code, and only as the checker or the timed CPU baseline; the shipped library
+// never links it.
+//
+// Parity pinning (which checks fix which function) is listed in the test
+// files under tests/ (test_oracle_*.py).  Functions without a closed-form,
+// known-answer or brute-force check: the quaternion op order (glam boundary)
+// and EPA's exact output on generic hulls -- "parity unpinned" beyond the
+// restatement itself.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <type_traits>
+#include <vector>
+
+namespace oracle {
+
+// A literal spelled as the reference spells it, rounded once into T (an f32
+// build parses the decimal straight to f32 like rustc; no double rounding).
+#define ORL(x) (std::is_same<T, float>::value ? (T)(x##f) : (T)(x))
+
+// ---------------------------------------------------------------- vectors --
+// glam Vec3A / DVec3 semantics (crates/bullet-math/src/lib.rs:30-54).
+template <class T>
+struct V3 {
+  T x, y, z;
+};
+template <class T> inline V3<T> mk(T x, T y, T z) { return V3<T>{x, y, z}; }
+template <class T> inline V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> inline V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> inline V3<T> operator-(V3<T> a) { return {-a.x, -a.y, -a.z}; }
+template <class T> inline V3<T> operator*(V3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> inline V3<T> operator*(T s, V3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class T> inline V3<T> operator/(V3<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class T> inline T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> inline V3<T> cross(V3<T> a, V3<T> b) {
+  return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+template <class T> inline T len2(V3<T> a) { return dot(a, a); }
+template <class T> inline T len(V3<T> a) { return std::sqrt(dot(a, a)); }
+// glam normalize: self * (1 / length)
+template <class T> inline V3<T> normalize(V3<T> a) { return a * (T(1) / len(a)); }
+template <class T> inline T fmin_(T a, T b) { return b < a ? b : a; }
+template <class T> inline T fmax_(T a, T b) { return b > a ? b : a; }
+template <class T> inline V3<T> vmin(V3<T> a, V3<T> b) { return {fmin_(a.x, b.x), fmin_(a.y, b.y), fmin_(a.z, b.z)}; }
+template <class T> inline V3<T> vmax(V3<T> a, V3<T> b) { return {fmax_(a.x, b.x), fmax_(a.y, b.y), fmax_(a.z, b.z)}; }
+template <class T> inline V3<T> vabs(V3<T> a) { return {std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)}; }
+template <class T> inline V3<T> splat(T s) { return {s, s, s}; }
+template <class T> inline T comp(V3<T> v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+template <class T> inline bool abs_diff_eq(V3<T> a, V3<T> b, T eps) {
+  return std::fabs(a.x - b.x) <= eps && std::fabs(a.y - b.y) <= eps && std::fabs(a.z - b.z) <= eps;
+}
+// Rust f32::signum for non-NaN input.
+template <class T> inline T signum(T x) { return std::isnan(x) ? x : std::copysign(T(1), x); }
+
+template <class T>
+struct Quat {
+  T x, y, z, w;
+};
+// Quaternion rotation v' = v (w^2 - b.b) + b (2 v.b) + (b x v)(2 w), b = (x,y,z).
+template <class T> inline V3<T> qrot(Quat<T> q, V3<T> v) {
+  V3<T> b{q.x, q.y, q.z};
+  T b2 = dot(b, b);
+  V3<T> t1 = v * (q.w * q.w - b2);
+  V3<T> t2 = b * (dot(v, b) * T(2));
+  V3<T> t3 = cross(b, v) * (q.w * T(2));
+  return (t1 + t2) + t3;
+}
+template <class T> inline Quat<T> qconj(Quat<T> q) { return {-q.x, -q.y, -q.z, q.w}; }
+
+// Column-major 3x3 (glam Mat3A).
+template <class T>
+struct M3 {
+  V3<T> c0, c1, c2;
+};
+template <class T> inline V3<T> col(const M3<T>& m, int i) { return i == 0 ? m.c0 : (i == 1 ? m.c1 : m.c2); }
+template <class T> inline M3<T> from_quat(Quat<T> q) {
+  T x2 = q.x + q.x, y2 = q.y + q.y, z2 = q.z + q.z;
+  T xx = q.x * x2, xy = q.x * y2, xz = q.x * z2;
+  T yy = q.y * y2, yz = q.y * z2, zz = q.z * z2;
+  T wx = q.w * x2, wy = q.w * y2, wz = q.w * z2;
+  M3<T> m;
+  m.c0 = {T(1) - (yy + zz), xy + wz, xz - wy};
+  m.c1 = {xy - wz, T(1) - (xx + zz), yz + wx};
+  m.c2 = {xz + wy, yz - wx, T(1) - (xx + yy)};
+  return m;
+}
+template <class T> inline V3<T> mmul(const M3<T>& m, V3<T> v) { return (m.c0 * v.x + m.c1 * v.y) + m.c2 * v.z; }
+template <class T> inline M3<T> transpose(const M3<T>& m) {
+  M3<T> t;
+  t.c0 = {m.c0.x, m.c1.x, m.c2.x};
+  t.c1 = {m.c0.y, m.c1.y, m.c2.y};
+  t.c2 = {m.c0.z, m.c1.z, m.c2.z};
+  return t;
+}
+template <class T> inline M3<T> mmulm(const M3<T>& a, const M3<T>& b) {
+  return M3<T>{mmul(a, b.c0), mmul(a, b.c1), mmul(a, b.c2)};
+}
+
+// ------------------------------------------------------------------- Aabb --
+// crates/bullet-math/src/util.rs:5-125
+template <class T>
+struct Aabb {
+  V3<T> min, max;
+  V3<T> center() const { return (min + max) * T(0.5); }
+  // util.rs:28-38 (closed intervals; touching counts)
+  bool overlaps(const Aabb& o) const {
+    return min.x <= o.max.x && min.y <= o.max.y && min.z <= o.max.z && o.min.x <= max.x &&
+           o.min.y <= max.y && o.min.z <= max.z;
+  }
+  // util.rs:52-55
+  Aabb expanded(T m) const {
+    V3<T> pad = splat(m);
+    return Aabb{min - pad, max + pad};
+  }
+};
+
+// ----------------------------------------------------------------- shapes --
+enum Kind : uint8_t { SPHERE = 0, BOX = 1, CAPSULE = 2, HULL = 3, PLANE = 4 };
+
+// CONTACT_BREAKING_THRESHOLD, crates/bullet-collision/src/manifold.rs:8
+template <class T> inline T contact_breaking_threshold_const() { return ORL(0.02); }
+
+template <class T>
+struct Shape {
+  Kind kind = SPHERE;
+  T radius = 0, half_height = 0;    // sphere, capsule
+  V3<T> half{0, 0, 0};              // box
+  V3<T> pnormal{0, 1, 0};           // plane (normalized)
+  T pconst = 0;                     // plane
+  const V3<T>* pts = nullptr;       // hull
+  int npts = 0;
+  Aabb<T> hull_bounds{{0, 0, 0}, {0, 0, 0}};
+
+  static Shape sphere(T r) { Shape s; s.kind = SPHERE; s.radius = r; return s; }
+  static Shape box(V3<T> h) { Shape s; s.kind = BOX; s.half = h; return s; }
+  static Shape capsule(T r, T hh) { Shape s; s.kind = CAPSULE; s.radius = r; s.half_height = hh; return s; }
+  // shapes/plane.rs:18-23: the normal is normalized on construction.
+  static Shape plane(V3<T> n, T c) { Shape s; s.kind = PLANE; s.pnormal = normalize(n); s.pconst = c; return s; }
+  // shapes/convex_hull.rs:21-36: cached local bounds, zero when empty.
+  static Shape hull(const V3<T>* p, int n) {
+    Shape s;
+    s.kind = HULL;
+    s.pts = p;
+    s.npts = n;
+    V3<T> mn = splat(std::numeric_limits<T>::infinity());
+    V3<T> mx = splat(-std::numeric_limits<T>::infinity());
+    for (int i = 0; i < n; ++i) {
+      mn = vmin(mn, p[i]);
+      mx = vmax(mx, p[i]);
+    }
+    if (n == 0) {
+      mn = splat(T(0));
+      mx = splat(T(0));
+    }
+    s.hull_bounds = Aabb<T>{mn, mx};
+    return s;
+  }
+
+  // shapes/mod.rs:236-251 + per-shape margin()
+  T margin() const {
+    switch (kind) {
+      case SPHERE: return radius;           // sphere.rs:17-20
+      case BOX: return ORL(0.04);           // box_shape.rs:17-21
+      case CAPSULE: return radius;          // capsule.rs:24-28
+      case HULL: return T(0);               // convex_hull.rs:43-45
+      case PLANE: return T(0);              // plane.rs:25-28
+    }
+    return T(0);
+  }
+  // shapes/mod.rs:254-270 + per-shape local_aabb()
+  Aabb<T> local_aabb() const {
+    switch (kind) {
+      case SPHERE: { V3<T> r = splat(radius); return Aabb<T>{-r, r}; }
+      case BOX: return Aabb<T>{-half, half};
+      case CAPSULE: { V3<T> r = mk(radius, half_height + radius, radius); return Aabb<T>{-r, r}; }
+      case HULL: return hull_bounds;
+      case PLANE: { V3<T> b = splat(ORL(1e30)); return Aabb<T>{-b, b}; }
+    }
+    return Aabb<T>{};
+  }
+  // shapes/mod.rs:273-290
+  V3<T> support_local(V3<T> dir) const {
+    switch (kind) {
+      case SPHERE: {  // sphere.rs:35-42
+        T l = len(dir);
+        if (l > T(0)) return (dir / l) * radius;
+        return mk(radius, T(0), T(0));
+      }
+      case BOX:  // box_shape.rs:30-37
+        return mk(dir.x >= T(0) ? half.x : -half.x, dir.y >= T(0) ? half.y : -half.y,
+                  dir.z >= T(0) ? half.z : -half.z);
+      case CAPSULE: {  // capsule.rs:38-56
+        T l = len(dir);
+        if (l > T(0)) {
+          V3<T> tip = mk(T(0), dir.y >= T(0) ? half_height : -half_height, T(0));
+          return tip + (dir / l) * radius;
+        }
+        return mk(radius, half_height, T(0));
+      }
+      case HULL: {  // convex_hull.rs:55-66 (strict > keeps the first maximum)
+        V3<T> best = splat(T(0));
+        T best_dot = -std::numeric_limits<T>::infinity();
+        for (int i = 0; i < npts; ++i) {
+          T d = dot(pts[i], dir);
+          if (d > best_dot) {
+            best_dot = d;
+            best = pts[i];
+          }
+        }
+        return best;
+      }
+      case PLANE: return splat(T(0));  // mod.rs:286-288
+    }
+    return splat(T(0));
+  }
+  // shapes/mod.rs:305-319 + per-shape core support
+  V3<T> core_support_local(V3<T> dir) const {
+    switch (kind) {
+      case SPHERE: return splat(T(0));  // sphere.rs:23-26
+      case BOX: {                       // box_shape.rs:42-53
+        V3<T> h = mk(fmax_(half.x - margin(), T(0)), fmax_(half.y - margin(), T(0)),
+                     fmax_(half.z - margin(), T(0)));
+        return mk(dir.x >= T(0) ? h.x : -h.x, dir.y >= T(0) ? h.y : -h.y, dir.z >= T(0) ? h.z : -h.z);
+      }
+      case CAPSULE:  // capsule.rs:59-69
+        return mk(T(0), dir.y >= T(0) ? half_height : -half_height, T(0));
+      default: return support_local(dir);  // hull: core == full shape
+    }
+  }
+  // shapes/mod.rs:323-332
+  V3<T> support_world(V3<T> p, Quat<T> q, V3<T> dir) const {
+    V3<T> ld = qrot(qconj(q), dir);
+    V3<T> l = support_local(ld);
+    return qrot(q, l) + p;
+  }
+  // shapes/mod.rs:337-346
+  V3<T> core_support_world(V3<T> p, Quat<T> q, V3<T> dir) const {
+    V3<T> ld = qrot(qconj(q), dir);
+    V3<T> l = core_support_local(ld);
+    return qrot(q, l) + p;
+  }
+  // shapes/mod.rs:350-375 + per-shape world_aabb()
+  Aabb<T> world_aabb(V3<T> p, Quat<T> q) const {
+    switch (kind) {
+      case SPHERE: { V3<T> r = splat(radius); return Aabb<T>{p - r, p + r}; }  // sphere.rs:45-48
+      case BOX: {  // box_shape.rs:57-72
+        V3<T> mn = splat(std::numeric_limits<T>::infinity());
+        V3<T> mx = splat(-std::numeric_limits<T>::infinity());
+        for (int i = 0; i < 8; ++i) {
+          V3<T> corner = mk((i & 1) == 0 ? half.x : -half.x, (i & 2) == 0 ? half.y : -half.y,
+                            (i & 4) == 0 ? half.z : -half.z);
+          V3<T> w = qrot(q, corner) + p;
+          mn = vmin(mn, w);
+          mx = vmax(mx, w);
+        }
+        return Aabb<T>{mn, mx};
+      }
+      case CAPSULE: {  // capsule.rs:74-79
+        V3<T> top = qrot(q, mk(T(0), half_height, T(0))) + p;
+        V3<T> bottom = qrot(q, mk(T(0), -half_height, T(0))) + p;
+        V3<T> r = splat(radius);
+        return Aabb<T>{vmin(top, bottom) - r, vmax(top, bottom) + r};
+      }
+      case HULL: {  // convex_hull.rs:70-72 -> shapes/mod.rs:448-469
+        Quat<T> qi = qconj(q);
+        V3<T> ux = qrot(qi, mk(T(1), T(0), T(0)));
+        V3<T> uy = qrot(qi, mk(T(0), T(1), T(0)));
+        V3<T> uz = qrot(qi, mk(T(0), T(0), T(1)));
+        V3<T> mn = mk(p.x + dot(support_local(-ux), ux), p.y + dot(support_local(-uy), uy),
+                      p.z + dot(support_local(-uz), uz));
+        V3<T> mx = mk(p.x + dot(support_local(ux), ux), p.y + dot(support_local(uy), uy),
+                      p.z + dot(support_local(uz), uz));
+        return Aabb<T>{mn, mx};
+      }
+      case PLANE: return local_aabb();  // plane.rs:37-40
+    }
+    return Aabb<T>{};
+  }
+  // shapes/mod.rs:388-392
+  T contact_breaking_threshold() const {
+    Aabb<T> a = local_aabb();
+    T disc = len(a.max - a.min) * T(0.5) + len(a.center());
+    return disc * contact_breaking_threshold_const<T>();
+  }
+};
+
+// ---------------------------------------------------------------- contacts --
+// manifold.rs:12-112
+template <class T>
+struct ContactPoint {
+  V3<T> position, normal;
+  T penetration;
+};
+template <class T>
+struct Manifold {
+  int n = 0;
+  ContactPoint<T> pts[8];
+  void push(V3<T> p, V3<T> nrm, T d) { pts[n++] = ContactPoint<T>{p, nrm, d}; }
+  // manifold.rs:68-74
+  void flip() {
+    for (int i = 0; i < n; ++i) pts[i].normal = -pts[i].normal;
+  }
+};
+
+// manifold.rs:81-112 (ContactManifold::reduce), kept for completeness: the
+// convex pairs restated here never produce more than four points.
+template <class T>
+void reduce_manifold(std::vector<ContactPoint<T>>& points, size_t max) {
+  if (points.size() <= max) return;
+  std::vector<ContactPoint<T>> cand = points;
+  // sort_by(b.pen.total_cmp(a.pen)): stable, descending by penetration.
+  std::stable_sort(cand.begin(), cand.end(), [](const ContactPoint<T>& a, const ContactPoint<T>& b) {
+    return a.penetration > b.penetration;
+  });
+  std::vector<ContactPoint<T>> kept;
+  kept.push_back(cand[0]);
+  std::vector<bool> used(cand.size(), false);
+  used[0] = true;
+  while (kept.size() < max) {
+    size_t best = 0;
+    T best_sep = -std::numeric_limits<T>::infinity();
+    for (size_t i = 0; i < cand.size(); ++i) {
+      if (used[i]) continue;
+      T sep = std::numeric_limits<T>::infinity();
+      for (auto& q : kept) sep = fmin_(sep, len2(cand[i].position - q.position));
+      if (sep > best_sep) {
+        best_sep = sep;
+        best = i;
+      }
+    }
+    kept.push_back(cand[best]);
+    used[best] = true;
+  }
+  points = kept;
+}
+
+// ------------------------------------------------------------ sphere.rs --
+// narrowphase/sphere.rs:10-32
+template <class T>
+bool sphere_sphere(const Shape<T>& sa, V3<T> pa, const Shape<T>& sb, V3<T> pb, Manifold<T>& m) {
+  V3<T> delta = pb - pa;
+  T dist_sq = len2(delta);
+  T radius_sum = sa.radius + sb.radius;
+  if (dist_sq >= radius_sum * radius_sum) return false;
+  T dist = std::sqrt(dist_sq);
+  V3<T> normal = dist > ORL(1e-12) ? delta / dist : mk(T(0), T(1), T(0));
+  T pen = radius_sum - dist;
+  V3<T> position = pa + normal * (sa.radius - pen * T(0.5));
+  m.push(position, normal, pen);
+  return true;
+}
+
+// narrowphase/sphere.rs:38-82 (normal from the box to the sphere)
+template <class T>
+bool sphere_box(const Shape<T>& s, V3<T> ps, const Shape<T>& b, V3<T> pb, Quat<T> ob, T threshold,
+                Manifold<T>& m) {
+  V3<T> local = qrot(qconj(ob), ps - pb);
+  V3<T> h = b.half;
+  V3<T> clamped = vmin(vmax(local, -h), h);
+  V3<T> delta = local - clamped;
+  T dist_sq = len2(delta);
+  T contact_dist = s.radius + threshold;
+  if (dist_sq >= contact_dist * contact_dist) return false;
+  T dist = std::sqrt(dist_sq);
+  V3<T> normal_local;
+  T pen;
+  if (dist > ORL(1e-12)) {
+    normal_local = delta / dist;
+    pen = s.radius - dist;
+  } else {
+    T dx = h.x - std::fabs(local.x);
+    T dy = h.y - std::fabs(local.y);
+    T dz = h.z - std::fabs(local.z);
+    V3<T> n;
+    T d;
+    if (dx <= dy && dx <= dz) {
+      n = mk(signum(local.x), T(0), T(0));
+      d = dx;
+    } else if (dy <= dz) {
+      n = mk(T(0), signum(local.y), T(0));
+      d = dy;
+    } else {
+      n = mk(T(0), T(0), signum(local.z));
+      d = dz;
+    }
+    normal_local = n;
+    pen = s.radius + d;
+  }
+  V3<T> normal = qrot(ob, normal_local);
+  V3<T> position = qrot(ob, clamped) + pb;
+  m.push(position, normal, pen);
+  return true;
+}
+
+// ------------------------------------------------------------- plane.rs --
+// narrowphase/plane.rs:19-56
+template <class T>
+bool plane_convex(const Shape<T>& plane, V3<T> pp, Quat<T> pq, const Shape<T>& convex, V3<T> cp,
+                  Quat<T> cq, T threshold, Manifold<T>& m) {
+  V3<T> n = qrot(pq, plane.pnormal);
+  T c = plane.pconst + dot(n, pp);
+  V3<T> normal = dot(n, cp - pp) >= T(0) ? n : -n;
+  if (!(normal.x == n.x && normal.y == n.y && normal.z == n.z)) c = -c;
+  V3<T> deepest = convex.support_world(cp, cq, -normal);
+  T depth = c - dot(normal, deepest);
+  if (depth < -threshold) return false;
+  m.push(deepest, normal, depth);
+  return true;
+}
+
+// ----------------------------------------------------------- box_box.rs --
+// narrowphase/box_box.rs:31-46
+template <class T>
+inline void line_closest_approach(V3<T> pa, V3<T> ua, V3<T> pb, V3<T> ub, T& alpha, T& beta) {
+  V3<T> p = pb - pa;
+  T uaub = dot(ua, ub);
+  T q1 = dot(ua, p);
+  T q2 = -dot(ub, p);
+  T d = T(1) - uaub * uaub;
+  if (d <= ORL(1e-4)) {
+    alpha = T(0);
+    beta = T(0);
+  } else {
+    d = T(1) / d;
+    alpha = (q1 + uaub * q2) * d;
+    beta = (uaub * q1 + q2) * d;
+  }
+}
+
+template <class T>
+struct ClipPoly {
+  T p[8][2];
+  int n = 0;
+  bool overflow = false;
+  void push(T a, T b) {
+    if (n >= 8) { overflow = true; return; }  // reference: debug_assert + index panic
+    p[n][0] = a;
+    p[n][1] = b;
+    ++n;
+  }
+};
+
+// box_box.rs:92-114
+template <class T>
+void clip_poly(const ClipPoly<T>& src, int axis, T sign, T h, ClipPoly<T>& out) {
+  int n = src.n;
+  out.n = 0;
+  out.overflow = src.overflow;
+  for (int i = 0; i < n; ++i) {
+    const T* cur = src.p[i];
+    const T* next = src.p[(i + 1) % n];
+    bool cur_in = sign * cur[axis] < h;
+    bool next_in = sign * next[axis] < h;
+    if (cur_in) out.push(cur[0], cur[1]);
+    if (cur_in != next_in) {
+      T t = (sign * h - cur[axis]) / (next[axis] - cur[axis]);
+      T q[2];
+      q[axis] = sign * h;
+      q[1 - axis] = cur[1 - axis] + (next[1 - axis] - cur[1 - axis]) * t;
+      out.push(q[0], q[1]);
+    }
+  }
+}
+
+// box_box.rs:131-174
+template <class T>
+int cull_points(const V3<T>* pts, const T* deps, int n, int maxc, V3<T>* opts, T* odeps) {
+  if (n <= maxc) {
+    for (int i = 0; i < n; ++i) { opts[i] = pts[i]; odeps[i] = deps[i]; }
+    return n;
+  }
+  int i0 = 0;
+  T deepest_depth = deps[0];
+  for (int i = 1; i < n; ++i) {
+    if (deps[i] >= deepest_depth) {
+      i0 = i;
+      deepest_depth = deps[i];
+    }
+  }
+  int ri[8];
+  T rd[8];
+  int rn = 0;
+  for (int i = 0; i < n; ++i) {
+    if (i != i0) {
+      ri[rn] = i;
+      rd[rn] = len2(pts[i] - pts[i0]);
+      ++rn;
+    }
+  }
+  for (int j = 1; j < rn; ++j) {
+    int ki = ri[j];
+    T kd = rd[j];
+    int k = j;
+    while (k > 0 && rd[k - 1] < kd) {
+      ri[k] = ri[k - 1];
+      rd[k] = rd[k - 1];
+      --k;
+    }
+    ri[k] = ki;
+    rd[k] = kd;
+  }
+  opts[0] = pts[i0];
+  odeps[0] = deps[i0];
+  int m = 1;
+  for (int j = 0; j < maxc - 1 && j < rn; ++j) {
+    opts[m] = pts[ri[j]];
+    odeps[m] = deps[ri[j]];
+    ++m;
+  }
+  return m;
+}
+
+// box_box.rs:178-528 (ODE dBoxBox2 SAT, 15 axes, face clipping, <= 4 points)
+template <class T>
+bool box_box(const Shape<T>& a, V3<T> p1, Quat<T> oa, const Shape<T>& b, V3<T> p2, Quat<T> ob,
+             Manifold<T>& out) {
+  const T EPS = ORL(1e-6), FUDGE = ORL(1.05);
+  M3<T> r1 = from_quat(oa), r2 = from_quat(ob);
+  V3<T> ha = a.half, hb = b.half;
+  V3<T> p = p2 - p1;
+  M3<T> r1t = transpose(r1);
+  V3<T> pp = mmul(r1t, p);
+  M3<T> r = mmulm(r1t, r2);
+  T r11 = r.c0.x, r12 = r.c1.x, r13 = r.c2.x;
+  T r21 = r.c0.y, r22 = r.c1.y, r23 = r.c2.y;
+  T r31 = r.c0.z, r32 = r.c1.z, r33 = r.c2.z;
+  T q11 = std::fabs(r11), q12 = std::fabs(r12), q13 = std::fabs(r13);
+  T q21 = std::fabs(r21), q22 = std::fabs(r22), q23 = std::fabs(r23);
+  T q31 = std::fabs(r31), q32 = std::fabs(r32), q33 = std::fabs(r33);
+
+  T s = -std::numeric_limits<T>::infinity();
+  bool invert_normal = false;
+  int code = 0;
+  bool normal_from_r2 = false;
+  int normal_axis = 0;
+  bool normal_is_cross = false;
+  V3<T> normal_c = splat(T(0));
+
+#define OR_TST_FACE(e1, e2, fr2, ax, cc)          \
+  {                                               \
+    T ex1 = (e1);                                 \
+    T s2 = std::fabs(ex1) - (e2);                 \
+    if (s2 > T(0)) return false;                  \
+    if (s2 > s) {                                 \
+      s = s2;                                     \
+      normal_is_cross = false;                    \
+      normal_from_r2 = fr2;                       \
+      normal_axis = ax;                           \
+      invert_normal = ex1 < T(0);                 \
+      code = cc;                                  \
+    }                                             \
+  }
+#define OR_TST_CROSS(e1, e2, n1, n2, n3, cc)                   \
+  {                                                            \
+    T ex1 = (e1);                                              \
+    T s2 = std::fabs(ex1) - (e2);                              \
+    if (s2 > EPS) return false;                                \
+    T N1 = (n1), N2 = (n2), N3 = (n3);                         \
+    T l = std::sqrt(N1 * N1 + N2 * N2 + N3 * N3);              \
+    if (l > EPS) {                                             \
+      s2 = s2 / l;                                             \
+      if (s2 * FUDGE > s) {                                    \
+        s = s2;                                                \
+        normal_is_cross = true;                                \
+        normal_c = mk(N1 / l, N2 / l, N3 / l);                 \
+        invert_normal = ex1 < T(0);                            \
+        code = cc;                                             \
+      }                                                        \
+    }                                                          \
+  }
+
+  OR_TST_FACE(pp.x, ha.x + hb.x * q11 + hb.y * q12 + hb.z * q13, false, 0, 1);
+  OR_TST_FACE(pp.y, ha.y + hb.x * q21 + hb.y * q22 + hb.z * q23, false, 1, 2);
+  OR_TST_FACE(pp.z, ha.z + hb.x * q31 + hb.y * q32 + hb.z * q33, false, 2, 3);
+  OR_TST_FACE(dot(r2.c0, p), ha.x * q11 + ha.y * q21 + ha.z * q31 + hb.x, true, 0, 4);
+  OR_TST_FACE(dot(r2.c1, p), ha.x * q12 + ha.y * q22 + ha.z * q32 + hb.y, true, 1, 5);
+  OR_TST_FACE(dot(r2.c2, p), ha.x * q13 + ha.y * q23 + ha.z * q33 + hb.z, true, 2, 6);
+
+  const T fudge2 = ORL(1e-5);
+  q11 += fudge2; q12 += fudge2; q13 += fudge2;
+  q21 += fudge2; q22 += fudge2; q23 += fudge2;
+  q31 += fudge2; q32 += fudge2; q33 += fudge2;
+
+  OR_TST_CROSS(pp.z * r21 - pp.y * r31, ha.y * q31 + ha.z * q21 + hb.y * q13 + hb.z * q12, T(0), -r31, r21, 7);
+  OR_TST_CROSS(pp.z * r22 - pp.y * r32, ha.y * q32 + ha.z * q22 + hb.x * q13 + hb.z * q11, T(0), -r32, r22, 8);
+  OR_TST_CROSS(pp.z * r23 - pp.y * r33, ha.y * q33 + ha.z * q23 + hb.x * q12 + hb.y * q11, T(0), -r33, r23, 9);
+  OR_TST_CROSS(pp.x * r31 - pp.z * r11, ha.x * q31 + ha.z * q11 + hb.y * q23 + hb.z * q22, r31, T(0), -r11, 10);
+  OR_TST_CROSS(pp.x * r32 - pp.z * r12, ha.x * q32 + ha.z * q12 + hb.x * q23 + hb.z * q21, r32, T(0), -r12, 11);
+  OR_TST_CROSS(pp.x * r33 - pp.z * r13, ha.x * q33 + ha.z * q13 + hb.x * q22 + hb.y * q21, r33, T(0), -r13, 12);
+  OR_TST_CROSS(pp.y * r11 - pp.x * r21, ha.x * q21 + ha.y * q11 + hb.y * q33 + hb.z * q32, -r21, r11, T(0), 13);
+  OR_TST_CROSS(pp.y * r12 - pp.x * r22, ha.x * q22 + ha.y * q12 + hb.x * q33 + hb.z * q31, -r22, r12, T(0), 14);
+  OR_TST_CROSS(pp.y * r13 - pp.x * r23, ha.x * q23 + ha.y * q13 + hb.x * q32 + hb.y * q31, -r23, r13, T(0), 15);
+#undef OR_TST_FACE
+#undef OR_TST_CROSS
+
+  V3<T> normal_sat = normal_is_cross ? mmul(r1, normal_c) : (normal_from_r2 ? col(r2, normal_axis) : col(r1, normal_axis));
+  if (invert_normal) normal_sat = -normal_sat;
+  T depth = -s;
+
+  if (code > 6) {  // edge-edge: box_box.rs:402-423
+    V3<T> pa = p1;
+    for (int j = 0; j < 3; ++j) {
+      V3<T> c = col(r1, j);
+      T sign = dot(normal_sat, c) > T(0) ? T(1) : T(-1);
+      pa = pa + (sign * comp(ha, j)) * c;
+    }
+    V3<T> pb = p2;
+    for (int j = 0; j < 3; ++j) {
+      V3<T> c = col(r2, j);
+      T sign = dot(normal_sat, c) > T(0) ? T(-1) : T(1);
+      pb = pb + (sign * comp(hb, j)) * c;
+    }
+    V3<T> ua = col(r1, (code - 7) / 3);
+    V3<T> ub = col(r2, (code - 7) % 3);
+    T alpha, beta;
+    line_closest_approach(pa, ua, pb, ub, alpha, beta);
+    pa = pa + ua * alpha;
+    pb = pb + ub * beta;
+    out.push(pb, normal_sat, depth);
+    return true;
+  }
+
+  // Face-face: box_box.rs:425-527
+  const M3<T>& ra = code <= 3 ? r1 : r2;
+  const M3<T>& rb = code <= 3 ? r2 : r1;
+  V3<T> pa_ref = code <= 3 ? p1 : p2;
+  V3<T> pb_inc = code <= 3 ? p2 : p1;
+  V3<T> sa = code <= 3 ? ha : hb;
+  V3<T> sb = code <= 3 ? hb : ha;
+  V3<T> normal2 = code <= 3 ? normal_sat : -normal_sat;
+  V3<T> nr = mmul(transpose(rb), normal2);
+  V3<T> anr = vabs(nr);
+  int lanr, a1, a2;
+  if (anr.y > anr.x) {
+    if (anr.y > anr.z) { lanr = 1; a1 = 0; a2 = 2; } else { lanr = 2; a1 = 0; a2 = 1; }
+  } else if (anr.x > anr.z) {
+    lanr = 0; a1 = 1; a2 = 2;
+  } else {
+    lanr = 2; a1 = 0; a2 = 1;
+  }
+  V3<T> center = comp(nr, lanr) < T(0) ? (pb_inc - pa_ref) + comp(sb, lanr) * col(rb, lanr)
+                                          : (pb_inc - pa_ref) - comp(sb, lanr) * col(rb, lanr);
+  int code_n = code <= 3 ? code - 1 : code - 4;
+  int code1, code2;
+  if (code_n == 0) { code1 = 1; code2 = 2; } else if (code_n == 1) { code1 = 0; code2 = 2; } else { code1 = 0; code2 = 1; }
+  T c1 = dot(center, col(ra, code1));
+  T c2 = dot(center, col(ra, code2));
+  T m11 = dot(col(ra, code1), col(rb, a1));
+  T m12 = dot(col(ra, code1), col(rb, a2));
+  T m21 = dot(col(ra, code2), col(rb, a1));
+  T m22 = dot(col(ra, code2), col(rb, a2));
+  T k1 = m11 * comp(sb, a1);
+  T k2 = m21 * comp(sb, a1);
+  T k3 = m12 * comp(sb, a2);
+  T k4 = m22 * comp(sb, a2);
+  ClipPoly<T> buf, tmp;
+  buf.n = 4;
+  buf.p[0][0] = c1 - k1 - k3; buf.p[0][1] = c2 - k2 - k4;
+  buf.p[1][0] = c1 - k1 + k3; buf.p[1][1] = c2 - k2 + k4;
+  buf.p[2][0] = c1 + k1 + k3; buf.p[2][1] = c2 + k2 + k4;
+  buf.p[3][0] = c1 + k1 - k3; buf.p[3][1] = c2 + k2 - k4;
+  T h0 = comp(sa, code1), h1 = comp(sa, code2);
+  clip_poly(buf, 0, T(-1), h0, tmp);
+  clip_poly(tmp, 0, T(1), h0, buf);
+  clip_poly(buf, 1, T(-1), h1, tmp);
+  clip_poly(tmp, 1, T(1), h1, buf);
+  if (buf.n == 0) return false;
+
+  T det1 = T(1) / (m11 * m22 - m12 * m21);
+  T n11 = m11 * det1, n12 = m12 * det1, n21 = m21 * det1, n22 = m22 * det1;
+  V3<T> pen_pts[8];
+  T pen_d[8];
+  int np = 0;
+  for (int i = 0; i < buf.n; ++i) {
+    T rx = buf.p[i][0], ry = buf.p[i][1];
+    T kk1 = n22 * (rx - c1) - n12 * (ry - c2);
+    T kk2 = -n21 * (rx - c1) + n11 * (ry - c2);
+    V3<T> point = (center + kk1 * col(rb, a1)) + kk2 * col(rb, a2);
+    T dep = comp(sa, code_n) - dot(normal2, point);
+    if (dep >= T(0)) {
+      pen_pts[np] = point;
+      pen_d[np] = dep;
+      ++np;
+    }
+  }
+  if (np == 0) return false;
+  V3<T> cp[4];
+  T cd[4];
+  int nc = cull_points(pen_pts, pen_d, np, 4, cp, cd);
+  for (int i = 0; i < nc; ++i) {
+    V3<T> pos = cp[i] + pa_ref;
+    if (code > 3) pos = pos + normal_sat * cd[i];
+    out.push(pos, normal_sat, cd[i]);
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------- gjk.rs --
+template <class T>
+struct SupportVertex {
+  V3<T> w, p_b;
+};
+template <class T>
+struct GjkOut {
+  int kind = 0;  // 0 = Separated, 1 = Band, 2 = Overlap
+  T distance = 0;
+  V3<T> normal{0, 0, 0}, point_on_b{0, 0, 0};
+  V3<T> simplex[4];
+  int iterations = 0;
+};
+template <class T>
+struct Reduce {
+  bool inside;
+  V3<T> v, p_b;
+};
+
+// gjk.rs:398-464
+template <class T>
+void closest_triangle_bary(const SupportVertex<T>& a, const SupportVertex<T>& b, const SupportVertex<T>& c,
+                           V3<T>& q, T wts[3]) {
+  V3<T> ab = b.w - a.w, ac = c.w - a.w, ap = -a.w;
+  T d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= T(0) && d2 <= T(0)) { q = a.w; wts[0] = 1; wts[1] = 0; wts[2] = 0; return; }
+  V3<T> bp = -b.w;
+  T d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= T(0) && d4 <= d3) { q = b.w; wts[0] = 0; wts[1] = 1; wts[2] = 0; return; }
+  T vc = d1 * d4 - d3 * d2;
+  if (vc <= T(0) && d1 >= T(0) && d3 <= T(0)) {
+    if (d1 == d3) { q = a.w; wts[0] = 1; wts[1] = 0; wts[2] = 0; return; }
+    T t = d1 / (d1 - d3);
+    q = a.w + ab * t; wts[0] = T(1) - t; wts[1] = t; wts[2] = 0; return;
+  }
+  V3<T> cp = -c.w;
+  T d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= T(0) && d5 <= d6) { q = c.w; wts[0] = 0; wts[1] = 0; wts[2] = 1; return; }
+  T vb = d5 * d2 - d1 * d6;
+  if (vb <= T(0) && d2 >= T(0) && d6 <= T(0)) {
+    if (d2 == d6) { q = a.w; wts[0] = 1; wts[1] = 0; wts[2] = 0; return; }
+    T t = d2 / (d2 - d6);
+    q = a.w + ac * t; wts[0] = T(1) - t; wts[1] = 0; wts[2] = t; return;
+  }
+  T va = d3 * d6 - d5 * d4;
+  if (va <= T(0) && (d4 - d3) >= T(0) && (d5 - d6) >= T(0)) {
+    if ((d4 - d3) + (d5 - d6) == T(0)) { q = b.w; wts[0] = 0; wts[1] = 1; wts[2] = 0; return; }
+    T t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    q = b.w + (c.w - b.w) * t; wts[0] = 0; wts[1] = T(1) - t; wts[2] = t; return;
+  }
+  T sum = va + vb + vc;
+  if (sum <= T(0) || std::isnan(sum)) {
+    if (len2(a.w) <= len2(b.w) && len2(a.w) <= len2(c.w)) { q = a.w; wts[0] = 1; wts[1] = 0; wts[2] = 0; }
+    else if (len2(b.w) <= len2(c.w)) { q = b.w; wts[0] = 0; wts[1] = 1; wts[2] = 0; }
+    else { q = c.w; wts[0] = 0; wts[1] = 0; wts[2] = 1; }
+    return;
+  }
+  T denom = T(1) / sum;
+  T v = vb * denom, w = vc * denom;
+  q = (a.w + ab * v) + ac * w;
+  wts[0] = T(1) - v - w; wts[1] = v; wts[2] = w;
+}
+
+// gjk.rs:366-393
+template <class T>
+Reduce<T> reduce_segment(SupportVertex<T>* buf, int& n) {
+  SupportVertex<T> a = buf[1], b = buf[0];
+  V3<T> ab = b.w - a.w;
+  T denom = len2(ab);
+  if (denom < ORL(1e-24)) { n = 1; buf[0] = a; return {false, a.w, a.p_b}; }
+  T t = -dot(a.w, ab) / denom;
+  if (t <= T(0)) { n = 1; buf[0] = a; return {false, a.w, a.p_b}; }
+  if (t >= T(1)) { n = 1; buf[0] = b; return {false, b.w, b.p_b}; }
+  n = 2;
+  buf[0] = a;
+  buf[1] = b;
+  return {false, a.w + ab * t, a.p_b + (b.p_b - a.p_b) * t};
+}
+
+// gjk.rs:466-486
+template <class T>
+Reduce<T> reduce_triangle(SupportVertex<T>* buf, int& n) {
+  SupportVertex<T> a = buf[2], b = buf[1], c = buf[0];
+  V3<T> v;
+  T wt[3];
+  closest_triangle_bary(a, b, c, v, wt);
+  SupportVertex<T> kept[4] = {a, a, a, a};
+  SupportVertex<T> src[3] = {a, b, c};
+  int k = 0;
+  for (int i = 0; i < 3; ++i)
+    if (wt[i] > T(0)) kept[k++] = src[i];
+  if (k == 0) { k = 1; kept[0] = a; }
+  for (int i = 0; i < k; ++i) buf[i] = kept[i];
+  n = k;
+  V3<T> p_b = (a.p_b * wt[0] + b.p_b * wt[1]) + c.p_b * wt[2];
+  return {false, v, p_b};
+}
+
+// gjk.rs:569-583
+template <class T>
+Reduce<T> triangle_step(SupportVertex<T>* buf, int& n, SupportVertex<T> a, SupportVertex<T> b, SupportVertex<T> c) {
+  buf[0] = c;
+  buf[1] = b;
+  buf[2] = a;
+  n = 3;
+  return reduce_triangle(buf, n);
+}
+
+// gjk.rs:488-567
+template <class T>
+Reduce<T> reduce_tetrahedron(SupportVertex<T>* buf, int& n) {
+  SupportVertex<T> a = buf[3], b = buf[2], c = buf[1], d = buf[0];
+  V3<T> ab = b.w - a.w, ac = c.w - a.w, ad = d.w - a.w;
+  if (std::fabs(dot(cross(ab, ac), ad)) < ORL(1e-12)) {
+    SupportVertex<T> faces[4][3] = {{a, b, c}, {a, b, d}, {a, c, d}, {b, c, d}};
+    int best_face = 0;
+    T best_v2 = std::numeric_limits<T>::infinity();
+    V3<T> best_q;
+    T best_w[3];
+    closest_triangle_bary(faces[0][0], faces[0][1], faces[0][2], best_q, best_w);
+    for (int f = 1; f < 4; ++f) {
+      V3<T> q;
+      T w[3];
+      closest_triangle_bary(faces[f][0], faces[f][1], faces[f][2], q, w);
+      T v2 = len2(q);
+      if (v2 < best_v2) {  // best_v2 starts at +inf (the face-0 result is never scored)
+        best_v2 = v2;
+        best_q = q;
+        best_w[0] = w[0]; best_w[1] = w[1]; best_w[2] = w[2];
+        best_face = f;
+      }
+    }
+    SupportVertex<T> kept[4] = {buf[0], buf[0], buf[0], buf[0]};
+    int k = 0;
+    for (int i = 0; i < 3; ++i)
+      if (best_w[i] > T(0)) kept[k++] = faces[best_face][i];
+    if (k == 0) { k = 1; kept[0] = faces[best_face][0]; }
+    for (int i = 0; i < k; ++i) buf[i] = kept[i];
+    n = k;
+    V3<T> p_b = (faces[best_face][0].p_b * best_w[0] + faces[best_face][1].p_b * best_w[1]) +
+                faces[best_face][2].p_b * best_w[2];
+    return {false, best_q, p_b};
+  }
+  V3<T> abc = cross(ab, ac), acd = cross(ac, ad), adb = cross(ad, ab);
+  V3<T> ao = -a.w, bo = -b.w;
+  V3<T> bcd = cross(c.w - b.w, d.w - b.w);
+  if (dot(abc, ao) * dot(abc, ad) < T(0)) return triangle_step(buf, n, a, b, c);
+  if (dot(acd, ao) * dot(acd, ab) < T(0)) return triangle_step(buf, n, a, c, d);
+  if (dot(adb, ao) * dot(adb, ac) < T(0)) return triangle_step(buf, n, a, d, b);
+  if (dot(bcd, bo) * dot(bcd, a.w - b.w) < T(0)) return triangle_step(buf, n, b, c, d);
+  return {true, splat(T(0)), splat(T(0))};
+}
+
+// gjk.rs:352-364
+template <class T>
+Reduce<T> reduce(SupportVertex<T>* buf, int& n) {
+  switch (n) {
+    case 1: return {false, buf[0].w, buf[0].p_b};
+    case 2: return reduce_segment(buf, n);
+    case 3: return reduce_triangle(buf, n);
+    default: return reduce_tetrahedron(buf, n);
+  }
+}
+
+// gjk.rs:668-680
+template <class T>
+V3<T> perpendicular(V3<T> v) {
+  V3<T> X = mk(T(1), T(0), T(0)), Y = mk(T(0), T(1), T(0));
+  V3<T> helper = std::fabs(v.x) < ORL(0.9) ? X : Y;
+  V3<T> perp = cross(v, helper);
+  if (len2(perp) > T(0)) return normalize(perp);
+  V3<T> alt = std::fabs(v.x) < ORL(0.9) ? Y : X;
+  perp = cross(v, alt);
+  if (len2(perp) > T(0)) return normalize(perp);
+  return mk(T(0), T(0), T(1));
+}
+
+// Strict-walk simplex, gjk.rs:17-51 (pts[0] is the most recent point).
+template <class T>
+struct Simplex {
+  V3<T> pts[4];
+  int n = 0;
+  void init(V3<T> p) { pts[0] = pts[1] = pts[2] = pts[3] = p; n = 1; }
+  void add(V3<T> v) {
+    for (int i = n; i > 0; --i) pts[i] = pts[i - 1];
+    pts[0] = v;
+    ++n;
+  }
+  void set2(V3<T> a, V3<T> b) { pts[0] = a; pts[1] = b; n = 2; }
+  void set3(V3<T> a, V3<T> b, V3<T> c) { pts[0] = a; pts[1] = b; pts[2] = c; n = 3; }
+  void set1(V3<T> a) { pts[0] = a; n = 1; }
+};
+
+// gjk.rs:785-807
+template <class T>
+bool shrink_or_normal(Simplex<T>& s, V3<T>& dir, V3<T> closest, V3<T> normal, V3<T> n0, V3<T> n1) {
+  T span2 = T(0);
+  for (int i = 0; i < s.n; ++i) span2 = std::fmax(span2, len2(s.pts[i]));
+  if (len2(closest) < span2 * ORL(1e-12)) {
+    dir = normal;
+  } else {
+    s.set2(n0, n1);
+    dir = -closest;
+  }
+  return false;
+}
+
+// gjk.rs:624-646
+template <class T>
+bool solve_segment(Simplex<T>& s, V3<T>& dir) {
+  V3<T> a = s.pts[1], b = s.pts[0];
+  V3<T> ab = b - a, ao = -a;
+  if (dot(ab, ao) > T(0)) {
+    V3<T> cr = cross(ab, ao);
+    if (len2(cr) > len2(ab) * len2(ao) * ORL(1e-10)) dir = cross(cr, ab);
+    else dir = perpendicular(ab);
+  } else {
+    s.set1(a);
+    dir = ao;
+  }
+  return false;
+}
+
+// gjk.rs:687-773
+template <class T>
+bool solve_triangle(Simplex<T>& s, V3<T>& dir) {
+  V3<T> a = s.pts[2], b = s.pts[1], c = s.pts[0];
+  V3<T> ab = b - a, ac = c - a;
+  V3<T> normal = cross(ab, ac);
+  V3<T> ap = -a;
+  T d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= T(0) && d2 <= T(0)) { s.set1(a); dir = -a; return false; }
+  V3<T> bp = -b;
+  T d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= T(0) && d4 <= d3) { s.set1(b); dir = -b; return false; }
+  T vc = d1 * d4 - d3 * d2;
+  if (vc <= T(0) && d1 >= T(0) && d3 <= T(0)) {
+    T v = d1 == d3 ? T(0) : d1 / (d1 - d3);
+    V3<T> closest = a + ab * v;
+    return shrink_or_normal(s, dir, closest, normal, b, a);
+  }
+  V3<T> cp = -c;
+  T d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= T(0) && d5 <= d6) { s.set1(c); dir = -c; return false; }
+  T vb = d5 * d2 - d1 * d6;
+  if (vb <= T(0) && d2 >= T(0) && d6 <= T(0)) {
+    T w = d2 == d6 ? T(0) : d2 / (d2 - d6);
+    V3<T> closest = a + ac * w;
+    return shrink_or_normal(s, dir, closest, normal, c, a);
+  }
+  T va = d3 * d6 - d5 * d4;
+  if (va <= T(0) && (d4 - d3) >= T(0) && (d5 - d6) >= T(0)) {
+    T w = (d4 - d3) + (d5 - d6) == T(0) ? T(0) : (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    V3<T> closest = b + (c - b) * w;
+    return shrink_or_normal(s, dir, closest, normal, c, b);
+  }
+  T sum = va + vb + vc;
+  if (sum <= T(0) || std::isnan(sum)) { dir = normal; return false; }
+  T denom = T(1) / sum;
+  T v = vb * denom, w = vc * denom;
+  V3<T> closest = (a + ab * v) + ac * w;
+  T span2 = std::fmax(std::fmax(len2(a), len2(b)), len2(c));
+  if (len2(closest) < span2 * ORL(1e-12)) dir = normal;
+  else dir = -closest;
+  return false;
+}
+
+// gjk.rs:849-858
+template <class T> inline V3<T> toward_origin(V3<T> n, V3<T> t) { return dot(n, t) < T(0) ? -n : n; }
+
+// gjk.rs:811-846
+template <class T>
+bool solve_tetrahedron(Simplex<T>& s, V3<T>& dir) {
+  V3<T> a = s.pts[3], b = s.pts[2], c = s.pts[1], d = s.pts[0];
+  V3<T> ab = b - a, ac = c - a, ad = d - a, ao = -a, bo = -b;
+  V3<T> abc = cross(ab, ac), acd = cross(ac, ad), adb = cross(ad, ab);
+  V3<T> bcd = cross(b - c, d - c);
+  if (dot(abc, ao) * dot(abc, ad) < T(0)) { s.set3(c, b, a); dir = toward_origin(abc, ao); return false; }
+  if (dot(acd, ao) * dot(acd, ab) < T(0)) { s.set3(d, c, a); dir = toward_origin(acd, ao); return false; }
+  if (dot(adb, ao) * dot(adb, ac) < T(0)) { s.set3(d, b, a); dir = toward_origin(adb, ao); return false; }
+  if (dot(bcd, bo) * dot(bcd, a - b) < T(0)) { s.set3(d, c, b); dir = toward_origin(bcd, bo); return false; }
+  return true;
+}
+
+// gjk.rs:609-620
+template <class T>
+bool solve(Simplex<T>& s, V3<T>& dir) {
+  switch (s.n) {
+    case 2: return solve_segment(s, dir);
+    case 3: return solve_triangle(s, dir);
+    case 4: return solve_tetrahedron(s, dir);
+    default: dir = -s.pts[0]; return false;
+  }
+}
+
+// gjk.rs:299-342 (strict origin-enclosing walk).
+template <class T, class F>
+bool enclosing_walk(F support, V3<T> fallback, const V3<T>* seed, Simplex<T>& simplex) {
+  V3<T> initial = (seed && len2(*seed) > ORL(1e-12)) ? *seed : fallback;
+  simplex.init(support(initial));
+  V3<T> dir = -simplex.pts[0];
+  for (int it = 0; it < 64; ++it) {
+    T dir2 = len2(dir);
+    if (dir2 <= T(0) || std::isnan(dir2)) dir = perpendicular(simplex.pts[0]);
+    V3<T> a = support(dir);
+    if (dot(a, dir) < T(0)) return false;
+    simplex.add(a);
+    if (solve(simplex, dir)) return true;
+    V3<T> reference = simplex.n >= 2 ? simplex.pts[0] - simplex.pts[simplex.n - 1] : simplex.pts[0];
+    if (len2(dir) < len2(reference) * ORL(1e-12)) dir = perpendicular(reference);
+  }
+  return false;
+}
+
+// gjk.rs:76-89 core Minkowski support.
+template <class T>
+struct PairPose {
+  const Shape<T>* a;
+  V3<T> pa;
+  Quat<T> qa;
+  const Shape<T>* b;
+  V3<T> pb;
+  Quat<T> qb;
+  SupportVertex<T> core_support(V3<T> dir) const {
+    V3<T> p_a = a->core_support_world(pa, qa, dir);
+    V3<T> p_b = b->core_support_world(pb, qb, -dir);
+    return SupportVertex<T>{p_a - p_b, p_b};
+  }
+  V3<T> full_support(V3<T> dir) const {
+    return a->support_world(pa, qa, dir) - b->support_world(pb, qb, -dir);
+  }
+};
+
+// gjk.rs:171-288 -- the margin-core distance walk.
+template <class T>
+GjkOut<T> gjk_contact(const PairPose<T>& pr, const V3<T>* seed, T threshold) {
+  GjkOut<T> out;
+  T margin_a = pr.a->margin(), margin_b = pr.b->margin();
+  T margin_sum = margin_a + margin_b;
+  V3<T> center_dir = pr.pb - pr.pa;
+  V3<T> fallback = len2(center_dir) > ORL(1e-12) ? center_dir : mk(T(1), T(0), T(0));
+  V3<T> initial = (seed && len2(*seed) > ORL(1e-12)) ? *seed : fallback;
+  SupportVertex<T> first = pr.core_support(initial);
+  SupportVertex<T> buf[4] = {first, first, first, first};
+  int n = 1;
+  V3<T> dir = -first.w;
+  T touch2 = ORL(1e-12) * std::fmax(len2(first.w), T(1));
+  const T REL_ERROR2 = ORL(1e-6);
+  for (int it = 0; it < 64; ++it) {
+    out.iterations = it + 1;
+    Reduce<T> r = reduce(buf, n);
+    if (r.inside) {
+      out.kind = 2;
+      for (int i = 0; i < 4; ++i) out.simplex[i] = buf[i].w;
+      return out;
+    }
+    V3<T> v = r.v;
+    T v2 = len2(v);
+    if (v2 < touch2) {
+      Simplex<T> s;
+      auto sup = [&](V3<T> d) { return pr.core_support(d).w; };
+      bool ok = enclosing_walk<T>(sup, fallback, &dir, s);
+      if (ok && s.n == 4) {
+        out.kind = 2;
+        for (int i = 0; i < 4; ++i) out.simplex[i] = s.pts[i];
+      } else {
+        out.kind = 0;
+      }
+      return out;
+    }
+    dir = -v;
+    SupportVertex<T> w = pr.core_support(dir);
+    T delta = dot(v, w.w);
+    T band = margin_sum + threshold;
+    if (delta > T(0) && delta * delta > v2 * band * band) { out.kind = 0; return out; }
+    bool duplicate = false;
+    for (int i = 0; i < n; ++i)
+      if (abs_diff_eq(w.w, buf[i].w, ORL(1e-6))) duplicate = true;
+    if (duplicate || delta >= v2 * (T(1) - REL_ERROR2)) {
+      T core_distance = std::sqrt(v2);
+      T distance = core_distance - margin_sum;
+      if (distance < threshold) {
+        V3<T> normal = dir / core_distance;
+        out.kind = 1;
+        out.distance = distance;
+        out.normal = normal;
+        out.point_on_b = r.p_b - normal * margin_b;
+        return out;
+      }
+      out.kind = 0;
+      return out;
+    }
+    buf[n] = w;
+    ++n;
+  }
+  out.kind = 0;
+  return out;
+}
+
+// gjk.rs:590-607 (full-shape strict walk).
+template <class T>
+bool gjk_intersect(const PairPose<T>& pr, const V3<T>* seed, Simplex<T>& s) {
+  V3<T> center_dir = pr.pb - pr.pa;
+  V3<T> fallback = len2(center_dir) > ORL(1e-12) ? center_dir : mk(T(1), T(0), T(0));
+  auto sup = [&](V3<T> d) { return pr.full_support(d); };
+  return enclosing_walk<T>(sup, fallback, seed, s);
+}
+
+// ---------------------------------------------------------------- epa.rs --
+enum EpaStatus { EPA_CONVERGED = 0, EPA_DEGENERATE = 1, EPA_ITERATION_LIMIT = 2, EPA_VERTEX_LIMIT = 3 };
+
+template <class T>
+struct Face {
+  int a, b, c;
+  V3<T> normal;
+  T distance, plane_distance;
+};
+
+// epa.rs:283-303
+template <class T>
+bool edge_distance(V3<T> pa, V3<T> pb, V3<T> normal, T& out) {
+  V3<T> ba = pb - pa;
+  V3<T> n_ab = cross(ba, normal);
+  if (dot(pa, n_ab) < T(0)) {
+    T ba_l2 = len2(ba);
+    T a_dot_ba = dot(pa, ba);
+    T b_dot_ba = dot(pb, ba);
+    if (a_dot_ba > T(0)) out = len(pa);
+    else if (b_dot_ba < T(0)) out = len(pb);
+    else {
+      T a_dot_b = dot(pa, pb);
+      out = std::sqrt(fmax_((len2(pa) * len2(pb) - a_dot_b * a_dot_b) / ba_l2, T(0)));
+    }
+    return true;
+  }
+  return false;
+}
+
+// epa.rs:262-278
+template <class T>
+T face_distance(V3<T> normal, V3<T> pa, V3<T> pb, V3<T> pc) {
+  T d;
+  if (edge_distance(pa, pb, normal, d)) return d;
+  if (edge_distance(pb, pc, normal, d)) return d;
+  if (edge_distance(pc, pa, normal, d)) return d;
+  return dot(pa, normal);
+}
+
+// epa.rs:395-398
+template <class T> inline bool face_visible(const Face<T>& f, V3<T> p) {
+  return dot(f.normal, p) - f.plane_distance > ORL(1e-5);
+}
+template <> inline bool face_visible<double>(const Face<double>& f, V3<double> p) {
+  return dot(f.normal, p) - f.plane_distance > 1e-14;
+}
+
+// epa.rs:309-388 (two-phase, transactional expansion)
+template <class T>
+bool expand(std::vector<Face<T>>& faces, const std::vector<V3<T>>& verts, std::vector<std::pair<int, int>>& boundary,
+            V3<T> new_point, int new_idx) {
+  boundary.clear();
+  bool any_visible = false;
+  for (const auto& f : faces) {
+    if (!face_visible(f, new_point)) continue;
+    any_visible = true;
+    int e[3][2] = {{f.a, f.b}, {f.b, f.c}, {f.c, f.a}};
+    for (auto& uv : e) {
+      int u = uv[0], v = uv[1];
+      int found = -1;
+      for (size_t i = 0; i < boundary.size(); ++i)
+        if (boundary[i].first == v && boundary[i].second == u) { found = (int)i; break; }
+      if (found >= 0) {
+        boundary[found] = boundary.back();  // swap_remove
+        boundary.pop_back();
+      } else {
+        boundary.push_back({u, v});
+      }
+    }
+  }
+  if (!any_visible) return true;
+  for (auto& uv : boundary) {
+    V3<T> cr = cross(verts[uv.second] - verts[uv.first], new_point - verts[uv.first]);
+    if (len(cr) < ORL(1e-12)) return false;
+  }
+  std::vector<Face<T>> kept;
+  kept.reserve(faces.size());
+  for (const auto& f : faces)
+    if (!face_visible(f, new_point)) kept.push_back(f);
+  faces.swap(kept);
+  for (auto& uv : boundary) {
+    int u = uv.first, v = uv.second;
+    V3<T> cr = cross(verts[v] - verts[u], new_point - verts[u]);
+    T l = len(cr);
+    V3<T> normal = cr / l;
+    T distance = face_distance(normal, verts[u], verts[v], new_point);
+    faces.push_back(Face<T>{u, v, new_idx, normal, distance, dot(normal, verts[u])});
+  }
+  boundary.clear();
+  return true;
+}
+
+// epa.rs:212-224 (first minimum of |distance|)
+template <class T>
+int find_best(const std::vector<Face<T>>& faces) {
+  int best = 0;
+  for (size_t i = 1; i < faces.size(); ++i)
+    if (std::fabs(faces[best].distance) > std::fabs(faces[i].distance)) best = (int)i;
+  return best;
+}
+
+// epa.rs:226-257
+template <class T>
+void build_initial_faces(const V3<T>* s, std::vector<Face<T>>& out) {
+  const int quads[4][4] = {{0, 1, 2, 3}, {0, 3, 1, 2}, {0, 2, 3, 1}, {1, 3, 2, 0}};
+  for (auto& q : quads) {
+    int a = q[0], b = q[1], c = q[2], opp = q[3];
+    V3<T> normal = cross(s[b] - s[a], s[c] - s[a]);
+    if (dot(normal, s[opp] - s[a]) > T(0)) {
+      normal = -normal;
+      std::swap(b, c);
+    }
+    T l = len(normal);
+    if (l < ORL(1e-12)) continue;
+    normal = normal / l;
+    out.push_back(Face<T>{a, b, c, normal, face_distance(normal, s[a], s[b], s[c]), dot(normal, s[a])});
+  }
+}
+
+template <class T>
+struct EpaOut {
+  int status;
+  V3<T> normal;
+  T depth;
+  int iterations;
+  int vertices;
+};
+
+// epa.rs:127-205
+template <class T, class F>
+EpaOut<T> epa_penetration(const V3<T>* simplex, F support) {
+  const T EPSILON = std::is_same<T, float>::value ? (T)1e-4f : (T)1e-12;
+  std::vector<V3<T>> verts(simplex, simplex + 4);
+  std::vector<Face<T>> faces;
+  std::vector<std::pair<int, int>> boundary;
+  build_initial_faces(simplex, faces);
+  if (faces.empty()) return EpaOut<T>{EPA_DEGENERATE, splat(T(0)), T(0), 0, 4};
+  int status = EPA_ITERATION_LIMIT;
+  V3<T> dirs[6] = {mk(T(1), T(0), T(0)), mk(T(-1), T(0), T(0)), mk(T(0), T(1), T(0)),
+                   mk(T(0), T(-1), T(0)), mk(T(0), T(0), T(1)), mk(T(0), T(0), T(-1))};
+  bool seeded_ok = true;
+  for (auto& d : dirs) {
+    int idx = (int)verts.size();
+    V3<T> p = support(d);
+    verts.push_back(p);
+    if (!expand(faces, verts, boundary, p, idx)) {
+      status = EPA_DEGENERATE;
+      seeded_ok = false;
+      break;
+    }
+  }
+  (void)seeded_ok;
+  int it = 0;
+  for (; it < 255; ++it) {
+    int bi = find_best(faces);
+    V3<T> bn = faces[bi].normal;
+    T bd = faces[bi].distance;
+    V3<T> np = support(bn);
+    T wdist = dot(bn, np) - bd;
+    if (wdist <= EPSILON) return EpaOut<T>{EPA_CONVERGED, bn, bd, it + 1, (int)verts.size()};
+    if ((int)verts.size() >= 128) { status = EPA_VERTEX_LIMIT; break; }
+    int nidx = (int)verts.size();
+    verts.push_back(np);
+    if (!expand(faces, verts, boundary, np, nidx)) { status = EPA_DEGENERATE; break; }
+  }
+  int bi = find_best(faces);
+  return EpaOut<T>{status, faces[bi].normal, faces[bi].distance, it, (int)verts.size()};
+}
+
+// ------------------------------------------------------ convex_convex.rs --
+// narrowphase/convex_convex.rs:30-155
+template <class T>
+bool convex_convex(const PairPose<T>& pr, const V3<T>* seed, T threshold, Manifold<T>& m, int* path = nullptr) {
+  GjkOut<T> g = gjk_contact(pr, seed, threshold);
+  if (g.kind == 0) { if (path) *path = 0; return false; }
+  if (g.kind == 1) {
+    if (path) *path = 1;
+    m.push(g.point_on_b, g.normal, -g.distance);
+    return true;
+  }
+  if (path) *path = 2;
+  auto sup = [&](V3<T> d) { return pr.core_support(d).w; };
+  EpaOut<T> e = epa_penetration<T>(g.simplex, sup);
+  if (e.depth <= T(0)) return false;
+  T depth = (e.depth + pr.a->margin()) + pr.b->margin();
+  V3<T> point_a = pr.a->support_world(pr.pa, pr.qa, e.normal);
+  V3<T> position = point_a - e.normal * depth;
+  m.push(position, e.normal, depth);
+  return true;
+}
+
+// ---------------------------------------------------------- dispatch ----
+// narrowphase/mod.rs:52-57
+template <class T>
+T pair_threshold(const Shape<T>& a, const Shape<T>& b) {
+  return fmin_(a.contact_breaking_threshold(), b.contact_breaking_threshold());
+}
+
+// narrowphase/mod.rs:63-286, restricted to the convex primitives and planes
+// (compound and concave arms are outside this path).  `threshold_override`
+// (NaN = unused) replaces the pair band, which lets the signed-distance
+// callers pass +inf to the same walk.
+template <class T>
+bool collide_seeded(const Shape<T>& sa, V3<T> pa, Quat<T> qa, const Shape<T>& sb, V3<T> pb, Quat<T> qb,
+                    const V3<T>* seed, Manifold<T>& m, int* path = nullptr) {
+  T threshold = pair_threshold(sa, sb);
+  m.n = 0;
+  if (path) *path = 3;
+  if (sa.kind == PLANE && sb.kind == PLANE) return false;
+  if (sa.kind == PLANE) return plane_convex(sa, pa, qa, sb, pb, qb, threshold, m);
+  if (sb.kind == PLANE) {
+    bool r = plane_convex(sb, pb, qb, sa, pa, qa, threshold, m);
+    if (r) m.flip();
+    return r;
+  }
+  if (sa.kind == SPHERE && sb.kind == SPHERE) return sphere_sphere(sa, pa, sb, pb, m);
+  if (sa.kind == SPHERE && sb.kind == BOX) {
+    bool r = sphere_box(sa, pa, sb, pb, qb, threshold, m);
+    if (r) m.flip();
+    return r;
+  }
+  if (sa.kind == BOX && sb.kind == SPHERE) return sphere_box(sb, pb, sa, pa, qa, threshold, m);
+  if (sa.kind == BOX && sb.kind == BOX) return box_box(sa, pa, qa, sb, pb, qb, m);
+  PairPose<T> pr{&sa, pa, qa, &sb, pb, qb};
+  return convex_convex(pr, seed, threshold, m, path);
+}
+
+}  // namespace oracle

@@ -1,0 +1,305 @@
+"""ctypes bridge to the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+The oracle is a C++ restatement of the reference collision path
+(crates/bullet-collision, see oracle.hpp for the per-function citations).
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may load
+it; the shipped library never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+SPHERE, BOX, CAPSULE, HULL, PLANE = 0, 1, 2, 3, 4
+
+
+class OrShape(C.Structure):
+    _fields_ = [("type", C.c_int32), ("p", C.c_float * 4),
+                ("hull", C.POINTER(C.c_float)), ("nhull", C.c_int32)]
+
+
+class OrScene(C.Structure):
+    _fields_ = [("n", C.c_int64), ("pos", C.c_void_p), ("quat", C.c_void_p),
+                ("type", C.c_void_p), ("params", C.c_void_p),
+                ("hull_first", C.c_void_p), ("hull_count", C.c_void_p),
+                ("hull_verts", C.c_void_p), ("n_hull_verts", C.c_int64)]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        f32p, f64p = C.POINTER(C.c_float), C.POINTER(C.c_double)
+        sp = C.POINTER(OrShape)
+        L.oracle_threshold_f32.restype = C.c_float
+        L.oracle_threshold_f32.argtypes = [sp]
+        L.oracle_world_aabb_f32.argtypes = [sp, f32p, f32p, f32p]
+        L.oracle_support_world_f32.argtypes = [sp, f32p, f32p, f32p, C.c_int, f32p]
+        L.oracle_collide_f32.argtypes = [sp, f32p, f32p, sp, f32p, f32p, f32p, f32p, C.POINTER(C.c_int)]
+        L.oracle_collide_f64.argtypes = [sp, f64p, f64p, sp, f64p, f64p, f64p, f64p, C.POINTER(C.c_int)]
+        L.oracle_convex_convex_f32.argtypes = [sp, f32p, f32p, sp, f32p, f32p, f32p, C.c_float, f32p,
+                                               C.POINTER(C.c_int)]
+        L.oracle_gjk_contact_f32.argtypes = [sp, f32p, f32p, sp, f32p, f32p, f32p, C.c_float, f32p, f32p]
+        L.oracle_gjk_contact_f64.argtypes = [sp, f64p, f64p, sp, f64p, f64p, f64p, C.c_double, f64p]
+        L.oracle_gjk_intersect_f32.argtypes = [sp, f32p, f32p, sp, f32p, f32p, f32p]
+        L.oracle_epa_pair_f32.argtypes = [f32p, sp, f32p, f32p, sp, f32p, f32p, C.c_int, f32p]
+        L.oracle_epa_points_f32.argtypes = [f32p, f32p, C.c_int, f32p]
+        scp = C.POINTER(OrScene)
+        L.oracle_scene_aabbs_f32.argtypes = [scp, C.c_float, C.c_void_p, C.c_void_p]
+        L.oracle_scene_thresholds_f32.argtypes = [scp, C.c_void_p]
+        L.oracle_brute_pairs.restype = C.c_int64
+        L.oracle_brute_pairs.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.oracle_sweep_pairs.restype = C.c_int64
+        L.oracle_sweep_pairs.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.oracle_collide_pairs_f32.argtypes = [scp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                               C.c_void_p]
+        L.oracle_collide_pairs_f64.argtypes = [scp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+        L.oracle_pass_f32.restype = C.c_int64
+        L.oracle_pass_f32.argtypes = [scp, C.c_float, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+# ------------------------------------------------------------------ shapes --
+class Shape:
+    """Single-shape descriptor for pair-level oracle calls."""
+
+    def __init__(self, type_: int, params=(0, 0, 0, 0), hull=None):
+        self.type = type_
+        self.params = list(params) + [0.0] * (4 - len(params))
+        self.hull = None if hull is None else np.ascontiguousarray(hull, dtype=np.float32).reshape(-1, 3)
+
+    def c(self) -> OrShape:
+        s = OrShape()
+        s.type = self.type
+        for i in range(4):
+            s.p[i] = float(self.params[i])
+        if self.hull is not None:
+            s.hull = self.hull.ctypes.data_as(C.POINTER(C.c_float))
+            s.nhull = self.hull.shape[0]
+        else:
+            s.nhull = 0
+        return s
+
+
+def sphere(r):
+    return Shape(SPHERE, (r,))
+
+
+def box(hx, hy=None, hz=None):
+    if hy is None:
+        hy = hz = hx
+    return Shape(BOX, (hx, hy, hz))
+
+
+def capsule(r, hh):
+    return Shape(CAPSULE, (r, hh))
+
+
+def plane(n, c):
+    return Shape(PLANE, (n[0], n[1], n[2], c))
+
+
+def hull(points):
+    return Shape(HULL, (), points)
+
+
+IDENT = (0.0, 0.0, 0.0, 1.0)
+
+
+def _f32(v, n):
+    a = (C.c_float * n)()
+    for i in range(n):
+        a[i] = float(v[i])
+    return a
+
+
+def _f64(v, n):
+    a = (C.c_double * n)()
+    for i in range(n):
+        a[i] = float(v[i])
+    return a
+
+
+def threshold(s: Shape) -> float:
+    sc = s.c()
+    return lib().oracle_threshold_f32(C.byref(sc))
+
+
+def world_aabb(s: Shape, pos, quat=IDENT):
+    sc = s.c()
+    out = (C.c_float * 6)()
+    lib().oracle_world_aabb_f32(C.byref(sc), _f32(pos, 3), _f32(quat, 4), out)
+    return np.array(out[:3], np.float32), np.array(out[3:], np.float32)
+
+
+def support_world(s: Shape, pos, quat, d, core=False):
+    sc = s.c()
+    out = (C.c_float * 3)()
+    lib().oracle_support_world_f32(C.byref(sc), _f32(pos, 3), _f32(quat, 4), _f32(d, 3), int(core), out)
+    return np.array(out[:], np.float32)
+
+
+def collide(a: Shape, pa, qa, b: Shape, pb, qb, seed=None, f64=False):
+    """collide_seeded (narrowphase/mod.rs:63-286) -> (points[k,7], path)."""
+    ac, bc = a.c(), b.c()
+    path = C.c_int(0)
+    if f64:
+        out = (C.c_double * 28)()
+        n = lib().oracle_collide_f64(C.byref(ac), _f64(pa, 3), _f64(qa, 4), C.byref(bc), _f64(pb, 3),
+                                     _f64(qb, 4), None if seed is None else _f64(seed, 3), out, C.byref(path))
+        return np.array(out[:], np.float64).reshape(4, 7)[:n], path.value
+    out = (C.c_float * 28)()
+    n = lib().oracle_collide_f32(C.byref(ac), _f32(pa, 3), _f32(qa, 4), C.byref(bc), _f32(pb, 3), _f32(qb, 4),
+                                 None if seed is None else _f32(seed, 3), out, C.byref(path))
+    return np.array(out[:], np.float32).reshape(4, 7)[:n], path.value
+
+
+def convex_convex(a: Shape, pa, qa, b: Shape, pb, qb, seed=None, threshold=-1.0):
+    """convex_convex (convex_convex.rs:30-155) -> (points[k,7], path)."""
+    ac, bc = a.c(), b.c()
+    out = (C.c_float * 28)()
+    path = C.c_int(0)
+    n = lib().oracle_convex_convex_f32(C.byref(ac), _f32(pa, 3), _f32(qa, 4), C.byref(bc), _f32(pb, 3),
+                                       _f32(qb, 4), None if seed is None else _f32(seed, 3), threshold, out,
+                                       C.byref(path))
+    return np.array(out[:], np.float32).reshape(4, 7)[:n], path.value
+
+
+def gjk_contact(a: Shape, pa, qa, b: Shape, pb, qb, threshold, seed=None, f64=False):
+    """gjk_contact (gjk.rs:171-288) -> (kind, distance, normal, point_on_b, simplex)."""
+    ac, bc = a.c(), b.c()
+    if f64:
+        out = (C.c_double * 7)()
+        k = lib().oracle_gjk_contact_f64(C.byref(ac), _f64(pa, 3), _f64(qa, 4), C.byref(bc), _f64(pb, 3),
+                                         _f64(qb, 4), None if seed is None else _f64(seed, 3), threshold, out)
+        o = np.array(out[:], np.float64)
+        return k, o[0], o[1:4], o[4:7], None
+    out = (C.c_float * 7)()
+    simp = (C.c_float * 12)()
+    k = lib().oracle_gjk_contact_f32(C.byref(ac), _f32(pa, 3), _f32(qa, 4), C.byref(bc), _f32(pb, 3), _f32(qb, 4),
+                                     None if seed is None else _f32(seed, 3), threshold, out, simp)
+    o = np.array(out[:], np.float32)
+    return k, o[0], o[1:4], o[4:7], np.array(simp[:], np.float32).reshape(4, 3)
+
+
+def gjk_intersect(a: Shape, pa, qa, b: Shape, pb, qb):
+    ac, bc = a.c(), b.c()
+    simp = (C.c_float * 12)()
+    n = lib().oracle_gjk_intersect_f32(C.byref(ac), _f32(pa, 3), _f32(qa, 4), C.byref(bc), _f32(pb, 3),
+                                       _f32(qb, 4), simp)
+    return n, np.array(simp[:], np.float32).reshape(4, 3)
+
+
+def epa_pair(simplex, a: Shape, pa, qa, b: Shape, pb, qb, core=False):
+    ac, bc = a.c(), b.c()
+    out = (C.c_float * 4)()
+    st = lib().oracle_epa_pair_f32(_f32(np.asarray(simplex).ravel(), 12), C.byref(ac), _f32(pa, 3), _f32(qa, 4),
+                                   C.byref(bc), _f32(pb, 3), _f32(qb, 4), int(core), out)
+    return st, np.array(out[:3], np.float32), float(out[3])
+
+
+def epa_points(simplex, points):
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    out = (C.c_float * 4)()
+    st = lib().oracle_epa_points_f32(_f32(np.asarray(simplex).ravel(), 12),
+                                     pts.ctypes.data_as(C.POINTER(C.c_float)), pts.shape[0], out)
+    return st, np.array(out[:3], np.float32), float(out[3])
+
+
+# ------------------------------------------------------------------ scenes --
+class _SceneRef:
+    """Keeps the numpy arrays of a scene alive behind an OrScene struct."""
+
+    def __init__(self, scene):
+        self.arrs = dict(
+            pos=np.ascontiguousarray(scene["position"], np.float32),
+            quat=np.ascontiguousarray(scene["orientation"], np.float32),
+            type=np.ascontiguousarray(scene["shape_type"], np.uint8),
+            params=np.ascontiguousarray(scene["shape_params"], np.float32),
+            hull_first=np.ascontiguousarray(scene["hull_first"], np.uint32),
+            hull_count=np.ascontiguousarray(scene["hull_count"], np.uint32),
+            hull_verts=np.ascontiguousarray(scene["hull_vertices"], np.float32).reshape(-1, 3),
+        )
+        s = OrScene()
+        s.n = self.arrs["type"].shape[0]
+        for k in ("pos", "quat", "type", "params", "hull_first", "hull_count", "hull_verts"):
+            setattr(s, k, self.arrs[k].ctypes.data)
+        s.n_hull_verts = self.arrs["hull_verts"].shape[0]
+        self.c = s
+
+
+def scene_aabbs(scene, margin):
+    r = _SceneRef(scene)
+    n = r.c.n
+    mn = np.empty((n, 3), np.float32)
+    mx = np.empty((n, 3), np.float32)
+    lib().oracle_scene_aabbs_f32(C.byref(r.c), margin, mn.ctypes.data, mx.ctypes.data)
+    return mn, mx
+
+
+def scene_thresholds(scene):
+    r = _SceneRef(scene)
+    out = np.empty(r.c.n, np.float32)
+    lib().oracle_scene_thresholds_f32(C.byref(r.c), out.ctypes.data)
+    return out
+
+
+def brute_pairs(mn, mx):
+    mn = np.ascontiguousarray(mn, np.float32)
+    mx = np.ascontiguousarray(mx, np.float32)
+    n = mn.shape[0]
+    cnt = lib().oracle_brute_pairs(n, mn.ctypes.data, mx.ctypes.data, None, 0)
+    out = np.empty((max(cnt, 1), 2), np.uint32)
+    lib().oracle_brute_pairs(n, mn.ctypes.data, mx.ctypes.data, out.ctypes.data, cnt)
+    return out[:cnt]
+
+
+def sweep_pairs(mn, mx):
+    mn = np.ascontiguousarray(mn, np.float32)
+    mx = np.ascontiguousarray(mx, np.float32)
+    n = mn.shape[0]
+    cnt = lib().oracle_sweep_pairs(n, mn.ctypes.data, mx.ctypes.data, None, 0)
+    out = np.empty((max(cnt, 1), 2), np.uint32)
+    lib().oracle_sweep_pairs(n, mn.ctypes.data, mx.ctypes.data, out.ctypes.data, cnt)
+    return out[:cnt]
+
+
+def collide_pairs(scene, pairs, seeds=None, f64=False):
+    """collide_seeded(body i, body j) per pair -> (count[P], points[P,4,7], path[P])."""
+    r = _SceneRef(scene)
+    pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+    p = pairs.shape[0]
+    cnt = np.zeros(p, np.uint8)
+    if f64:
+        pts = np.zeros((p, 4, 7), np.float64)
+        lib().oracle_collide_pairs_f64(C.byref(r.c), pairs.ctypes.data, p, cnt.ctypes.data, pts.ctypes.data)
+        return cnt, pts, None
+    pts = np.zeros((p, 4, 7), np.float32)
+    path = np.zeros(p, np.int8)
+    sd = None if seeds is None else np.ascontiguousarray(seeds, np.float32)
+    lib().oracle_collide_pairs_f32(C.byref(r.c), pairs.ctypes.data, p, None if sd is None else sd.ctypes.data,
+                                   cnt.ctypes.data, pts.ctypes.data, path.ctypes.data)
+    return cnt, pts, path
+
+
+def collision_pass(scene, margin):
+    """Timed CPU pass (AABBs + exact pairs + narrowphase): (pairs, contacts, seconds)."""
+    r = _SceneRef(scene)
+    nc = C.c_int64(0)
+    sec = C.c_double(0)
+    npairs = lib().oracle_pass_f32(C.byref(r.c), margin, C.byref(nc), C.byref(sec))
+    return int(npairs), int(nc.value), float(sec.value)

@@ -1,0 +1,408 @@
+// oracle_capi.cpp -- TEST INFRASTRUCTURE ONLY: C entry points over
+// oracle.hpp for the Python checkers (ctypes).  See oracle.hpp's header.
+#include <chrono>
+#include <cstdio>
+#include <numeric>
+
+#include "oracle.hpp"
+
+using namespace oracle;
+
+extern "C" {
+
+// A body/shape as the tests describe it: type (0 sphere, 1 box, 2 capsule,
+// 3 hull, 4 plane), params (sphere r | box hx,hy,hz | capsule r,hh |
+// plane nx,ny,nz,c), hull vertex pointer (xyz triplets) + count.
+typedef struct {
+  int32_t type;
+  float p[4];
+  const float* hull;
+  int32_t nhull;
+} or_shape;
+
+typedef struct {
+  int64_t n;
+  const float* pos;      // 3n
+  const float* quat;     // 4n (x,y,z,w)
+  const uint8_t* type;   // n
+  const float* params;   // 4n
+  const uint32_t* hull_first;  // n (hull bodies only)
+  const uint32_t* hull_count;  // n
+  const float* hull_verts;     // 3 * total
+  int64_t n_hull_verts;
+} or_scene;
+
+}  // extern "C"
+
+namespace {
+
+template <class T>
+struct HullStore {
+  std::vector<V3<T>> pts;
+};
+
+template <class T>
+Shape<T> make_shape(int type, const float* p, const float* hull, int nhull, std::vector<V3<T>>& store) {
+  switch (type) {
+    case SPHERE: return Shape<T>::sphere((T)p[0]);
+    case BOX: return Shape<T>::box(mk((T)p[0], (T)p[1], (T)p[2]));
+    case CAPSULE: return Shape<T>::capsule((T)p[0], (T)p[1]);
+    case PLANE: return Shape<T>::plane(mk((T)p[0], (T)p[1], (T)p[2]), (T)p[3]);
+    case HULL: {
+      store.resize(nhull);
+      for (int i = 0; i < nhull; ++i) store[i] = mk((T)hull[3 * i], (T)hull[3 * i + 1], (T)hull[3 * i + 2]);
+      return Shape<T>::hull(store.data(), nhull);
+    }
+  }
+  return Shape<T>::sphere(T(0));
+}
+
+template <class T>
+Shape<T> from_desc(const or_shape* s, std::vector<V3<T>>& store) {
+  return make_shape<T>(s->type, s->p, s->hull, s->nhull, store);
+}
+
+template <class T> V3<T> v3p(const float* p) { return mk((T)p[0], (T)p[1], (T)p[2]); }
+template <class T> Quat<T> q4p(const float* q) { return Quat<T>{(T)q[0], (T)q[1], (T)q[2], (T)q[3]}; }
+template <class T> V3<T> v3d(const double* p) { return mk((T)p[0], (T)p[1], (T)p[2]); }
+template <class T> Quat<T> q4d(const double* q) { return Quat<T>{(T)q[0], (T)q[1], (T)q[2], (T)q[3]}; }
+
+// All body shapes of a scene, with hull vertices converted once.
+template <class T>
+struct SceneShapes {
+  std::vector<Shape<T>> shapes;
+  std::vector<V3<T>> hull_pts;
+  std::vector<V3<T>> pos;
+  std::vector<Quat<T>> quat;
+  void build(const or_scene* sc) {
+    hull_pts.resize(sc->n_hull_verts);
+    for (int64_t i = 0; i < sc->n_hull_verts; ++i)
+      hull_pts[i] = mk((T)sc->hull_verts[3 * i], (T)sc->hull_verts[3 * i + 1], (T)sc->hull_verts[3 * i + 2]);
+    shapes.resize(sc->n);
+    pos.resize(sc->n);
+    quat.resize(sc->n);
+    for (int64_t i = 0; i < sc->n; ++i) {
+      const float* p = sc->params + 4 * i;
+      int t = sc->type[i];
+      if (t == HULL) {
+        shapes[i] = Shape<T>::hull(hull_pts.data() + sc->hull_first[i], (int)sc->hull_count[i]);
+      } else {
+        std::vector<V3<T>> dummy;
+        shapes[i] = make_shape<T>(t, p, nullptr, 0, dummy);
+      }
+      pos[i] = v3p<T>(sc->pos + 3 * i);
+      quat[i] = q4p<T>(sc->quat + 4 * i);
+    }
+  }
+};
+
+template <class T>
+void write_manifold(const Manifold<T>& m, T* out) {
+  for (int k = 0; k < m.n && k < 4; ++k) {
+    T* o = out + 7 * k;
+    o[0] = m.pts[k].position.x; o[1] = m.pts[k].position.y; o[2] = m.pts[k].position.z;
+    o[3] = m.pts[k].normal.x; o[4] = m.pts[k].normal.y; o[5] = m.pts[k].normal.z;
+    o[6] = m.pts[k].penetration;
+  }
+}
+
+// Sort-and-sweep over x with exact closed-interval filtering: the same pair
+// set as brute force (util.rs:28-38 over all i<j), ordered lexicographically.
+template <class T>
+int64_t sweep_pairs(int64_t n, const Aabb<T>* boxes, std::vector<uint64_t>& keys) {
+  std::vector<int64_t> order;
+  order.reserve(n);
+  for (int64_t i = 0; i < n; ++i)
+    if (!std::isnan(boxes[i].min.x)) order.push_back(i);
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return boxes[a].min.x < boxes[b].min.x; });
+  keys.clear();
+  for (size_t k = 0; k < order.size(); ++k) {
+    const Aabb<T>& a = boxes[order[k]];
+    for (size_t l = k + 1; l < order.size(); ++l) {
+      const Aabb<T>& b = boxes[order[l]];
+      if (b.min.x > a.max.x) break;
+      if (a.overlaps(b)) {
+        uint64_t i = (uint64_t)order[k], j = (uint64_t)order[l];
+        if (i > j) std::swap(i, j);
+        keys.push_back((i << 32) | j);
+      }
+    }
+  }
+  std::sort(keys.begin(), keys.end());
+  return (int64_t)keys.size();
+}
+
+}  // namespace
+
+extern "C" {
+
+float oracle_threshold_f32(const or_shape* s) {
+  std::vector<V3<float>> st;
+  return from_desc<float>(s, st).contact_breaking_threshold();
+}
+
+void oracle_world_aabb_f32(const or_shape* s, const float* pos, const float* quat, float* out6) {
+  std::vector<V3<float>> st;
+  Shape<float> sh = from_desc<float>(s, st);
+  Aabb<float> a = sh.world_aabb(v3p<float>(pos), q4p<float>(quat));
+  out6[0] = a.min.x; out6[1] = a.min.y; out6[2] = a.min.z;
+  out6[3] = a.max.x; out6[4] = a.max.y; out6[5] = a.max.z;
+}
+
+void oracle_support_world_f32(const or_shape* s, const float* pos, const float* quat, const float* dir, int core,
+                              float* out3) {
+  std::vector<V3<float>> st;
+  Shape<float> sh = from_desc<float>(s, st);
+  V3<float> r = core ? sh.core_support_world(v3p<float>(pos), q4p<float>(quat), v3p<float>(dir))
+                     : sh.support_world(v3p<float>(pos), q4p<float>(quat), v3p<float>(dir));
+  out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
+}
+
+// One collide_seeded call (narrowphase/mod.rs:63-286).  Returns the number
+// of contact points (0 = None); out holds 4 x (pos3, normal3, depth).
+int oracle_collide_f32(const or_shape* a, const float* pa, const float* qa, const or_shape* b, const float* pb,
+                       const float* qb, const float* seed, float* out, int* path) {
+  std::vector<V3<float>> sa_, sb_;
+  Shape<float> sa = from_desc<float>(a, sa_), sb = from_desc<float>(b, sb_);
+  Manifold<float> m;
+  V3<float> sd;
+  if (seed) sd = v3p<float>(seed);
+  int pth = 3;
+  collide_seeded(sa, v3p<float>(pa), q4p<float>(qa), sb, v3p<float>(pb), q4p<float>(qb), seed ? &sd : nullptr, m,
+                 &pth);
+  if (path) *path = pth;
+  write_manifold(m, out);
+  return m.n;
+}
+
+int oracle_collide_f64(const or_shape* a, const double* pa, const double* qa, const or_shape* b, const double* pb,
+                       const double* qb, const double* seed, double* out, int* path) {
+  std::vector<V3<double>> sa_, sb_;
+  Shape<double> sa = from_desc<double>(a, sa_), sb = from_desc<double>(b, sb_);
+  Manifold<double> m;
+  V3<double> sd;
+  if (seed) sd = v3d<double>(seed);
+  int pth = 3;
+  collide_seeded(sa, v3d<double>(pa), q4d<double>(qa), sb, v3d<double>(pb), q4d<double>(qb), seed ? &sd : nullptr,
+                 m, &pth);
+  if (path) *path = pth;
+  write_manifold(m, out);
+  return m.n;
+}
+
+// convex_convex (convex_convex.rs:30-155) directly, bypassing the dispatch
+// (the reference tests call it on box/sphere pairs too).  `threshold` < 0
+// selects the pair's own band (narrowphase/mod.rs:52-57).
+int oracle_convex_convex_f32(const or_shape* a, const float* pa, const float* qa, const or_shape* b,
+                             const float* pb, const float* qb, const float* seed, float threshold, float* out7,
+                             int* path) {
+  std::vector<V3<float>> sa_, sb_;
+  Shape<float> sa = from_desc<float>(a, sa_), sb = from_desc<float>(b, sb_);
+  PairPose<float> pr{&sa, v3p<float>(pa), q4p<float>(qa), &sb, v3p<float>(pb), q4p<float>(qb)};
+  V3<float> sd;
+  if (seed) sd = v3p<float>(seed);
+  float thr = threshold < 0.0f ? pair_threshold(sa, sb) : threshold;
+  Manifold<float> m;
+  int pth = 0;
+  convex_convex(pr, seed ? &sd : nullptr, thr, m, &pth);
+  if (path) *path = pth;
+  write_manifold(m, out7);
+  return m.n;
+}
+
+// gjk_contact (gjk.rs:171-288): returns 0 Separated / 1 Band / 2 Overlap.
+// out7 = distance, normal3, point_on_b3; simplex12 = the Overlap tetrahedron.
+int oracle_gjk_contact_f32(const or_shape* a, const float* pa, const float* qa, const or_shape* b, const float* pb,
+                           const float* qb, const float* seed, float threshold, float* out7, float* simplex12) {
+  std::vector<V3<float>> sa_, sb_;
+  Shape<float> sa = from_desc<float>(a, sa_), sb = from_desc<float>(b, sb_);
+  PairPose<float> pr{&sa, v3p<float>(pa), q4p<float>(qa), &sb, v3p<float>(pb), q4p<float>(qb)};
+  V3<float> sd;
+  if (seed) sd = v3p<float>(seed);
+  GjkOut<float> g = gjk_contact(pr, seed ? &sd : nullptr, threshold);
+  out7[0] = g.distance;
+  out7[1] = g.normal.x; out7[2] = g.normal.y; out7[3] = g.normal.z;
+  out7[4] = g.point_on_b.x; out7[5] = g.point_on_b.y; out7[6] = g.point_on_b.z;
+  if (simplex12)
+    for (int i = 0; i < 4; ++i) {
+      simplex12[3 * i] = g.simplex[i].x; simplex12[3 * i + 1] = g.simplex[i].y; simplex12[3 * i + 2] = g.simplex[i].z;
+    }
+  return g.kind;
+}
+
+int oracle_gjk_contact_f64(const or_shape* a, const double* pa, const double* qa, const or_shape* b,
+                           const double* pb, const double* qb, const double* seed, double threshold, double* out7) {
+  std::vector<V3<double>> sa_, sb_;
+  Shape<double> sa = from_desc<double>(a, sa_), sb = from_desc<double>(b, sb_);
+  PairPose<double> pr{&sa, v3d<double>(pa), q4d<double>(qa), &sb, v3d<double>(pb), q4d<double>(qb)};
+  V3<double> sd;
+  if (seed) sd = v3d<double>(seed);
+  GjkOut<double> g = gjk_contact(pr, seed ? &sd : nullptr, threshold);
+  out7[0] = g.distance;
+  out7[1] = g.normal.x; out7[2] = g.normal.y; out7[3] = g.normal.z;
+  out7[4] = g.point_on_b.x; out7[5] = g.point_on_b.y; out7[6] = g.point_on_b.z;
+  return g.kind;
+}
+
+// gjk_intersect_seeded (gjk.rs:590-607): 1 when an enclosing simplex exists.
+int oracle_gjk_intersect_f32(const or_shape* a, const float* pa, const float* qa, const or_shape* b,
+                             const float* pb, const float* qb, float* simplex12) {
+  std::vector<V3<float>> sa_, sb_;
+  Shape<float> sa = from_desc<float>(a, sa_), sb = from_desc<float>(b, sb_);
+  PairPose<float> pr{&sa, v3p<float>(pa), q4p<float>(qa), &sb, v3p<float>(pb), q4p<float>(qb)};
+  Simplex<float> s;
+  bool ok = gjk_intersect(pr, (const V3<float>*)nullptr, s);
+  if (simplex12)
+    for (int i = 0; i < 4; ++i) {
+      simplex12[3 * i] = s.pts[i].x; simplex12[3 * i + 1] = s.pts[i].y; simplex12[3 * i + 2] = s.pts[i].z;
+    }
+  return ok ? s.n : 0;
+}
+
+// EPA over the full (core=0) or core (core=1) Minkowski support of a pair.
+// out4 = normal3, depth.  Returns the EpaStatus (epa.rs:66-83).
+int oracle_epa_pair_f32(const float* simplex12, const or_shape* a, const float* pa, const float* qa,
+                        const or_shape* b, const float* pb, const float* qb, int core, float* out4) {
+  std::vector<V3<float>> sa_, sb_;
+  Shape<float> sa = from_desc<float>(a, sa_), sb = from_desc<float>(b, sb_);
+  PairPose<float> pr{&sa, v3p<float>(pa), q4p<float>(qa), &sb, v3p<float>(pb), q4p<float>(qb)};
+  V3<float> s[4];
+  for (int i = 0; i < 4; ++i) s[i] = v3p<float>(simplex12 + 3 * i);
+  EpaOut<float> e;
+  if (core) e = epa_penetration<float>(s, [&](V3<float> d) { return pr.core_support(d).w; });
+  else e = epa_penetration<float>(s, [&](V3<float> d) { return pr.full_support(d); });
+  out4[0] = e.normal.x; out4[1] = e.normal.y; out4[2] = e.normal.z; out4[3] = e.depth;
+  return e.status;
+}
+
+// EPA against a finite point set's support (the reference's point_support
+// helper, epa.rs:490-503): first maximum wins.
+int oracle_epa_points_f32(const float* simplex12, const float* pts, int npts, float* out4) {
+  std::vector<V3<float>> P(npts);
+  for (int i = 0; i < npts; ++i) P[i] = v3p<float>(pts + 3 * i);
+  V3<float> s[4];
+  for (int i = 0; i < 4; ++i) s[i] = v3p<float>(simplex12 + 3 * i);
+  auto sup = [&](V3<float> d) {
+    V3<float> best = P[0];
+    float bd = dot(best, d);
+    for (int i = 1; i < npts; ++i) {
+      float dd = dot(P[i], d);
+      if (dd > bd) { best = P[i]; bd = dd; }
+    }
+    return best;
+  };
+  EpaOut<float> e = epa_penetration<float>(s, sup);
+  out4[0] = e.normal.x; out4[1] = e.normal.y; out4[2] = e.normal.z; out4[3] = e.depth;
+  return e.status;
+}
+
+// World AABBs of a scene expanded by `margin` (world/step.rs:349-363).
+void oracle_scene_aabbs_f32(const or_scene* sc, float margin, float* out_min, float* out_max) {
+  SceneShapes<float> ss;
+  ss.build(sc);
+  for (int64_t i = 0; i < sc->n; ++i) {
+    Aabb<float> a = ss.shapes[i].world_aabb(ss.pos[i], ss.quat[i]).expanded(margin);
+    out_min[3 * i] = a.min.x; out_min[3 * i + 1] = a.min.y; out_min[3 * i + 2] = a.min.z;
+    out_max[3 * i] = a.max.x; out_max[3 * i + 1] = a.max.y; out_max[3 * i + 2] = a.max.z;
+  }
+}
+
+// Per-body contact_breaking_threshold (shapes/mod.rs:388-392).
+void oracle_scene_thresholds_f32(const or_scene* sc, float* out) {
+  SceneShapes<float> ss;
+  ss.build(sc);
+  for (int64_t i = 0; i < sc->n; ++i) out[i] = ss.shapes[i].contact_breaking_threshold();
+}
+
+// BruteForceBroadPhase::compute_pairs_into (brute_force.rs:54-67) over
+// AABBs given as min/max triplets.  Writes up to cap pairs, returns the total.
+int64_t oracle_brute_pairs(int64_t n, const float* mn, const float* mx, uint32_t* out, int64_t cap) {
+  int64_t cnt = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    Aabb<float> a{v3p<float>(mn + 3 * i), v3p<float>(mx + 3 * i)};
+    for (int64_t j = i + 1; j < n; ++j) {
+      Aabb<float> b{v3p<float>(mn + 3 * j), v3p<float>(mx + 3 * j)};
+      if (a.overlaps(b)) {
+        if (cnt < cap) { out[2 * cnt] = (uint32_t)i; out[2 * cnt + 1] = (uint32_t)j; }
+        ++cnt;
+      }
+    }
+  }
+  return cnt;
+}
+
+// The same pair set through an x-axis sweep (for sizes brute force cannot
+// reach); checked against oracle_brute_pairs by the tests.
+int64_t oracle_sweep_pairs(int64_t n, const float* mn, const float* mx, uint32_t* out, int64_t cap) {
+  std::vector<Aabb<float>> boxes(n);
+  for (int64_t i = 0; i < n; ++i) boxes[i] = Aabb<float>{v3p<float>(mn + 3 * i), v3p<float>(mx + 3 * i)};
+  std::vector<uint64_t> keys;
+  int64_t cnt = sweep_pairs<float>(n, boxes.data(), keys);
+  for (int64_t k = 0; k < cnt && k < cap; ++k) {
+    out[2 * k] = (uint32_t)(keys[k] >> 32);
+    out[2 * k + 1] = (uint32_t)(keys[k] & 0xffffffffu);
+  }
+  return cnt;
+}
+
+// collide_seeded for each (i, j) pair of a scene (shape_a = body i).
+// out_count[P] (0..4), out_pts[P*28] (4 x pos3 normal3 depth), out_path[P]
+// (0 gjk-separated, 1 gjk-band, 2 epa, 3 closed form).
+void oracle_collide_pairs_f32(const or_scene* sc, const uint32_t* pairs, int64_t npairs, const float* seeds,
+                              uint8_t* out_count, float* out_pts, int8_t* out_path) {
+  SceneShapes<float> ss;
+  ss.build(sc);
+  for (int64_t k = 0; k < npairs; ++k) {
+    uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
+    Manifold<float> m;
+    V3<float> sd;
+    if (seeds) sd = v3p<float>(seeds + 3 * k);
+    int pth = 3;
+    collide_seeded(ss.shapes[i], ss.pos[i], ss.quat[i], ss.shapes[j], ss.pos[j], ss.quat[j], seeds ? &sd : nullptr,
+                   m, &pth);
+    out_count[k] = (uint8_t)m.n;
+    if (out_path) out_path[k] = (int8_t)pth;
+    if (out_pts) write_manifold(m, out_pts + 28 * k);
+  }
+}
+
+void oracle_collide_pairs_f64(const or_scene* sc, const uint32_t* pairs, int64_t npairs, uint8_t* out_count,
+                              double* out_pts) {
+  SceneShapes<double> ss;
+  ss.build(sc);
+  for (int64_t k = 0; k < npairs; ++k) {
+    uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
+    Manifold<double> m;
+    collide_seeded(ss.shapes[i], ss.pos[i], ss.quat[i], ss.shapes[j], ss.pos[j], ss.quat[j], (const V3<double>*)nullptr, m);
+    out_count[k] = (uint8_t)m.n;
+    write_manifold(m, out_pts + 28 * k);
+  }
+}
+
+// One full collision pass on the CPU, the timed CPU baseline: world AABBs +
+// margin, the exact pair set (sweep), collide_seeded(body i, body j, None)
+// for every pair.  Returns the pair count; *contacts = total contact points;
+// *seconds = wall time of the pass.
+int64_t oracle_pass_f32(const or_scene* sc, float margin, int64_t* contacts, double* seconds) {
+  SceneShapes<float> ss;
+  ss.build(sc);
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<Aabb<float>> boxes(sc->n);
+  for (int64_t i = 0; i < sc->n; ++i) boxes[i] = ss.shapes[i].world_aabb(ss.pos[i], ss.quat[i]).expanded(margin);
+  std::vector<uint64_t> keys;
+  int64_t np = sweep_pairs<float>(sc->n, boxes.data(), keys);
+  int64_t nc = 0;
+  for (int64_t k = 0; k < np; ++k) {
+    uint32_t i = (uint32_t)(keys[k] >> 32), j = (uint32_t)(keys[k] & 0xffffffffu);
+    Manifold<float> m;
+    collide_seeded(ss.shapes[i], ss.pos[i], ss.quat[i], ss.shapes[j], ss.pos[j], ss.quat[j],
+                   (const V3<float>*)nullptr, m);
+    nc += m.n;
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  *contacts = nc;
+  *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return np;
+}
+
+}  // extern "C"

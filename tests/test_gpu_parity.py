@@ -1,0 +1,292 @@
+"""GPU parity: the HIP path against the CPU oracle on the same seeded inputs.
+
+Pair sets must be bit-exact (values and order).  Contacts must match in
+count exactly; positions and penetrations within TOL_M metres, normals
+within TOL_RAD radians (f32 tolerance of the collision path).  The number of
+bit-identical contacts is printed as well.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_M = 1e-5
+TOL_RAD = 1e-4
+MARGIN = 0.04
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _oracle_pass(oracle, scene, margin, pairs_fn="brute"):
+    mn, mx = oracle.scene_aabbs(scene, margin)
+    pairs = oracle.brute_pairs(mn, mx) if pairs_fn == "brute" else oracle.sweep_pairs(mn, mx)
+    cnt, pts, path = oracle.collide_pairs(scene, pairs)
+    return pairs, cnt, pts, path
+
+
+def _dense(cnt, pts):
+    """Oracle per-pair fixed slots -> CSR like the library's output."""
+    offs = np.concatenate(([0], np.cumsum(cnt.astype(np.int64))))
+    dense = np.concatenate([pts[k, :cnt[k]] for k in range(len(cnt))]) if offs[-1] else np.zeros((0, 7), np.float32)
+    return offs, dense
+
+
+def compare_contacts(ref_cnt, ref_pts, offs, cont, path=None, label=""):
+    """Assert count equality and the f32 tolerance; return the stats."""
+    got_cnt = np.diff(offs.astype(np.int64))
+    bad_cnt = np.nonzero(got_cnt != ref_cnt)[0]
+    assert bad_cnt.size == 0, f"{label}: contact counts differ at pairs {bad_cnt[:10]}"
+    ro, rd = _dense(ref_cnt, ref_pts)
+    assert rd.shape == cont.shape
+    if rd.shape[0] == 0:
+        return dict(n=0, exact=0)
+    dp = np.linalg.norm(rd[:, :3].astype(np.float64) - cont[:, :3], axis=1)
+    dd = np.abs(rd[:, 6].astype(np.float64) - cont[:, 6])
+    a = rd[:, 3:6].astype(np.float64)
+    b = cont[:, 3:6].astype(np.float64)
+    ang = np.arctan2(np.linalg.norm(np.cross(a, b), axis=1), np.sum(a * b, axis=1))
+    exact = int(np.sum(np.all(rd == cont, axis=1)))
+    bad = np.nonzero((dp > TOL_M) | (dd > TOL_M) | (ang > TOL_RAD))[0]
+    _flush(f"{label}: {rd.shape[0]} contacts, {exact} bit-identical, max |dp| {dp.max():.3g} m, "
+           f"max |dd| {dd.max():.3g} m, max angle {ang.max():.3g} rad, {bad.size} beyond tolerance")
+    assert bad.size == 0, f"{label}: {bad.size} contacts beyond tolerance"
+    return dict(n=rd.shape[0], exact=exact)
+
+
+def test_boxes_spheres_plane_pass(bpa, scenes, oracle, ctx):
+    """2,048 boxes/spheres over a ground plane (seed 1), margin 0.04: full pass."""
+    _flush("case: boxes_spheres_on_plane n=2048 seed=1")
+    sc = scenes.boxes_spheres_on_plane(2048, 1)
+    ref_pairs, ref_cnt, ref_pts, _ = _oracle_pass(oracle, sc, MARGIN)
+    w = bpa.CollisionWorld(sc, ctx)
+    P, Cn = w.collide(MARGIN)
+    pairs, offs, cont = w.results()
+    assert P == ref_pairs.shape[0]
+    assert np.array_equal(pairs, ref_pairs), "pair set differs from brute force"
+    compare_contacts(ref_cnt, ref_pts, offs, cont, label="boxes_spheres_plane")
+
+
+@pytest.mark.parametrize("margin", [0.02, 0.04])
+def test_margins(bpa, scenes, oracle, ctx, margin):
+    _flush(f"case: boxes_spheres_on_plane margin={margin}")
+    sc = scenes.boxes_spheres_on_plane(2048, 1)
+    ref_pairs, ref_cnt, ref_pts, _ = _oracle_pass(oracle, sc, margin)
+    w = bpa.CollisionWorld(sc, ctx)
+    w.collide(margin)
+    pairs, offs, cont = w.results()
+    assert np.array_equal(pairs, ref_pairs)
+    compare_contacts(ref_cnt, ref_pts, offs, cont, label=f"margin {margin}")
+
+
+def test_mixed_convex_pass(bpa, scenes, oracle, ctx):
+    """Boxes/spheres/capsules/hulls (40/30/20/10), 20k bodies: GJK + EPA paths."""
+    _flush("case: mixed_convex n=20000 seed=2")
+    sc = scenes.mixed_convex(20000, 2)
+    ref_pairs, ref_cnt, ref_pts, path = _oracle_pass(oracle, sc, MARGIN, "sweep")
+    w = bpa.CollisionWorld(sc, ctx)
+    w.collide(MARGIN)
+    pairs, offs, cont = w.results()
+    assert np.array_equal(pairs, ref_pairs)
+    _flush(f"paths: {np.bincount(path, minlength=4)} (gjk-sep, band, epa, closed)")
+    compare_contacts(ref_cnt, ref_pts, offs, cont, label="mixed_convex")
+
+
+def test_collide_batch_matches_oracle(bpa, scenes, oracle, ctx):
+    """bch_collide_batch on caller-given pairs (unsorted, with repeats)."""
+    _flush("case: collide_batch mixed 5000")
+    sc = scenes.mixed_convex(5000, 7)
+    mn, mx = oracle.scene_aabbs(sc, MARGIN)
+    pairs = oracle.sweep_pairs(mn, mx)
+    rng = np.random.default_rng(0)
+    sel = pairs[rng.permutation(pairs.shape[0])]
+    sel = np.concatenate([sel, sel[:50]])
+    cnt, pts = bpa.collide_batch(sc, sel, ctx=ctx)
+    rc, rp, _ = oracle.collide_pairs(sc, sel)
+    assert np.array_equal(cnt, rc)
+    offs = np.concatenate(([0], np.cumsum(cnt.astype(np.int64))))
+    _, dense = _dense(cnt, pts)
+    compare_contacts(rc, rp, offs, dense, label="collide_batch")
+
+
+def test_seeded_collide_matches_oracle(bpa, scenes, oracle, ctx):
+    """GJK warm-start seeds change the walk, not the contract (mod.rs:515-564)."""
+    _flush("case: seeded collide")
+    sc = scenes.mixed_convex(3000, 11)
+    mn, mx = oracle.scene_aabbs(sc, MARGIN)
+    pairs = oracle.sweep_pairs(mn, mx)
+    rng = np.random.default_rng(1)
+    seeds = rng.normal(size=(pairs.shape[0], 3)).astype(np.float32)
+    cnt, pts = bpa.collide_batch(sc, pairs, seeds, ctx=ctx)
+    rc, rp, _ = oracle.collide_pairs(sc, pairs, seeds)
+    offs = np.concatenate(([0], np.cumsum(cnt.astype(np.int64))))
+    _, dense = _dense(cnt, pts)
+    compare_contacts(rc, rp, offs, dense, label="seeded")
+
+
+def _ref_random_aabb(lcg_state):
+    """random_aabb of axis_sweep3.rs:855-859 with the reference LCG."""
+    A, Cc, M = 6364136223846793005, 1442695040888963407, (1 << 64) - 1
+
+    def unit():
+        lcg_state[0] = (lcg_state[0] * A + Cc) & M
+        return np.float32(float(lcg_state[0] >> 11)) / np.float32(2.0 ** 53)
+
+    def rng(lo, hi):
+        return np.float32(lo) + (np.float32(hi) - np.float32(lo)) * unit()
+
+    base = np.array([rng(-20, 20), rng(-20, 20), rng(-20, 20)], np.float32)
+    ext = np.array([rng(0.1, 2.0), rng(0.1, 2.0), rng(0.1, 2.0)], np.float32)
+    return base, base + ext
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_proxy_state_machine_matches_brute_force(bpa, oracle, ctx, seed):
+    """Randomized create/move/destroy, compared after every few ops with the
+    brute-force oracle (the reference's state-machine test pattern,
+    axis_sweep3.rs:937-957)."""
+    _flush(f"case: proxy state machine seed={seed}")
+    bp = bpa.BroadPhase(ctx=bpa._Context(0))
+    live = {}
+    boxes = {}
+    st = [seed]
+    A, Cc, M = 6364136223846793005, 1442695040888963407, (1 << 64) - 1
+    for op in range(300):
+        st[0] = (st[0] * A + Cc) & M
+        r = st[0] % 4
+        if r <= 1 or not live:
+            mn, mx = _ref_random_aabb(st)
+            i = bp.create_proxy(mn, mx)
+            free = [k for k in range(len(boxes) + 1) if k not in live]
+            assert i == min(free), "create_proxy must reuse the lowest free slot"
+            live[i] = True
+            boxes[i] = (mn, mx)
+        elif r == 2:
+            k = sorted(live)[int(st[0] >> 33) % len(live)]
+            mn, mx = _ref_random_aabb(st)
+            bp.set_aabb(k, mn, mx)
+            boxes[k] = (mn, mx)
+        else:
+            k = sorted(live)[int(st[0] >> 33) % len(live)]
+            bp.destroy_proxy(k)
+            del live[k]
+        if op % 25 == 24:
+            n = max(boxes) + 1 if boxes else 0
+            mn = np.full((n, 3), np.nan, np.float32)
+            mx = np.full((n, 3), np.nan, np.float32)
+            for k in live:
+                mn[k], mx[k] = boxes[k]
+            ref = oracle.brute_pairs(mn, mx)
+            got = bp.compute_pairs()
+            assert np.array_equal(got, ref), f"op {op}"
+    bp.ctx.close()
+
+
+def test_broadphase_edge_cases(bpa, oracle, ctx):
+    """Touching boxes (closed intervals), identical boxes, degenerate points,
+    a huge box, empty and single-proxy sets."""
+    _flush("case: broadphase edge cases")
+    bp = bpa.BroadPhase(ctx=bpa._Context(0))
+    assert bp.compute_pairs().shape == (0, 2)
+    bp.create_proxy([0, 0, 0], [1, 1, 1])
+    assert bp.compute_pairs().shape == (0, 2)
+    bp.create_proxy([1, 0, 0], [2, 1, 1])          # touching on x
+    bp.create_proxy([0, 0, 0], [1, 1, 1])          # identical
+    bp.create_proxy([0.5, 0.5, 0.5], [0.5, 0.5, 0.5])  # point
+    bp.create_proxy([-1e30, -1e30, -1e30], [1e30, 1e30, 1e30])  # plane-like
+    bp.create_proxy([0.5, 0.5, 5.0], [1.5, 1.5, 6.0])  # separated on z
+    mn = np.array([[0, 0, 0], [1, 0, 0], [0, 0, 0], [.5, .5, .5], [-1e30] * 3, [.5, .5, 5]], np.float32)
+    mx = np.array([[1, 1, 1], [2, 1, 1], [1, 1, 1], [.5, .5, .5], [1e30] * 3, [1.5, 1.5, 6]], np.float32)
+    assert np.array_equal(bp.compute_pairs(), oracle.brute_pairs(mn, mx))
+    assert bp.proxy_aabb(3) is not None
+    bp.destroy_proxy(3)
+    assert bp.proxy_aabb(3) is None
+    bp.set_aabb(3, [0, 0, 0], [9, 9, 9])  # dead slot: no-op
+    assert bp.proxy_aabb(3) is None
+    mn[3] = np.nan
+    mx[3] = np.nan
+    assert np.array_equal(bp.compute_pairs(), oracle.brute_pairs(mn, mx))
+    cand = bp.aabb_candidates([0.9, 0.9, 0.9], [1.1, 1.1, 1.1])
+    assert list(cand) == [0, 1, 2, 4]
+    bp.ctx.close()
+
+
+def test_nan_and_degenerate_poses(bpa, scenes, oracle, ctx):
+    """NaN poses never pair and never hang; coincident bodies stay finite."""
+    _flush("case: nan / degenerate poses")
+    sc = scenes.mixed_convex(2000, 5)
+    sc["position"][10] = np.nan
+    sc["orientation"][20] = np.nan
+    sc["position"][30] = sc["position"][31]
+    sc["orientation"][30] = sc["orientation"][31]
+    ref_pairs, ref_cnt, ref_pts, _ = _oracle_pass(oracle, sc, MARGIN, "sweep")
+    w = bpa.CollisionWorld(sc, ctx)
+    w.collide(MARGIN)
+    pairs, offs, cont = w.results()
+    assert np.array_equal(pairs, ref_pairs)
+    assert not np.any(pairs == 10)
+    compare_contacts(ref_cnt, ref_pts, offs, cont, label="degenerate")
+
+
+def test_empty_and_tiny_worlds(bpa, scenes, ctx):
+    _flush("case: empty / tiny worlds")
+    sc = scenes.boxes_spheres_on_plane(0, 1, with_plane=False)
+    w = bpa.CollisionWorld(sc, ctx)
+    assert w.collide(MARGIN) == (0, 0)
+    sc = scenes.boxes_spheres_on_plane(1, 1)
+    w = bpa.CollisionWorld(sc, ctx)
+    P, _ = w.collide(MARGIN)
+    assert P in (0, 1)
+
+
+def test_batched_worlds_equal_plain_per_world(bpa, scenes, oracle, ctx):
+    """Independent worlds in one pass == each world collided alone (plain path)."""
+    _flush("case: batched 64 worlds")
+    sc = scenes.batched_worlds(64, 3, 16, 512)
+    w = bpa.CollisionWorld(sc, ctx)
+    w.collide(MARGIN)
+    pairs, offs, cont = w.results()
+    wo = sc["world_offsets"]
+    world_of = np.searchsorted(wo, pairs[:, 0], side="right") - 1
+    assert np.all(pairs[:, 1] < wo[world_of + 1]), "pair crosses worlds"
+    plain = bpa.CollisionWorld(scenes.world_slice(sc, 0), bpa._Context(0))
+    for wi in range(len(wo) - 1):
+        sub = scenes.world_slice(sc, wi)
+        pw = bpa.CollisionWorld(sub, plain.ctx)
+        pw.collide(MARGIN)
+        pp, po, pc = pw.results()
+        m = world_of == wi
+        got = pairs[m] - wo[wi]
+        assert np.array_equal(got, pp), f"world {wi} pairs differ"
+        k = np.nonzero(m)[0]
+        seg = cont[offs[k[0]]:offs[k[-1] + 1]] if k.size else np.zeros((0, 7), np.float32)
+        assert np.array_equal(seg, pc), f"world {wi} contacts differ"
+        if wi < 4:
+            rp = oracle.brute_pairs(*oracle.scene_aabbs(sub, MARGIN))
+            assert np.array_equal(pp, rp)
+    plain.ctx.close()
+
+
+def test_large_scene_properties(bpa, scenes, oracle, ctx):
+    """200k mixed bodies: sorted unique pairs; every pair overlaps; sampled
+    bodies' partner sets equal brute force over all bodies."""
+    _flush("case: mixed_convex n=200000 properties")
+    sc = scenes.mixed_convex(200_000, 2)
+    w = bpa.CollisionWorld(sc, ctx)
+    P, Cn = w.collide(MARGIN)
+    pairs, offs, cont = w.results()
+    key = pairs[:, 0].astype(np.uint64) << np.uint64(32) | pairs[:, 1].astype(np.uint64)
+    assert np.all(pairs[:, 0] < pairs[:, 1])
+    assert np.all(np.diff(key.astype(np.int64)) > 0)
+    mn, mx = oracle.scene_aabbs(sc, MARGIN)
+    a, b = pairs[:, 0], pairs[:, 1]
+    assert np.all((mn[a] <= mx[b]) & (mn[b] <= mx[a]))
+    rng = np.random.default_rng(3)
+    for i in rng.choice(sc["shape_type"].shape[0], 64, replace=False):
+        ov = np.nonzero(np.all((mn[i] <= mx) & (mn <= mx[i]), axis=1))[0]
+        ov = ov[ov != i]
+        got = np.concatenate([pairs[pairs[:, 0] == i, 1], pairs[pairs[:, 1] == i, 0]])
+        assert np.array_equal(np.sort(got), ov), f"body {i}"
+    assert np.all(np.isfinite(cont))
+    assert offs[-1] == Cn
