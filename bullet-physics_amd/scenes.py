@@ -32,6 +32,8 @@ class Lcg:
         self.s = np.uint64(seed)
 
     def next_u64(self, k: int) -> np.ndarray:
+        if k <= 0:
+            return np.zeros(0, np.uint64)
         with np.errstate(over="ignore"):
             a = np.full(k, _A, dtype=np.uint64)
             ak = np.cumprod(a, dtype=np.uint64)              # a^1 .. a^k
