@@ -6,11 +6,10 @@
 //                 big / NaN classification
 //   k_morton      30-bit Morton code of the AABB centre (world id above it)
 //   radix sort    (rocPRIM) codes -> leaf order
-//   k_leaves      leaf nodes: box, body id, rope
-//   k_build       Karras (2012) internal nodes: left child, rope, parent
-//   k_refit       bottom-up boxes (second arriving child computes the union)
-//   k_traverse    one lane per leaf walks the rope-linked tree over the
-//                 leaves to its right: exact pair set without duplicates
+//   k_tree_*      implicit complete binary tree over the sorted leaves;
+//                 boxes unioned level by level in LDS
+//   k_traverse    one lane per leaf walks the tree over the leaves to its
+//                 right: exact pair set without duplicates
 //   k_big         bodies with unbounded AABBs (planes) against their world
 //   radix sort    (rocPRIM) pair keys i << 32 | j -> lexicographic order
 //   k_narrow      one lane per pair (pairs grouped by shape class)
@@ -21,17 +20,6 @@
 
 namespace bch {
 
-// Traversal node (32 B = two 16-byte loads): box, left child (>= 0) or
-// ~body (< 0) for a leaf, rope = next node in depth-first order after this
-// subtree (-1 = end).  The right child of an internal node is its left
-// child's rope.
-struct alignas(16) Node {
-  float lx, ly, lz;
-  int left;
-  float hx, hy, hz;
-  int rope;
-};
-
 struct Counters {
   uint32_t n_reg;       // bodies in the tree
   uint32_t n_big;       // bodies with unbounded AABBs
@@ -40,6 +28,8 @@ struct Counters {
   uint32_t n_big_pairs;
   uint32_t n_epa;
   uint32_t errors;      // bit 0 clip overflow, bit 1 epa store overflow, bit 2 big-pair overflow
+  uint32_t cls_start[8];  // pair range of each shape class after the class sort
+  uint32_t cls_end[8];
   uint32_t pad[4];
 };
 
@@ -145,8 +135,11 @@ __device__ __forceinline__ void classify_box(uint32_t i, const Box6& bx, uint8_t
   }
 }
 
-__device__ __forceinline__ void reduce_bounds(float* cmin, float* cmax, uint32_t nreg, Counters* cnt) {
-  // wave reduction then one atomic per wave
+// Block-level reduction of the regular-body centre bounds and count into a
+// per-block partial (8 floats); k_bounds_final folds the partials, so no two
+// blocks ever touch the same address.
+__device__ __forceinline__ void reduce_bounds(float* cmin, float* cmax, uint32_t nreg, float* partial) {
+  __shared__ float sred[7][16];
   for (int k = 0; k < 3; ++k) {
     float a = cmin[k], b = cmax[k];
     for (int off = 32; off > 0; off >>= 1) {
@@ -158,13 +151,68 @@ __device__ __forceinline__ void reduce_bounds(float* cmin, float* cmax, uint32_t
   }
   uint32_t r = nreg;
   for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off);
+  int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
   if ((threadIdx.x & 63) == 0) {
-    if (r) {
-      atomicAdd(&cnt->n_reg, r);
+    for (int k = 0; k < 3; ++k) {
+      sred[k][w] = cmin[k];
+      sred[3 + k][w] = cmax[k];
+    }
+    sred[6][w] = u2f(r);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float out[6];
+    uint32_t tot = 0;
+    for (int k = 0; k < 3; ++k) {
+      out[k] = INFINITY;
+      out[3 + k] = -INFINITY;
+    }
+    for (int i = 0; i < nw; ++i) {
       for (int k = 0; k < 3; ++k) {
-        atomicMin(&cnt->bounds[k], ord_enc(cmin[k]));
-        atomicMax(&cnt->bounds[3 + k], ord_enc(cmax[k]));
+        out[k] = mn(out[k], sred[k][i]);
+        out[3 + k] = mx(out[3 + k], sred[3 + k][i]);
       }
+      tot += f2u(sred[6][i]);
+    }
+    float* dst = partial + 8 * blockIdx.x;
+    for (int k = 0; k < 6; ++k) dst[k] = out[k];
+    dst[6] = u2f(tot);
+  }
+}
+
+// Fold the per-block partials of up to two kernels into the counters.
+__global__ void k_bounds_final(const float* __restrict__ pa, uint32_t na, const float* __restrict__ pb, uint32_t nb,
+                               Counters* cnt) {
+  __shared__ float sm[6][256];
+  __shared__ uint32_t sc[256];
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  uint32_t tot = 0;
+  for (uint32_t i = threadIdx.x; i < na + nb; i += blockDim.x) {
+    const float* p = i < na ? pa + 8 * i : pb + 8 * (i - na);
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = mn(lo[k], p[k]);
+      hi[k] = mx(hi[k], p[3 + k]);
+    }
+    tot += f2u(p[6]);
+  }
+  for (int k = 0; k < 3; ++k) {
+    sm[k][threadIdx.x] = lo[k];
+    sm[3 + k][threadIdx.x] = hi[k];
+  }
+  sc[threadIdx.x] = tot;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (uint32_t i = 1; i < blockDim.x; ++i) {
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = mn(lo[k], sm[k][i]);
+        hi[k] = mx(hi[k], sm[3 + k][i]);
+      }
+      tot += sc[i];
+    }
+    cnt->n_reg = tot;
+    for (int k = 0; k < 3; ++k) {
+      cnt->bounds[k] = ord_enc(lo[k]);
+      cnt->bounds[3 + k] = ord_enc(hi[k]);
     }
   }
 }
@@ -173,12 +221,13 @@ __device__ __forceinline__ void reduce_bounds(float* cmin, float* cmax, uint32_t
 // classification and the scene-bounds reduction.
 __global__ void k_aabb(uint32_t n, const BodyRec* __restrict__ bodies, const float4* __restrict__ hull, float margin,
                        float4* __restrict__ aabb, uint8_t* __restrict__ flags, uint32_t* __restrict__ big_ids,
-                       Counters* cnt) {
+                       Counters* cnt, float* __restrict__ partial) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
   uint32_t reg = 0;
-  if (i < n) {
-    BodyRec b = bodies[i];
+  BodyRec b;
+  if (i < n) b = bodies[i];
+  if (i < n && (b.type & 0xffu) != HULL) {  // hull boxes: k_hull_aabb
     ShapeRef s = shape_of(b, hull);
     Box6 bx = world_aabb(s, pos_of(b), quat_of(b));
     V3 pad = splat(margin);  // util.rs:52-55
@@ -189,13 +238,13 @@ __global__ void k_aabb(uint32_t n, const BodyRec* __restrict__ bodies, const flo
     classify_box(i, bx, flags, cnt, big_ids, cmin, cmax);
     reg = flags[i] == FLAG_REG;
   }
-  reduce_bounds(cmin, cmax, reg, cnt);
+  reduce_bounds(cmin, cmax, reg, partial);
 }
 
 // Same classification for caller-given AABBs (the proxy API); dead proxies
 // arrive as NaN boxes.
 __global__ void k_aabb_given(uint32_t n, float4* __restrict__ aabb, uint8_t* __restrict__ flags,
-                             uint32_t* __restrict__ big_ids, Counters* cnt) {
+                             uint32_t* __restrict__ big_ids, Counters* cnt, float* __restrict__ partial) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
   uint32_t reg = 0;
@@ -205,7 +254,77 @@ __global__ void k_aabb_given(uint32_t n, float4* __restrict__ aabb, uint8_t* __r
     classify_box(i, bx, flags, cnt, big_ids, cmin, cmax);
     reg = flags[i] == FLAG_REG;
   }
-  reduce_bounds(cmin, cmax, reg, cnt);
+  reduce_bounds(cmin, cmax, reg, partial);
+}
+
+// Hull world AABBs (shapes/mod.rs:448-469): 16 lanes per hull scan the
+// vertices for the six axis supports; each lane keeps its first maximiser
+// and the group reduces (largest value, then lowest index), which is the
+// first maximiser of the sequential scan (convex_hull.rs:55-66).
+__device__ __forceinline__ void argmax_merge(float& v, int& i, float ov, int oi) {
+  if (oi >= 0 && (i < 0 || ov > v || (ov == v && oi < i))) {
+    v = ov;
+    i = oi;
+  }
+}
+
+__global__ void k_hull_aabb(uint32_t n_hulls, const uint32_t* __restrict__ hull_ids,
+                            const BodyRec* __restrict__ bodies, const float4* __restrict__ hull, float margin,
+                            float4* __restrict__ aabb, uint8_t* __restrict__ flags, uint32_t* __restrict__ big_ids,
+                            Counters* cnt, float* __restrict__ partial) {
+  uint32_t h = blockIdx.x * (blockDim.x / 16) + threadIdx.x / 16;
+  int lane = threadIdx.x & 15;
+  float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+  uint32_t reg = 0;
+  bool active = h < n_hulls;
+  uint32_t body = active ? hull_ids[h] : 0u;
+  BodyRec b = bodies[body];
+  const float4* hv = hull + f2u(b.a0);
+  int nv = active ? (int)f2u(b.a1) : 0;
+  Q4 qi = qconj(quat_of(b));
+  V3 u[3] = {qrot(qi, V3{1.0f, 0.0f, 0.0f}), qrot(qi, V3{0.0f, 1.0f, 0.0f}), qrot(qi, V3{0.0f, 0.0f, 1.0f})};
+  float bv[6];
+  int bi[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    bv[k] = -INFINITY;
+    bi[k] = -1;
+  }
+  for (int v = lane; v < nv; v += 16) {
+    V3 p = hull_vertex(hv, v);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float d = dot(p, u[k]);
+      if (d > bv[k]) { bv[k] = d; bi[k] = v; }
+      float e = dot(p, -u[k]);
+      if (e > bv[3 + k]) { bv[3 + k] = e; bi[3 + k] = v; }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    for (int off = 8; off > 0; off >>= 1) {
+      float ov = __shfl_xor(bv[k], off, 16);
+      int oi = __shfl_xor(bi[k], off, 16);
+      argmax_merge(bv[k], bi[k], ov, oi);
+    }
+  }
+  if (active && lane == 0) {
+    V3 p = pos_of(b);
+    V3 sp[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sp[k] = bi[k] >= 0 ? hull_vertex(hv, bi[k]) : splat(0.0f);
+    Box6 bx;
+    bx.lo = V3{p.x + dot(sp[3], u[0]), p.y + dot(sp[4], u[1]), p.z + dot(sp[5], u[2])};
+    bx.hi = V3{p.x + dot(sp[0], u[0]), p.y + dot(sp[1], u[1]), p.z + dot(sp[2], u[2])};
+    V3 pad = splat(margin);
+    bx.lo = bx.lo - pad;
+    bx.hi = bx.hi + pad;
+    aabb[2 * body] = make_float4(bx.lo.x, bx.lo.y, bx.lo.z, 0.0f);
+    aabb[2 * body + 1] = make_float4(bx.hi.x, bx.hi.y, bx.hi.z, 0.0f);
+    classify_box(body, bx, flags, cnt, big_ids, cmin, cmax);
+    reg = flags[body] == FLAG_REG;
+  }
+  reduce_bounds(cmin, cmax, reg, partial);
 }
 
 __device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
@@ -246,155 +365,103 @@ __global__ void k_morton(uint32_t n, const float4* __restrict__ aabb, const uint
   vals[i] = i;
 }
 
-// Common-prefix length of sorted keys i, j with the index as tie-break
-// (Karras 2012); -1 outside [0, n).
-__device__ __forceinline__ int kdelta(const uint64_t* __restrict__ keys, int n, int i, int j) {
-  if (j < 0 || j >= n) return -1;
-  uint64_t a = keys[i], b = keys[j];
-  if (a == b) return 64 + __clz((uint32_t)(i ^ j));
-  return __clzll(a ^ b);
+// ------------------------------------------------------------------ tree --
+// Implicit complete binary tree over N = 2^L leaf slots, 1-based heap order:
+// node k has children 2k and 2k + 1, leaves are nodes N .. 2N - 1 and hold the
+// boxes in Morton order (slots past the regular bodies hold an empty box).
+// Each node is two float4 (lo, hi); a leaf's lo.w carries its body id.
+// Depth-first order needs no pointers: the next subtree after node k (its
+// rope) is found by stripping k's trailing one bits and stepping right.
+__device__ __forceinline__ uint32_t tree_rope(uint32_t k) {
+  k >>= __builtin_ctz(~k);
+  return k ? k + 1u : 0u;
 }
 
-__device__ __forceinline__ int kdir(const uint64_t* __restrict__ keys, int n, int i) {
-  return (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
-}
-
-// Rope of a subtree whose leaf range ends at `last`: the subtree starting at
-// last + 1 -- internal node last + 1 when that node's range starts there,
-// else leaf last + 1 (the right child of the nearest ancestor that holds
-// this subtree on its left).
-__device__ __forceinline__ int rope_after(const uint64_t* __restrict__ keys, int n, int last) {
-  int k = last + 1;
-  if (k >= n) return -1;
-  if (k <= n - 2 && kdir(keys, n, k) == 1) return k;
-  return (n - 1) + k;
-}
-
-__global__ void k_leaves(uint32_t n_total, const Counters* __restrict__ cnt, const uint64_t* __restrict__ keys,
-                         const uint32_t* __restrict__ order, const float4* __restrict__ aabb, Node* __restrict__ nodes,
-                         uint32_t* __restrict__ node_first) {
-  int n = (int)cnt->n_reg;
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  uint32_t body = order[s];
-  float4 lo = aabb[2 * body], hi = aabb[2 * body + 1];
-  Node L;
-  L.lx = lo.x; L.ly = lo.y; L.lz = lo.z;
-  L.hx = hi.x; L.hy = hi.y; L.hz = hi.z;
-  L.left = -(int)body - 1;
-  L.rope = rope_after(keys, n, s);
-  nodes[(n - 1) + s] = L;
-  if (node_first) node_first[(n - 1) + s] = (uint32_t)s;
-}
-
-__global__ void k_build(const Counters* __restrict__ cnt, const uint64_t* __restrict__ keys, Node* __restrict__ nodes,
-                        int* __restrict__ parent, uint32_t* __restrict__ refit_flag, uint32_t* __restrict__ node_first) {
-  int n = (int)cnt->n_reg;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n - 1) return;
-  int d = kdir(keys, n, i);
-  int dmin = kdelta(keys, n, i, i - d);
-  int lmax = 2;
-  while (kdelta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
-  int l = 0;
-  for (int t = lmax >> 1; t >= 1; t >>= 1)
-    if (kdelta(keys, n, i, i + (l + t) * d) > dmin) l += t;
-  int j = i + l * d;
-  int dnode = kdelta(keys, n, i, j);
-  int s = 0;
-  int t = l;
-  while (true) {
-    t = (t + 1) >> 1;
-    if (kdelta(keys, n, i, i + (s + t) * d) > dnode) s += t;
-    if (t <= 1) break;
+__global__ void k_tree_leaves(uint32_t N, const Counters* __restrict__ cnt, const uint32_t* __restrict__ order,
+                              const float4* __restrict__ aabb, float4* __restrict__ nodes) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+  if (s < cnt->n_reg) {
+    uint32_t b = order[s];
+    lo = aabb[2 * b];
+    hi = aabb[2 * b + 1];
+    lo.w = u2f(b);
   }
-  int gamma = i + s * d + (d < 0 ? -1 : 0);
-  int first = i < j ? i : j, last = i < j ? j : i;
-  int left = (first == gamma) ? (n - 1) + gamma : gamma;
-  int right = (last == gamma + 1) ? (n - 1) + gamma + 1 : gamma + 1;
-  nodes[i].left = left;
-  nodes[i].rope = rope_after(keys, n, last);
-  parent[left] = i;
-  parent[right] = i;
-  refit_flag[i] = 0u;
-  if (node_first) node_first[i] = (uint32_t)first;
-  if (i == 0) parent[0] = -1;
+  nodes[2 * (uint64_t)(N + s)] = lo;
+  nodes[2 * (uint64_t)(N + s) + 1] = hi;
 }
 
-// Bottom-up boxes: each leaf climbs; the first child to arrive at a parent
-// stops, the second computes the union (both children are then final).
-__global__ void k_refit(const Counters* __restrict__ cnt, Node* nodes, const int* __restrict__ parent,
-                        uint32_t* refit_flag) {
-  int n = (int)cnt->n_reg;
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n || n < 2) return;
-  int node = (n - 1) + s;
-  while (true) {
-    int p = parent[node];
-    if (p < 0) break;
-    __threadfence();
-    if (atomicAdd(&refit_flag[p], 1u) == 0u) break;
-    __threadfence();
-    int l = __atomic_load_n(&nodes[p].left, __ATOMIC_RELAXED);
-    int r = __atomic_load_n(&nodes[l].rope, __ATOMIC_RELAXED);
-    const volatile Node* A = &nodes[l];
-    const volatile Node* B = &nodes[r];
-    Node P;
-    P.lx = mn(A->lx, B->lx); P.ly = mn(A->ly, B->ly); P.lz = mn(A->lz, B->lz);
-    P.hx = mx(A->hx, B->hx); P.hy = mx(A->hy, B->hy); P.hz = mx(A->hz, B->hz);
-    volatile Node* D = &nodes[p];
-    D->lx = P.lx; D->ly = P.ly; D->lz = P.lz;
-    D->hx = P.hx; D->hy = P.hy; D->hz = P.hz;
-    node = p;
+// Boxes of `m` levels above the child level starting at node `first`
+// (= 2^level): each block unions 2^m consecutive children in LDS.
+__global__ void k_tree_up(float4* __restrict__ nodes, uint32_t first, int m) {
+  __shared__ float4 slo[512], shi[512];
+  uint32_t per = 1u << m;
+  uint64_t base = (uint64_t)first + (uint64_t)blockIdx.x * per;
+  for (uint32_t i = threadIdx.x; i < per; i += blockDim.x) {
+    slo[i] = nodes[2 * (base + i)];
+    shi[i] = nodes[2 * (base + i) + 1];
+  }
+  __syncthreads();
+  uint32_t cnt = per;
+  for (int t = 0; t < m; ++t) {
+    cnt >>= 1;
+    base >>= 1;
+    float4 l, h;
+    uint32_t i = threadIdx.x;
+    if (i < cnt) {
+      float4 a = slo[2 * i], b = slo[2 * i + 1], c = shi[2 * i], d = shi[2 * i + 1];
+      l = make_float4(mn(a.x, b.x), mn(a.y, b.y), mn(a.z, b.z), 0.0f);
+      h = make_float4(mx(c.x, d.x), mx(c.y, d.y), mx(c.z, d.z), 0.0f);
+    }
+    __syncthreads();
+    if (i < cnt) {
+      slo[i] = l;
+      shi[i] = h;
+      nodes[2 * (base + i)] = l;
+      nodes[2 * (base + i) + 1] = h;
+    }
+    __syncthreads();
   }
 }
 
-__device__ __forceinline__ void load_node(const Node* __restrict__ nodes, int i, Box6& b, int& left, int& rope) {
-  const float4* p = reinterpret_cast<const float4*>(nodes + i);
-  float4 a = p[0], c = p[1];
-  b.lo = V3{a.x, a.y, a.z};
-  b.hi = V3{c.x, c.y, c.z};
-  left = (int)f2u(a.w);
-  rope = (int)f2u(c.w);
-}
-
-// One lane per leaf (sorted position s): walk the subtrees to the right of
-// s in depth-first order via ropes; every overlapping leaf is a pair found
-// exactly once.  Pair keys go to the leaf's K slots; counts beyond K are
-// recorded so the host can rerun with more slots.  With `world_end`
-// (batches), the walk stops at the first node that starts past the query's
-// world.
-__global__ void k_traverse(const Counters* __restrict__ cnt, const Node* __restrict__ nodes,
-                           const uint32_t* __restrict__ node_first, const uint32_t* __restrict__ world_end_of_leaf,
-                           uint32_t K, uint64_t* __restrict__ slots, uint32_t* __restrict__ counts,
-                           Counters* cnt_rw) {
-  int n = (int)cnt->n_reg;
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+// One lane per leaf s: walk the subtrees right of s in depth-first order;
+// every overlapping leaf is a pair found exactly once.  Keys i << 32 | j go to
+// the leaf's K slots; counts beyond K are reported for a rerun with more
+// slots.  With `wend` (batches: leaves sorted world-major), the walk stops at
+// the first node whose first leaf lies past the query's world.
+__global__ void k_traverse(const Counters* __restrict__ cnt, const float4* __restrict__ nodes, uint32_t N, int L,
+                           const uint32_t* __restrict__ wend_of_leaf, uint32_t K, uint64_t* __restrict__ slots,
+                           uint32_t* __restrict__ counts, Counters* cnt_rw) {
+  uint32_t n = cnt->n_reg;
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  Box6 q;
-  int qleft, node;
-  load_node(nodes, (n - 1) + s, q, qleft, node);
-  uint32_t body = (uint32_t)(-qleft - 1);
-  uint32_t wend = world_end_of_leaf ? world_end_of_leaf[s] : 0xffffffffu;
+  float4 qlo = nodes[2 * (uint64_t)(N + s)], qhi = nodes[2 * (uint64_t)(N + s) + 1];
+  uint32_t body = f2u(qlo.w);
+  uint32_t wend = wend_of_leaf ? wend_of_leaf[s] : 0xffffffffu;
   uint32_t c = 0;
   uint64_t* my = slots + (uint64_t)s * K;
-  while (node >= 0) {
-    if (node_first && node_first[node] >= wend) break;
-    Box6 b;
-    int left, rope;
-    load_node(nodes, node, b, left, rope);
-    if (overlaps(q, b)) {
-      if (left < 0) {
-        uint32_t other = (uint32_t)(-left - 1);
-        uint32_t lo = body < other ? body : other, hi = body < other ? other : body;
-        if (c < K) my[c] = ((uint64_t)lo << 32) | hi;
+  uint32_t k = tree_rope(N + s);
+  while (k) {
+    if (wend_of_leaf) {
+      int d = 31 - __clz(k);
+      uint32_t first = (k << (L - d)) - N;
+      if (first >= wend) break;
+    }
+    float4 lo = nodes[2 * (uint64_t)k], hi = nodes[2 * (uint64_t)k + 1];
+    bool ov = qlo.x <= hi.x && qlo.y <= hi.y && qlo.z <= hi.z && lo.x <= qhi.x && lo.y <= qhi.y && lo.z <= qhi.z;
+    if (ov) {
+      if (k >= N) {
+        uint32_t other = f2u(lo.w);
+        uint32_t a = body < other ? body : other, b = body < other ? other : body;
+        if (c < K) my[c] = ((uint64_t)a << 32) | b;
         ++c;
-        node = rope;
+        k = tree_rope(k);
       } else {
-        node = left;
+        k <<= 1;
       }
     } else {
-      node = rope;
+      k = tree_rope(k);
     }
   }
   counts[s] = c < K ? c : K;
@@ -472,6 +539,7 @@ __global__ void k_split_pairs(uint32_t P, const uint64_t* __restrict__ keys, con
 }
 
 __device__ __forceinline__ void store_contacts(float4* __restrict__ cslots, uint32_t p, const Manifold4& m) {
+#pragma unroll
   for (int c = 0; c < 4; ++c) {
     if (c >= m.n) break;
     cslots[(uint64_t)p * 8 + 2 * c] = make_float4(m.c[c].pos.x, m.c[c].pos.y, m.c[c].pos.z, m.c[c].depth);
@@ -490,52 +558,65 @@ __device__ __forceinline__ Pair make_pair(const BodyRec& A, const BodyRec& B, co
   return pr;
 }
 
-// One lane per pair in shape-class order.  Closed forms and GJK finish here;
-// overlaps past the margin shell are queued for EPA with their tetrahedron.
-__global__ void k_narrow(uint32_t P, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ pairs,
-                         const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
-                         const float* __restrict__ seeds, uint32_t* __restrict__ ccount, float4* __restrict__ cslots,
-                         float4* __restrict__ epa_q, Counters* cnt) {
+// Pair ranges of each shape class in the class-sorted order.
+__global__ void k_class_bounds(uint32_t P, const uint32_t* __restrict__ cls_sorted, Counters* cnt) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  bool need_epa = false;
-  uint32_t p = 0;
-  GjkResult g;
-  if (t < P) {
-    p = perm[t];
-    uint32_t i = pairs[2 * p], j = pairs[2 * p + 1];
-    BodyRec A = bodies[i], B = bodies[j];
-    Pair pr = make_pair(A, B, hull);
-    float thr = mn(A.threshold, B.threshold);  // narrowphase/mod.rs:52-57
-    int cls = pair_class(pr.a.type, pr.b.type);
-    Manifold4 m;
-    V3 sd;
-    const V3* sp = nullptr;
-    if (seeds) {
-      sd = V3{seeds[3 * p], seeds[3 * p + 1], seeds[3 * p + 2]};
-      sp = &sd;
+  if (t >= P) return;
+  uint32_t c = cls_sorted[t];
+  if (t == 0 || cls_sorted[t - 1] != c) cnt->cls_start[c] = t;
+  if (t == P - 1 || cls_sorted[t + 1] != c) cnt->cls_end[c] = t + 1;
+}
+
+// One lane per pair of shape class CLS (grid-stride over the class range):
+// each class gets its own kernel and register budget.  Closed forms and the
+// GJK walk finish here; overlaps past the margin shell are queued for EPA
+// with their tetrahedron.
+template <int CLS>
+__global__ __launch_bounds__(256) void k_np(const Counters* __restrict__ cnt_ro, const uint32_t* __restrict__ perm,
+                     const uint32_t* __restrict__ pairs, const BodyRec* __restrict__ bodies,
+                     const float4* __restrict__ hull, const float* __restrict__ seeds, uint32_t* __restrict__ ccount,
+                     float4* __restrict__ cslots, float4* __restrict__ epa_q, Counters* cnt) {
+  uint32_t beg = cnt_ro->cls_start[CLS], end = cnt_ro->cls_end[CLS];
+  for (uint32_t base = beg + blockIdx.x * blockDim.x; base < end; base += gridDim.x * blockDim.x) {
+    uint32_t t = base + threadIdx.x;
+    bool need_epa = false;
+    uint32_t p = 0;
+    GjkResult g;
+    if (t < end) {
+      p = perm[t];
+      uint32_t i = pairs[2 * p], j = pairs[2 * p + 1];
+      BodyRec A = bodies[i], B = bodies[j];
+      Pair pr = make_pair(A, B, hull);
+      float thr = mn(A.threshold, B.threshold);  // narrowphase/mod.rs:52-57
+      Manifold4 m;
+      V3 sd{0.0f, 0.0f, 0.0f};
+      bool has_seed = CLS == CLS_GJK && seeds != nullptr;
+      if (has_seed) sd = V3{seeds[3 * p], seeds[3 * p + 1], seeds[3 * p + 2]};
+      int r = collide_pair(pr, thr, has_seed, sd, m, g, CLS);
+      if (m.n < 0) {
+        atomicOr(&cnt->errors, 1u);
+        m.n = 0;
+      }
+      need_epa = r == 1;
+      ccount[p] = need_epa ? 0u : (uint32_t)m.n;
+      if (!need_epa) store_contacts(cslots, p, m);
     }
-    int r = collide_pair(pr, thr, sp, m, g, cls);
-    if (m.n < 0) {
-      atomicOr(&cnt->errors, 1u);
-      m.n = 0;
-    }
-    need_epa = r == 1;
-    ccount[p] = need_epa ? 0u : (uint32_t)m.n;
-    if (!need_epa) store_contacts(cslots, p, m);
-  }
-  uint64_t mask = __ballot(need_epa);
-  if (mask) {
-    uint32_t lane = threadIdx.x & 63;
-    uint32_t leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&cnt->n_epa, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    if (need_epa) {
-      uint32_t q = base + __popcll(mask & ((1ull << lane) - 1ull));
-      epa_q[4 * (uint64_t)q] = make_float4(g.t0.x, g.t0.y, g.t0.z, u2f(p));
-      epa_q[4 * (uint64_t)q + 1] = make_float4(g.t1.x, g.t1.y, g.t1.z, 0.0f);
-      epa_q[4 * (uint64_t)q + 2] = make_float4(g.t2.x, g.t2.y, g.t2.z, 0.0f);
-      epa_q[4 * (uint64_t)q + 3] = make_float4(g.t3.x, g.t3.y, g.t3.z, 0.0f);
+    if (CLS == CLS_GJK) {
+      uint64_t mask = __ballot(need_epa);
+      if (mask) {
+        uint32_t lane = threadIdx.x & 63;
+        uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+        uint32_t qb = 0;
+        if (lane == leader) qb = atomicAdd(&cnt->n_epa, (uint32_t)__popcll(mask));
+        qb = __shfl(qb, leader);
+        if (need_epa) {
+          uint32_t q = qb + __popcll(mask & ((1ull << lane) - 1ull));
+          epa_q[4 * (uint64_t)q] = make_float4(g.t0.x, g.t0.y, g.t0.z, u2f(p));
+          epa_q[4 * (uint64_t)q + 1] = make_float4(g.t1.x, g.t1.y, g.t1.z, 0.0f);
+          epa_q[4 * (uint64_t)q + 2] = make_float4(g.t2.x, g.t2.y, g.t2.z, 0.0f);
+          epa_q[4 * (uint64_t)q + 3] = make_float4(g.t3.x, g.t3.y, g.t3.z, 0.0f);
+        }
+      }
     }
   }
 }
@@ -606,6 +687,7 @@ __global__ void k_reset_epa(Counters* c) {
   if (threadIdx.x == 0) {
     c->n_epa = 0;
     c->errors &= ~2u;
+    for (int k = 0; k < 8; ++k) c->cls_start[k] = c->cls_end[k] = 0u;
   }
 }
 

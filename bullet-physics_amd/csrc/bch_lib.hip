@@ -13,6 +13,7 @@
 
 #include "../../include/bullet_collision_hip.h"
 #include "bch_kernels.hpp"
+#include "bch_epa_wave.hpp"
 
 using namespace bch;
 
@@ -46,7 +47,8 @@ struct DevBuf {
 };
 
 constexpr int kBlock = 256;
-constexpr int kEpaGrid = 1024;  // blocks of 64 lanes, one EPA store per lane
+constexpr int kEpaGrid = 2048;  // blocks of four wavefronts, one EPA pair per wavefront
+constexpr uint32_t kNpGrid = 2048;  // narrowphase blocks per shape class (grid-stride)
 inline uint32_t blocks(uint64_t n, int b = kBlock) { return (uint32_t)((n + b - 1) / b); }
 
 }  // namespace
@@ -64,7 +66,8 @@ struct bch_context {
   DevBuf bodies, hull, world_offsets, world_of, staging_a, staging_b, staging_c, staging_d, staging_e, staging_f;
 
   // pass buffers
-  DevBuf aabb, flags, big_ids, counters, keys, keys_alt, vals, vals_alt, nodes, parent, refit, node_first, wend;
+  DevBuf aabb, flags, big_ids, counters, partials, keys, keys_alt, vals, vals_alt, nodes, wend, hull_ids;
+  uint32_t n_hulls = 0;
   DevBuf leaf_counts, leaf_offsets, slots, pair_keys, pair_keys_alt, big_pairs, pairs, cls, cls_alt, perm,
       perm_alt, ccount, coffsets, cslots, contacts, epa_q, epa_v, epa_f, epa_e, tmp;
   uint32_t K = 16;
@@ -119,7 +122,9 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   Counters* cnt = c->counters.as<Counters>();
   if (given_boxes) {
     hipLaunchKernelGGL(k_aabb_given, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->aabb.as<float4>(),
-                       c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(), cnt);
+                       c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(), cnt, c->partials.as<float>());
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, c->partials.as<float>(), blocks(n),
+                       (const float*)nullptr, 0u, cnt);
   }
   hipLaunchKernelGGL(k_morton, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->aabb.as<float4>(), c->flags.as<uint8_t>(),
                      d_world_of, cnt, c->keys.as<uint64_t>(), c->vals.as<uint32_t>());
@@ -133,13 +138,18 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   if (rc) return rc;
   const uint64_t* skeys = c->keys_alt.as<uint64_t>();
   const uint32_t* order = c->vals_alt.as<uint32_t>();
-  uint32_t* nfirst = d_world_of ? c->node_first.as<uint32_t>() : nullptr;
-  hipLaunchKernelGGL(k_leaves, dim3(blocks(n)), dim3(kBlock), 0, s, n, cnt, skeys, order, c->aabb.as<float4>(),
-                     c->nodes.as<Node>(), nfirst);
-  hipLaunchKernelGGL(k_build, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, skeys, c->nodes.as<Node>(),
-                     c->parent.as<int>(), c->refit.as<uint32_t>(), nfirst);
-  hipLaunchKernelGGL(k_refit, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, c->nodes.as<Node>(), c->parent.as<int>(),
-                     c->refit.as<uint32_t>());
+  int L = 0;
+  while ((1u << L) < n) ++L;
+  uint32_t N = 1u << L;
+  float4* nodes = c->nodes.as<float4>();
+  hipLaunchKernelGGL(k_tree_leaves, dim3(blocks(N)), dim3(kBlock), 0, s, N, cnt, order, c->aabb.as<float4>(), nodes);
+  for (int lc = L; lc > 0;) {
+    int m = std::min(9, lc);
+    uint32_t per = 1u << m;
+    hipLaunchKernelGGL(k_tree_up, dim3((1u << lc) / per), dim3(std::max<uint32_t>(per / 2, 1u)), 0, s, nodes,
+                       1u << lc, m);
+    lc -= m;
+  }
   uint32_t* wend = nullptr;
   if (d_world_of) {
     wend = c->wend.as<uint32_t>();
@@ -148,8 +158,8 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   (void)hipEventRecord(c->ev[2], s);
   for (int attempt = 0; attempt < 3; ++attempt) {
     ENSURE(c, c->slots, (size_t)n * c->K * sizeof(uint64_t));
-    hipLaunchKernelGGL(k_traverse, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, c->nodes.as<Node>(), nfirst, wend,
-                       c->K, c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), cnt);
+    hipLaunchKernelGGL(k_traverse, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, nodes, N, L, wend, c->K,
+                       c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), cnt);
     (void)hipEventRecord(c->ev[3], s);
     HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
@@ -213,14 +223,14 @@ int ensure_pass_buffers(bch_context* c, uint32_t n) {
   ENSURE(c, c->flags, (size_t)n);
   ENSURE(c, c->big_ids, (size_t)n * sizeof(uint32_t));
   ENSURE(c, c->counters, sizeof(Counters));
+  ENSURE(c, c->partials, ((size_t)blocks(n) + (size_t)blocks(n, kBlock / 16) + 2) * 8 * sizeof(float));
   ENSURE(c, c->keys, (size_t)n * sizeof(uint64_t));
   ENSURE(c, c->keys_alt, (size_t)n * sizeof(uint64_t));
   ENSURE(c, c->vals, (size_t)n * sizeof(uint32_t));
   ENSURE(c, c->vals_alt, (size_t)n * sizeof(uint32_t));
-  ENSURE(c, c->nodes, (size_t)(2 * n + 1) * sizeof(Node));
-  ENSURE(c, c->parent, (size_t)(2 * n + 1) * sizeof(int));
-  ENSURE(c, c->refit, (size_t)(n + 1) * sizeof(uint32_t));
-  ENSURE(c, c->node_first, (size_t)(2 * n + 1) * sizeof(uint32_t));
+  uint64_t N = 1;
+  while (N < n) N <<= 1;
+  ENSURE(c, c->nodes, (size_t)N * 4 * sizeof(float4));
   ENSURE(c, c->wend, (size_t)(n + 1) * sizeof(uint32_t));
   ENSURE(c, c->leaf_counts, (size_t)(n + 1) * sizeof(uint32_t));
   ENSURE(c, c->leaf_offsets, (size_t)(n + 1) * sizeof(uint32_t));
@@ -243,9 +253,6 @@ int run_narrowphase(bch_context* c, const float* d_seeds) {
   ENSURE(c, c->coffsets, (Pm + 1) * sizeof(uint32_t));
   ENSURE(c, c->cslots, Pm * 8 * sizeof(float4));
   ENSURE(c, c->epa_q, Pm * 4 * sizeof(float4));
-  ENSURE(c, c->epa_v, (size_t)kEpaGrid * 64 * BCH_EPA_MAX_VERTS * sizeof(float4));
-  ENSURE(c, c->epa_f, (size_t)kEpaGrid * 64 * BCH_EPA_MAX_FACES * sizeof(EpaFace));
-  ENSURE(c, c->epa_e, (size_t)kEpaGrid * 64 * BCH_EPA_MAX_EDGES * sizeof(uint32_t));
   hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
   if (P) {
     hipLaunchKernelGGL(k_split_pairs, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->pair_keys.as<uint64_t>(),
@@ -255,14 +262,26 @@ int run_narrowphase(bch_context* c, const float* d_seeds) {
                                        c->perm_alt.as<uint32_t>(), (size_t)P, 0, 3, s);
     });
     if (rc) return rc;
+    hipLaunchKernelGGL(k_class_bounds, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->cls_alt.as<uint32_t>(),
+                       cnt);
     (void)hipEventRecord(c->ev[4], s);
-    hipLaunchKernelGGL(k_narrow, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->perm_alt.as<uint32_t>(),
-                       c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_seeds,
-                       c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->epa_q.as<float4>(), cnt);
+    const uint32_t* perm = c->perm_alt.as<uint32_t>();
+    uint32_t g = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(P), kNpGrid));
+#define BCH_NP(CLS)                                                                                          \
+  hipLaunchKernelGGL(k_np<CLS>, dim3(g), dim3(kBlock), 0, s, cnt, perm, c->pairs.as<uint32_t>(),           \
+                     c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_seeds, c->ccount.as<uint32_t>(),     \
+                     c->cslots.as<float4>(), c->epa_q.as<float4>(), cnt)
+    BCH_NP(CLS_NONE);
+    BCH_NP(CLS_SPHERE_SPHERE);
+    BCH_NP(CLS_SPHERE_BOX);
+    BCH_NP(CLS_PLANE);
+    BCH_NP(CLS_BOX_BOX);
+    BCH_NP(CLS_GJK);
+#undef BCH_NP
     (void)hipEventRecord(c->ev[5], s);
-    hipLaunchKernelGGL(k_epa, dim3(kEpaGrid), dim3(64), 0, s, cnt, c->epa_q.as<float4>(), c->pairs.as<uint32_t>(),
-                       c->bodies.as<BodyRec>(), c->hull.as<float4>(), c->ccount.as<uint32_t>(), c->cslots.as<float4>(),
-                       c->epa_v.as<float4>(), c->epa_f.as<EpaFace>(), c->epa_e.as<uint32_t>(), cnt);
+    hipLaunchKernelGGL(k_epa_wave, dim3(kEpaGrid), dim3(256), 0, s, cnt, c->epa_q.as<float4>(),
+                       c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(), c->ccount.as<uint32_t>(),
+                       c->cslots.as<float4>(), cnt);
     (void)hipEventRecord(c->ev[6], s);
   } else {
     (void)hipEventRecord(c->ev[4], s);
@@ -326,6 +345,16 @@ int upload_bodies(bch_context* c, const bch_bodies* b, const uint32_t* world_of_
     HIP_TRY(c, hipMemcpyAsync(c->world_of.p, world_of_host, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     wo = c->world_of.as<uint32_t>();
   }
+  {
+    std::vector<uint32_t> hid;
+    for (uint32_t i = 0; i < n; ++i)
+      if (b->shape_type[i] == BCH_HULL) hid.push_back(i);
+    c->n_hulls = (uint32_t)hid.size();
+    ENSURE(c, c->hull_ids, std::max<size_t>(hid.size(), 1) * sizeof(uint32_t));
+    if (!hid.empty())
+      HIP_TRY(c, hipMemcpyAsync(c->hull_ids.p, hid.data(), hid.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+  }
   if (n)
     hipLaunchKernelGGL(k_pack_bodies, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->staging_a.as<float>(),
                        c->staging_b.as<float>(), c->staging_c.as<uint8_t>(), c->staging_d.as<float>(), hf, hc,
@@ -380,8 +409,8 @@ void bch_destroy(bch_context* c) {
   (void)hipStreamSynchronize(c->stream);
   DevBuf* all[] = {&c->bodies, &c->hull, &c->world_offsets, &c->world_of, &c->staging_a, &c->staging_b,
                    &c->staging_c, &c->staging_d, &c->staging_e, &c->staging_f, &c->aabb, &c->flags, &c->big_ids,
-                   &c->counters, &c->keys, &c->keys_alt, &c->vals, &c->vals_alt, &c->nodes, &c->parent, &c->refit,
-                   &c->node_first, &c->wend, &c->leaf_counts, &c->leaf_offsets, &c->slots, &c->pair_keys,
+                   &c->counters, &c->partials, &c->keys, &c->keys_alt, &c->vals, &c->vals_alt, &c->nodes, &c->hull_ids,
+                   &c->wend, &c->leaf_counts, &c->leaf_offsets, &c->slots, &c->pair_keys,
                    &c->pair_keys_alt, &c->big_pairs, &c->pairs, &c->cls, &c->cls_alt, &c->perm, &c->perm_alt,
                    &c->ccount, &c->coffsets, &c->cslots, &c->contacts, &c->epa_q, &c->epa_v, &c->epa_f, &c->epa_e,
                    &c->tmp};
@@ -640,6 +669,14 @@ int bch_world_collide(bch_context* c, float aabb_margin, uint64_t* out_pairs, ui
   if (n) {
     hipLaunchKernelGGL(k_aabb, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                        aabb_margin, c->aabb.as<float4>(), c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(),
+                       c->counters.as<Counters>(), c->partials.as<float>());
+    uint32_t hb = c->n_hulls ? blocks(c->n_hulls, kBlock / 16) : 0;
+    float* hpart = c->partials.as<float>() + 8 * (size_t)blocks(n);
+    if (c->n_hulls)
+      hipLaunchKernelGGL(k_hull_aabb, dim3(hb), dim3(kBlock), 0, s, c->n_hulls, c->hull_ids.as<uint32_t>(),
+                         c->bodies.as<BodyRec>(), c->hull.as<float4>(), aabb_margin, c->aabb.as<float4>(),
+                         c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(), c->counters.as<Counters>(), hpart);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, c->partials.as<float>(), blocks(n), hpart, hb,
                        c->counters.as<Counters>());
     (void)hipEventRecord(c->ev[1], s);
     rc = run_broadphase(c, n, c->batched ? c->world_of.as<uint32_t>() : nullptr,

@@ -56,6 +56,8 @@ BCH_HD V3 vmax(V3 a, V3 b) { return V3{mx(a.x, b.x), mx(a.y, b.y), mx(a.z, b.z)}
 BCH_HD V3 vabs(V3 a) { return V3{fabsf(a.x), fabsf(a.y), fabsf(a.z)}; }
 BCH_HD V3 splat(float s) { return V3{s, s, s}; }
 BCH_HD float comp(V3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+// Component-wise select: scalar selects only, so structs stay in registers.
+BCH_HD V3 sel(bool c, V3 a, V3 b) { return V3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
 BCH_HD bool near_eq(V3 a, V3 b, float eps) {
   return fabsf(a.x - b.x) <= eps && fabsf(a.y - b.y) <= eps && fabsf(a.z - b.z) <= eps;
 }
@@ -78,7 +80,8 @@ BCH_HD Q4 qconj(Q4 q) { return Q4{-q.x, -q.y, -q.z, q.w}; }
 struct M3 {
   V3 c0, c1, c2;
 };
-BCH_HD V3 col(const M3& m, int i) { return i == 0 ? m.c0 : (i == 1 ? m.c1 : m.c2); }
+BCH_HD V3 col(const M3& m, int i) { return sel(i == 0, m.c0, sel(i == 1, m.c1, m.c2)); }
+BCH_HD M3 sel(bool c, const M3& a, const M3& b) { return M3{sel(c, a.c0, b.c0), sel(c, a.c1, b.c1), sel(c, a.c2, b.c2)}; }
 BCH_HD M3 from_quat(Q4 q) {
   float x2 = q.x + q.x, y2 = q.y + q.y, z2 = q.z + q.z;
   float xx = q.x * x2, xy = q.x * y2, xz = q.x * z2;

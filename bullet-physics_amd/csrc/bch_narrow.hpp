@@ -27,8 +27,13 @@ struct Contact {
 struct Manifold4 {
   int n;
   Contact c[4];
+  // Constant-index stores keep the manifold in registers on the device.
   BCH_HD void push(V3 p, V3 nrm, float d) {
-    if (n < 4) c[n] = Contact{p, nrm, d};
+    Contact ct{p, nrm, d};
+    if (n == 0) c[0] = ct;
+    else if (n == 1) c[1] = ct;
+    else if (n == 2) c[2] = ct;
+    else if (n == 3) c[3] = ct;
     ++n;
   }
   BCH_HD void flip() {
@@ -45,7 +50,7 @@ BCH_HD bool sphere_sphere(float ra, V3 pa, float rb, V3 pb, Manifold4& m) {
   float radius_sum = ra + rb;
   if (dist_sq >= radius_sum * radius_sum) return false;
   float dist = fsqrt(dist_sq);
-  V3 normal = dist > 1e-12f ? delta / dist : V3{0.0f, 1.0f, 0.0f};
+  V3 normal = sel(dist > 1e-12f, delta / dist, V3{0.0f, 1.0f, 0.0f});
   float pen = radius_sum - dist;
   m.push(pa + normal * (ra - pen * 0.5f), normal, pen);
   return true;
@@ -91,7 +96,7 @@ BCH_HD bool plane_convex(V3 pn, float pc, V3 pp, Q4 pq, const ShapeRef& cv, V3 c
                          Manifold4& m) {
   V3 n = qrot(pq, pn);
   float c = pc + dot(n, pp);
-  V3 normal = dot(n, cp - pp) >= 0.0f ? n : -n;
+  V3 normal = sel(dot(n, cp - pp) >= 0.0f, n, -n);
   if (!(normal.x == n.x && normal.y == n.y && normal.z == n.z)) c = -c;
   V3 deepest = support_world(cv, cp, cq, -normal);
   float depth = c - dot(normal, deepest);
@@ -218,7 +223,7 @@ BCH_HD bool box_box(V3 ha, V3 p1, Q4 oa, V3 hb, V3 p2, Q4 ob, Manifold4& out) {
 #undef BCH_FACE
 #undef BCH_CROSS
 
-  V3 normal = is_cross ? mmul(r1, nc) : (from_r2 ? col(r2, axis) : col(r1, axis));
+  V3 normal = sel(is_cross, mmul(r1, nc), sel(from_r2, col(r2, axis), col(r1, axis)));
   if (invert) normal = -normal;
   float depth = -s;
 
@@ -254,13 +259,13 @@ BCH_HD bool box_box(V3 ha, V3 p1, Q4 oa, V3 hb, V3 p2, Q4 ob, Manifold4& out) {
   }
 
   bool a_ref = code <= 3;
-  const M3& ra = a_ref ? r1 : r2;
-  const M3& rb = a_ref ? r2 : r1;
-  V3 pa_ref = a_ref ? p1 : p2;
-  V3 pb_inc = a_ref ? p2 : p1;
-  V3 sa = a_ref ? ha : hb;
-  V3 sb = a_ref ? hb : ha;
-  V3 normal2 = a_ref ? normal : -normal;
+  M3 ra = sel(a_ref, r1, r2);
+  M3 rb = sel(a_ref, r2, r1);
+  V3 pa_ref = sel(a_ref, p1, p2);
+  V3 pb_inc = sel(a_ref, p2, p1);
+  V3 sa = sel(a_ref, ha, hb);
+  V3 sb = sel(a_ref, hb, ha);
+  V3 normal2 = sel(a_ref, normal, -normal);
   V3 nr = mmul(transpose(rb), normal2);
   V3 anr = vabs(nr);
   int lanr, a1, a2;
@@ -271,8 +276,8 @@ BCH_HD bool box_box(V3 ha, V3 p1, Q4 oa, V3 hb, V3 p2, Q4 ob, Manifold4& out) {
   } else {
     lanr = 2; a1 = 0; a2 = 1;
   }
-  V3 center = comp(nr, lanr) < 0.0f ? (pb_inc - pa_ref) + comp(sb, lanr) * col(rb, lanr)
-                                     : (pb_inc - pa_ref) - comp(sb, lanr) * col(rb, lanr);
+  V3 center = sel(comp(nr, lanr) < 0.0f, (pb_inc - pa_ref) + comp(sb, lanr) * col(rb, lanr),
+                  (pb_inc - pa_ref) - comp(sb, lanr) * col(rb, lanr));
   int code_n = a_ref ? code - 1 : code - 4;
   int code1 = code_n == 0 ? 1 : 0;
   int code2 = code_n == 2 ? 1 : 2;
@@ -392,46 +397,50 @@ struct Reduce {
   V3 v, p_b;
 };
 
-// gjk.rs:398-464
-BCH_HD void closest_tri(V3 a, V3 b, V3 c, V3& q, float& w0, float& w1, float& w2) {
+// gjk.rs:398-464: closest point of triangle (a, b, c) to the origin with
+// barycentric weights.
+struct Tri {
+  V3 q;
+  float w0, w1, w2;
+};
+
+BCH_HD Tri closest_tri(V3 a, V3 b, V3 c) {
   V3 ab = b - a, ac = c - a, ap = -a;
   float d1 = dot(ab, ap), d2 = dot(ac, ap);
-  if (d1 <= 0.0f && d2 <= 0.0f) { q = a; w0 = 1; w1 = 0; w2 = 0; return; }
+  if (d1 <= 0.0f && d2 <= 0.0f) return Tri{a, 1, 0, 0};
   V3 bp = -b;
   float d3 = dot(ab, bp), d4 = dot(ac, bp);
-  if (d3 >= 0.0f && d4 <= d3) { q = b; w0 = 0; w1 = 1; w2 = 0; return; }
+  if (d3 >= 0.0f && d4 <= d3) return Tri{b, 0, 1, 0};
   float vc = d1 * d4 - d3 * d2;
   if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
-    if (d1 == d3) { q = a; w0 = 1; w1 = 0; w2 = 0; return; }
+    if (d1 == d3) return Tri{a, 1, 0, 0};
     float t = d1 / (d1 - d3);
-    q = a + ab * t; w0 = 1.0f - t; w1 = t; w2 = 0; return;
+    return Tri{a + ab * t, 1.0f - t, t, 0};
   }
   V3 cp = -c;
   float d5 = dot(ab, cp), d6 = dot(ac, cp);
-  if (d6 >= 0.0f && d5 <= d6) { q = c; w0 = 0; w1 = 0; w2 = 1; return; }
+  if (d6 >= 0.0f && d5 <= d6) return Tri{c, 0, 0, 1};
   float vb = d5 * d2 - d1 * d6;
   if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
-    if (d2 == d6) { q = a; w0 = 1; w1 = 0; w2 = 0; return; }
+    if (d2 == d6) return Tri{a, 1, 0, 0};
     float t = d2 / (d2 - d6);
-    q = a + ac * t; w0 = 1.0f - t; w1 = 0; w2 = t; return;
+    return Tri{a + ac * t, 1.0f - t, 0, t};
   }
   float va = d3 * d6 - d5 * d4;
   if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
-    if ((d4 - d3) + (d5 - d6) == 0.0f) { q = b; w0 = 0; w1 = 1; w2 = 0; return; }
+    if ((d4 - d3) + (d5 - d6) == 0.0f) return Tri{b, 0, 1, 0};
     float t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
-    q = b + (c - b) * t; w0 = 0; w1 = 1.0f - t; w2 = t; return;
+    return Tri{b + (c - b) * t, 0, 1.0f - t, t};
   }
   float sum = va + vb + vc;
   if (sum <= 0.0f || is_nan(sum)) {
-    if (len2(a) <= len2(b) && len2(a) <= len2(c)) { q = a; w0 = 1; w1 = 0; w2 = 0; }
-    else if (len2(b) <= len2(c)) { q = b; w0 = 0; w1 = 1; w2 = 0; }
-    else { q = c; w0 = 0; w1 = 0; w2 = 1; }
-    return;
+    if (len2(a) <= len2(b) && len2(a) <= len2(c)) return Tri{a, 1, 0, 0};
+    if (len2(b) <= len2(c)) return Tri{b, 0, 1, 0};
+    return Tri{c, 0, 0, 1};
   }
   float denom = 1.0f / sum;
   float v = vb * denom, w = vc * denom;
-  q = (a + ab * v) + ac * w;
-  w0 = 1.0f - v - w; w1 = v; w2 = w;
+  return Tri{(a + ab * v) + ac * w, 1.0f - v - w, v, w};
 }
 
 // The walk's simplex: s[0..n), newest appended last by the walk; the
@@ -441,27 +450,32 @@ struct Simp {
   int n;
 };
 
-BCH_HD SV simp_get(const Simp& S, int i) { return i == 0 ? S.s0 : (i == 1 ? S.s1 : (i == 2 ? S.s2 : S.s3)); }
-BCH_HD void simp_set(Simp& S, int i, SV v) {
-  if (i == 0) S.s0 = v;
-  else if (i == 1) S.s1 = v;
-  else if (i == 2) S.s2 = v;
-  else S.s3 = v;
+
+BCH_HD SV sel(bool c, const SV& a, const SV& b) { return SV{sel(c, a.w, b.w), sel(c, a.p_b, b.p_b)}; }
+
+// Append at position n (1..3): selects on every slot, so the simplex stays in
+// registers.
+BCH_HD void simp_append(Simp& S, SV v) {
+  S.s1 = sel(S.n == 1, v, S.s1);
+  S.s2 = sel(S.n == 2, v, S.s2);
+  S.s3 = sel(S.n == 3, v, S.s3);
+  S.n += 1;
 }
 
-// gjk.rs:466-486 on (a, b, c) = (buf[2], buf[1], buf[0]).
+// gjk.rs:466-486 on (a, b, c) = (buf[2], buf[1], buf[0]): keep the vertices
+// with positive weight, in order a, b, c (a alone when none).
 BCH_HD Reduce reduce_tri_abc(Simp& S, SV a, SV b, SV c) {
-  V3 q;
-  float w0, w1, w2;
-  closest_tri(a.w, b.w, c.w, q, w0, w1, w2);
-  int k = 0;
-  if (w0 > 0.0f) { simp_set(S, k, a); ++k; }
-  if (w1 > 0.0f) { simp_set(S, k, b); ++k; }
-  if (w2 > 0.0f) { simp_set(S, k, c); ++k; }
-  if (k == 0) { S.s0 = a; k = 1; }
-  S.n = k;
-  V3 p_b = (a.p_b * w0 + b.p_b * w1) + c.p_b * w2;
-  return Reduce{false, q, p_b};
+  Tri T = closest_tri(a.w, b.w, c.w);
+  bool p0 = T.w0 > 0.0f, p1 = T.w1 > 0.0f, p2 = T.w2 > 0.0f;
+  int k = (int)p0 + (int)p1 + (int)p2;
+  SV k0 = sel(p0, a, sel(p1, b, sel(p2, c, a)));
+  SV k1 = sel(p0 && p1, b, c);
+  S.s0 = k0;
+  S.s1 = sel(k >= 2, k1, S.s1);
+  S.s2 = sel(k >= 3, c, S.s2);
+  S.n = k > 0 ? k : 1;
+  V3 p_b = (a.p_b * T.w0 + b.p_b * T.w1) + c.p_b * T.w2;
+  return Reduce{false, T.q, p_b};
 }
 
 // gjk.rs:352-567
@@ -488,34 +502,34 @@ BCH_HD Reduce reduce(Simp& S) {
     // Closest of faces (a,b,d), (a,c,d), (b,c,d); face (a,b,c) is only kept
     // when every other face scores NaN (its score starts at +inf).
     SV fa = a, fb = b, fc = c;
-    V3 bq;
-    float b0, b1, b2;
-    closest_tri(a.w, b.w, c.w, bq, b0, b1, b2);
+    Tri T0 = closest_tri(a.w, b.w, c.w);
+    V3 bq = T0.q;
+    float b0 = T0.w0, b1 = T0.w1, b2 = T0.w2;
     float best_v2 = INFINITY;
     for (int f = 1; f < 4; ++f) {
-      SV x = f == 3 ? b : a;
-      SV y = f == 1 ? b : c;
+      SV x{sel(f == 3, b.w, a.w), sel(f == 3, b.p_b, a.p_b)};
+      SV y{sel(f == 1, b.w, c.w), sel(f == 1, b.p_b, c.p_b)};
       SV z = d;
-      V3 q;
-      float w0, w1, w2;
-      closest_tri(x.w, y.w, z.w, q, w0, w1, w2);
-      float v2 = len2(q);
-      if (v2 < best_v2) {
-        best_v2 = v2;
-        bq = q; b0 = w0; b1 = w1; b2 = w2;
-        fa = x; fb = y; fc = z;
-      }
+      Tri T = closest_tri(x.w, y.w, z.w);
+      float v2 = len2(T.q);
+      bool better = v2 < best_v2;
+      best_v2 = better ? v2 : best_v2;
+      bq = sel(better, T.q, bq);
+      b0 = better ? T.w0 : b0;
+      b1 = better ? T.w1 : b1;
+      b2 = better ? T.w2 : b2;
+      fa = sel(better, x, fa);
+      fb = sel(better, y, fb);
+      fc = sel(better, z, fc);
     }
-    int k = 0;
-    SV k0 = S.s0, k1 = S.s0, k2 = S.s0;
-    if (b0 > 0.0f) { k0 = fa; ++k; }
-    if (b1 > 0.0f) { if (k == 0) k0 = fb; else k1 = fb; ++k; }
-    if (b2 > 0.0f) { if (k == 0) k0 = fc; else if (k == 1) k1 = fc; else k2 = fc; ++k; }
-    if (k == 0) { k0 = fa; k = 1; }
+    bool p0 = b0 > 0.0f, p1 = b1 > 0.0f, p2 = b2 > 0.0f;
+    int k = (int)p0 + (int)p1 + (int)p2;
+    SV k0 = sel(p0, fa, sel(p1, fb, sel(p2, fc, fa)));
+    SV k1 = sel(p0 && p1, fb, fc);
     S.s0 = k0;
-    if (k > 1) S.s1 = k1;
-    if (k > 2) S.s2 = k2;
-    S.n = k;
+    S.s1 = sel(k >= 2, k1, S.s1);
+    S.s2 = sel(k >= 3, fc, S.s2);
+    S.n = k > 0 ? k : 1;
     V3 p_b = (fa.p_b * b0 + fb.p_b * b1) + fc.p_b * b2;
     return Reduce{false, bq, p_b};
   }
@@ -532,10 +546,10 @@ BCH_HD Reduce reduce(Simp& S) {
 // gjk.rs:668-680
 BCH_HD V3 perpendicular(V3 v) {
   bool small = fabsf(v.x) < 0.9f;
-  V3 helper = small ? V3{1.0f, 0.0f, 0.0f} : V3{0.0f, 1.0f, 0.0f};
+  V3 helper = sel(small, V3{1.0f, 0.0f, 0.0f}, V3{0.0f, 1.0f, 0.0f});
   V3 perp = cross(v, helper);
   if (len2(perp) > 0.0f) return normalize(perp);
-  V3 alt = small ? V3{0.0f, 1.0f, 0.0f} : V3{1.0f, 0.0f, 0.0f};
+  V3 alt = sel(small, V3{0.0f, 1.0f, 0.0f}, V3{1.0f, 0.0f, 0.0f});
   perp = cross(v, alt);
   if (len2(perp) > 0.0f) return normalize(perp);
   return V3{0.0f, 0.0f, 1.0f};
@@ -565,7 +579,7 @@ BCH_HD bool shrink_or_normal(WSimp& S, V3& dir, V3 closest, V3 normal, V3 n0, V3
   return false;
 }
 
-BCH_HD V3 toward_origin(V3 n, V3 t) { return dot(n, t) < 0.0f ? -n : n; }
+BCH_HD V3 toward_origin(V3 n, V3 t) { return sel(dot(n, t) < 0.0f, -n, n); }
 
 BCH_HD bool solve_simplex(WSimp& S, V3& dir) {
   if (S.n == 2) {  // gjk.rs:624-646
@@ -636,7 +650,7 @@ BCH_HD bool solve_simplex(WSimp& S, V3& dir) {
 }
 
 BCH_HD bool enclosing_walk(const Pair& pr, V3 fallback, V3 seed, WSimp& S) {
-  V3 initial = len2(seed) > 1e-12f ? seed : fallback;
+  V3 initial = sel(len2(seed) > 1e-12f, seed, fallback);
   S.p0 = core_support(pr, initial).w;
   S.p1 = S.p2 = S.p3 = S.p0;
   S.n = 1;
@@ -652,8 +666,8 @@ BCH_HD bool enclosing_walk(const Pair& pr, V3 fallback, V3 seed, WSimp& S) {
     S.p0 = a;
     S.n += 1;
     if (solve_simplex(S, dir)) return true;
-    V3 last = S.n == 1 ? S.p0 : (S.n == 2 ? S.p1 : (S.n == 3 ? S.p2 : S.p3));
-    V3 reference = S.n >= 2 ? S.p0 - last : S.p0;
+    V3 last = sel(S.n == 1, S.p0, sel(S.n == 2, S.p1, sel(S.n == 3, S.p2, S.p3)));
+    V3 reference = sel(S.n >= 2, S.p0 - last, S.p0);
     if (len2(dir) < len2(reference) * 1e-12f) dir = perpendicular(reference);
   }
   return false;
@@ -667,14 +681,14 @@ struct GjkResult {
 };
 
 // gjk.rs:171-288 -- the margin-core distance walk.
-BCH_HD GjkResult gjk_contact(const Pair& pr, const V3* seed, float threshold) {
+BCH_HD GjkResult gjk_contact(const Pair& pr, bool has_seed, V3 seed, float threshold) {
   GjkResult R;
   R.kind = 0;
   float margin_a = pr.a.margin, margin_b = pr.b.margin;
   float margin_sum = margin_a + margin_b;
   V3 center_dir = pr.pb - pr.pa;
-  V3 fallback = len2(center_dir) > 1e-12f ? center_dir : V3{1.0f, 0.0f, 0.0f};
-  V3 initial = (seed && len2(*seed) > 1e-12f) ? *seed : fallback;
+  V3 fallback = sel(len2(center_dir) > 1e-12f, center_dir, V3{1.0f, 0.0f, 0.0f});
+  V3 initial = sel(has_seed && len2(seed) > 1e-12f, seed, fallback);
   SV first = core_support(pr, initial);
   Simp S;
   S.s0 = S.s1 = S.s2 = S.s3 = first;
@@ -726,8 +740,7 @@ BCH_HD GjkResult gjk_contact(const Pair& pr, const V3* seed, float threshold) {
       }
       return R;
     }
-    simp_set(S, S.n, w);
-    S.n += 1;
+    simp_append(S, w);
   }
   return R;
 }
@@ -966,7 +979,8 @@ BCH_HD int pair_class(int ta, int tb) {
 
 // collide_seeded for every class but the EPA continuation.  Returns:
 //   0 -> done (m holds 0..4 points), 1 -> needs EPA (g holds the tetrahedron).
-BCH_HD int collide_pair(const Pair& pr, float threshold, const V3* seed, Manifold4& m, GjkResult& g, int cls) {
+BCH_HD int collide_pair(const Pair& pr, float threshold, bool has_seed, V3 seed, Manifold4& m, GjkResult& g,
+                        int cls) {
   m.n = 0;
   switch (cls) {
     case CLS_NONE: return 0;
@@ -990,7 +1004,7 @@ BCH_HD int collide_pair(const Pair& pr, float threshold, const V3* seed, Manifol
       box_box(V3{pr.a.a0, pr.a.a1, pr.a.a2}, pr.pa, pr.qa, V3{pr.b.a0, pr.b.a1, pr.b.a2}, pr.pb, pr.qb, m);
       return 0;
     default: {
-      g = gjk_contact(pr, seed, threshold);
+      g = gjk_contact(pr, has_seed, seed, threshold);
       if (g.kind == 1) {
         m.push(g.point_on_b, g.normal, -g.distance);
         return 0;
