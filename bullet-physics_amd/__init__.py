@@ -48,6 +48,25 @@ class _Context:
             raise BchError(rc, "bch_create failed (no HIP device?)")
         self.h = h
 
+    _KNOBS = {"hull_lanes": _lib.TUNE_HULL_LANES,
+              "pair_slots": _lib.TUNE_PAIR_SLOTS, "epa_grid": _lib.TUNE_EPA_GRID, "np_grid": _lib.TUNE_NP_GRID}
+
+    def set_tuning(self, **knobs):
+        """Launch-shape knobs (hull_lanes, pair_slots, epa_grid,
+        np_grid); they never change results."""
+        for k, v in knobs.items():
+            if k not in self._KNOBS:
+                raise KeyError(f"unknown tuning knob {k!r}")
+            _check(self.h, self.L.bch_set_tuning(self.h, self._KNOBS[k], int(v)))
+
+    def tuning(self) -> dict:
+        out = {}
+        for k, knob in self._KNOBS.items():
+            v = C.c_int()
+            _check(self.h, self.L.bch_get_tuning(self.h, knob, C.byref(v)))
+            out[k] = v.value
+        return out
+
     def close(self):
         if self.h:
             self.L.bch_destroy(self.h)

@@ -19,8 +19,11 @@ EXPORTS = [
     "bch_bp_create_proxy", "bch_bp_set_aabb", "bch_bp_destroy_proxy", "bch_bp_proxy_aabb", "bch_bp_set_aabbs",
     "bch_bp_compute_pairs", "bch_bp_aabb_candidates", "bch_collide_batch",
     "bch_world_set_bodies", "bch_world_set_poses", "bch_world_set_poses_device", "bch_world_collide",
-    "bch_world_results", "bch_world_results_device",
+    "bch_world_results", "bch_world_results_device", "bch_set_tuning", "bch_get_tuning",
 ]
+
+# launch-shape knobs (bch_set_tuning)
+TUNE_HULL_LANES, TUNE_PAIR_SLOTS, TUNE_EPA_GRID, TUNE_NP_GRID = 1, 2, 3, 4
 
 
 class BchBodies(C.Structure):
@@ -68,5 +71,7 @@ def load():
     L.bch_world_collide.argtypes = [vp, f32, C.POINTER(u64), C.POINTER(u64)]
     L.bch_world_results.argtypes = [vp, vp, vp, vp]
     L.bch_world_results_device.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
+    L.bch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+    L.bch_get_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     _lib = L
     return L

@@ -621,6 +621,185 @@ __global__ __launch_bounds__(256) void k_np(const Counters* __restrict__ cnt_ro,
   }
 }
 
+// ------------------------------------------------------ DPP reductions --
+// Cross-lane moves within 16-lane rows (gfx9 DPP controls): quad_perm xor 1
+// (0xB1), quad_perm xor 2 (0x4E), row half-mirror (0x141), row mirror
+// (0x140).  Combining a lane with these partners in turn leaves every lane
+// of the row with the row's reduction (the combine is commutative and
+// associative).  A few cycles each, against ~100 for ds_bpermute.
+// A partner lane that is switched off yields the lane's own value (old =
+// src, bound_ctrl off), which the combine absorbs.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+}
+
+// (value, index) argmax: larger value wins, ties and invalid (-1) by index.
+__device__ __forceinline__ void amax_merge(float& v, int& i, float ov, int oi) {
+  if (oi >= 0 && (i < 0 || ov > v || (ov == v && oi < i))) {
+    v = ov;
+    i = oi;
+  }
+}
+__device__ __forceinline__ void amin_merge(float& v, int& i, float ov, int oi) {
+  if (oi >= 0 && (i < 0 || ov < v || (ov == v && oi < i))) {
+    v = ov;
+    i = oi;
+  }
+}
+
+template <bool MAX, int WIDTH>
+__device__ __forceinline__ void row_reduce(float& v, int& i) {
+  // WIDTH in {4, 8, 16}: number of lanes of the group, aligned in the row
+  {
+    float ov = dpp_f<0xB1>(v);
+    int oi = dpp_i<0xB1>(i);
+    if (MAX) amax_merge(v, i, ov, oi); else amin_merge(v, i, ov, oi);
+  }
+  {
+    float ov = dpp_f<0x4E>(v);
+    int oi = dpp_i<0x4E>(i);
+    if (MAX) amax_merge(v, i, ov, oi); else amin_merge(v, i, ov, oi);
+  }
+  if (WIDTH >= 8) {
+    float ov = dpp_f<0x141>(v);
+    int oi = dpp_i<0x141>(i);
+    if (MAX) amax_merge(v, i, ov, oi); else amin_merge(v, i, ov, oi);
+  }
+  if (WIDTH >= 16) {
+    float ov = dpp_f<0x140>(v);
+    int oi = dpp_i<0x140>(i);
+    if (MAX) amax_merge(v, i, ov, oi); else amin_merge(v, i, ov, oi);
+  }
+}
+
+// Whole-wavefront reduction (all 64 lanes active): rows by DPP, then the
+// four row results combined from lanes 0, 16, 32, 48 (uniform result).
+template <bool MAX>
+__device__ __forceinline__ void wave_reduce(float& v, int& i) {
+  row_reduce<MAX, 16>(v, i);
+  float rv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+  int ri = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+  for (int r = 1; r < 4; ++r) {
+    float ov = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16 * r));
+    int oi = __builtin_amdgcn_readlane(i, 16 * r);
+    if (MAX) amax_merge(rv, ri, ov, oi); else amin_merge(rv, ri, ov, oi);
+  }
+  v = rv;
+  i = ri;
+}
+
+// ---------------------------------------------------- hull pairs by group --
+// Support of a hull split over the G lanes of a group: each lane keeps the
+// first maximiser of its vertex subset, the group reduces (largest value,
+// then lowest index) = the sequential first maximum (convex_hull.rs:55-66).
+template <int G>
+__device__ __forceinline__ V3 group_hull_support(const float4* h, int n, V3 d, int sub) {
+  float bv = -INFINITY;
+  int bi = -1;
+  for (int v = sub; v < n; v += G) {
+    float dd = dot(hull_vertex(h, v), d);
+    if (dd > bv) {
+      bv = dd;
+      bi = v;
+    }
+  }
+  static_assert(G == 4 || G == 8 || G == 16, "group of 4, 8 or 16 lanes within a DPP row");
+  row_reduce<true, G>(bv, bi);
+  return bi >= 0 ? hull_vertex(h, bi) : splat(0.0f);
+}
+
+template <int G>
+__device__ __forceinline__ V3 group_core_support_world(const ShapeRef& s, V3 p, Q4 q, V3 d, int sub) {
+  V3 ld = qrot(qconj(q), d);
+  V3 l = s.type == HULL ? group_hull_support<G>(s.hull, s.nhull, ld, sub) : core_support_local(s, ld);
+  return qrot(q, l) + p;
+}
+
+template <int G>
+struct GroupSupport {
+  const Pair* pr;
+  int sub;
+  __device__ SV operator()(V3 d) const {
+    V3 a = group_core_support_world<G>(pr->a, pr->pa, pr->qa, d, sub);
+    V3 b = group_core_support_world<G>(pr->b, pr->pb, pr->qb, -d, sub);
+    return SV{a - b, b};
+  }
+};
+
+// Pairs with a hull (class CLS_GJK_HULL), G lanes per pair: every lane of a
+// group runs the same walk; the hull support scans are split over the group.
+template <int G>
+__global__ __launch_bounds__(256) void k_np_hull(const Counters* __restrict__ cnt_ro, const uint32_t* __restrict__ perm,
+                                                 const uint32_t* __restrict__ pairs, const BodyRec* __restrict__ bodies,
+                                                 const float4* __restrict__ hull, const float* __restrict__ seeds,
+                                                 uint32_t* __restrict__ ccount, float4* __restrict__ cslots,
+                                                 float4* __restrict__ epa_q, Counters* cnt) {
+  // Small hulls of the pair are staged in LDS, one slot pair per group.
+  __shared__ float4 hstage[256 / G][2][64];
+  uint32_t beg = cnt_ro->cls_start[CLS_GJK_HULL], end = cnt_ro->cls_end[CLS_GJK_HULL];
+  const uint32_t per_block = blockDim.x / G;
+  int grp = threadIdx.x / G, sub = threadIdx.x % G;
+  for (uint32_t base = beg + blockIdx.x * per_block; base < end; base += gridDim.x * per_block) {
+    uint32_t t = base + grp;
+    bool need_epa = false;
+    uint32_t p = 0;
+    GjkResult g;
+    if (t < end) {
+      p = perm[t];
+      uint32_t i = pairs[2 * p], j = pairs[2 * p + 1];
+      BodyRec A = bodies[i], B = bodies[j];
+      Pair pr = make_pair(A, B, hull);
+      if (pr.a.type == HULL && pr.a.nhull <= 64) {
+        for (int v = sub; v < pr.a.nhull; v += G) hstage[grp][0][v] = pr.a.hull[v];
+        pr.a.hull = hstage[grp][0];
+      }
+      if (pr.b.type == HULL && pr.b.nhull <= 64) {
+        for (int v = sub; v < pr.b.nhull; v += G) hstage[grp][1][v] = pr.b.hull[v];
+        pr.b.hull = hstage[grp][1];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      float thr = mn(A.threshold, B.threshold);
+      V3 sd{0.0f, 0.0f, 0.0f};
+      bool has_seed = seeds != nullptr;
+      if (has_seed) sd = V3{seeds[3 * p], seeds[3 * p + 1], seeds[3 * p + 2]};
+      g = gjk_contact_t(pr, has_seed, sd, thr, GroupSupport<G>{&pr, sub});
+      need_epa = g.kind == 2;
+      if (sub == 0) {
+        Manifold4 m;
+        m.n = 0;
+        if (g.kind == 1) m.push(g.point_on_b, g.normal, -g.distance);
+        ccount[p] = (uint32_t)m.n;
+        if (!need_epa) store_contacts(cslots, p, m);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    bool enq = need_epa && sub == 0;
+    uint64_t mask = __ballot(enq);
+    if (mask) {
+      uint32_t lane = threadIdx.x & 63;
+      uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+      uint32_t qb = 0;
+      if (lane == leader) qb = atomicAdd(&cnt->n_epa, (uint32_t)__popcll(mask));
+      qb = __shfl(qb, leader);
+      if (enq) {
+        uint32_t q = qb + __popcll(mask & ((1ull << lane) - 1ull));
+        epa_q[4 * (uint64_t)q] = make_float4(g.t0.x, g.t0.y, g.t0.z, u2f(p));
+        epa_q[4 * (uint64_t)q + 1] = make_float4(g.t1.x, g.t1.y, g.t1.z, 0.0f);
+        epa_q[4 * (uint64_t)q + 2] = make_float4(g.t2.x, g.t2.y, g.t2.z, 0.0f);
+        epa_q[4 * (uint64_t)q + 3] = make_float4(g.t3.x, g.t3.y, g.t3.z, 0.0f);
+      }
+    }
+  }
+}
+
 // EPA, one lane per queued pair; lane-private polytope in global scratch
 // (grid-stride over the queue with a fixed grid, one store per lane).
 __global__ void k_epa(const Counters* __restrict__ cnt_ro, const float4* __restrict__ epa_q,

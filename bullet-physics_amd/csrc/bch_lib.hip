@@ -47,8 +47,6 @@ struct DevBuf {
 };
 
 constexpr int kBlock = 256;
-constexpr int kEpaGrid = 2048;  // blocks of four wavefronts, one EPA pair per wavefront
-constexpr uint32_t kNpGrid = 2048;  // narrowphase blocks per shape class (grid-stride)
 inline uint32_t blocks(uint64_t n, int b = kBlock) { return (uint32_t)((n + b - 1) / b); }
 
 }  // namespace
@@ -69,7 +67,10 @@ struct bch_context {
   DevBuf aabb, flags, big_ids, counters, partials, keys, keys_alt, vals, vals_alt, nodes, wend, hull_ids;
   uint32_t n_hulls = 0;
   DevBuf leaf_counts, leaf_offsets, slots, pair_keys, pair_keys_alt, big_pairs, pairs, cls, cls_alt, perm,
-      perm_alt, ccount, coffsets, cslots, contacts, epa_q, epa_v, epa_f, epa_e, tmp;
+      perm_alt, ccount, coffsets, cslots, contacts, epa_q, tmp;
+  int hull_lanes = 16;      // lanes per hull GJK pair (4, 8 or 16)
+  uint32_t epa_grid = 32768; // EPA blocks (one wavefront each)
+  uint32_t np_grid = 2048;  // narrowphase blocks per shape class
   uint32_t K = 16;
   uint64_t P = 0, C = 0;
   Counters host_cnt{};
@@ -266,7 +267,7 @@ int run_narrowphase(bch_context* c, const float* d_seeds) {
                        cnt);
     (void)hipEventRecord(c->ev[4], s);
     const uint32_t* perm = c->perm_alt.as<uint32_t>();
-    uint32_t g = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(P), kNpGrid));
+    uint32_t g = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(P), c->np_grid));
 #define BCH_NP(CLS)                                                                                          \
   hipLaunchKernelGGL(k_np<CLS>, dim3(g), dim3(kBlock), 0, s, cnt, perm, c->pairs.as<uint32_t>(),           \
                      c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_seeds, c->ccount.as<uint32_t>(),     \
@@ -278,10 +279,21 @@ int run_narrowphase(bch_context* c, const float* d_seeds) {
     BCH_NP(CLS_BOX_BOX);
     BCH_NP(CLS_GJK);
 #undef BCH_NP
+#define BCH_NP_HULL(G)                                                                                     \
+  hipLaunchKernelGGL(k_np_hull<G>, dim3(g), dim3(kBlock), 0, s, cnt, perm, c->pairs.as<uint32_t>(),        \
+                     c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_seeds, c->ccount.as<uint32_t>(),     \
+                     c->cslots.as<float4>(), c->epa_q.as<float4>(), cnt)
+    if (c->hull_lanes == 4)
+      BCH_NP_HULL(4);
+    else if (c->hull_lanes == 8)
+      BCH_NP_HULL(8);
+    else
+      BCH_NP_HULL(16);
+#undef BCH_NP_HULL
     (void)hipEventRecord(c->ev[5], s);
-    hipLaunchKernelGGL(k_epa_wave, dim3(kEpaGrid), dim3(256), 0, s, cnt, c->epa_q.as<float4>(),
-                       c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(), c->ccount.as<uint32_t>(),
-                       c->cslots.as<float4>(), cnt);
+    hipLaunchKernelGGL(k_epa_grp<64>, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
+                       c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
+                       c->ccount.as<uint32_t>(), c->cslots.as<float4>(), cnt);
     (void)hipEventRecord(c->ev[6], s);
   } else {
     (void)hipEventRecord(c->ev[4], s);
@@ -412,7 +424,7 @@ void bch_destroy(bch_context* c) {
                    &c->counters, &c->partials, &c->keys, &c->keys_alt, &c->vals, &c->vals_alt, &c->nodes, &c->hull_ids,
                    &c->wend, &c->leaf_counts, &c->leaf_offsets, &c->slots, &c->pair_keys,
                    &c->pair_keys_alt, &c->big_pairs, &c->pairs, &c->cls, &c->cls_alt, &c->perm, &c->perm_alt,
-                   &c->ccount, &c->coffsets, &c->cslots, &c->contacts, &c->epa_q, &c->epa_v, &c->epa_f, &c->epa_e,
+                   &c->ccount, &c->coffsets, &c->cslots, &c->contacts, &c->epa_q,
                    &c->tmp};
   for (DevBuf* b : all) b->release();
   for (auto& e : c->ev) (void)hipEventDestroy(e);
@@ -432,6 +444,41 @@ int bch_last_timings(bch_context* c, float* out_ms) {
   if (!c || !out_ms) return BCH_INVALID_ARG;
   for (int i = 0; i < 7; ++i) out_ms[i] = c->timings[i];
   return BCH_OK;
+}
+
+int bch_set_tuning(bch_context* c, int knob, int value) {
+  if (!c) return BCH_INVALID_ARG;
+  switch (knob) {
+    case BCH_TUNE_HULL_LANES:
+      if (value != 4 && value != 8 && value != 16) return fail(c, BCH_INVALID_ARG, "hull lanes must be 4, 8 or 16");
+      c->hull_lanes = value;
+      return BCH_OK;
+    case BCH_TUNE_PAIR_SLOTS:
+      if (value < 1 || value > 4096) return fail(c, BCH_INVALID_ARG, "pair slots per leaf must be in [1, 4096]");
+      c->K = (uint32_t)value;
+      return BCH_OK;
+    case BCH_TUNE_EPA_GRID:
+      if (value < 1 || value > (1 << 20)) return fail(c, BCH_INVALID_ARG, "EPA grid must be in [1, 2^20]");
+      c->epa_grid = (uint32_t)value;
+      return BCH_OK;
+    case BCH_TUNE_NP_GRID:
+      if (value < 1 || value > (1 << 20)) return fail(c, BCH_INVALID_ARG, "narrowphase grid must be in [1, 2^20]");
+      c->np_grid = (uint32_t)value;
+      return BCH_OK;
+    default:
+      return fail(c, BCH_INVALID_ARG, "unknown tuning knob");
+  }
+}
+
+int bch_get_tuning(bch_context* c, int knob, int* value) {
+  if (!c || !value) return BCH_INVALID_ARG;
+  switch (knob) {
+    case BCH_TUNE_HULL_LANES: *value = c->hull_lanes; return BCH_OK;
+    case BCH_TUNE_PAIR_SLOTS: *value = (int)c->K; return BCH_OK;
+    case BCH_TUNE_EPA_GRID: *value = (int)c->epa_grid; return BCH_OK;
+    case BCH_TUNE_NP_GRID: *value = (int)c->np_grid; return BCH_OK;
+    default: return fail(c, BCH_INVALID_ARG, "unknown tuning knob");
+  }
 }
 
 // ------------------------------------------------------------ proxies --

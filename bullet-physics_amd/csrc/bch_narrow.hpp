@@ -649,16 +649,17 @@ BCH_HD bool solve_simplex(WSimp& S, V3& dir) {
   return false;
 }
 
-BCH_HD bool enclosing_walk(const Pair& pr, V3 fallback, V3 seed, WSimp& S) {
+template <class SupF>
+BCH_HD bool enclosing_walk_t(SupF sup, V3 fallback, V3 seed, WSimp& S) {
   V3 initial = sel(len2(seed) > 1e-12f, seed, fallback);
-  S.p0 = core_support(pr, initial).w;
+  S.p0 = sup(initial).w;
   S.p1 = S.p2 = S.p3 = S.p0;
   S.n = 1;
   V3 dir = -S.p0;
   for (int it = 0; it < 64; ++it) {
     float dir2 = len2(dir);
     if (dir2 <= 0.0f || is_nan(dir2)) dir = perpendicular(S.p0);
-    V3 a = core_support(pr, dir).w;
+    V3 a = sup(dir).w;
     if (dot(a, dir) < 0.0f) return false;
     S.p3 = S.p2;
     S.p2 = S.p1;
@@ -681,7 +682,11 @@ struct GjkResult {
 };
 
 // gjk.rs:171-288 -- the margin-core distance walk.
-BCH_HD GjkResult gjk_contact(const Pair& pr, bool has_seed, V3 seed, float threshold) {
+// The support is a functor SV(V3 dir): one lane's scan (core_support) or a
+// lane group's cooperative scan of a hull; every lane of a group runs the
+// walk on identical values.
+template <class SupF>
+BCH_HD GjkResult gjk_contact_t(const Pair& pr, bool has_seed, V3 seed, float threshold, SupF sup) {
   GjkResult R;
   R.kind = 0;
   float margin_a = pr.a.margin, margin_b = pr.b.margin;
@@ -689,7 +694,7 @@ BCH_HD GjkResult gjk_contact(const Pair& pr, bool has_seed, V3 seed, float thres
   V3 center_dir = pr.pb - pr.pa;
   V3 fallback = sel(len2(center_dir) > 1e-12f, center_dir, V3{1.0f, 0.0f, 0.0f});
   V3 initial = sel(has_seed && len2(seed) > 1e-12f, seed, fallback);
-  SV first = core_support(pr, initial);
+  SV first = sup(initial);
   Simp S;
   S.s0 = S.s1 = S.s2 = S.s3 = first;
   S.n = 1;
@@ -709,7 +714,7 @@ BCH_HD GjkResult gjk_contact(const Pair& pr, bool has_seed, V3 seed, float thres
     float v2 = len2(v);
     if (v2 < touch2) {
       WSimp W;
-      bool ok = enclosing_walk(pr, fallback, dir, W);
+      bool ok = enclosing_walk_t(sup, fallback, dir, W);
       if (ok && W.n == 4) {
         R.kind = 2;
         R.t0 = W.p0;
@@ -720,7 +725,7 @@ BCH_HD GjkResult gjk_contact(const Pair& pr, bool has_seed, V3 seed, float thres
       return R;
     }
     dir = -v;
-    SV w = core_support(pr, dir);
+    SV w = sup(dir);
     float delta = dot(v, w.w);
     float band = margin_sum + threshold;
     if (delta > 0.0f && delta * delta > v2 * band * band) return R;
@@ -743,6 +748,15 @@ BCH_HD GjkResult gjk_contact(const Pair& pr, bool has_seed, V3 seed, float thres
     simp_append(S, w);
   }
   return R;
+}
+
+struct LaneSupport {
+  const Pair* pr;
+  BCH_HD SV operator()(V3 d) const { return core_support(*pr, d); }
+};
+
+BCH_HD GjkResult gjk_contact(const Pair& pr, bool has_seed, V3 seed, float threshold) {
+  return gjk_contact_t(pr, has_seed, seed, threshold, LaneSupport{&pr});
 }
 
 // -------------------------------------------------------------------- EPA --
@@ -966,7 +980,17 @@ BCH_HD bool epa_contact(const Pair& pr, const EpaResult& e, Manifold4& m) {
 
 // ---------------------------------------------------------- pair classes --
 // Which routine collide_seeded (narrowphase/mod.rs:99-285) sends a pair to.
-enum : int { CLS_NONE = 0, CLS_SPHERE_SPHERE = 1, CLS_SPHERE_BOX = 2, CLS_PLANE = 3, CLS_BOX_BOX = 4, CLS_GJK = 5 };
+// CLS_GJK_HULL: the same GJK/EPA route as CLS_GJK for pairs with a hull,
+// evaluated by lane groups (the hull support scan is split over the group).
+enum : int {
+  CLS_NONE = 0,
+  CLS_SPHERE_SPHERE = 1,
+  CLS_SPHERE_BOX = 2,
+  CLS_PLANE = 3,
+  CLS_BOX_BOX = 4,
+  CLS_GJK = 5,
+  CLS_GJK_HULL = 6
+};
 
 BCH_HD int pair_class(int ta, int tb) {
   if (ta == PLANE && tb == PLANE) return CLS_NONE;
@@ -974,6 +998,7 @@ BCH_HD int pair_class(int ta, int tb) {
   if (ta == SPHERE && tb == SPHERE) return CLS_SPHERE_SPHERE;
   if ((ta == SPHERE && tb == BOX) || (ta == BOX && tb == SPHERE)) return CLS_SPHERE_BOX;
   if (ta == BOX && tb == BOX) return CLS_BOX_BOX;
+  if (ta == HULL || tb == HULL) return CLS_GJK_HULL;
   return CLS_GJK;
 }
 

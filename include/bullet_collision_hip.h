@@ -75,6 +75,21 @@ int bch_synchronize(bch_context* ctx);
  * sort), [4] narrowphase, [5] EPA, [6] contact compaction. */
 int bch_last_timings(bch_context* ctx, float* out_ms);
 
+/* No reference counterpart: launch-shape knobs of the device passes.  They
+ * change how work is spread over the GPU, never the results.
+ *   BCH_TUNE_HULL_LANES  lanes per GJK pair with a hull: 4, 8, 16 (16)
+ *   BCH_TUNE_PAIR_SLOTS  initial pair slots per tree leaf (16; grows itself)
+ *   BCH_TUNE_EPA_GRID    EPA blocks of one wavefront (32768)
+ *   BCH_TUNE_NP_GRID     narrowphase blocks per shape class (2048) */
+enum {
+  BCH_TUNE_HULL_LANES = 1,
+  BCH_TUNE_PAIR_SLOTS = 2,
+  BCH_TUNE_EPA_GRID = 3,
+  BCH_TUNE_NP_GRID = 4
+};
+int bch_set_tuning(bch_context* ctx, int knob, int value);
+int bch_get_tuning(bch_context* ctx, int knob, int* value);
+
 /* ------------------------------------------ BroadPhase trait (proxies) -- */
 /* broadphase/mod.rs:94-130.  Slots follow the reference: create_proxy reuses
  * the lowest free slot (brute_force.rs:26-34); set_aabb / destroy on a dead
