@@ -25,31 +25,42 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-STAGES = ["k_aabb", "tree_build", "k_traverse", "pair_list", "k_narrow", "k_epa", "contact_compaction"]
+STAGES = ["aabb", "tree_build", "tree_traverse", "pair_list", "narrow_closed_gjk", "narrow_hull_gjk", "epa",
+          "contact_compaction"]
+# stages that are one kernel launch each: the roofline candidates
+SINGLE_KERNEL = {"tree_traverse": "k_traverse", "narrow_hull_gjk": "k_np_hull", "epa": "k_epa_wave"}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(sc, pairs, offs, stage):
-    """Algorithmic HBM bytes one launch of `stage` must move.
+def algorithmic_bytes(sc, pairs, offs, routes, kernel):
+    """Algorithmic HBM bytes one launch of `kernel` must move (each datum once).
 
-    k_aabb   per body: pose 28 B + shape record 16 B read, AABB 24 B written,
-             plus 12 B per hull vertex read.
-    k_narrow per pair: pair 8 B + both bodies' pose+shape 2 x 44 B read,
-             count 4 B + 28 B per contact written, plus 12 B per hull
-             vertex of each hull side read once.
+    Body record = pose 28 B + shape 16 B = 44 B; hull vertex = 12 B; contact
+    = position 12 + normal 12 + depth 4 = 28 B.
+    k_traverse  per body: its AABB 24 B read; per pair: 8 B written.
+    k_np_hull   per pair with a hull: pair index 4 + pair 8 + two bodies 88
+                + both hulls' vertices + count 4 + its band contacts 28 each,
+                + 64 B queue entry for a pair sent on to EPA.
+    k_epa_wave  per queued pair: queue entry 64 + pair 8 + two bodies 88 +
+                both hulls' vertices + count 4 + route 8 (read, write) +
+                28 per contact.
     """
-    t = sc["shape_type"]
+    P = pairs.shape[0]
     hc = sc["hull_count"].astype(np.int64)
-    if stage == "k_aabb":
-        return int(t.shape[0] * (28 + 16 + 24) + 12 * int(hc[t == 3].sum()))
-    if stage == "k_narrow":
-        P = pairs.shape[0]
-        Cn = int(offs[-1])
-        hv = int(hc[pairs[:, 0]].sum() + hc[pairs[:, 1]].sum())
-        return int(P * (8 + 88 + 4) + 28 * Cn + 12 * hv)
+    hv = hc[pairs[:, 0]] + hc[pairs[:, 1]]
+    ncon = np.diff(offs.astype(np.int64))
+    cls = routes & 7
+    epa = (routes & 8) != 0
+    if kernel == "k_traverse":
+        return int(sc["shape_type"].shape[0] * 24 + 8 * P)
+    if kernel == "k_np_hull":
+        m = cls == 6
+        return int(m.sum() * (4 + 8 + 88 + 4) + 12 * hv[m].sum() + 28 * ncon[m & ~epa].sum() + 64 * (m & epa).sum())
+    if kernel == "k_epa_wave":
+        return int(epa.sum() * (64 + 8 + 88 + 4 + 8) + 12 * hv[epa].sum() + 28 * ncon[epa].sum())
     return None
 
 
@@ -71,6 +82,42 @@ def cpu_baseline(scenes, oracle_mod, sc, target_bodies, margin):
                 contacts_per_s=Cn / sec, seconds=sec, pairs=P)
 
 
+def batched_vs_plain(bpa, n_worlds, margin, steps):
+    """4096 worlds (16-2048 boxes/spheres + plane each, seed 3): one batched
+    pass against the plain pass run world by world on the same GPU."""
+    t0 = time.time()
+    sc = bpa.scenes.batched_worlds(n_worlds, 3)
+    log(f"batched scene: {n_worlds} worlds, {sc['shape_type'].shape[0]} bodies in {time.time() - t0:.1f} s")
+    wb = bpa.CollisionWorld(sc)
+    Pb, Cb = wb.collide(margin)
+    wb.synchronize()
+    tb = []
+    for _ in range(max(steps, 1)):
+        t1 = time.perf_counter()
+        Pb, Cb = wb.collide(margin)
+        wb.synchronize()
+        tb.append(time.perf_counter() - t1)
+    tb = float(np.median(tb))
+    # plain path: one world at a time through the same context (upload untimed)
+    ctx = wb.ctx
+    tp = 0.0
+    Pp = Cp = 0
+    for w in range(n_worlds):
+        wp = bpa.CollisionWorld(bpa.scenes.world_slice(sc, w), ctx, batched=False)
+        wp.collide(margin)
+        t1 = time.perf_counter()
+        P, Cn = wp.collide(margin)
+        wp.synchronize()
+        tp += time.perf_counter() - t1
+        Pp += P
+        Cp += Cn
+    return {"workload": f"{n_worlds} worlds, 16-2048 boxes/spheres + plane each, {sc['shape_type'].shape[0]} bodies",
+            "pairs": int(Pb), "contacts": int(Cb), "batched_ms": round(tb * 1e3, 3),
+            "plain_per_world_ms": round(tp * 1e3, 3), "batched_pairs_per_s": round(Pb / tb, 1),
+            "plain_pairs_per_s": round(Pp / tp, 1), "speedup": round(tp / tb, 2),
+            "results_equal_counts": bool(Pp == Pb and Cp == Cb)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,7 +127,21 @@ def main():
     ap.add_argument("--margin", type=float, default=0.04)
     ap.add_argument("--cpu-bodies", type=int, default=250_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--worlds", type=int, default=4096)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device-to-device copy three times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 64 * 1024 * 1024
+        x = torch.ones(n, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"copied {4 * n} bytes three times")
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}), flush=True)
@@ -108,18 +169,42 @@ def main():
     pairs, offs, cont = w.results()
     value = P / (ms * 1e-3)
 
+    # the same pass with host buffers on both sides: poses uploaded, results
+    # (pairs, offsets, contacts) copied back -- reported, never `value`
+    pos = np.ascontiguousarray(sc["position"], np.float32)
+    quat = np.ascontiguousarray(sc["orientation"], np.float32)
+    th = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        w.set_poses(pos, quat)
+        w.collide(args.margin)
+        w.results()
+        th.append(time.perf_counter() - t1)
+    host_ms = float(np.median(th)) * 1e3
+    pcie = {"ms_per_step": round(host_ms, 3), "pairs_per_s": round(P / (host_ms * 1e-3), 1),
+            "includes": "pose upload (28 B/body) + pass + pairs, offsets and contacts copied to host"}
+    log(f"host-buffer pass: {host_ms:.2f} ms")
+    routes = w.routes()
     dom = STAGES[int(np.argmax(stage_ms))]
-    roof_stage = dom if dom in ("k_aabb", "k_narrow") else "k_narrow"
-    ab = algorithmic_bytes(sc, pairs, offs, roof_stage)
+    cand = [st for st in SINGLE_KERNEL]
+    roof_stage = max(cand, key=lambda st: stage_ms[STAGES.index(st)])
+    kern = SINGLE_KERNEL[roof_stage]
+    ab = algorithmic_bytes(sc, pairs, offs, routes, kern)
     dur = stage_ms[STAGES.index(roof_stage)] * 1e-3
     achieved = ab / dur / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", f"{roof_stage}_mixed_1m_traffic.json")
+    tf = os.path.join(ROOT, "profiles", f"{kern}_mixed_1m_traffic.json")
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get("bytes_per_launch")
-    roof = {"bound": "hbm", "kernel": roof_stage, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+    roof = {"bound": "hbm", "kernel": kern, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-            "algorithmic_bytes": ab, "kernel_ms": round(dur * 1e3, 4), "dominant_stage": dom}
+            "algorithmic_bytes": ab, "kernel_ms": round(dur * 1e3, 4), "dominant_stage": dom,
+            "routes": {"epa_pairs": int(((routes & 8) != 0).sum()), "hull_pairs": int(((routes & 7) == 6).sum())}}
+
+    batched = None
+    if not args.no_batched:
+        batched = batched_vs_plain(bpa, args.worlds, args.margin, args.steps)
+        log(f"batched: {batched}")
 
     cpu = None
     if not args.no_cpu:
@@ -144,6 +229,8 @@ def main():
         "contacts_per_s": round(Cn / (ms * 1e-3), 1),
         "stage_ms": {s: round(float(v), 4) for s, v in zip(STAGES, stage_ms)},
         "roofline": roof,
+        "host_buffers": pcie,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

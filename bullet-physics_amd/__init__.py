@@ -192,6 +192,19 @@ class BroadPhase:
                                                           out.ctypes.data, out.shape[0], C.byref(cnt)))
         return out[:cnt.value]
 
+    def raycast_candidates(self, ray_from, ray_to) -> np.ndarray:
+        """Live proxies whose AABB the segment crosses (broadphase/mod.rs:119-124)."""
+        f = np.ascontiguousarray(ray_from, np.float32)
+        t = np.ascontiguousarray(ray_to, np.float32)
+        cnt = C.c_uint64()
+        rc = self.L.bch_bp_raycast_candidates(self.ctx.h, f.ctypes.data, t.ctypes.data, None, 0, C.byref(cnt))
+        if rc not in (_lib.BCH_OK, _lib.BCH_OVERFLOW):
+            _check(self.ctx.h, rc)
+        out = np.zeros(max(cnt.value, 1), np.uint32)
+        _check(self.ctx.h, self.L.bch_bp_raycast_candidates(self.ctx.h, f.ctypes.data, t.ctypes.data,
+                                                             out.ctypes.data, out.shape[0], C.byref(cnt)))
+        return out[:cnt.value]
+
 
 # ------------------------------------------------------------- narrowphase --
 def collide_batch(scene: dict, pairs, seeds=None, ctx: _Context | None = None):
@@ -295,8 +308,21 @@ class CollisionWorld:
                                                      cont.ctypes.data))
         return pairs[:self.n_pairs], offs, cont[:self.n_contacts]
 
+    # stage names of timings(), in order
+    STAGES = ("aabb", "tree_build", "tree_traverse", "pair_list", "narrow_closed_gjk", "narrow_hull_gjk", "epa",
+              "contact_compaction")
+
+    def routes(self):
+        """Dispatch route of every pair of the last pass (u8[P]): 0 none,
+        1 sphere-sphere, 2 sphere-box, 3 plane-convex, 4 box-box, 5 GJK
+        without a hull, 6 GJK with a hull; +8 when EPA ran."""
+        out = np.zeros(max(self.n_pairs, 1), np.uint8)
+        _check(self.ctx.h, self.L.bch_world_routes(self.ctx.h, out.ctypes.data))
+        return out[:self.n_pairs]
+
     def timings(self):
-        out = np.zeros(7, np.float32)
+        """Device milliseconds of the last pass per stage (see STAGES)."""
+        out = np.zeros(8, np.float32)
         _check(self.ctx.h, self.L.bch_last_timings(self.ctx.h, out.ctypes.data))
         return out
 

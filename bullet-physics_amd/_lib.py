@@ -17,9 +17,9 @@ BCH_OK, BCH_OVERFLOW, BCH_INVALID_ARG, BCH_HIP_ERROR, BCH_NO_DEVICE = 0, 1, 2, 3
 EXPORTS = [
     "bch_create", "bch_destroy", "bch_last_error", "bch_synchronize", "bch_last_timings",
     "bch_bp_create_proxy", "bch_bp_set_aabb", "bch_bp_destroy_proxy", "bch_bp_proxy_aabb", "bch_bp_set_aabbs",
-    "bch_bp_compute_pairs", "bch_bp_aabb_candidates", "bch_collide_batch",
+    "bch_bp_compute_pairs", "bch_bp_aabb_candidates", "bch_bp_raycast_candidates", "bch_collide_batch",
     "bch_world_set_bodies", "bch_world_set_poses", "bch_world_set_poses_device", "bch_world_collide",
-    "bch_world_results", "bch_world_results_device", "bch_set_tuning", "bch_get_tuning",
+    "bch_world_results", "bch_world_results_device", "bch_world_routes", "bch_set_tuning", "bch_get_tuning",
 ]
 
 # launch-shape knobs (bch_set_tuning)
@@ -64,6 +64,7 @@ def load():
     L.bch_bp_set_aabbs.argtypes = [vp, u32, vp, vp, vp]
     L.bch_bp_compute_pairs.argtypes = [vp, vp, u64, C.POINTER(u64)]
     L.bch_bp_aabb_candidates.argtypes = [vp, vp, vp, vp, u64, C.POINTER(u64)]
+    L.bch_bp_raycast_candidates.argtypes = [vp, vp, vp, vp, u64, C.POINTER(u64)]
     L.bch_collide_batch.argtypes = [vp, C.POINTER(BchBodies), vp, u64, vp, vp, vp]
     L.bch_world_set_bodies.argtypes = [vp, C.POINTER(BchBodies), u32, vp]
     L.bch_world_set_poses.argtypes = [vp, vp, vp]
@@ -71,6 +72,7 @@ def load():
     L.bch_world_collide.argtypes = [vp, f32, C.POINTER(u64), C.POINTER(u64)]
     L.bch_world_results.argtypes = [vp, vp, vp, vp]
     L.bch_world_results_device.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
+    L.bch_world_routes.argtypes = [vp, vp]
     L.bch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.bch_get_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     _lib = L

@@ -1,20 +1,27 @@
-// bch_epa_wave.hpp -- EPA with a group of G lanes per pair (G = 16, 32 or
-// 64), polytope in LDS.
+// bch_epa_wave.hpp -- EPA with one wavefront per pair, polytope in LDS.
 //
 // Same algorithm, statement order and tie rules as the reference
 // (narrowphase/epa.rs:127-398): tetrahedron seed + six axis supports,
 // closest face by edge-aware distance (first minimum), visibility
 // n.p - d > 1e-5, transactional two-phase expansion, <= 255 iterations,
-// <= 128 vertices.  The G lanes of a group share the work that has no
-// order: face scans (closest face, visibility), hull support scans
-// (largest projection, lowest index = the sequential first maximum), face
-// compaction (ballot prefix counts keep the survivors' order) and the new
-// faces of the horizon fan.  The horizon is stitched in visible-face order
-// with the reference's swap-remove edge cancellation on a register-resident
-// edge list, so faces are created in exactly the reference order.
+// <= 128 vertices.
 //
-// A wavefront runs 64 / G pairs side by side: every instruction is issued
-// for the whole wavefront, so small groups cut the issue cost per pair.
+// The 64 lanes share the work that has no order:
+//  * hull supports: lane l holds hull vertices l and l + 64 in registers;
+//    a DPP max over the projections, then a ballot of the lanes that reach
+//    it, gives the lowest index with the largest projection = the
+//    sequential first maximum (convex_hull.rs:55-66);
+//  * closest face: the same with a min over |distance| (epa.rs:212-224);
+//  * visibility, compaction of the surviving faces (ballot prefix counts
+//    keep their order) and the faces of the horizon fan (one per lane).
+// The horizon is the reference's swap-remove edge list, replayed exactly:
+// every edge of the visible faces first finds, in parallel, the earlier
+// edge that is its reverse; a scalar replay of the pushes and swap-removes
+// over a lane-indexed list (v_readlane / lane selects, a few
+// instructions per edge) then yields the reference's horizon order.  A
+// polytope outside that fast path (more than 64 faces, more than 21 visible
+// faces, or a directed edge shared by two visible faces) runs the general
+// stitch, which searches the list with one ballot per 64 entries.
 #pragma once
 #include "bch_kernels.hpp"
 
@@ -22,6 +29,7 @@ namespace bch {
 
 #define BCH_WEPA_V 128
 #define BCH_WEPA_F 256
+#define BCH_WEPA_R 4  // horizon registers per lane in the general stitch (256 edges)
 
 struct EpaLds {
   float4 v[BCH_WEPA_V];
@@ -29,136 +37,130 @@ struct EpaLds {
   uint32_t abc[BCH_WEPA_F];
 };
 
-// Order LDS traffic between lanes of one wavefront (no cross-wave barrier:
-// groups run their own pairs).
+// Order LDS traffic between lanes of one wavefront.
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
 
-// Lane-group view of a wavefront: G lanes per pair, group `grp` owns lanes
-// [grp * G, grp * G + G).
-template <int G>
-struct Grp {
-  int sub;   // lane within the group
-  int base;  // first wavefront lane of the group
-  __device__ __forceinline__ uint64_t ballot(bool p) const {
-    uint64_t m = __ballot(p);
-    return G == 64 ? m : (m >> base) & ((1ull << G) - 1ull);
-  }
-  __device__ __forceinline__ uint64_t below() const { return sub ? ((1ull << sub) - 1ull) : 0ull; }
-  // value of `v` on group lane k (k uniform within the group); a whole
-  // wavefront reads it with v_readlane, smaller groups with a permute.
-  __device__ __forceinline__ uint32_t bcast(uint32_t v, int k) const {
-    if (G == 64) return (uint32_t)__builtin_amdgcn_readlane((int)v, __builtin_amdgcn_readfirstlane(k));
-    return (uint32_t)__shfl((int)v, base + k, 64);
-  }
-  __device__ __forceinline__ float bcastf(float v, int k) const { return __uint_as_float(bcast(__float_as_uint(v), k)); }
-};
+__device__ __forceinline__ int rdl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ uint32_t rdl_u(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
+__device__ __forceinline__ float rdl_f(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+// lane l of `old` replaced by the uniform value `val` (v_cmp + v_cndmask)
+__device__ __forceinline__ int wrl_i(int val, int l, int old, int lane) { return lane == l ? val : old; }
+__device__ __forceinline__ uint64_t lanes_below(int lane) { return lane ? (~0ull >> (64 - lane)) : 0ull; }
+__device__ __forceinline__ int ctz64(uint64_t m) { return (int)__builtin_ctzll(m); }
 
-// (max value, lowest index) or (min value, lowest index) over the group.
-template <bool MAX, int G>
-__device__ __forceinline__ void group_reduce(float& v, int& i) {
-  if (G == 64) {
-    wave_reduce<MAX>(v, i);
-    return;
-  }
-  row_reduce<MAX, 16>(v, i);
-  if (G == 32) {
-    float ov = __shfl_xor(v, 16, 64);
-    int oi = __shfl_xor(i, 16, 64);
-    if (MAX) amax_merge(v, i, ov, oi); else amin_merge(v, i, ov, oi);
-  }
+// Wavefront max / min of values without NaN (uniform result).  The DPP
+// row steps leave each row's extremum in every lane of the row; the four
+// rows are combined from lanes 0, 16, 32 and 48.
+template <bool MAX>
+__device__ __forceinline__ float wave_ext_f(float v) {
+  float o;
+  o = dpp_f<0xB1>(v);
+  v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+  o = dpp_f<0x4E>(v);
+  v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+  o = dpp_f<0x141>(v);
+  v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+  o = dpp_f<0x140>(v);
+  v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+  float a = rdl_f(v, 0), b = rdl_f(v, 16), c = rdl_f(v, 32), d = rdl_f(v, 48);
+  a = MAX ? (b > a ? b : a) : (b < a ? b : a);
+  c = MAX ? (d > c ? d : c) : (d < c ? d : c);
+  return MAX ? (c > a ? c : a) : (c < a ? c : a);
 }
 
-template <int G>
-__device__ __forceinline__ V3 grp_hull_support(const float4* h, int n, V3 d, const Grp<G>& g) {
+__device__ __forceinline__ int wave_min_i(int v) {
+  int o;
+  o = dpp_i<0xB1>(v);
+  v = o < v ? o : v;
+  o = dpp_i<0x4E>(v);
+  v = o < v ? o : v;
+  o = dpp_i<0x141>(v);
+  v = o < v ? o : v;
+  o = dpp_i<0x140>(v);
+  v = o < v ? o : v;
+  int a = rdl_i(v, 0), b = rdl_i(v, 16), c = rdl_i(v, 32), d = rdl_i(v, 48);
+  a = b < a ? b : a;
+  c = d < c ? d : c;
+  return c < a ? c : a;
+}
+
+// convex_hull.rs:55-66 over a hull in global memory (more vertices than the
+// registers hold): per-lane first maximum, then the lowest index among the
+// lanes that reach the wavefront maximum.
+__device__ __forceinline__ V3 wave_hull_support_mem(const float4* h, int n, V3 d, int lane) {
   float bv = -INFINITY;
-  int bi = -1;
-  for (int v = g.sub; v < n; v += G) {
+  int bi = 0x7fffffff;
+  for (int v = lane; v < n; v += 64) {
     float dd = dot(hull_vertex(h, v), d);
     if (dd > bv) {
       bv = dd;
       bi = v;
     }
   }
-  group_reduce<true, G>(bv, bi);
-  return bi >= 0 ? hull_vertex(h, bi) : splat(0.0f);
+  float m = wave_ext_f<true>(bv);
+  if (!(m > -INFINITY)) return splat(0.0f);
+  int idx = wave_min_i(bv == m ? bi : 0x7fffffff);
+  return hull_vertex(h, idx);
 }
 
-// Hull vertices of one shape held in registers: group lane `sub` keeps
-// vertices sub, sub + G, ... (NR of them).  Hulls with more vertices than
-// the group holds scan global memory instead.
-template <int G>
+// Hull vertices held in registers: lane l keeps vertices l and l + 64.
 struct RegHull {
-  static constexpr int NR = 128 / G;
-  float x[NR], y[NR], z[NR];
+  float x0, y0, z0, x1, y1, z1;
   const float4* h;
   int n;
   bool inreg;
-  __device__ __forceinline__ void load(const ShapeRef& s, const Grp<G>& g) {
+  __device__ __forceinline__ void load(const ShapeRef& s, int lane) {
     n = s.type == HULL ? s.nhull : 0;
     h = s.hull;
-    inreg = n <= G * NR;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      int v = g.sub + G * r;
-      float4 t = (inreg && v < n) ? h[v] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      x[r] = t.x;
-      y[r] = t.y;
-      z[r] = t.z;
-    }
+    inreg = n <= 128;
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a;
+    if (inreg && lane < n) a = h[lane];
+    if (inreg && lane + 64 < n) b = h[lane + 64];
+    x0 = a.x;
+    y0 = a.y;
+    z0 = a.z;
+    x1 = b.x;
+    y1 = b.y;
+    z1 = b.z;
   }
-  // convex_hull.rs:55-66: first maximiser of p.d over the vertices.
-  __device__ __forceinline__ V3 support(V3 d, const Grp<G>& g) const {
-    if (!inreg) return grp_hull_support<G>(h, n, d, g);
-    float bv = -INFINITY;
-    int bi = -1;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      int v = g.sub + G * r;
-      if (v < n) {
-        float dd = dot(V3{x[r], y[r], z[r]}, d);
-        if (dd > bv) {
-          bv = dd;
-          bi = v;
-        }
-      }
+  // convex_hull.rs:55-66: first maximiser of p.d (NaN projections never win).
+  __device__ __forceinline__ V3 support(V3 d, int lane) const {
+    if (!inreg) return wave_hull_support_mem(h, n, d, lane);
+    float a = dot(V3{x0, y0, z0}, d), b = dot(V3{x1, y1, z1}, d);
+    a = (lane < n && a > -INFINITY) ? a : -INFINITY;
+    b = (lane + 64 < n && b > -INFINITY) ? b : -INFINITY;
+    float m = wave_ext_f<true>(b > a ? b : a);
+    if (!(m > -INFINITY)) return splat(0.0f);
+    uint64_t ba = __ballot(a == m);
+    if (ba) {
+      int l = ctz64(ba);
+      return V3{rdl_f(x0, l), rdl_f(y0, l), rdl_f(z0, l)};
     }
-    group_reduce<true, G>(bv, bi);
-    if (bi < 0) return splat(0.0f);
-    int rr = bi / G, ll = bi % G;
-    float sx = x[0], sy = y[0], sz = z[0];
-#pragma unroll
-    for (int r = 1; r < NR; ++r) {
-      if (rr == r) {
-        sx = x[r];
-        sy = y[r];
-        sz = z[r];
-      }
-    }
-    return V3{g.bcastf(sx, ll), g.bcastf(sy, ll), g.bcastf(sz, ll)};
+    int l = ctz64(__ballot(b == m));
+    return V3{rdl_f(x1, l), rdl_f(y1, l), rdl_f(z1, l)};
   }
 };
 
-template <int G>
-__device__ __forceinline__ V3 grp_support_world(const ShapeRef& s, const RegHull<G>& rh, V3 p, Q4 q, V3 d,
-                                                const Grp<G>& g, bool core) {
+struct EpaPair {
+  Pair pr;
+  RegHull ha, hb;
+};
+
+__device__ __forceinline__ V3 wave_support_world(const ShapeRef& s, const RegHull& rh, V3 p, Q4 q, V3 d, int lane,
+                                                 bool core) {
   V3 ld = qrot(qconj(q), d);
-  V3 l = s.type == HULL ? rh.support(ld, g) : (core ? core_support_local(s, ld) : support_local(s, ld));
+  V3 l = s.type == HULL ? rh.support(ld, lane) : (core ? core_support_local(s, ld) : support_local(s, ld));
   return qrot(q, l) + p;
 }
 
-template <int G>
-struct EpaPair {
-  Pair pr;
-  RegHull<G> ha, hb;
-};
-
-template <int G>
-__device__ __forceinline__ V3 grp_minkowski(const EpaPair<G>& ep, V3 d, const Grp<G>& g) {
-  V3 a = grp_support_world<G>(ep.pr.a, ep.ha, ep.pr.pa, ep.pr.qa, d, g, true);
-  V3 b = grp_support_world<G>(ep.pr.b, ep.hb, ep.pr.pb, ep.pr.qb, -d, g, true);
+__device__ __forceinline__ V3 wave_minkowski(const EpaPair& ep, V3 d, int lane) {
+  V3 a = wave_support_world(ep.pr.a, ep.ha, ep.pr.pa, ep.pr.qa, d, lane, true);
+  V3 b = wave_support_world(ep.pr.b, ep.hb, ep.pr.pb, ep.pr.qb, -d, lane, true);
   return a - b;
 }
 
@@ -168,60 +170,78 @@ __device__ __forceinline__ V3 lv(const EpaLds& S, int i) {
 }
 
 // epa.rs:212-224: index of the first face with the smallest |distance|
-// (a NaN first face is never replaced).
-template <int G>
-__device__ __forceinline__ int grp_best(const EpaLds& S, int nfaces, const Grp<G>& g) {
+// (NaN distances are skipped; a NaN first face is never replaced).
+__device__ __forceinline__ int wave_best(const EpaLds& S, int nfaces, int lane) {
+  if (is_nan(S.dist[0])) return 0;
+  if (nfaces <= 64) {
+    float v = INFINITY;
+    if (lane < nfaces) {
+      float d = fabsf(S.dist[lane]);
+      v = is_nan(d) ? INFINITY : d;
+    }
+    float m = wave_ext_f<false>(v);
+    if (!(m < INFINITY)) return 0;
+    return ctz64(__ballot(v == m));
+  }
   float bv = INFINITY;
-  int bi = -1;
-  for (int f = g.sub; f < nfaces; f += G) {
-    float v = fabsf(S.dist[f]);
-    if (!is_nan(v) && (bi < 0 || v < bv)) {
-      bv = v;
+  int bi = 0x7fffffff;
+  for (int f = lane; f < nfaces; f += 64) {
+    float d = fabsf(S.dist[f]);
+    if (d < bv) {
+      bv = d;
       bi = f;
     }
   }
-  group_reduce<false, G>(bv, bi);
-  if (bi < 0 || is_nan(S.dist[0])) bi = 0;
-  return bi;
+  float m = wave_ext_f<false>(bv);
+  if (!(m < INFINITY)) return 0;
+  return wave_min_i(bv == m ? bi : 0x7fffffff);
 }
-
-// Horizon edge list capacity in registers: R entries per lane (G * R edges).
-template <int G>
-struct HorizonCap {
-  static constexpr int R = G >= 64 ? 4 : (G >= 32 ? 8 : 8);
-};
 
 __device__ __forceinline__ bool face_vis_lds(const EpaLds& S, int f, V3 np) {
   return dot(V3{S.nx[f], S.ny[f], S.nz[f]}, np) - S.pd[f] > 1e-5f;
 }
 
-// epa.rs:309-388.  Returns 1 ok (or nothing visible), 0 degenerate horizon
-// (hull untouched), -1 capacity exceeded (hull untouched).
-template <int G>
-__device__ __forceinline__ int grp_expand(EpaLds& S, int& nfaces, V3 np, int nidx, const Grp<G>& g) {
-  constexpr int R = HorizonCap<G>::R;
-  const int nchunks = (nfaces + G - 1) / G;
+// The fan face of horizon edge (u, w) and the new vertex (epa.rs:372-386).
+__device__ __forceinline__ void fan_face(EpaLds& S, int d, uint32_t u, uint32_t w, V3 vu, V3 vw, V3 np,
+                                         int nidx) {
+  V3 cr = cross(vw - vu, np - vu);
+  float l = len(cr);
+  V3 n = cr / l;
+  S.nx[d] = n.x;
+  S.ny[d] = n.y;
+  S.nz[d] = n.z;
+  S.dist[d] = face_dist(n, vu, vw, np);
+  S.pd[d] = dot(n, vu);
+  S.abc[d] = u | (w << 8) | ((uint32_t)nidx << 16);
+}
+
+// epa.rs:309-388, general form (any face count).  Returns 1 ok (or nothing
+// visible), 0 degenerate horizon (hull untouched), -1 store overflow (hull
+// untouched).
+__device__ __noinline__ int wave_expand_general(EpaLds& S, int& nfaces, V3 np, int nidx, int lane) {
+  constexpr int R = BCH_WEPA_R;
+  const int nchunks = (nfaces + 63) / 64;
   // Horizon in visible-face order with swap-remove cancellation; entry k of
-  // the edge list is held by group lane k % G in register bl[k / G].
+  // the edge list is held by lane k % 64 in register bl[k / 64].
   uint32_t bl[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) bl[k] = 0xffffffffu;
   int ne = 0;
   bool over = false, any = false;
   for (int c = 0; c < nchunks; ++c) {
-    int f = g.sub + G * c;
+    int f = lane + 64 * c;
     bool v = false;
     uint32_t ab = 0u;
     if (f < nfaces) {
       v = face_vis_lds(S, f, np);
       ab = S.abc[f];
     }
-    uint64_t m = g.ballot(v);
+    uint64_t m = __ballot(v);
     any = any || m != 0ull;
     while (m) {
-      int b = __builtin_ctzll(m);
+      int b = ctz64(m);
       m &= m - 1ull;
-      uint32_t t = g.bcast(ab, b);
+      uint32_t t = rdl_u(ab, b);
       uint32_t fa = t & 0xffu, fb = (t >> 8) & 0xffu, fc = (t >> 16) & 0xffu;
       for (int e = 0; e < 3; ++e) {
         uint32_t u = e == 0 ? fa : (e == 1 ? fb : fc);
@@ -230,9 +250,9 @@ __device__ __forceinline__ int grp_expand(EpaLds& S, int& nfaces, V3 np, int nid
         int found = -1;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-          if (found < 0 && G * k < ne) {
-            uint64_t h = g.ballot(g.sub + G * k < ne && bl[k] == rev);
-            if (h) found = G * k + __builtin_ctzll(h);
+          if (found < 0 && 64 * k < ne) {
+            uint64_t h = __ballot(lane + 64 * k < ne && bl[k] == rev);
+            if (h) found = 64 * k + ctz64(h);
           }
         }
         if (found >= 0) {
@@ -240,15 +260,15 @@ __device__ __forceinline__ int grp_expand(EpaLds& S, int& nfaces, V3 np, int nid
           uint32_t lastv = 0;
 #pragma unroll
           for (int k = 0; k < R; ++k)
-            if (last / G == k) lastv = g.bcast(bl[k], last % G);
+            if (last / 64 == k) lastv = rdl_u(bl[k], last % 64);
 #pragma unroll
           for (int k = 0; k < R; ++k)
-            if (found / G == k && g.sub == found % G) bl[k] = lastv;
+            if (found / 64 == k && lane == found % 64) bl[k] = lastv;
           --ne;
-        } else if (ne < G * R) {
+        } else if (ne < 64 * R) {
 #pragma unroll
           for (int k = 0; k < R; ++k)
-            if (ne / G == k && g.sub == ne % G) bl[k] = u | (w << 8);
+            if (ne / 64 == k && lane == ne % 64) bl[k] = u | (w << 8);
           ++ne;
         } else {
           over = true;
@@ -262,31 +282,31 @@ __device__ __forceinline__ int grp_expand(EpaLds& S, int& nfaces, V3 np, int nid
   bool bad = false;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
-    if (g.sub + G * k < ne) {
+    if (lane + 64 * k < ne) {
       uint32_t e = bl[k];
       V3 vu = lv(S, e & 0xffu);
       if (len(cross(lv(S, (e >> 8) & 0xffu) - vu, np - vu)) < 1e-12f) bad = true;
     }
   }
-  if (g.ballot(bad)) return 0;
+  if (__ballot(bad)) return 0;
   // Count the survivors first: a hull that would outgrow the store is left
   // untouched.
   int kept = 0;
   for (int c = 0; c < nchunks; ++c) {
-    int f = g.sub + G * c;
+    int f = lane + 64 * c;
     bool live = f < nfaces;
-    kept += __popcll(g.ballot(live && !face_vis_lds(S, live ? f : 0, np)));
+    kept += __popcll(__ballot(live && !face_vis_lds(S, live ? f : 0, np)));
   }
   if (kept + ne > BCH_WEPA_F) return -1;
   // Commit: stable compaction of the faces that stay (visibility re-evaluated
   // on the same values, as the reference does) ...
   int outp = 0;
   for (int c = 0; c < nchunks; ++c) {
-    int f = g.sub + G * c;
+    int f = lane + 64 * c;
     bool live = f < nfaces;
-    uint64_t keepm = g.ballot(live && !face_vis_lds(S, live ? f : 0, np));
-    bool keep = (keepm >> g.sub) & 1ull;
-    int d = outp + __popcll(keepm & g.below());
+    uint64_t keepm = __ballot(live && !face_vis_lds(S, live ? f : 0, np));
+    bool keep = (keepm >> lane) & 1ull;
+    int d = outp + __popcll(keepm & lanes_below(lane));
     outp += __popcll(keepm);
     float fnx = 0, fny = 0, fnz = 0, fd = 0, fpd = 0;
     uint32_t fabc = 0;
@@ -312,48 +332,136 @@ __device__ __forceinline__ int grp_expand(EpaLds& S, int& nfaces, V3 np, int nid
   // ... then the horizon fan, in horizon order (entry k built by its lane).
 #pragma unroll
   for (int kk = 0; kk < R; ++kk) {
-    int k = g.sub + G * kk;
+    int k = lane + 64 * kk;
     if (k >= ne) continue;
     uint32_t e = bl[kk];
     uint32_t u = e & 0xffu, w = (e >> 8) & 0xffu;
-    V3 vu = lv(S, u), vw = lv(S, w);
-    V3 cr = cross(vw - vu, np - vu);
-    float l = len(cr);
-    V3 n = cr / l;
-    int d = kept + k;
-    S.nx[d] = n.x;
-    S.ny[d] = n.y;
-    S.nz[d] = n.z;
-    S.dist[d] = face_dist(n, vu, vw, np);
-    S.pd[d] = dot(n, vu);
-    S.abc[d] = u | (w << 8) | ((uint32_t)nidx << 16);
+    fan_face(S, kept + k, u, w, lv(S, u), lv(S, w), np, nidx);
   }
   nfaces = kept + ne;
   wave_sync();
   return 1;
 }
 
-// epa.rs:127-205 for one pair; every lane of the group returns the result.
-template <int G>
-__device__ __forceinline__ EpaResult grp_epa(const EpaPair<G>& pr, V3 t0, V3 t1, V3 t2, V3 t3, EpaLds& S, const Grp<G>& g) {
-  EpaResult R;
-  int sub = g.sub;
-  if (sub < 4) {
-    V3 w = sel(sub == 0, t0, sel(sub == 1, t1, sel(sub == 2, t2, t3)));
-    S.v[sub] = make_float4(w.x, w.y, w.z, 0.0f);
+// epa.rs:309-388.  Same contract as wave_expand_general.
+__device__ __forceinline__ int wave_expand(EpaLds& S, int& nfaces, V3 np, int nidx, int lane) {
+  if (nfaces > 64) return wave_expand_general(S, nfaces, np, nidx, lane);
+  const bool live = lane < nfaces;
+  float fnx = 0.0f, fny = 0.0f, fnz = 0.0f, fd = 0.0f, fpd = 0.0f;
+  uint32_t fabc = 0u;
+  if (live) {
+    fnx = S.nx[lane];
+    fny = S.ny[lane];
+    fnz = S.nz[lane];
+    fd = S.dist[lane];
+    fpd = S.pd[lane];
+    fabc = S.abc[lane];
   }
-  // initial faces (epa.rs:226-257): group lane q builds quad q; survivors
-  // keep quad order.  Quads (a, b, c, opposite): (0,1,2,3) (0,3,1,2)
-  // (0,2,3,1) (1,3,2,0).
+  const bool vis = live && dot(V3{fnx, fny, fnz}, np) - fpd > 1e-5f;
+  const uint64_t vm = __ballot(vis);
+  if (!vm) return 1;
+  const int nv = __popcll(vm);
+  if (nv > 21) return wave_expand_general(S, nfaces, np, nidx, lane);
+  // Edge events: lane l < 3 nv holds edge l % 3 of the visible face of rank
+  // l / 3, in the reference's visiting order.
+  const int nev = 3 * nv;
+  const int rank = lane / 3;
+  const int e = lane - 3 * rank;
+  uint32_t eabc = 0u;
+  {
+    uint64_t mm = vm;
+    for (int r = 0; r < nv; ++r) {
+      int b = ctz64(mm);
+      mm &= mm - 1ull;
+      uint32_t t = rdl_u(fabc, b);
+      if (rank == r) eabc = t;
+    }
+  }
+  const uint32_t fa = eabc & 0xffu, fb = (eabc >> 8) & 0xffu, fc = (eabc >> 16) & 0xffu;
+  const uint32_t u = e == 0 ? fa : (e == 1 ? fb : fc);
+  const uint32_t w = e == 0 ? fb : (e == 1 ? fc : fa);
+  const uint32_t edge = u | (w << 8), rev = w | (u << 8);
+  const bool is_ev = lane < nev;
+  // The earlier event that is this edge's reverse (it is then still on the
+  // list, so this event cancels it); any repeated directed edge leaves the
+  // fast path.
+  int match = -1;
+  bool dup = false;
+  for (int k = 0; k < nev; ++k) {
+    uint32_t ek = rdl_u(edge, k);
+    if (ek == rev && k < lane) match = k;
+    if (ek == edge && k != lane) dup = true;
+  }
+  if (__ballot(is_ev && dup)) return wave_expand_general(S, nfaces, np, nidx, lane);
+  // Replay of the pushes and swap-removes: lane p of `list` holds the event
+  // at list position p, lane v of `pos` the position of event v.
+  const uint64_t cancel = __ballot(is_ev && match >= 0);
+  int list = 0, pos = 0, ne = 0;
+  for (int l = 0; l < nev; ++l) {
+    if ((cancel >> l) & 1ull) {
+      int p = rdl_i(pos, rdl_i(match, l));
+      int last = rdl_i(list, ne - 1);
+      list = wrl_i(last, p, list, lane);
+      pos = wrl_i(p, last, pos, lane);
+      --ne;
+    } else {
+      list = wrl_i(l, ne, list, lane);
+      pos = wrl_i(ne, l, pos, lane);
+      ++ne;
+    }
+  }
+  // Horizon entry k (lane k): the edge of the event at list position k.
+  const bool is_h = lane < ne;
+  const uint32_t hedge = (uint32_t)__shfl((int)edge, is_h ? list : 0, 64);
+  const uint32_t hu = hedge & 0xffu, hw = (hedge >> 8) & 0xffu;
+  V3 vu = splat(0.0f), vw = splat(0.0f);
+  bool bad = false;
+  if (is_h) {
+    vu = lv(S, hu);
+    vw = lv(S, hw);
+    bad = len(cross(vw - vu, np - vu)) < 1e-12f;
+  }
+  if (__ballot(bad)) return 0;
+  // Commit: stable compaction of the faces that stay, then the fan (at most
+  // 64 + 63 faces, within the store).
+  const uint64_t keepm = __ballot(live && !vis);
+  const int kept = __popcll(keepm);
+  wave_sync();
+  if (live && !vis) {
+    int d = __popcll(keepm & lanes_below(lane));
+    S.nx[d] = fnx;
+    S.ny[d] = fny;
+    S.nz[d] = fnz;
+    S.dist[d] = fd;
+    S.pd[d] = fpd;
+    S.abc[d] = fabc;
+  }
+  wave_sync();
+  if (is_h) fan_face(S, kept + lane, hu, hw, vu, vw, np, nidx);
+  nfaces = kept + ne;
+  wave_sync();
+  return 1;
+}
+
+// epa.rs:127-205 for one pair; every lane returns the result.
+__device__ __forceinline__ EpaResult wave_epa(const EpaPair& ep, V3 t0, V3 t1, V3 t2, V3 t3, EpaLds& S, int lane) {
+  EpaResult R;
+  if (lane < 4) {
+    V3 w = sel(lane == 0, t0, sel(lane == 1, t1, sel(lane == 2, t2, t3)));
+    S.v[lane] = make_float4(w.x, w.y, w.z, 0.0f);
+  }
+  // initial faces (epa.rs:226-257): lane q builds quad q; survivors keep
+  // quad order.  Quads (a, b, c, opposite): (0,1,2,3) (0,3,1,2) (0,2,3,1)
+  // (1,3,2,0).
   bool valid = false;
   V3 qn = splat(0.0f);
   float qd = 0.0f, qpd = 0.0f;
   uint32_t qabc = 0;
-  if (sub < 4) {
-    int a = sub == 3 ? 1 : 0;
-    int b = sub == 0 ? 1 : (sub == 2 ? 2 : 3);
-    int c = sub == 0 ? 2 : (sub == 1 ? 1 : (sub == 2 ? 3 : 2));
-    int opp = 3 - sub;
+  if (lane < 4) {
+    int a = lane == 3 ? 1 : 0;
+    int b = lane == 0 ? 1 : (lane == 2 ? 2 : 3);
+    int c = lane == 0 ? 2 : (lane == 1 ? 1 : (lane == 2 ? 3 : 2));
+    int opp = 3 - lane;
     V3 sa = sel(a == 0, t0, t1);
     V3 sb = sel(b == 1, t1, sel(b == 2, t2, t3));
     V3 sc = sel(c == 1, t1, sel(c == 2, t2, t3));
@@ -378,10 +486,10 @@ __device__ __forceinline__ EpaResult grp_epa(const EpaPair<G>& pr, V3 t0, V3 t1,
       qabc = (uint32_t)a | ((uint32_t)b << 8) | ((uint32_t)c << 16);
     }
   }
-  uint64_t vm = g.ballot(valid);
+  uint64_t vm = __ballot(valid);
   int nfaces = __popcll(vm);
   if (valid) {
-    int d = __popcll(vm & g.below());
+    int d = __popcll(vm & lanes_below(lane));
     S.nx[d] = qn.x;
     S.ny[d] = qn.y;
     S.nz[d] = qn.z;
@@ -402,12 +510,12 @@ __device__ __forceinline__ EpaResult grp_epa(const EpaPair<G>& pr, V3 t0, V3 t1,
   for (int k = 0; k < 6; ++k) {  // seeding; a degenerate expansion only ends the seeding
     V3 d = k == 0 ? V3{1.0f, 0.0f, 0.0f} : k == 1 ? V3{-1.0f, 0.0f, 0.0f} : k == 2 ? V3{0.0f, 1.0f, 0.0f}
          : k == 3 ? V3{0.0f, -1.0f, 0.0f} : k == 4 ? V3{0.0f, 0.0f, 1.0f} : V3{0.0f, 0.0f, -1.0f};
-    V3 p = grp_minkowski<G>(pr, d, g);
+    V3 p = wave_minkowski(ep, d, lane);
     int idx = nverts;
-    if (sub == 0) S.v[idx] = make_float4(p.x, p.y, p.z, 0.0f);
+    if (lane == 0) S.v[idx] = make_float4(p.x, p.y, p.z, 0.0f);
     ++nverts;
     wave_sync();
-    int e = grp_expand<G>(S, nfaces, p, idx, g);
+    int e = wave_expand(S, nfaces, p, idx, lane);
     if (e <= 0) {
       status = e == 0 ? 1 : 4;
       stop = e < 0;
@@ -416,10 +524,10 @@ __device__ __forceinline__ EpaResult grp_epa(const EpaPair<G>& pr, V3 t0, V3 t1,
   }
   if (!stop) {
     for (int it = 0; it < 255; ++it) {
-      int bi = grp_best<G>(S, nfaces, g);
+      int bi = wave_best(S, nfaces, lane);
       V3 bn{S.nx[bi], S.ny[bi], S.nz[bi]};
       float bd = S.dist[bi];
-      V3 np = grp_minkowski<G>(pr, bn, g);
+      V3 np = wave_minkowski(ep, bn, lane);
       float wdist = dot(bn, np) - bd;
       if (wdist <= 1e-4f) {
         R.status = 0;
@@ -432,74 +540,66 @@ __device__ __forceinline__ EpaResult grp_epa(const EpaPair<G>& pr, V3 t0, V3 t1,
         break;
       }
       int nidx = nverts;
-      if (sub == 0) S.v[nidx] = make_float4(np.x, np.y, np.z, 0.0f);
+      if (lane == 0) S.v[nidx] = make_float4(np.x, np.y, np.z, 0.0f);
       ++nverts;
       wave_sync();
-      int e = grp_expand<G>(S, nfaces, np, nidx, g);
+      int e = wave_expand(S, nfaces, np, nidx, lane);
       if (e <= 0) {
         status = e == 0 ? 1 : 4;
         break;
       }
     }
   }
-  int bi = grp_best<G>(S, nfaces, g);
+  int bi = wave_best(S, nfaces, lane);
   R.status = status;
   R.n = V3{S.nx[bi], S.ny[bi], S.nz[bi]};
   R.depth = S.dist[bi];
   return R;
 }
 
-// Grid-stride over the EPA queue: one wavefront per block, 64 / G pairs per
-// wavefront, one LDS polytope per group.
-template <int G>
-__global__ __launch_bounds__(64) void k_epa_grp(const uint32_t* __restrict__ n_queue, const float4* __restrict__ epa_q,
-                                               const uint32_t* __restrict__ pairs, const BodyRec* __restrict__ bodies,
-                                               const float4* __restrict__ hull, uint32_t* __restrict__ ccount,
-                                               float4* __restrict__ cslots, Counters* cnt) {
-  constexpr int NG = 64 / G;
-  __shared__ EpaLds lds[NG];
-  int lane = threadIdx.x & 63;
-  int grp = lane / G;
-  Grp<G> g{lane % G, grp * G};
-  EpaLds& S = lds[grp];
-  uint32_t nq = *n_queue;
-  for (uint32_t q0 = blockIdx.x * NG; q0 < nq; q0 += gridDim.x * NG) {
-    uint32_t q = q0 + grp;
-    if (q < nq) {
-      float4 a = epa_q[4 * (uint64_t)q], b = epa_q[4 * (uint64_t)q + 1], c = epa_q[4 * (uint64_t)q + 2],
-             d = epa_q[4 * (uint64_t)q + 3];
-      uint32_t p = f2u(a.w);
-      uint32_t i = pairs[2 * p], j = pairs[2 * p + 1];
-      BodyRec A = bodies[i], B = bodies[j];
-      EpaPair<G> ep;
-      Pair& pr = ep.pr;
-      pr.a = shape_of(A, hull);
-      pr.b = shape_of(B, hull);
-      pr.pa = pos_of(A);
-      pr.pb = pos_of(B);
-      pr.qa = quat_of(A);
-      pr.qb = quat_of(B);
-      ep.ha.load(pr.a, g);
-      ep.hb.load(pr.b, g);
-      EpaResult e = grp_epa<G>(ep, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, V3{d.x, d.y, d.z}, S, g);
-      {
-        // convex_convex.rs:117-154
-        bool hit = !(e.depth <= 0.0f);
-        V3 point_a = splat(0.0f);
-        float depth = 0.0f;
-        if (hit) {
-          depth = (e.depth + pr.a.margin) + pr.b.margin;
-          point_a = grp_support_world<G>(pr.a, ep.ha, pr.pa, pr.qa, e.n, g, false);
-        }
-        if (g.sub == 0) {
-          if (e.status == 4) atomicOr(&cnt->errors, 2u);
-          ccount[p] = hit ? 1u : 0u;
-          if (hit) {
-            V3 pos = point_a - e.n * depth;
-            cslots[(uint64_t)p * 8] = make_float4(pos.x, pos.y, pos.z, depth);
-            cslots[(uint64_t)p * 8 + 1] = make_float4(e.n.x, e.n.y, e.n.z, 0.0f);
-          }
-        }
+// Grid-stride over the EPA queue: one wavefront per block and per pair,
+// one LDS polytope per wavefront.
+__global__ __launch_bounds__(64) void k_epa_wave(const uint32_t* __restrict__ n_queue,
+                                                 const float4* __restrict__ epa_q, const uint32_t* __restrict__ pairs,
+                                                 const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
+                                                 uint32_t* __restrict__ ccount, float4* __restrict__ cslots,
+                                                 uint32_t* __restrict__ route, Counters* cnt) {
+  __shared__ EpaLds S;
+  const int lane = threadIdx.x & 63;
+  const uint32_t nq = *n_queue;
+  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+    float4 a = epa_q[4 * (uint64_t)q], b = epa_q[4 * (uint64_t)q + 1], c = epa_q[4 * (uint64_t)q + 2],
+           d = epa_q[4 * (uint64_t)q + 3];
+    uint32_t p = f2u(a.w);
+    uint32_t i = pairs[2 * p], j = pairs[2 * p + 1];
+    BodyRec A = bodies[i], B = bodies[j];
+    EpaPair ep;
+    Pair& pr = ep.pr;
+    pr.a = shape_of(A, hull);
+    pr.b = shape_of(B, hull);
+    pr.pa = pos_of(A);
+    pr.pb = pos_of(B);
+    pr.qa = quat_of(A);
+    pr.qb = quat_of(B);
+    ep.ha.load(pr.a, lane);
+    ep.hb.load(pr.b, lane);
+    EpaResult e = wave_epa(ep, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, V3{d.x, d.y, d.z}, S, lane);
+    // convex_convex.rs:117-154: margins added back, witness = supportA(n) - n depth.
+    bool hit = !(e.depth <= 0.0f);
+    V3 point_a = splat(0.0f);
+    float depth = 0.0f;
+    if (hit) {
+      depth = (e.depth + pr.a.margin) + pr.b.margin;
+      point_a = wave_support_world(pr.a, ep.ha, pr.pa, pr.qa, e.n, lane, false);
+    }
+    if (lane == 0) {
+      if (e.status == 4) atomicOr(&cnt->errors, 2u);
+      ccount[p] = hit ? 1u : 0u;
+      route[p] |= 8u;
+      if (hit) {
+        V3 pos = point_a - e.n * depth;
+        cslots[(uint64_t)p * 8] = make_float4(pos.x, pos.y, pos.z, depth);
+        cslots[(uint64_t)p * 8 + 1] = make_float4(e.n.x, e.n.y, e.n.z, 0.0f);
       }
     }
     wave_sync();

@@ -12,8 +12,10 @@
 //                 right: exact pair set without duplicates
 //   k_big         bodies with unbounded AABBs (planes) against their world
 //   radix sort    (rocPRIM) pair keys i << 32 | j -> lexicographic order
-//   k_narrow      one lane per pair (pairs grouped by shape class)
-//   k_epa         one lane per deep-overlap pair, polytope in scratch
+//   k_np<CLS>     one lane per pair, one kernel per shape class
+//   k_np_hull     a group of lanes per pair with a hull
+//   k_epa_wave    one wavefront per deep-overlap pair, polytope in LDS
+//                 (bch_epa_wave.hpp)
 //   k_gather      per-pair contact slots -> dense CSR contacts
 #pragma once
 #include "bch_narrow.hpp"
@@ -797,32 +799,6 @@ __global__ __launch_bounds__(256) void k_np_hull(const Counters* __restrict__ cn
         epa_q[4 * (uint64_t)q + 3] = make_float4(g.t3.x, g.t3.y, g.t3.z, 0.0f);
       }
     }
-  }
-}
-
-// EPA, one lane per queued pair; lane-private polytope in global scratch
-// (grid-stride over the queue with a fixed grid, one store per lane).
-__global__ void k_epa(const Counters* __restrict__ cnt_ro, const float4* __restrict__ epa_q,
-                      const uint32_t* __restrict__ pairs, const BodyRec* __restrict__ bodies,
-                      const float4* __restrict__ hull, uint32_t* __restrict__ ccount, float4* __restrict__ cslots,
-                      float4* __restrict__ sv, EpaFace* __restrict__ sf, uint32_t* __restrict__ se, Counters* cnt) {
-  uint32_t nq = cnt_ro->n_epa;
-  uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  EpaStore st{sv + (uint64_t)gid * BCH_EPA_MAX_VERTS, sf + (uint64_t)gid * BCH_EPA_MAX_FACES,
-              se + (uint64_t)gid * BCH_EPA_MAX_EDGES};
-  for (uint32_t q = gid; q < nq; q += gridDim.x * blockDim.x) {
-    float4 a = epa_q[4 * (uint64_t)q], b = epa_q[4 * (uint64_t)q + 1], c = epa_q[4 * (uint64_t)q + 2],
-           d = epa_q[4 * (uint64_t)q + 3];
-    uint32_t p = f2u(a.w);
-    uint32_t i = pairs[2 * p], j = pairs[2 * p + 1];
-    Pair pr = make_pair(bodies[i], bodies[j], hull);
-    EpaResult e = epa_solve(pr, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, V3{d.x, d.y, d.z}, st);
-    if (e.status == 4) atomicOr(&cnt->errors, 2u);
-    Manifold4 m;
-    m.n = 0;
-    epa_contact(pr, e, m);
-    ccount[p] = (uint32_t)m.n;
-    store_contacts(cslots, p, m);
   }
 }
 

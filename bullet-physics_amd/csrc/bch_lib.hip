@@ -55,8 +55,8 @@ struct bch_context {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  hipEvent_t ev[8] = {};
-  float timings[7] = {0, 0, 0, 0, 0, 0, 0};
+  hipEvent_t ev[9] = {};
+  float timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
   // world-resident bodies
   uint32_t n = 0, n_worlds = 1, max_world = 0;
@@ -279,6 +279,7 @@ int run_narrowphase(bch_context* c, const float* d_seeds) {
     BCH_NP(CLS_BOX_BOX);
     BCH_NP(CLS_GJK);
 #undef BCH_NP
+    (void)hipEventRecord(c->ev[5], s);
 #define BCH_NP_HULL(G)                                                                                     \
   hipLaunchKernelGGL(k_np_hull<G>, dim3(g), dim3(kBlock), 0, s, cnt, perm, c->pairs.as<uint32_t>(),        \
                      c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_seeds, c->ccount.as<uint32_t>(),     \
@@ -290,15 +291,13 @@ int run_narrowphase(bch_context* c, const float* d_seeds) {
     else
       BCH_NP_HULL(16);
 #undef BCH_NP_HULL
-    (void)hipEventRecord(c->ev[5], s);
-    hipLaunchKernelGGL(k_epa_grp<64>, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
+    (void)hipEventRecord(c->ev[6], s);
+    hipLaunchKernelGGL(k_epa_wave, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
                        c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
-                       c->ccount.as<uint32_t>(), c->cslots.as<float4>(), cnt);
-    (void)hipEventRecord(c->ev[6], s);
+                       c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(), cnt);
+    (void)hipEventRecord(c->ev[7], s);
   } else {
-    (void)hipEventRecord(c->ev[4], s);
-    (void)hipEventRecord(c->ev[5], s);
-    (void)hipEventRecord(c->ev[6], s);
+    for (int k = 4; k <= 7; ++k) (void)hipEventRecord(c->ev[k], s);
   }
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, 1u, c->ccount.as<uint32_t>() + P, 0u);
   int rc = with_tmp(c, [&](void* t, size_t& sz) {
@@ -314,7 +313,7 @@ int run_narrowphase(bch_context* c, const float* d_seeds) {
   if (P)
     hipLaunchKernelGGL(k_gather, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->ccount.as<uint32_t>(),
                        c->coffsets.as<uint32_t>(), c->cslots.as<float4>(), c->contacts.as<float>());
-  (void)hipEventRecord(c->ev[7], s);
+  (void)hipEventRecord(c->ev[8], s);
   HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
   HIP_TRY(c, hipStreamSynchronize(s));
   if (c->host_cnt.errors & 2u) return fail(c, BCH_HIP_ERROR, "EPA polytope store overflow");
@@ -442,7 +441,7 @@ int bch_synchronize(bch_context* c) {
 
 int bch_last_timings(bch_context* c, float* out_ms) {
   if (!c || !out_ms) return BCH_INVALID_ARG;
-  for (int i = 0; i < 7; ++i) out_ms[i] = c->timings[i];
+  for (int i = 0; i < 8; ++i) out_ms[i] = c->timings[i];
   return BCH_OK;
 }
 
@@ -583,6 +582,40 @@ int bch_bp_aabb_candidates(bch_context* c, const float* mn, const float* mx, uin
     const float* b = &c->proxy_box[6 * i];
     Box6 o{V3{b[0], b[1], b[2]}, V3{b[3], b[4], b[5]}};
     if (overlaps(o, q)) {
+      if (out && k < capacity) out[k] = (uint32_t)i;
+      ++k;
+    }
+  }
+  *out_count = k;
+  return k > capacity ? BCH_OVERFLOW : BCH_OK;
+}
+
+int bch_bp_raycast_candidates(bch_context* c, const float* from, const float* to, uint32_t* out,
+                              uint64_t capacity, uint64_t* out_count) {
+  if (!c || !from || !to || !out_count) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  // Host scan over the proxy arena with the slab test of
+  // broadphase/mod.rs:153-183 (t in [0, 1]; fmaxf / fminf keep the non-NaN
+  // operand, as f32::max / f32::min do).
+  uint64_t k = 0;
+  for (size_t i = 0; i < c->proxy_live.size(); ++i) {
+    if (!c->proxy_live[i]) continue;
+    const float* b = &c->proxy_box[6 * i];
+    float t_min = 0.0f, t_max = 1.0f;
+    bool hit = true;
+    for (int a = 0; a < 3 && hit; ++a) {
+      float d = to[a] - from[a];
+      float lo = d >= 0.0f ? b[a] : b[3 + a];
+      float hi = d >= 0.0f ? b[3 + a] : b[a];
+      if (d == 0.0f) {
+        if (from[a] < lo || from[a] > hi) hit = false;
+        continue;
+      }
+      float inv = 1.0f / d;
+      t_min = fmaxf(t_min, (lo - from[a]) * inv);
+      t_max = fminf(t_max, (hi - from[a]) * inv);
+      if (t_min > t_max) hit = false;
+    }
+    if (hit) {
       if (out && k < capacity) out[k] = (uint32_t)i;
       ++k;
     }
@@ -735,10 +768,21 @@ int bch_world_collide(bch_context* c, float aabb_margin, uint64_t* out_pairs, ui
   rc = run_narrowphase(c, nullptr);
   if (rc) return rc;
   HIP_TRY(c, hipGetLastError());
-  for (int k = 0; k < 7; ++k) (void)hipEventElapsedTime(&c->timings[k], c->ev[k], c->ev[k + 1]);
+  for (int k = 0; k < 8; ++k) (void)hipEventElapsedTime(&c->timings[k], c->ev[k], c->ev[k + 1]);
   if (c->host_cnt.errors & 1u) return fail(c, BCH_HIP_ERROR, "box-box clip polygon overflow");
   if (out_pairs) *out_pairs = c->P;
   if (out_contacts) *out_contacts = c->C;
+  return BCH_OK;
+}
+
+int bch_world_routes(bch_context* c, uint8_t* routes) {
+  if (!c || (!routes && c->P)) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (!c->P) return BCH_OK;
+  std::vector<uint32_t> cls(c->P);
+  HIP_TRY(c, hipMemcpyAsync(cls.data(), c->cls.p, c->P * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (uint64_t k = 0; k < c->P; ++k) routes[k] = (uint8_t)cls[k];
   return BCH_OK;
 }
 

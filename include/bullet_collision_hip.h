@@ -69,10 +69,12 @@ int bch_create(int device, bch_context** out);
 void bch_destroy(bch_context* ctx);
 const char* bch_last_error(const bch_context* ctx);
 int bch_synchronize(bch_context* ctx);
-/* Device times (ms) of the last bch_world_collide pass, 7 floats:
+/* Device times (ms) of the last bch_world_collide pass, 8 floats:
  * [0] world AABBs, [1] tree build (Morton codes, sort, nodes, refit),
  * [2] tree traversal (pair search), [3] pair list (planes, compaction,
- * sort), [4] narrowphase, [5] EPA, [6] contact compaction. */
+ * sort, shape-class split), [4] closed-form and GJK pairs without a hull
+ * (k_np kernels), [5] GJK pairs with a hull (k_np_hull), [6] EPA
+ * (k_epa_wave), [7] contact compaction. */
 int bch_last_timings(bch_context* ctx, float* out_ms);
 
 /* No reference counterpart: launch-shape knobs of the device passes.  They
@@ -113,6 +115,11 @@ int bch_bp_aabb_candidates(bch_context* ctx, const float aabb_min[3], const floa
                            uint64_t capacity, uint64_t* out_count);
 
 /* ------------------------------------------------- narrowphase (batch) -- */
+/* raycast_candidates (broadphase/mod.rs:119-124, brute_force.rs:70-72):
+ * live proxies whose AABB the segment ray_from -> ray_to crosses (slab test
+ * of broadphase/mod.rs:153-183), ascending ids; BCH_OVERFLOW as above. */
+int bch_bp_raycast_candidates(bch_context* ctx, const float ray_from[3], const float ray_to[3], uint32_t* out,
+                              uint64_t capacity, uint64_t* out_count);
 /* collide_seeded (narrowphase/mod.rs:63-286) for every pair: shape A =
  * pairs[2k], shape B = pairs[2k+1], seeds[3k..] an optional GJK warm-start
  * direction (NULL = None).  out_count[k] = number of contact points (0 =
@@ -144,6 +151,13 @@ int bch_world_collide(bch_context* ctx, float aabb_margin, uint64_t* out_pairs, 
  * offsets [P+1] (contacts of pair k are offsets[k] .. offsets[k+1]) and
  * contacts [7C].  Any pointer may be NULL to skip that array. */
 int bch_world_results(bch_context* ctx, uint32_t* pairs, uint32_t* contact_offsets, float* contacts);
+/* No reference counterpart (diagnostic): the route every pair of the last
+ * pass took through collide_seeded's dispatch (narrowphase/mod.rs:99-285),
+ * one byte per pair in pair order: 0 none (plane-plane), 1 sphere-sphere,
+ * 2 sphere-box, 3 plane-convex, 4 box-box, 5 convex-convex without a hull,
+ * 6 convex-convex with a hull; +8 when the pair went on to EPA
+ * (convex_convex.rs:117-154). */
+int bch_world_routes(bch_context* ctx, uint8_t* routes);
 /* Device pointers of the last pass's results (valid until the next pass). */
 int bch_world_results_device(bch_context* ctx, const uint32_t** d_pairs, const uint32_t** d_contact_offsets,
                              const float** d_contacts);

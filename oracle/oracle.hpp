@@ -129,6 +129,31 @@ struct Aabb {
   }
 };
 
+// broadphase/mod.rs:153-183: slab test of the segment from -> to (t in
+// [0, 1]); f32::max / f32::min keep the non-NaN operand like fmax / fmin.
+template <class T>
+inline bool segment_hits_aabb(const Aabb<T>& a, V3<T> from, V3<T> to) {
+  V3<T> dir = to - from;
+  T t_min = T(0), t_max = T(1);
+  for (int k = 0; k < 3; ++k) {
+    T d = k == 0 ? dir.x : (k == 1 ? dir.y : dir.z);
+    T f = k == 0 ? from.x : (k == 1 ? from.y : from.z);
+    T amin = k == 0 ? a.min.x : (k == 1 ? a.min.y : a.min.z);
+    T amax = k == 0 ? a.max.x : (k == 1 ? a.max.y : a.max.z);
+    T lo = d >= T(0) ? amin : amax;
+    T hi = d >= T(0) ? amax : amin;
+    if (d == T(0)) {
+      if (f < lo || f > hi) return false;
+      continue;
+    }
+    T inv = T(1) / d;
+    t_min = std::fmax(t_min, (lo - f) * inv);
+    t_max = std::fmin(t_max, (hi - f) * inv);
+    if (t_min > t_max) return false;
+  }
+  return true;
+}
+
 // ----------------------------------------------------------------- shapes --
 enum Kind : uint8_t { SPHERE = 0, BOX = 1, CAPSULE = 2, HULL = 3, PLANE = 4 };
 

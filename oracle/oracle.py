@@ -60,6 +60,9 @@ def lib():
         scp = C.POINTER(OrScene)
         L.oracle_scene_aabbs_f32.argtypes = [scp, C.c_float, C.c_void_p, C.c_void_p]
         L.oracle_scene_thresholds_f32.argtypes = [scp, C.c_void_p]
+        L.oracle_raycast_candidates.restype = C.c_int64
+        L.oracle_raycast_candidates.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                C.c_void_p, C.c_int64]
         L.oracle_brute_pairs.restype = C.c_int64
         L.oracle_brute_pairs.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         L.oracle_sweep_pairs.restype = C.c_int64
@@ -265,6 +268,19 @@ def brute_pairs(mn, mx):
     cnt = lib().oracle_brute_pairs(n, mn.ctypes.data, mx.ctypes.data, None, 0)
     out = np.empty((max(cnt, 1), 2), np.uint32)
     lib().oracle_brute_pairs(n, mn.ctypes.data, mx.ctypes.data, out.ctypes.data, cnt)
+    return out[:cnt]
+
+
+def raycast_candidates(mn, mx, ray_from, ray_to):
+    """aabb_raycast_candidates (broadphase/mod.rs:185-200): ascending ids."""
+    mn = np.ascontiguousarray(mn, np.float32)
+    mx = np.ascontiguousarray(mx, np.float32)
+    f = np.ascontiguousarray(ray_from, np.float32)
+    t = np.ascontiguousarray(ray_to, np.float32)
+    n = mn.shape[0]
+    out = np.zeros(max(n, 1), np.uint32)
+    cnt = lib().oracle_raycast_candidates(n, mn.ctypes.data, mx.ctypes.data, f.ctypes.data, t.ctypes.data,
+                                          out.ctypes.data, out.shape[0])
     return out[:cnt]
 
 

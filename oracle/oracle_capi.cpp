@@ -331,6 +331,20 @@ int64_t oracle_brute_pairs(int64_t n, const float* mn, const float* mx, uint32_t
   return cnt;
 }
 
+// aabb_raycast_candidates (broadphase/mod.rs:185-200) over n boxes.
+int64_t oracle_raycast_candidates(int64_t n, const float* mn, const float* mx, const float* from, const float* to,
+                                  uint32_t* out, int64_t cap) {
+  int64_t cnt = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    Aabb<float> a{v3p<float>(mn + 3 * i), v3p<float>(mx + 3 * i)};
+    if (segment_hits_aabb(a, v3p<float>(from), v3p<float>(to))) {
+      if (cnt < cap) out[cnt] = (uint32_t)i;
+      ++cnt;
+    }
+  }
+  return cnt;
+}
+
 // The same pair set through an x-axis sweep (for sizes brute force cannot
 // reach); checked against oracle_brute_pairs by the tests.
 int64_t oracle_sweep_pairs(int64_t n, const float* mn, const float* mx, uint32_t* out, int64_t cap) {

@@ -91,6 +91,21 @@ def test_mixed_convex_pass(bpa, scenes, oracle, ctx):
     assert np.array_equal(pairs, ref_pairs)
     _flush(f"paths: {np.bincount(path, minlength=4)} (gjk-sep, band, epa, closed)")
     compare_contacts(ref_cnt, ref_pts, offs, cont, label="mixed_convex")
+    # Dispatch route per pair: the shape-type class, and EPA exactly where the
+    # oracle's convex_convex took its EPA arm (path 2).
+    routes = w.routes()
+    t = sc["shape_type"]
+    ti, tj = t[pairs[:, 0]], t[pairs[:, 1]]
+    SPH, BOX, HULL, PLANE = 0, 1, 3, 4
+    cls = np.full(pairs.shape[0], 5, np.uint8)
+    cls[(ti == HULL) | (tj == HULL)] = 6
+    cls[(ti == BOX) & (tj == BOX)] = 4
+    cls[((ti == SPH) & (tj == BOX)) | ((ti == BOX) & (tj == SPH))] = 2
+    cls[(ti == SPH) & (tj == SPH)] = 1
+    cls[(ti == PLANE) | (tj == PLANE)] = 3
+    cls[(ti == PLANE) & (tj == PLANE)] = 0
+    assert np.array_equal(routes & 7, cls), "shape-class route differs"
+    assert np.array_equal((routes & 8) != 0, path == 2), "EPA route differs from the oracle's convex_convex"
 
 
 def test_collide_batch_matches_oracle(bpa, scenes, oracle, ctx):
@@ -290,3 +305,29 @@ def test_large_scene_properties(bpa, scenes, oracle, ctx):
         assert np.array_equal(np.sort(got), ov), f"body {i}"
     assert np.all(np.isfinite(cont))
     assert offs[-1] == Cn
+
+
+@pytest.mark.gpu
+def test_raycast_candidates_match_oracle(bpa, oracle, ctx):
+    """raycast_candidates through the C ABI: the reference KAT
+    (axis_sweep3.rs:734-748) and random segments against the oracle."""
+    _flush("case: raycast candidates")
+    bp = bpa.BroadPhase(ctx=bpa._Context(0))
+    hit = bp.create_proxy([1, 1, 1], [2, 2, 2])
+    miss = bp.create_proxy([10, 10, 10], [11, 11, 11])
+    assert bp.raycast_candidates([0, 0, 0], [1.5, 1.5, 1.5]).tolist() == [hit]
+    inside = bp.raycast_candidates([1.5, 1.5, 1.5], [4, 4, 4]).tolist()
+    assert hit in inside and miss not in inside
+    assert bp.raycast_candidates([3, 3, 3], [4, 4, 4]).tolist() == []
+    rng = np.random.default_rng(11)
+    bp2 = bpa.BroadPhase(ctx=bpa._Context(0))
+    lo = rng.uniform(-20, 20, (500, 3)).astype(np.float32)
+    hi = (lo + rng.uniform(0.1, 3.0, (500, 3))).astype(np.float32)
+    for i in range(500):
+        bp2.create_proxy(lo[i], hi[i])
+    for k in range(50):
+        f = rng.uniform(-25, 25, 3).astype(np.float32)
+        t = rng.uniform(-25, 25, 3).astype(np.float32)
+        if k % 7 == 3:
+            t[1] = f[1]
+        assert np.array_equal(bp2.raycast_candidates(f, t), oracle.raycast_candidates(lo, hi, f, t))

@@ -378,3 +378,12 @@ def test_epa_point_set_cube(oracle):
     assert st == 0
     assert depth == pytest.approx(1.0, abs=1e-4)
     assert np.max(np.abs(normal)) == pytest.approx(1.0, abs=1e-4)
+
+
+def test_raycast_candidates_filters_by_ray_path(oracle):  # axis_sweep3.rs:734-748
+    mn = np.array([[1, 1, 1], [10, 10, 10]], np.float32)
+    mx = np.array([[2, 2, 2], [11, 11, 11]], np.float32)
+    assert oracle.raycast_candidates(mn, mx, [0, 0, 0], [1.5, 1.5, 1.5]).tolist() == [0]
+    inside = oracle.raycast_candidates(mn, mx, [1.5, 1.5, 1.5], [4, 4, 4]).tolist()
+    assert 0 in inside and 1 not in inside
+    assert oracle.raycast_candidates(mn, mx, [3, 3, 3], [4, 4, 4]).tolist() == []

@@ -282,3 +282,34 @@ def test_box_box_routes_match_their_closed_forms(oracle):
     assert checked > 100
     assert cap_exits <= 0.1 * checked
     assert full > 100   # most deepest vertices lie inside the reference face
+
+
+def test_raycast_slab_test_against_dense_sampling(oracle):
+    """segment_hits_aabb (broadphase/mod.rs:153-183) against an independent
+    check: a segment that has a sampled point inside a box must be reported;
+    a reported hit whose overlap interval is long must have one.  Includes
+    axis-parallel and zero-length segments."""
+    rng = np.random.default_rng(5)
+    n = 300
+    lo = rng.uniform(-5, 5, (n, 3)).astype(np.float32)
+    mn, mx = lo, (lo + rng.uniform(0.1, 2.0, (n, 3))).astype(np.float32)
+    ts = np.linspace(0.0, 1.0, 4001)
+    for k in range(40):
+        f = rng.uniform(-6, 6, 3).astype(np.float32)
+        t = rng.uniform(-6, 6, 3).astype(np.float32)
+        if k % 5 == 1:
+            t[k % 3] = f[k % 3]          # parallel to a slab
+        if k % 10 == 7:
+            t = f.copy()                 # zero length
+        got = set(oracle.raycast_candidates(mn, mx, f, t).tolist())
+        pts = f.astype(np.float64)[None, :] + ts[:, None] * (t.astype(np.float64) - f)[None, :]
+        eps = 1e-4  # sampled points must be clearly inside (f32 rounding at the faces)
+        inside = ((pts[None, :, :] >= mn[:, None, :] + eps) & (pts[None, :, :] <= mx[:, None, :] - eps)).all(axis=2)
+        sampled = set(np.nonzero(inside.any(axis=1))[0].tolist())
+        assert sampled <= got, f"segment {k}: sampled hits missing {sorted(sampled - got)}"
+        deep = set(np.nonzero(inside.sum(axis=1) >= 8)[0].tolist())
+        assert deep <= got
+        for i in got - sampled:
+            # a reported hit that no sample found must graze: within 1e-3 of the box
+            d = np.maximum(np.maximum(mn[i] - pts, pts - mx[i]), 0).max(axis=1).min()
+            assert d < 1e-3 * (1 + np.abs(pts).max()), f"segment {k}: spurious hit on box {i} (gap {d})"
