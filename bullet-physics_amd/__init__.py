@@ -67,6 +67,15 @@ class _Context:
             out[k] = v.value
         return out
 
+    def set_stream(self, stream: int | None) -> None:
+        """Order this context's work on a caller stream (hipStream_t as an
+        int; 0 is the device's null stream); None restores the context's
+        own stream."""
+        if stream is None:
+            _check(self.h, self.L.bch_reset_stream(self.h))
+        else:
+            _check(self.h, self.L.bch_set_stream(self.h, C.c_void_p(stream)))
+
     def close(self):
         if self.h:
             self.L.bch_destroy(self.h)
@@ -307,6 +316,23 @@ class CollisionWorld:
         _check(self.ctx.h, self.L.bch_world_results(self.ctx.h, pairs.ctypes.data, offs.ctypes.data,
                                                      cont.ctypes.data))
         return pairs[:self.n_pairs], offs, cont[:self.n_contacts]
+
+    def results_device(self):
+        """Device pointers (pairs u32[2P], contact_offsets u32[P+1],
+        contacts f32[7C]) of the last pass, valid until the next pass."""
+        a, b, c = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        _check(self.ctx.h, self.L.bch_world_results_device(self.ctx.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value or 0, b.value or 0, c.value or 0
+
+    def sd_forward(self, d_pairs: int, n_pairs: int, d_out: int, d_status: int | None = None) -> None:
+        """Signed distance of device pairs [2P] into device out [10P] (sd,
+        normal A->B, witness on A, witness on B) at the current poses."""
+        _check(self.ctx.h, self.L.bch_sd_forward(self.ctx.h, d_pairs, n_pairs, d_out, d_status))
+
+    def sd_backward(self, d_pairs: int, n_pairs: int, d_grad_out: int, d_grad_pose: int,
+                    d_status: int | None = None) -> None:
+        """Adjoint: dL/d(out) [10P] -> dL/d(pose A, pose B) [14P] per pair."""
+        _check(self.ctx.h, self.L.bch_sd_backward(self.ctx.h, d_pairs, n_pairs, d_grad_out, d_grad_pose, d_status))
 
     # stage names of timings(), in order
     STAGES = ("aabb", "tree_build", "tree_traverse", "pair_list", "narrow_closed_gjk", "narrow_hull_gjk", "epa",

@@ -6,6 +6,7 @@ missing: there is no CPU fallback on this path.
 from __future__ import annotations
 
 import ctypes as C
+import importlib.util
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -20,6 +21,7 @@ EXPORTS = [
     "bch_bp_compute_pairs", "bch_bp_aabb_candidates", "bch_bp_raycast_candidates", "bch_collide_batch",
     "bch_world_set_bodies", "bch_world_set_poses", "bch_world_set_poses_device", "bch_world_collide",
     "bch_world_results", "bch_world_results_device", "bch_world_routes", "bch_set_tuning", "bch_get_tuning",
+    "bch_sd_forward", "bch_sd_backward", "bch_set_stream", "bch_reset_stream",
 ]
 
 # launch-shape knobs (bch_set_tuning)
@@ -41,6 +43,24 @@ class BchError(RuntimeError):
 _lib = None
 
 
+def _share_torch_hip_runtime():
+    """One HIP runtime per process.  PyTorch wheels carry their own
+    libamdhip64 (same soname, libamdhip64.so.7, loaded from torch/lib by
+    their own name libamdhip64.so): if this library loaded the system copy
+    first, a later `import torch` would bring up a second runtime that sees
+    no device.  Loading torch's copy first (without importing torch) makes
+    this library bind to it by soname, and torch finds it already loaded.
+    BCH_SYSTEM_HIP=1 keeps the system runtime."""
+    if os.environ.get("BCH_SYSTEM_HIP") == "1":
+        return
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return
+    path = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
 def load():
     """Load the in-tree HIP library (raises if it has not been built)."""
     global _lib
@@ -48,6 +68,7 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} not built: run python bullet-physics_amd/build.py (needs hipcc)")
+    _share_torch_hip_runtime()
     L = C.CDLL(LIB_PATH)
     vp, u32, u64, f32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_float
     L.bch_create.argtypes = [C.c_int, C.POINTER(vp)]
@@ -73,6 +94,10 @@ def load():
     L.bch_world_results.argtypes = [vp, vp, vp, vp]
     L.bch_world_results_device.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
     L.bch_world_routes.argtypes = [vp, vp]
+    L.bch_sd_forward.argtypes = [vp, vp, u64, vp, vp]
+    L.bch_sd_backward.argtypes = [vp, vp, u64, vp, vp, vp]
+    L.bch_set_stream.argtypes = [vp, vp]
+    L.bch_reset_stream.argtypes = [vp]
     L.bch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.bch_get_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     _lib = L

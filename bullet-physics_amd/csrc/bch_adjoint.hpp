@@ -1,0 +1,475 @@
+// bch_adjoint.hpp -- signed distance between convex pairs and its adjoint
+// with respect to both bodies' poses (position xyz, quaternion xyzw).
+//
+// Forward (per pair): the reference's margin-core walk (gjk.rs:171-288) with
+// an unbounded band (threshold = +inf), so a separated pair always ends in
+// its Band arm; overlapping cores go to EPA (epa.rs:127-205) and the contact
+// of convex_convex.rs:117-154.  A walk stopped by its 64-iteration cap
+// reports its last iterate as the Band arm would.  Outputs per pair, 10
+// floats: sd (> 0 separated), unit normal A -> B, witness on A, witness on B
+// (pa = pb - n sd).
+//
+// Backward: recomputation, no stored simplices.  The walk runs again; the
+// Band arm's closest point v = sum l_i w_i of the final active simplex (w_i
+// = a_i - b_i, support points of the two cores) is differentiated through
+// the affine least-norm problem that fixes the weights l; EPA runs again and
+// its reported face (vertices w_a, w_b, w_c) is differentiated through the
+// plane distance n.w_a with n = normalize((w_b - w_a) x (w_c - w_a)).  Each
+// support point is a fixed local point x of its shape (hull vertex, core box
+// corner, capsule tip, sphere centre) placed by qrot(q, x) + p, so pose
+// gradients come from the derivative of the quaternion-rotate formula.  The
+// face vertices' split into (a, b) is found by recomputing every candidate
+// difference of the two support sets and matching it exactly (the same
+// arithmetic produced them).
+#pragma once
+#include "bch_epa_wave.hpp"
+
+namespace bch {
+
+enum : uint32_t {
+  SD_WALK = 0,             // Band arm of the walk (separated, or inside the margins)
+  SD_EPA = 1,              // EPA converged
+  SD_CAPPED = 2,           // walk stopped at its iteration cap (last iterate)
+  SD_TOUCH = 3,            // cores touch and no enclosing tetrahedron: sd = -(margins), no normal
+  SD_EPA_UNCONVERGED = 4,  // EPA ended without converging (reported face)
+  SD_FLAG_EDGE = 8,        // EPA face distance came from an edge or vertex, not the plane
+  SD_FLAG_SPLIT = 16       // a face vertex could not be split into support points
+};
+
+// Pose gradient accumulator for one pair: position A, quaternion A,
+// position B, quaternion B.
+struct PoseGrad {
+  V3 pa, pb;
+  float qa[4], qb[4];
+  BCH_HD void zero() {
+    pa = pb = splat(0.0f);
+    for (int k = 0; k < 4; ++k) qa[k] = qb[k] = 0.0f;
+  }
+};
+
+// Band arm (or capped walk): outputs from the active simplex and the VJP.
+// act holds k = act.n vertices (w, b); v is the walk's closest point.
+struct SdOut {
+  float sd;
+  V3 n, pa, pb;
+};
+
+BCH_HD SdOut band_outputs(const Pair& pr, V3 v, V3 p_b) {
+  float core = fsqrt(len2(v));
+  SdOut o;
+  o.sd = core - (pr.a.margin + pr.b.margin);
+  o.n = (-v) / core;
+  o.pb = p_b - o.n * pr.b.margin;
+  o.pa = o.pb - o.n * o.sd;
+  return o;
+}
+
+// Core support point `idx` of a shape in local coordinates (the finite set
+// the core supports are drawn from) and the set's size.
+BCH_HD int core_set_size(const ShapeRef& s) {
+  return s.type == HULL ? s.nhull : (s.type == BOX ? 8 : (s.type == CAPSULE ? 2 : 1));
+}
+BCH_HD V3 core_set_point(const ShapeRef& s, int idx) {
+  if (s.type == HULL) return hull_vertex(s.hull, idx);
+  if (s.type == BOX) {
+    float hx = mx(s.a0 - 0.04f, 0.0f), hy = mx(s.a1 - 0.04f, 0.0f), hz = mx(s.a2 - 0.04f, 0.0f);
+    return V3{(idx & 1) ? -hx : hx, (idx & 2) ? -hy : hy, (idx & 4) ? -hz : hz};
+  }
+  if (s.type == CAPSULE) return V3{0.0f, idx ? -s.a1 : s.a1, 0.0f};
+  return splat(0.0f);
+}
+
+// ---- double-precision Band arm.  Near-touching cores make v = a - b a
+// difference of nearly equal points: its f32 rounding, amplified by the
+// 1/|v| of the normal's derivative, would dominate the gradient.  The walk
+// records which local point every support came from (hull vertex, core box
+// corner, capsule tip, sphere centre); the adjoint rebuilds a, b from them in
+// double and differentiates the least-norm point there.
+struct D3 {
+  double x, y, z;
+};
+BCH_HD D3 d3(V3 v) { return D3{(double)v.x, (double)v.y, (double)v.z}; }
+BCH_HD D3 operator+(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+BCH_HD D3 operator-(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+BCH_HD D3 operator-(D3 a) { return D3{-a.x, -a.y, -a.z}; }
+BCH_HD D3 operator*(D3 a, double s) { return D3{a.x * s, a.y * s, a.z * s}; }
+BCH_HD double dotd(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+BCH_HD D3 crossd(D3 a, D3 b) { return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+BCH_HD D3 qrotd(Q4 q, D3 v) {
+  D3 b{(double)q.x, (double)q.y, (double)q.z};
+  double w = q.w;
+  return (v * (w * w - dotd(b, b)) + b * (2.0 * dotd(v, b))) + crossd(b, v) * (2.0 * w);
+}
+// J^T g of qrot(q, x) = x (w^2 - b.b) + b (2 x.b) + (b x x)(2w) with respect
+// to q = (b, w), added to gq.
+BCH_HD void qrot_vjpd(Q4 q, D3 x, D3 g, double gq[4]) {
+  D3 b{(double)q.x, (double)q.y, (double)q.z};
+  double w = q.w;
+  D3 gb = ((g * dotd(x, b) + x * dotd(b, g)) - b * dotd(x, g) + crossd(x, g) * w) * 2.0;
+  gq[0] += gb.x;
+  gq[1] += gb.y;
+  gq[2] += gb.z;
+  gq[3] += 2.0 * (w * dotd(x, g) + dotd(crossd(b, x), g));
+}
+
+struct AffineD {
+  double lam[3], mu0, mu1, i00, i01, i11;
+  D3 D0, D1;
+};
+
+BCH_HD AffineD affine_weights_d(const D3* w, int k) {
+  AffineD A;
+  A.lam[0] = 1.0;
+  A.lam[1] = A.lam[2] = 0.0;
+  A.mu0 = A.mu1 = A.i00 = A.i01 = A.i11 = 0.0;
+  A.D0 = A.D1 = D3{0.0, 0.0, 0.0};
+  if (k == 2) {
+    A.D0 = w[1] - w[0];
+    A.i00 = 1.0 / dotd(A.D0, A.D0);
+    A.mu0 = -dotd(A.D0, w[0]) * A.i00;
+    A.lam[0] = 1.0 - A.mu0;
+    A.lam[1] = A.mu0;
+  } else if (k == 3) {
+    A.D0 = w[1] - w[0];
+    A.D1 = w[2] - w[0];
+    double m00 = dotd(A.D0, A.D0), m01 = dotd(A.D0, A.D1), m11 = dotd(A.D1, A.D1);
+    double det = m00 * m11 - m01 * m01;
+    A.i00 = m11 / det;
+    A.i01 = -m01 / det;
+    A.i11 = m00 / det;
+    double r0 = -dotd(A.D0, w[0]), r1 = -dotd(A.D1, w[0]);
+    A.mu0 = A.i00 * r0 + A.i01 * r1;
+    A.mu1 = A.i01 * r0 + A.i11 * r1;
+    A.lam[0] = (1.0 - A.mu0) - A.mu1;
+    A.lam[1] = A.mu0;
+    A.lam[2] = A.mu1;
+  }
+  return A;
+}
+
+BCH_HD void band_vjp(const Pair& pr, const Simp& act, const float* g10, PoseGrad& G) {
+  int k = act.n;
+  const SV* sv[3] = {&act.s0, &act.s1, &act.s2};
+  D3 xa[3], xb[3], w[3], b[3];
+  for (int i = 0; i < 3; ++i) {
+    xa[i] = d3(core_set_point(pr.a, sv[i]->ia));
+    xb[i] = d3(core_set_point(pr.b, sv[i]->ib));
+    D3 ai = qrotd(pr.qa, xa[i]) + d3(pr.pa);
+    b[i] = qrotd(pr.qb, xb[i]) + d3(pr.pb);
+    w[i] = ai - b[i];
+  }
+  // the least-norm point's feature: drop vertices whose exact weight is not
+  // positive (the f32 walk can keep one at rounding level)
+  AffineD A = affine_weights_d(w, k);
+  for (int pass = 0; pass < 2 && k > 1; ++pass) {
+    int drop = -1;
+    double low = 0.0;
+    for (int i = 0; i < 3; ++i)
+      if (i < k && A.lam[i] <= low) {
+        low = A.lam[i];
+        drop = i;
+      }
+    if (drop < 0) break;
+    for (int i = drop; i < 2; ++i) {
+      w[i] = w[i + 1];
+      b[i] = b[i + 1];
+      xa[i] = xa[i + 1];
+      xb[i] = xb[i + 1];
+    }
+    --k;
+    A = affine_weights_d(w, k);
+  }
+  D3 v{0.0, 0.0, 0.0};
+  for (int i = 0; i < 3; ++i)
+    if (i < k) v = v + w[i] * A.lam[i];
+  double ma = pr.a.margin, mb = pr.b.margin;
+  double core = sqrt(dotd(v, v));
+  D3 nv = v * (1.0 / core);
+  D3 n = -nv;
+  double sd = core - (ma + mb);
+  D3 gn{g10[1], g10[2], g10[3]}, gpa{g10[4], g10[5], g10[6]}, gpb{g10[7], g10[8], g10[9]};
+  D3 g_pbt = gpb + gpa;
+  D3 g_nt = (gn - gpa * sd) - g_pbt * mb;
+  double g_sdt = (double)g10[0] - dotd(n, gpa);
+  D3 g_v = nv * g_sdt - (g_nt - n * dotd(n, g_nt)) * (1.0 / core);
+  D3 gw[3], gb[3];
+  double glam[3];
+  for (int i = 0; i < 3; ++i) {
+    gw[i] = g_v * A.lam[i];
+    gb[i] = g_pbt * A.lam[i];
+    glam[i] = dotd(w[i], g_v) + dotd(b[i], g_pbt);
+  }
+  if (k >= 2) {
+    // g_mu_j = g_lam_j - g_lam_0; u = M^-1 g_mu;
+    // g_D = -(v u^T + (D u) mu^T), g_w0 -= D u
+    double gm0 = glam[1] - glam[0], gm1 = k == 3 ? glam[2] - glam[0] : 0.0;
+    double u0 = k == 3 ? A.i00 * gm0 + A.i01 * gm1 : A.i00 * gm0;
+    double u1 = k == 3 ? A.i01 * gm0 + A.i11 * gm1 : 0.0;
+    D3 Du = A.D0 * u0 + A.D1 * u1;
+    D3 gD0 = -(v * u0 + Du * A.mu0);
+    D3 gD1 = -(v * u1 + Du * A.mu1);
+    gw[0] = (gw[0] - Du) - gD0;
+    gw[1] = gw[1] + gD0;
+    if (k == 3) {
+      gw[2] = gw[2] + gD1;
+      gw[0] = gw[0] - gD1;
+    }
+  }
+  D3 gpA{0.0, 0.0, 0.0}, gpB{0.0, 0.0, 0.0};
+  double gqa[4] = {0, 0, 0, 0}, gqb[4] = {0, 0, 0, 0};
+  for (int i = 0; i < 3; ++i) {
+    if (i >= k) break;
+    gpA = gpA + gw[i];
+    qrot_vjpd(pr.qa, xa[i], gw[i], gqa);
+    D3 gbi = gb[i] - gw[i];
+    gpB = gpB + gbi;
+    qrot_vjpd(pr.qb, xb[i], gbi, gqb);
+  }
+  G.pa = G.pa + V3{(float)gpA.x, (float)gpA.y, (float)gpA.z};
+  G.pb = G.pb + V3{(float)gpB.x, (float)gpB.y, (float)gpB.z};
+  for (int c = 0; c < 4; ++c) {
+    G.qa[c] += (float)gqa[c];
+    G.qb[c] += (float)gqb[c];
+  }
+}
+
+// EPA arm in double: the reported face rebuilt from its exact local support
+// points, depth = n . w_a + margins with n = normalize((w_b - w_a) x
+// (w_c - w_a)), witness pa = supportA(n) (its support point chosen by the
+// forward normal n_fwd), pb = pa - n depth.
+BCH_HD void epa_vjp(const Pair& pr, V3 n_fwd, const V3* xa_f, const V3* xb_f, const float* g10, PoseGrad& G) {
+  D3 xa[3], xb[3], w[3];
+  for (int i = 0; i < 3; ++i) {
+    xa[i] = d3(xa_f[i]);
+    xb[i] = d3(xb_f[i]);
+    w[i] = (qrotd(pr.qa, xa[i]) + d3(pr.pa)) - (qrotd(pr.qb, xb[i]) + d3(pr.pb));
+  }
+  D3 e1 = w[1] - w[0], e2 = w[2] - w[0];
+  D3 cc = crossd(e1, e2);
+  double cl = sqrt(dotd(cc, cc));
+  D3 n = cc * (1.0 / cl);
+  double depth = (dotd(n, w[0]) + (double)pr.a.margin) + (double)pr.b.margin;
+  D3 gn{g10[1], g10[2], g10[3]}, gpa{g10[4], g10[5], g10[6]}, gpb{g10[7], g10[8], g10[9]};
+  D3 g_pat = gpa + gpb;
+  D3 g_nt = gn - gpb * depth;
+  double g_depth = -(double)g10[0] - dotd(n, gpb);
+  D3 gpA = g_pat, gpB{0.0, 0.0, 0.0};
+  double gqa[4] = {0, 0, 0, 0}, gqb[4] = {0, 0, 0, 0};
+  // witness on A
+  const ShapeRef& s = pr.a;
+  V3 ld = qrot(qconj(pr.qa), n_fwd);
+  if (s.type == SPHERE || s.type == CAPSULE) {
+    V3 tip = s.type == CAPSULE ? V3{0.0f, ld.y >= 0.0f ? s.a1 : -s.a1, 0.0f} : splat(0.0f);
+    qrot_vjpd(pr.qa, d3(tip), g_pat, gqa);
+    double r = s.a0, ng = dotd(n, g_pat);
+    // qrot(q, r ld/|ld|) = r |q|^2 n: d/dq = 2 r (n.g) q, d/dn = r (I - n n^T)
+    gqa[0] += 2.0 * r * ng * pr.qa.x;
+    gqa[1] += 2.0 * r * ng * pr.qa.y;
+    gqa[2] += 2.0 * r * ng * pr.qa.z;
+    gqa[3] += 2.0 * r * ng * pr.qa.w;
+    g_nt = g_nt + (g_pat - n * ng) * r;
+  } else {
+    qrot_vjpd(pr.qa, d3(support_local(s, ld)), g_pat, gqa);
+  }
+  // plane distance and face normal
+  g_nt = g_nt + w[0] * g_depth;
+  D3 g_c = (g_nt - n * dotd(n, g_nt)) * (1.0 / cl);
+  D3 g_e1 = crossd(e2, g_c), g_e2 = crossd(g_c, e1);
+  D3 gw[3] = {(n * g_depth - g_e1) - g_e2, g_e1, g_e2};
+  for (int i = 0; i < 3; ++i) {
+    gpA = gpA + gw[i];
+    gpB = gpB - gw[i];
+    qrot_vjpd(pr.qa, xa[i], gw[i], gqa);
+    qrot_vjpd(pr.qb, xb[i], -gw[i], gqb);
+  }
+  G.pa = G.pa + V3{(float)gpA.x, (float)gpA.y, (float)gpA.z};
+  G.pb = G.pb + V3{(float)gpB.x, (float)gpB.y, (float)gpB.z};
+  for (int c = 0; c < 4; ++c) {
+    G.qa[c] += (float)gqa[c];
+    G.qb[c] += (float)gqb[c];
+  }
+}
+
+BCH_HD void store_grad(float* g14, const PoseGrad& G) {
+  g14[0] = G.pa.x; g14[1] = G.pa.y; g14[2] = G.pa.z;
+  for (int k = 0; k < 4; ++k) g14[3 + k] = G.qa[k];
+  g14[7] = G.pb.x; g14[8] = G.pb.y; g14[9] = G.pb.z;
+  for (int k = 0; k < 4; ++k) g14[10 + k] = G.qb[k];
+}
+
+BCH_HD void store_out(float* o10, const SdOut& o) {
+  o10[0] = o.sd;
+  o10[1] = o.n.x; o10[2] = o.n.y; o10[3] = o.n.z;
+  o10[4] = o.pa.x; o10[5] = o.pa.y; o10[6] = o.pa.z;
+  o10[7] = o.pb.x; o10[8] = o.pb.y; o10[9] = o.pb.z;
+}
+
+// The walk for every pair, G lanes per pair (hull support scans split over
+// the group).  BWD = false writes out[10P]; BWD = true reads gin[10P] and
+// writes grad[14P].  Overlapping cores are queued for the EPA kernel.
+template <int G, bool BWD>
+__global__ __launch_bounds__(256) void k_sd_walk(uint32_t P, uint32_t nbodies, const uint32_t* __restrict__ pairs,
+                                                 const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
+                                                 const float* __restrict__ gin, float* __restrict__ out,
+                                                 uint8_t* __restrict__ status, float4* __restrict__ epa_q,
+                                                 Counters* cnt) {
+  __shared__ float4 hstage[256 / G][2][64];
+  const uint32_t per_block = blockDim.x / G;
+  int grp = threadIdx.x / G, sub = threadIdx.x % G;
+  for (uint32_t base = blockIdx.x * per_block; base < P; base += gridDim.x * per_block) {
+    uint32_t t = base + grp;
+    bool need_epa = false;
+    GjkResult g;
+    uint32_t i = 0, j = 0;
+    if (t < P) {
+      i = pairs[2 * t];
+      j = pairs[2 * t + 1];
+    }
+    bool bad = t < P && (i >= nbodies || j >= nbodies);
+    if (bad && sub == 0) {
+      // a pair index outside the bodies: NaN outputs, error status
+      if (status) status[t] = 0xffu;
+      for (int k = 0; k < (BWD ? 14 : 10); ++k) out[(BWD ? 14 : 10) * (uint64_t)t + k] = NAN;
+    }
+    if (t < P && !bad) {
+      Pair pr = make_pair(bodies[i], bodies[j], hull);
+      if (pr.a.type == HULL && pr.a.nhull <= 64) {
+        for (int v = sub; v < pr.a.nhull; v += G) hstage[grp][0][v] = pr.a.hull[v];
+        pr.a.hull = hstage[grp][0];
+      }
+      if (pr.b.type == HULL && pr.b.nhull <= 64) {
+        for (int v = sub; v < pr.b.nhull; v += G) hstage[grp][1][v] = pr.b.hull[v];
+        pr.b.hull = hstage[grp][1];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      g = gjk_contact_t<GroupSupportIdx<G>, true>(pr, false, splat(0.0f), INFINITY, GroupSupportIdx<G>{&pr, sub});
+      need_epa = g.kind == 2;
+      if (sub == 0 && !need_epa) {
+        bool band = g.kind == 1 || g.capped;
+        uint32_t st = band ? (g.kind == 1 ? SD_WALK : SD_CAPPED) : SD_TOUCH;
+        SdOut o;
+        if (band) {
+          if (g.kind == 1) {
+            o.sd = g.distance;
+            o.n = g.normal;
+            o.pb = g.point_on_b;
+            o.pa = o.pb - o.n * o.sd;
+          } else {
+            o = band_outputs(pr, g.last_v, g.last_pb);
+          }
+        } else {
+          o.sd = -(pr.a.margin + pr.b.margin);
+          o.n = o.pa = o.pb = splat(0.0f);
+        }
+        if (status) status[t] = (uint8_t)st;
+        if (!BWD) {
+          store_out(out + 10 * (uint64_t)t, o);
+        } else {
+          PoseGrad Gr;
+          Gr.zero();
+          if (band) band_vjp(pr, g.act, gin + 10 * (uint64_t)t, Gr);
+          store_grad(out + 14 * (uint64_t)t, Gr);
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    bool enq = need_epa && sub == 0;
+    uint64_t mask = __ballot(enq);
+    if (mask) {
+      uint32_t lane = threadIdx.x & 63;
+      uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+      uint32_t qb = 0;
+      if (lane == leader) qb = atomicAdd(&cnt->n_epa, (uint32_t)__popcll(mask));
+      qb = __shfl(qb, leader);
+      if (enq) {
+        uint32_t q = qb + __popcll(mask & ((1ull << lane) - 1ull));
+        epa_q[4 * (uint64_t)q] = make_float4(g.t0.x, g.t0.y, g.t0.z, u2f(t));
+        epa_q[4 * (uint64_t)q + 1] = make_float4(g.t1.x, g.t1.y, g.t1.z, 0.0f);
+        epa_q[4 * (uint64_t)q + 2] = make_float4(g.t2.x, g.t2.y, g.t2.z, 0.0f);
+        epa_q[4 * (uint64_t)q + 3] = make_float4(g.t3.x, g.t3.y, g.t3.z, 0.0f);
+      }
+    }
+  }
+}
+
+// Split Minkowski vertex w into a support point of A's core (local xa) and
+// one of B's (local xb) with (qrot(qa, xa) + pa) - (qrot(qb, xb) + pb) == w.
+// Lane l tries A candidates l, l + 64, ...; B candidates are looped.
+__device__ __forceinline__ bool wave_split(const Pair& pr, V3 w, V3& xa, V3& xb, int lane) {
+  int na = core_set_size(pr.a), nb = core_set_size(pr.b);
+  for (int ca = 0; ca < na; ca += 64) {
+    int ia = ca + lane;
+    bool va = ia < na;
+    V3 la = va ? core_set_point(pr.a, ia) : splat(0.0f);
+    V3 wa = qrot(pr.qa, la) + pr.pa;
+    for (int ib = 0; ib < nb; ++ib) {
+      V3 lb = core_set_point(pr.b, ib);
+      V3 wb = qrot(pr.qb, lb) + pr.pb;
+      V3 d = wa - wb;
+      uint64_t hit = __ballot(va && d.x == w.x && d.y == w.y && d.z == w.z);
+      if (hit) {
+        int l = ctz64(hit);
+        xa = V3{rdl_f(la.x, l), rdl_f(la.y, l), rdl_f(la.z, l)};
+        xb = lb;
+        return true;
+      }
+    }
+  }
+  return false;
+}
+
+// EPA for the queued pairs (one wavefront each) and the EPA arm of the
+// outputs (BWD = false) or of the VJP (BWD = true).
+template <bool BWD>
+__global__ __launch_bounds__(64) void k_sd_epa(const uint32_t* __restrict__ n_queue, const float4* __restrict__ epa_q,
+                                              const uint32_t* __restrict__ pairs, const BodyRec* __restrict__ bodies,
+                                              const float4* __restrict__ hull, const float* __restrict__ gin,
+                                              float* __restrict__ out, uint8_t* __restrict__ status) {
+  __shared__ EpaLds S;
+  const int lane = threadIdx.x & 63;
+  const uint32_t nq = *n_queue;
+  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+    float4 a = epa_q[4 * (uint64_t)q], b = epa_q[4 * (uint64_t)q + 1], c = epa_q[4 * (uint64_t)q + 2],
+           d = epa_q[4 * (uint64_t)q + 3];
+    uint32_t t = f2u(a.w);
+    uint32_t i = pairs[2 * t], j = pairs[2 * t + 1];
+    EpaPair ep;
+    Pair& pr = ep.pr;
+    pr = make_pair(bodies[i], bodies[j], hull);
+    ep.ha.load(pr.a, lane);
+    ep.hb.load(pr.b, lane);
+    EpaResult e = wave_epa(ep, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, V3{d.x, d.y, d.z}, S, lane);
+    float depth = (e.depth + pr.a.margin) + pr.b.margin;
+    SdOut o;
+    o.sd = -depth;
+    o.n = e.n;
+    o.pa = wave_support_world(pr.a, ep.ha, pr.pa, pr.qa, e.n, lane, false);
+    o.pb = o.pa - e.n * depth;
+    uint32_t st = e.status == 0 ? SD_EPA : SD_EPA_UNCONVERGED;
+    if (!BWD) {
+      if (lane == 0) store_out(out + 10 * (uint64_t)t, o);
+    } else {
+      PoseGrad Gr;
+      Gr.zero();
+      if (e.face >= 0) {
+        uint32_t abc = S.abc[e.face];
+        V3 w0 = lv(S, abc & 0xffu), w1 = lv(S, (abc >> 8) & 0xffu), w2 = lv(S, (abc >> 16) & 0xffu);
+        float fd_plane = dot(w0, e.n);
+        if (!(fd_plane == e.depth)) st |= SD_FLAG_EDGE;
+        V3 xa[3], xb[3];
+        bool ok = wave_split(pr, w0, xa[0], xb[0], lane);
+        ok = wave_split(pr, w1, xa[1], xb[1], lane) && ok;
+        ok = wave_split(pr, w2, xa[2], xb[2], lane) && ok;
+        if (!ok) st |= SD_FLAG_SPLIT;
+        if (ok) epa_vjp(pr, e.n, xa, xb, gin + 10 * (uint64_t)t, Gr);
+      }
+      if (lane == 0) store_grad(out + 14 * (uint64_t)t, Gr);
+    }
+    if (lane == 0 && status) status[t] = (uint8_t)st;
+    wave_sync();
+  }
+}
+
+}  // namespace bch

@@ -502,6 +502,7 @@ __device__ __forceinline__ EpaResult wave_epa(const EpaPair& ep, V3 t0, V3 t1, V
     R.status = 1;
     R.n = splat(0.0f);
     R.depth = 0.0f;
+    R.face = -1;
     return R;
   }
   int nverts = 4;
@@ -533,6 +534,7 @@ __device__ __forceinline__ EpaResult wave_epa(const EpaPair& ep, V3 t0, V3 t1, V
         R.status = 0;
         R.n = bn;
         R.depth = bd;
+        R.face = bi;
         return R;
       }
       if (nverts >= BCH_WEPA_V) {
@@ -554,6 +556,7 @@ __device__ __forceinline__ EpaResult wave_epa(const EpaPair& ep, V3 t0, V3 t1, V
   R.status = status;
   R.n = V3{S.nx[bi], S.ny[bi], S.nz[bi]};
   R.depth = S.dist[bi];
+  R.face = bi;
   return R;
 }
 

@@ -723,6 +723,46 @@ __device__ __forceinline__ V3 group_core_support_world(const ShapeRef& s, V3 p, 
   return qrot(q, l) + p;
 }
 
+// The same support with the index of the local support point (hull vertex,
+// core box corner by sign bits, capsule tip, sphere centre), for walks whose
+// adjoint rebuilds the support points.
+template <int G>
+__device__ __forceinline__ V3 group_core_support_world_idx(const ShapeRef& s, V3 p, Q4 q, V3 d, int sub, int& idx) {
+  V3 ld = qrot(qconj(q), d);
+  V3 l;
+  if (s.type == HULL) {
+    float bv = -INFINITY;
+    int bi = -1;
+    for (int v = sub; v < s.nhull; v += G) {
+      float dd = dot(hull_vertex(s.hull, v), ld);
+      if (dd > bv) {
+        bv = dd;
+        bi = v;
+      }
+    }
+    row_reduce<true, G>(bv, bi);
+    l = bi >= 0 ? hull_vertex(s.hull, bi) : splat(0.0f);
+    idx = bi >= 0 ? bi : 0;
+  } else {
+    l = core_support_local(s, ld);
+    idx = s.type == BOX ? ((ld.x >= 0.0f ? 0 : 1) | (ld.y >= 0.0f ? 0 : 2) | (ld.z >= 0.0f ? 0 : 4))
+                        : (s.type == CAPSULE ? (ld.y >= 0.0f ? 0 : 1) : 0);
+  }
+  return qrot(q, l) + p;
+}
+
+template <int G>
+struct GroupSupportIdx {
+  const Pair* pr;
+  int sub;
+  __device__ SV operator()(V3 d) const {
+    int ia, ib;
+    V3 a = group_core_support_world_idx<G>(pr->a, pr->pa, pr->qa, d, sub, ia);
+    V3 b = group_core_support_world_idx<G>(pr->b, pr->pb, pr->qb, -d, sub, ib);
+    return SV{a - b, b, ia, ib};
+  }
+};
+
 template <int G>
 struct GroupSupport {
   const Pair* pr;

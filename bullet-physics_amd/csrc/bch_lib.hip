@@ -14,6 +14,7 @@
 #include "../../include/bullet_collision_hip.h"
 #include "bch_kernels.hpp"
 #include "bch_epa_wave.hpp"
+#include "bch_adjoint.hpp"
 
 using namespace bch;
 
@@ -53,7 +54,8 @@ inline uint32_t blocks(uint64_t n, int b = kBlock) { return (uint32_t)((n + b - 
 
 struct bch_context {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // the stream every call is ordered on
+  hipStream_t own_stream = nullptr;  // the context's own stream
   std::string err;
   hipEvent_t ev[9] = {};
   float timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -405,10 +407,11 @@ int bch_create(int device, bch_context** out) {
   if (hipSetDevice(device) != hipSuccess) return BCH_NO_DEVICE;
   bch_context* c = new bch_context();
   c->device = device;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return BCH_HIP_ERROR;
   }
+  c->stream = c->own_stream;
   for (auto& e : c->ev) (void)hipEventCreate(&e);
   *out = c;
   return BCH_OK;
@@ -427,7 +430,7 @@ void bch_destroy(bch_context* c) {
                    &c->tmp};
   for (DevBuf* b : all) b->release();
   for (auto& e : c->ev) (void)hipEventDestroy(e);
-  (void)hipStreamDestroy(c->stream);
+  (void)hipStreamDestroy(c->own_stream);
   delete c;
 }
 
@@ -442,6 +445,22 @@ int bch_synchronize(bch_context* c) {
 int bch_last_timings(bch_context* c, float* out_ms) {
   if (!c || !out_ms) return BCH_INVALID_ARG;
   for (int i = 0; i < 8; ++i) out_ms[i] = c->timings[i];
+  return BCH_OK;
+}
+
+int bch_set_stream(bch_context* c, void* stream) {
+  if (!c) return BCH_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->stream = (hipStream_t)stream;  // NULL is the device's null stream
+  return BCH_OK;
+}
+
+int bch_reset_stream(bch_context* c) {
+  if (!c) return BCH_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->stream = c->own_stream;
   return BCH_OK;
 }
 
@@ -773,6 +792,47 @@ int bch_world_collide(bch_context* c, float aabb_margin, uint64_t* out_pairs, ui
   if (out_pairs) *out_pairs = c->P;
   if (out_contacts) *out_contacts = c->C;
   return BCH_OK;
+}
+
+static int run_sd(bch_context* c, const uint32_t* d_pairs, uint64_t P, const float* d_gin, float* d_out,
+                  uint8_t* d_status, bool bwd) {
+  if (!c) return BCH_INVALID_ARG;
+  if (P && (!d_pairs || !d_out || (bwd && !d_gin))) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  if (P > 0xffffffffull) return fail(c, BCH_INVALID_ARG, "more than 2^32 - 1 pairs");
+  if (P && !c->n) return fail(c, BCH_INVALID_ARG, "no bodies: call bch_world_set_bodies first");
+  HIP_TRY(c, hipSetDevice(c->device));
+  ENSURE(c, c->epa_q, std::max<uint64_t>(P, 1) * 4 * sizeof(float4));
+  hipStream_t s = c->stream;
+  Counters* cnt = c->counters.as<Counters>();
+  hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
+  if (P) {
+    constexpr int G = 16;
+    uint32_t g = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(P, kBlock / G), 8192u));
+    if (bwd) {
+      hipLaunchKernelGGL((k_sd_walk<G, true>), dim3(g), dim3(kBlock), 0, s, (uint32_t)P, c->n, d_pairs,
+                         c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_gin, d_out, d_status,
+                         c->epa_q.as<float4>(), cnt);
+      hipLaunchKernelGGL(k_sd_epa<true>, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
+                         d_pairs, c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_gin, d_out, d_status);
+    } else {
+      hipLaunchKernelGGL((k_sd_walk<G, false>), dim3(g), dim3(kBlock), 0, s, (uint32_t)P, c->n, d_pairs,
+                         c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_gin, d_out, d_status,
+                         c->epa_q.as<float4>(), cnt);
+      hipLaunchKernelGGL(k_sd_epa<false>, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
+                         d_pairs, c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_gin, d_out, d_status);
+    }
+  }
+  HIP_TRY(c, hipGetLastError());
+  return BCH_OK;
+}
+
+int bch_sd_forward(bch_context* c, const uint32_t* d_pairs, uint64_t n_pairs, float* d_out, uint8_t* d_status) {
+  return run_sd(c, d_pairs, n_pairs, nullptr, d_out, d_status, false);
+}
+
+int bch_sd_backward(bch_context* c, const uint32_t* d_pairs, uint64_t n_pairs, const float* d_grad_out,
+                    float* d_grad_pose, uint8_t* d_status) {
+  return run_sd(c, d_pairs, n_pairs, d_grad_out, d_grad_pose, d_status, true);
 }
 
 int bch_world_routes(bch_context* c, uint8_t* routes) {

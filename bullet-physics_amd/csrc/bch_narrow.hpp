@@ -384,6 +384,9 @@ struct Pair {
 
 struct SV {
   V3 w, p_b;
+  // local support points the vertex came from (index into the shape's core
+  // support set); only the signed-distance walk fills and reads them
+  int ia = 0, ib = 0;
 };
 
 BCH_HD SV core_support(const Pair& pr, V3 d) {
@@ -451,7 +454,9 @@ struct Simp {
 };
 
 
-BCH_HD SV sel(bool c, const SV& a, const SV& b) { return SV{sel(c, a.w, b.w), sel(c, a.p_b, b.p_b)}; }
+BCH_HD SV sel(bool c, const SV& a, const SV& b) {
+  return SV{sel(c, a.w, b.w), sel(c, a.p_b, b.p_b), c ? a.ia : b.ia, c ? a.ib : b.ib};
+}
 
 // Append at position n (1..3): selects on every slot, so the simplex stays in
 // registers.
@@ -507,8 +512,8 @@ BCH_HD Reduce reduce(Simp& S) {
     float b0 = T0.w0, b1 = T0.w1, b2 = T0.w2;
     float best_v2 = INFINITY;
     for (int f = 1; f < 4; ++f) {
-      SV x{sel(f == 3, b.w, a.w), sel(f == 3, b.p_b, a.p_b)};
-      SV y{sel(f == 1, b.w, c.w), sel(f == 1, b.p_b, c.p_b)};
+      SV x = sel(f == 3, b, a);
+      SV y = sel(f == 1, b, c);
       SV z = d;
       Tri T = closest_tri(x.w, y.w, z.w);
       float v2 = len2(T.q);
@@ -679,16 +684,22 @@ struct GjkResult {
   float distance;
   V3 normal, point_on_b;
   V3 t0, t1, t2, t3;
+  // Signed-distance walks only (SD = true): the active simplex at a Band
+  // exit or at the iteration cap, and the last iterate at the cap.
+  bool capped;
+  V3 last_v, last_pb;
+  Simp act;
 };
 
 // gjk.rs:171-288 -- the margin-core distance walk.
 // The support is a functor SV(V3 dir): one lane's scan (core_support) or a
 // lane group's cooperative scan of a hull; every lane of a group runs the
 // walk on identical values.
-template <class SupF>
+template <class SupF, bool SD = false>
 BCH_HD GjkResult gjk_contact_t(const Pair& pr, bool has_seed, V3 seed, float threshold, SupF sup) {
   GjkResult R;
   R.kind = 0;
+  if (SD) R.capped = false;
   float margin_a = pr.a.margin, margin_b = pr.b.margin;
   float margin_sum = margin_a + margin_b;
   V3 center_dir = pr.pb - pr.pa;
@@ -742,11 +753,18 @@ BCH_HD GjkResult gjk_contact_t(const Pair& pr, bool has_seed, V3 seed, float thr
         R.distance = distance;
         R.normal = normal;
         R.point_on_b = r.p_b - normal * margin_b;
+        if (SD) R.act = S;
       }
       return R;
     }
+    if (SD) {
+      R.act = S;
+      R.last_v = v;
+      R.last_pb = r.p_b;
+    }
     simp_append(S, w);
   }
+  if (SD) R.capped = true;
   return R;
 }
 
@@ -782,6 +800,7 @@ struct EpaResult {
   int status;  // 0 converged, 1 degenerate, 2 iteration limit, 3 vertex limit, 4 store overflow
   V3 n;
   float depth;
+  int face;    // index of the reported face (wavefront EPA)
 };
 
 BCH_HD V3 ev(const EpaStore& st, int i) {

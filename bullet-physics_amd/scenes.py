@@ -236,3 +236,135 @@ def world_slice(sc: dict, w: int) -> dict:
         out["hull_first"] = np.zeros(b - a, np.uint32)
         out["hull_vertices"] = np.zeros((0, 3), np.float32)
     return out
+
+
+def _qrot_np(q: np.ndarray, v: np.ndarray) -> np.ndarray:
+    """Rotate v[..., 3] by unit quaternions q[..., 4] (x, y, z, w), float64."""
+    b, w = q[..., :3], q[..., 3:4]
+    return v * (w * w - np.sum(b * b, -1, keepdims=True)) + b * (2.0 * np.sum(v * b, -1, keepdims=True)) \
+        + np.cross(b, v) * (2.0 * w)
+
+
+def hull_pairs(n_pairs: int = 262_144, seed: int = 4, npts: int = 32, semi_lo: float = 0.3, semi_hi: float = 1.0,
+               gap: float = 0.2, spacing: float = 6.0, shape_mix: bool = False) -> dict:
+    """``n_pairs`` independent convex pairs (bodies 2k, 2k+1) for the signed
+    distance and its adjoint.
+
+    Every body is a hull of ``npts`` points on a random ellipsoid (semi-axes
+    U[semi_lo, semi_hi]) with a uniform random orientation.  Pair k sits at
+    its own grid cell; B is placed along a random unit direction u at the
+    position where B's support point along -u lies at A's support point along
+    u plus d u: d ~ +U[0, gap] for even k (then the signed distance is d:
+    the supports face each other across the plane normal to u) and -U[0, gap]
+    for odd k (overlapping, penetration at most |d|).  ``shape_mix`` makes every fourth A a box, sphere or
+    capsule instead (for the rounded-shape paths).  ``pairs`` holds the
+    (2k, 2k+1) index pairs."""
+    rng = Lcg(seed)
+    nb = 2 * n_pairs
+    axes = rng.range(semi_lo, semi_hi, 3 * nb).astype(np.float64).reshape(nb, 1, 3)
+    g = _gaussian(rng, 3 * npts * nb).reshape(nb, npts, 3)
+    d = g / np.maximum(np.linalg.norm(g, axis=2, keepdims=True), 1e-30)
+    verts = (d * axes).astype(np.float32)
+    quat = random_quaternions(rng, nb)
+    u = _gaussian(rng, 3 * n_pairs).reshape(n_pairs, 3)
+    u /= np.maximum(np.linalg.norm(u, axis=1, keepdims=True), 1e-30)
+    target = rng.unit(n_pairs).astype(np.float64) * gap
+    target = np.where(np.arange(n_pairs) % 2 == 0, target, -target)
+    sc = _empty(nb)
+    sc["shape_type"][:] = HULL
+    sc["hull_count"][:] = npts
+    sc["hull_first"][:] = (np.arange(nb) * npts).astype(np.uint32)
+    sc["hull_vertices"] = verts.reshape(-1, 3)
+    sc["orientation"] = quat
+    if shape_mix:
+        a_idx = np.arange(0, nb, 2)
+        pick = a_idx[(np.arange(n_pairs) % 4) == 3]
+        kind = rng.unit(pick.shape[0])
+        k3 = np.where(kind < 1 / 3, BOX, np.where(kind < 2 / 3, SPHERE, CAPSULE)).astype(np.uint8)
+        sc["shape_type"][pick] = k3
+        sc["hull_count"][pick] = 0
+        sc["hull_first"][pick] = 0
+        p = rng.range(0.25, 0.8, 3 * pick.shape[0]).reshape(-1, 3)
+        sc["shape_params"][pick, :3] = p
+    # world offsets (from the centre) of A's support point along u and B's
+    # along -u; B is placed so that they sit d apart along u
+    qa = quat[0::2].astype(np.float64)
+    qb = quat[1::2].astype(np.float64)
+    wa = _qrot_np(qa[:, None, :], verts[0::2].astype(np.float64))
+    wb = _qrot_np(qb[:, None, :], verts[1::2].astype(np.float64))
+    rows = np.arange(n_pairs)
+    sa = wa[rows, np.argmax(np.einsum("pvk,pk->pv", wa, u), axis=1)]
+    sb = wb[rows, np.argmax(np.einsum("pvk,pk->pv", wb, -u), axis=1)]
+    if shape_mix:
+        m = (np.arange(n_pairs) % 4) == 3
+        ta = sc["shape_type"][0::2]
+        par = sc["shape_params"][0::2].astype(np.float64)
+        qconj = qa * np.array([-1, -1, -1, 1.0])
+        ul = _qrot_np(qconj, u)  # u in A's frame
+        box = _qrot_np(qa, np.where(ul >= 0, 1.0, -1.0) * par[:, :3])
+        sph = u * par[:, :1]
+        tip = np.zeros((n_pairs, 3))
+        tip[:, 1] = np.where(ul[:, 1] >= 0, par[:, 1], -par[:, 1])
+        cap = _qrot_np(qa, tip) + u * par[:, :1]
+        sa = np.where((m & (ta == BOX))[:, None], box,
+                      np.where((m & (ta == SPHERE))[:, None], sph, np.where((m & (ta == CAPSULE))[:, None], cap, sa)))
+    k = np.arange(n_pairs)
+    side = max(1, int(round(n_pairs ** (1 / 3))) + 1)
+    cell = np.stack([k % side, (k // side) % side, k // (side * side)], axis=1).astype(np.float64) * spacing
+    # (spacing = 0 puts every pair at the origin: independent pairs need no
+    # room, and small coordinates keep f32 differences of nearby points exact)
+    pa = cell
+    pb = pa + sa + u * target[:, None] - sb
+    pos = np.empty((nb, 3), np.float32)
+    pos[0::2] = pa
+    pos[1::2] = pb
+    sc["position"] = pos
+    sc["pairs"] = np.stack([2 * k, 2 * k + 1], axis=1).astype(np.uint32)
+    sc["target"] = target.astype(np.float32)
+    return sc
+
+
+def batched_boxes_hulls(n_worlds: int = 1024, seed: int = 5, lo_count: int = 64, hi_count: int = 1024,
+                        fill: float = 0.35) -> dict:
+    """``n_worlds`` independent worlds of boxes and hulls (50/50), body count
+    log-uniform in [lo_count, hi_count], for the differentiable batch.
+
+    Boxes: half-extents U[0.25, 1.0] per axis; hulls: 16-64 points on
+    ellipsoids with semi-axes U[0.3, 1.2]; orientations uniform.  World w
+    fills its own cube (side chosen so that the bodies' bounding volume is
+    ``fill`` of it, which leaves many pairs interpenetrating) and the worlds
+    sit side by side along x, 10 m apart."""
+    rng = Lcg(seed)
+    u = rng.unit(n_worlds).astype(np.float64)
+    counts = np.floor(np.exp(math.log(lo_count) + u * (math.log(hi_count) - math.log(lo_count)))).astype(np.int64)
+    counts = counts.clip(lo_count, hi_count)
+    N = int(counts.sum())
+    sc = _empty(N)
+    kind = rng.unit(N) < np.float32(0.5)
+    ext = np.stack([rng.range(0.25, 1.0, N) for _ in range(3)], axis=1)
+    sc["shape_type"][:] = np.where(kind, BOX, HULL)
+    sc["shape_params"][:, :3] = np.where(kind[:, None], ext, 0.0)
+    hidx = np.nonzero(~kind)[0]
+    nh = hidx.shape[0]
+    npts = np.floor(rng.range(16.0, 65.0, nh)).astype(np.int64).clip(16, 64)
+    # every hull draws 64 ellipsoid points and keeps its first npts
+    axes = rng.range(0.3, 1.2, 3 * nh).astype(np.float64).reshape(nh, 1, 3)
+    g = _gaussian(rng, 3 * 64 * nh).reshape(nh, 64, 3)
+    d = g / np.maximum(np.linalg.norm(g, axis=2, keepdims=True), 1e-30)
+    pts = (d * axes).astype(np.float32)
+    keep = np.arange(64)[None, :] < npts[:, None]
+    sc["hull_vertices"] = pts[keep].reshape(-1, 3)
+    first = np.concatenate(([0], np.cumsum(npts)[:-1]))
+    sc["hull_first"][hidx] = first.astype(np.uint32)
+    sc["hull_count"][hidx] = npts.astype(np.uint32)
+    sc["orientation"] = random_quaternions(rng, N)
+    offs = np.concatenate(([0], np.cumsum(counts))).astype(np.uint32)
+    side = (counts * 8 * 0.6 ** 3 / fill) ** (1.0 / 3.0)
+    x0 = np.concatenate(([0.0], np.cumsum(side + 10.0)[:-1]))
+    wid = np.repeat(np.arange(n_worlds), counts)
+    u3 = rng.unit(3 * N).astype(np.float64).reshape(N, 3)
+    pos = u3 * side[wid, None]
+    pos[:, 0] += x0[wid]
+    sc["position"] = pos.astype(np.float32)
+    sc["world_offsets"] = offs
+    return sc

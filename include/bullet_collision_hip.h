@@ -158,6 +158,37 @@ int bch_world_results(bch_context* ctx, uint32_t* pairs, uint32_t* contact_offse
  * 6 convex-convex with a hull; +8 when the pair went on to EPA
  * (convex_convex.rs:117-154). */
 int bch_world_routes(bch_context* ctx, uint8_t* routes);
+/* ------------------------------------------------- signed distance -- */
+/* Signed distance of each pair of the bodies loaded by bch_world_set_bodies
+ * (at their current poses): the margin-core walk of gjk_contact
+ * (narrowphase/gjk.rs:171-288) with threshold = +inf, so a separated pair of
+ * any distance ends in its Band arm, and EPA plus the contact of
+ * convex_convex.rs:117-154 for overlapping cores.  A walk stopped by its
+ * iteration cap reports its last iterate like the Band arm.  All pointers
+ * are DEVICE pointers; the calls are ordered on the context stream and do
+ * not synchronise.
+ *   d_pairs  [2P] body indices (i, j): shape A = body i, shape B = body j
+ *   d_out    [10P] per pair: sd (> 0 separated, < 0 penetrating), unit normal
+ *            A -> B, witness on A, witness on B (pa = pb - n sd)
+ *   d_status [P] or NULL: 0 walk, 1 EPA, 2 capped walk, 3 touching cores (no
+ *            normal), 4 EPA unconverged; +8 EPA depth from an edge, +16 a
+ *            face vertex not split (backward only); 255 pair index out of
+ *            range (outputs NaN).
+ * No reference counterpart beyond the walks it reuses. */
+int bch_sd_forward(bch_context* ctx, const uint32_t* d_pairs, uint64_t n_pairs, float* d_out, uint8_t* d_status);
+/* Adjoint of bch_sd_forward at the same poses: d_grad_out [10P] = dL/d(out),
+ * d_grad_pose [14P] = dL/d(position A xyz, quaternion A xyzw, position B,
+ * quaternion B) per pair (the caller sums over pairs sharing a body).  The
+ * walks are recomputed, nothing is stored between the two calls. */
+int bch_sd_backward(bch_context* ctx, const uint32_t* d_pairs, uint64_t n_pairs, const float* d_grad_out,
+                    float* d_grad_pose, uint8_t* d_status);
+/* Order this context's work on `stream` (a hipStream_t of the same device,
+ * e.g. the caller's framework stream; NULL is the device's null stream).
+ * The previous stream is drained first. */
+int bch_set_stream(bch_context* ctx, void* stream);
+/* Return to the context's own stream (the previous one is drained first). */
+int bch_reset_stream(bch_context* ctx);
+
 /* Device pointers of the last pass's results (valid until the next pass). */
 int bch_world_results_device(bch_context* ctx, const uint32_t** d_pairs, const uint32_t** d_contact_offsets,
                              const float** d_contacts);

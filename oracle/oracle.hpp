@@ -747,6 +747,13 @@ struct GjkOut {
   V3<T> normal{0, 0, 0}, point_on_b{0, 0, 0};
   V3<T> simplex[4];
   int iterations = 0;
+  // Not in the reference (signed-distance callers only): the walk's last
+  // closest point v and B witness when it stopped at the 64-iteration cap.
+  bool capped = false;
+  V3<T> last_v{0, 0, 0}, last_pb{0, 0, 0};
+  // the active simplex (w, b) at a Band exit or at the cap
+  SupportVertex<T> active[4];
+  int n_active = 0;
 };
 template <class T>
 struct Reduce {
@@ -1122,15 +1129,22 @@ GjkOut<T> gjk_contact(const PairPose<T>& pr, const V3<T>* seed, T threshold) {
         out.distance = distance;
         out.normal = normal;
         out.point_on_b = r.p_b - normal * margin_b;
+        for (int i = 0; i < n; ++i) out.active[i] = buf[i];
+        out.n_active = n;
         return out;
       }
       out.kind = 0;
       return out;
     }
+    for (int i = 0; i < n; ++i) out.active[i] = buf[i];
+    out.n_active = n;
     buf[n] = w;
     ++n;
+    out.last_v = v;
+    out.last_pb = r.p_b;
   }
   out.kind = 0;
+  out.capped = true;
   return out;
 }
 
@@ -1270,6 +1284,7 @@ struct EpaOut {
   T depth;
   int iterations;
   int vertices;
+  V3<T> face[3];  // vertices of the reported face (a, b, c order)
 };
 
 // epa.rs:127-205
@@ -1280,7 +1295,7 @@ EpaOut<T> epa_penetration(const V3<T>* simplex, F support) {
   std::vector<Face<T>> faces;
   std::vector<std::pair<int, int>> boundary;
   build_initial_faces(simplex, faces);
-  if (faces.empty()) return EpaOut<T>{EPA_DEGENERATE, splat(T(0)), T(0), 0, 4};
+  if (faces.empty()) return EpaOut<T>{EPA_DEGENERATE, splat(T(0)), T(0), 0, 4, {}};
   int status = EPA_ITERATION_LIMIT;
   V3<T> dirs[6] = {mk(T(1), T(0), T(0)), mk(T(-1), T(0), T(0)), mk(T(0), T(1), T(0)),
                    mk(T(0), T(-1), T(0)), mk(T(0), T(0), T(1)), mk(T(0), T(0), T(-1))};
@@ -1303,14 +1318,17 @@ EpaOut<T> epa_penetration(const V3<T>* simplex, F support) {
     T bd = faces[bi].distance;
     V3<T> np = support(bn);
     T wdist = dot(bn, np) - bd;
-    if (wdist <= EPSILON) return EpaOut<T>{EPA_CONVERGED, bn, bd, it + 1, (int)verts.size()};
+    if (wdist <= EPSILON)
+      return EpaOut<T>{EPA_CONVERGED, bn, bd, it + 1, (int)verts.size(),
+                       {verts[faces[bi].a], verts[faces[bi].b], verts[faces[bi].c]}};
     if ((int)verts.size() >= 128) { status = EPA_VERTEX_LIMIT; break; }
     int nidx = (int)verts.size();
     verts.push_back(np);
     if (!expand(faces, verts, boundary, np, nidx)) { status = EPA_DEGENERATE; break; }
   }
   int bi = find_best(faces);
-  return EpaOut<T>{status, faces[bi].normal, faces[bi].distance, it, (int)verts.size()};
+  return EpaOut<T>{status, faces[bi].normal, faces[bi].distance, it, (int)verts.size(),
+                   {verts[faces[bi].a], verts[faces[bi].b], verts[faces[bi].c]}};
 }
 
 // ------------------------------------------------------ convex_convex.rs --
@@ -1333,6 +1351,240 @@ bool convex_convex(const PairPose<T>& pr, const V3<T>* seed, T threshold, Manifo
   V3<T> position = point_a - e.normal * depth;
   m.push(position, e.normal, depth);
   return true;
+}
+
+// ------------------------------------------------------ signed distance --
+// Not a reference function: the signed distance between two convex shapes
+// built from the reference's own walks.  The margin-core walk (gjk.rs:171-288)
+// runs with threshold = +inf, so a separated pair ends in its Band arm at any
+// distance; overlapping cores go to EPA as convex_convex.rs:117-154 does.  A
+// walk stopped by its 64-iteration cap reports its last iterate the way the
+// Band arm would.  Outputs: sd (> 0 separated), unit normal A -> B, witness
+// points on A and on B with pa = pb - n sd.
+enum SdStatus { SD_WALK = 0, SD_EPA = 1, SD_CAPPED = 2, SD_TOUCH = 3, SD_EPA_UNCONVERGED = 4 };
+template <class T>
+struct SdOut {
+  int status = SD_TOUCH;
+  T sd = 0;
+  V3<T> n{0, 0, 0}, pa{0, 0, 0}, pb{0, 0, 0};
+};
+template <class T>
+SdOut<T> signed_distance(const PairPose<T>& pr) {
+  SdOut<T> o;
+  T ma = pr.a->margin(), mb = pr.b->margin();
+  GjkOut<T> g = gjk_contact(pr, (const V3<T>*)nullptr, std::numeric_limits<T>::infinity());
+  if (g.kind == 1 || (g.kind == 0 && g.capped)) {
+    if (g.kind == 1) {
+      o.status = SD_WALK;
+      o.sd = g.distance;
+      o.n = g.normal;
+      o.pb = g.point_on_b;
+    } else {
+      T core_distance = std::sqrt(len2(g.last_v));
+      o.status = SD_CAPPED;
+      o.sd = core_distance - (ma + mb);
+      o.n = (-g.last_v) / core_distance;
+      o.pb = g.last_pb - o.n * mb;
+    }
+    o.pa = o.pb - o.n * o.sd;
+    return o;
+  }
+  if (g.kind == 2) {
+    auto sup = [&](V3<T> d) { return pr.core_support(d).w; };
+    EpaOut<T> e = epa_penetration<T>(g.simplex, sup);
+    T depth = (e.depth + ma) + mb;
+    o.status = e.status == EPA_CONVERGED ? SD_EPA : SD_EPA_UNCONVERGED;
+    o.sd = -depth;
+    o.n = e.normal;
+    o.pa = pr.a->support_world(pr.pa, pr.qa, e.normal);
+    o.pb = o.pa - e.normal * depth;
+    return o;
+  }
+  o.status = SD_TOUCH;
+  o.sd = -(ma + mb);
+  return o;
+}
+
+// ------------------------------------------------ signed distance adjoint --
+// Not a reference function: the vector-Jacobian product of signed_distance
+// with respect to both poses (position xyz, quaternion xyzw; 14 values), for
+// an f64 reference gradient.  The Band arm differentiates the affine
+// least-norm point of the active simplex; the EPA arm differentiates the
+// reported face's plane distance and normal; support points are fixed local
+// points placed by qrot(q, x) + p.  Pieces where a support feature changes
+// are not smooth; this is the derivative of the piece the pose is on.
+template <class T>
+void qrot_vjp(Quat<T> q, V3<T> x, V3<T> g, T gq[4]) {
+  V3<T> b = mk(q.x, q.y, q.z);
+  T w = q.w;
+  V3<T> gb = ((g * dot(x, b) + x * dot(b, g)) - b * dot(x, g) + cross(x, g) * w) * T(2);
+  gq[0] += gb.x;
+  gq[1] += gb.y;
+  gq[2] += gb.z;
+  gq[3] += T(2) * (w * dot(x, g) + dot(cross(b, x), g));
+}
+template <class T>
+V3<T> sd_to_local(Quat<T> q, V3<T> p, V3<T> world) {
+  T q2 = ((q.x * q.x + q.y * q.y) + q.z * q.z) + q.w * q.w;
+  return qrot(qconj(q), world - p) * (T(1) / (q2 * q2));
+}
+// the finite set a core support point is drawn from (local coordinates)
+template <class T>
+int core_set_size(const Shape<T>& s) {
+  return s.kind == HULL ? s.npts : (s.kind == BOX ? 8 : (s.kind == CAPSULE ? 2 : 1));
+}
+template <class T>
+V3<T> core_set_point(const Shape<T>& s, int i) {
+  if (s.kind == HULL) return s.pts[i];
+  if (s.kind == BOX) {
+    V3<T> h = mk(fmax_(s.half.x - s.margin(), T(0)), fmax_(s.half.y - s.margin(), T(0)),
+                 fmax_(s.half.z - s.margin(), T(0)));
+    return mk((i & 1) ? -h.x : h.x, (i & 2) ? -h.y : h.y, (i & 4) ? -h.z : h.z);
+  }
+  if (s.kind == CAPSULE) return mk(T(0), i ? -s.half_height : s.half_height, T(0));
+  return splat(T(0));
+}
+// split a Minkowski vertex into local support points (exact recomputation)
+template <class T>
+bool split_vertex(const PairPose<T>& pr, V3<T> w, V3<T>& xa, V3<T>& xb) {
+  for (int i = 0; i < core_set_size(*pr.a); ++i) {
+    V3<T> la = core_set_point(*pr.a, i);
+    V3<T> wa = qrot(pr.qa, la) + pr.pa;
+    for (int j = 0; j < core_set_size(*pr.b); ++j) {
+      V3<T> lb = core_set_point(*pr.b, j);
+      V3<T> d = wa - (qrot(pr.qb, lb) + pr.pb);
+      if (d.x == w.x && d.y == w.y && d.z == w.z) {
+        xa = la;
+        xb = lb;
+        return true;
+      }
+    }
+  }
+  return false;
+}
+// returns false when a face vertex could not be split (gradient left 0)
+template <class T>
+bool sd_vjp(const PairPose<T>& pr, const T* g10, T* g14) {
+  for (int k = 0; k < 14; ++k) g14[k] = T(0);
+  T ma = pr.a->margin(), mb = pr.b->margin();
+  V3<T> gpa_pos = splat(T(0)), gpb_pos = splat(T(0));
+  T gqa[4] = {0, 0, 0, 0}, gqb[4] = {0, 0, 0, 0};
+  V3<T> gn = mk(g10[1], g10[2], g10[3]), gpa = mk(g10[4], g10[5], g10[6]), gpb = mk(g10[7], g10[8], g10[9]);
+  GjkOut<T> g = gjk_contact(pr, (const V3<T>*)nullptr, std::numeric_limits<T>::infinity());
+  bool ok = true;
+  if (g.kind == 1 || (g.kind == 0 && g.capped)) {
+    int k = g.n_active;
+    V3<T> w[3], b[3];
+    for (int i = 0; i < k && i < 3; ++i) {
+      w[i] = g.active[i].w;
+      b[i] = g.active[i].p_b;
+    }
+    T lam[3] = {1, 0, 0}, mu0 = 0, mu1 = 0, i00 = 0, i01 = 0, i11 = 0;
+    V3<T> D0 = splat(T(0)), D1 = splat(T(0));
+    if (k == 2) {
+      D0 = w[1] - w[0];
+      i00 = T(1) / len2(D0);
+      mu0 = -dot(D0, w[0]) * i00;
+      lam[0] = T(1) - mu0;
+      lam[1] = mu0;
+    } else if (k == 3) {
+      D0 = w[1] - w[0];
+      D1 = w[2] - w[0];
+      T m00 = len2(D0), m01 = dot(D0, D1), m11 = len2(D1);
+      T det = m00 * m11 - m01 * m01;
+      i00 = m11 / det;
+      i01 = -m01 / det;
+      i11 = m00 / det;
+      T r0 = -dot(D0, w[0]), r1 = -dot(D1, w[0]);
+      mu0 = i00 * r0 + i01 * r1;
+      mu1 = i01 * r0 + i11 * r1;
+      lam[0] = (T(1) - mu0) - mu1;
+      lam[1] = mu0;
+      lam[2] = mu1;
+    }
+    V3<T> v = splat(T(0));
+    for (int i = 0; i < k; ++i) v = v + w[i] * lam[i];
+    T core = std::sqrt(len2(v));
+    V3<T> n = (-v) / core;
+    T sd = core - (ma + mb);
+    V3<T> g_pbt = gpb + gpa;
+    V3<T> g_nt = (gn - gpa * sd) - g_pbt * mb;
+    T g_sdt = g10[0] - dot(n, gpa);
+    V3<T> g_v = (v / core) * g_sdt - (g_nt - n * dot(n, g_nt)) / core;
+    V3<T> gw[3], gb[3];
+    T glam[3];
+    for (int i = 0; i < 3; ++i) {
+      gw[i] = g_v * lam[i];
+      gb[i] = g_pbt * lam[i];
+      glam[i] = i < k ? dot(w[i], g_v) + dot(b[i], g_pbt) : T(0);
+    }
+    if (k >= 2) {
+      T gm0 = glam[1] - glam[0], gm1 = k == 3 ? glam[2] - glam[0] : T(0);
+      T u0 = k == 3 ? i00 * gm0 + i01 * gm1 : i00 * gm0;
+      T u1 = k == 3 ? i01 * gm0 + i11 * gm1 : T(0);
+      V3<T> Du = D0 * u0 + D1 * u1;
+      V3<T> gD0 = -(v * u0 + Du * mu0), gD1 = -(v * u1 + Du * mu1);
+      gw[0] = ((gw[0] - Du) - gD0) - (k == 3 ? gD1 : splat(T(0)));
+      gw[1] = gw[1] + gD0;
+      if (k == 3) gw[2] = gw[2] + gD1;
+    }
+    for (int i = 0; i < k; ++i) {
+      V3<T> a_i = w[i] + b[i];
+      gpa_pos = gpa_pos + gw[i];
+      qrot_vjp(pr.qa, sd_to_local(pr.qa, pr.pa, a_i), gw[i], gqa);
+      V3<T> gbi = gb[i] - gw[i];
+      gpb_pos = gpb_pos + gbi;
+      qrot_vjp(pr.qb, sd_to_local(pr.qb, pr.pb, b[i]), gbi, gqb);
+    }
+  } else if (g.kind == 2) {
+    auto sup = [&](V3<T> d) { return pr.core_support(d).w; };
+    EpaOut<T> e = epa_penetration<T>(g.simplex, sup);
+    T depth = (e.depth + ma) + mb;
+    V3<T> n = e.normal;
+    V3<T> g_pat = gpa + gpb;
+    V3<T> g_nt = gn - gpb * depth;
+    T g_depth = -g10[0] - dot(n, gpb);
+    // witness on A: supportA(n), with the rounded shapes' n-dependence
+    gpa_pos = gpa_pos + g_pat;
+    V3<T> ld = qrot(qconj(pr.qa), n);
+    if (pr.a->kind == SPHERE || pr.a->kind == CAPSULE) {
+      V3<T> tip = pr.a->kind == CAPSULE ? mk(T(0), ld.y >= T(0) ? pr.a->half_height : -pr.a->half_height, T(0))
+                                        : splat(T(0));
+      qrot_vjp(pr.qa, tip, g_pat, gqa);
+      T r = pr.a->radius, ng = dot(n, g_pat);
+      gqa[0] += T(2) * r * ng * pr.qa.x;
+      gqa[1] += T(2) * r * ng * pr.qa.y;
+      gqa[2] += T(2) * r * ng * pr.qa.z;
+      gqa[3] += T(2) * r * ng * pr.qa.w;
+      g_nt = g_nt + (g_pat - n * ng) * r;
+    } else {
+      qrot_vjp(pr.qa, pr.a->support_local(ld), g_pat, gqa);
+    }
+    V3<T> xa[3], xb[3];
+    for (int i = 0; i < 3; ++i) ok = split_vertex(pr, e.face[i], xa[i], xb[i]) && ok;
+    if (ok) {
+      V3<T> w0 = e.face[0], w1 = e.face[1], w2 = e.face[2];
+      g_nt = g_nt + w0 * g_depth;
+      V3<T> e1 = w1 - w0, e2 = w2 - w0;
+      V3<T> cc = cross(e1, e2);
+      T cl = std::sqrt(len2(cc));
+      V3<T> nn = cc / cl;
+      V3<T> g_c = (g_nt - nn * dot(nn, g_nt)) / cl;
+      V3<T> g_e1 = cross(e2, g_c), g_e2 = cross(g_c, e1);
+      V3<T> gw[3] = {(n * g_depth - g_e1) - g_e2, g_e1, g_e2};
+      for (int i = 0; i < 3; ++i) {
+        gpa_pos = gpa_pos + gw[i];
+        gpb_pos = gpb_pos - gw[i];
+        qrot_vjp(pr.qa, xa[i], gw[i], gqa);
+        qrot_vjp(pr.qb, xb[i], -gw[i], gqb);
+      }
+    }
+  }
+  g14[0] = gpa_pos.x; g14[1] = gpa_pos.y; g14[2] = gpa_pos.z;
+  for (int k = 0; k < 4; ++k) g14[3 + k] = gqa[k];
+  g14[7] = gpb_pos.x; g14[8] = gpb_pos.y; g14[9] = gpb_pos.z;
+  for (int k = 0; k < 4; ++k) g14[10 + k] = gqb[k];
+  return ok;
 }
 
 // ---------------------------------------------------------- dispatch ----

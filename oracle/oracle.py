@@ -60,6 +60,12 @@ def lib():
         scp = C.POINTER(OrScene)
         L.oracle_scene_aabbs_f32.argtypes = [scp, C.c_float, C.c_void_p, C.c_void_p]
         L.oracle_scene_thresholds_f32.argtypes = [scp, C.c_void_p]
+        L.oracle_sd_pairs_f32.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+        L.oracle_sd_pairs_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.oracle_sd_f64.argtypes = [C.c_void_p] * 7
+        L.oracle_sd_vjp_pairs_f64.restype = C.c_int64
+        L.oracle_sd_vjp_pairs_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+        L.oracle_sd_f32.argtypes = [C.c_void_p] * 7
         L.oracle_raycast_candidates.restype = C.c_int64
         L.oracle_raycast_candidates.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                                 C.c_void_p, C.c_int64]
@@ -310,6 +316,53 @@ def collide_pairs(scene, pairs, seeds=None, f64=False):
     lib().oracle_collide_pairs_f32(C.byref(r.c), pairs.ctypes.data, p, None if sd is None else sd.ctypes.data,
                                    cnt.ctypes.data, pts.ctypes.data, path.ctypes.data)
     return cnt, pts, path
+
+
+SD_WALK, SD_EPA, SD_CAPPED, SD_TOUCH, SD_EPA_UNCONVERGED = 0, 1, 2, 3, 4
+
+
+def signed_distance(a: Shape, pa, qa, b: Shape, pb, qb, f64=False):
+    """signed_distance (oracle.hpp) of one pair -> (out[10] = sd, n3, pa3, pb3; status)."""
+    ac, bc = a.c(), b.c()
+    if f64:
+        out = (C.c_double * 10)()
+        st = lib().oracle_sd_f64(C.byref(ac), _f64(pa, 3), _f64(qa, 4), C.byref(bc), _f64(pb, 3), _f64(qb, 4), out)
+        return np.array(out[:], np.float64), st
+    out = (C.c_float * 10)()
+    st = lib().oracle_sd_f32(C.byref(ac), _f32(pa, 3), _f32(qa, 4), C.byref(bc), _f32(pb, 3), _f32(qb, 4), out)
+    return np.array(out[:], np.float32), st
+
+
+def sd_pairs(scene, pairs, pose14=None, f64=False):
+    """signed_distance over scene pairs -> (out[P, 10], status[P]).  pose14
+    (float64 [P, 14]: pos_a, quat_a, pos_b, quat_b) overrides the scene
+    poses (f64 only)."""
+    r = _SceneRef(scene)
+    pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+    p = pairs.shape[0]
+    st = np.zeros(p, np.int8)
+    if f64:
+        out = np.zeros((p, 10), np.float64)
+        ps = None if pose14 is None else np.ascontiguousarray(pose14, np.float64)
+        lib().oracle_sd_pairs_f64(C.byref(r.c), pairs.ctypes.data, p, None if ps is None else ps.ctypes.data,
+                                  out.ctypes.data, st.ctypes.data)
+        return out, st
+    assert pose14 is None
+    out = np.zeros((p, 10), np.float32)
+    lib().oracle_sd_pairs_f32(C.byref(r.c), pairs.ctypes.data, p, out.ctypes.data, st.ctypes.data)
+    return out, st
+
+
+def sd_vjp_pairs(scene, pairs, g10):
+    """f64 reference adjoint of sd_pairs at the scene poses: g10 [P, 10] ->
+    (g14 [P, 14], number of EPA faces not split)."""
+    r = _SceneRef(scene)
+    pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+    g = np.ascontiguousarray(g10, np.float64)
+    out = np.zeros((pairs.shape[0], 14), np.float64)
+    fails = lib().oracle_sd_vjp_pairs_f64(C.byref(r.c), pairs.ctypes.data, pairs.shape[0], g.ctypes.data,
+                                          out.ctypes.data)
+    return out, fails
 
 
 def collision_pass(scene, margin):

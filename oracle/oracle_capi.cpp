@@ -64,6 +64,8 @@ Shape<T> from_desc(const or_shape* s, std::vector<V3<T>>& store) {
 
 template <class T> V3<T> v3p(const float* p) { return mk((T)p[0], (T)p[1], (T)p[2]); }
 template <class T> Quat<T> q4p(const float* q) { return Quat<T>{(T)q[0], (T)q[1], (T)q[2], (T)q[3]}; }
+template <class T> V3<T> v3p(const double* p) { return mk((T)p[0], (T)p[1], (T)p[2]); }
+template <class T> Quat<T> q4p(const double* q) { return Quat<T>{(T)q[0], (T)q[1], (T)q[2], (T)q[3]}; }
 template <class T> V3<T> v3d(const double* p) { return mk((T)p[0], (T)p[1], (T)p[2]); }
 template <class T> Quat<T> q4d(const double* q) { return Quat<T>{(T)q[0], (T)q[1], (T)q[2], (T)q[3]}; }
 
@@ -208,6 +210,28 @@ int oracle_convex_convex_f32(const or_shape* a, const float* pa, const float* qa
   if (path) *path = pth;
   write_manifold(m, out7);
   return m.n;
+}
+
+// signed_distance (oracle.hpp) of one pair: out10 = sd, n3, pa3, pb3;
+// returns the SdStatus.
+extern "C++" template <class T>
+int sd_one(const or_shape* a, const T* pa, const T* qa, const or_shape* b, const T* pb, const T* qb,
+                  T* out10) {
+  std::vector<V3<T>> sa_, sb_;
+  Shape<T> sa = from_desc<T>(a, sa_), sb = from_desc<T>(b, sb_);
+  PairPose<T> pr{&sa, v3p<T>(pa), q4p<T>(qa), &sb, v3p<T>(pb), q4p<T>(qb)};
+  SdOut<T> o = signed_distance(pr);
+  T v[10] = {o.sd, o.n.x, o.n.y, o.n.z, o.pa.x, o.pa.y, o.pa.z, o.pb.x, o.pb.y, o.pb.z};
+  for (int k = 0; k < 10; ++k) out10[k] = v[k];
+  return o.status;
+}
+int oracle_sd_f32(const or_shape* a, const float* pa, const float* qa, const or_shape* b, const float* pb,
+                  const float* qb, float* out10) {
+  return sd_one<float>(a, pa, qa, b, pb, qb, out10);
+}
+int oracle_sd_f64(const or_shape* a, const double* pa, const double* qa, const or_shape* b, const double* pb,
+                  const double* qb, double* out10) {
+  return sd_one<double>(a, pa, qa, b, pb, qb, out10);
 }
 
 // gjk_contact (gjk.rs:171-288): returns 0 Separated / 1 Band / 2 Overlap.
@@ -378,6 +402,71 @@ void oracle_collide_pairs_f32(const or_scene* sc, const uint32_t* pairs, int64_t
     if (out_path) out_path[k] = (int8_t)pth;
     if (out_pts) write_manifold(m, out_pts + 28 * k);
   }
+}
+
+// signed_distance over scene pairs.  pose14 (per pair: pos_a3 quat_a4 pos_b3
+// quat_b4) overrides the scene poses when given (finite differences).
+extern "C++" template <class T, class P>
+void sd_pairs(const or_scene* sc, const uint32_t* pairs, int64_t npairs, const P* pose14, T* out10,
+                     int8_t* status) {
+  SceneShapes<T> ss;
+  ss.build(sc);
+  for (int64_t k = 0; k < npairs; ++k) {
+    uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
+    PairPose<T> pr{&ss.shapes[i], ss.pos[i], ss.quat[i], &ss.shapes[j], ss.pos[j], ss.quat[j]};
+    if (pose14) {
+      const P* q = pose14 + 14 * k;
+      pr.pa = v3p<T>(q);
+      pr.qa = q4p<T>(q + 3);
+      pr.pb = v3p<T>(q + 7);
+      pr.qb = q4p<T>(q + 10);
+    }
+    SdOut<T> o = signed_distance(pr);
+    T v[10] = {o.sd, o.n.x, o.n.y, o.n.z, o.pa.x, o.pa.y, o.pa.z, o.pb.x, o.pb.y, o.pb.z};
+    for (int c = 0; c < 10; ++c) out10[10 * k + c] = v[c];
+    if (status) status[k] = (int8_t)o.status;
+  }
+}
+// The walk's active simplex (Band or capped) of one scene pair: returns its
+// size k (0 when the walk did not end in that arm) and w[3k], b[3k].
+extern "C++" template <class T>
+int sd_active(const or_scene* sc, uint32_t i, uint32_t j, double* w, double* b) {
+  SceneShapes<T> ss;
+  ss.build(sc);
+  PairPose<T> pr{&ss.shapes[i], ss.pos[i], ss.quat[i], &ss.shapes[j], ss.pos[j], ss.quat[j]};
+  GjkOut<T> g = gjk_contact(pr, (const V3<T>*)nullptr, std::numeric_limits<T>::infinity());
+  if (!(g.kind == 1 || (g.kind == 0 && g.capped))) return 0;
+  for (int k = 0; k < g.n_active; ++k) {
+    w[3 * k] = g.active[k].w.x; w[3 * k + 1] = g.active[k].w.y; w[3 * k + 2] = g.active[k].w.z;
+    b[3 * k] = g.active[k].p_b.x; b[3 * k + 1] = g.active[k].p_b.y; b[3 * k + 2] = g.active[k].p_b.z;
+  }
+  return g.n_active;
+}
+int oracle_sd_active(const or_scene* sc, uint32_t i, uint32_t j, int f64, double* w, double* b) {
+  return f64 ? sd_active<double>(sc, i, j, w, b) : sd_active<float>(sc, i, j, w, b);
+}
+
+// sd_vjp over scene pairs at the scene poses (f64): g10 [10P] -> g14 [14P];
+// returns the number of pairs whose EPA face could not be split.
+int64_t oracle_sd_vjp_pairs_f64(const or_scene* sc, const uint32_t* pairs, int64_t npairs, const double* g10,
+                                double* g14) {
+  SceneShapes<double> ss;
+  ss.build(sc);
+  int64_t fails = 0;
+  for (int64_t k = 0; k < npairs; ++k) {
+    uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
+    PairPose<double> pr{&ss.shapes[i], ss.pos[i], ss.quat[i], &ss.shapes[j], ss.pos[j], ss.quat[j]};
+    if (!sd_vjp(pr, g10 + 10 * k, g14 + 14 * k)) ++fails;
+  }
+  return fails;
+}
+
+void oracle_sd_pairs_f32(const or_scene* sc, const uint32_t* pairs, int64_t npairs, float* out10, int8_t* status) {
+  sd_pairs<float, float>(sc, pairs, npairs, (const float*)nullptr, out10, status);
+}
+void oracle_sd_pairs_f64(const or_scene* sc, const uint32_t* pairs, int64_t npairs, const double* pose14,
+                         double* out10, int8_t* status) {
+  sd_pairs<double, double>(sc, pairs, npairs, pose14, out10, status);
 }
 
 void oracle_collide_pairs_f64(const or_scene* sc, const uint32_t* pairs, int64_t npairs, uint8_t* out_count,

@@ -1,0 +1,55 @@
+"""Finite-difference helpers for the signed-distance adjoint tests (f64,
+through the oracle)."""
+import numpy as np
+
+
+def pose14(sc):
+    pos = sc["position"].astype(np.float64)
+    q = sc["orientation"].astype(np.float64)
+    p = sc["pairs"].astype(np.int64)
+    return np.concatenate([pos[p[:, 0]], q[p[:, 0]], pos[p[:, 1]], q[p[:, 1]]], axis=1)
+
+
+def fd(oracle, sc, g10, h):
+    """f64 central differences of the oracle's signed distance: VJP g10^T J
+    [P, 14] and the second differences |g.(f+ - 2 f0 + f-)| / h per pose
+    coordinate [P, 14] (proportional to h where the function is smooth)."""
+    base = pose14(sc)
+    f0, st0 = oracle.sd_pairs(sc, sc["pairs"], base, f64=True)
+    P = base.shape[0]
+    vjp = np.zeros((P, 14))
+    jump = np.zeros((P, 14))
+    for c in range(14):
+        xp = base.copy()
+        xm = base.copy()
+        xp[:, c] += h
+        xm[:, c] -= h
+        fp, _ = oracle.sd_pairs(sc, sc["pairs"], xp, f64=True)
+        fm, _ = oracle.sd_pairs(sc, sc["pairs"], xm, f64=True)
+        vjp[:, c] = np.sum(g10 * (fp - fm), axis=1) / (2 * h)
+        jump[:, c] = np.abs(np.sum(g10 * (fp - 2 * f0 + fm), axis=1)) / h
+    return vjp, jump, f0, st0
+
+
+def witness_margin(sc, n):
+    """How far A's support along n is from a tie (projection gap between the
+    best local support point and the next one it could switch to)."""
+    p = sc["pairs"].astype(np.int64)
+    ia = p[:, 0]
+    q = sc["orientation"][ia].astype(np.float64)
+    b, w = q[:, :3], q[:, 3:4]
+    nl = (n * (w * w - np.sum(b * b, 1, keepdims=True)) + (-b) * (2 * np.sum(n * -b, 1, keepdims=True))
+          + np.cross(-b, n) * (2 * w))  # n in A's frame (rotation by the conjugate)
+    t = sc["shape_type"][ia]
+    par = sc["shape_params"][ia].astype(np.float64)
+    out = np.full(p.shape[0], np.inf)
+    for k in range(p.shape[0]):
+        if t[k] == 3:
+            v = sc["hull_vertices"][sc["hull_first"][ia[k]]: sc["hull_first"][ia[k]] + sc["hull_count"][ia[k]]]
+            pr = np.sort(v.astype(np.float64) @ nl[k])[::-1]
+            out[k] = pr[0] - pr[1]
+        elif t[k] == 1:
+            out[k] = 2 * np.min(np.abs(nl[k]) * par[k, :3])
+        elif t[k] == 2:
+            out[k] = 2 * abs(nl[k, 1]) * par[k, 1]
+    return out
