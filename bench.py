@@ -118,6 +118,71 @@ def batched_vs_plain(bpa, n_worlds, margin, steps):
             "results_equal_counts": bool(Pp == Pb and Cp == Cb)}
 
 
+def differentiation(bpa, steps, n_pairs=262_144):
+    """262,144 hull pairs (32 points each, half separated by U[0, 0.2] m, half
+    penetrating): signed distance forward and adjoint, device buffers."""
+    import torch
+    t0 = time.time()
+    sc = bpa.scenes.hull_pairs(n_pairs, 4)
+    log(f"hull pairs: {n_pairs} in {time.time() - t0:.1f} s")
+    w = bpa.CollisionWorld(sc, batched=False)
+    w.ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    pairs = torch.from_numpy(sc["pairs"].astype(np.int32)).cuda()
+    P = n_pairs
+    out = torch.empty((P, 10), device="cuda")
+    st = torch.empty(P, dtype=torch.uint8, device="cuda")
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    g10 = torch.randn((P, 10), device="cuda", generator=gen)
+    g14 = torch.empty((P, 14), device="cuda")
+    w.sd_forward(pairs.data_ptr(), P, out.data_ptr(), st.data_ptr())
+    w.sd_backward(pairs.data_ptr(), P, g10.data_ptr(), g14.data_ptr(), None)
+    torch.cuda.synchronize()
+    k = max(steps, 1)
+    t1 = time.perf_counter()
+    for _ in range(k):
+        w.sd_forward(pairs.data_ptr(), P, out.data_ptr(), st.data_ptr())
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(k):
+        w.sd_backward(pairs.data_ptr(), P, g10.data_ptr(), g14.data_ptr(), None)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    fwd = (t2 - t1) / k
+    bwd = (t3 - t2) / k
+    status = np.bincount(st.cpu().numpy() & 7, minlength=5)
+    w.ctx.set_stream(None)
+    return {"workload": f"{P} convex-hull pairs, 32 points each, half separated by U[0,0.2] m, half penetrating",
+            "forward_ms": round(fwd * 1e3, 3), "backward_ms": round(bwd * 1e3, 3),
+            "pairs_per_s_forward": round(P / fwd, 1), "pairs_per_s_forward_backward": round(P / (fwd + bwd), 1),
+            "routes": {"walk": int(status[0]), "epa": int(status[1]), "capped": int(status[2])}}
+
+
+def depenetration(bpa, steps=100):
+    """1,024 worlds of 64-1,024 boxes and hulls: penetration penalty
+    sum(max(0, -sd)^2) over the pairs of each world pass, gradient steps on
+    all poses (broadphase, narrowphase signed distance and adjoint per step)."""
+    import torch
+    diff = importlib.import_module("bullet-physics_amd.diff")
+    t0 = time.time()
+    sc = bpa.scenes.batched_boxes_hulls(1024, 5)
+    log(f"depenetration scene: {sc['shape_type'].shape[0]} bodies in {time.time() - t0:.1f} s")
+    w = bpa.CollisionWorld(sc)
+    pos = torch.from_numpy(sc["position"]).cuda()
+    quat = torch.from_numpy(sc["orientation"]).cuda()
+    diff.depenetrate(w, pos, quat, steps=1)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _, _, hist = diff.depenetrate(w, pos, quat, steps=steps)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t1
+    w.ctx.set_stream(None)
+    pairs = sum(h[2] for h in hist)
+    return {"workload": f"1024 worlds, 64-1024 boxes and hulls each, {sc['shape_type'].shape[0]} bodies",
+            "steps": steps, "ms_per_step": round(secs * 1e3 / steps, 3), "pairs_per_s": round(pairs / secs, 1),
+            "loss": [round(hist[0][0], 6), hist[-1][0]], "max_penetration_m": [round(hist[0][1], 6), hist[-1][1]],
+            "pairs_first_last": [hist[0][2], hist[-1][2]]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +193,7 @@ def main():
     ap.add_argument("--cpu-bodies", type=int, default=250_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-diff", action="store_true")
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy three times (HBM counter calibration)")
@@ -206,6 +272,11 @@ def main():
         batched = batched_vs_plain(bpa, args.worlds, args.margin, args.steps)
         log(f"batched: {batched}")
 
+    diff_out = None
+    if not args.no_diff:
+        diff_out = {"signed_distance": differentiation(bpa, args.steps), "depenetration": depenetration(bpa)}
+        log(f"differentiation: {diff_out}")
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -231,6 +302,7 @@ def main():
         "roofline": roof,
         "host_buffers": pcie,
         "batched": batched,
+        "differentiation": diff_out,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -118,5 +118,6 @@ def depenetrate(world, position, orientation, steps: int = 100, lr: float = 0.2,
             pos -= lr * pos.grad
             quat -= lr * quat.grad
             quat /= quat.norm(dim=1, keepdim=True)
-        history.append((float(loss), float(pen.max()) if pen.numel() else 0.0, int(pairs.shape[0])))
+        history.append((float(loss.detach()), float(pen.detach().max()) if pen.numel() else 0.0,
+                        int(pairs.shape[0])))
     return pos.detach(), quat.detach(), history
