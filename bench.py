@@ -89,6 +89,7 @@ def batched_vs_plain(bpa, n_worlds, margin, steps):
     sc = bpa.scenes.batched_worlds(n_worlds, 3)
     log(f"batched scene: {n_worlds} worlds, {sc['shape_type'].shape[0]} bodies in {time.time() - t0:.1f} s")
     wb = bpa.CollisionWorld(sc)
+    tuned = wb.autotune(margin, trials=3)
     Pb, Cb = wb.collide(margin)
     wb.synchronize()
     tb = []
@@ -115,7 +116,7 @@ def batched_vs_plain(bpa, n_worlds, margin, steps):
             "pairs": int(Pb), "contacts": int(Cb), "batched_ms": round(tb * 1e3, 3),
             "plain_per_world_ms": round(tp * 1e3, 3), "batched_pairs_per_s": round(Pb / tb, 1),
             "plain_pairs_per_s": round(Pp / tp, 1), "speedup": round(tp / tb, 2),
-            "results_equal_counts": bool(Pp == Pb and Cp == Cb)}
+            "results_equal_counts": bool(Pp == Pb and Cp == Cb), "tuning": tuned}
 
 
 def differentiation(bpa, steps, n_pairs=262_144):
@@ -218,6 +219,8 @@ def main():
     sc = bpa.scenes.mixed_convex(args.bodies, 2)
     log(f"scene: {args.bodies} bodies, cube side {sc['side']:.1f} m, generated in {time.time() - t0:.1f} s")
     w = bpa.CollisionWorld(sc)
+    tuned = w.autotune(args.margin, trials=3)
+    log(f"autotuned: {tuned}")
     for k in range(args.warmup):
         P, Cn = w.collide(args.margin)
         log(f"warmup {k}: {P} pairs, {Cn} contacts")
@@ -296,7 +299,7 @@ def main():
         "dtype": "f32", "data": "synthetic (seeded reference LCG)",
         "config": {"workload": f"mixed convex scene, {args.bodies} bodies (40% box, 30% sphere, 20% capsule, "
                                f"10% hull 16-64 pts), ~4 AABB pairs/body, margin {args.margin} m, one pass",
-                   "bodies": args.bodies, "pairs": int(P), "contacts": int(Cn)},
+                   "bodies": args.bodies, "pairs": int(P), "contacts": int(Cn), "tuning": tuned},
         "contacts_per_s": round(Cn / (ms * 1e-3), 1),
         "stage_ms": {s: round(float(v), 4) for s, v in zip(STAGES, stage_ms)},
         "roofline": roof,

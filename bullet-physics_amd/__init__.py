@@ -317,6 +317,15 @@ class CollisionWorld:
                                                      cont.ctypes.data))
         return pairs[:self.n_pairs], offs, cont[:self.n_contacts]
 
+    def autotune(self, aabb_margin: float = 0.02, trials: int = 3) -> dict:
+        """Pick the launch-shape knobs for these bodies (see bch_autotune);
+        returns the chosen knobs and the best pass time in ms."""
+        ms = C.c_float()
+        _check(self.ctx.h, self.L.bch_autotune(self.ctx.h, aabb_margin, trials, C.byref(ms)))
+        out = self.ctx.tuning()
+        out["pass_ms"] = ms.value
+        return out
+
     def results_device(self):
         """Device pointers (pairs u32[2P], contact_offsets u32[P+1],
         contacts f32[7C]) of the last pass, valid until the next pass."""

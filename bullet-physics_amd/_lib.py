@@ -21,7 +21,7 @@ EXPORTS = [
     "bch_bp_compute_pairs", "bch_bp_aabb_candidates", "bch_bp_raycast_candidates", "bch_collide_batch",
     "bch_world_set_bodies", "bch_world_set_poses", "bch_world_set_poses_device", "bch_world_collide",
     "bch_world_results", "bch_world_results_device", "bch_world_routes", "bch_set_tuning", "bch_get_tuning",
-    "bch_sd_forward", "bch_sd_backward", "bch_set_stream", "bch_reset_stream",
+    "bch_sd_forward", "bch_sd_backward", "bch_set_stream", "bch_reset_stream", "bch_autotune",
 ]
 
 # launch-shape knobs (bch_set_tuning)
@@ -98,6 +98,7 @@ def load():
     L.bch_sd_backward.argtypes = [vp, vp, u64, vp, vp, vp]
     L.bch_set_stream.argtypes = [vp, vp]
     L.bch_reset_stream.argtypes = [vp]
+    L.bch_autotune.argtypes = [vp, f32, C.c_int, C.POINTER(f32)]
     L.bch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.bch_get_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     _lib = L

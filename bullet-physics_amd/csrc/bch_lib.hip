@@ -488,6 +488,82 @@ int bch_set_tuning(bch_context* c, int knob, int value) {
   }
 }
 
+// Autotuner: time the world pass (device time from its own events, median
+// of `trials`) over the launch-shape knobs and keep the fastest.  Results
+// do not depend on the knobs; only the time does.
+static int time_pass(bch_context* c, float margin, int trials, float* ms) {
+  std::vector<float> t;
+  for (int k = 0; k < trials; ++k) {
+    uint64_t P = 0, Cn = 0;
+    int rc = bch_world_collide(c, margin, &P, &Cn);
+    if (rc) return rc;
+    float s = 0.0f;
+    for (int i = 0; i < 8; ++i) s += c->timings[i];
+    t.push_back(s);
+  }
+  std::sort(t.begin(), t.end());
+  *ms = t[t.size() / 2];
+  return BCH_OK;
+}
+
+int bch_autotune(bch_context* c, float aabb_margin, int trials, float* out_ms) {
+  if (!c || trials < 1) return BCH_INVALID_ARG;
+  if (!c->n) return fail(c, BCH_INVALID_ARG, "no bodies: call bch_world_set_bodies first");
+  float best = 0.0f;
+  int rc = time_pass(c, aabb_margin, trials, &best);  // also grows the pair slots to fit
+  if (rc) return rc;
+  // one knob at a time, in the order of their weight in the pass
+  const int hull_opts[] = {8, 16};
+  const uint32_t epa_opts[] = {16384, 32768, 65536, 131072};
+  const uint32_t np_opts[] = {2048, 4096, 8192};
+  {
+    int keep = c->hull_lanes;
+    for (int v : hull_opts) {
+      c->hull_lanes = v;
+      float ms;
+      if ((rc = time_pass(c, aabb_margin, trials, &ms))) return rc;
+      if (ms < best) { best = ms; keep = v; }
+    }
+    c->hull_lanes = keep;
+  }
+  {
+    uint32_t keep = c->epa_grid;
+    for (uint32_t v : epa_opts) {
+      c->epa_grid = v;
+      float ms;
+      if ((rc = time_pass(c, aabb_margin, trials, &ms))) return rc;
+      if (ms < best) { best = ms; keep = v; }
+    }
+    c->epa_grid = keep;
+  }
+  {
+    uint32_t keep = c->np_grid;
+    for (uint32_t v : np_opts) {
+      c->np_grid = v;
+      float ms;
+      if ((rc = time_pass(c, aabb_margin, trials, &ms))) return rc;
+      if (ms < best) { best = ms; keep = v; }
+    }
+    c->np_grid = keep;
+  }
+  {
+    // per-leaf pair slots: the smallest power of two that held every leaf
+    // (fewer slots, less to compact), or twice that
+    uint32_t fit = 8;
+    while (fit < c->host_cnt.max_leaf) fit *= 2;
+    uint32_t keep = c->K;
+    for (uint32_t v : {fit, 2 * fit}) {
+      c->K = v;
+      float ms;
+      if ((rc = time_pass(c, aabb_margin, trials, &ms))) return rc;
+      if (ms < best) { best = ms; keep = v; }
+    }
+    c->K = keep;
+  }
+  if (out_ms) *out_ms = best;
+  return BCH_OK;
+}
+
 int bch_get_tuning(bch_context* c, int knob, int* value) {
   if (!c || !value) return BCH_INVALID_ARG;
   switch (knob) {

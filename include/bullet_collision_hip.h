@@ -91,6 +91,11 @@ enum {
 };
 int bch_set_tuning(bch_context* ctx, int knob, int value);
 int bch_get_tuning(bch_context* ctx, int knob, int* value);
+/* Pick the knobs for the bodies loaded by bch_world_set_bodies: times the
+ * pass (median of `trials`, device time) over hull lanes, EPA grid,
+ * narrowphase grid and per-leaf pair slots, one knob at a time, and keeps
+ * the fastest; *out_ms (may be NULL) receives the best pass time. */
+int bch_autotune(bch_context* ctx, float aabb_margin, int trials, float* out_ms);
 
 /* ------------------------------------------ BroadPhase trait (proxies) -- */
 /* broadphase/mod.rs:94-130.  Slots follow the reference: create_proxy reuses

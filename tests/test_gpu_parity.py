@@ -331,3 +331,20 @@ def test_raycast_candidates_match_oracle(bpa, oracle, ctx):
         if k % 7 == 3:
             t[1] = f[1]
         assert np.array_equal(bp2.raycast_candidates(f, t), oracle.raycast_candidates(lo, hi, f, t))
+
+
+def test_autotune_keeps_results(bpa, scenes, ctx):
+    """The autotuner changes launch shapes only: the pass after tuning gives
+    the same pairs and bit-identical contacts."""
+    _flush("case: autotune")
+    sc = scenes.mixed_convex(30000, 12)
+    w = bpa.CollisionWorld(sc, ctx)
+    w.collide(MARGIN)
+    before = w.results()
+    tuned = w.autotune(MARGIN, trials=2)
+    _flush(f"autotune: {tuned}")
+    w.collide(MARGIN)
+    after = w.results()
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)
+    assert tuned["hull_lanes"] in (8, 16) and tuned["pass_ms"] > 0
