@@ -4,7 +4,6 @@
 
 #include <cstring>
 
-#include <rocprim/rocprim.hpp>
 
 #include <algorithm>
 #include <cstdio>
@@ -15,6 +14,7 @@
 #include "bch_kernels.hpp"
 #include "bch_epa_wave.hpp"
 #include "bch_adjoint.hpp"
+#include "bch_sort.hpp"
 
 using namespace bch;
 
@@ -69,7 +69,7 @@ struct bch_context {
   DevBuf aabb, flags, big_ids, counters, partials, keys, keys_alt, vals, vals_alt, nodes, wend, hull_ids;
   uint32_t n_hulls = 0;
   DevBuf leaf_counts, leaf_offsets, slots, pair_keys, pair_keys_alt, big_pairs, pairs, cls, cls_alt, perm,
-      perm_alt, ccount, coffsets, cslots, contacts, epa_q, tmp;
+      perm_alt, ccount, coffsets, cslots, contacts, epa_q, scan_part, rs_hist, rs_offs;
   int hull_lanes = 16;      // lanes per hull GJK pair (4, 8 or 16)
   uint32_t epa_grid = 32768; // EPA blocks (one wavefront each)
   uint32_t np_grid = 2048;  // narrowphase blocks per shape class
@@ -101,14 +101,56 @@ int fail(bch_context* c, int code, const std::string& msg) {
     if (!(buf).ensure(bytes)) return fail(ctx, BCH_HIP_ERROR, "device allocation failed"); \
   } while (0)
 
-template <class F>
-int with_tmp(bch_context* c, F f) {
-  size_t need = 0;
-  if (f(nullptr, need) != hipSuccess) return fail(c, BCH_HIP_ERROR, "rocprim size query failed");
-  ENSURE(c, c->tmp, need);
-  size_t have = c->tmp.bytes;
-  if (f(c->tmp.p, have) != hipSuccess) return fail(c, BCH_HIP_ERROR, "rocprim call failed");
+// Exclusive scan of n u32 values: out[0..n) offsets, out[n] the total.
+int dev_scan(bch_context* c, const uint32_t* in, uint32_t* out, uint64_t n, DevBuf& part, hipStream_t s) {
+  if (n > 0xffffffffull) return fail(c, BCH_INVALID_ARG, "scan longer than 2^32 - 1");
+  uint32_t nb = (uint32_t)std::max<uint64_t>(1, (n + kScanTile - 1) / kScanTile);
+  ENSURE(c, part, (size_t)nb * sizeof(uint32_t));
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanThreads), 0, s, in, (uint32_t)n, part.as<uint32_t>());
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, part.as<uint32_t>(), nb, out + n);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanThreads), 0, s, in, (uint32_t)n, part.as<uint32_t>(), out);
   return BCH_OK;
+}
+
+// Stable LSD radix sort of n keys (with values when v0 != nullptr) over the
+// 8-bit digits starting at the given bit shifts.  Ping-pongs between (k0, v0)
+// and (k1, v1); *in_second tells which pair holds the result.
+template <class K>
+int dev_radix_sort(bch_context* c, K* k0, K* k1, uint32_t* v0, uint32_t* v1, uint64_t n, const int* shifts,
+                   int nshift, hipStream_t s, bool* in_second) {
+  *in_second = false;
+  if (n <= 1 || nshift == 0) return BCH_OK;
+  if (n > 0xffffffffull) return fail(c, BCH_INVALID_ARG, "sort longer than 2^32 - 1");
+  uint32_t tiles = (uint32_t)((n + kRsTile - 1) / kRsTile);
+  ENSURE(c, c->rs_hist, (size_t)256 * tiles * sizeof(uint32_t));
+  ENSURE(c, c->rs_offs, ((size_t)256 * tiles + 1) * sizeof(uint32_t));
+  K* kin = k0;
+  K* kout = k1;
+  uint32_t* vin = v0;
+  uint32_t* vout = v1;
+  for (int p = 0; p < nshift; ++p) {
+    hipLaunchKernelGGL(k_rs_hist<K>, dim3(tiles), dim3(kRsThreads), 0, s, kin, (uint32_t)n, shifts[p], tiles,
+                       c->rs_hist.as<uint32_t>());
+    int rc = dev_scan(c, c->rs_hist.as<uint32_t>(), c->rs_offs.as<uint32_t>(), (uint64_t)256 * tiles, c->scan_part, s);
+    if (rc) return rc;
+    if (v0)
+      hipLaunchKernelGGL((k_rs_scatter<K, true>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n,
+                         shifts[p], tiles, c->rs_offs.as<uint32_t>(), kout, vout);
+    else
+      hipLaunchKernelGGL((k_rs_scatter<K, false>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n,
+                         shifts[p], tiles, c->rs_offs.as<uint32_t>(), kout, vout);
+    std::swap(kin, kout);
+    std::swap(vin, vout);
+  }
+  *in_second = (nshift & 1) != 0;
+  return BCH_OK;
+}
+
+// digit shifts covering bits [lo, hi)
+int digit_shifts(int lo, int hi, int* out) {
+  int n = 0;
+  for (int b = lo; b < hi; b += 8) out[n++] = b;
+  return n;
 }
 
 int bits_for(uint64_t v) {
@@ -134,13 +176,14 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   int wbits = d_world_of ? bits_for(c->n_worlds) : 0;
   int end_bit = std::min(64, 31 + wbits);
   // all-ones keys of excluded bodies still sort last within end_bit bits
-  int rc = with_tmp(c, [&](void* t, size_t& sz) {
-    return rocprim::radix_sort_pairs(t, sz, c->keys.as<uint64_t>(), c->keys_alt.as<uint64_t>(),
-                                     c->vals.as<uint32_t>(), c->vals_alt.as<uint32_t>(), (size_t)n, 0, end_bit, s);
-  });
+  int shifts[8];
+  int ns = digit_shifts(0, end_bit, shifts);
+  bool alt = false;
+  int rc = dev_radix_sort<uint64_t>(c, c->keys.as<uint64_t>(), c->keys_alt.as<uint64_t>(), c->vals.as<uint32_t>(),
+                                    c->vals_alt.as<uint32_t>(), n, shifts, ns, s, &alt);
   if (rc) return rc;
-  const uint64_t* skeys = c->keys_alt.as<uint64_t>();
-  const uint32_t* order = c->vals_alt.as<uint32_t>();
+  const uint64_t* skeys = alt ? c->keys_alt.as<uint64_t>() : c->keys.as<uint64_t>();
+  const uint32_t* order = alt ? c->vals_alt.as<uint32_t>() : c->vals.as<uint32_t>();
   int L = 0;
   while ((1u << L) < n) ++L;
   uint32_t N = 1u << L;
@@ -172,10 +215,7 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   if (c->host_cnt.max_leaf > c->K) return fail(c, BCH_HIP_ERROR, "per-leaf pair slots did not converge");
   uint32_t nreg = c->host_cnt.n_reg;
   // prefix offsets of the per-leaf counts
-  rc = with_tmp(c, [&](void* t, size_t& sz) {
-    return rocprim::exclusive_scan(t, sz, c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), 0u,
-                                   (size_t)nreg + 1, rocprim::plus<uint32_t>(), s);
-  });
+  rc = dev_scan(c, c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), nreg, c->scan_part, s);
   if (rc) return rc;
   uint32_t preg = 0;
   HIP_TRY(c, hipMemcpyAsync(&preg, c->leaf_offsets.as<uint32_t>() + nreg, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -210,13 +250,16 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
     HIP_TRY(c, hipMemcpyAsync(c->pair_keys.as<uint64_t>() + preg, c->big_pairs.p, pbig * sizeof(uint64_t),
                               hipMemcpyDeviceToDevice, s));
   if (c->P > 1) {
-    int eb = 32 + bits_for(n);
-    rc = with_tmp(c, [&](void* t, size_t& sz) {
-      return rocprim::radix_sort_keys(t, sz, c->pair_keys.as<uint64_t>(), c->pair_keys_alt.as<uint64_t>(),
-                                      (size_t)c->P, 0, eb, s);
-    });
+    // keys are i << 32 | j with i, j < n: only the low bits of each half vary
+    int bn = bits_for(n);
+    int ps[16];
+    int np = digit_shifts(0, bn, ps);
+    np += digit_shifts(32, 32 + bn, ps + np);
+    bool palt = false;
+    rc = dev_radix_sort<uint64_t>(c, c->pair_keys.as<uint64_t>(), c->pair_keys_alt.as<uint64_t>(), nullptr, nullptr,
+                                  c->P, ps, np, s, &palt);
     if (rc) return rc;
-    std::swap(c->pair_keys, c->pair_keys_alt);
+    if (palt) std::swap(c->pair_keys, c->pair_keys_alt);
   }
   return BCH_OK;
 }
@@ -260,11 +303,16 @@ int run_narrowphase(bch_context* c, const float* d_seeds) {
   if (P) {
     hipLaunchKernelGGL(k_split_pairs, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->pair_keys.as<uint64_t>(),
                        c->bodies.as<BodyRec>(), c->pairs.as<uint32_t>(), c->cls.as<uint32_t>(), c->perm.as<uint32_t>());
-    int rc = with_tmp(c, [&](void* t, size_t& sz) {
-      return rocprim::radix_sort_pairs(t, sz, c->cls.as<uint32_t>(), c->cls_alt.as<uint32_t>(), c->perm.as<uint32_t>(),
-                                       c->perm_alt.as<uint32_t>(), (size_t)P, 0, 3, s);
-    });
+    // one 8-bit digit holds the class; one pass leaves the result in the alt pair
+    int cs[1] = {0};
+    bool calt = false;
+    int rc = dev_radix_sort<uint32_t>(c, c->cls.as<uint32_t>(), c->cls_alt.as<uint32_t>(), c->perm.as<uint32_t>(),
+                                      c->perm_alt.as<uint32_t>(), P, cs, 1, s, &calt);
     if (rc) return rc;
+    if (!calt) {  // P <= 1: nothing moved
+      HIP_TRY(c, hipMemcpyAsync(c->cls_alt.p, c->cls.p, P * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+      HIP_TRY(c, hipMemcpyAsync(c->perm_alt.p, c->perm.p, P * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+    }
     hipLaunchKernelGGL(k_class_bounds, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->cls_alt.as<uint32_t>(),
                        cnt);
     (void)hipEventRecord(c->ev[4], s);
@@ -302,10 +350,7 @@ int run_narrowphase(bch_context* c, const float* d_seeds) {
     for (int k = 4; k <= 7; ++k) (void)hipEventRecord(c->ev[k], s);
   }
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, 1u, c->ccount.as<uint32_t>() + P, 0u);
-  int rc = with_tmp(c, [&](void* t, size_t& sz) {
-    return rocprim::exclusive_scan(t, sz, c->ccount.as<uint32_t>(), c->coffsets.as<uint32_t>(), 0u, (size_t)P + 1,
-                                   rocprim::plus<uint32_t>(), s);
-  });
+  int rc = dev_scan(c, c->ccount.as<uint32_t>(), c->coffsets.as<uint32_t>(), P, c->scan_part, s);
   if (rc) return rc;
   uint32_t total = 0;
   HIP_TRY(c, hipMemcpyAsync(&total, c->coffsets.as<uint32_t>() + P, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -427,7 +472,7 @@ void bch_destroy(bch_context* c) {
                    &c->wend, &c->leaf_counts, &c->leaf_offsets, &c->slots, &c->pair_keys,
                    &c->pair_keys_alt, &c->big_pairs, &c->pairs, &c->cls, &c->cls_alt, &c->perm, &c->perm_alt,
                    &c->ccount, &c->coffsets, &c->cslots, &c->contacts, &c->epa_q,
-                   &c->tmp};
+                   &c->scan_part, &c->rs_hist, &c->rs_offs};
   for (DevBuf* b : all) b->release();
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->own_stream);

@@ -1,0 +1,176 @@
+// bch_sort.hpp -- exclusive scan and LSD radix sort, with no memsets: every
+// buffer a launch reads was fully written by an earlier launch of the same
+// sequence, so the sequence can be captured in a HIP graph and replayed.
+//
+// Scan (u32, n items -> n + 1 offsets, the last one the total):
+//   k_scan_reduce   per-block sums of 2048-item tiles
+//   k_scan_top      one block: exclusive scan of the tile sums (any count),
+//                   writes the total at out[n]
+//   k_scan_apply    per-tile exclusive scan plus the tile's offset
+// Radix sort (u64 keys, optional u32 values, 8-bit digits, stable):
+//   k_rs_hist       per-tile digit histograms, digit-major [256][tiles]
+//   scan            over the 256 x tiles counts -> global digit offsets
+//   k_rs_scatter    per tile, in item order: each wavefront ranks its items
+//                   among equal digits with eight ballots, wavefront counts
+//                   meet in LDS, a running per-digit base keeps the order
+//                   across the tile's rounds (stable, as LSD needs)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace bch {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 2048
+constexpr int kRsThreads = 256;
+constexpr int kRsRounds = 16;
+constexpr int kRsTile = kRsThreads * kRsRounds;  // 4096
+
+// Exclusive scan of one value per thread over a block of kScanThreads
+// (= 4 wavefronts); returns the block total in `total`.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& total) {
+  __shared__ uint32_t wsum[kScanThreads / 64];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kScanThreads / 64; ++k) {
+    uint32_t s = wsum[k];
+    base += k < w ? s : 0u;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return base + x - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n,
+                                                              uint32_t* __restrict__ partials) {
+  uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < n) s += in[base + k];
+  uint32_t tot;
+  (void)block_exclusive_scan(s, tot);
+  if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
+// Exclusive scan in place of nb partials by one block; out_total[0] = sum.
+__global__ __launch_bounds__(kScanThreads) void k_scan_top(uint32_t* __restrict__ partials, uint32_t nb,
+                                                           uint32_t* __restrict__ out_total) {
+  uint32_t per = (nb + kScanThreads - 1) / kScanThreads;
+  uint32_t b0 = threadIdx.x * per;
+  uint32_t s = 0;
+  for (uint32_t k = 0; k < per; ++k)
+    if (b0 + k < nb) s += partials[b0 + k];
+  uint32_t tot;
+  uint32_t run = block_exclusive_scan(s, tot);
+  for (uint32_t k = 0; k < per; ++k) {
+    if (b0 + k < nb) {
+      uint32_t v = partials[b0 + k];
+      partials[b0 + k] = run;
+      run += v;
+    }
+  }
+  if (threadIdx.x == 0) out_total[0] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t n,
+                                                             const uint32_t* __restrict__ partials,
+                                                             uint32_t* __restrict__ out) {
+  uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0u;
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = block_exclusive_scan(s, tot) + partials[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+}
+
+template <class K>
+__device__ __forceinline__ uint32_t rs_digit(K key, int shift) { return (uint32_t)((uint64_t)key >> shift) & 0xffu; }
+
+template <class K>
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const K* __restrict__ keys, uint32_t n, int shift,
+                                                        uint32_t tiles, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0u;
+  __syncthreads();
+  uint64_t base = (uint64_t)blockIdx.x * kRsTile;
+  for (int r = 0; r < kRsRounds; ++r) {
+    uint64_t i = base + (uint64_t)r * kRsThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[rs_digit(keys[i], shift)], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
+}
+
+template <class K, bool VALS>
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                           uint32_t n, int shift, uint32_t tiles,
+                                                           const uint32_t* __restrict__ offs, K* __restrict__ kout,
+                                                           uint32_t* __restrict__ vout) {
+  __shared__ uint32_t running[256];
+  __shared__ uint32_t wcount[kRsThreads / 64][256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  running[threadIdx.x] = offs[(uint64_t)threadIdx.x * tiles + blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kRsThreads / 64; ++k) wcount[k][threadIdx.x] = 0u;
+  __syncthreads();
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  uint64_t base = (uint64_t)blockIdx.x * kRsTile;
+  for (int r = 0; r < kRsRounds; ++r) {
+    uint64_t i = base + (uint64_t)r * kRsThreads + threadIdx.x;
+    bool valid = i < n;
+    K key = valid ? kin[i] : (K)0;
+    uint32_t val = 0u;
+    if (VALS && valid) val = vin[i];
+    uint32_t d = rs_digit(key, shift);
+    // lanes of this wavefront holding the same digit
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    uint32_t rank = (uint32_t)__popcll(peers & lt);
+    uint32_t cnt = (uint32_t)__popcll(peers);
+    if (valid && rank + 1 == cnt) wcount[w][d] = cnt;
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = running[d] + rank;
+      for (int k = 0; k < kRsThreads / 64; ++k)
+        if (k < w) pos += wcount[k][d];
+      kout[pos] = key;
+      if (VALS) vout[pos] = val;
+    }
+    __syncthreads();
+    uint32_t add = 0u;
+#pragma unroll
+    for (int k = 0; k < kRsThreads / 64; ++k) {
+      add += wcount[k][threadIdx.x];
+      wcount[k][threadIdx.x] = 0u;
+    }
+    running[threadIdx.x] += add;
+    __syncthreads();
+  }
+}
+
+}  // namespace bch
