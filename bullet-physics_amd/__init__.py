@@ -291,6 +291,7 @@ class CollisionWorld:
             _check(self.ctx.h, self.L.bch_world_set_bodies(self.ctx.h, C.byref(self._bodies.c), 1, None))
         self.n_pairs = 0
         self.n_contacts = 0
+        self.found = (0, 0)  # counts the last captured-pass replay found
 
     def set_poses(self, position, orientation) -> None:
         p = np.ascontiguousarray(position, np.float32)
@@ -325,6 +326,59 @@ class CollisionWorld:
         out = self.ctx.tuning()
         out["pass_ms"] = ms.value
         return out
+
+    # ------------------------------------------------ captured pass --
+    def capture(self, aabb_margin: float = 0.02, pair_capacity: int | None = None,
+                contact_capacity: int | None = None, headroom: float = 1.5, attempts: int = 4):
+        """Capture the world pass in a HIP graph (bch_world_capture).  Missing
+        capacities come from an eager pass at the current poses times
+        `headroom`; when the capture's own pass overflows, the capacities
+        grow to fit and the capture is repeated.  Returns (pair_capacity,
+        contact_capacity) and leaves the capture's counts in n_pairs /
+        n_contacts."""
+        if pair_capacity is None or contact_capacity is None:
+            P0, C0 = self.collide(aabb_margin)
+            pair_capacity = pair_capacity or max(1024, int(P0 * headroom) + 1024)
+            contact_capacity = contact_capacity or max(1024, int(C0 * headroom) + 1024)
+        for _ in range(attempts):
+            rc = self.L.bch_world_capture(self.ctx.h, aabb_margin, int(pair_capacity), int(contact_capacity))
+            if rc == _lib.BCH_OVERFLOW:
+                try:
+                    self._read_counts()
+                except BchError:
+                    pass
+                P, Cn = self.found
+                pair_capacity = max(int(pair_capacity), int(P * headroom) + 1024)
+                contact_capacity = max(int(contact_capacity), int(Cn * headroom) + 1024)
+                continue
+            _check(self.ctx.h, rc)
+            self._caps = (int(pair_capacity), int(contact_capacity))
+            self._read_counts()
+            return self._caps
+        raise BchError(_lib.BCH_OVERFLOW, "capture kept overflowing")
+
+    def replay(self) -> None:
+        """Enqueue the captured pass (one graph launch, no synchronisation)."""
+        _check(self.ctx.h, self.L.bch_world_replay(self.ctx.h))
+
+    def _read_counts(self):
+        P, Cn, FP, FC = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        rc = self.L.bch_world_replay_counts(self.ctx.h, C.byref(P), C.byref(Cn), C.byref(FP), C.byref(FC))
+        # held counts size results(); set them before any error is raised
+        self.n_pairs, self.n_contacts = P.value, Cn.value
+        self.found = (FP.value, FC.value)
+        _check(self.ctx.h, rc)
+        return self.n_pairs, self.n_contacts
+
+    def replay_counts(self):
+        """Wait for the last replay; (n_pairs, n_contacts) held.  Raises
+        BchError with code BCH_OVERFLOW when a capacity was exceeded
+        (n_pairs / n_contacts then hold what the buffers kept, `found` what
+        the pass found)."""
+        return self._read_counts()
+
+    def release_graph(self) -> None:
+        _check(self.ctx.h, self.L.bch_world_release_graph(self.ctx.h))
 
     def results_device(self):
         """Device pointers (pairs u32[2P], contact_offsets u32[P+1],

@@ -22,6 +22,7 @@ EXPORTS = [
     "bch_world_set_bodies", "bch_world_set_poses", "bch_world_set_poses_device", "bch_world_collide",
     "bch_world_results", "bch_world_results_device", "bch_world_routes", "bch_set_tuning", "bch_get_tuning",
     "bch_sd_forward", "bch_sd_backward", "bch_set_stream", "bch_reset_stream", "bch_autotune",
+    "bch_world_capture", "bch_world_replay", "bch_world_replay_counts", "bch_world_release_graph",
 ]
 
 # launch-shape knobs (bch_set_tuning)
@@ -101,5 +102,9 @@ def load():
     L.bch_autotune.argtypes = [vp, f32, C.c_int, C.POINTER(f32)]
     L.bch_set_tuning.argtypes = [vp, C.c_int, C.c_int]
     L.bch_get_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    L.bch_world_capture.argtypes = [vp, f32, u64, u64]
+    L.bch_world_replay.argtypes = [vp]
+    L.bch_world_replay_counts.argtypes = [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]
+    L.bch_world_release_graph.argtypes = [vp]
     _lib = L
     return L

@@ -29,10 +29,15 @@ struct Counters {
   uint32_t max_leaf;    // largest per-leaf pair count seen
   uint32_t n_big_pairs;
   uint32_t n_epa;
-  uint32_t errors;      // bit 0 clip overflow, bit 1 epa store overflow, bit 2 big-pair overflow
+  uint32_t errors;      // bit 0 clip overflow, bit 1 epa store overflow, bit 2 big-pair overflow,
+                        // bit 3 pair capacity overflow, bit 4 contact capacity overflow
   uint32_t cls_start[8];  // pair range of each shape class after the class sort
   uint32_t cls_end[8];
-  uint32_t pad[4];
+  // capacity-mode pass (counts stay on the device)
+  uint32_t n_pairs;        // pairs held (<= capacity)
+  uint32_t n_pairs_total;  // pairs found
+  uint32_t n_contacts;     // contacts found
+  uint32_t pad;
 };
 
 __device__ __forceinline__ uint32_t ord_enc(float f) {
@@ -59,6 +64,9 @@ __global__ void k_reset(Counters* c) {
     c->n_big_pairs = 0;
     c->n_epa = 0;
     c->errors = 0;
+    c->n_pairs = 0;
+    c->n_pairs_total = 0;
+    c->n_contacts = 0;
   }
 }
 
@@ -432,12 +440,17 @@ __global__ void k_tree_up(float4* __restrict__ nodes, uint32_t first, int m) {
 // the leaf's K slots; counts beyond K are reported for a rerun with more
 // slots.  With `wend` (batches: leaves sorted world-major), the walk stops at
 // the first node whose first leaf lies past the query's world.
+// Counts of slots n_reg .. n_all - 1 are cleared, so a scan over all n_all
+// slots needs no host-side n_reg.
 __global__ void k_traverse(const Counters* __restrict__ cnt, const float4* __restrict__ nodes, uint32_t N, int L,
                            const uint32_t* __restrict__ wend_of_leaf, uint32_t K, uint64_t* __restrict__ slots,
-                           uint32_t* __restrict__ counts, Counters* cnt_rw) {
+                           uint32_t* __restrict__ counts, uint32_t n_all, Counters* cnt_rw) {
   uint32_t n = cnt->n_reg;
   uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+  if (s >= n) {
+    if (s < n_all) counts[s] = 0u;
+    return;
+  }
   float4 qlo = nodes[2 * (uint64_t)(N + s)], qhi = nodes[2 * (uint64_t)(N + s) + 1];
   uint32_t body = f2u(qlo.w);
   uint32_t wend = wend_of_leaf ? wend_of_leaf[s] : 0xffffffffu;
@@ -470,14 +483,37 @@ __global__ void k_traverse(const Counters* __restrict__ cnt, const float4* __res
   if (c > K) atomicMax(&cnt_rw->max_leaf, c);
 }
 
+// Leaf slots -> dense keys at the scanned offsets; keys past `cap` are
+// dropped (the capacity-mode pass reports them as an overflow).
 __global__ void k_compact_slots(const Counters* __restrict__ cnt, uint32_t K, const uint64_t* __restrict__ slots,
                                 const uint32_t* __restrict__ counts, const uint32_t* __restrict__ offsets,
-                                uint64_t* __restrict__ keys) {
+                                uint64_t cap, uint64_t* __restrict__ keys) {
   int n = (int)cnt->n_reg;
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   uint32_t c = counts[s], o = offsets[s];
-  for (uint32_t k = 0; k < c; ++k) keys[o + k] = slots[(uint64_t)s * K + k];
+  for (uint32_t k = 0; k < c; ++k)
+    if ((uint64_t)o + k < cap) keys[o + k] = slots[(uint64_t)s * K + k];
+}
+
+// Capacity mode: big-body pairs follow the tree pairs (whose count the scan
+// left at tree_total); sets the held and found pair counts.
+__global__ void k_append_big(const uint32_t* __restrict__ tree_total, const uint64_t* __restrict__ big_pairs,
+                             uint32_t big_cap, uint32_t cap, uint64_t* __restrict__ keys, Counters* cnt) {
+  uint32_t preg = *tree_total;
+  uint32_t found = cnt->n_big_pairs;
+  uint32_t nb = found < big_cap ? found : big_cap;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nb; t += gridDim.x * blockDim.x) {
+    uint64_t idx = (uint64_t)preg + t;
+    if (idx < cap) keys[idx] = big_pairs[t];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    uint64_t held = (uint64_t)preg + nb;
+    uint64_t total = (uint64_t)preg + found;
+    cnt->n_pairs = (uint32_t)(held < cap ? held : cap);
+    cnt->n_pairs_total = (uint32_t)(total < 0xffffffffull ? total : 0xffffffffull);
+    if (total > cap) atomicOr(&cnt->errors, 8u);
+  }
 }
 
 // Bodies with unbounded AABBs: tested against every other live body of their
@@ -528,8 +564,17 @@ __global__ void k_big(const Counters* __restrict__ cnt, const uint32_t* __restri
 }
 
 // Sorted keys -> (i, j) pairs and the shape-class sort keys.
-__global__ void k_split_pairs(uint32_t P, const uint64_t* __restrict__ keys, const BodyRec* __restrict__ bodies,
-                              uint32_t* __restrict__ pairs, uint32_t* __restrict__ cls, uint32_t* __restrict__ iota) {
+// P_dev (when given) caps P with a count the device computed.
+__device__ __forceinline__ uint32_t dev_count(uint32_t n, const uint32_t* n_dev) {
+  if (!n_dev) return n;
+  uint32_t m = *n_dev;
+  return m < n ? m : n;
+}
+
+__global__ void k_split_pairs(uint32_t P, const uint32_t* __restrict__ P_dev, const uint64_t* __restrict__ keys,
+                              const BodyRec* __restrict__ bodies, uint32_t* __restrict__ pairs,
+                              uint32_t* __restrict__ cls, uint32_t* __restrict__ iota) {
+  P = dev_count(P, P_dev);
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
   uint64_t k = keys[p];
@@ -561,7 +606,9 @@ __device__ __forceinline__ Pair make_pair(const BodyRec& A, const BodyRec& B, co
 }
 
 // Pair ranges of each shape class in the class-sorted order.
-__global__ void k_class_bounds(uint32_t P, const uint32_t* __restrict__ cls_sorted, Counters* cnt) {
+__global__ void k_class_bounds(uint32_t P, const uint32_t* __restrict__ P_dev, const uint32_t* __restrict__ cls_sorted,
+                               Counters* cnt) {
+  P = dev_count(P, P_dev);
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= P) return;
   uint32_t c = cls_sorted[t];
@@ -842,12 +889,19 @@ __global__ __launch_bounds__(256) void k_np_hull(const Counters* __restrict__ cn
   }
 }
 
-// Dense contacts: contact c of pair p -> contacts[7 * (offsets[p] + c)].
-__global__ void k_gather(uint32_t P, const uint32_t* __restrict__ ccount, const uint32_t* __restrict__ offsets,
-                         const float4* __restrict__ cslots, float* __restrict__ contacts) {
+// Dense contacts: contact c of pair p -> contacts[7 * (offsets[p] + c)];
+// contacts past `cap` are dropped and flagged.
+__global__ void k_gather(uint32_t P, const uint32_t* __restrict__ P_dev, const uint32_t* __restrict__ ccount,
+                         const uint32_t* __restrict__ offsets, const float4* __restrict__ cslots, uint64_t cap,
+                         float* __restrict__ contacts, Counters* cnt) {
+  P = dev_count(P, P_dev);
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
   uint32_t c = ccount[p], o = offsets[p];
+  if ((uint64_t)o + c > cap) {
+    atomicOr(&cnt->errors, 16u);
+    c = (uint64_t)o < cap ? (uint32_t)(cap - o) : 0u;
+  }
   for (uint32_t k = 0; k < c; ++k) {
     float4 a = cslots[(uint64_t)p * 8 + 2 * k], b = cslots[(uint64_t)p * 8 + 2 * k + 1];
     float* dst = contacts + 7 * (uint64_t)(o + k);

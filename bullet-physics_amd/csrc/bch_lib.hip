@@ -76,6 +76,15 @@ struct bch_context {
   uint32_t K = 16;
   uint64_t P = 0, C = 0;
   Counters host_cnt{};
+  uint32_t n_planes = 0;
+
+  // capacity-mode pass: fixed buffers, counts and overflow flags stay on the
+  // device, no host round trip, so the pass can be captured in a HIP graph
+  bool cap_mode = false, capturing = false;
+  uint64_t P_cap = 0, C_cap = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  uint64_t realloc_gen = 0, graph_gen = 0;  // a reallocation invalidates the graph
 
   // proxy API state (host mirror of the broadphase slots)
   std::vector<float> proxy_box;  // 6 per slot
@@ -96,19 +105,34 @@ int fail(bch_context* c, int code, const std::string& msg) {
       return fail(ctx, BCH_HIP_ERROR, std::string(#expr ": ") + hipGetErrorString(e_)); \
   } while (0)
 
-#define ENSURE(ctx, buf, bytes)                                                      \
-  do {                                                                               \
-    if (!(buf).ensure(bytes)) return fail(ctx, BCH_HIP_ERROR, "device allocation failed"); \
+#define ENSURE(ctx, buf, want)                                                               \
+  do {                                                                                       \
+    size_t need_ = (want);                                                                   \
+    if (need_ > (buf).bytes) {                                                               \
+      if ((ctx)->capturing) return fail(ctx, BCH_HIP_ERROR, "buffer growth during capture"); \
+      ++(ctx)->realloc_gen;                                                                  \
+    }                                                                                        \
+    if (!(buf).ensure(need_)) return fail(ctx, BCH_HIP_ERROR, "device allocation failed");  \
   } while (0)
 
-// Exclusive scan of n u32 values: out[0..n) offsets, out[n] the total.
-int dev_scan(bch_context* c, const uint32_t* in, uint32_t* out, uint64_t n, DevBuf& part, hipStream_t s) {
+// Stage boundary event (not while capturing: the graph is timed as a whole).
+void mark(bch_context* c, int k) {
+  if (!c->capturing) (void)hipEventRecord(c->ev[k], c->stream);
+}
+
+// Exclusive scan of n u32 values: out[0..n) offsets, out[n] the total.  With
+// n_dev the count is min(n, *n_dev), read on the device; total_copy, when
+// given, receives the total as well.
+int dev_scan(bch_context* c, const uint32_t* in, uint32_t* out, uint64_t n, DevBuf& part, hipStream_t s,
+             const uint32_t* n_dev = nullptr, uint32_t* total_copy = nullptr) {
   if (n > 0xffffffffull) return fail(c, BCH_INVALID_ARG, "scan longer than 2^32 - 1");
   uint32_t nb = (uint32_t)std::max<uint64_t>(1, (n + kScanTile - 1) / kScanTile);
   ENSURE(c, part, (size_t)nb * sizeof(uint32_t));
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanThreads), 0, s, in, (uint32_t)n, part.as<uint32_t>());
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, part.as<uint32_t>(), nb, out + n);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanThreads), 0, s, in, (uint32_t)n, part.as<uint32_t>(), out);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanThreads), 0, s, in, (uint32_t)n, n_dev, part.as<uint32_t>());
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, part.as<uint32_t>(), nb, out, (uint32_t)n, n_dev,
+                     total_copy);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanThreads), 0, s, in, (uint32_t)n, n_dev, part.as<uint32_t>(),
+                     out);
   return BCH_OK;
 }
 
@@ -117,7 +141,7 @@ int dev_scan(bch_context* c, const uint32_t* in, uint32_t* out, uint64_t n, DevB
 // and (k1, v1); *in_second tells which pair holds the result.
 template <class K>
 int dev_radix_sort(bch_context* c, K* k0, K* k1, uint32_t* v0, uint32_t* v1, uint64_t n, const int* shifts,
-                   int nshift, hipStream_t s, bool* in_second) {
+                   int nshift, hipStream_t s, bool* in_second, const uint32_t* n_dev = nullptr) {
   *in_second = false;
   if (n <= 1 || nshift == 0) return BCH_OK;
   if (n > 0xffffffffull) return fail(c, BCH_INVALID_ARG, "sort longer than 2^32 - 1");
@@ -129,15 +153,15 @@ int dev_radix_sort(bch_context* c, K* k0, K* k1, uint32_t* v0, uint32_t* v1, uin
   uint32_t* vin = v0;
   uint32_t* vout = v1;
   for (int p = 0; p < nshift; ++p) {
-    hipLaunchKernelGGL(k_rs_hist<K>, dim3(tiles), dim3(kRsThreads), 0, s, kin, (uint32_t)n, shifts[p], tiles,
+    hipLaunchKernelGGL(k_rs_hist<K>, dim3(tiles), dim3(kRsThreads), 0, s, kin, (uint32_t)n, n_dev, shifts[p], tiles,
                        c->rs_hist.as<uint32_t>());
     int rc = dev_scan(c, c->rs_hist.as<uint32_t>(), c->rs_offs.as<uint32_t>(), (uint64_t)256 * tiles, c->scan_part, s);
     if (rc) return rc;
     if (v0)
-      hipLaunchKernelGGL((k_rs_scatter<K, true>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n,
+      hipLaunchKernelGGL((k_rs_scatter<K, true>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n, n_dev,
                          shifts[p], tiles, c->rs_offs.as<uint32_t>(), kout, vout);
     else
-      hipLaunchKernelGGL((k_rs_scatter<K, false>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n,
+      hipLaunchKernelGGL((k_rs_scatter<K, false>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n, n_dev,
                          shifts[p], tiles, c->rs_offs.as<uint32_t>(), kout, vout);
     std::swap(kin, kout);
     std::swap(vin, vout);
@@ -159,8 +183,60 @@ int bits_for(uint64_t v) {
   return b;
 }
 
+// The pair sort's digits: keys are i << 32 | j with i, j < n, so only the low
+// bits of each half vary.
+int pair_key_shifts(uint32_t n, int* ps) {
+  int bn = bits_for(n);
+  int np = digit_shifts(0, bn, ps);
+  np += digit_shifts(32, 32 + bn, ps + np);
+  return np;
+}
+
+// Capacity-mode tail of the broadphase: one traversal with the current slot
+// count, a scan over every leaf slot, big-body pairs appended on the device,
+// and the sort over the device-held count.  Nothing is read back.
+int broadphase_capacity(bch_context* c, uint32_t n, const uint32_t* d_world_offsets, const float4* nodes, uint32_t N,
+                        int L, const uint32_t* wend, int wbits) {
+  (void)wbits;
+  hipStream_t s = c->stream;
+  Counters* cnt = c->counters.as<Counters>();
+  uint64_t cap = c->P_cap;
+  ENSURE(c, c->slots, (size_t)n * c->K * sizeof(uint64_t));
+  hipLaunchKernelGGL(k_traverse, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, nodes, N, L, wend, c->K,
+                     c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), n, cnt);
+  mark(c, 3);
+  int rc = dev_scan(c, c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), n, c->scan_part, s);
+  if (rc) return rc;
+  uint32_t big_cap = (uint32_t)std::min<uint64_t>(cap, 0xffffffffull);
+  ENSURE(c, c->big_pairs, (size_t)big_cap * sizeof(uint64_t));
+  hipLaunchKernelGGL(k_reset_big, dim3(1), dim3(64), 0, s, cnt);
+  uint32_t span = c->batched ? c->max_world : n;
+  uint32_t gx = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(span), 1024u));
+  uint32_t gy = std::max<uint32_t>(1u, std::min<uint32_t>(std::max(c->n_planes, c->host_cnt.n_big), 65535u));
+  hipLaunchKernelGGL(k_big, dim3(gx, gy), dim3(kBlock), 0, s, cnt, c->big_ids.as<uint32_t>(), c->aabb.as<float4>(),
+                     c->flags.as<uint8_t>(), c->bodies.as<BodyRec>(), d_world_offsets, n, c->big_pairs.as<uint64_t>(),
+                     big_cap, cnt);
+  ENSURE(c, c->pair_keys, cap * sizeof(uint64_t));
+  ENSURE(c, c->pair_keys_alt, cap * sizeof(uint64_t));
+  hipLaunchKernelGGL(k_compact_slots, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, c->K, c->slots.as<uint64_t>(),
+                     c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), cap, c->pair_keys.as<uint64_t>());
+  hipLaunchKernelGGL(k_append_big, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(blocks(big_cap), 1024u))),
+                     dim3(kBlock), 0, s, c->leaf_offsets.as<uint32_t>() + n, c->big_pairs.as<uint64_t>(), big_cap,
+                     (uint32_t)cap, c->pair_keys.as<uint64_t>(), cnt);
+  c->P = cap;
+  int ps[16];
+  int np = pair_key_shifts(n, ps);
+  bool palt = false;
+  rc = dev_radix_sort<uint64_t>(c, c->pair_keys.as<uint64_t>(), c->pair_keys_alt.as<uint64_t>(), nullptr, nullptr, cap,
+                                ps, np, s, &palt, &cnt->n_pairs);
+  if (rc) return rc;
+  if (palt) std::swap(c->pair_keys, c->pair_keys_alt);
+  return BCH_OK;
+}
+
 // Broadphase over the AABBs already in c->aabb (n boxes).  Leaves the sorted
-// pair keys in c->pair_keys and c->P set.
+// pair keys in c->pair_keys and c->P set (in capacity mode c->P is the
+// capacity and the held count is Counters::n_pairs on the device).
 int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const uint32_t* d_world_offsets,
                    bool given_boxes) {
   hipStream_t s = c->stream;
@@ -201,12 +277,13 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
     wend = c->wend.as<uint32_t>();
     hipLaunchKernelGGL(k_world_end, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, skeys, wend);
   }
-  (void)hipEventRecord(c->ev[2], s);
+  mark(c, 2);
+  if (c->cap_mode) return broadphase_capacity(c, n, d_world_offsets, nodes, N, L, wend, wbits);
   for (int attempt = 0; attempt < 3; ++attempt) {
     ENSURE(c, c->slots, (size_t)n * c->K * sizeof(uint64_t));
     hipLaunchKernelGGL(k_traverse, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, nodes, N, L, wend, c->K,
-                       c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), cnt);
-    (void)hipEventRecord(c->ev[3], s);
+                       c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), n, cnt);
+    mark(c, 3);
     HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
     if (c->host_cnt.max_leaf <= c->K) break;
@@ -245,16 +322,13 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   ENSURE(c, c->pair_keys, std::max<uint64_t>(c->P, 1) * sizeof(uint64_t));
   ENSURE(c, c->pair_keys_alt, std::max<uint64_t>(c->P, 1) * sizeof(uint64_t));
   hipLaunchKernelGGL(k_compact_slots, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, c->K, c->slots.as<uint64_t>(),
-                     c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), c->pair_keys.as<uint64_t>());
+                     c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), c->P, c->pair_keys.as<uint64_t>());
   if (pbig)
     HIP_TRY(c, hipMemcpyAsync(c->pair_keys.as<uint64_t>() + preg, c->big_pairs.p, pbig * sizeof(uint64_t),
                               hipMemcpyDeviceToDevice, s));
   if (c->P > 1) {
-    // keys are i << 32 | j with i, j < n: only the low bits of each half vary
-    int bn = bits_for(n);
     int ps[16];
-    int np = digit_shifts(0, bn, ps);
-    np += digit_shifts(32, 32 + bn, ps + np);
+    int np = pair_key_shifts(n, ps);
     bool palt = false;
     rc = dev_radix_sort<uint64_t>(c, c->pair_keys.as<uint64_t>(), c->pair_keys_alt.as<uint64_t>(), nullptr, nullptr,
                                   c->P, ps, np, s, &palt);
@@ -299,23 +373,26 @@ int run_narrowphase(bch_context* c, const float* d_seeds) {
   ENSURE(c, c->coffsets, (Pm + 1) * sizeof(uint32_t));
   ENSURE(c, c->cslots, Pm * 8 * sizeof(float4));
   ENSURE(c, c->epa_q, Pm * 4 * sizeof(float4));
+  // capacity mode: P is the capacity, the device holds the count
+  const uint32_t* P_dev = c->cap_mode ? &cnt->n_pairs : nullptr;
   hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
   if (P) {
-    hipLaunchKernelGGL(k_split_pairs, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->pair_keys.as<uint64_t>(),
-                       c->bodies.as<BodyRec>(), c->pairs.as<uint32_t>(), c->cls.as<uint32_t>(), c->perm.as<uint32_t>());
+    hipLaunchKernelGGL(k_split_pairs, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, P_dev,
+                       c->pair_keys.as<uint64_t>(), c->bodies.as<BodyRec>(), c->pairs.as<uint32_t>(),
+                       c->cls.as<uint32_t>(), c->perm.as<uint32_t>());
     // one 8-bit digit holds the class; one pass leaves the result in the alt pair
     int cs[1] = {0};
     bool calt = false;
     int rc = dev_radix_sort<uint32_t>(c, c->cls.as<uint32_t>(), c->cls_alt.as<uint32_t>(), c->perm.as<uint32_t>(),
-                                      c->perm_alt.as<uint32_t>(), P, cs, 1, s, &calt);
+                                      c->perm_alt.as<uint32_t>(), P, cs, 1, s, &calt, P_dev);
     if (rc) return rc;
-    if (!calt) {  // P <= 1: nothing moved
+    if (!calt) {  // P <= 1 (eager pass only): nothing moved
       HIP_TRY(c, hipMemcpyAsync(c->cls_alt.p, c->cls.p, P * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
       HIP_TRY(c, hipMemcpyAsync(c->perm_alt.p, c->perm.p, P * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
     }
-    hipLaunchKernelGGL(k_class_bounds, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->cls_alt.as<uint32_t>(),
-                       cnt);
-    (void)hipEventRecord(c->ev[4], s);
+    hipLaunchKernelGGL(k_class_bounds, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, P_dev,
+                       c->cls_alt.as<uint32_t>(), cnt);
+    mark(c, 4);
     const uint32_t* perm = c->perm_alt.as<uint32_t>();
     uint32_t g = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(P), c->np_grid));
 #define BCH_NP(CLS)                                                                                          \
@@ -329,7 +406,7 @@ int run_narrowphase(bch_context* c, const float* d_seeds) {
     BCH_NP(CLS_BOX_BOX);
     BCH_NP(CLS_GJK);
 #undef BCH_NP
-    (void)hipEventRecord(c->ev[5], s);
+    mark(c, 5);
 #define BCH_NP_HULL(G)                                                                                     \
   hipLaunchKernelGGL(k_np_hull<G>, dim3(g), dim3(kBlock), 0, s, cnt, perm, c->pairs.as<uint32_t>(),        \
                      c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_seeds, c->ccount.as<uint32_t>(),     \
@@ -341,13 +418,25 @@ int run_narrowphase(bch_context* c, const float* d_seeds) {
     else
       BCH_NP_HULL(16);
 #undef BCH_NP_HULL
-    (void)hipEventRecord(c->ev[6], s);
+    mark(c, 6);
     hipLaunchKernelGGL(k_epa_wave, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
                        c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                        c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(), cnt);
-    (void)hipEventRecord(c->ev[7], s);
+    mark(c, 7);
   } else {
-    for (int k = 4; k <= 7; ++k) (void)hipEventRecord(c->ev[k], s);
+    for (int k = 4; k <= 7; ++k) mark(c, k);
+  }
+  if (c->cap_mode) {
+    int rc = dev_scan(c, c->ccount.as<uint32_t>(), c->coffsets.as<uint32_t>(), P, c->scan_part, s, P_dev,
+                      &cnt->n_contacts);
+    if (rc) return rc;
+    c->C = c->C_cap;
+    ENSURE(c, c->contacts, c->C_cap * 7 * sizeof(float));
+    if (P)
+      hipLaunchKernelGGL(k_gather, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, P_dev, c->ccount.as<uint32_t>(),
+                         c->coffsets.as<uint32_t>(), c->cslots.as<float4>(), c->C_cap, c->contacts.as<float>(), cnt);
+    mark(c, 8);
+    return BCH_OK;
   }
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, 1u, c->ccount.as<uint32_t>() + P, 0u);
   int rc = dev_scan(c, c->ccount.as<uint32_t>(), c->coffsets.as<uint32_t>(), P, c->scan_part, s);
@@ -358,9 +447,10 @@ int run_narrowphase(bch_context* c, const float* d_seeds) {
   c->C = total;
   ENSURE(c, c->contacts, std::max<uint64_t>(c->C, 1) * 7 * sizeof(float));
   if (P)
-    hipLaunchKernelGGL(k_gather, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->ccount.as<uint32_t>(),
-                       c->coffsets.as<uint32_t>(), c->cslots.as<float4>(), c->contacts.as<float>());
-  (void)hipEventRecord(c->ev[8], s);
+    hipLaunchKernelGGL(k_gather, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, (const uint32_t*)nullptr,
+                       c->ccount.as<uint32_t>(), c->coffsets.as<uint32_t>(), c->cslots.as<float4>(), c->C,
+                       c->contacts.as<float>(), cnt);
+  mark(c, 8);
   HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
   HIP_TRY(c, hipStreamSynchronize(s));
   if (c->host_cnt.errors & 2u) return fail(c, BCH_HIP_ERROR, "EPA polytope store overflow");
@@ -405,8 +495,11 @@ int upload_bodies(bch_context* c, const bch_bodies* b, const uint32_t* world_of_
   }
   {
     std::vector<uint32_t> hid;
-    for (uint32_t i = 0; i < n; ++i)
+    c->n_planes = 0;
+    for (uint32_t i = 0; i < n; ++i) {
       if (b->shape_type[i] == BCH_HULL) hid.push_back(i);
+      c->n_planes += b->shape_type[i] == BCH_PLANE;
+    }
     c->n_hulls = (uint32_t)hid.size();
     ENSURE(c, c->hull_ids, std::max<size_t>(hid.size(), 1) * sizeof(uint32_t));
     if (!hid.empty())
@@ -440,6 +533,76 @@ int validate_bodies(bch_context* c, const bch_bodies* b) {
   return BCH_OK;
 }
 
+// One world pass (AABBs, broadphase, narrowphase) on c->stream.  Eager mode
+// reads counts back as it goes; capacity mode (c->cap_mode) keeps them on the
+// device.
+int run_pass(bch_context* c, float aabb_margin) {
+  uint32_t n = c->n;
+  hipStream_t s = c->stream;
+  if (!c->cap_mode) {
+    c->P = 0;
+    c->C = 0;
+  }
+  int rc = ensure_pass_buffers(c, std::max<uint32_t>(n, 1));
+  if (rc) return rc;
+  mark(c, 0);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, c->counters.as<Counters>());
+  if (n) {
+    hipLaunchKernelGGL(k_aabb, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->bodies.as<BodyRec>(), c->hull.as<float4>(),
+                       aabb_margin, c->aabb.as<float4>(), c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(),
+                       c->counters.as<Counters>(), c->partials.as<float>());
+    uint32_t hb = c->n_hulls ? blocks(c->n_hulls, kBlock / 16) : 0;
+    float* hpart = c->partials.as<float>() + 8 * (size_t)blocks(n);
+    if (c->n_hulls)
+      hipLaunchKernelGGL(k_hull_aabb, dim3(hb), dim3(kBlock), 0, s, c->n_hulls, c->hull_ids.as<uint32_t>(),
+                         c->bodies.as<BodyRec>(), c->hull.as<float4>(), aabb_margin, c->aabb.as<float4>(),
+                         c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(), c->counters.as<Counters>(), hpart);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, c->partials.as<float>(), blocks(n), hpart, hb,
+                       c->counters.as<Counters>());
+    mark(c, 1);
+    rc = run_broadphase(c, n, c->batched ? c->world_of.as<uint32_t>() : nullptr,
+                        c->batched ? c->world_offsets.as<uint32_t>() : nullptr, false);
+    if (rc) return rc;
+  } else {
+    for (int k = 1; k <= 3; ++k) mark(c, k);
+    if (c->cap_mode) c->P = c->P_cap;
+  }
+  rc = run_narrowphase(c, nullptr);
+  if (rc) return rc;
+  HIP_TRY(c, hipGetLastError());
+  return BCH_OK;
+}
+
+void release_graph(bch_context* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+}
+
+// Counts of the last capacity-mode pass (waits for the stream): held (what
+// the buffers hold, also left in c->P and c->C for the result getters) and
+// found.
+int read_capacity_counts(bch_context* c, uint64_t* n_pairs, uint64_t* n_contacts, uint64_t* found_pairs,
+                         uint64_t* found_contacts) {
+  HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const Counters& h = c->host_cnt;
+  c->P = std::min<uint64_t>(h.n_pairs, c->P_cap);
+  c->C = std::min<uint64_t>(h.n_contacts, c->C_cap);
+  if (n_pairs) *n_pairs = c->P;
+  if (n_contacts) *n_contacts = c->C;
+  if (found_pairs) *found_pairs = h.n_pairs_total;
+  if (found_contacts) *found_contacts = h.n_contacts;
+  if (h.errors & 1u) return fail(c, BCH_HIP_ERROR, "box-box clip polygon overflow");
+  if (h.errors & 2u) return fail(c, BCH_HIP_ERROR, "EPA polytope store overflow");
+  if (h.max_leaf > c->K || (h.errors & 4u))
+    return fail(c, BCH_OVERFLOW, "pair slots overflowed: capture again at these poses");
+  if (h.errors & 8u) return fail(c, BCH_OVERFLOW, "pair capacity exceeded");
+  if (h.errors & 16u) return fail(c, BCH_OVERFLOW, "contact capacity exceeded");
+  return BCH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -466,6 +629,7 @@ void bch_destroy(bch_context* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  release_graph(c);
   DevBuf* all[] = {&c->bodies, &c->hull, &c->world_offsets, &c->world_of, &c->staging_a, &c->staging_b,
                    &c->staging_c, &c->staging_d, &c->staging_e, &c->staging_f, &c->aabb, &c->flags, &c->big_ids,
                    &c->counters, &c->partials, &c->keys, &c->keys_alt, &c->vals, &c->vals_alt, &c->nodes, &c->hull_ids,
@@ -838,6 +1002,7 @@ int bch_world_set_bodies(bch_context* c, const bch_bodies* bodies, uint32_t n_wo
   } else if (n_worlds > 1) {
     return fail(c, BCH_INVALID_ARG, "n_worlds > 1 needs world_offsets");
   }
+  release_graph(c);  // the captured pass is shaped for the old bodies
   rc = upload_bodies(c, bodies, world_offsets ? world_of.data() : nullptr);
   if (rc) return rc;
   c->n = bodies->n;
@@ -878,40 +1043,92 @@ int bch_world_set_poses_device(bch_context* c, const float* d_position, const fl
 int bch_world_collide(bch_context* c, float aabb_margin, uint64_t* out_pairs, uint64_t* out_contacts) {
   if (!c) return BCH_INVALID_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
-  uint32_t n = c->n;
-  hipStream_t s = c->stream;
-  c->P = 0;
-  c->C = 0;
-  int rc = ensure_pass_buffers(c, std::max<uint32_t>(n, 1));
+  int rc = run_pass(c, aabb_margin);
   if (rc) return rc;
-  (void)hipEventRecord(c->ev[0], s);
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, c->counters.as<Counters>());
-  if (n) {
-    hipLaunchKernelGGL(k_aabb, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->bodies.as<BodyRec>(), c->hull.as<float4>(),
-                       aabb_margin, c->aabb.as<float4>(), c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(),
-                       c->counters.as<Counters>(), c->partials.as<float>());
-    uint32_t hb = c->n_hulls ? blocks(c->n_hulls, kBlock / 16) : 0;
-    float* hpart = c->partials.as<float>() + 8 * (size_t)blocks(n);
-    if (c->n_hulls)
-      hipLaunchKernelGGL(k_hull_aabb, dim3(hb), dim3(kBlock), 0, s, c->n_hulls, c->hull_ids.as<uint32_t>(),
-                         c->bodies.as<BodyRec>(), c->hull.as<float4>(), aabb_margin, c->aabb.as<float4>(),
-                         c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(), c->counters.as<Counters>(), hpart);
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, c->partials.as<float>(), blocks(n), hpart, hb,
-                       c->counters.as<Counters>());
-    (void)hipEventRecord(c->ev[1], s);
-    rc = run_broadphase(c, n, c->batched ? c->world_of.as<uint32_t>() : nullptr,
-                        c->batched ? c->world_offsets.as<uint32_t>() : nullptr, false);
-    if (rc) return rc;
-  } else {
-    for (int k = 1; k <= 3; ++k) (void)hipEventRecord(c->ev[k], s);
-  }
-  rc = run_narrowphase(c, nullptr);
-  if (rc) return rc;
-  HIP_TRY(c, hipGetLastError());
   for (int k = 0; k < 8; ++k) (void)hipEventElapsedTime(&c->timings[k], c->ev[k], c->ev[k + 1]);
   if (c->host_cnt.errors & 1u) return fail(c, BCH_HIP_ERROR, "box-box clip polygon overflow");
   if (out_pairs) *out_pairs = c->P;
   if (out_contacts) *out_contacts = c->C;
+  return BCH_OK;
+}
+
+// ---------------------------------------------------- captured world pass --
+int bch_world_capture(bch_context* c, float aabb_margin, uint64_t pair_capacity, uint64_t contact_capacity) {
+  if (!c) return BCH_INVALID_ARG;
+  if (pair_capacity < 2 || pair_capacity > 0xffffffffull || contact_capacity < 1 ||
+      contact_capacity > 0xffffffffull)
+    return fail(c, BCH_INVALID_ARG, "capacities must be in [2, 2^32 - 1] pairs and [1, 2^32 - 1] contacts");
+  HIP_TRY(c, hipSetDevice(c->device));
+  release_graph(c);
+  c->P_cap = pair_capacity;
+  c->C_cap = contact_capacity;
+  // One uncaptured pass in capacity mode sizes every buffer and settles the
+  // per-leaf slot count for these poses; its counts are what the capture
+  // reports.
+  c->cap_mode = true;
+  int rc = BCH_OK;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    rc = run_pass(c, aabb_margin);
+    if (rc) break;
+    HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->host_cnt.max_leaf <= c->K) break;
+    c->K = std::max<uint32_t>(c->K * 2, (c->host_cnt.max_leaf + 7) & ~7u);
+  }
+  if (!rc && c->host_cnt.max_leaf > c->K) rc = fail(c, BCH_HIP_ERROR, "per-leaf pair slots did not converge");
+  if (rc) {
+    c->cap_mode = false;
+    return rc;
+  }
+  hipStream_t caller = c->stream;
+  c->stream = c->own_stream;
+  c->capturing = true;
+  hipError_t e = hipStreamBeginCapture(c->own_stream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) rc = run_pass(c, aabb_margin);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(c->own_stream, &g) : e;
+  c->capturing = false;
+  c->cap_mode = false;
+  c->stream = caller;
+  if (rc || e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    return fail(c, BCH_HIP_ERROR, std::string("stream capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(c, BCH_HIP_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  c->graph = g;
+  c->graph_exec = x;
+  c->graph_gen = c->realloc_gen;
+  return read_capacity_counts(c, nullptr, nullptr, nullptr, nullptr);
+}
+
+int bch_world_replay(bch_context* c) {
+  if (!c) return BCH_INVALID_ARG;
+  if (!c->graph_exec) return fail(c, BCH_INVALID_ARG, "no captured pass: call bch_world_capture first");
+  if (c->graph_gen != c->realloc_gen)
+    return fail(c, BCH_INVALID_ARG, "a pass buffer was reallocated since the capture: capture again");
+  HIP_TRY(c, hipSetDevice(c->device));
+  HIP_TRY(c, hipGraphLaunch(c->graph_exec, c->stream));
+  return BCH_OK;
+}
+
+int bch_world_replay_counts(bch_context* c, uint64_t* n_pairs, uint64_t* n_contacts, uint64_t* found_pairs,
+                            uint64_t* found_contacts) {
+  if (!c) return BCH_INVALID_ARG;
+  if (!c->graph_exec) return fail(c, BCH_INVALID_ARG, "no captured pass: call bch_world_capture first");
+  HIP_TRY(c, hipSetDevice(c->device));
+  return read_capacity_counts(c, n_pairs, n_contacts, found_pairs, found_contacts);
+}
+
+int bch_world_release_graph(bch_context* c) {
+  if (!c) return BCH_INVALID_ARG;
+  (void)hipSetDevice(c->device);
+  release_graph(c);
   return BCH_OK;
 }
 

@@ -7,6 +7,8 @@
 //   k_scan_top      one block: exclusive scan of the tile sums (any count),
 //                   writes the total at out[n]
 //   k_scan_apply    per-tile exclusive scan plus the tile's offset
+// An optional device count n_dev caps n (a count an earlier kernel of the
+// same sequence wrote), so a fixed launch shape serves any count up to n.
 // Radix sort (u64 keys, optional u32 values, 8-bit digits, stable):
 //   k_rs_hist       per-tile digit histograms, digit-major [256][tiles]
 //   scan            over the 256 x tiles counts -> global digit offsets
@@ -52,8 +54,16 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& t
   return base + x - v;
 }
 
+__device__ __forceinline__ uint32_t capped_count(uint32_t n, const uint32_t* n_dev) {
+  if (!n_dev) return n;
+  uint32_t m = *n_dev;
+  return m < n ? m : n;
+}
+
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n,
+                                                              const uint32_t* __restrict__ n_dev,
                                                               uint32_t* __restrict__ partials) {
+  n = capped_count(n, n_dev);
   uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
   uint32_t s = 0;
 #pragma unroll
@@ -64,9 +74,12 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __
   if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
-// Exclusive scan in place of nb partials by one block; out_total[0] = sum.
+// Exclusive scan in place of nb partials by one block; the sum goes to
+// out[n] (n capped by n_dev) and, when given, to total_copy.
 __global__ __launch_bounds__(kScanThreads) void k_scan_top(uint32_t* __restrict__ partials, uint32_t nb,
-                                                           uint32_t* __restrict__ out_total) {
+                                                           uint32_t* __restrict__ out, uint32_t n,
+                                                           const uint32_t* __restrict__ n_dev,
+                                                           uint32_t* __restrict__ total_copy) {
   uint32_t per = (nb + kScanThreads - 1) / kScanThreads;
   uint32_t b0 = threadIdx.x * per;
   uint32_t s = 0;
@@ -81,12 +94,17 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_top(uint32_t* __restrict_
       run += v;
     }
   }
-  if (threadIdx.x == 0) out_total[0] = tot;
+  if (threadIdx.x == 0) {
+    out[capped_count(n, n_dev)] = tot;
+    if (total_copy) total_copy[0] = tot;
+  }
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t n,
+                                                             const uint32_t* __restrict__ n_dev,
                                                              const uint32_t* __restrict__ partials,
                                                              uint32_t* __restrict__ out) {
+  n = capped_count(n, n_dev);
   uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint32_t s = 0;
@@ -108,8 +126,10 @@ template <class K>
 __device__ __forceinline__ uint32_t rs_digit(K key, int shift) { return (uint32_t)((uint64_t)key >> shift) & 0xffu; }
 
 template <class K>
-__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const K* __restrict__ keys, uint32_t n, int shift,
-                                                        uint32_t tiles, uint32_t* __restrict__ hist) {
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(const K* __restrict__ keys, uint32_t n,
+                                                        const uint32_t* __restrict__ n_dev, int shift, uint32_t tiles,
+                                                        uint32_t* __restrict__ hist) {
+  n = capped_count(n, n_dev);
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0u;
   __syncthreads();
@@ -124,9 +144,10 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const K* __restrict__ ke
 
 template <class K, bool VALS>
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                           uint32_t n, int shift, uint32_t tiles,
-                                                           const uint32_t* __restrict__ offs, K* __restrict__ kout,
-                                                           uint32_t* __restrict__ vout) {
+                                                           uint32_t n, const uint32_t* __restrict__ n_dev, int shift,
+                                                           uint32_t tiles, const uint32_t* __restrict__ offs,
+                                                           K* __restrict__ kout, uint32_t* __restrict__ vout) {
+  n = capped_count(n, n_dev);
   __shared__ uint32_t running[256];
   __shared__ uint32_t wcount[kRsThreads / 64][256];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;

@@ -198,6 +198,36 @@ int bch_reset_stream(bch_context* ctx);
 int bch_world_results_device(bch_context* ctx, const uint32_t** d_pairs, const uint32_t** d_contact_offsets,
                              const float** d_contacts);
 
+/* ------------------------------------------------ captured world pass -- */
+/* No reference counterpart: the world pass of bch_world_collide for callers
+ * that step many frames (the per-step pair and contact pass of
+ * world/step.rs:445-854 over every world of a batch) without a host round
+ * trip.  The pass is run in capacity mode, with buffers for pair_capacity
+ * pairs and contact_capacity contacts, counts and overflow flags kept on the
+ * device, and captured in a HIP graph.  Before capturing, one uncaptured
+ * pass at the current poses sizes the buffers and the per-leaf pair slots;
+ * its counts are returned like bch_world_replay_counts (BCH_OVERFLOW if that
+ * pass already overflowed: capture again with larger capacities).  Results
+ * are identical to bch_world_collide's. */
+int bch_world_capture(bch_context* ctx, float aabb_margin, uint64_t pair_capacity, uint64_t contact_capacity);
+/* Launch the captured pass on the context stream (one hipGraphLaunch, no
+ * synchronisation), reading the poses the bodies hold when it runs.
+ * BCH_INVALID_ARG if nothing is captured or a pass buffer was reallocated
+ * since (e.g. an eager pass with more pairs than the capacity): capture
+ * again. */
+int bch_world_replay(bch_context* ctx);
+/* Wait for the last replayed (or capture) pass and read its counts:
+ * *n_pairs / *n_contacts are what the buffers hold, the sizes
+ * bch_world_results then writes; *found_pairs / *found_contacts what the
+ * pass found (any pointer may be NULL).  The two agree unless the result is
+ * BCH_OVERFLOW: a capacity (or the per-leaf pair slots) was exceeded, the
+ * held results are incomplete, capture again with capacities above the
+ * found counts. */
+int bch_world_replay_counts(bch_context* ctx, uint64_t* n_pairs, uint64_t* n_contacts, uint64_t* found_pairs,
+                            uint64_t* found_contacts);
+/* Free the captured graph (bch_world_set_bodies and bch_destroy do too). */
+int bch_world_release_graph(bch_context* ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,147 @@
+"""Captured world pass (bch_world_capture / bch_world_replay): graph replays
+give the eager pass's results bit for bit, across pose changes and with
+eager work between replays, and capacity overflows are reported, not
+written past.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MARGIN = 0.04
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _jittered(sc, seed, scale):
+    rng = np.random.default_rng(seed)
+    p = sc["position"] + rng.normal(0.0, scale, sc["position"].shape).astype(np.float32)
+    q = sc["orientation"] + rng.normal(0.0, scale, sc["orientation"].shape).astype(np.float32)
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    planes = sc["shape_type"] == 4
+    p[planes] = sc["position"][planes]
+    q[planes] = sc["orientation"][planes]
+    return p.astype(np.float32), q.astype(np.float32)
+
+
+def _same(a, b, label):
+    for name, x, y in zip(("pairs", "offsets", "contacts"), a, b):
+        assert x.shape == y.shape, f"{label}: {name} shape {x.shape} vs {y.shape}"
+        assert np.array_equal(x, y), f"{label}: {name} differ"
+
+
+def _replays_match(bpa, w, poses, refs, order, label):
+    for k in order:
+        w.set_poses(*poses[k])
+        w.replay()
+        P, Cn = w.replay_counts()
+        assert (P, Cn) == (refs[k][0].shape[0], refs[k][2].shape[0]), f"{label}: counts at pose {k}"
+        _same(w.results(), refs[k], f"{label} replay at pose {k}")
+        # the program's own eager work between replays: an eager pass at
+        # another pose, then a synchronisation
+        j = (k + 1) % len(poses)
+        w.set_poses(*poses[j])
+        w.collide(MARGIN)
+        _same(w.results(), refs[j], f"{label} eager pass at pose {j} between replays")
+        w.synchronize()
+
+
+def test_replays_match_eager_mixed_scene(bpa, scenes, ctx):
+    """20,000 mixed convex bodies at four poses: every replay equals the eager pass."""
+    _flush("case: captured pass, mixed_convex n=20000 seed=2, 4 poses")
+    sc = scenes.mixed_convex(20000, 2)
+    w = bpa.CollisionWorld(sc, ctx)
+    poses = [(sc["position"], sc["orientation"])] + [_jittered(sc, s, 0.05) for s in (1, 2, 3)]
+    refs = []
+    for p, q in poses:
+        w.set_poses(p, q)
+        w.collide(MARGIN)
+        refs.append(w.results())
+    Pm = max(r[0].shape[0] for r in refs)
+    Cm = max(r[2].shape[0] for r in refs)
+    w.set_poses(*poses[0])
+    caps = w.capture(MARGIN, pair_capacity=int(Pm * 1.2) + 64, contact_capacity=int(Cm * 1.2) + 64)
+    _flush(f"capacities {caps}; pairs per pose {[r[0].shape[0] for r in refs]}")
+    assert (w.n_pairs, w.n_contacts) == (refs[0][0].shape[0], refs[0][2].shape[0])
+    _replays_match(bpa, w, poses, refs, [1, 2, 0, 3, 1, 1], "mixed")
+    w.release_graph()
+
+
+def test_replays_match_eager_batched_worlds(bpa, scenes, ctx):
+    """64 worlds with a plane each: replays equal the eager batched pass."""
+    _flush("case: captured pass, batched 64 worlds, 3 poses")
+    sc = scenes.batched_worlds(64, 3, 16, 512)
+    w = bpa.CollisionWorld(sc, ctx)
+    poses = [(sc["position"], sc["orientation"])] + [_jittered(sc, s, 0.03) for s in (4, 5)]
+    refs = []
+    for p, q in poses:
+        w.set_poses(p, q)
+        w.collide(MARGIN)
+        refs.append(w.results())
+    w.set_poses(*poses[0])
+    caps = w.capture(MARGIN, headroom=1.3)
+    _flush(f"capacities {caps}; pairs per pose {[r[0].shape[0] for r in refs]}")
+    _replays_match(bpa, w, poses, refs, [2, 1, 0, 2], "batched")
+
+
+def test_capacity_overflow_is_reported(bpa, scenes, ctx):
+    """Capacities below the counts: the capture's own pass and a later replay
+    report BCH_OVERFLOW (held results stay within the buffers); an eager
+    pass that outgrows the buffers makes replay refuse the stale graph; a
+    fresh capture then matches the eager pass again."""
+    _flush("case: captured pass overflow, boxes_spheres_on_plane n=2048")
+    from importlib import import_module
+    lib = import_module("bullet-physics_amd._lib")
+    sc = scenes.boxes_spheres_on_plane(2048, 1)
+    w = bpa.CollisionWorld(sc, bpa._Context(0))
+    dense = (sc["position"], sc["orientation"])
+    sp = sc["position"].copy()
+    sp[:, [0, 2]] *= 4.0
+    spread = (sp, sc["orientation"])
+    # contact capacity below the capture pass's own count
+    w.set_poses(*dense)
+    rc = w.L.bch_world_capture(w.ctx.h, MARGIN, 1 << 16, 100)
+    assert rc == lib.BCH_OVERFLOW
+    P, Cn, FP, FC = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+    rc = w.L.bch_world_replay_counts(w.ctx.h, C.byref(P), C.byref(Cn), C.byref(FP), C.byref(FC))
+    assert rc == lib.BCH_OVERFLOW
+    assert Cn.value == 100 and FC.value > 100 and P.value == FP.value
+    _flush(f"contact overflow: {FP.value} pairs, {FC.value} contacts found, {Cn.value} held")
+    w.ctx.close()
+    # a fresh world on a fresh context (its buffers sized by this capture
+    # alone): capture at spread poses with a tight pair capacity, replay at
+    # the dense poses
+    w2 = bpa.CollisionWorld(sc, bpa._Context(0))
+    w2.set_poses(*spread)
+    w2.capture(MARGIN, pair_capacity=2200, contact_capacity=1 << 16)
+    ref_spread = w2.results()
+    w2.set_poses(*dense)
+    w2.replay()
+    with pytest.raises(lib.BchError) as ei:
+        w2.replay_counts()
+    assert ei.value.code == lib.BCH_OVERFLOW
+    held = w2.results()
+    assert held[0].shape[0] == w2.n_pairs <= 2200 < w2.found[0]
+    assert np.all(held[0][:, 0] < held[0][:, 1])
+    _flush(f"replay overflow reported: {w2.n_pairs} pairs held of capacity 2200, {w2.found[0]} found")
+    # an eager pass with more pairs than the capacity reallocates: the graph is stale
+    P_dense, _ = w2.collide(MARGIN)
+    ref_dense = w2.results()
+    assert P_dense > 2200
+    with pytest.raises(lib.BchError) as ei:
+        w2.replay()
+    assert ei.value.code == lib.BCH_INVALID_ARG
+    # capture again at the dense poses: replays equal the eager passes
+    w2.capture(MARGIN)
+    w2.replay()
+    w2.replay_counts()
+    _same(w2.results(), ref_dense, "recaptured dense")
+    w2.set_poses(*spread)
+    w2.replay()
+    w2.replay_counts()
+    _same(w2.results(), ref_spread, "recaptured, spread poses")
+    w2.ctx.close()
