@@ -82,6 +82,26 @@ def cpu_baseline(scenes, oracle_mod, sc, target_bodies, margin):
                 contacts_per_s=Cn / sec, seconds=sec, pairs=P)
 
 
+def captured(w, margin, steps, ref):
+    """The same pass captured in a HIP graph (bch_world_capture): per step
+    one graph launch plus the count read-back.  Checks that the replay's
+    results equal the eager pass's `ref`."""
+    caps = w.capture(margin, headroom=1.25)
+    w.replay()
+    w.replay_counts()
+    got = w.results()
+    same = all(np.array_equal(a, b) for a, b in zip(got, ref))
+    w.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(steps, 1)):
+        w.replay()
+        P, Cn = w.replay_counts()
+    secs = (time.perf_counter() - t0) / max(steps, 1)
+    w.release_graph()
+    return {"ms_per_step": round(secs * 1e3, 4), "pairs_per_s": round(P / secs, 1), "capacities": list(caps),
+            "results_equal_eager": bool(same)}
+
+
 def batched_vs_plain(bpa, n_worlds, margin, steps):
     """4096 worlds (16-2048 boxes/spheres + plane each, seed 3): one batched
     pass against the plain pass run world by world on the same GPU."""
@@ -99,6 +119,8 @@ def batched_vs_plain(bpa, n_worlds, margin, steps):
         wb.synchronize()
         tb.append(time.perf_counter() - t1)
     tb = float(np.median(tb))
+    graph = captured(wb, margin, steps, wb.results())
+    log(f"batched captured pass: {graph}")
     # plain path: one world at a time through the same context (upload untimed)
     ctx = wb.ctx
     tp = 0.0
@@ -116,7 +138,8 @@ def batched_vs_plain(bpa, n_worlds, margin, steps):
             "pairs": int(Pb), "contacts": int(Cb), "batched_ms": round(tb * 1e3, 3),
             "plain_per_world_ms": round(tp * 1e3, 3), "batched_pairs_per_s": round(Pb / tb, 1),
             "plain_pairs_per_s": round(Pp / tp, 1), "speedup": round(tp / tb, 2),
-            "results_equal_counts": bool(Pp == Pb and Cp == Cb), "tuning": tuned}
+            "results_equal_counts": bool(Pp == Pb and Cp == Cb), "tuning": tuned, "captured": graph,
+            "captured_speedup": round(tp / (graph["ms_per_step"] * 1e-3), 2)}
 
 
 def differentiation(bpa, steps, n_pairs=262_144):
@@ -176,12 +199,26 @@ def depenetration(bpa, steps=100):
     _, _, hist = diff.depenetrate(w, pos, quat, steps=steps)
     torch.cuda.synchronize()
     secs = time.perf_counter() - t1
-    w.ctx.set_stream(None)
     pairs = sum(h[2] for h in hist)
+    # the same descent, each step one captured graph (history read every 25 steps)
+    diff.depenetrate_graph(w, pos, quat, steps=1)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _, _, hg = diff.depenetrate_graph(w, pos, quat, steps=steps, chunk=25)
+    torch.cuda.synchronize()
+    gsecs = time.perf_counter() - t1
+    w.ctx.set_stream(None)
+    gpairs = sum(h[2] for h in hg)
     return {"workload": f"1024 worlds, 64-1024 boxes and hulls each, {sc['shape_type'].shape[0]} bodies",
             "steps": steps, "ms_per_step": round(secs * 1e3 / steps, 3), "pairs_per_s": round(pairs / secs, 1),
             "loss": [round(hist[0][0], 6), hist[-1][0]], "max_penetration_m": [round(hist[0][1], 6), hist[-1][1]],
-            "pairs_first_last": [hist[0][2], hist[-1][2]]}
+            "pairs_first_last": [hist[0][2], hist[-1][2]],
+            "captured": {"ms_per_step": round(gsecs * 1e3 / steps, 3), "pairs_per_s": round(gpairs / gsecs, 1),
+                         "loss": [round(hg[0][0], 6), hg[-1][0]],
+                         "max_penetration_m": [round(hg[0][1], 6), hg[-1][1]],
+                         "pairs_first_last": [hg[0][2], hg[-1][2]],
+                         "includes": "capture (one uncaptured sizing step) + graph launches + a history read "
+                                     "every 25 steps"}}
 
 
 def main():
@@ -253,6 +290,9 @@ def main():
     pcie = {"ms_per_step": round(host_ms, 3), "pairs_per_s": round(P / (host_ms * 1e-3), 1),
             "includes": "pose upload (28 B/body) + pass + pairs, offsets and contacts copied to host"}
     log(f"host-buffer pass: {host_ms:.2f} ms")
+    w.set_poses(pos, quat)
+    graph = captured(w, args.margin, args.steps, (pairs, offs, cont))
+    log(f"captured pass: {graph}")
     routes = w.routes()
     dom = STAGES[int(np.argmax(stage_ms))]
     cand = [st for st in SINGLE_KERNEL]
@@ -304,6 +344,7 @@ def main():
         "stage_ms": {s: round(float(v), 4) for s, v in zip(STAGES, stage_ms)},
         "roofline": roof,
         "host_buffers": pcie,
+        "captured": graph,
         "batched": batched,
         "differentiation": diff_out,
         "cpu_baseline": cpu,

@@ -23,6 +23,7 @@ EXPORTS = [
     "bch_world_results", "bch_world_results_device", "bch_world_routes", "bch_set_tuning", "bch_get_tuning",
     "bch_sd_forward", "bch_sd_backward", "bch_set_stream", "bch_reset_stream", "bch_autotune",
     "bch_world_capture", "bch_world_replay", "bch_world_replay_counts", "bch_world_release_graph",
+    "bch_depen_capture", "bch_depen_replay", "bch_depen_history", "bch_depen_release",
 ]
 
 # launch-shape knobs (bch_set_tuning)
@@ -106,5 +107,9 @@ def load():
     L.bch_world_replay.argtypes = [vp]
     L.bch_world_replay_counts.argtypes = [vp, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]
     L.bch_world_release_graph.argtypes = [vp]
+    L.bch_depen_capture.argtypes = [vp, f32, f32, vp, vp, u64, u32, C.POINTER(u64)]
+    L.bch_depen_replay.argtypes = [vp, u32]
+    L.bch_depen_release.argtypes = [vp]
+    L.bch_depen_history.argtypes = [vp, vp, vp, vp, vp, u32, C.POINTER(u32), C.c_int]
     _lib = L
     return L

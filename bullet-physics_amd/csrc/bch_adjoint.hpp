@@ -306,13 +306,16 @@ BCH_HD void store_out(float* o10, const SdOut& o) {
 
 // The walk for every pair, G lanes per pair (hull support scans split over
 // the group).  BWD = false writes out[10P]; BWD = true reads gin[10P] and
-// writes grad[14P].  Overlapping cores are queued for the EPA kernel.
+// writes grad[14P].  Overlapping cores are queued for the EPA kernel.  P_dev
+// (when given) caps P with a count computed on the device.
 template <int G, bool BWD>
-__global__ __launch_bounds__(256) void k_sd_walk(uint32_t P, uint32_t nbodies, const uint32_t* __restrict__ pairs,
+__global__ __launch_bounds__(256) void k_sd_walk(uint32_t P, const uint32_t* __restrict__ P_dev, uint32_t nbodies,
+                                                 const uint32_t* __restrict__ pairs,
                                                  const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
                                                  const float* __restrict__ gin, float* __restrict__ out,
                                                  uint8_t* __restrict__ status, float4* __restrict__ epa_q,
                                                  Counters* cnt) {
+  P = dev_count(P, P_dev);
   __shared__ float4 hstage[256 / G][2][64];
   const uint32_t per_block = blockDim.x / G;
   int grp = threadIdx.x / G, sub = threadIdx.x % G;

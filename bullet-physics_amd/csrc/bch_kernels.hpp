@@ -37,7 +37,7 @@ struct Counters {
   uint32_t n_pairs;        // pairs held (<= capacity)
   uint32_t n_pairs_total;  // pairs found
   uint32_t n_contacts;     // contacts found
-  uint32_t pad;
+  uint32_t n_pen;          // penetrating pairs (de-penetration step)
 };
 
 __device__ __forceinline__ uint32_t ord_enc(float f) {
@@ -67,6 +67,7 @@ __global__ void k_reset(Counters* c) {
     c->n_pairs = 0;
     c->n_pairs_total = 0;
     c->n_contacts = 0;
+    c->n_pen = 0;
   }
 }
 

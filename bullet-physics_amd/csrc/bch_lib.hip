@@ -15,6 +15,7 @@
 #include "bch_epa_wave.hpp"
 #include "bch_adjoint.hpp"
 #include "bch_sort.hpp"
+#include "bch_depen.hpp"
 
 using namespace bch;
 
@@ -85,6 +86,14 @@ struct bch_context {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   uint64_t realloc_gen = 0, graph_gen = 0;  // a reallocation invalidates the graph
+  uint64_t graph_P_cap = 0, graph_C_cap = 0;
+
+  // captured de-penetration step
+  hipGraph_t dgraph = nullptr;
+  hipGraphExec_t dgraph_exec = nullptr;
+  uint64_t dgraph_gen = 0;
+  uint32_t dhist_cap = 0;
+  DevBuf sd_out, sd_status, pen_pairs, pen_g, g14, body_grad, pen_part, dhist, dstep;
 
   // proxy API state (host mirror of the broadphase slots)
   std::vector<float> proxy_box;  // 6 per slot
@@ -533,6 +542,23 @@ int validate_bodies(bch_context* c, const bch_bodies* b) {
   return BCH_OK;
 }
 
+// World AABBs (padded by the margin), body classes and scene bounds.
+void run_aabbs(bch_context* c, float aabb_margin) {
+  uint32_t n = c->n;
+  hipStream_t s = c->stream;
+  hipLaunchKernelGGL(k_aabb, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->bodies.as<BodyRec>(), c->hull.as<float4>(),
+                     aabb_margin, c->aabb.as<float4>(), c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(),
+                     c->counters.as<Counters>(), c->partials.as<float>());
+  uint32_t hb = c->n_hulls ? blocks(c->n_hulls, kBlock / 16) : 0;
+  float* hpart = c->partials.as<float>() + 8 * (size_t)blocks(n);
+  if (c->n_hulls)
+    hipLaunchKernelGGL(k_hull_aabb, dim3(hb), dim3(kBlock), 0, s, c->n_hulls, c->hull_ids.as<uint32_t>(),
+                       c->bodies.as<BodyRec>(), c->hull.as<float4>(), aabb_margin, c->aabb.as<float4>(),
+                       c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(), c->counters.as<Counters>(), hpart);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, c->partials.as<float>(), blocks(n), hpart, hb,
+                     c->counters.as<Counters>());
+}
+
 // One world pass (AABBs, broadphase, narrowphase) on c->stream.  Eager mode
 // reads counts back as it goes; capacity mode (c->cap_mode) keeps them on the
 // device.
@@ -548,17 +574,7 @@ int run_pass(bch_context* c, float aabb_margin) {
   mark(c, 0);
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, c->counters.as<Counters>());
   if (n) {
-    hipLaunchKernelGGL(k_aabb, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->bodies.as<BodyRec>(), c->hull.as<float4>(),
-                       aabb_margin, c->aabb.as<float4>(), c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(),
-                       c->counters.as<Counters>(), c->partials.as<float>());
-    uint32_t hb = c->n_hulls ? blocks(c->n_hulls, kBlock / 16) : 0;
-    float* hpart = c->partials.as<float>() + 8 * (size_t)blocks(n);
-    if (c->n_hulls)
-      hipLaunchKernelGGL(k_hull_aabb, dim3(hb), dim3(kBlock), 0, s, c->n_hulls, c->hull_ids.as<uint32_t>(),
-                         c->bodies.as<BodyRec>(), c->hull.as<float4>(), aabb_margin, c->aabb.as<float4>(),
-                         c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(), c->counters.as<Counters>(), hpart);
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, c->partials.as<float>(), blocks(n), hpart, hb,
-                       c->counters.as<Counters>());
+    run_aabbs(c, aabb_margin);
     mark(c, 1);
     rc = run_broadphase(c, n, c->batched ? c->world_of.as<uint32_t>() : nullptr,
                         c->batched ? c->world_offsets.as<uint32_t>() : nullptr, false);
@@ -580,6 +596,111 @@ void release_graph(bch_context* c) {
   c->graph = nullptr;
 }
 
+void release_depen_graph(bch_context* c) {
+  if (c->dgraph_exec) (void)hipGraphExecDestroy(c->dgraph_exec);
+  if (c->dgraph) (void)hipGraphDestroy(c->dgraph);
+  c->dgraph_exec = nullptr;
+  c->dgraph = nullptr;
+}
+
+// Capture whatever `body` enqueues on the context's own stream into a graph.
+template <class F>
+int capture_graph(bch_context* c, F body, hipGraph_t* out_g, hipGraphExec_t* out_x) {
+  *out_g = nullptr;
+  *out_x = nullptr;
+  hipStream_t caller = c->stream;
+  c->stream = c->own_stream;
+  c->capturing = true;
+  int rc = BCH_OK;
+  hipError_t e = hipStreamBeginCapture(c->own_stream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) rc = body();
+  hipGraph_t g = nullptr;
+  hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(c->own_stream, &g) : e;
+  c->capturing = false;
+  c->stream = caller;
+  if (rc || e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    return fail(c, BCH_HIP_ERROR, std::string("stream capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(c, BCH_HIP_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  *out_g = g;
+  *out_x = x;
+  return BCH_OK;
+}
+
+// One de-penetration step in capacity mode (c->P_cap pairs): poses packed
+// from the caller's arrays, AABBs, broadphase, signed distance of every pair,
+// loss record, then (update) the adjoint of the penetrating pairs, the
+// per-body gradient and the pose update written back to the caller's arrays.
+int run_depen_step(bch_context* c, float aabb_margin, float lr, float* d_pos, float* d_quat, bool update) {
+  uint32_t n = c->n;
+  hipStream_t s = c->stream;
+  Counters* cnt = c->counters.as<Counters>();
+  uint64_t cap = c->P_cap;
+  int rc = ensure_pass_buffers(c, n);
+  if (rc) return rc;
+  ENSURE(c, c->pairs, cap * 2 * sizeof(uint32_t));
+  ENSURE(c, c->cls, cap * sizeof(uint32_t));
+  ENSURE(c, c->perm, cap * sizeof(uint32_t));
+  ENSURE(c, c->epa_q, cap * 4 * sizeof(float4));
+  ENSURE(c, c->sd_out, cap * 10 * sizeof(float));
+  ENSURE(c, c->sd_status, cap);
+  ENSURE(c, c->pen_pairs, cap * 2 * sizeof(uint32_t));
+  ENSURE(c, c->pen_g, cap * 10 * sizeof(float));
+  ENSURE(c, c->g14, cap * 14 * sizeof(float));
+  ENSURE(c, c->body_grad, (size_t)n * 7 * sizeof(float));
+  uint32_t nb = blocks(cap, kDepenBlock);
+  ENSURE(c, c->pen_part, (size_t)nb * sizeof(float2));
+  hipLaunchKernelGGL(k_pack_poses, dim3(blocks(n)), dim3(kBlock), 0, s, n, (const float*)d_pos,
+                     (const float*)d_quat, c->bodies.as<BodyRec>());
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, cnt);
+  run_aabbs(c, aabb_margin);
+  rc = run_broadphase(c, n, c->batched ? c->world_of.as<uint32_t>() : nullptr,
+                      c->batched ? c->world_offsets.as<uint32_t>() : nullptr, false);
+  if (rc) return rc;
+  const uint32_t* P_dev = &cnt->n_pairs;
+  hipLaunchKernelGGL(k_split_pairs, dim3(blocks(cap)), dim3(kBlock), 0, s, (uint32_t)cap, P_dev,
+                     c->pair_keys.as<uint64_t>(), c->bodies.as<BodyRec>(), c->pairs.as<uint32_t>(),
+                     c->cls.as<uint32_t>(), c->perm.as<uint32_t>());
+  constexpr int G = 16;
+  uint32_t g = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(cap, kBlock / G), 8192u));
+  hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
+  hipLaunchKernelGGL((k_sd_walk<G, false>), dim3(g), dim3(kBlock), 0, s, (uint32_t)cap, P_dev, n,
+                     c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(), (const float*)nullptr,
+                     c->sd_out.as<float>(), c->sd_status.as<uint8_t>(), c->epa_q.as<float4>(), cnt);
+  hipLaunchKernelGGL(k_sd_epa<false>, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
+                     c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(), (const float*)nullptr,
+                     c->sd_out.as<float>(), c->sd_status.as<uint8_t>());
+  hipLaunchKernelGGL(k_depen_terms, dim3(nb), dim3(kDepenBlock), 0, s, (uint32_t)cap, P_dev, c->sd_out.as<float>(),
+                     c->pairs.as<uint32_t>(), c->pen_pairs.as<uint32_t>(), c->pen_g.as<float>(), &cnt->n_pen,
+                     c->pen_part.as<float2>());
+  hipLaunchKernelGGL(k_depen_record, dim3(1), dim3(kDepenBlock), 0, s, c->pen_part.as<float2>(), nb, cnt, c->K,
+                     c->dhist.as<DepenRecord>(), c->dhist_cap, c->dstep.as<uint32_t>());
+  if (!update) return BCH_OK;
+  const uint32_t* pen_dev = &cnt->n_pen;
+  hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
+  hipLaunchKernelGGL((k_sd_walk<G, true>), dim3(g), dim3(kBlock), 0, s, (uint32_t)cap, pen_dev, n,
+                     c->pen_pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
+                     c->pen_g.as<float>(), c->g14.as<float>(), (uint8_t*)nullptr, c->epa_q.as<float4>(), cnt);
+  hipLaunchKernelGGL(k_sd_epa<true>, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
+                     c->pen_pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
+                     c->pen_g.as<float>(), c->g14.as<float>(), (uint8_t*)nullptr);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks((uint64_t)n * 7)), dim3(kBlock), 0, s, n * 7,
+                     c->body_grad.as<uint32_t>(), 0u);
+  float* gpos = c->body_grad.as<float>();
+  float* gquat = gpos + (size_t)n * 3;
+  hipLaunchKernelGGL(k_depen_scatter, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(blocks(cap), 4096u))),
+                     dim3(kBlock), 0, s, pen_dev, c->pen_pairs.as<uint32_t>(), c->g14.as<float>(), gpos, gquat);
+  hipLaunchKernelGGL(k_depen_update, dim3(blocks(n)), dim3(kBlock), 0, s, n, lr, gpos, gquat, d_pos, d_quat);
+  return BCH_OK;
+}
+
 // Counts of the last capacity-mode pass (waits for the stream): held (what
 // the buffers hold, also left in c->P and c->C for the result getters) and
 // found.
@@ -588,8 +709,8 @@ int read_capacity_counts(bch_context* c, uint64_t* n_pairs, uint64_t* n_contacts
   HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   const Counters& h = c->host_cnt;
-  c->P = std::min<uint64_t>(h.n_pairs, c->P_cap);
-  c->C = std::min<uint64_t>(h.n_contacts, c->C_cap);
+  c->P = std::min<uint64_t>(h.n_pairs, c->graph_P_cap);
+  c->C = std::min<uint64_t>(h.n_contacts, c->graph_C_cap);
   if (n_pairs) *n_pairs = c->P;
   if (n_contacts) *n_contacts = c->C;
   if (found_pairs) *found_pairs = h.n_pairs_total;
@@ -630,13 +751,15 @@ void bch_destroy(bch_context* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   release_graph(c);
+  release_depen_graph(c);
   DevBuf* all[] = {&c->bodies, &c->hull, &c->world_offsets, &c->world_of, &c->staging_a, &c->staging_b,
                    &c->staging_c, &c->staging_d, &c->staging_e, &c->staging_f, &c->aabb, &c->flags, &c->big_ids,
                    &c->counters, &c->partials, &c->keys, &c->keys_alt, &c->vals, &c->vals_alt, &c->nodes, &c->hull_ids,
                    &c->wend, &c->leaf_counts, &c->leaf_offsets, &c->slots, &c->pair_keys,
                    &c->pair_keys_alt, &c->big_pairs, &c->pairs, &c->cls, &c->cls_alt, &c->perm, &c->perm_alt,
                    &c->ccount, &c->coffsets, &c->cslots, &c->contacts, &c->epa_q,
-                   &c->scan_part, &c->rs_hist, &c->rs_offs};
+                   &c->scan_part, &c->rs_hist, &c->rs_offs, &c->sd_out, &c->sd_status, &c->pen_pairs,
+                   &c->pen_g, &c->g14, &c->body_grad, &c->pen_part, &c->dhist, &c->dstep};
   for (DevBuf* b : all) b->release();
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->own_stream);
@@ -1002,7 +1125,8 @@ int bch_world_set_bodies(bch_context* c, const bch_bodies* bodies, uint32_t n_wo
   } else if (n_worlds > 1) {
     return fail(c, BCH_INVALID_ARG, "n_worlds > 1 needs world_offsets");
   }
-  release_graph(c);  // the captured pass is shaped for the old bodies
+  release_graph(c);  // the captured graphs are shaped for the old bodies
+  release_depen_graph(c);
   rc = upload_bodies(c, bodies, world_offsets ? world_of.data() : nullptr);
   if (rc) return rc;
   c->n = bodies->n;
@@ -1080,30 +1204,16 @@ int bch_world_capture(bch_context* c, float aabb_margin, uint64_t pair_capacity,
     c->cap_mode = false;
     return rc;
   }
-  hipStream_t caller = c->stream;
-  c->stream = c->own_stream;
-  c->capturing = true;
-  hipError_t e = hipStreamBeginCapture(c->own_stream, hipStreamCaptureModeThreadLocal);
-  if (e == hipSuccess) rc = run_pass(c, aabb_margin);
   hipGraph_t g = nullptr;
-  hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(c->own_stream, &g) : e;
-  c->capturing = false;
-  c->cap_mode = false;
-  c->stream = caller;
-  if (rc || e != hipSuccess || e2 != hipSuccess) {
-    if (g) (void)hipGraphDestroy(g);
-    if (rc) return rc;
-    return fail(c, BCH_HIP_ERROR, std::string("stream capture: ") + hipGetErrorString(e != hipSuccess ? e : e2));
-  }
   hipGraphExec_t x = nullptr;
-  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-  if (e != hipSuccess) {
-    (void)hipGraphDestroy(g);
-    return fail(c, BCH_HIP_ERROR, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-  }
+  rc = capture_graph(c, [&] { return run_pass(c, aabb_margin); }, &g, &x);
+  c->cap_mode = false;
+  if (rc) return rc;
   c->graph = g;
   c->graph_exec = x;
   c->graph_gen = c->realloc_gen;
+  c->graph_P_cap = c->P_cap;
+  c->graph_C_cap = c->C_cap;
   return read_capacity_counts(c, nullptr, nullptr, nullptr, nullptr);
 }
 
@@ -1125,10 +1235,106 @@ int bch_world_replay_counts(bch_context* c, uint64_t* n_pairs, uint64_t* n_conta
   return read_capacity_counts(c, n_pairs, n_contacts, found_pairs, found_contacts);
 }
 
+// ---------------------------------------------- captured de-penetration --
+int bch_depen_capture(bch_context* c, float aabb_margin, float lr, float* d_position, float* d_orientation,
+                      uint64_t pair_capacity, uint32_t history_capacity, uint64_t* found_pairs) {
+  if (!c) return BCH_INVALID_ARG;
+  if (found_pairs) *found_pairs = 0;
+  if (!c->n) return fail(c, BCH_INVALID_ARG, "no bodies: call bch_world_set_bodies first");
+  if (!d_position || !d_orientation) return fail(c, BCH_INVALID_ARG, "NULL pose arrays");
+  if (pair_capacity < 2 || pair_capacity > 0xffffffffull)
+    return fail(c, BCH_INVALID_ARG, "pair capacity must be in [2, 2^32 - 1]");
+  HIP_TRY(c, hipSetDevice(c->device));
+  release_depen_graph(c);
+  c->P_cap = pair_capacity;
+  c->dhist_cap = std::max<uint32_t>(history_capacity, 1u);
+  ENSURE(c, c->dhist, (size_t)c->dhist_cap * sizeof(DepenRecord));
+  ENSURE(c, c->dstep, sizeof(uint32_t));
+  // An uncaptured step without the update sizes the buffers and settles the
+  // per-leaf pair slots at the current poses (which it leaves unchanged).
+  c->cap_mode = true;
+  int rc = BCH_OK;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    rc = run_depen_step(c, aabb_margin, lr, d_position, d_orientation, false);
+    if (rc) break;
+    HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->host_cnt.max_leaf <= c->K) break;
+    c->K = std::max<uint32_t>(c->K * 2, (c->host_cnt.max_leaf + 7) & ~7u);
+  }
+  if (!rc && c->host_cnt.max_leaf > c->K) rc = fail(c, BCH_HIP_ERROR, "per-leaf pair slots did not converge");
+  if (!rc && found_pairs) *found_pairs = c->host_cnt.n_pairs_total;
+  if (!rc && (c->host_cnt.n_pairs_total > pair_capacity || (c->host_cnt.errors & 4u)))
+    rc = fail(c, BCH_OVERFLOW, "pair capacity exceeded at the current poses");
+  if (rc) {
+    c->cap_mode = false;
+    return rc;
+  }
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  rc = capture_graph(c, [&] { return run_depen_step(c, aabb_margin, lr, d_position, d_orientation, true); }, &g, &x);
+  c->cap_mode = false;
+  if (rc) return rc;
+  c->dgraph = g;
+  c->dgraph_exec = x;
+  c->dgraph_gen = c->realloc_gen;
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, 1u, c->dstep.as<uint32_t>(), 0u);
+  HIP_TRY(c, hipGetLastError());
+  return BCH_OK;
+}
+
+int bch_depen_replay(bch_context* c, uint32_t steps) {
+  if (!c) return BCH_INVALID_ARG;
+  if (!c->dgraph_exec) return fail(c, BCH_INVALID_ARG, "no captured step: call bch_depen_capture first");
+  if (c->dgraph_gen != c->realloc_gen)
+    return fail(c, BCH_INVALID_ARG, "a pass buffer was reallocated since the capture: capture again");
+  HIP_TRY(c, hipSetDevice(c->device));
+  for (uint32_t k = 0; k < steps; ++k) HIP_TRY(c, hipGraphLaunch(c->dgraph_exec, c->stream));
+  return BCH_OK;
+}
+
+int bch_depen_history(bch_context* c, float* loss, float* max_pen, uint32_t* n_pairs, uint32_t* n_pen,
+                      uint32_t capacity, uint32_t* count, int reset) {
+  if (!c || !count) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  if (!c->dgraph_exec) return fail(c, BCH_INVALID_ARG, "no captured step: call bch_depen_capture first");
+  HIP_TRY(c, hipSetDevice(c->device));
+  uint32_t steps = 0;
+  HIP_TRY(c, hipMemcpyAsync(&steps, c->dstep.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  uint32_t held = std::min(steps, c->dhist_cap);
+  std::vector<DepenRecord> rec(held);
+  if (held)
+    HIP_TRY(c, hipMemcpyAsync(rec.data(), c->dhist.p, held * sizeof(DepenRecord), hipMemcpyDeviceToHost, c->stream));
+  if (reset) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, 1u, c->dstep.as<uint32_t>(), 0u);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *count = steps;
+  bool over = false;
+  for (uint32_t k = 0; k < held; ++k) {
+    if (k < capacity) {
+      if (loss) loss[k] = rec[k].loss;
+      if (max_pen) max_pen[k] = rec[k].max_pen;
+      if (n_pairs) n_pairs[k] = rec[k].n_pairs;
+      if (n_pen) n_pen[k] = rec[k].n_pen;
+    }
+    over = over || (rec[k].flags & (4u | 8u | 32u)) != 0;
+  }
+  if (over) return fail(c, BCH_OVERFLOW, "a step's pairs exceeded the capacity: capture again with a larger one");
+  if (steps > c->dhist_cap || steps > capacity) return fail(c, BCH_OVERFLOW, "more steps than history capacity");
+  return BCH_OK;
+}
+
 int bch_world_release_graph(bch_context* c) {
   if (!c) return BCH_INVALID_ARG;
   (void)hipSetDevice(c->device);
   release_graph(c);
+  return BCH_OK;
+}
+
+int bch_depen_release(bch_context* c) {
+  if (!c) return BCH_INVALID_ARG;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  release_depen_graph(c);
   return BCH_OK;
 }
 
@@ -1147,13 +1353,15 @@ static int run_sd(bch_context* c, const uint32_t* d_pairs, uint64_t P, const flo
     constexpr int G = 16;
     uint32_t g = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(P, kBlock / G), 8192u));
     if (bwd) {
-      hipLaunchKernelGGL((k_sd_walk<G, true>), dim3(g), dim3(kBlock), 0, s, (uint32_t)P, c->n, d_pairs,
+      hipLaunchKernelGGL((k_sd_walk<G, true>), dim3(g), dim3(kBlock), 0, s, (uint32_t)P, (const uint32_t*)nullptr,
+                         c->n, d_pairs,
                          c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_gin, d_out, d_status,
                          c->epa_q.as<float4>(), cnt);
       hipLaunchKernelGGL(k_sd_epa<true>, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
                          d_pairs, c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_gin, d_out, d_status);
     } else {
-      hipLaunchKernelGGL((k_sd_walk<G, false>), dim3(g), dim3(kBlock), 0, s, (uint32_t)P, c->n, d_pairs,
+      hipLaunchKernelGGL((k_sd_walk<G, false>), dim3(g), dim3(kBlock), 0, s, (uint32_t)P, (const uint32_t*)nullptr,
+                         c->n, d_pairs,
                          c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_gin, d_out, d_status,
                          c->epa_q.as<float4>(), cnt);
       hipLaunchKernelGGL(k_sd_epa<false>, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),

@@ -121,3 +121,66 @@ def depenetrate(world, position, orientation, steps: int = 100, lr: float = 0.2,
         history.append((float(loss.detach()), float(pen.detach().max()) if pen.numel() else 0.0,
                         int(pairs.shape[0])))
     return pos.detach(), quat.detach(), history
+
+
+def depenetrate_graph(world, position, orientation, steps: int = 100, lr: float = 0.2, aabb_margin: float = 0.04,
+                      headroom: float = 1.5, chunk: int = 25):
+    """:func:`depenetrate` with each step one captured HIP graph
+    (bch_depen_capture): broadphase, signed distance, the adjoint of the
+    penetrating pairs, the per-body gradient and the pose update all run on
+    the device, with no host round trip inside a chunk of steps.  The
+    history is read back every `chunk` steps; a chunk in which the pairs
+    overflowed the capacity is rolled back and rerun after a capture with a
+    larger one.  Returns the final poses and per step (loss, max
+    penetration, pairs), like :func:`depenetrate`."""
+    import ctypes as C
+
+    import numpy as np
+
+    from . import BchError, _check, _lib
+
+    L, h = world.L, world.ctx.h
+    pos = position.detach().to(torch.float32).contiguous().clone()
+    quat = orientation.detach().to(torch.float32).contiguous().clone()
+    _bind_stream(world)
+    world.set_poses_device(pos.data_ptr(), quat.data_ptr())
+    P0, _ = world.collide(aabb_margin)
+    cap = max(1024, int(P0 * headroom) + 1024)
+    found = C.c_uint64()
+
+    def capture():
+        nonlocal cap
+        for _ in range(4):
+            rc = L.bch_depen_capture(h, aabb_margin, lr, pos.data_ptr(), quat.data_ptr(), cap, chunk,
+                                     C.byref(found))
+            if rc == _lib.BCH_OVERFLOW:
+                cap = max(2 * cap, int(found.value * headroom) + 1024)
+                continue
+            _check(h, rc)
+            return
+        raise BchError(_lib.BCH_OVERFLOW, "de-penetration capture kept overflowing")
+
+    capture()
+    loss = np.zeros(chunk, np.float32)
+    maxpen = np.zeros(chunk, np.float32)
+    npairs = np.zeros(chunk, np.uint32)
+    count = C.c_uint32()
+    history = []
+    try:
+        while len(history) < steps:
+            k = min(chunk, steps - len(history))
+            saved = (pos.clone(), quat.clone())
+            _check(h, L.bch_depen_replay(h, k))
+            rc = L.bch_depen_history(h, loss.ctypes.data, maxpen.ctypes.data, npairs.ctypes.data, None, chunk,
+                                     C.byref(count), 1)
+            if rc == _lib.BCH_OVERFLOW:
+                pos.copy_(saved[0])
+                quat.copy_(saved[1])
+                cap = max(2 * cap, int(npairs[:k].max() * headroom) + 1024)
+                capture()
+                continue
+            _check(h, rc)
+            history += [(float(loss[s]), float(maxpen[s]), int(npairs[s])) for s in range(k)]
+    finally:
+        L.bch_depen_release(h)
+    return pos, quat, history

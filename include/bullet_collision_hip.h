@@ -225,8 +225,43 @@ int bch_world_replay(bch_context* ctx);
  * found counts. */
 int bch_world_replay_counts(bch_context* ctx, uint64_t* n_pairs, uint64_t* n_contacts, uint64_t* found_pairs,
                             uint64_t* found_contacts);
-/* Free the captured graph (bch_world_set_bodies and bch_destroy do too). */
+/* Free the captured pass (bch_world_set_bodies and bch_destroy do too). */
 int bch_world_release_graph(bch_context* ctx);
+
+/* ------------------------------------------ captured de-penetration step -- */
+/* No reference counterpart: gradient descent on the penetration penalty
+ * L = sum of max(0, -sd)^2 over the pairs the broadphase finds (AABBs padded
+ * by aabb_margin), sd the signed distance of bch_sd_forward, for the bodies
+ * loaded by bch_world_set_bodies.  One step, on the device with no host
+ * round trip:
+ *   broadphase -> signed distance of every pair -> record (loss, largest
+ *   penetration, pair and penetrating-pair counts) -> adjoint of the
+ *   penetrating pairs (bch_sd_backward with dL/dsd = -2 max(0, -sd)) ->
+ *   per-body gradient sum -> d_position -= lr dL/dposition,
+ *   d_orientation = normalize(d_orientation - lr dL/dorientation).
+ * d_position [3n] / d_orientation [4n] are DEVICE arrays, read at the start
+ * of each step and updated in place; they must outlive the capture.  The
+ * step is captured in a HIP graph after one uncaptured step without the
+ * update (poses unchanged) has sized the buffers for pair_capacity pairs;
+ * *found_pairs (may be NULL) receives that step's pair count, BCH_OVERFLOW
+ * when it exceeds the capacity.  history_capacity steps are recorded. */
+int bch_depen_capture(bch_context* ctx, float aabb_margin, float lr, float* d_position, float* d_orientation,
+                      uint64_t pair_capacity, uint32_t history_capacity, uint64_t* found_pairs);
+/* Launch the captured step `steps` times on the context stream (no
+ * synchronisation).  BCH_INVALID_ARG when nothing is captured or a buffer
+ * was reallocated since. */
+int bch_depen_replay(bch_context* ctx, uint32_t steps);
+/* Wait, then copy the records of the steps run since the capture (or the
+ * last reset) into the arrays (any may be NULL; at most `capacity` each):
+ * loss, largest penetration, broadphase pairs found, penetrating pairs.
+ * *count = steps run.  reset != 0 starts a new history.  BCH_OVERFLOW when a
+ * step found more pairs than the capacity (its gradient missed pairs:
+ * capture again with a larger capacity) or ran past the history capacity. */
+int bch_depen_history(bch_context* ctx, float* loss, float* max_pen, uint32_t* n_pairs, uint32_t* n_pen,
+                      uint32_t capacity, uint32_t* count, int reset);
+/* Wait, then free the captured step (it holds the pose array pointers;
+ * bch_world_set_bodies and bch_destroy free it too). */
+int bch_depen_release(bch_context* ctx);
 
 #ifdef __cplusplus
 }

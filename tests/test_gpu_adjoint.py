@@ -316,3 +316,57 @@ def test_depenetration_reduces_penetration(torch, bpa, scenes, ctx, diff):
            f"{hist[0][1]:.4g} -> {hist[-1][1]:.4g}")
     assert hist[-1][0] < 0.05 * hist[0][0]
     assert hist[-1][1] < 0.5 * hist[0][1]
+
+
+def test_captured_depenetration_matches_autograd_loop(torch, bpa, scenes, ctx, diff):
+    """The captured step (bch_depen_capture: broadphase, signed distance,
+    adjoint of the penetrating pairs, gradient sum and pose update in one
+    graph) against the autograd loop from the same poses:
+      - the first step's record (loss, largest penetration, pairs) agrees to
+        f32 summation order;
+      - the poses after one step agree to the rounding that the order of
+        the per-body gradient sums (float atomics on both sides) leaves;
+      - later steps start from poses that differ in the last bits, and EPA's
+        1e-4 convergence tolerance lets that grow, so the next step's loss
+        is held to 1e-3 only."""
+    _flush("case: captured depenetration vs autograd loop, 16 worlds")
+    sc = scenes.batched_boxes_hulls(16, 21, 64, 256)
+    w = bpa.CollisionWorld(sc, ctx)
+    pos = torch.from_numpy(sc["position"]).cuda()
+    quat = torch.from_numpy(sc["orientation"]).cuda()
+    p1_ref, q1_ref, h1_ref = diff.depenetrate(w, pos, quat, steps=1)
+    p1_g, q1_g, h1_g = diff.depenetrate_graph(w, pos, quat, steps=1)
+    (l_a, m_a, n_a), (l_b, m_b, n_b) = h1_ref[0], h1_g[0]
+    _flush(f"step 0: loss autograd {l_a:.8g} graph {l_b:.8g}; max pen {m_a:.7g} / {m_b:.7g}; pairs {n_a} / {n_b}")
+    assert n_a == n_b and m_a == m_b
+    assert abs(l_a - l_b) <= 1e-6 * l_a
+    dp = (p1_g - p1_ref).abs().max().item()
+    dq = (q1_g - q1_ref).abs().max().item()
+    moved = (p1_ref - pos).abs().max().item()
+    _flush(f"after 1 step: max |dpos| {dp:.3g} (largest move {moved:.3g}), max |dquat| {dq:.3g}")
+    assert dp < 1e-5 and dq < 1e-5
+    # two steps, history read between chunks of one
+    _, _, h_ref = diff.depenetrate(w, pos, quat, steps=2)
+    _, _, h_g = diff.depenetrate_graph(w, pos, quat, steps=2, chunk=1)
+    assert len(h_g) == 2
+    _flush(f"step 1: loss autograd {h_ref[1][0]:.7g} graph {h_g[1][0]:.7g}; pairs {h_ref[1][2]} / {h_g[1][2]}")
+    assert abs(h_ref[1][0] - h_g[1][0]) <= 1e-3 * h_ref[1][0]
+    assert abs(h_ref[1][2] - h_g[1][2]) <= 2
+    # the input poses are untouched
+    assert torch.equal(pos, torch.from_numpy(sc["position"]).cuda())
+
+
+def test_captured_depenetration_reduces_penetration(torch, bpa, scenes, ctx, diff):
+    """40 captured steps push the bodies apart like the autograd loop does."""
+    _flush("case: captured depenetration, 16 worlds, 40 steps")
+    sc = scenes.batched_boxes_hulls(16, 21, 64, 256)
+    w = bpa.CollisionWorld(sc, ctx)
+    pos = torch.from_numpy(sc["position"]).cuda()
+    quat = torch.from_numpy(sc["orientation"]).cuda()
+    p, q, hist = diff.depenetrate_graph(w, pos, quat, steps=40, chunk=16)
+    _flush(f"captured depenetration: loss {hist[0][0]:.4g} -> {hist[-1][0]:.4g}, max penetration "
+           f"{hist[0][1]:.4g} -> {hist[-1][1]:.4g}")
+    assert hist[-1][0] < 0.05 * hist[0][0]
+    assert hist[-1][1] < 0.5 * hist[0][1]
+    n = q.norm(dim=1)
+    assert torch.all((n - 1).abs() < 1e-5)
