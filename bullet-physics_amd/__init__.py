@@ -292,6 +292,8 @@ class CollisionWorld:
         self.n_pairs = 0
         self.n_contacts = 0
         self.found = (0, 0)  # counts the last captured-pass replay found
+        self.n_mpairs = 0
+        self.n_mpoints = 0
 
     def set_poses(self, position, orientation) -> None:
         p = np.ascontiguousarray(position, np.float32)
@@ -326,6 +328,62 @@ class CollisionWorld:
         out = self.ctx.tuning()
         out["pass_ms"] = ms.value
         return out
+
+    # ------------------------------------------ persistent manifolds --
+    # body flags and pair routes of the manifold pass
+    BODY_STATIC, BODY_ACTIVE, BODY_FAST = 1, 2, 4
+    ROUTES = ("filtered", "sleeping", "static", "served", "detected", "removed")
+
+    def set_body_flags(self, flags=None, group_mask=None) -> None:
+        """Per-body flags (BODY_STATIC | BODY_ACTIVE | BODY_FAST) and
+        collision group | mask << 16; None = active, dynamic, slow bodies
+        that collide with everything."""
+        self._bflags = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+        self._bgm = None if group_mask is None else np.ascontiguousarray(group_mask, np.uint32)
+        _check(self.ctx.h, self.L.bch_world_set_body_flags(
+            self.ctx.h, None if self._bflags is None else self._bflags.ctypes.data,
+            None if self._bgm is None else self._bgm.ctypes.data))
+
+    def manifold_pass(self, aabb_margin: float = 0.02, linear_slop: float = 0.005, warm_start: float = 0.85):
+        """The step's pair pass with persistent manifolds (step.rs:660-854)
+        at the current poses; returns (n_pairs, n_points)."""
+        P, Np = C.c_uint64(), C.c_uint64()
+        _check(self.ctx.h, self.L.bch_world_manifold_pass(self.ctx.h, aabb_margin, linear_slop, warm_start,
+                                                          C.byref(P), C.byref(Np)))
+        self.n_mpairs, self.n_mpoints = P.value, Np.value
+        return self.n_mpairs, self.n_mpoints
+
+    def manifold_results(self):
+        """(pairs u32[P, 2], route u8[P], point_offsets u32[P+1], points
+        f32[Np, 12]) of the last manifold pass; a point is position on B,
+        normal A->B, penetration, warm-started impulse, lifetime (u32 bits),
+        A-side anchor."""
+        P, Np = self.n_mpairs, self.n_mpoints
+        pairs = np.zeros((max(P, 1), 2), np.uint32)
+        route = np.zeros(max(P, 1), np.uint8)
+        offs = np.zeros(P + 1, np.uint32)
+        pts = np.zeros((max(Np, 1), 12), np.float32)
+        _check(self.ctx.h, self.L.bch_world_manifold_results(self.ctx.h, pairs.ctypes.data, route.ctypes.data,
+                                                             offs.ctypes.data, pts.ctypes.data))
+        return pairs[:P], route[:P], offs, pts[:Np]
+
+    def manifold_impulses(self, impulses) -> None:
+        """Post-solve writeback of the last pass's point impulses."""
+        imp = np.ascontiguousarray(impulses, np.float32)
+        if imp.shape[0] != self.n_mpoints:
+            raise ValueError(f"{imp.shape[0]} impulses for {self.n_mpoints} points")
+        _check(self.ctx.h, self.L.bch_world_manifold_impulses(self.ctx.h, imp.ctypes.data))
+
+    def manifold_cache(self):
+        """(valid u8[P], entries f32[P, 64]) after the last manifold pass."""
+        P = self.n_mpairs
+        valid = np.zeros(max(P, 1), np.uint8)
+        ent = np.zeros((max(P, 1), 64), np.float32)
+        _check(self.ctx.h, self.L.bch_world_manifold_cache(self.ctx.h, valid.ctypes.data, ent.ctypes.data))
+        return valid[:P], ent[:P]
+
+    def manifold_reset(self) -> None:
+        _check(self.ctx.h, self.L.bch_world_manifold_reset(self.ctx.h))
 
     # ------------------------------------------------ captured pass --
     def capture(self, aabb_margin: float = 0.02, pair_capacity: int | None = None,

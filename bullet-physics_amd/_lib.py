@@ -24,6 +24,8 @@ EXPORTS = [
     "bch_sd_forward", "bch_sd_backward", "bch_set_stream", "bch_reset_stream", "bch_autotune",
     "bch_world_capture", "bch_world_replay", "bch_world_replay_counts", "bch_world_release_graph",
     "bch_depen_capture", "bch_depen_replay", "bch_depen_history", "bch_depen_release",
+    "bch_world_set_body_flags", "bch_world_manifold_pass", "bch_world_manifold_results",
+    "bch_world_manifold_impulses", "bch_world_manifold_cache", "bch_world_manifold_reset",
 ]
 
 # launch-shape knobs (bch_set_tuning)
@@ -109,6 +111,12 @@ def load():
     L.bch_world_release_graph.argtypes = [vp]
     L.bch_depen_capture.argtypes = [vp, f32, f32, vp, vp, u64, u32, C.POINTER(u64)]
     L.bch_depen_replay.argtypes = [vp, u32]
+    L.bch_world_set_body_flags.argtypes = [vp, vp, vp]
+    L.bch_world_manifold_pass.argtypes = [vp, f32, f32, f32, C.POINTER(u64), C.POINTER(u64)]
+    L.bch_world_manifold_results.argtypes = [vp, vp, vp, vp, vp]
+    L.bch_world_manifold_impulses.argtypes = [vp, vp]
+    L.bch_world_manifold_cache.argtypes = [vp, vp, vp]
+    L.bch_world_manifold_reset.argtypes = [vp]
     L.bch_depen_release.argtypes = [vp]
     L.bch_depen_history.argtypes = [vp, vp, vp, vp, vp, u32, C.POINTER(u32), C.c_int]
     _lib = L

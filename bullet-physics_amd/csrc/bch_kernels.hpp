@@ -640,7 +640,8 @@ __global__ __launch_bounds__(256) void k_np(const Counters* __restrict__ cnt_ro,
       float thr = mn(A.threshold, B.threshold);  // narrowphase/mod.rs:52-57
       Manifold4 m;
       V3 sd{0.0f, 0.0f, 0.0f};
-      bool has_seed = CLS == CLS_GJK && seeds != nullptr;
+      // a seed whose x is NaN is None for that pair
+      bool has_seed = CLS == CLS_GJK && seeds != nullptr && !is_nan(seeds[3 * p]);
       if (has_seed) sd = V3{seeds[3 * p], seeds[3 * p + 1], seeds[3 * p + 2]};
       int r = collide_pair(pr, thr, has_seed, sd, m, g, CLS);
       if (m.n < 0) {
@@ -857,7 +858,7 @@ __global__ __launch_bounds__(256) void k_np_hull(const Counters* __restrict__ cn
       __builtin_amdgcn_wave_barrier();
       float thr = mn(A.threshold, B.threshold);
       V3 sd{0.0f, 0.0f, 0.0f};
-      bool has_seed = seeds != nullptr;
+      bool has_seed = seeds != nullptr && !is_nan(seeds[3 * p]);  // NaN x: None
       if (has_seed) sd = V3{seeds[3 * p], seeds[3 * p + 1], seeds[3 * p + 2]};
       g = gjk_contact_t(pr, has_seed, sd, thr, GroupSupport<G>{&pr, sub});
       need_epa = g.kind == 2;

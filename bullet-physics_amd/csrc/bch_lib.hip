@@ -16,6 +16,7 @@
 #include "bch_adjoint.hpp"
 #include "bch_sort.hpp"
 #include "bch_depen.hpp"
+#include "bch_manifold.hpp"
 
 using namespace bch;
 
@@ -94,6 +95,16 @@ struct bch_context {
   uint64_t dgraph_gen = 0;
   uint32_t dhist_cap = 0;
   DevBuf sd_out, sd_status, pen_pairs, pen_g, g14, body_grad, pen_part, dhist, dstep;
+
+  // persistent manifolds: the cache written by the last pair pass, one entry
+  // per pair of that pass (keys, valid flags, 256-B entries), double-buffered
+  DevBuf mkeys[2], mvalid[2], ment[2];
+  int mcur = 0;     // the cache the next pass reads
+  uint64_t mM = 0;  // its pair count
+  DevBuf bflags, bgm;
+  bool has_bflags = false, has_bgm = false;
+  DevBuf mroute, msrc, mcnt, mslots, moff, mpts, mseeds, mimp;
+  uint64_t mNP = 0;  // points emitted by the last pass
 
   // proxy API state (host mirror of the broadphase slots)
   std::vector<float> proxy_box;  // 6 per slot
@@ -366,9 +377,9 @@ int ensure_pass_buffers(bch_context* c, uint32_t n) {
   return BCH_OK;
 }
 
-// Narrowphase over c->pair_keys (sorted, c->P pairs) with the world bodies.
-// Fills c->pairs (2P), c->coffsets (P+1), c->contacts (7C), c->C.
-int run_narrowphase(bch_context* c, const float* d_seeds) {
+// First part of the narrowphase: buffers, and the pair split of
+// c->pair_keys into (i, j), shape classes and the identity permutation.
+int narrow_split(bch_context* c) {
   hipStream_t s = c->stream;
   Counters* cnt = c->counters.as<Counters>();
   uint64_t P = c->P;
@@ -385,10 +396,22 @@ int run_narrowphase(bch_context* c, const float* d_seeds) {
   // capacity mode: P is the capacity, the device holds the count
   const uint32_t* P_dev = c->cap_mode ? &cnt->n_pairs : nullptr;
   hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
-  if (P) {
+  if (P)
     hipLaunchKernelGGL(k_split_pairs, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, P_dev,
                        c->pair_keys.as<uint64_t>(), c->bodies.as<BodyRec>(), c->pairs.as<uint32_t>(),
                        c->cls.as<uint32_t>(), c->perm.as<uint32_t>());
+  return BCH_OK;
+}
+
+// Rest of the narrowphase: class sort, the per-class kernels (with per-pair
+// GJK seeds when d_seeds is given; NaN x = none), EPA, dense contacts.
+// Fills c->coffsets (P+1), c->contacts (7C), c->C.
+int narrow_finish(bch_context* c, const float* d_seeds) {
+  hipStream_t s = c->stream;
+  Counters* cnt = c->counters.as<Counters>();
+  uint64_t P = c->P;
+  const uint32_t* P_dev = c->cap_mode ? &cnt->n_pairs : nullptr;
+  if (P) {
     // one 8-bit digit holds the class; one pass leaves the result in the alt pair
     int cs[1] = {0};
     bool calt = false;
@@ -464,6 +487,14 @@ int run_narrowphase(bch_context* c, const float* d_seeds) {
   HIP_TRY(c, hipStreamSynchronize(s));
   if (c->host_cnt.errors & 2u) return fail(c, BCH_HIP_ERROR, "EPA polytope store overflow");
   return BCH_OK;
+}
+
+// Narrowphase over c->pair_keys (sorted, c->P pairs) with the world bodies.
+// Fills c->pairs (2P), c->coffsets (P+1), c->contacts (7C), c->C.
+int run_narrowphase(bch_context* c, const float* d_seeds) {
+  int rc = narrow_split(c);
+  if (rc) return rc;
+  return narrow_finish(c, d_seeds);
 }
 
 int upload_bodies(bch_context* c, const bch_bodies* b, const uint32_t* world_of_host) {
@@ -759,7 +790,9 @@ void bch_destroy(bch_context* c) {
                    &c->pair_keys_alt, &c->big_pairs, &c->pairs, &c->cls, &c->cls_alt, &c->perm, &c->perm_alt,
                    &c->ccount, &c->coffsets, &c->cslots, &c->contacts, &c->epa_q,
                    &c->scan_part, &c->rs_hist, &c->rs_offs, &c->sd_out, &c->sd_status, &c->pen_pairs,
-                   &c->pen_g, &c->g14, &c->body_grad, &c->pen_part, &c->dhist, &c->dstep};
+                   &c->pen_g, &c->g14, &c->body_grad, &c->pen_part, &c->dhist, &c->dstep, &c->mkeys[0],
+                   &c->mkeys[1], &c->mvalid[0], &c->mvalid[1], &c->ment[0], &c->ment[1], &c->bflags, &c->bgm,
+                   &c->mroute, &c->msrc, &c->mcnt, &c->mslots, &c->moff, &c->mpts, &c->mseeds, &c->mimp};
   for (DevBuf* b : all) b->release();
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->own_stream);
@@ -1127,6 +1160,8 @@ int bch_world_set_bodies(bch_context* c, const bch_bodies* bodies, uint32_t n_wo
   }
   release_graph(c);  // the captured graphs are shaped for the old bodies
   release_depen_graph(c);
+  c->mM = 0;  // cached manifolds name bodies by index
+  c->has_bflags = c->has_bgm = false;
   rc = upload_bodies(c, bodies, world_offsets ? world_of.data() : nullptr);
   if (rc) return rc;
   c->n = bodies->n;
@@ -1233,6 +1268,156 @@ int bch_world_replay_counts(bch_context* c, uint64_t* n_pairs, uint64_t* n_conta
   if (!c->graph_exec) return fail(c, BCH_INVALID_ARG, "no captured pass: call bch_world_capture first");
   HIP_TRY(c, hipSetDevice(c->device));
   return read_capacity_counts(c, n_pairs, n_contacts, found_pairs, found_contacts);
+}
+
+// --------------------------------------------- persistent contact manifolds --
+int bch_world_set_body_flags(bch_context* c, const uint8_t* flags, const uint32_t* group_mask) {
+  if (!c) return BCH_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  size_t n = std::max<uint32_t>(c->n, 1);
+  c->has_bflags = flags != nullptr && c->n;
+  c->has_bgm = group_mask != nullptr && c->n;
+  if (c->has_bflags) {
+    ENSURE(c, c->bflags, n);
+    HIP_TRY(c, hipMemcpyAsync(c->bflags.p, flags, c->n, hipMemcpyHostToDevice, s));
+  }
+  if (c->has_bgm) {
+    ENSURE(c, c->bgm, n * sizeof(uint32_t));
+    HIP_TRY(c, hipMemcpyAsync(c->bgm.p, group_mask, c->n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  }
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return BCH_OK;
+}
+
+int bch_world_manifold_pass(bch_context* c, float aabb_margin, float linear_slop, float warm_start,
+                            uint64_t* out_pairs, uint64_t* out_points) {
+  if (!c) return BCH_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  uint32_t n = c->n;
+  hipStream_t s = c->stream;
+  c->P = 0;
+  c->C = 0;
+  int rc = ensure_pass_buffers(c, std::max<uint32_t>(n, 1));
+  if (rc) return rc;
+  mark(c, 0);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, c->counters.as<Counters>());
+  if (n) {
+    run_aabbs(c, aabb_margin);
+    mark(c, 1);
+    rc = run_broadphase(c, n, c->batched ? c->world_of.as<uint32_t>() : nullptr,
+                        c->batched ? c->world_offsets.as<uint32_t>() : nullptr, false);
+    if (rc) return rc;
+  } else {
+    for (int k = 1; k <= 3; ++k) mark(c, k);
+  }
+  uint64_t P = c->P;
+  size_t Pm = std::max<uint64_t>(P, 1);
+  int nw = c->mcur ^ 1;
+  ENSURE(c, c->mkeys[nw], Pm * sizeof(uint64_t));
+  ENSURE(c, c->mvalid[nw], Pm);
+  ENSURE(c, c->ment[nw], Pm * 16 * sizeof(float4));
+  ENSURE(c, c->mroute, Pm);
+  ENSURE(c, c->msrc, Pm * sizeof(uint32_t));
+  ENSURE(c, c->mcnt, (Pm + 1) * sizeof(uint32_t));
+  ENSURE(c, c->moff, (Pm + 1) * sizeof(uint32_t));
+  ENSURE(c, c->mslots, Pm * 12 * sizeof(float4));
+  ENSURE(c, c->mseeds, Pm * 3 * sizeof(float));
+  rc = narrow_split(c);
+  if (rc) return rc;
+  int cur = c->mcur;
+  if (P)
+    hipLaunchKernelGGL(k_mf_pre, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->pair_keys.as<uint64_t>(),
+                       c->bodies.as<BodyRec>(), c->has_bflags ? c->bflags.as<uint8_t>() : nullptr,
+                       c->has_bgm ? c->bgm.as<uint32_t>() : nullptr, c->mkeys[cur].as<uint64_t>(),
+                       c->mvalid[cur].as<uint8_t>(), c->ment[cur].as<float4>(), (uint32_t)c->mM, linear_slop,
+                       warm_start, c->cls.as<uint32_t>(), c->ccount.as<uint32_t>(), c->mseeds.as<float>(),
+                       c->mroute.as<uint8_t>(), c->msrc.as<uint32_t>(), c->mvalid[nw].as<uint8_t>(),
+                       c->ment[nw].as<float4>(), c->mcnt.as<uint32_t>(), c->mslots.as<float4>());
+  rc = narrow_finish(c, c->mseeds.as<float>());
+  if (rc) return rc;
+  if (P)
+    hipLaunchKernelGGL(k_mf_post, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->pair_keys.as<uint64_t>(),
+                       c->bodies.as<BodyRec>(), c->ment[cur].as<float4>(), c->ccount.as<uint32_t>(),
+                       c->cslots.as<float4>(), warm_start, c->mroute.as<uint8_t>(), c->msrc.as<uint32_t>(),
+                       c->mvalid[nw].as<uint8_t>(), c->ment[nw].as<float4>(), c->mcnt.as<uint32_t>(),
+                       c->mslots.as<float4>());
+  rc = dev_scan(c, c->mcnt.as<uint32_t>(), c->moff.as<uint32_t>(), P, c->scan_part, s);
+  if (rc) return rc;
+  uint32_t total = 0;
+  HIP_TRY(c, hipMemcpyAsync(&total, c->moff.as<uint32_t>() + P, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  ENSURE(c, c->mpts, (size_t)std::max<uint32_t>(total, 1) * 3 * sizeof(float4));
+  if (P) {
+    hipLaunchKernelGGL(k_mf_gather, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->mcnt.as<uint32_t>(),
+                       c->moff.as<uint32_t>(), c->mslots.as<float4>(), c->mpts.as<float4>());
+    HIP_TRY(c, hipMemcpyAsync(c->mkeys[nw].p, c->pair_keys.p, P * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+  }
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipStreamSynchronize(s));
+  c->mcur = nw;
+  c->mM = P;
+  c->mNP = total;
+  for (int k = 0; k < 8; ++k) (void)hipEventElapsedTime(&c->timings[k], c->ev[k], c->ev[k + 1]);
+  if (c->host_cnt.errors & 1u) return fail(c, BCH_HIP_ERROR, "box-box clip polygon overflow");
+  if (out_pairs) *out_pairs = P;
+  if (out_points) *out_points = total;
+  return BCH_OK;
+}
+
+int bch_world_manifold_results(bch_context* c, uint32_t* pairs, uint8_t* route, uint32_t* point_offsets,
+                               float* points) {
+  if (!c) return BCH_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  uint64_t P = c->mM;
+  if (pairs && P) HIP_TRY(c, hipMemcpyAsync(pairs, c->pairs.p, P * 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (route && P) HIP_TRY(c, hipMemcpyAsync(route, c->mroute.p, P, hipMemcpyDeviceToHost, s));
+  if (point_offsets) {
+    if (P)
+      HIP_TRY(c, hipMemcpyAsync(point_offsets, c->moff.p, (P + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    else
+      point_offsets[0] = 0;
+  }
+  if (points && c->mNP)
+    HIP_TRY(c, hipMemcpyAsync(points, c->mpts.p, c->mNP * 12 * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return BCH_OK;
+}
+
+int bch_world_manifold_impulses(bch_context* c, const float* impulses) {
+  if (!c) return BCH_INVALID_ARG;
+  if (!impulses && c->mNP) return fail(c, BCH_INVALID_ARG, "NULL impulses");
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (!c->mM || !c->mNP) return BCH_OK;
+  hipStream_t s = c->stream;
+  ENSURE(c, c->mimp, c->mNP * sizeof(float));
+  HIP_TRY(c, hipMemcpyAsync(c->mimp.p, impulses, c->mNP * sizeof(float), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_mf_impulses, dim3(blocks(c->mM)), dim3(kBlock), 0, s, (uint32_t)c->mM, c->mcnt.as<uint32_t>(),
+                     c->moff.as<uint32_t>(), c->mvalid[c->mcur].as<uint8_t>(), c->mimp.as<float>(),
+                     c->ment[c->mcur].as<float4>());
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return BCH_OK;
+}
+
+int bch_world_manifold_cache(bch_context* c, uint8_t* valid, float* entries) {
+  if (!c) return BCH_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  uint64_t M = c->mM;
+  if (valid && M) HIP_TRY(c, hipMemcpyAsync(valid, c->mvalid[c->mcur].p, M, hipMemcpyDeviceToHost, s));
+  if (entries && M)
+    HIP_TRY(c, hipMemcpyAsync(entries, c->ment[c->mcur].p, M * 64 * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return BCH_OK;
+}
+
+int bch_world_manifold_reset(bch_context* c) {
+  if (!c) return BCH_INVALID_ARG;
+  c->mM = 0;
+  c->mNP = 0;
+  return BCH_OK;
 }
 
 // ---------------------------------------------- captured de-penetration --

@@ -127,7 +127,8 @@ int bch_bp_raycast_candidates(bch_context* ctx, const float ray_from[3], const f
                               uint64_t capacity, uint64_t* out_count);
 /* collide_seeded (narrowphase/mod.rs:63-286) for every pair: shape A =
  * pairs[2k], shape B = pairs[2k+1], seeds[3k..] an optional GJK warm-start
- * direction (NULL = None).  out_count[k] = number of contact points (0 =
+ * direction (NULL = None for every pair; a NaN x component = None for that
+ * pair).  out_count[k] = number of contact points (0 =
  * None); out_points[28k + 7c ..] = contact c.  Host pointers. */
 int bch_collide_batch(bch_context* ctx, const bch_bodies* bodies, const uint32_t* pairs, uint64_t n_pairs,
                       const float* seeds, uint8_t* out_count, float* out_points);
@@ -227,6 +228,64 @@ int bch_world_replay_counts(bch_context* ctx, uint64_t* n_pairs, uint64_t* n_con
                             uint64_t* found_contacts);
 /* Free the captured pass (bch_world_set_bodies and bch_destroy do too). */
 int bch_world_release_graph(bch_context* ctx);
+
+/* ------------------------------------------- persistent contact manifolds -- */
+/* The rigid-pair arm of the world step's pair pass with persistent
+ * manifolds (PersistentManifold, crates/bullet-collision/src/manifold.rs:
+ * 124-386, driven as in crates/bullet-dynamics/src/world/step.rs:660-854),
+ * for the bodies loaded by bch_world_set_bodies at their current poses
+ * (positions are the step's predicted positions).  Per broadphase pair:
+ *   - group / mask filter (step.rs:660-664): the pair is dropped;
+ *   - both bodies inactive, or both static / kinematic: the cached manifold
+ *     is kept as it is, nothing is emitted (step.rs:683-713);
+ *   - a cached manifold within the redetect drift (5 mm, 0.01 rad) is
+ *     refreshed and its points emitted, unless a body is fast and a point
+ *     is deeper than linear_slop (step.rs:715-776);
+ *   - otherwise collide_seeded runs with the cache's seed; its manifold is
+ *     captured (merge, impulse inheritance, deepest four) and the refreshed
+ *     points emitted, or the entry is dropped when it finds nothing
+ *     (step.rs:777-852).
+ * Emitted impulses are the cached ones times warm_start (step.rs:751, 821).
+ * The cache lives on the device between calls; entries of pairs that are
+ * no longer broadphase pairs are dropped (the prune of step.rs:1006-1010).
+ * Per-body flags: BCH_BODY_STATIC (static or kinematic), BCH_BODY_ACTIVE,
+ * BCH_BODY_FAST (settled speed^2 above the redetect speed, world.rs:62). */
+#define BCH_BODY_STATIC 1
+#define BCH_BODY_ACTIVE 2
+#define BCH_BODY_FAST 4
+/* Routes per pair. */
+#define BCH_ROUTE_FILTERED 0
+#define BCH_ROUTE_SLEEPING 1
+#define BCH_ROUTE_STATIC 2
+#define BCH_ROUTE_SERVED 3   /* refreshed from the cache */
+#define BCH_ROUTE_DETECTED 4 /* collide_seeded + capture */
+#define BCH_ROUTE_REMOVED 5  /* collide_seeded found nothing: entry dropped */
+/* Per-body flags [n] and collision group | mask << 16 [n] (host arrays; NULL
+ * = every body active and dynamic and slow / everything collides).  Kept
+ * until the next call or bch_world_set_bodies. */
+int bch_world_set_body_flags(bch_context* ctx, const uint8_t* flags, const uint32_t* group_mask);
+/* One pair pass at the current poses (synchronous).  *out_pairs = broadphase
+ * pairs, *out_points = emitted points. */
+int bch_world_manifold_pass(bch_context* ctx, float aabb_margin, float linear_slop, float warm_start,
+                            uint64_t* out_pairs, uint64_t* out_points);
+/* The last pass's results (any pointer may be NULL): pairs [2P], route [P],
+ * per-pair point offsets [P+1] and the points [12 * n_points]: position on
+ * B xyz, normal A->B xyz, penetration, normal impulse (warm-started),
+ * lifetime (uint32 bits), A-side anchor xyz (position A + orientation A *
+ * the cached local anchor, for friction anchoring). */
+int bch_world_manifold_results(bch_context* ctx, uint32_t* pairs, uint8_t* route, uint32_t* point_offsets,
+                               float* points);
+/* Post-solve writeback (step.rs:978-1005): impulses [n_points] of the last
+ * pass's points, in results order, into their cached points. */
+int bch_world_manifold_impulses(bch_context* ctx, const float* impulses);
+/* No reference counterpart (diagnostic): the cache after the last pass, per
+ * pair: valid [P] and entries [64 P] floats (4 points x (local anchor A xyz,
+ * penetration, local anchor B xyz, impulse, normal xyz, lifetime bits),
+ * detect position A, threshold, detect orientation A, detect position B,
+ * count | initialised << 8 bits, detect orientation B). */
+int bch_world_manifold_cache(bch_context* ctx, uint8_t* valid, float* entries);
+/* Drop every cached manifold. */
+int bch_world_manifold_reset(bch_context* ctx);
 
 /* ------------------------------------------ captured de-penetration step -- */
 /* No reference counterpart: gradient descent on the penetration penalty

@@ -1623,4 +1623,265 @@ bool collide_seeded(const Shape<T>& sa, V3<T> pa, Quat<T> qa, const Shape<T>& sb
   return convex_convex(pr, seed, threshold, m, path);
 }
 
+// --------------------------------------------------- persistent manifold --
+// PersistentManifold (crates/bullet-collision/src/manifold.rs:124-386) and
+// the rigid-pair arm of the world step's pair pass
+// (crates/bullet-dynamics/src/world/step.rs:660-854), in f32 like the
+// reference.  Quaternion products follow glam's scalar Quat * Quat order.
+
+// manifold.rs:146-152, world.rs:62
+constexpr float PM_REDRIFT_T = 0.005f;
+constexpr float PM_REDRIFT_T2 = PM_REDRIFT_T * PM_REDRIFT_T;
+constexpr float PM_REDRIFT_ANGLE = 0.01f;
+constexpr int PM_CACHE = 4;
+
+// glam Quat * Quat (Hamilton product, scalar path).
+inline Quat<float> qmul(Quat<float> a, Quat<float> b) {
+  return {a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+          a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
+}
+
+// manifold.rs:154-158: 2 acos(clamp(w, -1, 1))
+inline float rotation_angle(Quat<float> q) {
+  float w = q.w < -1.0f ? -1.0f : (q.w > 1.0f ? 1.0f : q.w);
+  return 2.0f * std::acos(w);
+}
+
+// manifold.rs:124-144
+struct PPoint {
+  V3<float> local_a, local_b, normal;
+  float penetration = 0.0f, normal_impulse = 0.0f;
+  uint32_t lifetime = 0;
+};
+
+// manifold.rs:160-184 (points may exceed four inside capture only)
+struct PMan {
+  PPoint pts[PM_CACHE + 8];
+  int n = 0;
+  V3<float> dpa{0.0f, 0.0f, 0.0f}, dpb{0.0f, 0.0f, 0.0f};
+  Quat<float> dqa{0.0f, 0.0f, 0.0f, 1.0f}, dqb{0.0f, 0.0f, 0.0f, 1.0f};
+  bool initialized = false;
+  float threshold = 0.02f;  // CONTACT_BREAKING_THRESHOLD, manifold.rs:8
+};
+
+// A refreshed point as the step emits it (ContactPoint, manifold.rs:12-41).
+struct PContact {
+  V3<float> position, normal;
+  float penetration, normal_impulse;
+  uint32_t lifetime;
+};
+
+// refresh (manifold.rs:252-298): re-project, drop separated or sideways-
+// drifted points, emit the survivors in cache order.
+inline int pm_refresh(PMan& m, V3<float> pa, Quat<float> qa, V3<float> pb, Quat<float> qb, PContact* out) {
+  float thr = m.threshold;
+  float thr2 = thr * thr;
+  int w = 0;
+  for (int r = 0; r < m.n; ++r) {
+    PPoint p = m.pts[r];
+    V3<float> wa = pa + qrot(qa, p.local_a);
+    V3<float> wb = pb + qrot(qb, p.local_b);
+    V3<float> d = wa - wb;
+    float pen = dot(d, p.normal);
+    if (pen < -thr) continue;
+    V3<float> lat = d - p.normal * pen;
+    if (len2(lat) > thr2) continue;
+    p.penetration = pen;
+    p.lifetime += 1;
+    if (out) out[w] = PContact{wb, p.normal, pen, p.normal_impulse, p.lifetime};
+    m.pts[w++] = p;
+  }
+  m.n = w;
+  return w;
+}
+
+// needs_redetect (manifold.rs:300-318)
+inline bool pm_needs_redetect(const PMan& m, V3<float> pa, Quat<float> qa, V3<float> pb, Quat<float> qb) {
+  if (!m.initialized) return true;
+  V3<float> drift = (pb - m.dpb) - (pa - m.dpa);
+  if (len2(drift) > PM_REDRIFT_T2) return true;
+  Quat<float> ra = qmul(qa, qconj(m.dqa));
+  Quat<float> rb = qmul(qb, qconj(m.dqb));
+  return rotation_angle(ra) + rotation_angle(rb) > PM_REDRIFT_ANGLE;
+}
+
+// f32::total_cmp key (larger key = larger value)
+inline int64_t total_key(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  int32_t s = (int32_t)u;
+  return s < 0 ? -(int64_t)(u & 0x7fffffffu) - 1 : (int64_t)u;
+}
+
+// capture (manifold.rs:320-376): merge by A-side anchor (< threshold) and
+// normal (dot >= 0.9), inherit the impulse, append the rest; deepest four
+// on overflow (stable sort, total order).
+inline void pm_capture(PMan& m, const ContactPoint<float>* cps, int ncp, V3<float> pa, Quat<float> qa, V3<float> pb,
+                       Quat<float> qb, float threshold) {
+  m.threshold = threshold;
+  Quat<float> inv_a = qconj(qa), inv_b = qconj(qb);
+  float thr2 = threshold * threshold;
+  for (int k = 0; k < ncp; ++k) {
+    const ContactPoint<float>& cp = cps[k];
+    V3<float> wb = cp.position;
+    V3<float> wa = cp.position + cp.normal * cp.penetration;
+    PPoint pt;
+    pt.local_a = qrot(inv_a, wa - pa);
+    pt.local_b = qrot(inv_b, wb - pb);
+    pt.normal = cp.normal;
+    pt.penetration = cp.penetration;
+    int slot = -1;
+    for (int o = 0; o < m.n; ++o) {
+      if (len2(m.pts[o].local_a - pt.local_a) < thr2 && dot(m.pts[o].normal, pt.normal) >= 0.9f) {
+        slot = o;
+        break;
+      }
+    }
+    if (slot >= 0) {
+      pt.normal_impulse = m.pts[slot].normal_impulse;
+      m.pts[slot] = pt;
+    } else {
+      m.pts[m.n++] = pt;
+    }
+  }
+  if (m.n > PM_CACHE) {
+    std::stable_sort(m.pts, m.pts + m.n,
+                     [](const PPoint& a, const PPoint& b) { return total_key(a.penetration) > total_key(b.penetration); });
+    m.n = PM_CACHE;
+  }
+  m.dpa = pa;
+  m.dqa = qa;
+  m.dpb = pb;
+  m.dqb = qb;
+  m.initialized = true;
+}
+
+// seed (manifold.rs:378-385)
+inline bool pm_seed(const PMan& m, V3<float>* out) {
+  if (m.n == 0 || !(len2(m.pts[0].normal) > 0.5f)) return false;
+  *out = m.pts[0].normal;
+  return true;
+}
+
+// Per-body flags of the pair pass (the body state step.rs reads).
+enum : uint8_t { PM_STATIC = 1, PM_ACTIVE = 2, PM_FAST = 4 };
+// Routes of a pair through the pass.
+enum : uint8_t { PM_FILTERED = 0, PM_SLEEPING = 1, PM_STATIC_PAIR = 2, PM_SERVED = 3, PM_DETECTED = 4, PM_REMOVED = 5 };
+
+struct PmPassOut {
+  uint8_t route = PM_FILTERED;
+  int n_out = 0;
+  PContact out[PM_CACHE];
+  V3<float> anchor_a[PM_CACHE];  // pa + qa * cached local_a of each emitted point
+  bool keep = false;              // the pair has a cache entry after the pass
+};
+
+// The rigid-pair arm of the pair pass for one pair (step.rs:660-854):
+// group/mask filter (step.rs:660-664), sleeping and static pairs keep their
+// entry untouched (step.rs:683-704, 711-713), the refresh path unless the
+// pair is fast and a refreshed point is deeper than the slop
+// (step.rs:715-776), otherwise collide_seeded with the cache's seed and
+// capture + refresh (step.rs:777-847) or removal (step.rs:848-852).
+// Emitted impulses are scaled by the warm-start factor (step.rs:751, 821).
+inline PmPassOut pm_pair_pass(bool has_entry, PMan& m, const Shape<float>& sa, V3<float> pa, Quat<float> qa,
+                              const Shape<float>& sb, V3<float> pb, Quat<float> qb, uint8_t fa, uint8_t fb,
+                              uint32_t gma, uint32_t gmb, float slop, float warm) {
+  PmPassOut r;
+  uint32_t ga = gma & 0xffffu, ma = gma >> 16, gb = gmb & 0xffffu, mb = gmb >> 16;
+  if ((ga & mb) == 0 || (gb & ma) == 0) return r;
+  if (!(fa & PM_ACTIVE) && !(fb & PM_ACTIVE)) {
+    r.route = PM_SLEEPING;
+    r.keep = has_entry;
+    return r;
+  }
+  if ((fa & PM_STATIC) && (fb & PM_STATIC)) {
+    r.route = PM_STATIC_PAIR;
+    r.keep = has_entry;
+    return r;
+  }
+  bool fast = (fa & PM_FAST) || (fb & PM_FAST);
+  auto emit = [&](int n) {
+    r.n_out = n;
+    for (int k = 0; k < n; ++k) {
+      r.out[k].normal_impulse = r.out[k].normal_impulse * warm;
+      r.anchor_a[k] = pa + qrot(qa, m.pts[k].local_a);
+    }
+  };
+  if (has_entry && !pm_needs_redetect(m, pa, qa, pb, qb)) {
+    int n = pm_refresh(m, pa, qa, pb, qb, r.out);
+    bool deep = false;
+    for (int k = 0; k < n; ++k) deep = deep || r.out[k].penetration > slop;
+    if (!fast || !deep) {
+      emit(n);
+      r.route = PM_SERVED;
+      r.keep = true;
+      return r;
+    }
+  }
+  V3<float> seed;
+  bool has_seed = has_entry && pm_seed(m, &seed);
+  Manifold<float> man;
+  collide_seeded(sa, pa, qa, sb, pb, qb, has_seed ? &seed : nullptr, man);
+  if (man.n > 0) {
+    if (!has_entry) m = PMan();
+    pm_capture(m, man.pts, man.n, pa, qa, pb, qb, pair_threshold(sa, sb));
+    int n = pm_refresh(m, pa, qa, pb, qb, r.out);
+    emit(n);
+    r.route = PM_DETECTED;
+    r.keep = true;
+  } else {
+    r.route = PM_REMOVED;
+    r.keep = false;
+  }
+  return r;
+}
+
+// Entry serialisation shared with the device cache: 64 floats (256 B).
+//   point k (k < 4): [12k .. 12k+2] local_a, [12k+3] penetration,
+//                    [12k+4 .. +6] local_b, [12k+7] normal_impulse,
+//                    [12k+8 .. +10] normal, [12k+11] lifetime (u32 bits)
+//   [48..50] detect position A, [51] threshold, [52..55] detect orientation A,
+//   [56..58] detect position B, [59] n | initialized << 8 (u32 bits),
+//   [60..63] detect orientation B
+inline void pm_store(const PMan& m, float* e) {
+  std::memset(e, 0, 64 * sizeof(float));
+  for (int k = 0; k < m.n && k < PM_CACHE; ++k) {
+    const PPoint& p = m.pts[k];
+    float* q = e + 12 * k;
+    q[0] = p.local_a.x; q[1] = p.local_a.y; q[2] = p.local_a.z; q[3] = p.penetration;
+    q[4] = p.local_b.x; q[5] = p.local_b.y; q[6] = p.local_b.z; q[7] = p.normal_impulse;
+    q[8] = p.normal.x; q[9] = p.normal.y; q[10] = p.normal.z;
+    std::memcpy(q + 11, &p.lifetime, 4);
+  }
+  e[48] = m.dpa.x; e[49] = m.dpa.y; e[50] = m.dpa.z; e[51] = m.threshold;
+  e[52] = m.dqa.x; e[53] = m.dqa.y; e[54] = m.dqa.z; e[55] = m.dqa.w;
+  e[56] = m.dpb.x; e[57] = m.dpb.y; e[58] = m.dpb.z;
+  uint32_t nb = (uint32_t)m.n | (m.initialized ? 0x100u : 0u);
+  std::memcpy(e + 59, &nb, 4);
+  e[60] = m.dqb.x; e[61] = m.dqb.y; e[62] = m.dqb.z; e[63] = m.dqb.w;
+}
+
+inline void pm_load(const float* e, PMan& m) {
+  uint32_t nb;
+  std::memcpy(&nb, e + 59, 4);
+  m = PMan();
+  m.n = (int)(nb & 0xffu);
+  m.initialized = (nb & 0x100u) != 0;
+  for (int k = 0; k < m.n; ++k) {
+    const float* q = e + 12 * k;
+    PPoint& p = m.pts[k];
+    p.local_a = mk(q[0], q[1], q[2]);
+    p.penetration = q[3];
+    p.local_b = mk(q[4], q[5], q[6]);
+    p.normal_impulse = q[7];
+    p.normal = mk(q[8], q[9], q[10]);
+    std::memcpy(&p.lifetime, q + 11, 4);
+  }
+  m.dpa = mk(e[48], e[49], e[50]);
+  m.threshold = e[51];
+  m.dqa = Quat<float>{e[52], e[53], e[54], e[55]};
+  m.dpb = mk(e[56], e[57], e[58]);
+  m.dqb = Quat<float>{e[60], e[61], e[62], e[63]};
+}
+
 }  // namespace oracle

@@ -76,6 +76,15 @@ def lib():
         L.oracle_collide_pairs_f32.argtypes = [scp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                                C.c_void_p]
         L.oracle_collide_pairs_f64.argtypes = [scp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
+        L.oracle_pm_capture.argtypes = [C.c_void_p, C.c_void_p, C.c_int] + [C.c_void_p] * 4 + [C.c_float]
+        L.oracle_pm_refresh.restype = C.c_int
+        L.oracle_pm_refresh.argtypes = [C.c_void_p] * 6
+        L.oracle_pm_needs_redetect.restype = C.c_int
+        L.oracle_pm_needs_redetect.argtypes = [C.c_void_p] * 5
+        L.oracle_pm_seed.restype = C.c_int
+        L.oracle_pm_seed.argtypes = [C.c_void_p] * 2
+        L.oracle_pm_pass.argtypes = ([scp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_float, C.c_float]
+                                     + [C.c_void_p] * 3 + [C.c_int64] + [C.c_void_p] * 5)
         L.oracle_pass_f32.restype = C.c_int64
         L.oracle_pass_f32.argtypes = [scp, C.c_float, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         _lib = L
@@ -372,3 +381,83 @@ def collision_pass(scene, margin):
     sec = C.c_double(0)
     npairs = lib().oracle_pass_f32(C.byref(r.c), margin, C.byref(nc), C.byref(sec))
     return int(npairs), int(nc.value), float(sec.value)
+
+
+# ----------------------------------------------------- persistent manifold --
+# One cache entry is 64 float32 (layout in oracle.hpp::pm_store); an emitted
+# point is 12 float32: position, normal, penetration, impulse, lifetime
+# (uint32 bits), A-side anchor.
+PM_FILTERED, PM_SLEEPING, PM_STATIC_PAIR, PM_SERVED, PM_DETECTED, PM_REMOVED = 0, 1, 2, 3, 4, 5
+PM_STATIC, PM_ACTIVE, PM_FAST = 1, 2, 4
+
+
+def pm_new():
+    """An empty PersistentManifold (manifold.rs:188-199)."""
+    e = np.zeros(64, np.float32)
+    e[51] = np.float32(0.02)
+    e[55] = e[63] = np.float32(1.0)
+    return e
+
+
+def pm_count(e):
+    return int(np.asarray(e, np.float32).view(np.uint32)[59] & 0xff)
+
+
+def pm_points(e):
+    """[n, 12] rows of the cached points (local_a, pen, local_b, impulse, normal, lifetime bits)."""
+    return np.asarray(e, np.float32)[:12 * pm_count(e)].reshape(-1, 12)
+
+
+def pm_capture(e, points7, pa, qa, pb, qb, threshold=0.02):
+    """capture (manifold.rs:320-376) in place; points7 [k, 7] = position, normal, penetration."""
+    pts = np.ascontiguousarray(points7, np.float32).reshape(-1, 7)
+    lib().oracle_pm_capture(e.ctypes.data, pts.ctypes.data, pts.shape[0], _f32(pa, 3), _f32(qa, 4), _f32(pb, 3),
+                            _f32(qb, 4), threshold)
+
+
+def pm_refresh(e, pa, qa, pb, qb):
+    """refresh (manifold.rs:252-298) in place -> emitted points [n, 12]."""
+    out = np.zeros((4, 12), np.float32)
+    n = lib().oracle_pm_refresh(e.ctypes.data, _f32(pa, 3), _f32(qa, 4), _f32(pb, 3), _f32(qb, 4), out.ctypes.data)
+    return out[:n]
+
+
+def pm_needs_redetect(e, pa, qa, pb, qb):
+    return bool(lib().oracle_pm_needs_redetect(e.ctypes.data, _f32(pa, 3), _f32(qa, 4), _f32(pb, 3), _f32(qb, 4)))
+
+
+def pm_seed(e):
+    out = np.zeros(3, np.float32)
+    return out if lib().oracle_pm_seed(e.ctypes.data, out.ctypes.data) else None
+
+
+def pm_pass(scene, pairs, cache, flags=None, group_mask=None, slop=0.005, warm=0.85):
+    """One step of the pair pass with persistent manifolds (step.rs:660-854).
+    cache = (keys u64[M], entries f32[M, 64], valid u8[M]) of the previous
+    step (or None).  Returns dict(route, count, points [P, 4, 12], cache=(keys,
+    entries, valid) of this step)."""
+    r = _SceneRef(scene)
+    pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+    P = pairs.shape[0]
+    if cache is None:
+        keys = np.zeros(1, np.uint64)
+        ent = np.zeros((1, 64), np.float32)
+        val = np.zeros(1, np.uint8)
+        M = 0
+    else:
+        keys, ent, val = (np.ascontiguousarray(cache[0], np.uint64), np.ascontiguousarray(cache[1], np.float32),
+                          np.ascontiguousarray(cache[2], np.uint8))
+        M = keys.shape[0]
+    fl = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+    gm = None if group_mask is None else np.ascontiguousarray(group_mask, np.uint32)
+    route = np.zeros(P, np.uint8)
+    valid = np.zeros(P, np.uint8)
+    ent_out = np.zeros((max(P, 1), 64), np.float32)
+    cnt = np.zeros(P, np.uint8)
+    pts = np.zeros((max(P, 1), 4, 12), np.float32)
+    lib().oracle_pm_pass(C.byref(r.c), pairs.ctypes.data, P, None if fl is None else fl.ctypes.data,
+                         None if gm is None else gm.ctypes.data, slop, warm, keys.ctypes.data, ent.ctypes.data,
+                         val.ctypes.data, M, route.ctypes.data, valid.ctypes.data, ent_out.ctypes.data,
+                         cnt.ctypes.data, pts.ctypes.data)
+    new_keys = (pairs[:, 0].astype(np.uint64) << np.uint64(32)) | pairs[:, 1].astype(np.uint64)
+    return dict(route=route, count=cnt, points=pts[:P], cache=(new_keys, ent_out[:P], valid))

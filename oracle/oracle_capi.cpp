@@ -394,9 +394,10 @@ void oracle_collide_pairs_f32(const or_scene* sc, const uint32_t* pairs, int64_t
     uint32_t i = pairs[2 * k], j = pairs[2 * k + 1];
     Manifold<float> m;
     V3<float> sd;
-    if (seeds) sd = v3p<float>(seeds + 3 * k);
+    bool has_seed = seeds && !std::isnan(seeds[3 * k]);  // NaN x: None for this pair
+    if (has_seed) sd = v3p<float>(seeds + 3 * k);
     int pth = 3;
-    collide_seeded(ss.shapes[i], ss.pos[i], ss.quat[i], ss.shapes[j], ss.pos[j], ss.quat[j], seeds ? &sd : nullptr,
+    collide_seeded(ss.shapes[i], ss.pos[i], ss.quat[i], ss.shapes[j], ss.pos[j], ss.quat[j], has_seed ? &sd : nullptr,
                    m, &pth);
     out_count[k] = (uint8_t)m.n;
     if (out_path) out_path[k] = (int8_t)pth;
@@ -506,6 +507,93 @@ int64_t oracle_pass_f32(const or_scene* sc, float margin, int64_t* contacts, dou
   *contacts = nc;
   *seconds = std::chrono::duration<double>(t1 - t0).count();
   return np;
+}
+
+// ---------------------------------------------------- persistent manifold --
+// PersistentManifold operations on one serialised entry (64 floats, layout in
+// oracle.hpp::pm_store).  Poses are (x, y, z) and (x, y, z, w).
+static void pm_emit(const PContact& c, V3<float> anchor, float* o) {
+  o[0] = c.position.x; o[1] = c.position.y; o[2] = c.position.z;
+  o[3] = c.normal.x; o[4] = c.normal.y; o[5] = c.normal.z;
+  o[6] = c.penetration; o[7] = c.normal_impulse;
+  std::memcpy(o + 8, &c.lifetime, 4);
+  o[9] = anchor.x; o[10] = anchor.y; o[11] = anchor.z;
+}
+
+// capture (manifold.rs:320-376) of n points (7 floats: position, normal, penetration).
+void oracle_pm_capture(float* e64, const float* pts7, int n, const float* pa, const float* qa, const float* pb,
+                       const float* qb, float threshold) {
+  PMan m;
+  pm_load(e64, m);
+  ContactPoint<float> cps[8];
+  for (int k = 0; k < n && k < 8; ++k)
+    cps[k] = ContactPoint<float>{v3p<float>(pts7 + 7 * k), v3p<float>(pts7 + 7 * k + 3), pts7[7 * k + 6]};
+  pm_capture(m, cps, n < 8 ? n : 8, v3p<float>(pa), q4p<float>(qa), v3p<float>(pb), q4p<float>(qb), threshold);
+  pm_store(m, e64);
+}
+
+// refresh (manifold.rs:252-298): returns the surviving count; out12[k] as
+// the pass emits it (impulse unscaled, anchor = pa + qa * local_a).
+int oracle_pm_refresh(float* e64, const float* pa, const float* qa, const float* pb, const float* qb, float* out12) {
+  PMan m;
+  pm_load(e64, m);
+  PContact out[PM_CACHE];
+  int n = pm_refresh(m, v3p<float>(pa), q4p<float>(qa), v3p<float>(pb), q4p<float>(qb), out);
+  for (int k = 0; k < n; ++k)
+    pm_emit(out[k], v3p<float>(pa) + qrot(q4p<float>(qa), m.pts[k].local_a), out12 + 12 * k);
+  pm_store(m, e64);
+  return n;
+}
+
+int oracle_pm_needs_redetect(const float* e64, const float* pa, const float* qa, const float* pb, const float* qb) {
+  PMan m;
+  pm_load(e64, m);
+  return pm_needs_redetect(m, v3p<float>(pa), q4p<float>(qa), v3p<float>(pb), q4p<float>(qb)) ? 1 : 0;
+}
+
+int oracle_pm_seed(const float* e64, float* out3) {
+  PMan m;
+  pm_load(e64, m);
+  V3<float> s;
+  if (!pm_seed(m, &s)) return 0;
+  out3[0] = s.x; out3[1] = s.y; out3[2] = s.z;
+  return 1;
+}
+
+// One step of the pair pass with persistent manifolds (step.rs:660-854) over
+// the sorted pairs of a scene.  The cache comes in as the previous step's
+// sorted keys with entries and valid flags, and goes out per pair of this
+// step (entries of pairs absent from this step are pruned, step.rs:1006-1010).
+// flags (NULL: every body active, dynamic, slow) and group_mask (group | mask
+// << 16; NULL: everything collides) per body.  Per pair: route, entry valid,
+// entry, emitted count and 4 x 12 floats (position, normal, penetration,
+// impulse x warm, lifetime bits, A-side anchor).
+void oracle_pm_pass(const or_scene* sc, const uint32_t* pairs, int64_t npairs, const uint8_t* flags,
+                    const uint32_t* group_mask, float slop, float warm, const uint64_t* keys_in, const float* ent_in,
+                    const uint8_t* valid_in, int64_t n_in, uint8_t* route, uint8_t* valid_out, float* ent_out,
+                    uint8_t* out_cnt, float* out_pts) {
+  SceneShapes<float> ss;
+  ss.build(sc);
+  for (int64_t p = 0; p < npairs; ++p) {
+    uint32_t i = pairs[2 * p], j = pairs[2 * p + 1];
+    uint64_t key = ((uint64_t)i << 32) | j;
+    const uint64_t* it = std::lower_bound(keys_in, keys_in + n_in, key);
+    int64_t e = (it != keys_in + n_in && *it == key) ? (int64_t)(it - keys_in) : -1;
+    bool has = e >= 0 && valid_in[e];
+    PMan m;
+    if (has) pm_load(ent_in + 64 * e, m);
+    uint8_t fa = flags ? flags[i] : (uint8_t)PM_ACTIVE, fb = flags ? flags[j] : (uint8_t)PM_ACTIVE;
+    uint32_t ga = group_mask ? group_mask[i] : 0xffffffffu, gb = group_mask ? group_mask[j] : 0xffffffffu;
+    PmPassOut r = pm_pair_pass(has, m, ss.shapes[i], ss.pos[i], ss.quat[i], ss.shapes[j], ss.pos[j], ss.quat[j], fa, fb,
+                               ga, gb, slop, warm);
+    route[p] = r.route;
+    valid_out[p] = r.keep ? 1 : 0;
+    if (r.keep) pm_store(m, ent_out + 64 * p);
+    else std::memset(ent_out + 64 * p, 0, 64 * sizeof(float));
+    out_cnt[p] = (uint8_t)r.n_out;
+    std::memset(out_pts + 48 * p, 0, 48 * sizeof(float));
+    for (int k = 0; k < r.n_out; ++k) pm_emit(r.out[k], r.anchor_a[k], out_pts + 48 * p + 12 * k);
+  }
 }
 
 }  // extern "C"

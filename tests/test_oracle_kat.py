@@ -387,3 +387,142 @@ def test_raycast_candidates_filters_by_ray_path(oracle):  # axis_sweep3.rs:734-7
     inside = oracle.raycast_candidates(mn, mx, [1.5, 1.5, 1.5], [4, 4, 4]).tolist()
     assert 0 in inside and 1 not in inside
     assert oracle.raycast_candidates(mn, mx, [3, 3, 3], [4, 4, 4]).tolist() == []
+
+
+# ------------------------------------------------------ persistent manifold --
+# The reference's own PersistentManifold tests (crates/bullet-collision/src/
+# manifold.rs:388-868) against the oracle's restatement.
+_ID = [0.0, 0.0, 0.0, 1.0]
+_Y = [0.0, 1.0, 0.0]
+_BT = np.float32(0.02)  # CONTACT_BREAKING_THRESHOLD
+
+
+def _resting(pen):
+    """manifold.rs:397-407: box A at (0, 0.5 - pen, 0) on plane B at the origin."""
+    pa = np.array([0.0, 0.5 - pen, 0.0], np.float32)
+    pb = np.zeros(3, np.float32)
+    return np.array([[0, 0, 0, 0, 1, 0, pen]], np.float32), pa, pb
+
+
+def test_pm_refresh_reprojects_and_keeps_band_points(oracle):
+    """manifold.rs:409-441"""
+    m, pa, pb = _resting(0.01)
+    e = oracle.pm_new()
+    oracle.pm_capture(e, m, pa, _ID, pb, _ID, _BT)
+    out = oracle.pm_refresh(e, pa - np.float32(0.015) * np.array(_Y, np.float32), _ID, pb, _ID)
+    assert out.shape[0] == 1
+    assert abs(out[0, 6] - (-0.005)) < 1e-6
+    assert np.array_equal(out[0, 0:3], [0, 0, 0]) and np.array_equal(out[0, 3:6], _Y)
+    assert abs(oracle.pm_points(e)[0, 3] - (-0.005)) < 1e-6
+
+
+def test_pm_refresh_drops_separated_and_drifted_points(oracle):
+    """manifold.rs:443-484"""
+    m, pa, pb = _resting(0.01)
+    e = oracle.pm_new()
+    oracle.pm_capture(e, m, pa, _ID, pb, _ID, _BT)
+    gone = oracle.pm_refresh(e, pa - np.float32(0.05) * np.array(_Y, np.float32), _ID, pb, _ID)
+    assert gone.shape[0] == 0 and oracle.pm_count(e) == 0
+    oracle.pm_capture(e, m, pa, _ID, pb, _ID, _BT)
+    drifted = oracle.pm_refresh(e, pa + np.array([0.03, 0, 0], np.float32), _ID, pb, _ID)
+    assert drifted.shape[0] == 0 and oracle.pm_count(e) == 0
+
+
+def test_pm_capture_anchors_round_trip(oracle):
+    """manifold.rs:486-508"""
+    m, pa, pb = _resting(0.01)
+    e = oracle.pm_new()
+    oracle.pm_capture(e, m, pa, _ID, pb, _ID, _BT)
+    p = oracle.pm_points(e)[0]
+    wa = pa + p[0:3]
+    wb = pb + p[4:7]
+    assert np.array_equal(wb, [0, 0, 0])
+    assert abs(wa[0]) < 1e-6 and abs(wa[1] - 0.01) < 1e-6 and abs(wa[2]) < 1e-6
+
+
+def test_pm_capture_merges_and_inherits_the_impulse(oracle):
+    """manifold.rs:510-558"""
+    m, pa, pb = _resting(0.01)
+    e = oracle.pm_new()
+    oracle.pm_capture(e, m, pa, _ID, pb, _ID, _BT)
+    e[7] = 5.0  # point_mut(0).normal_impulse
+    oracle.pm_capture(e, m, pa, _ID, pb, _ID, _BT)
+    assert oracle.pm_points(e)[0, 7] == 5.0
+    flipped = np.array([[0, 0, 0, 0, -1, 0, 0.01]], np.float32)
+    oracle.pm_capture(e, flipped, pa, _ID, pb, _ID, _BT)
+    pts = oracle.pm_points(e)
+    assert pts.shape[0] == 2 and pts[0, 7] == 5.0 and pts[1, 7] == 0.0
+
+
+def test_pm_refresh_increments_lifetime_and_capture_resets_it(oracle):
+    """manifold.rs:560-622"""
+    m, pa, pb = _resting(0.01)
+    e = oracle.pm_new()
+    oracle.pm_capture(e, m, pa, _ID, pb, _ID, _BT)
+    assert oracle.pm_points(e)[0].view(np.uint32)[11] == 0
+    for step in range(1, 6):
+        out = oracle.pm_refresh(e, pa, _ID, pb, _ID)
+        assert out.shape[0] == 1
+        assert oracle.pm_points(e)[0].view(np.uint32)[11] == step
+        assert out[0].view(np.uint32)[8] == step
+    oracle.pm_capture(e, m, pa, _ID, pb, _ID, _BT)
+    assert oracle.pm_points(e)[0].view(np.uint32)[11] == 0
+
+
+def test_pm_redetect_triggers_on_relative_drift_only(oracle):
+    """manifold.rs:624-662 (from_rotation_z(0.02) = (0, 0, sin 0.01, cos 0.01))"""
+    m, pa, pb = _resting(0.01)
+    e = oracle.pm_new()
+    assert oracle.pm_needs_redetect(e, pa, _ID, pb, _ID)
+    oracle.pm_capture(e, m, pa, _ID, pb, _ID, _BT)
+    y = np.array(_Y, np.float32)
+    assert not oracle.pm_needs_redetect(e, pa + np.float32(0.004) * y, _ID, pb, _ID)
+    assert oracle.pm_needs_redetect(e, pa + np.float32(0.006) * y, _ID, pb, _ID)
+    q = [0.0, 0.0, float(np.sin(np.float32(0.01))), float(np.cos(np.float32(0.01)))]
+    assert oracle.pm_needs_redetect(e, pa, q, pb, _ID)
+    x = np.array([1, 0, 0], np.float32)
+    assert not oracle.pm_needs_redetect(e, pa + x, _ID, pb + x, _ID)
+
+
+def test_pm_capture_overflow_keeps_deepest_four(oracle):
+    """manifold.rs:778-826: depth truncation keeps the deep point and the
+    first three shallow points in input order (the clustered ones)."""
+    pt = lambda x, y, z, pen: [x, y, z, 0, 1, 0, pen]
+    batch = np.array([pt(0, 0, 0, 0.5), pt(0.03, 0, 0, 0.1), pt(0, 0, 0.03, 0.1), pt(0, 0.03, 0, 0.1),
+                      pt(1, 0, 0, 0.1), pt(0, 1, 0, 0.1)], np.float32)
+    e = oracle.pm_new()
+    o = np.zeros(3, np.float32)
+    oracle.pm_capture(e, batch, o, _ID, o, _ID, _BT)
+    pts = oracle.pm_points(e)
+    assert pts.shape[0] == 4
+    anchors = pts[:, 0:3]
+    assert abs(anchors[0, 1] - 0.5) < 1e-6
+    shallow = anchors[1:4]
+    spread = max(np.linalg.norm(a - b) for a in shallow for b in shallow)
+    assert spread < 0.1
+    assert np.all(anchors.max(axis=1) < 0.6)
+
+
+def test_pm_capture_overflow_on_spread_inputs(oracle):
+    """manifold.rs:828-868: a shallow near-duplicate outside the matching
+    radius is dropped by depth truncation."""
+    base = [([0, 0, 0], 0.4), ([1, 0, 0], 0.3), ([0, 0, 1], 0.2), ([1, 0, 1], 0.1)]
+    e = oracle.pm_new()
+    o = np.zeros(3, np.float32)
+    for p, pen in base:
+        oracle.pm_capture(e, np.array([p + _Y + [pen]], np.float32), o, _ID, o, _ID, _BT)
+    dup = np.array([[0.015, 0, 0.015] + _Y + [0.05]], np.float32)
+    oracle.pm_capture(e, dup, o, _ID, o, _ID, _BT)
+    anchors = oracle.pm_points(e)[:, 0:3]
+    assert anchors.shape[0] == 4
+    assert np.any(anchors[:, 1] > 0.3)
+    assert not np.any((np.abs(anchors[:, 0] - 0.015) < 1e-6) & (np.abs(anchors[:, 2] - 0.015) < 1e-6))
+
+
+def test_pm_seed_is_the_first_normal(oracle):
+    """manifold.rs:378-385"""
+    m, pa, pb = _resting(0.01)
+    e = oracle.pm_new()
+    assert oracle.pm_seed(e) is None
+    oracle.pm_capture(e, m, pa, _ID, pb, _ID, _BT)
+    assert np.array_equal(oracle.pm_seed(e), _Y)
