@@ -272,6 +272,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
+    ap.add_argument("--only-pass", action="store_true",
+                    help="skip the host-buffer, captured and manifold legs (for kernel traces of the pass alone)")
     ap.add_argument("--worlds", type=int, default=4096)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy three times (HBM counter calibration)")
@@ -320,18 +322,18 @@ def main():
     pos = np.ascontiguousarray(sc["position"], np.float32)
     quat = np.ascontiguousarray(sc["orientation"], np.float32)
     th = []
-    for _ in range(3):
+    for _ in range(0 if args.only_pass else 3):
         t1 = time.perf_counter()
         w.set_poses(pos, quat)
         w.collide(args.margin)
         w.results()
         th.append(time.perf_counter() - t1)
-    host_ms = float(np.median(th)) * 1e3
-    pcie = {"ms_per_step": round(host_ms, 3), "pairs_per_s": round(P / (host_ms * 1e-3), 1),
+    host_ms = float(np.median(th)) * 1e3 if th else float("nan")
+    pcie = None if not th else {"ms_per_step": round(host_ms, 3), "pairs_per_s": round(P / (host_ms * 1e-3), 1),
             "includes": "pose upload (28 B/body) + pass + pairs, offsets and contacts copied to host"}
     log(f"host-buffer pass: {host_ms:.2f} ms")
     w.set_poses(pos, quat)
-    graph = captured(w, args.margin, args.steps, (pairs, offs, cont))
+    graph = None if args.only_pass else captured(w, args.margin, args.steps, (pairs, offs, cont))
     log(f"captured pass: {graph}")
     routes = w.routes()
     dom = STAGES[int(np.argmax(stage_ms))]
@@ -351,7 +353,7 @@ def main():
             "routes": {"epa_pairs": int(((routes & 8) != 0).sum()), "hull_pairs": int(((routes & 7) == 6).sum())}}
 
     # last: it moves the bodies
-    manifold = manifold_frames(w, sc, args.margin, 5)
+    manifold = None if args.only_pass else manifold_frames(w, sc, args.margin, 5)
 
     batched = None
     if not args.no_batched:

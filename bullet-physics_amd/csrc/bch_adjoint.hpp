@@ -438,17 +438,29 @@ __global__ __launch_bounds__(64) void k_sd_epa(const uint32_t* __restrict__ n_qu
            d = epa_q[4 * (uint64_t)q + 3];
     uint32_t t = f2u(a.w);
     uint32_t i = pairs[2 * t], j = pairs[2 * t + 1];
-    EpaPair ep;
-    Pair& pr = ep.pr;
-    pr = make_pair(bodies[i], bodies[j], hull);
-    ep.ha.load(pr.a, lane);
-    ep.hb.load(pr.b, lane);
-    EpaResult e = wave_epa(ep, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, V3{d.x, d.y, d.z}, S, lane);
+    Pair pr = make_pair(bodies[i], bodies[j], hull);
+    V3 t0{a.x, a.y, a.z}, t1{b.x, b.y, b.z}, t2{c.x, c.y, c.z}, t3{d.x, d.y, d.z};
+    EpaResult e;
+    V3 wa;
+    if (SplitPair::fits(pr)) {
+      SplitPair sp;
+      sp.load(pr, lane);
+      e = wave_epa(sp, t0, t1, t2, t3, S, lane);
+      wa = sp.support_a(e.n, lane);
+    } else {
+      EpaPair ep;
+      ep.pr = pr;
+      ep.ha.load(pr.a, lane);
+      ep.hb.load(pr.b, lane);
+      SeqPair sq{&ep};
+      e = wave_epa(sq, t0, t1, t2, t3, S, lane);
+      wa = sq.support_a(e.n, lane);
+    }
     float depth = (e.depth + pr.a.margin) + pr.b.margin;
     SdOut o;
     o.sd = -depth;
     o.n = e.n;
-    o.pa = wave_support_world(pr.a, ep.ha, pr.pa, pr.qa, e.n, lane, false);
+    o.pa = wa;
     o.pb = o.pa - e.n * depth;
     uint32_t st = e.status == 0 ? SD_EPA : SD_EPA_UNCONVERGED;
     if (!BWD) {

@@ -164,6 +164,111 @@ __device__ __forceinline__ V3 wave_minkowski(const EpaPair& ep, V3 d, int lane) 
   return a - b;
 }
 
+// Both core supports of one Minkowski point in the two halves of the
+// wavefront at once: lanes 0-31 evaluate shape A at d, lanes 32-63 shape B
+// at -d, each with the same arithmetic as the one-side form above; the two
+// world points are then read from lanes 0 and 32.  Hulls of at most 64
+// vertices: lane l holds vertices l % 32 and l % 32 + 32 of its side.
+struct SplitPair {
+  int type;
+  float a0, a1, a2;
+  V3 p;
+  Q4 q;
+  float x0, y0, z0, x1, y1, z1;
+  int n;
+  // pairs whose hulls fit: at most 64 vertices each
+  __device__ __forceinline__ static bool fits(const Pair& pr) {
+    return (pr.a.type != HULL || pr.a.nhull <= 64) && (pr.b.type != HULL || pr.b.nhull <= 64);
+  }
+  __device__ __forceinline__ void load(const Pair& pr, int lane) {
+    bool hi = lane >= 32;
+    const ShapeRef& sa = pr.a;
+    const ShapeRef& sb = pr.b;
+    type = hi ? sb.type : sa.type;
+    a0 = hi ? sb.a0 : sa.a0;
+    a1 = hi ? sb.a1 : sa.a1;
+    a2 = hi ? sb.a2 : sa.a2;
+    p = sel(hi, pr.pb, pr.pa);
+    q = Q4{hi ? pr.qb.x : pr.qa.x, hi ? pr.qb.y : pr.qa.y, hi ? pr.qb.z : pr.qa.z, hi ? pr.qb.w : pr.qa.w};
+    n = type == HULL ? (hi ? sb.nhull : sa.nhull) : 0;
+    const float4* h = hi ? sb.hull : sa.hull;
+    int hl = lane & 31;
+    float4 u = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v = u;
+    if (hl < n) u = h[hl];
+    if (hl + 32 < n) v = h[hl + 32];
+    x0 = u.x; y0 = u.y; z0 = u.z;
+    x1 = v.x; y1 = v.y; z1 = v.z;
+  }
+  // convex_hull.rs:55-66 per half: first maximiser of p.d.
+  __device__ __forceinline__ V3 hull_support(V3 d, int lane) const {
+    const bool hi = lane >= 32;
+    const int hl = lane & 31;
+    float a = dot(V3{x0, y0, z0}, d), b = dot(V3{x1, y1, z1}, d);
+    a = (hl < n && a > -INFINITY) ? a : -INFINITY;
+    b = (hl + 32 < n && b > -INFINITY) ? b : -INFINITY;
+    float v = b > a ? b : a;
+    float o;
+    o = dpp_f<0xB1>(v);
+    v = o > v ? o : v;
+    o = dpp_f<0x4E>(v);
+    v = o > v ? o : v;
+    o = dpp_f<0x141>(v);
+    v = o > v ? o : v;
+    o = dpp_f<0x140>(v);
+    v = o > v ? o : v;
+    float r0 = rdl_f(v, 0), r1 = rdl_f(v, 16), r2 = rdl_f(v, 32), r3 = rdl_f(v, 48);
+    float mA = r1 > r0 ? r1 : r0, mB = r3 > r2 ? r3 : r2;
+    float m = hi ? mB : mA;
+    uint64_t ba = __ballot(a == m), bb = __ballot(b == m);
+    uint32_t aA = (uint32_t)ba, aB = (uint32_t)(ba >> 32), bA = (uint32_t)bb, bB = (uint32_t)(bb >> 32);
+    int lA = aA ? ctz64(aA) : (bA ? ctz64(bA) : 0);
+    int lB = 32 + (aB ? ctz64(aB) : (bB ? ctz64(bB) : 0));
+    V3 vA = aA ? V3{rdl_f(x0, lA), rdl_f(y0, lA), rdl_f(z0, lA)} : V3{rdl_f(x1, lA), rdl_f(y1, lA), rdl_f(z1, lA)};
+    V3 vB = aB ? V3{rdl_f(x0, lB), rdl_f(y0, lB), rdl_f(z0, lB)} : V3{rdl_f(x1, lB), rdl_f(y1, lB), rdl_f(z1, lB)};
+    V3 r = sel(hi, vB, vA);
+    return m > -INFINITY ? r : splat(0.0f);
+  }
+  // support_world / core_support_world of this lane's side at its direction
+  __device__ __forceinline__ V3 world(V3 d, int lane, bool core) const {
+    ShapeRef s;
+    s.type = type;
+    s.a0 = a0;
+    s.a1 = a1;
+    s.a2 = a2;
+    s.a3 = 0.0f;
+    s.hull = nullptr;
+    s.nhull = n;
+    s.margin = 0.0f;
+    V3 ld = qrot(qconj(q), d);
+    V3 l;
+    if (type == HULL) l = hull_support(ld, lane);
+    else l = core ? core_support_local(s, ld) : support_local(s, ld);
+    return qrot(q, l) + p;
+  }
+  // the Minkowski point supportA(d) - supportB(-d) (cores), uniform
+  __device__ __forceinline__ V3 operator()(V3 d, int lane) const {
+    const bool hi = lane >= 32;
+    V3 w = world(sel(hi, -d, d), lane, true);
+    V3 a{rdl_f(w.x, 0), rdl_f(w.y, 0), rdl_f(w.z, 0)};
+    V3 b{rdl_f(w.x, 32), rdl_f(w.y, 32), rdl_f(w.z, 32)};
+    return a - b;
+  }
+  // supportA(d) (full shape), uniform
+  __device__ __forceinline__ V3 support_a(V3 d, int lane) const {
+    V3 w = world(d, lane, false);
+    return V3{rdl_f(w.x, 0), rdl_f(w.y, 0), rdl_f(w.z, 0)};
+  }
+};
+
+// The one-side form as the same functor interface (hulls of any size).
+struct SeqPair {
+  const EpaPair* ep;
+  __device__ __forceinline__ V3 operator()(V3 d, int lane) const { return wave_minkowski(*ep, d, lane); }
+  __device__ __forceinline__ V3 support_a(V3 d, int lane) const {
+    return wave_support_world(ep->pr.a, ep->ha, ep->pr.pa, ep->pr.qa, d, lane, false);
+  }
+};
+
 __device__ __forceinline__ V3 lv(const EpaLds& S, int i) {
   float4 v = S.v[i];
   return V3{v.x, v.y, v.z};
@@ -443,8 +548,10 @@ __device__ __forceinline__ int wave_expand(EpaLds& S, int& nfaces, V3 np, int ni
   return 1;
 }
 
-// epa.rs:127-205 for one pair; every lane returns the result.
-__device__ __forceinline__ EpaResult wave_epa(const EpaPair& ep, V3 t0, V3 t1, V3 t2, V3 t3, EpaLds& S, int lane) {
+// epa.rs:127-205 for one pair; every lane returns the result.  `mink(d,
+// lane)` is the wavefront-uniform Minkowski support.
+template <class Mink>
+__device__ __forceinline__ EpaResult wave_epa(const Mink& mink, V3 t0, V3 t1, V3 t2, V3 t3, EpaLds& S, int lane) {
   EpaResult R;
   if (lane < 4) {
     V3 w = sel(lane == 0, t0, sel(lane == 1, t1, sel(lane == 2, t2, t3)));
@@ -511,7 +618,7 @@ __device__ __forceinline__ EpaResult wave_epa(const EpaPair& ep, V3 t0, V3 t1, V
   for (int k = 0; k < 6; ++k) {  // seeding; a degenerate expansion only ends the seeding
     V3 d = k == 0 ? V3{1.0f, 0.0f, 0.0f} : k == 1 ? V3{-1.0f, 0.0f, 0.0f} : k == 2 ? V3{0.0f, 1.0f, 0.0f}
          : k == 3 ? V3{0.0f, -1.0f, 0.0f} : k == 4 ? V3{0.0f, 0.0f, 1.0f} : V3{0.0f, 0.0f, -1.0f};
-    V3 p = wave_minkowski(ep, d, lane);
+    V3 p = mink(d, lane);
     int idx = nverts;
     if (lane == 0) S.v[idx] = make_float4(p.x, p.y, p.z, 0.0f);
     ++nverts;
@@ -528,7 +635,7 @@ __device__ __forceinline__ EpaResult wave_epa(const EpaPair& ep, V3 t0, V3 t1, V
       int bi = wave_best(S, nfaces, lane);
       V3 bn{S.nx[bi], S.ny[bi], S.nz[bi]};
       float bd = S.dist[bi];
-      V3 np = wave_minkowski(ep, bn, lane);
+      V3 np = mink(bn, lane);
       float wdist = dot(bn, np) - bd;
       if (wdist <= 1e-4f) {
         R.status = 0;
@@ -576,25 +683,34 @@ __global__ __launch_bounds__(64) void k_epa_wave(const uint32_t* __restrict__ n_
     uint32_t p = f2u(a.w);
     uint32_t i = pairs[2 * p], j = pairs[2 * p + 1];
     BodyRec A = bodies[i], B = bodies[j];
-    EpaPair ep;
-    Pair& pr = ep.pr;
+    Pair pr;
     pr.a = shape_of(A, hull);
     pr.b = shape_of(B, hull);
     pr.pa = pos_of(A);
     pr.pb = pos_of(B);
     pr.qa = quat_of(A);
     pr.qb = quat_of(B);
-    ep.ha.load(pr.a, lane);
-    ep.hb.load(pr.b, lane);
-    EpaResult e = wave_epa(ep, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, V3{d.x, d.y, d.z}, S, lane);
-    // convex_convex.rs:117-154: margins added back, witness = supportA(n) - n depth.
-    bool hit = !(e.depth <= 0.0f);
+    V3 t0{a.x, a.y, a.z}, t1{b.x, b.y, b.z}, t2{c.x, c.y, c.z}, t3{d.x, d.y, d.z};
+    EpaResult e;
     V3 point_a = splat(0.0f);
-    float depth = 0.0f;
-    if (hit) {
-      depth = (e.depth + pr.a.margin) + pr.b.margin;
-      point_a = wave_support_world(pr.a, ep.ha, pr.pa, pr.qa, e.n, lane, false);
+    // convex_convex.rs:117-154: margins added back, witness = supportA(n) - n depth.
+    if (SplitPair::fits(pr)) {
+      SplitPair sp;
+      sp.load(pr, lane);
+      e = wave_epa(sp, t0, t1, t2, t3, S, lane);
+      if (!(e.depth <= 0.0f)) point_a = sp.support_a(e.n, lane);
+    } else {
+      EpaPair ep;
+      ep.pr = pr;
+      ep.ha.load(pr.a, lane);
+      ep.hb.load(pr.b, lane);
+      SeqPair sq{&ep};
+      e = wave_epa(sq, t0, t1, t2, t3, S, lane);
+      if (!(e.depth <= 0.0f)) point_a = sq.support_a(e.n, lane);
     }
+    bool hit = !(e.depth <= 0.0f);
+    float depth = 0.0f;
+    if (hit) depth = (e.depth + pr.a.margin) + pr.b.margin;
     if (lane == 0) {
       if (e.status == 4) atomicOr(&cnt->errors, 2u);
       ccount[p] = hit ? 1u : 0u;
