@@ -295,10 +295,17 @@ class CollisionWorld:
         self.n_mpairs = 0
         self.n_mpoints = 0
 
-    def set_poses(self, position, orientation) -> None:
+    def set_poses(self, position, orientation, linear_velocity=None, dt: float = 0.0) -> None:
+        """New poses; with velocities, active bodies go to their predicted
+        position + linear_velocity * dt (step.rs:340-343)."""
         p = np.ascontiguousarray(position, np.float32)
         q = np.ascontiguousarray(orientation, np.float32)
-        _check(self.ctx.h, self.L.bch_world_set_poses(self.ctx.h, p.ctypes.data, q.ctypes.data))
+        if linear_velocity is None:
+            _check(self.ctx.h, self.L.bch_world_set_poses(self.ctx.h, p.ctypes.data, q.ctypes.data))
+            return
+        v = np.ascontiguousarray(linear_velocity, np.float32)
+        _check(self.ctx.h, self.L.bch_world_set_poses_predicted(self.ctx.h, p.ctypes.data, q.ctypes.data,
+                                                                v.ctypes.data, dt))
 
     def set_poses_device(self, pos_ptr: int, quat_ptr: int) -> None:
         _check(self.ctx.h, self.L.bch_world_set_poses_device(self.ctx.h, pos_ptr, quat_ptr))

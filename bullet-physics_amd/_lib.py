@@ -26,6 +26,7 @@ EXPORTS = [
     "bch_depen_capture", "bch_depen_replay", "bch_depen_history", "bch_depen_release",
     "bch_world_set_body_flags", "bch_world_manifold_pass", "bch_world_manifold_results",
     "bch_world_manifold_impulses", "bch_world_manifold_cache", "bch_world_manifold_reset",
+    "bch_world_set_poses_predicted", "bch_world_set_poses_predicted_device",
 ]
 
 # launch-shape knobs (bch_set_tuning)
@@ -117,6 +118,8 @@ def load():
     L.bch_world_manifold_impulses.argtypes = [vp, vp]
     L.bch_world_manifold_cache.argtypes = [vp, vp, vp]
     L.bch_world_manifold_reset.argtypes = [vp]
+    L.bch_world_set_poses_predicted.argtypes = [vp, vp, vp, vp, f32]
+    L.bch_world_set_poses_predicted_device.argtypes = [vp, vp, vp, vp, f32]
     L.bch_depen_release.argtypes = [vp]
     L.bch_depen_history.argtypes = [vp, vp, vp, vp, vp, u32, C.POINTER(u32), C.c_int]
     _lib = L

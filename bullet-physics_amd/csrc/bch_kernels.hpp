@@ -111,13 +111,23 @@ __global__ void k_pack_bodies(uint32_t n, const float* __restrict__ pos, const f
   out[i] = b;
 }
 
+// Poses into the body records.  With velocities, an active body (flag bit 1,
+// every body without flags) is placed at its predicted position
+// position + linear_velocity * dt (body.rs:1175-1181, step.rs:340-343).
 __global__ void k_pack_poses(uint32_t n, const float* __restrict__ pos, const float* __restrict__ quat,
-                             BodyRec* __restrict__ out) {
+                             BodyRec* __restrict__ out, const float* __restrict__ vel = nullptr, float dt = 0.0f,
+                             const uint8_t* __restrict__ bflags = nullptr) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[i].px = pos[3 * i];
-  out[i].py = pos[3 * i + 1];
-  out[i].pz = pos[3 * i + 2];
+  float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+  if (vel && (!bflags || (bflags[i] & 2u))) {
+    px = px + vel[3 * i] * dt;
+    py = py + vel[3 * i + 1] * dt;
+    pz = pz + vel[3 * i + 2] * dt;
+  }
+  out[i].px = px;
+  out[i].py = py;
+  out[i].pz = pz;
   out[i].qx = quat[4 * i];
   out[i].qy = quat[4 * i + 1];
   out[i].qz = quat[4 * i + 2];
@@ -228,24 +238,38 @@ __global__ void k_bounds_final(const float* __restrict__ pa, uint32_t na, const 
   }
 }
 
+// A body whose AABB stays as the previous pass left it: inactive bodies
+// (flag bit 1 clear) when `keep` is set (step.rs:348-353: sleeping bodies
+// keep their broadphase AABB).
+__device__ __forceinline__ bool keeps_aabb(const uint8_t* bflags, uint32_t i, int keep) {
+  return keep && bflags && !(bflags[i] & 2u);
+}
+
 // World AABB + margin per body (world/step.rs:349-363), fused with
 // classification and the scene-bounds reduction.
 __global__ void k_aabb(uint32_t n, const BodyRec* __restrict__ bodies, const float4* __restrict__ hull, float margin,
                        float4* __restrict__ aabb, uint8_t* __restrict__ flags, uint32_t* __restrict__ big_ids,
-                       Counters* cnt, float* __restrict__ partial) {
+                       Counters* cnt, float* __restrict__ partial, const uint8_t* __restrict__ bflags = nullptr,
+                       int keep = 0) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
   uint32_t reg = 0;
   BodyRec b;
   if (i < n) b = bodies[i];
   if (i < n && (b.type & 0xffu) != HULL) {  // hull boxes: k_hull_aabb
-    ShapeRef s = shape_of(b, hull);
-    Box6 bx = world_aabb(s, pos_of(b), quat_of(b));
-    V3 pad = splat(margin);  // util.rs:52-55
-    bx.lo = bx.lo - pad;
-    bx.hi = bx.hi + pad;
-    aabb[2 * i] = make_float4(bx.lo.x, bx.lo.y, bx.lo.z, 0.0f);
-    aabb[2 * i + 1] = make_float4(bx.hi.x, bx.hi.y, bx.hi.z, 0.0f);
+    Box6 bx;
+    if (keeps_aabb(bflags, i, keep)) {
+      float4 lo = aabb[2 * i], hi = aabb[2 * i + 1];
+      bx = Box6{V3{lo.x, lo.y, lo.z}, V3{hi.x, hi.y, hi.z}};
+    } else {
+      ShapeRef s = shape_of(b, hull);
+      bx = world_aabb(s, pos_of(b), quat_of(b));
+      V3 pad = splat(margin);  // util.rs:52-55
+      bx.lo = bx.lo - pad;
+      bx.hi = bx.hi + pad;
+      aabb[2 * i] = make_float4(bx.lo.x, bx.lo.y, bx.lo.z, 0.0f);
+      aabb[2 * i + 1] = make_float4(bx.hi.x, bx.hi.y, bx.hi.z, 0.0f);
+    }
     classify_box(i, bx, flags, cnt, big_ids, cmin, cmax);
     reg = flags[i] == FLAG_REG;
   }
@@ -282,7 +306,8 @@ __device__ __forceinline__ void argmax_merge(float& v, int& i, float ov, int oi)
 __global__ void k_hull_aabb(uint32_t n_hulls, const uint32_t* __restrict__ hull_ids,
                             const BodyRec* __restrict__ bodies, const float4* __restrict__ hull, float margin,
                             float4* __restrict__ aabb, uint8_t* __restrict__ flags, uint32_t* __restrict__ big_ids,
-                            Counters* cnt, float* __restrict__ partial) {
+                            Counters* cnt, float* __restrict__ partial, const uint8_t* __restrict__ bflags = nullptr,
+                            int keep = 0) {
   uint32_t h = blockIdx.x * (blockDim.x / 16) + threadIdx.x / 16;
   int lane = threadIdx.x & 15;
   float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -320,18 +345,23 @@ __global__ void k_hull_aabb(uint32_t n_hulls, const uint32_t* __restrict__ hull_
     }
   }
   if (active && lane == 0) {
-    V3 p = pos_of(b);
-    V3 sp[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) sp[k] = bi[k] >= 0 ? hull_vertex(hv, bi[k]) : splat(0.0f);
     Box6 bx;
-    bx.lo = V3{p.x + dot(sp[3], u[0]), p.y + dot(sp[4], u[1]), p.z + dot(sp[5], u[2])};
-    bx.hi = V3{p.x + dot(sp[0], u[0]), p.y + dot(sp[1], u[1]), p.z + dot(sp[2], u[2])};
-    V3 pad = splat(margin);
-    bx.lo = bx.lo - pad;
-    bx.hi = bx.hi + pad;
-    aabb[2 * body] = make_float4(bx.lo.x, bx.lo.y, bx.lo.z, 0.0f);
-    aabb[2 * body + 1] = make_float4(bx.hi.x, bx.hi.y, bx.hi.z, 0.0f);
+    if (keeps_aabb(bflags, body, keep)) {
+      float4 lo = aabb[2 * body], hi = aabb[2 * body + 1];
+      bx = Box6{V3{lo.x, lo.y, lo.z}, V3{hi.x, hi.y, hi.z}};
+    } else {
+      V3 p = pos_of(b);
+      V3 sp[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) sp[k] = bi[k] >= 0 ? hull_vertex(hv, bi[k]) : splat(0.0f);
+      bx.lo = V3{p.x + dot(sp[3], u[0]), p.y + dot(sp[4], u[1]), p.z + dot(sp[5], u[2])};
+      bx.hi = V3{p.x + dot(sp[0], u[0]), p.y + dot(sp[1], u[1]), p.z + dot(sp[2], u[2])};
+      V3 pad = splat(margin);
+      bx.lo = bx.lo - pad;
+      bx.hi = bx.hi + pad;
+      aabb[2 * body] = make_float4(bx.lo.x, bx.lo.y, bx.lo.z, 0.0f);
+      aabb[2 * body + 1] = make_float4(bx.hi.x, bx.hi.y, bx.hi.z, 0.0f);
+    }
     classify_box(body, bx, flags, cnt, big_ids, cmin, cmax);
     reg = flags[body] == FLAG_REG;
   }

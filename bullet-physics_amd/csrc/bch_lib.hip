@@ -105,6 +105,8 @@ struct bch_context {
   bool has_bflags = false, has_bgm = false;
   DevBuf mroute, msrc, mcnt, mslots, moff, mpts, mseeds, mimp;
   uint64_t mNP = 0;  // points emitted by the last pass
+  bool aabb_valid = false;  // c->aabb holds the previous pass's boxes (kept for inactive bodies)
+  DevBuf vel;
 
   // proxy API state (host mirror of the broadphase slots)
   std::vector<float> proxy_box;  // 6 per slot
@@ -577,17 +579,21 @@ int validate_bodies(bch_context* c, const bch_bodies* b) {
 void run_aabbs(bch_context* c, float aabb_margin) {
   uint32_t n = c->n;
   hipStream_t s = c->stream;
+  // with body flags, inactive bodies keep the previous pass's box
+  const uint8_t* bf = c->has_bflags ? c->bflags.as<uint8_t>() : nullptr;
+  int keep = c->aabb_valid ? 1 : 0;
   hipLaunchKernelGGL(k_aabb, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                      aabb_margin, c->aabb.as<float4>(), c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(),
-                     c->counters.as<Counters>(), c->partials.as<float>());
+                     c->counters.as<Counters>(), c->partials.as<float>(), bf, keep);
   uint32_t hb = c->n_hulls ? blocks(c->n_hulls, kBlock / 16) : 0;
   float* hpart = c->partials.as<float>() + 8 * (size_t)blocks(n);
   if (c->n_hulls)
     hipLaunchKernelGGL(k_hull_aabb, dim3(hb), dim3(kBlock), 0, s, c->n_hulls, c->hull_ids.as<uint32_t>(),
                        c->bodies.as<BodyRec>(), c->hull.as<float4>(), aabb_margin, c->aabb.as<float4>(),
-                       c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(), c->counters.as<Counters>(), hpart);
+                       c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(), c->counters.as<Counters>(), hpart, bf, keep);
   hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, c->partials.as<float>(), blocks(n), hpart, hb,
                      c->counters.as<Counters>());
+  c->aabb_valid = true;
 }
 
 // One world pass (AABBs, broadphase, narrowphase) on c->stream.  Eager mode
@@ -792,7 +798,8 @@ void bch_destroy(bch_context* c) {
                    &c->scan_part, &c->rs_hist, &c->rs_offs, &c->sd_out, &c->sd_status, &c->pen_pairs,
                    &c->pen_g, &c->g14, &c->body_grad, &c->pen_part, &c->dhist, &c->dstep, &c->mkeys[0],
                    &c->mkeys[1], &c->mvalid[0], &c->mvalid[1], &c->ment[0], &c->ment[1], &c->bflags, &c->bgm,
-                   &c->mroute, &c->msrc, &c->mcnt, &c->mslots, &c->moff, &c->mpts, &c->mseeds, &c->mimp};
+                   &c->mroute, &c->msrc, &c->mcnt, &c->mslots, &c->moff, &c->mpts, &c->mseeds, &c->mimp,
+                   &c->vel};
   for (DevBuf* b : all) b->release();
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->own_stream);
@@ -1162,6 +1169,7 @@ int bch_world_set_bodies(bch_context* c, const bch_bodies* bodies, uint32_t n_wo
   release_depen_graph(c);
   c->mM = 0;  // cached manifolds name bodies by index
   c->has_bflags = c->has_bgm = false;
+  c->aabb_valid = false;
   rc = upload_bodies(c, bodies, world_offsets ? world_of.data() : nullptr);
   if (rc) return rc;
   c->n = bodies->n;
@@ -1185,6 +1193,38 @@ int bch_world_set_poses(bch_context* c, const float* position, const float* orie
   HIP_TRY(c, hipMemcpyAsync(c->staging_b.p, orientation, (size_t)c->n * 4 * sizeof(float), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_pack_poses, dim3(blocks(c->n)), dim3(kBlock), 0, s, c->n, c->staging_a.as<float>(),
                      c->staging_b.as<float>(), c->bodies.as<BodyRec>());
+  HIP_TRY(c, hipGetLastError());
+  return BCH_OK;
+}
+
+int bch_world_set_poses_predicted(bch_context* c, const float* position, const float* orientation,
+                                  const float* linear_velocity, float dt) {
+  if (!c || (c->n && (!position || !orientation || !linear_velocity)))
+    return fail(c, BCH_INVALID_ARG, "NULL argument");
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  if (!c->n) return BCH_OK;
+  ENSURE(c, c->vel, (size_t)c->n * 3 * sizeof(float));
+  HIP_TRY(c, hipMemcpyAsync(c->staging_a.p, position, (size_t)c->n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(c->staging_b.p, orientation, (size_t)c->n * 4 * sizeof(float), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(c->vel.p, linear_velocity, (size_t)c->n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_pack_poses, dim3(blocks(c->n)), dim3(kBlock), 0, s, c->n, c->staging_a.as<float>(),
+                     c->staging_b.as<float>(), c->bodies.as<BodyRec>(), (const float*)c->vel.as<float>(), dt,
+                     c->has_bflags ? (const uint8_t*)c->bflags.as<uint8_t>() : nullptr);
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipStreamSynchronize(s));
+  return BCH_OK;
+}
+
+int bch_world_set_poses_predicted_device(bch_context* c, const float* d_position, const float* d_orientation,
+                                         const float* d_linear_velocity, float dt) {
+  if (!c || (c->n && (!d_position || !d_orientation || !d_linear_velocity)))
+    return fail(c, BCH_INVALID_ARG, "NULL argument");
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (!c->n) return BCH_OK;
+  hipLaunchKernelGGL(k_pack_poses, dim3(blocks(c->n)), dim3(kBlock), 0, c->stream, c->n, d_position, d_orientation,
+                     c->bodies.as<BodyRec>(), d_linear_velocity, dt,
+                     c->has_bflags ? (const uint8_t*)c->bflags.as<uint8_t>() : nullptr);
   HIP_TRY(c, hipGetLastError());
   return BCH_OK;
 }

@@ -149,6 +149,15 @@ int bch_world_set_bodies(bch_context* ctx, const bch_bodies* bodies, uint32_t n_
 int bch_world_set_poses(bch_context* ctx, const float* position, const float* orientation);
 /* Per-step pose refresh from device memory on the same device. */
 int bch_world_set_poses_device(bch_context* ctx, const float* d_position, const float* d_orientation);
+/* Pose refresh at the step's predicted positions (step.rs:340-343,
+ * body.rs:1175-1181): position + linear_velocity [3n] * dt for bodies that
+ * are active (BCH_BODY_ACTIVE of bch_world_set_body_flags; every body when
+ * no flags are set), the position as given for the others; orientations as
+ * given.  Static bodies carry zero velocity.  Host / device arrays. */
+int bch_world_set_poses_predicted(bch_context* ctx, const float* position, const float* orientation,
+                                  const float* linear_velocity, float dt);
+int bch_world_set_poses_predicted_device(bch_context* ctx, const float* d_position, const float* d_orientation,
+                                         const float* d_linear_velocity, float dt);
 /* Run one pass on the context stream; returns after enqueueing plus one
  * small device->host read of the pair count.  *out_pairs / *out_contacts
  * receive the counts. */
@@ -262,7 +271,9 @@ int bch_world_release_graph(bch_context* ctx);
 #define BCH_ROUTE_REMOVED 5  /* collide_seeded found nothing: entry dropped */
 /* Per-body flags [n] and collision group | mask << 16 [n] (host arrays; NULL
  * = every body active and dynamic and slow / everything collides).  Kept
- * until the next call or bch_world_set_bodies. */
+ * until the next call or bch_world_set_bodies.  While flags are set, a body
+ * without BCH_BODY_ACTIVE keeps the broadphase AABB of the previous pass
+ * (step.rs:348-353: sleeping bodies keep theirs) in every pass. */
 int bch_world_set_body_flags(bch_context* ctx, const uint8_t* flags, const uint32_t* group_mask);
 /* One pair pass at the current poses (synchronous).  *out_pairs = broadphase
  * pairs, *out_points = emitted points. */

@@ -60,6 +60,8 @@ def lib():
         scp = C.POINTER(OrScene)
         L.oracle_scene_aabbs_f32.argtypes = [scp, C.c_float, C.c_void_p, C.c_void_p]
         L.oracle_scene_thresholds_f32.argtypes = [scp, C.c_void_p]
+        L.oracle_scene_aabbs_predicted_f32.argtypes = [scp, C.c_void_p, C.c_float] + [C.c_void_p] * 3 + [
+            C.c_float] + [C.c_void_p] * 3
         L.oracle_sd_pairs_f32.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
         L.oracle_sd_pairs_f64.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_sd_f64.argtypes = [C.c_void_p] * 7
@@ -267,6 +269,25 @@ def scene_aabbs(scene, margin):
     mx = np.empty((n, 3), np.float32)
     lib().oracle_scene_aabbs_f32(C.byref(r.c), margin, mn.ctypes.data, mx.ctypes.data)
     return mn, mx
+
+
+def scene_aabbs_predicted(scene, vel, dt, margin, flags=None, prev=None):
+    """The step's AABB update (step.rs:340-363) -> (predicted positions, min, max)."""
+    r = _SceneRef(scene)
+    n = r.c.n
+    v = np.ascontiguousarray(vel, np.float32)
+    fl = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+    pm = px = None
+    if prev is not None:
+        pm, px = (np.ascontiguousarray(prev[0], np.float32), np.ascontiguousarray(prev[1], np.float32))
+    pos = np.empty((n, 3), np.float32)
+    mn = np.empty((n, 3), np.float32)
+    mx = np.empty((n, 3), np.float32)
+    lib().oracle_scene_aabbs_predicted_f32(C.byref(r.c), v.ctypes.data, dt, None if fl is None else fl.ctypes.data,
+                                           None if pm is None else pm.ctypes.data,
+                                           None if px is None else px.ctypes.data, margin, pos.ctypes.data,
+                                           mn.ctypes.data, mx.ctypes.data)
+    return pos, mn, mx
 
 
 def scene_thresholds(scene):

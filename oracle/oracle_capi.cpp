@@ -331,6 +331,34 @@ void oracle_scene_aabbs_f32(const or_scene* sc, float margin, float* out_min, fl
   }
 }
 
+// The step's AABB update (step.rs:340-363): predicted positions
+// position + linear_velocity * dt for active bodies (flags bit 1; all bodies
+// when flags is NULL, body.rs:1175-1181), world AABBs at the predicted pose
+// expanded by the margin; inactive bodies keep prev_min / prev_max when
+// given.  out_pos receives the predicted positions.
+void oracle_scene_aabbs_predicted_f32(const or_scene* sc, const float* vel, float dt, const uint8_t* flags,
+                                      const float* prev_min, const float* prev_max, float margin, float* out_pos,
+                                      float* out_min, float* out_max) {
+  SceneShapes<float> ss;
+  ss.build(sc);
+  for (int64_t i = 0; i < sc->n; ++i) {
+    bool active = !flags || (flags[i] & 2u);
+    V3<float> p = ss.pos[i];
+    if (active) p = p + mk(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]) * dt;
+    out_pos[3 * i] = p.x; out_pos[3 * i + 1] = p.y; out_pos[3 * i + 2] = p.z;
+    if (!active && prev_min && prev_max) {
+      for (int k = 0; k < 3; ++k) {
+        out_min[3 * i + k] = prev_min[3 * i + k];
+        out_max[3 * i + k] = prev_max[3 * i + k];
+      }
+      continue;
+    }
+    Aabb<float> a = ss.shapes[i].world_aabb(p, ss.quat[i]).expanded(margin);
+    out_min[3 * i] = a.min.x; out_min[3 * i + 1] = a.min.y; out_min[3 * i + 2] = a.min.z;
+    out_max[3 * i] = a.max.x; out_max[3 * i + 1] = a.max.y; out_max[3 * i + 2] = a.max.z;
+  }
+}
+
 // Per-body contact_breaking_threshold (shapes/mod.rs:388-392).
 void oracle_scene_thresholds_f32(const or_scene* sc, float* out) {
   SceneShapes<float> ss;

@@ -61,6 +61,7 @@ def _run(bpa, oracle, ctx, sc, label, n_frames=5, seed=7, flags=None, gm=None):
     cache = None
     rng = np.random.default_rng(seed + 100)
     tot_routes = np.zeros(6, np.int64)
+    prev = None  # with flags, inactive bodies keep the previous frame's AABB (step.rs:348-353)
     for f, (pos, quat) in enumerate(_frames(sc, seed, n_frames)):
         _flush(f"case: {label} frame {f}")
         w.set_poses(pos, quat)
@@ -70,7 +71,12 @@ def _run(bpa, oracle, ctx, sc, label, n_frames=5, seed=7, flags=None, gm=None):
         valid, ent = w.manifold_cache()
         osc = dict(sc)
         osc["position"], osc["orientation"] = pos, quat
-        ref_pairs = oracle.sweep_pairs(*oracle.scene_aabbs(osc, MARGIN))
+        if flags is None:
+            mn, mx = oracle.scene_aabbs(osc, MARGIN)
+        else:
+            _, mn, mx = oracle.scene_aabbs_predicted(osc, np.zeros_like(pos), 0.0, MARGIN, flags=flags, prev=prev)
+            prev = (mn, mx)
+        ref_pairs = oracle.sweep_pairs(mn, mx)
         assert np.array_equal(pairs, ref_pairs), f"{label} frame {f}: pair sets differ"
         ref = oracle.pm_pass(osc, ref_pairs, cache, flags=flags, group_mask=gm, slop=SLOP, warm=WARM)
         bad = np.nonzero(route != ref["route"])[0]

@@ -348,3 +348,40 @@ def test_autotune_keeps_results(bpa, scenes, ctx):
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
     assert tuned["hull_lanes"] in (8, 16) and tuned["pass_ms"] > 0
+
+
+def test_predicted_poses_and_sleeping_aabbs(bpa, scenes, oracle, ctx):
+    """The step's AABB update (step.rs:340-363): active bodies at position +
+    velocity * dt, inactive bodies keep the previous pass's AABB even though
+    their pose moved; pairs bit-exact and contacts against the oracle at the
+    predicted poses."""
+    _flush("case: predicted poses, 20k mixed, 10% inactive")
+    sc = scenes.mixed_convex(20000, 2)
+    n = sc["shape_type"].shape[0]
+    rng = np.random.default_rng(21)
+    w = bpa.CollisionWorld(sc, bpa._Context(0))
+    w.collide(MARGIN)  # previous pass: AABBs at the scene poses
+    prev = oracle.scene_aabbs(sc, MARGIN)
+    flags = np.full(n, 2, np.uint8)
+    flags[rng.random(n) < 0.1] = 0
+    w.set_body_flags(flags)
+    pos2 = (sc["position"] + rng.normal(0, 0.02, (n, 3))).astype(np.float32)
+    vel = rng.normal(0, 1.0, (n, 3)).astype(np.float32)
+    dt = np.float32(1.0 / 60.0)
+    moved = dict(sc)
+    moved["position"] = pos2
+    ppos, mn, mx = oracle.scene_aabbs_predicted(moved, vel, dt, MARGIN, flags=flags, prev=prev)
+    assert np.array_equal(mn[flags == 0], prev[0][flags == 0])
+    w.set_poses(pos2, sc["orientation"], vel, float(dt))
+    P, Cn = w.collide(MARGIN)
+    pairs, offs, cont = w.results()
+    ref_pairs = oracle.sweep_pairs(mn, mx)
+    assert np.array_equal(pairs, ref_pairs), "pairs at predicted poses differ"
+    pred = dict(sc)
+    pred["position"] = ppos
+    cnt, pts, _ = oracle.collide_pairs(pred, ref_pairs)
+    compare_contacts(cnt, pts, offs, cont, label="predicted poses")
+    # the same pair set differs from one without retention (the test bites)
+    _, mn2, mx2 = oracle.scene_aabbs_predicted(moved, vel, dt, MARGIN, flags=flags, prev=None)
+    assert not np.array_equal(oracle.sweep_pairs(mn2, mx2), ref_pairs)
+    w.ctx.close()
