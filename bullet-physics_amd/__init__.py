@@ -274,6 +274,40 @@ def collide(shape_a, pa, qa, shape_b, pb, qb, ctx=None):
 
 
 # ------------------------------------------------------------------- world --
+class TriangleMesh:
+    """A static triangle mesh on a context (TriangleMesh::new,
+    shapes/triangle_mesh.rs:84-123); `id` names it in collide_mesh_batch."""
+
+    def __init__(self, vertices, indices, ctx: _Context | None = None):
+        self.ctx = ctx or _Context(0)
+        self.v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        self.i = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        out = C.c_uint32()
+        _check(self.ctx.h, self.ctx.L.bch_mesh_add(self.ctx.h, self.v.ctypes.data, self.v.shape[0],
+                                                   self.i.ctypes.data, self.i.shape[0], C.byref(out)))
+        self.id = out.value
+
+
+def collide_mesh_batch(scene: dict, pairs, mesh_poses, mesh_first=None, seeds=None, ctx: _Context | None = None):
+    """collide_seeded for (convex body, mesh) pairs (narrowphase/mod.rs:177-215):
+    pairs [P, 2] = (body of `scene`, TriangleMesh.id), mesh_poses [P, 7];
+    returns (count u8[P], points f32[P, 4, 7])."""
+    ctx = ctx or _Context(0)
+    b = Bodies(scene)
+    pr = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+    P = pr.shape[0]
+    mp = np.ascontiguousarray(mesh_poses, np.float32).reshape(-1, 7)
+    mf = None if mesh_first is None else np.ascontiguousarray(mesh_first, np.uint8)
+    sd = None if seeds is None else np.ascontiguousarray(seeds, np.float32)
+    cnt = np.zeros(max(P, 1), np.uint8)
+    pts = np.zeros((max(P, 1), 4, 7), np.float32)
+    _check(ctx.h, ctx.L.bch_collide_mesh_batch(ctx.h, C.byref(b.c), pr.ctypes.data, mp.ctypes.data,
+                                               None if mf is None else mf.ctypes.data, P,
+                                               None if sd is None else sd.ctypes.data, cnt.ctypes.data,
+                                               pts.ctypes.data))
+    return cnt[:P], pts[:P]
+
+
 class CollisionWorld:
     """Bodies resident in HBM; one detection pass per :meth:`collide`."""
 

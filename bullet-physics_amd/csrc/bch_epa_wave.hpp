@@ -667,20 +667,12 @@ __device__ __forceinline__ EpaResult wave_epa(const Mink& mink, V3 t0, V3 t1, V3
   return R;
 }
 
-// Grid-stride over the EPA queue: one wavefront per block and per pair,
-// one LDS polytope per wavefront.
-__global__ __launch_bounds__(64) void k_epa_wave(const uint32_t* __restrict__ n_queue,
-                                                 const float4* __restrict__ epa_q, const uint32_t* __restrict__ pairs,
-                                                 const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
-                                                 uint32_t* __restrict__ ccount, float4* __restrict__ cslots,
-                                                 uint32_t* __restrict__ route, Counters* cnt) {
-  __shared__ EpaLds S;
-  const int lane = threadIdx.x & 63;
-  const uint32_t nq = *n_queue;
-  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
-    float4 a = epa_q[4 * (uint64_t)q], b = epa_q[4 * (uint64_t)q + 1], c = epa_q[4 * (uint64_t)q + 2],
-           d = epa_q[4 * (uint64_t)q + 3];
-    uint32_t p = f2u(a.w);
+// The world pass's pairs as an EPA source: entry p is pair p of the list.
+struct WorldEpaSrc {
+  const uint32_t* pairs;
+  const BodyRec* bodies;
+  const float4* hull;
+  __device__ __forceinline__ Pair pair(uint32_t p) const {
     uint32_t i = pairs[2 * p], j = pairs[2 * p + 1];
     BodyRec A = bodies[i], B = bodies[j];
     Pair pr;
@@ -690,10 +682,29 @@ __global__ __launch_bounds__(64) void k_epa_wave(const uint32_t* __restrict__ n_
     pr.pb = pos_of(B);
     pr.qa = quat_of(A);
     pr.qb = quat_of(B);
+    return pr;
+  }
+};
+
+// Grid-stride over the EPA queue, one wavefront per entry and one LDS
+// polytope per wavefront; the contact of entry p goes to ccount[p] and
+// cslots[stride * p ..] (convex_convex.rs:117-154: margins added back,
+// witness = supportA(n) - n depth).  `route` (may be NULL) gets bit 3.
+template <class Src>
+__device__ __forceinline__ void epa_queue_run(const uint32_t* __restrict__ n_queue, const float4* __restrict__ epa_q,
+                                              const Src& src, uint32_t* __restrict__ ccount,
+                                              float4* __restrict__ cslots, uint32_t stride,
+                                              uint32_t* __restrict__ route, Counters* cnt, EpaLds& S) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t nq = *n_queue;
+  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+    float4 a = epa_q[4 * (uint64_t)q], b = epa_q[4 * (uint64_t)q + 1], c = epa_q[4 * (uint64_t)q + 2],
+           d = epa_q[4 * (uint64_t)q + 3];
+    uint32_t p = f2u(a.w);
+    Pair pr = src.pair(p);
     V3 t0{a.x, a.y, a.z}, t1{b.x, b.y, b.z}, t2{c.x, c.y, c.z}, t3{d.x, d.y, d.z};
     EpaResult e;
     V3 point_a = splat(0.0f);
-    // convex_convex.rs:117-154: margins added back, witness = supportA(n) - n depth.
     if (SplitPair::fits(pr)) {
       SplitPair sp;
       sp.load(pr, lane);
@@ -714,15 +725,24 @@ __global__ __launch_bounds__(64) void k_epa_wave(const uint32_t* __restrict__ n_
     if (lane == 0) {
       if (e.status == 4) atomicOr(&cnt->errors, 2u);
       ccount[p] = hit ? 1u : 0u;
-      route[p] |= 8u;
+      if (route) route[p] |= 8u;
       if (hit) {
         V3 pos = point_a - e.n * depth;
-        cslots[(uint64_t)p * 8] = make_float4(pos.x, pos.y, pos.z, depth);
-        cslots[(uint64_t)p * 8 + 1] = make_float4(e.n.x, e.n.y, e.n.z, 0.0f);
+        cslots[(uint64_t)p * stride] = make_float4(pos.x, pos.y, pos.z, depth);
+        cslots[(uint64_t)p * stride + 1] = make_float4(e.n.x, e.n.y, e.n.z, 0.0f);
       }
     }
     wave_sync();
   }
+}
+
+__global__ __launch_bounds__(64) void k_epa_wave(const uint32_t* __restrict__ n_queue,
+                                                 const float4* __restrict__ epa_q, const uint32_t* __restrict__ pairs,
+                                                 const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
+                                                 uint32_t* __restrict__ ccount, float4* __restrict__ cslots,
+                                                 uint32_t* __restrict__ route, Counters* cnt) {
+  __shared__ EpaLds S;
+  epa_queue_run(n_queue, epa_q, WorldEpaSrc{pairs, bodies, hull}, ccount, cslots, 8u, route, cnt, S);
 }
 
 }  // namespace bch

@@ -17,6 +17,7 @@
 #include "bch_sort.hpp"
 #include "bch_depen.hpp"
 #include "bch_manifold.hpp"
+#include "bch_mesh.hpp"
 
 using namespace bch;
 
@@ -107,6 +108,16 @@ struct bch_context {
   uint64_t mNP = 0;  // points emitted by the last pass
   bool aabb_valid = false;  // c->aabb holds the previous pass's boxes (kept for inactive bodies)
   DevBuf vel;
+
+  // triangle meshes (host copies; uploaded when changed) and the buffers of
+  // the convex-vs-mesh batch
+  std::vector<MeshRec> mesh_recs;
+  std::vector<float4> mesh_nodes, mesh_tris, mesh_tri_aabb;
+  std::vector<uint32_t> mesh_order;
+  bool mesh_dirty = false;
+  DevBuf d_mesh_recs, d_mesh_nodes, d_mesh_tris, d_mesh_tri_aabb, d_mesh_order;
+  DevBuf mx_pairs, mx_poses, mx_seeds, mx_first, mx_count, mx_offs, mx_sub, mx_sub_cnt, mx_sub_slots, mx_epa_q,
+      mx_out_count, mx_out_pts;
 
   // proxy API state (host mirror of the broadphase slots)
   std::vector<float> proxy_box;  // 6 per slot
@@ -596,6 +607,63 @@ void run_aabbs(bch_context* c, float aabb_margin) {
   c->aabb_valid = true;
 }
 
+// ----------------------------------------------------- mesh BVH (host) --
+// The TriangleMesh BVH (shapes/bvh.rs:40-56, 170-269): a node's box covers
+// its triangles' vertices; a node of more than four triangles splits its
+// slice at the middle after a stable sort of the triangle centres
+// ((a + b) + c) / 3 along the longest axis of the box (ties x, y, z).
+struct MeshBuilder {
+  const float* v;
+  const uint32_t* idx;
+  std::vector<uint32_t> order;
+  std::vector<float4> nodes;  // two per node
+  float coord(uint32_t t, int k, int axis) const { return v[3 * idx[3 * t + k] + axis]; }
+  float centre(uint32_t t, int axis) const {
+    return ((coord(t, 0, axis) + coord(t, 1, axis)) + coord(t, 2, axis)) * (1.0f / 3.0f);
+  }
+  static int64_t key(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return (u & 0x80000000u) ? -(int64_t)(u & 0x7fffffffu) - 1 : (int64_t)u;
+  }
+  uint32_t build(uint32_t start, uint32_t count) {
+    float lo[3] = {0.0f, 0.0f, 0.0f}, hi[3] = {0.0f, 0.0f, 0.0f};
+    bool any = false;
+    for (uint32_t k = start; k < start + count; ++k)
+      for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < 3; ++a) {
+          float x = coord(order[k], c, a);
+          if (!any) {
+            lo[a] = hi[a] = x;
+          } else {
+            lo[a] = x < lo[a] ? x : lo[a];
+            hi[a] = x > hi[a] ? x : hi[a];
+          }
+          if (a == 2) any = true;
+        }
+    uint32_t index = (uint32_t)(nodes.size() / 2);
+    nodes.push_back(make_float4(lo[0], lo[1], lo[2], u2f(start)));
+    nodes.push_back(make_float4(hi[0], hi[1], hi[2], u2f(count | kMeshLeaf)));
+    if (count <= 4) return index;
+    float e[3] = {(hi[0] - lo[0]) * 0.5f, (hi[1] - lo[1]) * 0.5f, (hi[2] - lo[2]) * 0.5f};
+    int axis = 0;
+    float m = e[0];
+    if (e[1] > m) {
+      axis = 1;
+      m = e[1];
+    }
+    if (e[2] > m) axis = 2;
+    std::stable_sort(order.begin() + start, order.begin() + start + count,
+                     [&](uint32_t x, uint32_t y) { return key(centre(x, axis)) < key(centre(y, axis)); });
+    uint32_t mid = count / 2;
+    uint32_t l = build(start, mid);
+    uint32_t r = build(start + mid, count - mid);
+    nodes[2 * index].w = u2f(l);
+    nodes[2 * index + 1].w = u2f(r);
+    return index;
+  }
+};
+
 // One world pass (AABBs, broadphase, narrowphase) on c->stream.  Eager mode
 // reads counts back as it goes; capacity mode (c->cap_mode) keeps them on the
 // device.
@@ -799,7 +867,10 @@ void bch_destroy(bch_context* c) {
                    &c->pen_g, &c->g14, &c->body_grad, &c->pen_part, &c->dhist, &c->dstep, &c->mkeys[0],
                    &c->mkeys[1], &c->mvalid[0], &c->mvalid[1], &c->ment[0], &c->ment[1], &c->bflags, &c->bgm,
                    &c->mroute, &c->msrc, &c->mcnt, &c->mslots, &c->moff, &c->mpts, &c->mseeds, &c->mimp,
-                   &c->vel};
+                   &c->vel, &c->d_mesh_recs, &c->d_mesh_nodes, &c->d_mesh_tris, &c->d_mesh_tri_aabb,
+                   &c->d_mesh_order, &c->mx_pairs, &c->mx_poses, &c->mx_seeds, &c->mx_first, &c->mx_count,
+                   &c->mx_offs, &c->mx_sub, &c->mx_sub_cnt, &c->mx_sub_slots, &c->mx_epa_q, &c->mx_out_count,
+                   &c->mx_out_pts};
   for (DevBuf* b : all) b->release();
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->own_stream);
@@ -1457,6 +1528,172 @@ int bch_world_manifold_reset(bch_context* c) {
   if (!c) return BCH_INVALID_ARG;
   c->mM = 0;
   c->mNP = 0;
+  return BCH_OK;
+}
+
+// ------------------------------------------------------ triangle meshes --
+int bch_mesh_add(bch_context* c, const float* vertices, uint32_t n_vertices, const uint32_t* indices,
+                 uint32_t n_triangles, uint32_t* out_mesh) {
+  if (!c || !out_mesh || (n_vertices && !vertices) || (n_triangles && !indices))
+    return fail(c, BCH_INVALID_ARG, "NULL argument");
+  for (uint64_t k = 0; k < 3ull * n_triangles; ++k)
+    if (indices[k] >= n_vertices) return fail(c, BCH_INVALID_ARG, "triangle index out of range");
+  MeshRec r{};
+  r.node_off = (uint32_t)(c->mesh_nodes.size() / 2);
+  r.order_off = (uint32_t)c->mesh_order.size();
+  r.tri_off = (uint32_t)(c->mesh_tris.size() / 3);
+  r.ntri = n_triangles;
+  // triangles, their AABBs and the local bounds (triangle_mesh.rs:99-113)
+  float blo[3] = {0.0f, 0.0f, 0.0f}, bhi[3] = {0.0f, 0.0f, 0.0f};
+  for (uint32_t t = 0; t < n_triangles; ++t) {
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+      const float* p = vertices + 3 * (size_t)indices[3 * (size_t)t + k];
+      c->mesh_tris.push_back(make_float4(p[0], p[1], p[2], 0.0f));
+    }
+    const float* a = vertices + 3 * (size_t)indices[3 * (size_t)t];
+    const float* b = vertices + 3 * (size_t)indices[3 * (size_t)t + 1];
+    const float* d = vertices + 3 * (size_t)indices[3 * (size_t)t + 2];
+    for (int k = 0; k < 3; ++k) {  // a.min(b.min(c)), a.max(b.max(c))
+      float mbc = d[k] < b[k] ? d[k] : b[k], xbc = d[k] > b[k] ? d[k] : b[k];
+      lo[k] = mbc < a[k] ? mbc : a[k];
+      hi[k] = xbc > a[k] ? xbc : a[k];
+    }
+    c->mesh_tri_aabb.push_back(make_float4(lo[0], lo[1], lo[2], 0.0f));
+    c->mesh_tri_aabb.push_back(make_float4(hi[0], hi[1], hi[2], 0.0f));
+    for (int k = 0; k < 3; ++k) {
+      blo[k] = t == 0 ? lo[k] : (lo[k] < blo[k] ? lo[k] : blo[k]);
+      bhi[k] = t == 0 ? hi[k] : (hi[k] > bhi[k] ? hi[k] : bhi[k]);
+    }
+  }
+  // contact_breaking_threshold over the local bounds (shapes/mod.rs:388-392)
+  float dx = bhi[0] - blo[0], dy = bhi[1] - blo[1], dz = bhi[2] - blo[2];
+  float cx = (blo[0] + bhi[0]) * 0.5f, cy = (blo[1] + bhi[1]) * 0.5f, cz = (blo[2] + bhi[2]) * 0.5f;
+  float disc = sqrtf((dx * dx + dy * dy) + dz * dz) * 0.5f + sqrtf((cx * cx + cy * cy) + cz * cz);
+  r.threshold = disc * 0.02f;
+  MeshBuilder mb{vertices, indices, {}, {}};
+  mb.order.resize(n_triangles);
+  for (uint32_t t = 0; t < n_triangles; ++t) mb.order[t] = t;
+  mb.build(0, n_triangles);
+  c->mesh_nodes.insert(c->mesh_nodes.end(), mb.nodes.begin(), mb.nodes.end());
+  c->mesh_order.insert(c->mesh_order.end(), mb.order.begin(), mb.order.end());
+  *out_mesh = (uint32_t)c->mesh_recs.size();
+  c->mesh_recs.push_back(r);
+  c->mesh_dirty = true;
+  return BCH_OK;
+}
+
+int bch_mesh_clear(bch_context* c) {
+  if (!c) return BCH_INVALID_ARG;
+  c->mesh_recs.clear();
+  c->mesh_nodes.clear();
+  c->mesh_tris.clear();
+  c->mesh_tri_aabb.clear();
+  c->mesh_order.clear();
+  c->mesh_dirty = true;
+  return BCH_OK;
+}
+
+int bch_collide_mesh_batch(bch_context* c, const bch_bodies* bodies, const uint32_t* pairs, const float* mesh_poses,
+                           const uint8_t* mesh_first, uint64_t n_pairs, const float* seeds, uint8_t* out_count,
+                           float* out_points) {
+  if (!c) return BCH_INVALID_ARG;
+  if (n_pairs && (!pairs || !mesh_poses || !out_count || !out_points)) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  if (n_pairs > 0x7fffffffull) return fail(c, BCH_INVALID_ARG, "too many pairs");
+  int rc = validate_bodies(c, bodies);
+  if (rc) return rc;
+  for (uint64_t k = 0; k < n_pairs; ++k) {
+    uint32_t b = pairs[2 * k], m = pairs[2 * k + 1];
+    if (b >= bodies->n || m >= c->mesh_recs.size()) return fail(c, BCH_INVALID_ARG, "pair index out of range");
+    if (bodies->shape_type[b] == BCH_PLANE)
+      return fail(c, BCH_INVALID_ARG, "plane-vs-mesh pairs are not part of this path");
+  }
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  rc = upload_bodies(c, bodies, nullptr);
+  if (rc) return rc;
+  if (c->mesh_dirty) {
+    size_t nm = std::max<size_t>(c->mesh_recs.size(), 1);
+    ENSURE(c, c->d_mesh_recs, nm * sizeof(MeshRec));
+    ENSURE(c, c->d_mesh_nodes, std::max<size_t>(c->mesh_nodes.size(), 1) * sizeof(float4));
+    ENSURE(c, c->d_mesh_tris, std::max<size_t>(c->mesh_tris.size(), 1) * sizeof(float4));
+    ENSURE(c, c->d_mesh_tri_aabb, std::max<size_t>(c->mesh_tri_aabb.size(), 1) * sizeof(float4));
+    ENSURE(c, c->d_mesh_order, std::max<size_t>(c->mesh_order.size(), 1) * sizeof(uint32_t));
+    if (!c->mesh_recs.empty())
+      HIP_TRY(c, hipMemcpyAsync(c->d_mesh_recs.p, c->mesh_recs.data(), c->mesh_recs.size() * sizeof(MeshRec),
+                                hipMemcpyHostToDevice, s));
+    if (!c->mesh_nodes.empty())
+      HIP_TRY(c, hipMemcpyAsync(c->d_mesh_nodes.p, c->mesh_nodes.data(), c->mesh_nodes.size() * sizeof(float4),
+                                hipMemcpyHostToDevice, s));
+    if (!c->mesh_tris.empty()) {
+      HIP_TRY(c, hipMemcpyAsync(c->d_mesh_tris.p, c->mesh_tris.data(), c->mesh_tris.size() * sizeof(float4),
+                                hipMemcpyHostToDevice, s));
+      HIP_TRY(c, hipMemcpyAsync(c->d_mesh_tri_aabb.p, c->mesh_tri_aabb.data(),
+                                c->mesh_tri_aabb.size() * sizeof(float4), hipMemcpyHostToDevice, s));
+    }
+    if (!c->mesh_order.empty())
+      HIP_TRY(c, hipMemcpyAsync(c->d_mesh_order.p, c->mesh_order.data(), c->mesh_order.size() * sizeof(uint32_t),
+                                hipMemcpyHostToDevice, s));
+    c->mesh_dirty = false;
+  }
+  if (!n_pairs) return BCH_OK;
+  uint64_t P = n_pairs;
+  ENSURE(c, c->counters, sizeof(Counters));
+  ENSURE(c, c->mx_pairs, P * sizeof(uint2));
+  ENSURE(c, c->mx_poses, P * 7 * sizeof(float));
+  ENSURE(c, c->mx_seeds, P * 3 * sizeof(float));
+  ENSURE(c, c->mx_first, P);
+  ENSURE(c, c->mx_count, (P + 1) * sizeof(uint32_t));
+  ENSURE(c, c->mx_offs, (P + 1) * sizeof(uint32_t));
+  ENSURE(c, c->mx_out_count, P);
+  ENSURE(c, c->mx_out_pts, P * 28 * sizeof(float));
+  HIP_TRY(c, hipMemcpyAsync(c->mx_pairs.p, pairs, P * sizeof(uint2), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(c->mx_poses.p, mesh_poses, P * 7 * sizeof(float), hipMemcpyHostToDevice, s));
+  if (seeds) HIP_TRY(c, hipMemcpyAsync(c->mx_seeds.p, seeds, P * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+  if (mesh_first) HIP_TRY(c, hipMemcpyAsync(c->mx_first.p, mesh_first, P, hipMemcpyHostToDevice, s));
+  Counters* cnt = c->counters.as<Counters>();
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, cnt);
+  const MeshRec* recs = c->d_mesh_recs.as<MeshRec>();
+  hipLaunchKernelGGL(k_mesh_count, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->mx_pairs.as<uint2>(),
+                     (const float*)c->mx_poses.as<float>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(), recs,
+                     c->d_mesh_nodes.as<float4>(), c->d_mesh_order.as<uint32_t>(), c->d_mesh_tri_aabb.as<float4>(),
+                     c->mx_count.as<uint32_t>(), cnt);
+  rc = dev_scan(c, c->mx_count.as<uint32_t>(), c->mx_offs.as<uint32_t>(), P, c->scan_part, s);
+  if (rc) return rc;
+  uint32_t S = 0;
+  HIP_TRY(c, hipMemcpyAsync(&S, c->mx_offs.as<uint32_t>() + P, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  size_t Sm = std::max<uint32_t>(S, 1);
+  ENSURE(c, c->mx_sub, Sm * sizeof(uint2));
+  ENSURE(c, c->mx_sub_cnt, Sm * sizeof(uint32_t));
+  ENSURE(c, c->mx_sub_slots, Sm * 2 * sizeof(float4));
+  ENSURE(c, c->mx_epa_q, Sm * 4 * sizeof(float4));
+  hipLaunchKernelGGL(k_mesh_emit, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->mx_pairs.as<uint2>(),
+                     (const float*)c->mx_poses.as<float>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(), recs,
+                     c->d_mesh_nodes.as<float4>(), c->d_mesh_order.as<uint32_t>(), c->d_mesh_tri_aabb.as<float4>(),
+                     c->mx_offs.as<uint32_t>(), c->mx_sub.as<uint2>());
+  if (S) {
+    hipLaunchKernelGGL(k_mesh_detect, dim3(blocks(S)), dim3(kBlock), 0, s, S, c->mx_sub.as<uint2>(),
+                       c->mx_pairs.as<uint2>(), (const float*)c->mx_poses.as<float>(),
+                       seeds ? (const float*)c->mx_seeds.as<float>() : nullptr, c->bodies.as<BodyRec>(),
+                       c->hull.as<float4>(), recs, c->d_mesh_tris.as<float4>(), c->mx_sub_cnt.as<uint32_t>(),
+                       c->mx_sub_slots.as<float4>(), c->mx_epa_q.as<float4>(), cnt);
+    MeshEpaSrc src{c->mx_sub.as<uint2>(), c->mx_pairs.as<uint2>(), c->mx_poses.as<float>(), c->bodies.as<BodyRec>(),
+                   c->hull.as<float4>(), recs, c->d_mesh_tris.as<float4>()};
+    hipLaunchKernelGGL(k_epa_mesh, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->mx_epa_q.as<float4>(), src,
+                       c->mx_sub_cnt.as<uint32_t>(), c->mx_sub_slots.as<float4>(), cnt);
+  }
+  hipLaunchKernelGGL(k_mesh_reduce, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->mx_offs.as<uint32_t>(),
+                     c->mx_sub_cnt.as<uint32_t>(), c->mx_sub_slots.as<float4>(),
+                     mesh_first ? (const uint8_t*)c->mx_first.as<uint8_t>() : nullptr, c->mx_out_count.as<uint8_t>(),
+                     c->mx_out_pts.as<float>());
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipMemcpyAsync(out_count, c->mx_out_count.p, P, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(out_points, c->mx_out_pts.p, P * 28 * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  if (c->host_cnt.errors & 32u) return fail(c, BCH_HIP_ERROR, "mesh BVH deeper than the traversal stack");
+  if (c->host_cnt.errors & 2u) return fail(c, BCH_HIP_ERROR, "EPA polytope store overflow");
   return BCH_OK;
 }
 

@@ -133,6 +133,29 @@ int bch_bp_raycast_candidates(bch_context* ctx, const float ray_from[3], const f
 int bch_collide_batch(bch_context* ctx, const bch_bodies* bodies, const uint32_t* pairs, uint64_t n_pairs,
                       const float* seeds, uint8_t* out_count, float* out_points);
 
+/* ------------------------------------------------- triangle meshes -- */
+/* A static triangle mesh (TriangleMesh::new, shapes/triangle_mesh.rs:84-123):
+ * vertices [3 * n_vertices], triangles [3 * n_triangles] vertex indices.  The
+ * BVH is built as the reference builds it (shapes/bvh.rs:170-269), so
+ * candidate triangles come in the reference's order.  *out_mesh = its id. */
+int bch_mesh_add(bch_context* ctx, const float* vertices, uint32_t n_vertices, const uint32_t* indices,
+                 uint32_t n_triangles, uint32_t* out_mesh);
+/* Drop every mesh (ids restart at 0). */
+int bch_mesh_clear(bch_context* ctx);
+/* collide_seeded's convex-vs-mesh arms (narrowphase/mod.rs:177-215,
+ * convex_mesh.rs:24-145, without internal-edge info) for a batch: pair k =
+ * convex body pairs[2k] of `bodies` against mesh pairs[2k+1] at mesh pose
+ * mesh_poses[7k ..] (position xyz, orientation xyzw).  mesh_first[k] != 0
+ * (array may be NULL) makes the mesh shape A: the manifold is flipped.
+ * Contact band = min(convex, mesh) contact_breaking_threshold.  seeds as in
+ * bch_collide_batch.  out_count[k] <= 4 points (ContactManifold::reduce,
+ * manifold.rs:76-110), out_points[28k + 7c ..] as in bch_collide_batch.
+ * Plane bodies are rejected (plane-vs-mesh is not part of this path).  Host
+ * pointers. */
+int bch_collide_mesh_batch(bch_context* ctx, const bch_bodies* bodies, const uint32_t* pairs, const float* mesh_poses,
+                           const uint8_t* mesh_first, uint64_t n_pairs, const float* seeds, uint8_t* out_count,
+                           float* out_points);
+
 /* ---------------------------------------------- world-resident pipeline -- */
 /* The per-step detection pass of PhysicsWorld (world/step.rs:349-416 +
  * 445-854 for rigid pairs): world AABBs padded by `aabb_margin`, the sorted

@@ -1884,4 +1884,259 @@ inline void pm_load(const float* e, PMan& m) {
   m.dqb = Quat<float>{e[60], e[61], e[62], e[63]};
 }
 
+// ------------------------------------------------------- triangle meshes --
+// TriangleMesh (crates/bullet-collision/src/shapes/triangle_mesh.rs:84-154),
+// its Bvh (shapes/bvh.rs:40-269) and convex_mesh
+// (narrowphase/convex_mesh.rs:24-272), f32.
+struct TriMesh {
+  std::vector<V3<float>> v;
+  std::vector<uint32_t> idx;  // 3 per triangle
+  std::vector<Aabb<float>> tri_aabb;
+  std::vector<V3<float>> tri_pts;  // 3 per triangle: the three-vertex hulls
+  Aabb<float> bounds{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+  struct Node {
+    Aabb<float> box;
+    bool leaf;
+    uint32_t left, right, start, count;
+  };
+  std::vector<Node> nodes;
+  std::vector<uint32_t> order;
+  size_t ntri() const { return idx.size() / 3; }
+  V3<float> vert(uint32_t t, int k) const { return v[idx[3 * t + k]]; }
+
+  // bvh.rs:252-255
+  V3<float> center(uint32_t t) const { return ((vert(t, 0) + vert(t, 1)) + vert(t, 2)) * (1.0f / 3.0f); }
+  // bvh.rs:204-222
+  Aabb<float> slice_aabb(uint32_t start, uint32_t count) const {
+    bool any = false;
+    Aabb<float> a{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+    for (uint32_t k = start; k < start + count; ++k)
+      for (int c = 0; c < 3; ++c) {
+        V3<float> p = vert(order[k], c);
+        if (!any) {
+          a = Aabb<float>{p, p};
+          any = true;
+        } else {
+          a = Aabb<float>{vmin(a.min, p), vmax(a.max, p)};  // util.rs:41-46
+        }
+      }
+    return a;
+  }
+  // bvh.rs:170-202 (longest axis bvh.rs:257-269; stable sort on total order)
+  uint32_t build_node(uint32_t start, uint32_t count) {
+    Aabb<float> box = slice_aabb(start, count);
+    uint32_t index = (uint32_t)nodes.size();
+    nodes.push_back(Node{box, true, 0, 0, start, count});
+    if (count <= 4) return index;
+    V3<float> e = (box.max - box.min) * 0.5f;
+    int axis = 0;
+    float mxv = e.x;
+    if (e.y > mxv) {
+      axis = 1;
+      mxv = e.y;
+    }
+    if (e.z > mxv) axis = 2;
+    std::stable_sort(order.begin() + start, order.begin() + start + count, [&](uint32_t a, uint32_t b) {
+      return total_key(comp(center(a), axis)) < total_key(comp(center(b), axis));
+    });
+    uint32_t mid = count / 2;
+    uint32_t l = build_node(start, mid);
+    uint32_t r = build_node(start + mid, count - mid);
+    nodes[index].leaf = false;
+    nodes[index].left = l;
+    nodes[index].right = r;
+    return index;
+  }
+  // triangle_mesh.rs:84-123 (one part)
+  void build(const float* verts, uint32_t nv, const uint32_t* ind, uint32_t nt) {
+    v.resize(nv);
+    for (uint32_t i = 0; i < nv; ++i) v[i] = mk(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+    idx.assign(ind, ind + 3 * (size_t)nt);
+    order.resize(nt);
+    for (uint32_t t = 0; t < nt; ++t) order[t] = t;
+    nodes.clear();
+    build_node(0, nt);
+    tri_aabb.resize(nt);
+    tri_pts.resize(3 * (size_t)nt);
+    for (uint32_t t = 0; t < nt; ++t) {
+      V3<float> a = vert(t, 0), b = vert(t, 1), c = vert(t, 2);
+      tri_pts[3 * t] = a;
+      tri_pts[3 * t + 1] = b;
+      tri_pts[3 * t + 2] = c;
+      tri_aabb[t] = Aabb<float>{vmin(a, vmin(b, c)), vmax(a, vmax(b, c))};
+      bounds = t == 0 ? tri_aabb[0] : Aabb<float>{vmin(bounds.min, tri_aabb[t].min), vmax(bounds.max, tri_aabb[t].max)};
+    }
+  }
+  // bvh.rs:58-79: depth first, right child popped first
+  void query(const Aabb<float>& q, std::vector<uint32_t>& out) const {
+    out.clear();
+    if (nodes.empty()) return;
+    std::vector<uint32_t> stack{0u};
+    while (!stack.empty()) {
+      uint32_t k = stack.back();
+      stack.pop_back();
+      const Node& n = nodes[k];
+      if (!n.box.overlaps(q)) continue;
+      if (!n.leaf) {
+        stack.push_back(n.left);
+        stack.push_back(n.right);
+      } else {
+        for (uint32_t i = n.start; i < n.start + n.count; ++i) out.push_back(order[i]);
+      }
+    }
+  }
+  // shapes/mod.rs:388-392 over the local bounds
+  float threshold() const {
+    float disc = len((bounds.max - bounds.min)) * 0.5f + len(bounds.center());
+    return disc * 0.02f;
+  }
+};
+
+// convex_mesh.rs:237-255
+inline float seg_closest_sq(V3<float> from, V3<float> to, V3<float> p, V3<float>& nearest) {
+  V3<float> v = to - from;
+  float t = dot(v, p - from);
+  if (t > 0.0f) {
+    float dvv = dot(v, v);
+    if (t < dvv) t /= dvv;
+    else t = 1.0f;
+  } else {
+    t = 0.0f;
+  }
+  nearest = from + v * t;
+  return len2(p - nearest);
+}
+
+// convex_mesh.rs:257-272
+inline bool point_in_triangle(V3<float> p, V3<float> v1, V3<float> v2, V3<float> v3, V3<float> normal) {
+  V3<float> e1 = v2 - v1, e2 = v3 - v2, e3 = v1 - v3;
+  float r1 = dot(cross(e1, normal), p - v1);
+  float r2 = dot(cross(e2, normal), p - v2);
+  float r3 = dot(cross(e3, normal), p - v3);
+  return (r1 > 0.0f && r2 > 0.0f && r3 > 0.0f) || (r1 <= 0.0f && r2 <= 0.0f && r3 <= 0.0f);
+}
+
+// convex_mesh.rs:168-235: point (mesh frame), centre-pointing normal, penetration
+inline bool sphere_triangle_contact(V3<float> center, float radius, float threshold, V3<float> a, V3<float> b,
+                                    V3<float> c, V3<float>& point, V3<float>& normal_out, float& pen) {
+  const float EPS = std::numeric_limits<float>::epsilon();
+  float rt = radius + threshold;
+  V3<float> normal = cross(b - a, c - a);
+  float l2 = len2(normal);
+  if (l2 < EPS * EPS) return false;
+  normal = normal * (1.0f / std::sqrt(l2));
+  float dist = dot(normal, center - a);
+  if (dist < 0.0f) {
+    dist = -dist;
+    normal = -normal;
+  }
+  if (dist >= rt) return false;
+  V3<float> cp;
+  if (point_in_triangle(center, a, b, c, normal)) {
+    cp = center - normal * dist;
+  } else {
+    float min_d2 = rt * rt;
+    bool found = false;
+    V3<float> ends[3][2] = {{a, b}, {b, c}, {c, a}};
+    for (int k = 0; k < 3; ++k) {
+      V3<float> on;
+      float d2 = seg_closest_sq(ends[k][0], ends[k][1], center, on);
+      if (d2 < min_d2) {
+        min_d2 = d2;
+        cp = on;
+        found = true;
+      }
+    }
+    if (!found) return false;
+  }
+  V3<float> to_c = center - cp;
+  float d2 = len2(to_c);
+  if (d2 >= rt * rt) return false;
+  point = cp;
+  if (d2 > EPS) {
+    float d = std::sqrt(d2);
+    normal_out = to_c * (1.0f / d);
+    pen = radius - d;
+  } else {
+    normal_out = normal;
+    pen = radius;
+  }
+  return true;
+}
+
+// ContactManifold::reduce (manifold.rs:76-110): deepest first, then greedily
+// the candidate farthest (min squared distance) from those kept.
+inline void reduce_manifold(std::vector<ContactPoint<float>>& pts, size_t max) {
+  if (pts.size() <= max) return;
+  std::vector<ContactPoint<float>> cand = pts;
+  std::stable_sort(cand.begin(), cand.end(), [](const ContactPoint<float>& a, const ContactPoint<float>& b) {
+    return total_key(a.penetration) > total_key(b.penetration);
+  });
+  std::vector<ContactPoint<float>> kept{cand[0]};
+  std::vector<bool> used(cand.size(), false);
+  used[0] = true;
+  while (kept.size() < max) {
+    size_t best = 0;
+    float best_sep = -std::numeric_limits<float>::infinity();
+    for (size_t i = 0; i < cand.size(); ++i) {
+      if (used[i]) continue;
+      float sep = std::numeric_limits<float>::infinity();
+      for (const auto& q : kept) sep = std::fmin(sep, len2(cand[i].position - q.position));
+      if (sep > best_sep) {
+        best_sep = sep;
+        best = i;
+      }
+    }
+    kept.push_back(cand[best]);
+    used[best] = true;
+  }
+  pts = kept;
+}
+
+// convex_mesh (convex_mesh.rs:24-145) without internal-edge info: the convex
+// is A, the mesh B, the normal points from the convex to the mesh.
+inline bool convex_mesh(const Shape<float>& convex, V3<float> pc, Quat<float> qc, const TriMesh& mesh, V3<float> pm,
+                        Quat<float> qm, const V3<float>* seed, float threshold,
+                        std::vector<ContactPoint<float>>& out) {
+  out.clear();
+  Aabb<float> cw = convex.world_aabb(pc, qc);
+  Quat<float> inv_m = qconj(qm);
+  V3<float> lmin = splat(std::numeric_limits<float>::infinity());
+  V3<float> lmax = splat(-std::numeric_limits<float>::infinity());
+  V3<float> c = cw.center(), e = (cw.max - cw.min) * 0.5f;
+  for (int i = 0; i < 8; ++i) {
+    V3<float> corner = mk((i & 1) == 0 ? c.x + e.x : c.x - e.x, (i & 2) == 0 ? c.y + e.y : c.y - e.y,
+                          (i & 4) == 0 ? c.z + e.z : c.z - e.z);
+    V3<float> p = qrot(inv_m, corner - pm);
+    lmin = vmin(lmin, p);
+    lmax = vmax(lmax, p);
+  }
+  Aabb<float> query{lmin, lmax};
+  std::vector<uint32_t> cand;
+  mesh.query(query, cand);
+  if (convex.kind == SPHERE) {
+    V3<float> center = qrot(inv_m, pc - pm);
+    float r = convex.radius;
+    for (uint32_t t : cand) {
+      if (!mesh.tri_aabb[t].overlaps(query)) continue;
+      V3<float> pl, nl;
+      float pen;
+      if (sphere_triangle_contact(center, r, threshold, mesh.vert(t, 0), mesh.vert(t, 1), mesh.vert(t, 2), pl, nl, pen))
+        out.push_back(ContactPoint<float>{pm + qrot(qm, pl), qrot(qm, -nl), pen});
+    }
+  } else {
+    for (uint32_t t : cand) {
+      if (!mesh.tri_aabb[t].overlaps(query)) continue;
+      Shape<float> tri = Shape<float>::hull(&mesh.tri_pts[3 * (size_t)t], 3);
+      PairPose<float> pr{&convex, pc, qc, &tri, pm, qm};
+      Manifold<float> m;
+      if (convex_convex(pr, seed, threshold, m))
+        for (int k = 0; k < m.n; ++k) out.push_back(m.pts[k]);
+    }
+  }
+  if (out.empty()) return false;
+  reduce_manifold(out, 4);
+  return true;
+}
+
 }  // namespace oracle

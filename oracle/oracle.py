@@ -87,6 +87,20 @@ def lib():
         L.oracle_pm_seed.argtypes = [C.c_void_p] * 2
         L.oracle_pm_pass.argtypes = ([scp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_float, C.c_float]
                                      + [C.c_void_p] * 3 + [C.c_int64] + [C.c_void_p] * 5)
+        L.oracle_sphere_triangle.restype = C.c_int
+        L.oracle_sphere_triangle.argtypes = [C.c_void_p, C.c_float, C.c_float] + [C.c_void_p] * 4
+        L.oracle_reduce.restype = C.c_int
+        L.oracle_reduce.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.oracle_mesh_create.restype = C.c_void_p
+        L.oracle_mesh_create.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]
+        L.oracle_mesh_destroy.argtypes = [C.c_void_p]
+        L.oracle_mesh_threshold.restype = C.c_float
+        L.oracle_mesh_threshold.argtypes = [C.c_void_p]
+        L.oracle_mesh_query.restype = C.c_int64
+        L.oracle_mesh_query.argtypes = [C.c_void_p] * 4 + [C.c_int64]
+        L.oracle_mesh_collide.restype = C.c_int
+        L.oracle_mesh_collide.argtypes = [C.c_void_p] * 7 + [C.c_int, C.c_void_p]
+        L.oracle_mesh_collide_scene.argtypes = [scp, C.c_void_p, C.c_void_p, C.c_int64] + [C.c_void_p] * 5
         L.oracle_pass_f32.restype = C.c_int64
         L.oracle_pass_f32.argtypes = [scp, C.c_float, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         _lib = L
@@ -482,3 +496,68 @@ def pm_pass(scene, pairs, cache, flags=None, group_mask=None, slop=0.005, warm=0
                          cnt.ctypes.data, pts.ctypes.data)
     new_keys = (pairs[:, 0].astype(np.uint64) << np.uint64(32)) | pairs[:, 1].astype(np.uint64)
     return dict(route=route, count=cnt, points=pts[:P], cache=(new_keys, ent_out[:P], valid))
+
+
+# ----------------------------------------------------------- triangle meshes --
+class Mesh:
+    """TriangleMesh (triangle_mesh.rs:84-123) with its BVH, in the oracle."""
+
+    def __init__(self, vertices, indices):
+        self.v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        self.i = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        self.h = lib().oracle_mesh_create(self.v.ctypes.data, self.v.shape[0], self.i.ctypes.data, self.i.shape[0])
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.oracle_mesh_destroy(self.h)
+            self.h = None
+
+    @property
+    def threshold(self):
+        return float(lib().oracle_mesh_threshold(self.h))
+
+    def query(self, qmin, qmax):
+        """triangles_in_aabb in the BVH's order."""
+        out = np.zeros(max(self.i.shape[0], 1), np.uint32)
+        n = lib().oracle_mesh_query(self.h, _f32(qmin, 3), _f32(qmax, 3), out.ctypes.data, out.shape[0])
+        return out[:n]
+
+    def collide(self, convex: Shape, pc, qc, pm, qm, seed=None, mesh_first=False):
+        """collide_seeded(convex, mesh) (or (mesh, convex) with mesh_first) -> points [k, 7]."""
+        out = (C.c_float * 28)()
+        n = lib().oracle_mesh_collide(C.byref(convex.c()), _f32(pc, 3), _f32(qc, 4), self.h, _f32(pm, 3),
+                                      _f32(qm, 4), None if seed is None else _f32(seed, 3), int(mesh_first), out)
+        return np.array(out[:], np.float32).reshape(4, 7)[:min(n, 4)], n
+
+
+def mesh_collide_scene(scene, meshes, pairs, mesh_pose, seeds=None, mesh_first=None):
+    """Batch of (body, mesh) pairs -> (count [P], points [P, 4, 7])."""
+    r = _SceneRef(scene)
+    pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+    P = pairs.shape[0]
+    hs = (C.c_void_p * max(len(meshes), 1))(*[m.h for m in meshes])
+    mp = np.ascontiguousarray(mesh_pose, np.float32).reshape(-1, 7)
+    sd = None if seeds is None else np.ascontiguousarray(seeds, np.float32)
+    mf = None if mesh_first is None else np.ascontiguousarray(mesh_first, np.uint8)
+    cnt = np.zeros(P, np.uint8)
+    pts = np.zeros((max(P, 1), 4, 7), np.float32)
+    lib().oracle_mesh_collide_scene(C.byref(r.c), hs, pairs.ctypes.data, P, mp.ctypes.data,
+                                    None if sd is None else sd.ctypes.data, None if mf is None else mf.ctypes.data,
+                                    cnt.ctypes.data, pts.ctypes.data)
+    return cnt, pts[:P]
+
+
+def sphere_triangle(center, radius, threshold, a, b, c):
+    """sphere_triangle_contact (convex_mesh.rs:168-235) -> (point, normal, penetration) or None."""
+    out = (C.c_float * 7)()
+    if not lib().oracle_sphere_triangle(_f32(center, 3), radius, threshold, _f32(a, 3), _f32(b, 3), _f32(c, 3), out):
+        return None
+    o = np.array(out[:], np.float32)
+    return o[0:3], o[3:6], float(o[6])
+
+
+def reduce(points7, max_points=4):
+    """ContactManifold::reduce (manifold.rs:76-110) -> points [k, 7]."""
+    p = np.ascontiguousarray(points7, np.float32).reshape(-1, 7).copy()
+    n = lib().oracle_reduce(p.ctypes.data, p.shape[0], max_points)
+    return p[:n]

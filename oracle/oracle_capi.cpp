@@ -624,4 +624,102 @@ void oracle_pm_pass(const or_scene* sc, const uint32_t* pairs, int64_t npairs, c
   }
 }
 
+// ----------------------------------------------------------- triangle meshes --
+void* oracle_mesh_create(const float* verts, uint32_t nv, const uint32_t* ind, uint32_t nt) {
+  TriMesh* m = new TriMesh();
+  m->build(verts, nv, ind, nt);
+  return m;
+}
+
+void oracle_mesh_destroy(void* h) { delete static_cast<TriMesh*>(h); }
+
+float oracle_mesh_threshold(const void* h) { return static_cast<const TriMesh*>(h)->threshold(); }
+
+// triangles_in_aabb (triangle_mesh.rs:150-154, bvh.rs:58-79) in query order.
+int64_t oracle_mesh_query(const void* h, const float* qmin, const float* qmax, uint32_t* out, int64_t cap) {
+  std::vector<uint32_t> c;
+  static_cast<const TriMesh*>(h)->query(Aabb<float>{v3p<float>(qmin), v3p<float>(qmax)}, c);
+  for (size_t k = 0; k < c.size() && (int64_t)k < cap; ++k) out[k] = c[k];
+  return (int64_t)c.size();
+}
+
+// collide_seeded's convex-vs-mesh arms (narrowphase/mod.rs:177-215): the
+// pair threshold min(convex, mesh) (mod.rs:52-57); with mesh_first the mesh
+// is shape A and the manifold is flipped.  Returns the point count; out 4 x 7.
+static int mesh_pair(const Shape<float>& convex, V3<float> pc, Quat<float> qc, const TriMesh& mesh, V3<float> pm,
+                     Quat<float> qm, const float* seed, int mesh_first, float* out) {
+  float thr = fmin_(convex.contact_breaking_threshold(), mesh.threshold());
+  if (mesh_first) thr = fmin_(mesh.threshold(), convex.contact_breaking_threshold());
+  V3<float> sd;
+  bool has_seed = seed && !std::isnan(seed[0]);
+  if (has_seed) sd = v3p<float>(seed);
+  std::vector<ContactPoint<float>> pts;
+  if (!convex_mesh(convex, pc, qc, mesh, pm, qm, has_seed ? &sd : nullptr, thr, pts)) return 0;
+  for (size_t k = 0; k < pts.size() && k < 4; ++k) {
+    float* o = out + 7 * k;
+    V3<float> n = mesh_first ? -pts[k].normal : pts[k].normal;
+    o[0] = pts[k].position.x; o[1] = pts[k].position.y; o[2] = pts[k].position.z;
+    o[3] = n.x; o[4] = n.y; o[5] = n.z;
+    o[6] = pts[k].penetration;
+  }
+  return (int)pts.size();
+}
+
+int oracle_mesh_collide(const or_shape* convex, const float* pc, const float* qc, const void* mesh, const float* pm,
+                        const float* qm, const float* seed, int mesh_first, float* out) {
+  std::vector<V3<float>> st;
+  Shape<float> cv = from_desc<float>(convex, st);
+  return mesh_pair(cv, v3p<float>(pc), q4p<float>(qc), *static_cast<const TriMesh*>(mesh), v3p<float>(pm),
+                   q4p<float>(qm), seed, mesh_first, out);
+}
+
+// A batch over scene bodies: pair k = (body pairs[2k], mesh handle
+// meshes[pairs[2k+1]]) at mesh pose mesh_pose[7k..] (position, orientation);
+// seeds[3k..] (NULL, or NaN x = None); mesh_first[k] (NULL = 0).
+void oracle_mesh_collide_scene(const or_scene* sc, void* const* meshes, const uint32_t* pairs, int64_t npairs,
+                               const float* mesh_pose, const float* seeds, const uint8_t* mesh_first,
+                               uint8_t* out_count, float* out_pts) {
+  SceneShapes<float> ss;
+  ss.build(sc);
+  for (int64_t k = 0; k < npairs; ++k) {
+    uint32_t b = pairs[2 * k];
+    const TriMesh& m = *static_cast<const TriMesh*>(meshes[pairs[2 * k + 1]]);
+    std::memset(out_pts + 28 * k, 0, 28 * sizeof(float));
+    out_count[k] = (uint8_t)mesh_pair(ss.shapes[b], ss.pos[b], ss.quat[b], m, v3p<float>(mesh_pose + 7 * k),
+                                      q4p<float>(mesh_pose + 7 * k + 3), seeds ? seeds + 3 * k : nullptr,
+                                      mesh_first ? mesh_first[k] : 0, out_pts + 28 * k);
+  }
+}
+
+// sphere_triangle_contact (convex_mesh.rs:168-235): 1 and point, normal,
+// penetration when it reports a contact.
+int oracle_sphere_triangle(const float* center, float radius, float threshold, const float* a, const float* b,
+                           const float* c, float* out7) {
+  V3<float> p, n;
+  float pen;
+  if (!sphere_triangle_contact(v3p<float>(center), radius, threshold, v3p<float>(a), v3p<float>(b), v3p<float>(c), p, n,
+                               pen))
+    return 0;
+  out7[0] = p.x; out7[1] = p.y; out7[2] = p.z;
+  out7[3] = n.x; out7[4] = n.y; out7[5] = n.z;
+  out7[6] = pen;
+  return 1;
+}
+
+// ContactManifold::reduce (manifold.rs:76-110) of n points (7 floats each) to
+// at most max, in place; returns the new count.
+int oracle_reduce(float* pts7, int n, int max) {
+  std::vector<ContactPoint<float>> v(n);
+  for (int k = 0; k < n; ++k)
+    v[k] = ContactPoint<float>{v3p<float>(pts7 + 7 * k), v3p<float>(pts7 + 7 * k + 3), pts7[7 * k + 6]};
+  reduce_manifold(v, (size_t)max);
+  for (size_t k = 0; k < v.size(); ++k) {
+    float* o = pts7 + 7 * k;
+    o[0] = v[k].position.x; o[1] = v[k].position.y; o[2] = v[k].position.z;
+    o[3] = v[k].normal.x; o[4] = v[k].normal.y; o[5] = v[k].normal.z;
+    o[6] = v[k].penetration;
+  }
+  return (int)v.size();
+}
+
 }  // extern "C"

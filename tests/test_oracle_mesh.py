@@ -1,0 +1,148 @@
+"""The oracle's convex-vs-triangle-mesh path (oracle.hpp: TriMesh, the BVH,
+sphere_triangle_contact, reduce_manifold, convex_mesh) against the
+reference's own tests (crates/bullet-collision/src/narrowphase/
+convex_mesh.rs:274-581, shapes/triangle_mesh.rs:255-300, manifold.rs:
+680-707) and against what the mathematics fixes (the BVH query returns
+exactly the triangles whose AABB a brute force finds among its leaves'
+boxes; reduce keeps the deepest point and a subset).
+"""
+import numpy as np
+import pytest
+
+ID = [0.0, 0.0, 0.0, 1.0]
+
+
+def _flat_ground(oracle):
+    """triangle_mesh.rs / convex_mesh.rs flat_ground(): the unit square at y = 0."""
+    return oracle.Mesh([[0, 0, 0], [1, 0, 0], [1, 0, 1], [0, 0, 1]], [[0, 1, 2], [0, 2, 3]])
+
+
+def _grid(oracle, n=5, spacing=1.0, height=None):
+    """A flat n x n vertex grid centred on the origin (the triangulated
+    heightfield of convex_mesh.rs:555-580)."""
+    h = (n - 1) * spacing / 2.0
+    xs = np.linspace(-h, h, n)
+    v = np.array([[x, 0.0 if height is None else height(x, z), z] for z in xs for x in xs], np.float32)
+    t = []
+    for r in range(n - 1):
+        for c in range(n - 1):
+            a, b, d, e = r * n + c, r * n + c + 1, (r + 1) * n + c, (r + 1) * n + c + 1
+            t += [[a, d, b], [b, d, e]]
+    return oracle.Mesh(v, np.array(t, np.uint32))
+
+
+def _sphere(oracle, r):
+    return oracle.Shape(oracle.SPHERE, [r])
+
+
+def test_local_aabb_and_query_of_flat_ground(oracle):
+    """triangle_mesh.rs:258-281: both triangles share the unit box, so the
+    query over the second one reports both."""
+    m = _flat_ground(oracle)
+    hits = m.query([0.8, -1.0, 0.8], [1.2, 1.0, 1.2])
+    assert 1 in hits.tolist() and len(hits) == 2
+
+
+def test_sphere_resting_on_mesh(oracle):
+    """convex_mesh.rs:283-307"""
+    m = _flat_ground(oracle)
+    pts, n = m.collide(_sphere(oracle, 0.5), [0.5, 0.4, 0.5], ID, [0, 0, 0], ID)
+    assert n >= 1
+    assert abs(pts[0, 4] - (-1.0)) < 1e-3 and abs(pts[0, 6] - 0.1) < 1e-3
+
+
+def test_sphere_fast_path_matches_gjk(oracle):
+    """convex_mesh.rs:309-374: the analytic sphere path against GJK+EPA on the
+    triangle as a three-vertex hull."""
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0.3, 0.2, 0.9]], np.float32)
+    m = oracle.Mesh(tri, [[0, 1, 2]])
+    s = _sphere(oracle, 0.5)
+    th = float(oracle.lib().oracle_threshold_f32(s.c()))
+    hull = oracle.Shape(oracle.HULL, hull=tri)
+    for c in ([0.4, 0.3, 0.4], [0.95, 0.15, 0.1], [-0.3, 0.3, 0.2], [0.3, 0.55, 0.3], [0.6, 0.05, 0.4]):
+        fast, nf = m.collide(s, c, ID, [0, 0, 0], ID)
+        slow, path = oracle.convex_convex(s, c, ID, hull, [0, 0, 0], ID, threshold=th)
+        assert (nf > 0) == (slow.shape[0] > 0), f"presence at {c}"
+        if nf:
+            assert abs(fast[0, 6] - slow[0, 6]) <= 5e-3
+            assert float(np.dot(fast[0, 3:6], slow[0, 3:6])) > 0.999
+            assert np.all(np.abs(fast[0, 0:3] - slow[0, 0:3]) <= 1e-2)
+
+
+def test_sphere_triangle_edge_vertex_band_backface(oracle):
+    """convex_mesh.rs:376-438"""
+    m = oracle.Mesh([[0, 0, 0], [1, 0, 0], [0, 0, 1]], [[0, 1, 2]])
+    s = _sphere(oracle, 0.5)
+    p, _ = m.collide(s, [0.0, 0.4, 0.0], ID, [0, 0, 0], ID)
+    assert abs(p[0, 6] - 0.1) < 1e-4 and abs(p[0, 4] + 1.0) < 1e-3 and float(np.dot(p[0, :3], p[0, :3])) < 1e-6
+    p, _ = m.collide(s, [0.5, 0.4, 0.0], ID, [0, 0, 0], ID)
+    assert abs(p[0, 6] - 0.1) < 1e-4 and abs(p[0, 4] + 1.0) < 1e-3
+    assert abs(p[0, 0] - 0.5) < 1e-6 and abs(p[0, 1]) < 1e-6
+    th = float(oracle.lib().oracle_threshold_f32(s.c()))
+    a, b, c = [0, 0, 0], [1, 0, 0], [0, 0, 1]
+    point, normal, pen = oracle.sphere_triangle([0.3, 0.51, 0.3], 0.5, th, a, b, c)
+    assert abs(pen + 0.01) < 1e-4 and abs(normal[1] - 1.0) < 1e-3 and abs(point[1]) < 1e-6
+    assert oracle.sphere_triangle([0.3, 0.6, 0.3], 0.5, th, a, b, c) is None
+    p, _ = m.collide(s, [0.3, -0.4, 0.3], ID, [0, 0, 0], ID)
+    assert abs(p[0, 6] - 0.1) < 1e-4 and abs(p[0, 4] - 1.0) < 1e-3 and abs(p[0, 1]) < 1e-6
+
+
+def test_separated_sphere_does_not_touch_mesh(oracle):
+    """convex_mesh.rs:439-456"""
+    _, n = _flat_ground(oracle).collide(_sphere(oracle, 0.5), [0.5, 5.0, 0.5], ID, [0, 0, 0], ID)
+    assert n == 0
+
+
+def test_box_over_many_triangles_reduces_contacts(oracle):
+    """convex_mesh.rs:554-580: a unit box over the four cells around the
+    origin of a flat 5 x 5 grid: at most four points, the deepest first
+    with the -Y normal and penetration 0.1."""
+    m = _grid(oracle, 5)
+    box = oracle.Shape(oracle.BOX, [0.5, 0.5, 0.5])
+    pts, n = m.collide(box, [0.0, 0.4, 0.0], ID, [0, 0, 0], ID)
+    assert 1 <= n <= 4
+    assert abs(pts[0, 4] + 1.0) < 1e-3 and abs(pts[0, 6] - 0.1) < 1e-3
+
+
+def test_mesh_first_flips_the_normal(oracle):
+    """narrowphase/mod.rs:177-190: mesh as shape A -> the manifold flipped."""
+    m = _flat_ground(oracle)
+    s = _sphere(oracle, 0.5)
+    a, _ = m.collide(s, [0.5, 0.4, 0.5], ID, [0, 0, 0], ID)
+    b, _ = m.collide(s, [0.5, 0.4, 0.5], ID, [0, 0, 0], ID, mesh_first=True)
+    assert np.array_equal(a[:, 3:6], -b[:, 3:6]) and np.array_equal(a[:, [0, 1, 2, 6]], b[:, [0, 1, 2, 6]])
+
+
+def test_bvh_query_is_the_brute_force_leaf_set(oracle):
+    """The query returns each triangle at most once, and a triangle whose own
+    AABB overlaps the query box is always among them (the leaves' boxes
+    contain their triangles)."""
+    rng = np.random.default_rng(5)
+    m = _grid(oracle, 17, 0.5, height=lambda x, z: 0.3 * np.sin(x) * np.cos(z))
+    tri = m.v[m.i]
+    tmin, tmax = tri.min(axis=1), tri.max(axis=1)
+    for _ in range(200):
+        c = rng.uniform(-4.5, 4.5, 3).astype(np.float32)
+        e = rng.uniform(0.05, 1.5, 3).astype(np.float32)
+        qmin, qmax = c - e, c + e
+        hits = m.query(qmin, qmax)
+        assert len(set(hits.tolist())) == len(hits)
+        must = np.nonzero(np.all((tmin <= qmax) & (qmin <= tmax), axis=1))[0]
+        assert set(must.tolist()) <= set(hits.tolist())
+
+
+def test_reduce_keeps_the_deepest_and_spreads_the_rest(oracle):
+    """manifold.rs:680-700"""
+    pt = lambda x, y, z, pen: [x, y, z, 0, 1, 0, pen]
+    pts = np.array([pt(0, 0, 0, 0.5), pt(0.01, 0, 0, 0.4), pt(1, 0, 0, 0.1), pt(0, 1, 0, 0.1), pt(0, 0, 1, 0.1),
+                    pt(1, 1, 1, 0.05)], np.float32)
+    r = oracle.reduce(pts, 4)
+    assert r.shape[0] == 4
+    assert np.array_equal(r[0, :3], [0, 0, 0]) and r[0, 6] == np.float32(0.5)
+    assert np.all(np.sum(r[1:, :3] ** 2, axis=1) > 0.5)
+
+
+def test_reduce_leaves_small_manifolds_alone(oracle):
+    """manifold.rs:702-707"""
+    pts = np.array([[0, 0, 0, 0, 1, 0, 0.2], [1, 0, 0, 0, 1, 0, 0.1], [0, 1, 0, 0, 1, 0, 0.1]], np.float32)
+    assert np.array_equal(oracle.reduce(pts, 4), pts)
