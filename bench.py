@@ -142,6 +142,67 @@ def manifold_frames(w, sc, margin, frames):
             "warm_pairs_per_s": round(out[-1]["pairs"] / (float(np.median(warm)) * 1e-3), 1)}
 
 
+def mesh_batch(bpa, steps, n_bodies=100_000, grid=362, cpu_sample=2000, cpu=True):
+    """Convex bodies against a static terrain mesh (bch_collide_mesh_batch):
+    `n_bodies` mixed convex bodies (boxes, spheres, capsules, hulls) near a
+    grid x grid-vertex wavy terrain (2 (grid-1)^2 triangles, 0.25 m cells),
+    one (body, mesh) pair each.  The call takes host buffers (bodies, pairs,
+    poses in; contacts out), so the time includes those transfers.  The
+    oracle's convex_mesh on `cpu_sample` of the pairs gives the CPU rate."""
+    rng = np.random.default_rng(13)
+    h = (grid - 1) * 0.25 / 2.0
+    xs = np.linspace(-h, h, grid).astype(np.float32)
+    X, Z = np.meshgrid(xs, xs)
+    Y = (0.8 * np.sin(0.3 * X) * np.cos(0.25 * Z)).astype(np.float32)
+    v = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.float32)
+    r = np.arange(grid - 1)
+    a = (r[:, None] * grid + r[None, :]).ravel()
+    tris = np.concatenate([np.stack([a, a + grid, a + 1], 1), np.stack([a + 1, a + grid, a + grid + 1], 1)])
+    tris = tris.astype(np.uint32)
+    ctx = bpa._Context(0)
+    t0 = time.perf_counter()
+    mesh = bpa.TriangleMesh(v, tris, ctx=ctx)
+    build_s = time.perf_counter() - t0
+    sc = bpa.scenes.mixed_convex(n_bodies, 17)
+    keep = np.nonzero(sc["shape_type"] != 4)[0]
+    x, z = rng.uniform(-h + 1, h - 1, (2, keep.shape[0]))
+    pos = sc["position"].copy()
+    pos[keep] = np.stack([x, 0.8 * np.sin(0.3 * x) * np.cos(0.25 * z) + rng.uniform(-0.3, 0.6, keep.shape[0]), z], 1)
+    sc = dict(sc)
+    sc["position"] = pos.astype(np.float32)
+    pairs = np.stack([keep, np.full(keep.shape[0], mesh.id)], 1).astype(np.uint32)
+    poses = np.tile(np.array([0, 0, 0, 0, 0, 0, 1], np.float32), (pairs.shape[0], 1))
+    bpa.collide_mesh_batch(sc, pairs, poses, ctx=ctx)
+    ts = []
+    for _ in range(max(steps, 1)):
+        t1 = time.perf_counter()
+        cnt, pts = bpa.collide_mesh_batch(sc, pairs, poses, ctx=ctx)
+        ts.append(time.perf_counter() - t1)
+    ms = float(np.median(ts)) * 1e3
+    out = {"workload": f"{pairs.shape[0]} convex bodies on a {tris.shape[0]}-triangle terrain",
+           "bvh_build_s": round(build_s, 3), "ms_per_batch": round(ms, 3),
+           "pairs_per_s": round(pairs.shape[0] / (ms * 1e-3), 1), "touching": int((cnt > 0).sum()),
+           "points": int(cnt.sum()), "includes": "bodies, pairs and poses uploaded, contacts copied back"}
+    if not cpu:
+        ctx.close()
+        return out
+    try:  # CPU baseline: the oracle's convex_mesh on a sample
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        om = O.Mesh(v, tris)
+        k = min(cpu_sample, pairs.shape[0])
+        t1 = time.perf_counter()
+        O.mesh_collide_scene(sc, [om], pairs[:k] * np.array([1, 0], np.uint32), poses[:k])
+        cs = time.perf_counter() - t1
+        out["cpu_pairs_per_s"] = round(k / cs, 1)
+        out["cpu_sample"] = f"oracle convex_mesh, first {k} pairs, 1 thread"
+    except Exception as e:  # the oracle is optional on a box without it built
+        out["cpu_pairs_per_s"] = None
+        out["cpu_error"] = str(e)[:120]
+    ctx.close()
+    return out
+
+
 def batched_vs_plain(bpa, n_worlds, margin, steps):
     """4096 worlds (16-2048 boxes/spheres + plane each, seed 3): one batched
     pass against the plain pass run world by world on the same GPU."""
@@ -360,6 +421,10 @@ def main():
         batched = batched_vs_plain(bpa, args.worlds, args.margin, args.steps)
         log(f"batched: {batched}")
 
+    mesh = None if args.only_pass else mesh_batch(bpa, args.steps, cpu=not args.no_cpu)
+    if mesh:
+        log(f"mesh batch: {mesh}")
+
     diff_out = None
     if not args.no_diff:
         diff_out = {"signed_distance": differentiation(bpa, args.steps), "depenetration": depenetration(bpa)}
@@ -391,6 +456,7 @@ def main():
         "host_buffers": pcie,
         "captured": graph,
         "manifold": manifold,
+        "mesh": mesh,
         "batched": batched,
         "differentiation": diff_out,
         "cpu_baseline": cpu,
