@@ -2139,4 +2139,364 @@ inline bool convex_mesh(const Shape<float>& convex, V3<float> pc, Quat<float> qc
   return true;
 }
 
+// ------------------------------------------------------------ CCD sweeps --
+// convex_sweep.rs (swept_sphere_cast :35-57, convex_cast :357-385) over the
+// simplex reduction closest_origin of gjk.rs:864-1043, f32.
+struct CoSimplex {
+  V3<float> p[4];
+  int n = 0;
+};
+
+// gjk.rs:887-906
+inline V3<float> co_segment(V3<float> a, V3<float> b, CoSimplex& k) {
+  V3<float> ab = b - a;
+  if (len2(ab) == 0.0f) {
+    k.p[0] = a;
+    k.n = 1;
+    return a;
+  }
+  float t = -dot(a, ab) / len2(ab);
+  if (t <= 0.0f) {
+    k.p[0] = a;
+    k.n = 1;
+    return a;
+  }
+  if (t >= 1.0f) {
+    k.p[0] = b;
+    k.n = 1;
+    return b;
+  }
+  k.p[0] = b;
+  k.p[1] = a;
+  k.n = 2;
+  return a + ab * t;
+}
+
+// gjk.rs:908-985
+inline V3<float> co_triangle(V3<float> a, V3<float> b, V3<float> c, CoSimplex& k) {
+  V3<float> ab = b - a, ac = c - a, ap = -a;
+  float d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0.0f && d2 <= 0.0f) { k.p[0] = a; k.n = 1; return a; }
+  V3<float> bp = -b;
+  float d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= 0.0f && d4 <= d3) { k.p[0] = b; k.n = 1; return b; }
+  float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+    if (d1 == d3) { k.p[0] = a; k.n = 1; return a; }
+    float v = d1 / (d1 - d3);
+    k.p[0] = b; k.p[1] = a; k.n = 2;
+    return a + ab * v;
+  }
+  V3<float> cp = -c;
+  float d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0.0f && d5 <= d6) { k.p[0] = c; k.n = 1; return c; }
+  float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+    if (d2 == d6) { k.p[0] = a; k.n = 1; return a; }
+    float w = d2 / (d2 - d6);
+    k.p[0] = c; k.p[1] = a; k.n = 2;
+    return a + ac * w;
+  }
+  float va = d3 * d6 - d5 * d4;
+  if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+    if ((d4 - d3) + (d5 - d6) == 0.0f) { k.p[0] = b; k.n = 1; return b; }
+    float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    k.p[0] = c; k.p[1] = b; k.n = 2;
+    return b + (c - b) * w;
+  }
+  float sum = (va + vb) + vc;
+  if (sum <= 0.0f || std::isnan(sum)) {
+    CoSimplex kab, kbc, kca;
+    V3<float> qab = co_segment(a, b, kab), qbc = co_segment(b, c, kbc), qca = co_segment(c, a, kca);
+    if (len2(qab) <= len2(qbc) && len2(qab) <= len2(qca)) { k = kab; return qab; }
+    if (len2(qbc) <= len2(qca)) { k = kbc; return qbc; }
+    k = kca;
+    return qca;
+  }
+  float denom = 1.0f / sum;
+  float v = vb * denom, w = vc * denom;
+  k.p[0] = c; k.p[1] = b; k.p[2] = a; k.n = 3;
+  return (a + ab * v) + ac * w;
+}
+
+// gjk.rs:987-1043
+inline V3<float> co_tetra(V3<float> a, V3<float> b, V3<float> c, V3<float> d, CoSimplex& k) {
+  V3<float> ao = -a, ab = b - a, ac = c - a, ad = d - a;
+  if (std::fabs(dot(cross(ab, ac), ad)) < 1e-12f) {
+    V3<float> f[4][3] = {{a, b, c}, {a, b, d}, {a, c, d}, {b, c, d}};
+    V3<float> best;
+    CoSimplex bk;
+    for (int i = 0; i < 4; ++i) {  // min_by keeps the first of equal minima
+      CoSimplex kk;
+      V3<float> q = co_triangle(f[i][0], f[i][1], f[i][2], kk);
+      if (i == 0 || len2(q) < len2(best)) { best = q; bk = kk; }
+    }
+    k = bk;
+    return best;
+  }
+  V3<float> abc = cross(ab, ac), acd = cross(ac, ad), adb = cross(ad, ab);
+  V3<float> bo = -b;
+  V3<float> bcd = cross(c - b, d - b);
+  if (dot(abc, ao) * dot(abc, ad) < 0.0f) return co_triangle(a, b, c, k);
+  if (dot(acd, ao) * dot(acd, ab) < 0.0f) return co_triangle(a, c, d, k);
+  if (dot(adb, ao) * dot(adb, ac) < 0.0f) return co_triangle(a, d, b, k);
+  if (dot(bcd, bo) * dot(bcd, a - b) < 0.0f) return co_triangle(b, c, d, k);
+  k.p[0] = a; k.p[1] = b; k.p[2] = c; k.p[3] = d; k.n = 4;
+  return splat(0.0f);
+}
+
+inline V3<float> closest_origin(CoSimplex& s) {
+  CoSimplex k;
+  V3<float> v;
+  switch (s.n) {
+    case 0: return splat(0.0f);
+    case 1: return s.p[0];
+    case 2: v = co_segment(s.p[0], s.p[1], k); break;
+    case 3: v = co_triangle(s.p[0], s.p[1], s.p[2], k); break;
+    default: v = co_tetra(s.p[0], s.p[1], s.p[2], s.p[3], k); break;
+  }
+  s = k;
+  return v;
+}
+
+// point_convex_distance (convex_sweep.rs:59-103): squared distance and the
+// unit direction from the shape toward `point`.
+template <class F>
+inline void point_convex_distance(const F& support, V3<float> point, float& d2, V3<float>& dir) {
+  auto mink = [&](V3<float> v) { return point - support(-v); };
+  V3<float> v = point;
+  if (len2(v) < 1e-12f) v = mk(1.0f, 0.0f, 0.0f);
+  CoSimplex s;
+  s.p[0] = mink(v);
+  s.n = 1;
+  v = s.p[0];
+  for (int it = 0; it < 64; ++it) {
+    if (len2(v) < 1e-10f) { d2 = 0.0f; dir = splat(0.0f); return; }
+    V3<float> w = mink(-v);
+    for (int i = 0; i < s.n; ++i)
+      if (len2(s.p[i] - w) < 1e-12f) { d2 = len2(v); dir = v / len(v); return; }
+    s.p[s.n++] = w;
+    v = closest_origin(s);
+  }
+  if (len2(v) < 1e-10f) { d2 = 0.0f; dir = splat(0.0f); return; }
+  d2 = len2(v);
+  dir = v / len(v);
+}
+
+struct SweepHit {
+  float fraction;
+  V3<float> normal, point;
+};
+
+// sweep_with_dist (convex_sweep.rs:150-206)
+template <class D>
+inline bool sweep_with_dist(V3<float> from, V3<float> dir, float length, float radius, const D& dist, SweepHit& h) {
+  auto dist2 = [&](V3<float> p) {
+    float d2;
+    V3<float> v;
+    dist(p, d2, v);
+    return d2;
+  };
+  float lo = 0.0f, hi = 1.0f;
+  for (int i = 0; i < 40; ++i) {
+    float m1 = lo + (hi - lo) / 3.0f;
+    float m2 = hi - (hi - lo) / 3.0f;
+    if (dist2(from + dir * (m1 * length)) < dist2(from + dir * (m2 * length))) hi = m2;
+    else lo = m1;
+  }
+  float s_min = (lo + hi) * 0.5f;
+  if (dist2(from + dir * (s_min * length)) > 1e-8f) return false;
+  float low = 0.0f, high = s_min;
+  for (int i = 0; i < 60; ++i) {
+    float mid = (low + high) * 0.5f;
+    if (dist2(from + dir * (mid * length)) <= 1e-8f) high = mid;
+    else low = mid;
+  }
+  float fraction = (low + high) * 0.5f;
+  V3<float> probe = from + dir * fmax_((fraction - 2e-4f) * length, 0.0f);
+  float d2;
+  V3<float> v;
+  dist(probe, d2, v);
+  V3<float> normal = len2(v) > 1e-12f ? v / len(v) : -dir;
+  V3<float> entry = from + dir * (fraction * length);
+  h = SweepHit{fraction, normal, entry - normal * radius};
+  return true;
+}
+
+// point_triangle_distance (convex_sweep.rs:208-256)
+inline float point_triangle_distance(V3<float> p, V3<float> a, V3<float> b, V3<float> c, V3<float>& dv) {
+  V3<float> ab = b - a, ac = c - a, ap = p - a;
+  float d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0.0f && d2 <= 0.0f) { dv = p - a; return len2(p - a); }
+  V3<float> bp = p - b;
+  float d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= 0.0f && d4 <= d3) { dv = p - b; return len2(p - b); }
+  float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+    float v = d1 / (d1 - d3);
+    V3<float> q = a + ab * v;
+    dv = p - q;
+    return len2(p - q);
+  }
+  V3<float> cp = p - c;
+  float d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0.0f && d5 <= d6) { dv = p - c; return len2(p - c); }
+  float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+    float w = d2 / (d2 - d6);
+    V3<float> q = a + ac * w;
+    dv = p - q;
+    return len2(p - q);
+  }
+  float va = d3 * d6 - d5 * d4;
+  if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+    float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    V3<float> q = b + (c - b) * w;
+    dv = p - q;
+    return len2(p - q);
+  }
+  float denom = 1.0f / ((va + vb) + vc);
+  float v = vb * denom, w = vc * denom;
+  V3<float> q = (a + ab * v) + ac * w;
+  dv = p - q;
+  return len2(p - q);
+}
+
+// swept_sphere_cast (convex_sweep.rs:35-57) against a convex or plane target.
+inline bool sweep_sphere(V3<float> from, V3<float> to, float radius, const Shape<float>& other, V3<float> op,
+                         Quat<float> oq, SweepHit& h) {
+  V3<float> delta = to - from;
+  float length = len(delta);
+  if (length <= std::numeric_limits<float>::epsilon()) return false;
+  V3<float> dir = delta / length;
+  if (other.kind == PLANE) {  // sweep_plane (convex_sweep.rs:258-292)
+    V3<float> n = qrot(oq, other.pnormal);
+    float dn0 = dot(n, from) - other.pconst;
+    if (std::fabs(dn0) <= radius) {
+      h = SweepHit{0.0f, dn0 >= 0.0f ? n : -n, from - n * dn0};
+      return true;
+    }
+    float dd = dot(n, dir * length);
+    if (dd >= 0.0f) return false;
+    float sign = dn0 > 0.0f ? 1.0f : -1.0f;
+    float fraction = (sign * radius - dn0) / dd;
+    if (!(fraction >= 0.0f && fraction <= 1.0f)) return false;
+    V3<float> normal = sign > 0.0f ? n : -n;
+    V3<float> entry = from + (dir * length) * fraction;
+    h = SweepHit{fraction, normal, entry - normal * radius};
+    return true;
+  }
+  // sweep_convex (convex_sweep.rs:105-125)
+  auto support = [&](V3<float> d) {
+    V3<float> sph = len2(d) > 1e-12f ? (d / len(d)) * radius : splat(0.0f);
+    return other.support_world(op, oq, d) + sph;
+  };
+  auto dist = [&](V3<float> p, float& d2, V3<float>& v) { point_convex_distance(support, p, d2, v); };
+  return sweep_with_dist(from, dir, length, radius, dist, h);
+}
+
+// sweep_mesh (convex_sweep.rs:294-344): per-triangle casts in mesh space, the
+// smallest fraction wins (the first on ties).
+inline bool sweep_sphere_mesh(V3<float> from, V3<float> to, float radius, const TriMesh& mesh, V3<float> pos,
+                              Quat<float> ori, SweepHit& out) {
+  Quat<float> inv = qconj(ori);
+  V3<float> lf = qrot(inv, from - pos), lt = qrot(inv, to - pos);
+  V3<float> delta = lt - lf;
+  float length = len(delta);
+  if (length <= std::numeric_limits<float>::epsilon()) return false;
+  V3<float> dir = delta / length;
+  V3<float> r = splat(radius);
+  Aabb<float> box{vmin(lf, lt) - r, vmax(lf, lt) + r};
+  std::vector<uint32_t> cand;
+  mesh.query(box, cand);
+  bool any = false;
+  SweepHit best{};
+  for (uint32_t t : cand) {
+    V3<float> a = mesh.vert(t, 0), b = mesh.vert(t, 1), c = mesh.vert(t, 2);
+    auto dist = [&](V3<float> p, float& d2, V3<float>& v) {
+      V3<float> dv;
+      float q2 = point_triangle_distance(p, a, b, c, dv);
+      if (q2 <= radius * radius) {
+        d2 = 0.0f;
+        v = splat(0.0f);
+      } else {
+        float d = std::sqrt(q2);
+        d2 = (d - radius) * (d - radius);
+        v = dv / d;
+      }
+    };
+    SweepHit hh;
+    if (sweep_with_dist(lf, dir, length, radius, dist, hh) && (!any || hh.fraction < best.fraction)) {
+      best = hh;
+      any = true;
+    }
+  }
+  if (!any) return false;
+  out = SweepHit{best.fraction, qrot(ori, best.normal), qrot(ori, best.point) + pos};
+  return true;
+}
+
+// convex_cast (convex_sweep.rs:357-385) of `shape` (orientation `ori`) against
+// a convex or plane target.
+inline bool convex_cast(const Shape<float>& shape, V3<float> from, Quat<float> ori, V3<float> to,
+                        const Shape<float>& other, V3<float> op, Quat<float> oq, SweepHit& h) {
+  V3<float> delta = to - from;
+  float length = len(delta);
+  if (length <= std::numeric_limits<float>::epsilon()) return false;
+  if (other.kind == PLANE) {  // sweep_convex_plane (convex_sweep.rs:431-478)
+    V3<float> n = qrot(oq, other.pnormal);
+    float c = other.pconst + dot(n, op);
+    V3<float> normal = dot(n, from) >= c ? n : -n;
+    auto deepest = [&](V3<float> p) { return shape.support_world(p, ori, -normal); };
+    float da = dot(normal, deepest(from)) - c;
+    float db = dot(normal, deepest(to)) - c;
+    if (db >= da) return false;
+    if (da <= 0.0f) {
+      h = SweepHit{0.0f, normal, deepest(from)};
+      return true;
+    }
+    float fraction = da / (da - db);
+    if (!(fraction >= 0.0f && fraction <= 1.0f)) return false;
+    V3<float> entry = from + delta * fraction;
+    h = SweepHit{fraction, normal, deepest(entry)};
+    return true;
+  }
+  // sweep_convex_shape (convex_sweep.rs:387-429)
+  V3<float> dir = delta / length;
+  auto dist = [&](V3<float> p, float& d2, V3<float>& v) {
+    auto support = [&](V3<float> d) { return other.support_world(op, oq, d) - shape.support_world(p, ori, -d); };
+    point_convex_distance(support, splat(0.0f), d2, v);
+  };
+  if (!sweep_with_dist(from, dir, length, 0.0f, dist, h)) return false;
+  h.point = other.support_world(op, oq, h.normal);
+  return true;
+}
+
+// sweep_convex_mesh (convex_sweep.rs:480-524): BVH-pruned per-triangle convex
+// casts in mesh space (no triangle-AABB filter), the smallest fraction wins.
+inline bool convex_cast_mesh(const Shape<float>& shape, V3<float> from, Quat<float> ori, V3<float> to,
+                             const TriMesh& mesh, V3<float> op, Quat<float> oq, SweepHit& out) {
+  Quat<float> inv = qconj(oq);
+  V3<float> fl = qrot(inv, from - op), tl = qrot(inv, to - op);
+  Quat<float> ol = qmul(inv, ori);
+  Aabb<float> a = shape.world_aabb(fl, ol), b = shape.world_aabb(tl, ol);
+  Aabb<float> box{vmin(a.min, b.min), vmax(a.max, b.max)};
+  std::vector<uint32_t> cand;
+  mesh.query(box, cand);
+  bool any = false;
+  SweepHit best{};
+  for (uint32_t t : cand) {
+    Shape<float> tri = Shape<float>::hull(&mesh.tri_pts[3 * (size_t)t], 3);
+    SweepHit hh;
+    if (convex_cast(shape, fl, ol, tl, tri, splat(0.0f), Quat<float>{0.0f, 0.0f, 0.0f, 1.0f}, hh) &&
+        (!any || hh.fraction < best.fraction)) {
+      best = hh;
+      any = true;
+    }
+  }
+  if (!any) return false;
+  out = SweepHit{best.fraction, qrot(oq, best.normal), qrot(oq, best.point) + op};
+  return true;
+}
+
 }  // namespace oracle

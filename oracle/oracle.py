@@ -91,6 +91,14 @@ def lib():
         L.oracle_sphere_triangle.argtypes = [C.c_void_p, C.c_float, C.c_float] + [C.c_void_p] * 4
         L.oracle_reduce.restype = C.c_int
         L.oracle_reduce.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.oracle_sweep_sphere.restype = C.c_int
+        L.oracle_sweep_sphere.argtypes = [C.c_void_p] * 5 + [C.c_float, C.c_void_p]
+        L.oracle_sweep_sphere_mesh.restype = C.c_int
+        L.oracle_sweep_sphere_mesh.argtypes = [C.c_void_p] * 5 + [C.c_float, C.c_void_p]
+        L.oracle_convex_cast.restype = C.c_int
+        L.oracle_convex_cast.argtypes = [C.c_void_p] * 8
+        L.oracle_sweep_scene.argtypes = [scp] + [C.c_void_p] * 5 + [C.c_int64, C.c_void_p, C.c_void_p]
+        L.oracle_sweep_mesh_scene.argtypes = [C.c_void_p] * 8 + [C.c_int64, C.c_void_p, C.c_void_p]
         L.oracle_mesh_create.restype = C.c_void_p
         L.oracle_mesh_create.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32]
         L.oracle_mesh_destroy.argtypes = [C.c_void_p]
@@ -561,3 +569,69 @@ def reduce(points7, max_points=4):
     p = np.ascontiguousarray(points7, np.float32).reshape(-1, 7).copy()
     n = lib().oracle_reduce(p.ctypes.data, p.shape[0], max_points)
     return p[:n]
+
+
+# ------------------------------------------------------------------ sweeps --
+def _hit(ok, out):
+    o = np.array(out[:], np.float32)
+    return (float(o[0]), o[1:4], o[4:7]) if ok else None
+
+
+def sweep_sphere(frm, to, radius, other: Shape, op=(0, 0, 0), oq=(0, 0, 0, 1)):
+    """swept_sphere_cast (convex_sweep.rs:35-57) -> (fraction, normal, point) or None."""
+    oc = other.c()
+    out = (C.c_float * 7)()
+    ok = lib().oracle_sweep_sphere(C.byref(oc), _f32(op, 3), _f32(oq, 4), _f32(frm, 3), _f32(to, 3), radius, out)
+    return _hit(ok, out)
+
+
+def sweep_sphere_mesh(frm, to, radius, mesh, pos=(0, 0, 0), ori=(0, 0, 0, 1)):
+    out = (C.c_float * 7)()
+    ok = lib().oracle_sweep_sphere_mesh(mesh.h, _f32(pos, 3), _f32(ori, 4), _f32(frm, 3), _f32(to, 3), radius, out)
+    return _hit(ok, out)
+
+
+def convex_cast(shape: Shape, frm, ori, to, other: Shape, op=(0, 0, 0), oq=(0, 0, 0, 1)):
+    """convex_cast (convex_sweep.rs:357-385) -> (fraction, normal, point) or None."""
+    a, o = shape.c(), other.c()
+    out = (C.c_float * 7)()
+    ok = lib().oracle_convex_cast(C.byref(a), _f32(frm, 3), _f32(ori, 4), _f32(to, 3), C.byref(o), _f32(op, 3),
+                                  _f32(oq, 4), out)
+    return _hit(ok, out)
+
+
+def sweep_scene(scene, target, frm, to, radius=None, caster=None):
+    """Batch over scene bodies -> (hit u8[n], out [n, 7] = fraction, normal, point)."""
+    r = _SceneRef(scene)
+    t = np.ascontiguousarray(target, np.uint32)
+    n = t.shape[0]
+    f = np.ascontiguousarray(frm, np.float32)
+    e = np.ascontiguousarray(to, np.float32)
+    rad = None if radius is None else np.ascontiguousarray(radius, np.float32)
+    cs = None if caster is None else np.ascontiguousarray(caster, np.uint32)
+    hit = np.zeros(n, np.uint8)
+    out = np.zeros((max(n, 1), 7), np.float32)
+    lib().oracle_sweep_scene(C.byref(r.c), t.ctypes.data, f.ctypes.data, e.ctypes.data,
+                             None if rad is None else rad.ctypes.data, None if cs is None else cs.ctypes.data, n,
+                             hit.ctypes.data, out.ctypes.data)
+    return hit, out[:n]
+
+
+def sweep_mesh_scene(scene, meshes, mesh_of, mesh_pose, frm, to, radius=None, caster=None):
+    """Sphere sweeps (radius) or convex casts (caster bodies of scene) against
+    meshes -> (hit u8[n], out [n, 7])."""
+    r = None if scene is None else _SceneRef(scene)
+    mo = np.ascontiguousarray(mesh_of, np.uint32)
+    n = mo.shape[0]
+    hs = (C.c_void_p * max(len(meshes), 1))(*[m.h for m in meshes])
+    mp = np.ascontiguousarray(mesh_pose, np.float32)
+    f = np.ascontiguousarray(frm, np.float32)
+    e = np.ascontiguousarray(to, np.float32)
+    rad = None if radius is None else np.ascontiguousarray(radius, np.float32)
+    cs = None if caster is None else np.ascontiguousarray(caster, np.uint32)
+    hit = np.zeros(n, np.uint8)
+    out = np.zeros((max(n, 1), 7), np.float32)
+    lib().oracle_sweep_mesh_scene(None if r is None else C.byref(r.c), hs, mo.ctypes.data, mp.ctypes.data,
+                                  f.ctypes.data, e.ctypes.data, None if rad is None else rad.ctypes.data,
+                                  None if cs is None else cs.ctypes.data, n, hit.ctypes.data, out.ctypes.data)
+    return hit, out[:n]

@@ -722,4 +722,91 @@ int oracle_reduce(float* pts7, int n, int max) {
   return (int)v.size();
 }
 
+// ---------------------------------------------------------------- sweeps --
+static int emit_hit(bool hit, const SweepHit& h, float* out7) {
+  if (!hit) {
+    for (int k = 0; k < 7; ++k) out7[k] = 0.0f;
+    return 0;
+  }
+  out7[0] = h.fraction;
+  out7[1] = h.normal.x; out7[2] = h.normal.y; out7[3] = h.normal.z;
+  out7[4] = h.point.x; out7[5] = h.point.y; out7[6] = h.point.z;
+  return 1;
+}
+
+// swept_sphere_cast against a convex / plane shape: out7 = fraction, normal, point.
+int oracle_sweep_sphere(const or_shape* other, const float* op, const float* oq, const float* from, const float* to,
+                        float radius, float* out7) {
+  std::vector<V3<float>> st;
+  Shape<float> o = from_desc<float>(other, st);
+  SweepHit h;
+  bool hit = sweep_sphere(v3p<float>(from), v3p<float>(to), radius, o, v3p<float>(op), q4p<float>(oq), h);
+  return emit_hit(hit, h, out7);
+}
+
+int oracle_sweep_sphere_mesh(const void* mesh, const float* pos, const float* ori, const float* from, const float* to,
+                             float radius, float* out7) {
+  SweepHit h;
+  bool hit = sweep_sphere_mesh(v3p<float>(from), v3p<float>(to), radius, *static_cast<const TriMesh*>(mesh),
+                               v3p<float>(pos), q4p<float>(ori), h);
+  return emit_hit(hit, h, out7);
+}
+
+int oracle_convex_cast(const or_shape* shape, const float* from, const float* ori, const float* to,
+                       const or_shape* other, const float* op, const float* oq, float* out7) {
+  std::vector<V3<float>> s1, s2;
+  Shape<float> a = from_desc<float>(shape, s1), o = from_desc<float>(other, s2);
+  SweepHit h;
+  bool hit = convex_cast(a, v3p<float>(from), q4p<float>(ori), v3p<float>(to), o, v3p<float>(op), q4p<float>(oq), h);
+  return emit_hit(hit, h, out7);
+}
+
+// Batches over scene bodies (targets at their scene poses).  Sphere sweeps:
+// query k casts radius[k] from from[3k] to to[3k] against body target[k].
+// Convex casts: body caster[k]'s shape at its scene orientation from from[3k]
+// to to[3k] against body target[k].  hit[k], out7[7k ..].
+void oracle_sweep_scene(const or_scene* sc, const uint32_t* target, const float* from, const float* to,
+                        const float* radius, const uint32_t* caster, int64_t n, uint8_t* hit, float* out7) {
+  SceneShapes<float> ss;
+  ss.build(sc);
+  for (int64_t k = 0; k < n; ++k) {
+    uint32_t t = target[k];
+    SweepHit h;
+    bool ok;
+    if (caster) {
+      uint32_t c = caster[k];
+      ok = convex_cast(ss.shapes[c], v3p<float>(from + 3 * k), ss.quat[c], v3p<float>(to + 3 * k), ss.shapes[t],
+                       ss.pos[t], ss.quat[t], h);
+    } else {
+      ok = sweep_sphere(v3p<float>(from + 3 * k), v3p<float>(to + 3 * k), radius[k], ss.shapes[t], ss.pos[t],
+                        ss.quat[t], h);
+    }
+    hit[k] = (uint8_t)emit_hit(ok, h, out7 + 7 * k);
+  }
+}
+
+// Sweeps against meshes over scene casters: query k casts (sphere radius[k]
+// when caster is NULL, else body caster[k]'s shape at its scene orientation)
+// from from[3k] to to[3k] against mesh meshes[mesh_of[k]] at mesh_pose[7k ..].
+void oracle_sweep_mesh_scene(const or_scene* sc, void* const* meshes, const uint32_t* mesh_of, const float* mesh_pose,
+                             const float* from, const float* to, const float* radius, const uint32_t* caster,
+                             int64_t n, uint8_t* hit, float* out7) {
+  SceneShapes<float> ss;
+  if (sc) ss.build(sc);
+  for (int64_t k = 0; k < n; ++k) {
+    const TriMesh& m = *static_cast<const TriMesh*>(meshes[mesh_of[k]]);
+    V3<float> mp = v3p<float>(mesh_pose + 7 * k);
+    Quat<float> mq = q4p<float>(mesh_pose + 7 * k + 3);
+    SweepHit h;
+    bool ok;
+    if (caster) {
+      uint32_t c = caster[k];
+      ok = convex_cast_mesh(ss.shapes[c], v3p<float>(from + 3 * k), ss.quat[c], v3p<float>(to + 3 * k), m, mp, mq, h);
+    } else {
+      ok = sweep_sphere_mesh(v3p<float>(from + 3 * k), v3p<float>(to + 3 * k), radius[k], m, mp, mq, h);
+    }
+    hit[k] = (uint8_t)emit_hit(ok, h, out7 + 7 * k);
+  }
+}
+
 }  // extern "C"

@@ -146,3 +146,96 @@ def test_reduce_leaves_small_manifolds_alone(oracle):
     """manifold.rs:702-707"""
     pts = np.array([[0, 0, 0, 0, 1, 0, 0.2], [1, 0, 0, 0, 1, 0, 0.1], [0, 1, 0, 0, 1, 0, 0.1]], np.float32)
     assert np.array_equal(oracle.reduce(pts, 4), pts)
+
+
+# ------------------------------------------------------------- CCD sweeps --
+# The reference's own sweep tests (crates/bullet-collision/src/narrowphase/
+# convex_sweep.rs:526-668) against the oracle.
+def test_sweep_head_on_box_hit(oracle):
+    """convex_sweep.rs:535-552"""
+    b = oracle.Shape(oracle.BOX, [1, 1, 1])
+    f, n, p = oracle.sweep_sphere([5, 0, 0], [0, 0, 0], 0.5, b)
+    assert abs(f - 3.5 / 5.0) < 1e-4 and abs(n[0] - 1.0) < 1e-3
+    assert abs(p[0] - 1.0) < 1e-3 and abs(p[1]) < 1e-3 and abs(p[2]) < 1e-3
+
+
+def test_sweep_offset_cast_misses_the_box(oracle):
+    """convex_sweep.rs:554-559"""
+    assert oracle.sweep_sphere([5, 3, 0], [-5, 3, 0], 0.5, oracle.Shape(oracle.BOX, [1, 1, 1])) is None
+
+
+def test_sweep_plane_cases(oracle):
+    """convex_sweep.rs:561-586: analytic entry, moving away, already touching."""
+    pl = oracle.Shape(oracle.PLANE, [0, 1, 0, 0])
+    f, n, p = oracle.sweep_sphere([0, 5, 0], [0, -5, 0], 0.5, pl)
+    assert abs(f - 0.45) < 1e-5 and abs(n[1] - 1.0) < 1e-5 and abs(p[1]) < 1e-5
+    assert oracle.sweep_sphere([0, 2, 0], [0, 5, 0], 0.5, pl) is None
+    f, n, _ = oracle.sweep_sphere([0, 0.3, 0], [0, -3, 0], 0.5, pl)
+    assert f == 0.0 and n[1] == 1.0
+
+
+def test_sweep_sphere_against_a_sloped_triangle(oracle):
+    """convex_sweep.rs:588-610"""
+    m = oracle.Mesh([[0, 0, 0], [2, 2, 0], [0, 0, 2]], [[0, 1, 2]])
+    f, n, _ = oracle.sweep_sphere_mesh([1, 3, 0.6], [1, -1, 0.6], 0.2, m)
+    assert abs(f - (3.0 - (1.0 + 0.4 * np.sqrt(0.5))) / 4.0) < 1e-3
+    assert n[1] > 0.6 and n[0] < -0.1
+
+
+def test_convex_cast_cases(oracle):
+    """convex_sweep.rs:616-650: box head on, box offset, capsule on a plane."""
+    mv = oracle.Shape(oracle.BOX, [0.5, 0.5, 0.5])
+    tg = oracle.Shape(oracle.BOX, [1, 1, 1])
+    f, n, p = oracle.convex_cast(mv, [5, 0, 0], [0, 0, 0, 1], [0, 0, 0], tg)
+    assert abs(f - 3.5 / 5.0) < 1e-4 and abs(n[0] - 1.0) < 1e-3 and abs(p[0] - 1.0) < 1e-3
+    assert oracle.convex_cast(mv, [5, 3, 0], [0, 0, 0, 1], [-5, 3, 0], tg) is None
+    cap = oracle.Shape(oracle.CAPSULE, [0.4, 0.8])
+    f, n, p = oracle.convex_cast(cap, [0, 3, 0], [0, 0, 0, 1], [0, -1, 0], oracle.Shape(oracle.PLANE, [0, 1, 0, 0]))
+    assert abs(f - (3.0 - 1.2) / 4.0) < 1e-4 and abs(n[1] - 1.0) < 1e-4 and abs(p[1]) < 1e-4
+
+
+def test_sweep_matches_a_dense_scan(oracle):
+    """Closed-form check of the search: for a segment that ends inside the
+    inflated box, the entry fraction is where the centre first comes within
+    the radius of the box, which a dense scan of the segment against the
+    exact box distance brackets; a segment that stays clear reports no hit.
+    (A segment that passes through and leaves again is not used: the
+    reference's ternary search settles at the far edge of the overlap and
+    can then report no hit, convex_sweep.rs:161-176, which the restatement
+    reproduces.  The hit normal is not checked here: it is the direction of
+    the GJK distance walk at a probe point, and the walk's tetrahedron step
+    keeps the first face whose side test fails rather than the nearest one,
+    gjk.rs:1024-1039, so near edges it can be far from the exact normal.)"""
+    rng = np.random.default_rng(4)
+    h = np.array([0.7, 0.4, 0.9])
+    b = oracle.Shape(oracle.BOX, list(h))
+    for k in range(60):
+        frm = rng.uniform(-4, 4, 3)
+        r = rng.uniform(0.1, 0.6)
+        to = rng.uniform(-h, h) if k % 3 else frm + rng.normal(0, 0.5, 3)
+        s = np.linspace(0, 1, 20001)
+        pts = frm[None, :] + (to - frm)[None, :] * s[:, None]
+        q = np.clip(pts, -h, h)
+        d = np.linalg.norm(pts - q, axis=1)
+        inside = d <= r
+        hit = oracle.sweep_sphere(frm, to, r, b)
+        if k % 3 == 0:
+            if d.min() > r + 1e-3:
+                assert hit is None
+            continue
+        assert inside[-1] and hit is not None
+        first = s[np.argmax(inside)]
+        assert abs(hit[0] - first) <= 1e-4
+
+
+def test_box_cast_against_a_mesh_slope(oracle):
+    """convex_sweep.rs:651-668"""
+    m = oracle.Mesh([[0, 0, 0], [2, 2, 0], [0, 0, 2]], [[0, 1, 2]])
+    sc = {"position": np.zeros((1, 3), np.float32), "orientation": np.array([[0, 0, 0, 1]], np.float32),
+          "shape_type": np.array([oracle.BOX], np.uint8), "shape_params": np.array([[0.4, 0.4, 0.4, 0]], np.float32),
+          "hull_first": np.zeros(1, np.uint32), "hull_count": np.zeros(1, np.uint32),
+          "hull_vertices": np.zeros((0, 3), np.float32)}
+    hit, out = oracle.sweep_mesh_scene(sc, [m], [0], [[0, 0, 0, 0, 0, 0, 1]], [[1, 3, 0.6]], [[1, -1, 0.6]],
+                                       caster=[0])
+    assert hit[0] == 1
+    assert out[0, 2] > 0.6 and 0.0 < out[0, 0] < 1.0
