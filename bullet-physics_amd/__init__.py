@@ -308,6 +308,61 @@ def collide_mesh_batch(scene: dict, pairs, mesh_poses, mesh_first=None, seeds=No
     return cnt[:P], pts[:P]
 
 
+def sweep_batch(scene: dict, target, frm, to, radius=None, caster=None, ctx: _Context | None = None):
+    """Sweeps against scene bodies (narrowphase/convex_sweep.rs): query k moves
+    from frm[k] to to[k] against body target[k]; a sphere of radius[k]
+    (swept_sphere_cast) or, with caster, body caster[k]'s shape at its
+    orientation (convex_cast).  Returns (hit u8[n], out f32[n, 7] = fraction,
+    normal, point)."""
+    ctx = ctx or _Context(0)
+    b = Bodies(scene)
+    t = np.ascontiguousarray(target, np.uint32).reshape(-1)
+    n = t.shape[0]
+    f = np.ascontiguousarray(frm, np.float32).reshape(-1, 3)
+    e = np.ascontiguousarray(to, np.float32).reshape(-1, 3)
+    if f.shape[0] != n or e.shape[0] != n:
+        raise ValueError("from/to need one point per query")
+    rad = None if radius is None else np.ascontiguousarray(radius, np.float32).reshape(-1)
+    cs = None if caster is None else np.ascontiguousarray(caster, np.uint32).reshape(-1)
+    if (rad if cs is None else cs) is None or (rad if cs is None else cs).shape[0] != n:
+        raise ValueError("need one radius (sphere sweeps) or one caster (convex casts) per query")
+    hit = np.zeros(max(n, 1), np.uint8)
+    out = np.zeros((max(n, 1), 7), np.float32)
+    _check(ctx.h, ctx.L.bch_sweep_batch(ctx.h, C.byref(b.c), t.ctypes.data, None if cs is None else cs.ctypes.data,
+                                        f.ctypes.data, e.ctypes.data, None if rad is None else rad.ctypes.data, n,
+                                        hit.ctypes.data, out.ctypes.data))
+    return hit[:n], out[:n]
+
+
+def sweep_mesh_batch(mesh_of, mesh_poses, frm, to, radius=None, caster=None, scene: dict | None = None,
+                     ctx: _Context | None = None):
+    """The mesh arms of the sweeps (sweep_mesh / sweep_convex_mesh): query k
+    against TriangleMesh id mesh_of[k] at mesh_poses[k]; a sphere of radius[k]
+    or body caster[k] of `scene`.  Returns (hit u8[n], out f32[n, 7])."""
+    ctx = ctx or _Context(0)
+    mo = np.ascontiguousarray(mesh_of, np.uint32).reshape(-1)
+    n = mo.shape[0]
+    mp = np.ascontiguousarray(mesh_poses, np.float32).reshape(-1, 7)
+    f = np.ascontiguousarray(frm, np.float32).reshape(-1, 3)
+    e = np.ascontiguousarray(to, np.float32).reshape(-1, 3)
+    if mp.shape[0] != n or f.shape[0] != n or e.shape[0] != n:
+        raise ValueError("mesh poses and from/to need one row per query")
+    rad = None if radius is None else np.ascontiguousarray(radius, np.float32).reshape(-1)
+    cs = None if caster is None else np.ascontiguousarray(caster, np.uint32).reshape(-1)
+    if (rad if cs is None else cs) is None or (rad if cs is None else cs).shape[0] != n:
+        raise ValueError("need one radius (sphere sweeps) or one caster (convex casts) per query")
+    if cs is not None and scene is None:
+        raise ValueError("convex casts need the scene of the caster bodies")
+    b = None if scene is None else Bodies(scene)
+    hit = np.zeros(max(n, 1), np.uint8)
+    out = np.zeros((max(n, 1), 7), np.float32)
+    _check(ctx.h, ctx.L.bch_sweep_mesh_batch(ctx.h, None if b is None else C.byref(b.c), mo.ctypes.data,
+                                             mp.ctypes.data, None if cs is None else cs.ctypes.data, f.ctypes.data,
+                                             e.ctypes.data, None if rad is None else rad.ctypes.data, n,
+                                             hit.ctypes.data, out.ctypes.data))
+    return hit[:n], out[:n]
+
+
 class CollisionWorld:
     """Bodies resident in HBM; one detection pass per :meth:`collide`."""
 

@@ -27,7 +27,7 @@ EXPORTS = [
     "bch_world_set_body_flags", "bch_world_manifold_pass", "bch_world_manifold_results",
     "bch_world_manifold_impulses", "bch_world_manifold_cache", "bch_world_manifold_reset",
     "bch_world_set_poses_predicted", "bch_world_set_poses_predicted_device",
-    "bch_mesh_add", "bch_mesh_clear", "bch_collide_mesh_batch",
+    "bch_mesh_add", "bch_mesh_clear", "bch_collide_mesh_batch", "bch_sweep_batch", "bch_sweep_mesh_batch",
 ]
 
 # launch-shape knobs (bch_set_tuning)
@@ -123,6 +123,8 @@ def load():
     L.bch_mesh_add.argtypes = [vp, vp, u32, vp, u32, C.POINTER(u32)]
     L.bch_mesh_clear.argtypes = [vp]
     L.bch_collide_mesh_batch.argtypes = [vp, C.POINTER(BchBodies), vp, vp, vp, u64, vp, vp, vp]
+    L.bch_sweep_batch.argtypes = [vp, C.POINTER(BchBodies), vp, vp, vp, vp, vp, u64, vp, vp]
+    L.bch_sweep_mesh_batch.argtypes = [vp, C.POINTER(BchBodies), vp, vp, vp, vp, vp, vp, u64, vp, vp]
     L.bch_world_set_poses_predicted_device.argtypes = [vp, vp, vp, vp, f32]
     L.bch_depen_release.argtypes = [vp]
     L.bch_depen_history.argtypes = [vp, vp, vp, vp, vp, u32, C.POINTER(u32), C.c_int]

@@ -18,6 +18,7 @@
 #include "bch_depen.hpp"
 #include "bch_manifold.hpp"
 #include "bch_mesh.hpp"
+#include "bch_sweep.hpp"
 
 using namespace bch;
 
@@ -118,6 +119,7 @@ struct bch_context {
   DevBuf d_mesh_recs, d_mesh_nodes, d_mesh_tris, d_mesh_tri_aabb, d_mesh_order;
   DevBuf mx_pairs, mx_poses, mx_seeds, mx_first, mx_count, mx_offs, mx_sub, mx_sub_cnt, mx_sub_slots, mx_epa_q,
       mx_out_count, mx_out_pts;
+  DevBuf sw_target, sw_caster, sw_from, sw_to, sw_radius, sw_pose, sw_hit, sw_out;
 
   // proxy API state (host mirror of the broadphase slots)
   std::vector<float> proxy_box;  // 6 per slot
@@ -870,7 +872,8 @@ void bch_destroy(bch_context* c) {
                    &c->vel, &c->d_mesh_recs, &c->d_mesh_nodes, &c->d_mesh_tris, &c->d_mesh_tri_aabb,
                    &c->d_mesh_order, &c->mx_pairs, &c->mx_poses, &c->mx_seeds, &c->mx_first, &c->mx_count,
                    &c->mx_offs, &c->mx_sub, &c->mx_sub_cnt, &c->mx_sub_slots, &c->mx_epa_q, &c->mx_out_count,
-                   &c->mx_out_pts};
+                   &c->mx_out_pts, &c->sw_target, &c->sw_caster, &c->sw_from, &c->sw_to, &c->sw_radius,
+                   &c->sw_pose, &c->sw_hit, &c->sw_out};
   for (DevBuf* b : all) b->release();
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->own_stream);
@@ -1594,6 +1597,34 @@ int bch_mesh_clear(bch_context* c) {
   return BCH_OK;
 }
 
+// Upload the host-side mesh tables when a mesh was added or cleared.
+static int sync_meshes(bch_context* c, hipStream_t s) {
+  if (!c->mesh_dirty) return BCH_OK;
+  size_t nm = std::max<size_t>(c->mesh_recs.size(), 1);
+  ENSURE(c, c->d_mesh_recs, nm * sizeof(MeshRec));
+  ENSURE(c, c->d_mesh_nodes, std::max<size_t>(c->mesh_nodes.size(), 1) * sizeof(float4));
+  ENSURE(c, c->d_mesh_tris, std::max<size_t>(c->mesh_tris.size(), 1) * sizeof(float4));
+  ENSURE(c, c->d_mesh_tri_aabb, std::max<size_t>(c->mesh_tri_aabb.size(), 1) * sizeof(float4));
+  ENSURE(c, c->d_mesh_order, std::max<size_t>(c->mesh_order.size(), 1) * sizeof(uint32_t));
+  if (!c->mesh_recs.empty())
+    HIP_TRY(c, hipMemcpyAsync(c->d_mesh_recs.p, c->mesh_recs.data(), c->mesh_recs.size() * sizeof(MeshRec),
+                              hipMemcpyHostToDevice, s));
+  if (!c->mesh_nodes.empty())
+    HIP_TRY(c, hipMemcpyAsync(c->d_mesh_nodes.p, c->mesh_nodes.data(), c->mesh_nodes.size() * sizeof(float4),
+                              hipMemcpyHostToDevice, s));
+  if (!c->mesh_tris.empty()) {
+    HIP_TRY(c, hipMemcpyAsync(c->d_mesh_tris.p, c->mesh_tris.data(), c->mesh_tris.size() * sizeof(float4),
+                              hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(c->d_mesh_tri_aabb.p, c->mesh_tri_aabb.data(),
+                              c->mesh_tri_aabb.size() * sizeof(float4), hipMemcpyHostToDevice, s));
+  }
+  if (!c->mesh_order.empty())
+    HIP_TRY(c, hipMemcpyAsync(c->d_mesh_order.p, c->mesh_order.data(), c->mesh_order.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, s));
+  c->mesh_dirty = false;
+  return BCH_OK;
+}
+
 int bch_collide_mesh_batch(bch_context* c, const bch_bodies* bodies, const uint32_t* pairs, const float* mesh_poses,
                            const uint8_t* mesh_first, uint64_t n_pairs, const float* seeds, uint8_t* out_count,
                            float* out_points) {
@@ -1612,30 +1643,8 @@ int bch_collide_mesh_batch(bch_context* c, const bch_bodies* bodies, const uint3
   hipStream_t s = c->stream;
   rc = upload_bodies(c, bodies, nullptr);
   if (rc) return rc;
-  if (c->mesh_dirty) {
-    size_t nm = std::max<size_t>(c->mesh_recs.size(), 1);
-    ENSURE(c, c->d_mesh_recs, nm * sizeof(MeshRec));
-    ENSURE(c, c->d_mesh_nodes, std::max<size_t>(c->mesh_nodes.size(), 1) * sizeof(float4));
-    ENSURE(c, c->d_mesh_tris, std::max<size_t>(c->mesh_tris.size(), 1) * sizeof(float4));
-    ENSURE(c, c->d_mesh_tri_aabb, std::max<size_t>(c->mesh_tri_aabb.size(), 1) * sizeof(float4));
-    ENSURE(c, c->d_mesh_order, std::max<size_t>(c->mesh_order.size(), 1) * sizeof(uint32_t));
-    if (!c->mesh_recs.empty())
-      HIP_TRY(c, hipMemcpyAsync(c->d_mesh_recs.p, c->mesh_recs.data(), c->mesh_recs.size() * sizeof(MeshRec),
-                                hipMemcpyHostToDevice, s));
-    if (!c->mesh_nodes.empty())
-      HIP_TRY(c, hipMemcpyAsync(c->d_mesh_nodes.p, c->mesh_nodes.data(), c->mesh_nodes.size() * sizeof(float4),
-                                hipMemcpyHostToDevice, s));
-    if (!c->mesh_tris.empty()) {
-      HIP_TRY(c, hipMemcpyAsync(c->d_mesh_tris.p, c->mesh_tris.data(), c->mesh_tris.size() * sizeof(float4),
-                                hipMemcpyHostToDevice, s));
-      HIP_TRY(c, hipMemcpyAsync(c->d_mesh_tri_aabb.p, c->mesh_tri_aabb.data(),
-                                c->mesh_tri_aabb.size() * sizeof(float4), hipMemcpyHostToDevice, s));
-    }
-    if (!c->mesh_order.empty())
-      HIP_TRY(c, hipMemcpyAsync(c->d_mesh_order.p, c->mesh_order.data(), c->mesh_order.size() * sizeof(uint32_t),
-                                hipMemcpyHostToDevice, s));
-    c->mesh_dirty = false;
-  }
+  rc = sync_meshes(c, s);
+  if (rc) return rc;
   if (!n_pairs) return BCH_OK;
   uint64_t P = n_pairs;
   ENSURE(c, c->counters, sizeof(Counters));
@@ -1695,6 +1704,117 @@ int bch_collide_mesh_batch(bch_context* c, const bch_bodies* bodies, const uint3
   if (c->host_cnt.errors & 32u) return fail(c, BCH_HIP_ERROR, "mesh BVH deeper than the traversal stack");
   if (c->host_cnt.errors & 2u) return fail(c, BCH_HIP_ERROR, "EPA polytope store overflow");
   return BCH_OK;
+}
+
+// --------------------------------------------------------- CCD sweeps --
+// Shared checks and uploads of the sweep batches: query arrays to the device.
+static int sweep_upload(bch_context* c, uint64_t n, const uint32_t* idx, const uint32_t* caster, const float* from,
+                        const float* to, const float* radius, const float* pose, hipStream_t s) {
+  ENSURE(c, c->counters, sizeof(Counters));
+  ENSURE(c, c->sw_target, n * sizeof(uint32_t));
+  ENSURE(c, c->sw_from, n * 3 * sizeof(float));
+  ENSURE(c, c->sw_to, n * 3 * sizeof(float));
+  ENSURE(c, c->sw_hit, n);
+  ENSURE(c, c->sw_out, n * 7 * sizeof(float));
+  HIP_TRY(c, hipMemcpyAsync(c->sw_target.p, idx, n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(c->sw_from.p, from, n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipMemcpyAsync(c->sw_to.p, to, n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+  if (caster) {
+    ENSURE(c, c->sw_caster, n * sizeof(uint32_t));
+    HIP_TRY(c, hipMemcpyAsync(c->sw_caster.p, caster, n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  } else {
+    ENSURE(c, c->sw_radius, n * sizeof(float));
+    HIP_TRY(c, hipMemcpyAsync(c->sw_radius.p, radius, n * sizeof(float), hipMemcpyHostToDevice, s));
+  }
+  if (pose) {
+    ENSURE(c, c->sw_pose, n * 7 * sizeof(float));
+    HIP_TRY(c, hipMemcpyAsync(c->sw_pose.p, pose, n * 7 * sizeof(float), hipMemcpyHostToDevice, s));
+  }
+  Counters* cnt = c->counters.as<Counters>();
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, cnt);
+  return BCH_OK;
+}
+
+static int sweep_download(bch_context* c, uint64_t n, uint8_t* out_hit, float* out_sweep, hipStream_t s) {
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipMemcpyAsync(out_hit, c->sw_hit.p, n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(out_sweep, c->sw_out.p, n * 7 * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  if (c->host_cnt.errors & 32u) return fail(c, BCH_HIP_ERROR, "mesh BVH deeper than the traversal stack");
+  return BCH_OK;
+}
+
+int bch_sweep_batch(bch_context* c, const bch_bodies* bodies, const uint32_t* target, const uint32_t* caster,
+                    const float* from, const float* to, const float* radius, uint64_t n, uint8_t* out_hit,
+                    float* out_sweep) {
+  if (!c) return BCH_INVALID_ARG;
+  if (n && (!target || !from || !to || !out_hit || !out_sweep || (!caster && !radius)))
+    return fail(c, BCH_INVALID_ARG, "NULL argument");
+  if (n > 0x7fffffffull) return fail(c, BCH_INVALID_ARG, "too many queries");
+  int rc = validate_bodies(c, bodies);
+  if (rc) return rc;
+  for (uint64_t k = 0; k < n; ++k) {
+    if (target[k] >= bodies->n) return fail(c, BCH_INVALID_ARG, "target index out of range");
+    if (caster) {
+      if (caster[k] >= bodies->n) return fail(c, BCH_INVALID_ARG, "caster index out of range");
+      if (bodies->shape_type[caster[k]] == BCH_PLANE) return fail(c, BCH_INVALID_ARG, "a cast shape must be convex");
+    }
+  }
+  if (!n) return BCH_OK;
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  rc = upload_bodies(c, bodies, nullptr);
+  if (rc) return rc;
+  rc = sweep_upload(c, n, target, caster, from, to, radius, nullptr, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_sweep, dim3(blocks(n)), dim3(kBlock), 0, s, (uint32_t)n, c->sw_target.as<uint32_t>(),
+                     caster ? (const uint32_t*)c->sw_caster.as<uint32_t>() : nullptr, c->sw_from.as<float>(),
+                     c->sw_to.as<float>(), caster ? nullptr : (const float*)c->sw_radius.as<float>(),
+                     c->bodies.as<BodyRec>(), c->hull.as<float4>(), c->sw_hit.as<uint8_t>(), c->sw_out.as<float>());
+  return sweep_download(c, n, out_hit, out_sweep, s);
+}
+
+int bch_sweep_mesh_batch(bch_context* c, const bch_bodies* bodies, const uint32_t* mesh_of, const float* mesh_poses,
+                         const uint32_t* caster, const float* from, const float* to, const float* radius, uint64_t n,
+                         uint8_t* out_hit, float* out_sweep) {
+  if (!c) return BCH_INVALID_ARG;
+  if (n && (!mesh_of || !mesh_poses || !from || !to || !out_hit || !out_sweep || (!caster && !radius)))
+    return fail(c, BCH_INVALID_ARG, "NULL argument");
+  if (caster && !bodies) return fail(c, BCH_INVALID_ARG, "convex casts need the bodies");
+  if (n > 0x7fffffffull) return fail(c, BCH_INVALID_ARG, "too many queries");
+  int rc;
+  if (caster) {
+    rc = validate_bodies(c, bodies);
+    if (rc) return rc;
+  }
+  for (uint64_t k = 0; k < n; ++k) {
+    if (mesh_of[k] >= c->mesh_recs.size()) return fail(c, BCH_INVALID_ARG, "mesh index out of range");
+    if (caster) {
+      if (caster[k] >= bodies->n) return fail(c, BCH_INVALID_ARG, "caster index out of range");
+      if (bodies->shape_type[caster[k]] == BCH_PLANE) return fail(c, BCH_INVALID_ARG, "a cast shape must be convex");
+    }
+  }
+  if (!n) return BCH_OK;
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  if (caster) {
+    rc = upload_bodies(c, bodies, nullptr);
+    if (rc) return rc;
+  }
+  rc = sync_meshes(c, s);
+  if (rc) return rc;
+  rc = sweep_upload(c, n, mesh_of, caster, from, to, radius, mesh_poses, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_sweep_mesh, dim3(blocks(n)), dim3(kBlock), 0, s, (uint32_t)n, c->sw_target.as<uint32_t>(),
+                     (const float*)c->sw_pose.as<float>(), caster ? (const uint32_t*)c->sw_caster.as<uint32_t>() : nullptr,
+                     c->sw_from.as<float>(), c->sw_to.as<float>(),
+                     caster ? nullptr : (const float*)c->sw_radius.as<float>(),
+                     caster ? (const BodyRec*)c->bodies.as<BodyRec>() : nullptr,
+                     caster ? (const float4*)c->hull.as<float4>() : nullptr, c->d_mesh_recs.as<MeshRec>(),
+                     c->d_mesh_nodes.as<float4>(), c->d_mesh_order.as<uint32_t>(), c->d_mesh_tris.as<float4>(),
+                     c->sw_hit.as<uint8_t>(), c->sw_out.as<float>(), c->counters.as<Counters>());
+  return sweep_download(c, n, out_hit, out_sweep, s);
 }
 
 // ---------------------------------------------- captured de-penetration --

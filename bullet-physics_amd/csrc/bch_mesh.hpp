@@ -52,8 +52,8 @@ __device__ __forceinline__ Box6 mesh_query_box(const BodyRec& B, const float4* h
 }
 
 // triangles_in_aabb (bvh.rs:58-79) filtered by the triangle's own AABB
-// (convex_mesh.rs:96-104): f(t) for each candidate, in the reference's order.
-// Returns false when the stack would overflow (the walk stops).
+// (convex_mesh.rs:96-104) unless tri_aabb is null: f(t) for each candidate,
+// in the reference's order. Returns false when the stack would overflow (the walk stops).
 template <class F>
 __device__ __forceinline__ bool mesh_walk(const MeshRec& M, const float4* __restrict__ nodes,
                                           const uint32_t* __restrict__ order, const float4* __restrict__ tri_aabb,
@@ -71,6 +71,10 @@ __device__ __forceinline__ bool mesh_walk(const MeshRec& M, const float4* __rest
       uint32_t cnt = b & ~kMeshLeaf;
       for (uint32_t i = a; i < a + cnt; ++i) {
         uint32_t t = order[M.order_off + i];
+        if (!tri_aabb) {  // the sweeps walk without the triangle filter
+          f(t);
+          continue;
+        }
         float4 tl = tri_aabb[2 * (uint64_t)(M.tri_off + t)], th = tri_aabb[2 * (uint64_t)(M.tri_off + t) + 1];
         Box6 tb{V3{tl.x, tl.y, tl.z}, V3{th.x, th.y, th.z}};
         if (overlaps(tb, q)) f(t);

@@ -156,6 +156,31 @@ int bch_collide_mesh_batch(bch_context* ctx, const bch_bodies* bodies, const uin
                            const uint8_t* mesh_first, uint64_t n_pairs, const float* seeds, uint8_t* out_count,
                            float* out_points);
 
+/* ------------------------------------------------------ CCD sweeps -- */
+/* Sweeps (narrowphase/convex_sweep.rs) for a batch of queries, one per call
+ * of the reference's sweep functions: query k moves from from[3k ..] to
+ * to[3k ..] against body target[k] of `bodies` at its pose.  caster == NULL:
+ * swept_sphere_cast (convex_sweep.rs:37-57) of a sphere of radius[k], as
+ * the CCD pass of DiscreteDynamicsWorld::step calls it
+ * (bullet-dynamics/src/world/step.rs:1141-1148).  Else convex_cast
+ * (convex_sweep.rs:370-385) of body caster[k]'s shape at its orientation, as
+ * convex_sweep_closest_filtered calls it (bullet-dynamics/src/world/
+ * query.rs:210-218); cast shapes must be convex.  Targets may be any convex
+ * shape or a plane.  out_hit[k] = 1 for Some(hit); out_sweep[7k ..] =
+ * fraction, normal (from the target toward the caster), point on the
+ * target (zeros for None).  Host pointers. */
+int bch_sweep_batch(bch_context* ctx, const bch_bodies* bodies, const uint32_t* target, const uint32_t* caster,
+                    const float* from, const float* to, const float* radius, uint64_t n, uint8_t* out_hit,
+                    float* out_sweep);
+/* The mesh arms of the same functions (sweep_mesh, convex_sweep.rs:294-344;
+ * sweep_convex_mesh, :480-524): query k against mesh mesh_of[k] at
+ * mesh_poses[7k ..] (position xyz, orientation xyzw); the earliest triangle
+ * hit (the first in BVH order on equal fractions).  bodies may be NULL when
+ * caster is NULL.  Outputs as in bch_sweep_batch. */
+int bch_sweep_mesh_batch(bch_context* ctx, const bch_bodies* bodies, const uint32_t* mesh_of, const float* mesh_poses,
+                         const uint32_t* caster, const float* from, const float* to, const float* radius, uint64_t n,
+                         uint8_t* out_hit, float* out_sweep);
+
 /* ---------------------------------------------- world-resident pipeline -- */
 /* The per-step detection pass of PhysicsWorld (world/step.rs:349-416 +
  * 445-854 for rigid pairs): world AABBs padded by `aabb_margin`, the sorted
