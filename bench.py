@@ -203,6 +203,59 @@ def mesh_batch(bpa, steps, n_bodies=100_000, grid=362, cpu_sample=2000, cpu=True
     return out
 
 
+def sweeps(bpa, steps, n_bodies=100_000, n_sphere=262_144, n_cast=65_536, cpu_sample=1000, cpu=True):
+    """CCD sweeps (bch_sweep_batch): `n_sphere` sphere sweeps (radius
+    0.05-0.5 m, the CCD pass of step.rs:1141-1148) and `n_cast` convex casts
+    of scene bodies (query.rs:210-218), each from 1-3 m away toward a random
+    target of a mixed convex scene of `n_bodies` bodies.  Host buffers in and
+    out, so the time includes the transfers.  The oracle on `cpu_sample`
+    queries of each kind gives the CPU rate."""
+    rng = np.random.default_rng(21)
+    sc = bpa.scenes.mixed_convex(n_bodies, 23)
+    ctx = bpa._Context(0)
+
+    def queries(n):
+        t = rng.integers(0, n_bodies, n).astype(np.uint32)
+        c = sc["position"][t].astype(np.float64)
+        d = rng.normal(size=(n, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        frm = (c + d * rng.uniform(1.0, 3.0, (n, 1))).astype(np.float32)
+        to = (c + rng.normal(0, 0.3, (n, 3))).astype(np.float32)
+        return t, frm, to
+
+    st, sf, se = queries(n_sphere)
+    rad = rng.uniform(0.05, 0.5, n_sphere).astype(np.float32)
+    ct, cf, ce = queries(n_cast)
+    caster = rng.integers(0, n_bodies, n_cast).astype(np.uint32)
+    out = {"workload": f"{n_sphere} sphere sweeps + {n_cast} convex casts against a {n_bodies}-body mixed scene",
+           "includes": "bodies and queries uploaded, hits copied back"}
+    for tag, kw, (t, f, e), n in (("sphere", {"radius": rad}, (st, sf, se), n_sphere),
+                                  ("cast", {"caster": caster}, (ct, cf, ce), n_cast)):
+        hit, _ = bpa.sweep_batch(sc, t, f, e, ctx=ctx, **kw)
+        ts = []
+        for _ in range(max(steps, 1)):
+            t1 = time.perf_counter()
+            bpa.sweep_batch(sc, t, f, e, ctx=ctx, **kw)
+            ts.append(time.perf_counter() - t1)
+        ms = float(np.median(ts)) * 1e3
+        out[tag] = {"ms_per_batch": round(ms, 3), "queries_per_s": round(n / (ms * 1e-3), 1), "hits": int(hit.sum())}
+        if cpu:
+            try:
+                sys.path.insert(0, os.path.join(ROOT, "oracle"))
+                import oracle as O
+                k = min(cpu_sample, n)
+                kk = {key: v[:k] for key, v in kw.items()}
+                t1 = time.perf_counter()
+                O.sweep_scene(sc, t[:k], f[:k], e[:k], **kk)
+                out[tag]["cpu_queries_per_s"] = round(k / (time.perf_counter() - t1), 1)
+                out[tag]["cpu_sample"] = f"oracle, first {k} queries, 1 thread"
+            except Exception as ex:  # the oracle is optional on a box without it built
+                out[tag]["cpu_queries_per_s"] = None
+                out[tag]["cpu_error"] = str(ex)[:120]
+    ctx.close()
+    return out
+
+
 def batched_vs_plain(bpa, n_worlds, margin, steps):
     """4096 worlds (16-2048 boxes/spheres + plane each, seed 3): one batched
     pass against the plain pass run world by world on the same GPU."""
@@ -424,6 +477,9 @@ def main():
     mesh = None if args.only_pass else mesh_batch(bpa, args.steps, cpu=not args.no_cpu)
     if mesh:
         log(f"mesh batch: {mesh}")
+    sweep = None if args.only_pass else sweeps(bpa, args.steps, cpu=not args.no_cpu)
+    if sweep:
+        log(f"sweeps: {sweep}")
 
     diff_out = None
     if not args.no_diff:
@@ -457,6 +513,7 @@ def main():
         "captured": graph,
         "manifold": manifold,
         "mesh": mesh,
+        "sweeps": sweep,
         "batched": batched,
         "differentiation": diff_out,
         "cpu_baseline": cpu,

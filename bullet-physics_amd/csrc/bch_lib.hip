@@ -119,7 +119,7 @@ struct bch_context {
   DevBuf d_mesh_recs, d_mesh_nodes, d_mesh_tris, d_mesh_tri_aabb, d_mesh_order;
   DevBuf mx_pairs, mx_poses, mx_seeds, mx_first, mx_count, mx_offs, mx_sub, mx_sub_cnt, mx_sub_slots, mx_epa_q,
       mx_out_count, mx_out_pts;
-  DevBuf sw_target, sw_caster, sw_from, sw_to, sw_radius, sw_pose, sw_hit, sw_out;
+  DevBuf sw_target, sw_caster, sw_from, sw_to, sw_radius, sw_pose, sw_hit, sw_out, sw_keys, sw_order;
 
   // proxy API state (host mirror of the broadphase slots)
   std::vector<float> proxy_box;  // 6 per slot
@@ -873,7 +873,7 @@ void bch_destroy(bch_context* c) {
                    &c->d_mesh_order, &c->mx_pairs, &c->mx_poses, &c->mx_seeds, &c->mx_first, &c->mx_count,
                    &c->mx_offs, &c->mx_sub, &c->mx_sub_cnt, &c->mx_sub_slots, &c->mx_epa_q, &c->mx_out_count,
                    &c->mx_out_pts, &c->sw_target, &c->sw_caster, &c->sw_from, &c->sw_to, &c->sw_radius,
-                   &c->sw_pose, &c->sw_hit, &c->sw_out};
+                   &c->sw_pose, &c->sw_hit, &c->sw_out, &c->sw_keys, &c->sw_order};
   for (DevBuf* b : all) b->release();
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->own_stream);
@@ -1768,7 +1768,20 @@ int bch_sweep_batch(bch_context* c, const bch_bodies* bodies, const uint32_t* ta
   if (rc) return rc;
   rc = sweep_upload(c, n, target, caster, from, to, radius, nullptr, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_sweep, dim3(blocks(n)), dim3(kBlock), 0, s, (uint32_t)n, c->sw_target.as<uint32_t>(),
+  // group the queries by shape class (one 8-bit radix pass, stable)
+  ENSURE(c, c->sw_keys, n * 2 * sizeof(uint32_t));
+  ENSURE(c, c->sw_order, n * 2 * sizeof(uint32_t));
+  uint32_t* keys = c->sw_keys.as<uint32_t>();
+  uint32_t* ord = c->sw_order.as<uint32_t>();
+  hipLaunchKernelGGL(k_sweep_keys, dim3(blocks(n)), dim3(kBlock), 0, s, (uint32_t)n, c->sw_target.as<uint32_t>(),
+                     caster ? (const uint32_t*)c->sw_caster.as<uint32_t>() : nullptr, c->bodies.as<BodyRec>(), keys,
+                     ord);
+  const int shift0 = 0;
+  bool second = false;
+  rc = dev_radix_sort<uint32_t>(c, keys, keys + n, ord, ord + n, n, &shift0, 1, s, &second);
+  if (rc) return rc;
+  const uint32_t* order = second ? ord + n : ord;
+  hipLaunchKernelGGL(k_sweep, dim3(blocks(n)), dim3(kBlock), 0, s, (uint32_t)n, order, c->sw_target.as<uint32_t>(),
                      caster ? (const uint32_t*)c->sw_caster.as<uint32_t>() : nullptr, c->sw_from.as<float>(),
                      c->sw_to.as<float>(), caster ? nullptr : (const float*)c->sw_radius.as<float>(),
                      c->bodies.as<BodyRec>(), c->hull.as<float4>(), c->sw_hit.as<uint8_t>(), c->sw_out.as<float>());

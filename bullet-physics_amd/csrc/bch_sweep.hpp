@@ -144,6 +144,7 @@ __device__ __forceinline__ void point_convex_distance(const F& support, V3 point
     }
     V3 w = point - support(v);  // mink(-v)
     bool dup = false;
+#pragma unroll
     for (int i = 0; i < 4; ++i)
       if (i < s.n && len2(s.p[i] - w) < 1e-12f) dup = true;
     if (dup) {
@@ -151,7 +152,11 @@ __device__ __forceinline__ void point_convex_distance(const F& support, V3 point
       dir = v / len(v);
       return;
     }
-    s.p[s.n++] = w;
+    // append with constant indices only, so the simplex stays in registers
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (i == s.n) s.p[i] = w;
+    s.n++;
     v = closest_origin(s);
   }
   if (len2(v) < 1e-10f) {
@@ -331,14 +336,32 @@ __device__ __forceinline__ bool convex_cast(const ShapeRef& shape, V3 from, Q4 o
   return true;
 }
 
-// Body-target sweeps: query k against body target[k]; a sphere of radius[k]
-// when caster is NULL, else body caster[k]'s shape at its orientation.
-__global__ void k_sweep(uint32_t n, const uint32_t* __restrict__ target, const uint32_t* __restrict__ caster,
-                        const float* __restrict__ from, const float* __restrict__ to, const float* __restrict__ radius,
-                        const BodyRec* __restrict__ bodies, const float4* __restrict__ hull, uint8_t* __restrict__ hit,
-                        float* __restrict__ out) {
+// Shape class of a body-target query (target type x caster type, 0 for a
+// sphere sweep): the sort key that groups like queries into wavefronts, so
+// a wavefront runs one support map instead of every branch.
+__global__ void k_sweep_keys(uint32_t n, const uint32_t* __restrict__ target, const uint32_t* __restrict__ caster,
+                             const BodyRec* __restrict__ bodies, uint32_t* __restrict__ keys,
+                             uint32_t* __restrict__ idx) {
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
+  uint32_t tt = bodies[target[k]].type & 0xffu;
+  uint32_t ct = caster ? (bodies[caster[k]].type & 0xffu) + 1u : 0u;
+  keys[k] = tt * 8u + ct;
+  idx[k] = k;
+}
+
+// Body-target sweeps: query k against body target[k]; a sphere of radius[k]
+// when caster is NULL, else body caster[k]'s shape at its orientation.
+// Thread i runs query order[i] (the class-sorted order) when order is set.
+__global__ __launch_bounds__(256) void k_sweep(uint32_t n, const uint32_t* __restrict__ order,
+                                               const uint32_t* __restrict__ target,
+                                               const uint32_t* __restrict__ caster, const float* __restrict__ from,
+                                               const float* __restrict__ to, const float* __restrict__ radius,
+                                               const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
+                                               uint8_t* __restrict__ hit, float* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k = order ? order[i] : i;
   BodyRec T = bodies[target[k]];
   ShapeRef ts = shape_of(T, hull);
   V3 f{from[3 * (uint64_t)k], from[3 * (uint64_t)k + 1], from[3 * (uint64_t)k + 2]};
@@ -365,7 +388,8 @@ __global__ void k_sweep(uint32_t n, const uint32_t* __restrict__ target, const u
 // Mesh sweeps (sweep_mesh :294-344, sweep_convex_mesh :480-524): the BVH walk
 // (no triangle-AABB filter), per-triangle casts in mesh space, the smallest
 // fraction (first on ties), mapped back to world space.
-__global__ void k_sweep_mesh(uint32_t n, const uint32_t* __restrict__ mesh_of, const float* __restrict__ mesh_pose,
+__global__ __launch_bounds__(256) void k_sweep_mesh(uint32_t n, const uint32_t* __restrict__ mesh_of,
+                                                    const float* __restrict__ mesh_pose,
                              const uint32_t* __restrict__ caster, const float* __restrict__ from,
                              const float* __restrict__ to, const float* __restrict__ radius,
                              const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,

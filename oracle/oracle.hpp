@@ -17,7 +17,11 @@
 // files under tests/ (test_oracle_*.py).  Functions without a closed-form,
 // known-answer or brute-force check: the quaternion op order (glam boundary)
 // and EPA's exact output on generic hulls -- "parity unpinned" beyond the
-// restatement itself.
+// restatement itself.  The CCD sweeps' entry fractions are pinned (the
+// reference's sweep tests, a dense scan, the slab entry of box casts, the
+// sphere cast against the sphere sweep); their hit normal is pinned only on
+// the reference's axis-aligned cases and is otherwise "parity unpinned": it
+// is the distance walk's direction at a probe point, restated step by step.
 #pragma once
 #include <algorithm>
 #include <cmath>
