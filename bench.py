@@ -64,8 +64,11 @@ def algorithmic_bytes(sc, pairs, offs, routes, kernel):
     return None
 
 
-def cpu_baseline(scenes, oracle_mod, sc, target_bodies, margin):
-    """Single-thread oracle pass over the bodies inside a corner sub-cube."""
+def cpu_baseline(scenes, oracle_mod, sc, target_bodies, margin, repeats=5):
+    """The oracle's pass (AABBs, DbvtBroadPhase pairs, collide_seeded per pair)
+    over the bodies inside a corner sub-cube: one warm-up, then the median of
+    `repeats` runs, single-threaded (the reference is single-threaded), and
+    the same with the AABBs and the narrowphase on all the host's cores."""
     side = float(sc["side"])
     n = sc["shape_type"].shape[0]
     f = (target_bodies / n) ** (1.0 / 3.0)
@@ -74,12 +77,21 @@ def cpu_baseline(scenes, oracle_mod, sc, target_bodies, margin):
     sub = {k: sc[k][idx] for k in ("position", "orientation", "shape_type", "shape_params", "hull_first",
                                    "hull_count")}
     sub["hull_vertices"] = sc["hull_vertices"]
-    P, Cn, sec = oracle_mod.collision_pass(sub, margin)
+    runs = [oracle_mod.collision_pass(sub, margin) for _ in range(repeats + 1)][1:]
+    P, Cn = runs[0][0], runs[0][1]
+    sec = float(np.median([r[2] for r in runs]))
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    mt = [oracle_mod.collision_pass_mt(sub, margin, threads) for _ in range(repeats + 1)][1:]
+    assert all(r[0] == P and r[1] == Cn for r in runs + mt)
+    sec_mt = float(np.median([r[2] for r in mt]))
     return dict(value=P / sec, unit="pairs/s", cores=1, kind="port",
-                sample=(f"single-thread C++ oracle pass (AABBs + exact sort-and-sweep pairs + collide_seeded per "
-                        f"pair) over the {idx.shape[0]} bodies of the same scene inside a corner cube of side "
-                        f"{side * f:.1f} m: {P} pairs, {Cn} contacts in {sec:.2f} s"),
-                contacts_per_s=Cn / sec, seconds=sec, pairs=P)
+                sample=(f"single-thread C++ oracle pass (AABBs + DbvtBroadPhase pairs + collide_seeded per pair) "
+                        f"over the {idx.shape[0]} bodies of the same scene inside a corner cube of side "
+                        f"{side * f:.1f} m: {P} pairs, {Cn} contacts; median of {repeats} runs after a warm-up, "
+                        f"{sec:.2f} s each"),
+                contacts_per_s=Cn / sec, seconds=sec, pairs=P,
+                all_cores={"value": P / sec_mt, "cores": threads, "seconds": sec_mt,
+                           "note": "AABBs and narrowphase on OpenMP threads, DBVT pairs serial"})
 
 
 def captured(w, margin, steps, ref):
@@ -496,7 +508,7 @@ def main():
                 "Model name:")[1].splitlines()[0].strip()
         except Exception:
             cpu["cpu_model"] = platform.processor()
-        log(f"cpu baseline: {cpu['value']:.0f} pairs/s ({cpu['sample']})")
+        log(f"cpu baseline: {cpu['value']:.0f} pairs/s, all cores {cpu['all_cores']['value']:.0f} ({cpu['sample']})")
 
     out = {
         "metric": "contact pairs/sec (broad+narrow, f32) on 1 MI355X; batched vs plain speedup",

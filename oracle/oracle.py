@@ -75,6 +75,8 @@ def lib():
         L.oracle_brute_pairs.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         L.oracle_sweep_pairs.restype = C.c_int64
         L.oracle_sweep_pairs.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.oracle_dbvt_pairs.restype = C.c_int64
+        L.oracle_dbvt_pairs.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         L.oracle_collide_pairs_f32.argtypes = [scp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                                C.c_void_p]
         L.oracle_collide_pairs_f64.argtypes = [scp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
@@ -111,6 +113,8 @@ def lib():
         L.oracle_mesh_collide_scene.argtypes = [scp, C.c_void_p, C.c_void_p, C.c_int64] + [C.c_void_p] * 5
         L.oracle_pass_f32.restype = C.c_int64
         L.oracle_pass_f32.argtypes = [scp, C.c_float, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.oracle_pass_f32_mt.restype = C.c_int64
+        L.oracle_pass_f32_mt.argtypes = [scp, C.c_float, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -352,6 +356,17 @@ def sweep_pairs(mn, mx):
     return out[:cnt]
 
 
+def dbvt_pairs(mn, mx):
+    """DbvtBroadPhase's pair set on a fresh scene (dbvt.rs), sorted."""
+    mn = np.ascontiguousarray(mn, np.float32)
+    mx = np.ascontiguousarray(mx, np.float32)
+    n = mn.shape[0]
+    cnt = lib().oracle_dbvt_pairs(n, mn.ctypes.data, mx.ctypes.data, None, 0)
+    out = np.empty((max(cnt, 1), 2), np.uint32)
+    lib().oracle_dbvt_pairs(n, mn.ctypes.data, mx.ctypes.data, out.ctypes.data, cnt)
+    return out[:cnt]
+
+
 def collide_pairs(scene, pairs, seeds=None, f64=False):
     """collide_seeded(body i, body j) per pair -> (count[P], points[P,4,7], path[P])."""
     r = _SceneRef(scene)
@@ -423,6 +438,16 @@ def collision_pass(scene, margin):
     nc = C.c_int64(0)
     sec = C.c_double(0)
     npairs = lib().oracle_pass_f32(C.byref(r.c), margin, C.byref(nc), C.byref(sec))
+    return int(npairs), int(nc.value), float(sec.value)
+
+
+def collision_pass_mt(scene, margin, threads):
+    """The same pass with AABBs and narrowphase on `threads` OpenMP threads:
+    (pairs, contacts, seconds)."""
+    r = _SceneRef(scene)
+    nc = C.c_int64(0)
+    sec = C.c_double(0)
+    npairs = lib().oracle_pass_f32_mt(C.byref(r.c), margin, int(threads), C.byref(nc), C.byref(sec))
     return int(npairs), int(nc.value), float(sec.value)
 
 

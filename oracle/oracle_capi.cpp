@@ -134,6 +134,110 @@ int64_t sweep_pairs(int64_t n, const Aabb<T>* boxes, std::vector<uint64_t>& keys
   return (int64_t)keys.size();
 }
 
+// DbvtBroadPhase on a fresh scene (broadphase/dbvt.rs): create_proxy for
+// slots 0..n-1 in order (:679-697) inserts each AABB into the dynamic tree
+// (insert_leaf, :159-224) and runs collide_leaf (:629-645), which walks the
+// tree for leaves overlapping the new one (walk_overlapping, :421-443) and
+// records (other, slot).  On a fresh scene each pair is found once, when its
+// later proxy arrives.  The reference keeps them in a sorted Vec by binary
+// search and insertion (:593-598); here they are collected and sorted once:
+// the same set in the same order, without the O(P) shifting per insertion.
+// compute_pairs' tick (:647-677, one rotation path) changes no leaf volume
+// on a fresh scene and its exact filter (:768-806) passes every pair, so
+// both are left out.  Glam's Vec3::min / max (merged, util.rs:41-46) keep
+// the non-NaN operand, as std::fmin / fmax do.
+struct DbvtOracle {
+  static constexpr uint32_t NONE = 0xffffffffu;
+  struct Node {
+    Aabb<float> vol;
+    uint32_t parent, c0, c1;  // c0 == NONE: leaf, c1 = payload
+  };
+  std::vector<Node> nodes;
+  uint32_t root = NONE;
+
+  static Aabb<float> merged(const Aabb<float>& a, const Aabb<float>& b) {
+    return Aabb<float>{mk(std::fmin(a.min.x, b.min.x), std::fmin(a.min.y, b.min.y), std::fmin(a.min.z, b.min.z)),
+                       mk(std::fmax(a.max.x, b.max.x), std::fmax(a.max.y, b.max.y), std::fmax(a.max.z, b.max.z))};
+  }
+  // util.rs:61-68
+  static bool contains(const Aabb<float>& a, const Aabb<float>& b) {
+    return a.min.x <= b.min.x && a.min.y <= b.min.y && a.min.z <= b.min.z && b.max.x <= a.max.x &&
+           b.max.y <= a.max.y && b.max.z <= a.max.z;
+  }
+  // dbvt.rs:77-80
+  static float proximity(const Aabb<float>& a, const Aabb<float>& b) {
+    V3<float> d = (a.min + a.max) - (b.min + b.max);
+    return (std::fabs(d.x) + std::fabs(d.y)) + std::fabs(d.z);
+  }
+  bool leaf(uint32_t n) const { return nodes[n].c0 == NONE; }
+
+  uint32_t insert(const Aabb<float>& v, uint32_t payload) {
+    uint32_t lf = (uint32_t)nodes.size();
+    nodes.push_back(Node{v, NONE, NONE, payload});
+    if (root == NONE) {
+      root = lf;
+      return lf;
+    }
+    uint32_t node = root;
+    while (!leaf(node)) {
+      uint32_t c0 = nodes[node].c0, c1 = nodes[node].c1;
+      node = proximity(v, nodes[c0].vol) < proximity(v, nodes[c1].vol) ? c0 : c1;
+    }
+    uint32_t prev = nodes[node].parent;
+    uint32_t in = (uint32_t)nodes.size();
+    nodes.push_back(Node{merged(nodes[node].vol, v), prev, node, lf});
+    nodes[node].parent = in;
+    nodes[lf].parent = in;
+    if (prev != NONE) {
+      if (nodes[prev].c1 == node) nodes[prev].c1 = in;
+      else nodes[prev].c0 = in;
+    } else {
+      root = in;
+    }
+    uint32_t child = in, p = prev;
+    while (p != NONE) {
+      if (contains(nodes[p].vol, nodes[child].vol)) break;
+      nodes[p].vol = merged(nodes[nodes[p].c0].vol, nodes[nodes[p].c1].vol);
+      child = p;
+      p = nodes[p].parent;
+    }
+    return lf;
+  }
+
+  template <class F>
+  void walk(const Aabb<float>& v, F f) const {
+    if (root == NONE) return;
+    std::vector<uint32_t> stack{root};
+    while (!stack.empty()) {
+      uint32_t n = stack.back();
+      stack.pop_back();
+      if (!nodes[n].vol.overlaps(v)) continue;
+      if (leaf(n)) {
+        f(nodes[n].c1);
+      } else {
+        stack.push_back(nodes[n].c0);  // the right child is visited first
+        stack.push_back(nodes[n].c1);
+      }
+    }
+  }
+};
+
+template <class T>
+int64_t dbvt_pairs(int64_t n, const Aabb<T>* boxes, std::vector<uint64_t>& keys) {
+  DbvtOracle t;
+  t.nodes.reserve(2 * (size_t)n);
+  keys.clear();
+  for (int64_t s = 0; s < n; ++s) {
+    Aabb<float> v = boxes[s];
+    t.insert(v, (uint32_t)s);
+    t.walk(v, [&](uint32_t other) {
+      if (other != (uint32_t)s) keys.push_back(((uint64_t)other << 32) | (uint64_t)s);
+    });
+  }
+  std::sort(keys.begin(), keys.end());
+  return (int64_t)keys.size();
+}
+
 }  // namespace
 
 extern "C" {
@@ -411,6 +515,20 @@ int64_t oracle_sweep_pairs(int64_t n, const float* mn, const float* mx, uint32_t
   return cnt;
 }
 
+// The same pair set through the DbvtBroadPhase restatement (dbvt_pairs);
+// checked against oracle_brute_pairs by the tests.
+int64_t oracle_dbvt_pairs(int64_t n, const float* mn, const float* mx, uint32_t* out, int64_t cap) {
+  std::vector<Aabb<float>> boxes(n);
+  for (int64_t i = 0; i < n; ++i) boxes[i] = Aabb<float>{v3p<float>(mn + 3 * i), v3p<float>(mx + 3 * i)};
+  std::vector<uint64_t> keys;
+  int64_t cnt = dbvt_pairs<float>(n, boxes.data(), keys);
+  for (int64_t k = 0; k < cnt && k < cap; ++k) {
+    out[2 * k] = (uint32_t)(keys[k] >> 32);
+    out[2 * k + 1] = (uint32_t)(keys[k] & 0xffffffffu);
+  }
+  return cnt;
+}
+
 // collide_seeded for each (i, j) pair of a scene (shape_a = body i).
 // out_count[P] (0..4), out_pts[P*28] (4 x pos3 normal3 depth), out_path[P]
 // (0 gjk-separated, 1 gjk-band, 2 epa, 3 closed form).
@@ -522,8 +640,35 @@ int64_t oracle_pass_f32(const or_scene* sc, float margin, int64_t* contacts, dou
   std::vector<Aabb<float>> boxes(sc->n);
   for (int64_t i = 0; i < sc->n; ++i) boxes[i] = ss.shapes[i].world_aabb(ss.pos[i], ss.quat[i]).expanded(margin);
   std::vector<uint64_t> keys;
-  int64_t np = sweep_pairs<float>(sc->n, boxes.data(), keys);
+  int64_t np = dbvt_pairs<float>(sc->n, boxes.data(), keys);
   int64_t nc = 0;
+  for (int64_t k = 0; k < np; ++k) {
+    uint32_t i = (uint32_t)(keys[k] >> 32), j = (uint32_t)(keys[k] & 0xffffffffu);
+    Manifold<float> m;
+    collide_seeded(ss.shapes[i], ss.pos[i], ss.quat[i], ss.shapes[j], ss.pos[j], ss.quat[j],
+                   (const V3<float>*)nullptr, m);
+    nc += m.n;
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  *contacts = nc;
+  *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return np;
+}
+
+// The same pass with the per-body AABBs and the per-pair narrowphase spread
+// over `threads` OpenMP threads (the pair sweep stays serial): the all-cores
+// CPU figure.  Pairs are independent, so the counts equal the serial pass's.
+int64_t oracle_pass_f32_mt(const or_scene* sc, float margin, int threads, int64_t* contacts, double* seconds) {
+  SceneShapes<float> ss;
+  ss.build(sc);
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<Aabb<float>> boxes(sc->n);
+#pragma omp parallel for schedule(static) num_threads(threads)
+  for (int64_t i = 0; i < sc->n; ++i) boxes[i] = ss.shapes[i].world_aabb(ss.pos[i], ss.quat[i]).expanded(margin);
+  std::vector<uint64_t> keys;
+  int64_t np = dbvt_pairs<float>(sc->n, boxes.data(), keys);
+  int64_t nc = 0;
+#pragma omp parallel for schedule(dynamic, 512) reduction(+ : nc) num_threads(threads)
   for (int64_t k = 0; k < np; ++k) {
     uint32_t i = (uint32_t)(keys[k] >> 32), j = (uint32_t)(keys[k] & 0xffffffffu);
     Manifold<float> m;

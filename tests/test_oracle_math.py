@@ -2,9 +2,10 @@
 
 Closed forms, brute force and invariants chosen so that a dropped term, a
 wrong sign/index or a transposed operand in the restatement fails a check:
-  * broadphase: the sweep equals brute force (the definition, brute_force.rs:54-67)
-    on the reference's own scattered-box inputs (axis_sweep3.rs:570-593) and
-    on random/degenerate sets;
+  * broadphase: the sweep and the DbvtBroadPhase restatement (dbvt.rs, the
+    CPU baseline's broadphase) equal brute force (the definition,
+    brute_force.rs:54-67) on the reference's own scattered-box inputs
+    (axis_sweep3.rs:570-593) and on random/degenerate sets;
   * world AABBs against float64 corner/segment/vertex sweeps;
   * hull support = brute-force argmax (first maximum);
   * sphere-sphere / sphere-plane depths in closed form;
@@ -47,6 +48,7 @@ def test_sweep_equals_brute_on_reference_scattered_boxes(oracle):
     mn, mx = np.array(mn), np.array(mx)
     bp = oracle.brute_pairs(mn, mx)
     assert np.array_equal(oracle.sweep_pairs(mn, mx), bp)
+    assert np.array_equal(oracle.dbvt_pairs(mn, mx), bp)
     # brute force against numpy's reading of util.rs:28-38
     ov = np.all((mn[:, None] <= mx[None]) & (mn[None] <= mx[:, None]), axis=2)
     ii, jj = np.nonzero(np.triu(ov, 1))
@@ -65,7 +67,9 @@ def test_sweep_equals_brute_random_and_degenerate(oracle, seed):
     mn[60], mx[60] = -1e30, 1e30         # plane-like
     mn[61:64] = np.nan                   # dead proxies
     mx[61:64] = np.nan
-    assert np.array_equal(oracle.sweep_pairs(mn, mx), oracle.brute_pairs(mn, mx))
+    bp = oracle.brute_pairs(mn, mx)
+    assert np.array_equal(oracle.sweep_pairs(mn, mx), bp)
+    assert np.array_equal(oracle.dbvt_pairs(mn, mx), bp)
 
 
 def test_world_aabbs_match_float64_sweeps(oracle, scenes):
