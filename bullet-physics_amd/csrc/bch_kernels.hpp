@@ -861,8 +861,11 @@ __global__ __launch_bounds__(256) void k_np_hull(const Counters* __restrict__ cn
                                                  const float4* __restrict__ hull, const float* __restrict__ seeds,
                                                  uint32_t* __restrict__ ccount, float4* __restrict__ cslots,
                                                  float4* __restrict__ epa_q, Counters* cnt) {
-  // Small hulls of the pair are staged in LDS, one slot pair per group.
-  __shared__ float4 hstage[256 / G][2][64];
+  // One hull of the pair (A if it is a hull of <= 64 vertices, else B) is
+  // staged in LDS, one slot per group; a second hull is read through the
+  // cache.  32 KB per block at G = 8 keeps three blocks per CU, where
+  // staging both (64 KB) allowed two.
+  __shared__ float4 hstage[256 / G][64];
   uint32_t beg = cnt_ro->cls_start[CLS_GJK_HULL], end = cnt_ro->cls_end[CLS_GJK_HULL];
   const uint32_t per_block = blockDim.x / G;
   int grp = threadIdx.x / G, sub = threadIdx.x % G;
@@ -877,12 +880,11 @@ __global__ __launch_bounds__(256) void k_np_hull(const Counters* __restrict__ cn
       BodyRec A = bodies[i], B = bodies[j];
       Pair pr = make_pair(A, B, hull);
       if (pr.a.type == HULL && pr.a.nhull <= 64) {
-        for (int v = sub; v < pr.a.nhull; v += G) hstage[grp][0][v] = pr.a.hull[v];
-        pr.a.hull = hstage[grp][0];
-      }
-      if (pr.b.type == HULL && pr.b.nhull <= 64) {
-        for (int v = sub; v < pr.b.nhull; v += G) hstage[grp][1][v] = pr.b.hull[v];
-        pr.b.hull = hstage[grp][1];
+        for (int v = sub; v < pr.a.nhull; v += G) hstage[grp][v] = pr.a.hull[v];
+        pr.a.hull = hstage[grp];
+      } else if (pr.b.type == HULL && pr.b.nhull <= 64) {
+        for (int v = sub; v < pr.b.nhull; v += G) hstage[grp][v] = pr.b.hull[v];
+        pr.b.hull = hstage[grp];
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
