@@ -38,6 +38,7 @@ struct Counters {
   uint32_t n_pairs_total;  // pairs found
   uint32_t n_contacts;     // contacts found
   uint32_t n_pen;          // penetrating pairs (de-penetration step)
+  uint32_t n_epa_defer;    // EPA entries handed from the two-pair kernel to the one-pair kernel
 };
 
 __device__ __forceinline__ uint32_t ord_enc(float f) {
@@ -969,6 +970,7 @@ __global__ void k_reset_big(Counters* c) {
 __global__ void k_reset_epa(Counters* c) {
   if (threadIdx.x == 0) {
     c->n_epa = 0;
+    c->n_epa_defer = 0;
     c->errors &= ~2u;
     for (int k = 0; k < 8; ++k) c->cls_start[k] = c->cls_end[k] = 0u;
   }

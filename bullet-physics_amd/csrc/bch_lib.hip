@@ -13,6 +13,7 @@
 #include "../../include/bullet_collision_hip.h"
 #include "bch_kernels.hpp"
 #include "bch_epa_wave.hpp"
+#include "bch_epa_dual.hpp"
 #include "bch_adjoint.hpp"
 #include "bch_sort.hpp"
 #include "bch_depen.hpp"
@@ -52,6 +53,7 @@ struct DevBuf {
 };
 
 constexpr int kBlock = 256;
+constexpr uint32_t kEpaDeferGrid = 4096;  // one-pair EPA blocks for the deferred entries
 inline uint32_t blocks(uint64_t n, int b = kBlock) { return (uint32_t)((n + b - 1) / b); }
 
 }  // namespace
@@ -73,7 +75,7 @@ struct bch_context {
   DevBuf aabb, flags, big_ids, counters, partials, keys, keys_alt, vals, vals_alt, nodes, wend, hull_ids;
   uint32_t n_hulls = 0;
   DevBuf leaf_counts, leaf_offsets, slots, pair_keys, pair_keys_alt, big_pairs, pairs, cls, cls_alt, perm,
-      perm_alt, ccount, coffsets, cslots, contacts, epa_q, scan_part, rs_hist, rs_offs;
+      perm_alt, ccount, coffsets, cslots, contacts, epa_q, epa_q2, scan_part, rs_hist, rs_offs;
   int hull_lanes = 16;      // lanes per hull GJK pair (4, 8 or 16)
   uint32_t epa_grid = 32768; // EPA blocks (one wavefront each)
   uint32_t np_grid = 2048;  // narrowphase blocks per shape class
@@ -408,6 +410,7 @@ int narrow_split(bch_context* c) {
   ENSURE(c, c->coffsets, (Pm + 1) * sizeof(uint32_t));
   ENSURE(c, c->cslots, Pm * 8 * sizeof(float4));
   ENSURE(c, c->epa_q, Pm * 4 * sizeof(float4));
+  ENSURE(c, c->epa_q2, Pm * 4 * sizeof(float4));
   // capacity mode: P is the capacity, the device holds the count
   const uint32_t* P_dev = c->cap_mode ? &cnt->n_pairs : nullptr;
   hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
@@ -466,7 +469,13 @@ int narrow_finish(bch_context* c, const float* d_seeds) {
       BCH_NP_HULL(16);
 #undef BCH_NP_HULL
     mark(c, 6);
-    hipLaunchKernelGGL(k_epa_wave, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
+    // two pairs per wavefront; the entries that leave its fast path run
+    // through the one-pair kernel from the start
+    hipLaunchKernelGGL(k_epa_dual, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
+                       c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
+                       c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(),
+                       c->epa_q2.as<float4>(), cnt);
+    hipLaunchKernelGGL(k_epa_wave, dim3(kEpaDeferGrid), dim3(64), 0, s, &cnt->n_epa_defer, c->epa_q2.as<float4>(),
                        c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                        c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(), cnt);
     mark(c, 7);
@@ -864,7 +873,7 @@ void bch_destroy(bch_context* c) {
                    &c->counters, &c->partials, &c->keys, &c->keys_alt, &c->vals, &c->vals_alt, &c->nodes, &c->hull_ids,
                    &c->wend, &c->leaf_counts, &c->leaf_offsets, &c->slots, &c->pair_keys,
                    &c->pair_keys_alt, &c->big_pairs, &c->pairs, &c->cls, &c->cls_alt, &c->perm, &c->perm_alt,
-                   &c->ccount, &c->coffsets, &c->cslots, &c->contacts, &c->epa_q,
+                   &c->ccount, &c->coffsets, &c->cslots, &c->contacts, &c->epa_q, &c->epa_q2,
                    &c->scan_part, &c->rs_hist, &c->rs_offs, &c->sd_out, &c->sd_status, &c->pen_pairs,
                    &c->pen_g, &c->g14, &c->body_grad, &c->pen_part, &c->dhist, &c->dstep, &c->mkeys[0],
                    &c->mkeys[1], &c->mvalid[0], &c->mvalid[1], &c->ment[0], &c->ment[1], &c->bflags, &c->bgm,
