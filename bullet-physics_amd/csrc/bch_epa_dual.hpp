@@ -465,21 +465,20 @@ __device__ __forceinline__ void dual_epa(const DualShapes& ds, V3 t0, V3 t1, V3 
   }
 }
 
-// Grid-stride over the EPA queue, two entries per wavefront.  Outputs as
-// epa_queue_run; an entry that leaves the fast path is appended to defer_q
-// (count in cnt->n_epa_defer) for k_epa_wave.
-__global__ __launch_bounds__(64) void k_epa_dual(const uint32_t* __restrict__ n_queue,
-                                                 const float4* __restrict__ epa_q, const uint32_t* __restrict__ pairs,
-                                                 const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
-                                                 uint32_t* __restrict__ ccount, float4* __restrict__ cslots,
-                                                 uint32_t* __restrict__ route, float4* __restrict__ defer_q,
-                                                 Counters* cnt) {
-  __shared__ EpaHalf S[2];
+// Grid-stride over an EPA queue, two entries per wavefront (Src as in
+// epa_queue_run).  Outputs as epa_queue_run; an entry that leaves the fast
+// path is appended to defer_q (count in cnt->n_epa_defer) for the one-pair
+// kernel.
+template <class Src>
+__device__ __forceinline__ void epa_dual_run(const uint32_t* __restrict__ n_queue, const float4* __restrict__ epa_q,
+                                             const Src& src, uint32_t* __restrict__ ccount,
+                                             float4* __restrict__ cslots, uint32_t stride,
+                                             uint32_t* __restrict__ route, float4* __restrict__ defer_q,
+                                             Counters* cnt, EpaHalf* S) {
   const int lane = threadIdx.x & 63;
   const int h = lane >> 5;
   const int hl = lane & 31;
   const uint32_t nq = *n_queue;
-  const WorldEpaSrc src{pairs, bodies, hull};
   for (uint32_t base = 2u * blockIdx.x; base < nq; base += 2u * gridDim.x) {
     const uint32_t q = base + (uint32_t)h;
     const bool has = q < nq;
@@ -513,13 +512,24 @@ __global__ __launch_bounds__(64) void k_epa_dual(const uint32_t* __restrict__ n_
         if (route) route[p] |= 8u;
         if (hit) {
           V3 pos = point_a - e.n * depth;
-          cslots[(uint64_t)p * 8] = make_float4(pos.x, pos.y, pos.z, depth);
-          cslots[(uint64_t)p * 8 + 1] = make_float4(e.n.x, e.n.y, e.n.z, 0.0f);
+          cslots[(uint64_t)p * stride] = make_float4(pos.x, pos.y, pos.z, depth);
+          cslots[(uint64_t)p * stride + 1] = make_float4(e.n.x, e.n.y, e.n.z, 0.0f);
         }
       }
     }
     wave_sync();
   }
+}
+
+// The world pass's EPA queue (pairs of the pass, 8 contact slots per pair).
+__global__ __launch_bounds__(64) void k_epa_dual(const uint32_t* __restrict__ n_queue,
+                                                 const float4* __restrict__ epa_q, const uint32_t* __restrict__ pairs,
+                                                 const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
+                                                 uint32_t* __restrict__ ccount, float4* __restrict__ cslots,
+                                                 uint32_t* __restrict__ route, float4* __restrict__ defer_q,
+                                                 Counters* cnt) {
+  __shared__ EpaHalf S[2];
+  epa_dual_run(n_queue, epa_q, WorldEpaSrc{pairs, bodies, hull}, ccount, cslots, 8u, route, defer_q, cnt, S);
 }
 
 }  // namespace bch

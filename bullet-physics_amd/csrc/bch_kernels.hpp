@@ -69,6 +69,7 @@ __global__ void k_reset(Counters* c) {
     c->n_pairs_total = 0;
     c->n_contacts = 0;
     c->n_pen = 0;
+    c->n_epa_defer = 0;
   }
 }
 

@@ -120,7 +120,7 @@ struct bch_context {
   bool mesh_dirty = false;
   DevBuf d_mesh_recs, d_mesh_nodes, d_mesh_tris, d_mesh_tri_aabb, d_mesh_order;
   DevBuf mx_pairs, mx_poses, mx_seeds, mx_first, mx_count, mx_offs, mx_sub, mx_sub_cnt, mx_sub_slots, mx_epa_q,
-      mx_out_count, mx_out_pts;
+      mx_epa_q2, mx_out_count, mx_out_pts;
   DevBuf sw_target, sw_caster, sw_from, sw_to, sw_radius, sw_pose, sw_hit, sw_out, sw_keys, sw_order;
 
   // proxy API state (host mirror of the broadphase slots)
@@ -880,9 +880,9 @@ void bch_destroy(bch_context* c) {
                    &c->mroute, &c->msrc, &c->mcnt, &c->mslots, &c->moff, &c->mpts, &c->mseeds, &c->mimp,
                    &c->vel, &c->d_mesh_recs, &c->d_mesh_nodes, &c->d_mesh_tris, &c->d_mesh_tri_aabb,
                    &c->d_mesh_order, &c->mx_pairs, &c->mx_poses, &c->mx_seeds, &c->mx_first, &c->mx_count,
-                   &c->mx_offs, &c->mx_sub, &c->mx_sub_cnt, &c->mx_sub_slots, &c->mx_epa_q, &c->mx_out_count,
-                   &c->mx_out_pts, &c->sw_target, &c->sw_caster, &c->sw_from, &c->sw_to, &c->sw_radius,
-                   &c->sw_pose, &c->sw_hit, &c->sw_out, &c->sw_keys, &c->sw_order};
+                   &c->mx_offs, &c->mx_sub, &c->mx_sub_cnt, &c->mx_sub_slots, &c->mx_epa_q, &c->mx_epa_q2,
+                   &c->mx_out_count, &c->mx_out_pts, &c->sw_target, &c->sw_caster, &c->sw_from, &c->sw_to,
+                   &c->sw_radius, &c->sw_pose, &c->sw_hit, &c->sw_out, &c->sw_keys, &c->sw_order};
   for (DevBuf* b : all) b->release();
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->own_stream);
@@ -1686,6 +1686,7 @@ int bch_collide_mesh_batch(bch_context* c, const bch_bodies* bodies, const uint3
   ENSURE(c, c->mx_sub_cnt, Sm * sizeof(uint32_t));
   ENSURE(c, c->mx_sub_slots, Sm * 2 * sizeof(float4));
   ENSURE(c, c->mx_epa_q, Sm * 4 * sizeof(float4));
+  ENSURE(c, c->mx_epa_q2, Sm * 4 * sizeof(float4));
   hipLaunchKernelGGL(k_mesh_emit, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->mx_pairs.as<uint2>(),
                      (const float*)c->mx_poses.as<float>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(), recs,
                      c->d_mesh_nodes.as<float4>(), c->d_mesh_order.as<uint32_t>(), c->d_mesh_tri_aabb.as<float4>(),
@@ -1698,8 +1699,12 @@ int bch_collide_mesh_batch(bch_context* c, const bch_bodies* bodies, const uint3
                        c->mx_sub_slots.as<float4>(), c->mx_epa_q.as<float4>(), cnt);
     MeshEpaSrc src{c->mx_sub.as<uint2>(), c->mx_pairs.as<uint2>(), c->mx_poses.as<float>(), c->bodies.as<BodyRec>(),
                    c->hull.as<float4>(), recs, c->d_mesh_tris.as<float4>()};
-    hipLaunchKernelGGL(k_epa_mesh, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->mx_epa_q.as<float4>(), src,
-                       c->mx_sub_cnt.as<uint32_t>(), c->mx_sub_slots.as<float4>(), cnt);
+    hipLaunchKernelGGL(k_epa_mesh_dual, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->mx_epa_q.as<float4>(),
+                       src, c->mx_sub_cnt.as<uint32_t>(), c->mx_sub_slots.as<float4>(), c->mx_epa_q2.as<float4>(),
+                       cnt);
+    hipLaunchKernelGGL(k_epa_mesh, dim3(kEpaDeferGrid), dim3(64), 0, s, &cnt->n_epa_defer,
+                       c->mx_epa_q2.as<float4>(), src, c->mx_sub_cnt.as<uint32_t>(), c->mx_sub_slots.as<float4>(),
+                       cnt);
   }
   hipLaunchKernelGGL(k_mesh_reduce, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->mx_offs.as<uint32_t>(),
                      c->mx_sub_cnt.as<uint32_t>(), c->mx_sub_slots.as<float4>(),

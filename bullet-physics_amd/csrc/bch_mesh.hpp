@@ -303,6 +303,16 @@ __global__ __launch_bounds__(64) void k_epa_mesh(const uint32_t* __restrict__ n_
   epa_queue_run(n_queue, epa_q, src, sub_cnt, sub_slots, 2u, (uint32_t*)nullptr, cnt, S);
 }
 
+// The same queue two sub-pairs per wavefront (bch_epa_dual.hpp); entries
+// outside its fast path go to defer_q for k_epa_mesh.
+__global__ __launch_bounds__(64) void k_epa_mesh_dual(const uint32_t* __restrict__ n_queue,
+                                                      const float4* __restrict__ epa_q, MeshEpaSrc src,
+                                                      uint32_t* __restrict__ sub_cnt, float4* __restrict__ sub_slots,
+                                                      float4* __restrict__ defer_q, Counters* cnt) {
+  __shared__ EpaHalf S[2];
+  epa_dual_run(n_queue, epa_q, src, sub_cnt, sub_slots, 2u, (uint32_t*)nullptr, defer_q, cnt, S);
+}
+
 // ContactManifold::reduce (manifold.rs:76-110) over the pair's sub-pair
 // points in candidate order, without sorting: the stable sort by depth makes
 // ties fall to the earlier candidate, which the comparisons below encode.
