@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 STAGES = ["aabb", "tree_build", "tree_traverse", "pair_list", "narrow_closed_gjk", "narrow_hull_gjk", "epa",
           "contact_compaction"]
 # stages that are one kernel launch each: the roofline candidates
-SINGLE_KERNEL = {"tree_traverse": "k_traverse", "narrow_hull_gjk": "k_np_hull", "epa": "k_epa_wave"}
+SINGLE_KERNEL = {"tree_traverse": "k_traverse", "narrow_hull_gjk": "k_np_hull", "epa": "k_epa_dual"}
 
 
 def log(msg):
@@ -44,9 +44,10 @@ def algorithmic_bytes(sc, pairs, offs, routes, kernel):
     k_np_hull   per pair with a hull: pair index 4 + pair 8 + two bodies 88
                 + both hulls' vertices + count 4 + its band contacts 28 each,
                 + 64 B queue entry for a pair sent on to EPA.
-    k_epa_wave  per queued pair: queue entry 64 + pair 8 + two bodies 88 +
+    k_epa_dual  per queued pair: queue entry 64 + pair 8 + two bodies 88 +
                 both hulls' vertices + count 4 + route 8 (read, write) +
-                28 per contact.
+                28 per contact (the stage includes the one-pair k_epa_wave
+                launch over the few entries the two-pair kernel defers).
     """
     P = pairs.shape[0]
     hc = sc["hull_count"].astype(np.int64)
@@ -59,7 +60,7 @@ def algorithmic_bytes(sc, pairs, offs, routes, kernel):
     if kernel == "k_np_hull":
         m = cls == 6
         return int(m.sum() * (4 + 8 + 88 + 4) + 12 * hv[m].sum() + 28 * ncon[m & ~epa].sum() + 64 * (m & epa).sum())
-    if kernel == "k_epa_wave":
+    if kernel == "k_epa_dual":
         return int(epa.sum() * (64 + 8 + 88 + 4 + 8) + 12 * hv[epa].sum() + 28 * ncon[epa].sum())
     return None
 

@@ -1,16 +1,16 @@
 // bch_epa_dual.hpp -- EPA for two pairs per wavefront, one per 32-lane half.
 //
 // The polytopes of the world pass are small: the closest-face search and
-// the expansions see at most 32 faces for ~99% of the pairs, and the visible
-// set rarely exceeds a handful of faces.  One wavefront per pair leaves most
+// the expansions see at most 64 faces for all but a few pairs in 10^5, and
+// the visible set rarely exceeds a handful of faces.  One wavefront per pair leaves most
 // lanes idle while every instruction still costs the whole wavefront, so
 // here each half runs its own pair with the same algorithm, statement order
-// and tie rules as k_epa_wave (narrowphase/epa.rs:127-398): one face per
-// lane, the reference's swap-remove horizon replayed per half, the shapes'
-// supports on 16 lanes each.
+// and tie rules as k_epa_wave (narrowphase/epa.rs:127-398): faces l and
+// l + 32 in lane l, the reference's swap-remove horizon replayed per half,
+// the shapes' supports on 16 lanes each.
 //
 // A pair that leaves that fast path (a hull of more than 64 vertices, more
-// than 32 faces, more than 10 visible faces, a directed edge shared by two
+// than 64 faces, more than 10 visible faces, a directed edge shared by two
 // visible faces) is not finished here: its queue entry goes to a second
 // queue, which k_epa_wave runs from the start afterwards, so every result is
 // the one-pair kernel's.
@@ -23,7 +23,7 @@
 
 namespace bch {
 
-constexpr int kDualF = 64;  // face slots per half: <= 32 live + <= 30 fan faces
+constexpr int kDualF = 64;  // face slots per half (two per lane)
 
 struct EpaHalf {
   float4 v[BCH_WEPA_V];
@@ -176,18 +176,32 @@ __device__ __forceinline__ V3 hv(const EpaHalf& S, int i) {
   return V3{v.x, v.y, v.z};
 }
 
-// epa.rs:212-224 over the (<= 32) faces of the half: first face with the
-// smallest |distance|, NaN distances skipped, a NaN first face kept.
-__device__ __forceinline__ int dual_best(const EpaHalf& S, int nfaces, int hl, int h) {
+// epa.rs:212-224 over the (<= 64) faces of the half, faces hl and hl + 32
+// in lane hl: first face with the smallest |distance|, NaN distances
+// skipped, a NaN first face kept.
+template <bool TWO>
+__device__ __forceinline__ int dual_best_t(const EpaHalf& S, int nfaces, int hl, int h) {
   const bool nan0 = is_nan(S.dist[0]);
-  float v = INFINITY;
+  float v0 = INFINITY, v1 = INFINITY;
   if (hl < nfaces) {
     float d = fabsf(S.dist[hl]);
-    v = is_nan(d) ? INFINITY : d;
+    v0 = is_nan(d) ? INFINITY : d;
   }
-  float m = half_ext_f<false>(v, h);
-  uint32_t bm = hballot(v == m, h);
-  return (nan0 || !(m < INFINITY) || !bm) ? 0 : ctz64((uint64_t)bm);
+  if constexpr (TWO) {
+    if (hl + 32 < nfaces) {
+      float d = fabsf(S.dist[hl + 32]);
+      v1 = is_nan(d) ? INFINITY : d;
+    }
+  }
+  float m = half_ext_f<false>(v1 < v0 ? v1 : v0, h);
+  uint32_t b0 = hballot(v0 == m, h);
+  uint32_t b1 = TWO ? hballot(v1 == m, h) : 0u;
+  if (nan0 || !(m < INFINITY)) return 0;
+  return b0 ? ctz64((uint64_t)b0) : (b1 ? 32 + ctz64((uint64_t)b1) : 0);
+}
+// The second face slot is only scanned while a half holds more than 32 faces.
+__device__ __forceinline__ int dual_best(const EpaHalf& S, int nfaces, int hl, int h) {
+  return __ballot(nfaces > 32) ? dual_best_t<true>(S, nfaces, hl, h) : dual_best_t<false>(S, nfaces, hl, h);
 }
 
 // The fan face of horizon edge (u, w) and the new vertex (epa.rs:372-386).
@@ -204,38 +218,63 @@ __device__ __forceinline__ void dual_fan_face(EpaHalf& S, int d, uint32_t u, uin
   S.abc[d] = u | (w << 8) | ((uint32_t)nidx << 16);
 }
 
-// epa.rs:309-388 for the halves with `act` set.  Per-half result: 1 done
-// (or nothing visible), 0 degenerate horizon (polytope untouched), -2 the
-// pair leaves the fast path (polytope untouched).
-__device__ __forceinline__ int dual_expand(EpaHalf& S, int& nfaces, V3 np, int nidx, int hl, int h, bool act) {
-  const bool live = act && hl < nfaces;
-  float fnx = 0.0f, fny = 0.0f, fnz = 0.0f, fd = 0.0f, fpd = 0.0f;
-  uint32_t fabc = 0u;
-  if (live) {
-    fnx = S.nx[hl];
-    fny = S.ny[hl];
-    fnz = S.nz[hl];
-    fd = S.dist[hl];
-    fpd = S.pd[hl];
-    fabc = S.abc[hl];
+// One face slot of a lane: face f of the half's store, with its
+// visibility against the new point.
+struct DualFace {
+  float nx, ny, nz, d, pd;
+  uint32_t abc;
+  bool live, vis;
+  __device__ __forceinline__ void load(const EpaHalf& S, int f, bool l, V3 np) {
+    live = l;
+    nx = ny = nz = d = pd = 0.0f;
+    abc = 0u;
+    if (l) {
+      nx = S.nx[f];
+      ny = S.ny[f];
+      nz = S.nz[f];
+      d = S.dist[f];
+      pd = S.pd[f];
+      abc = S.abc[f];
+    }
+    vis = l && dot(V3{nx, ny, nz}, np) - pd > 1e-5f;
   }
-  const bool vis = live && dot(V3{fnx, fny, fnz}, np) - fpd > 1e-5f;
-  const uint32_t vm = hballot(vis, h);
+  __device__ __forceinline__ void store(EpaHalf& S, int f) const {
+    S.nx[f] = nx;
+    S.ny[f] = ny;
+    S.nz[f] = nz;
+    S.dist[f] = d;
+    S.pd[f] = pd;
+    S.abc[f] = abc;
+  }
+};
+
+// epa.rs:309-388 for the halves with `act` set, faces hl and hl + 32 in lane
+// hl.  Per-half result: 1 done (or nothing visible), 0 degenerate horizon
+// (polytope untouched), -2 the pair leaves the fast path (polytope
+// untouched).
+template <bool TWO>
+__device__ __forceinline__ int dual_expand_t(EpaHalf& S, int& nfaces, V3 np, int nidx, int hl, int h, bool act) {
+  DualFace f0, f1;
+  f0.load(S, hl, act && hl < nfaces, np);
+  f1.load(S, hl + 32, TWO && act && hl + 32 < nfaces, np);
+  const uint32_t vm0 = hballot(f0.vis, h), vm1 = TWO ? hballot(f1.vis, h) : 0u;
   int res = 1;
-  bool go = act && vm != 0u;
-  const int nv = __popc(vm);
+  bool go = act && (vm0 | vm1) != 0u;
+  const int nv0 = __popc(vm0);
+  const int nv = nv0 + __popc(vm1);
   if (go && nv > 10) {
     res = -2;
     go = false;
   }
   if (!__ballot(go)) return res;
   // Edge events: lane l < 3 nv holds edge l % 3 of the visible face of rank
-  // l / 3, in the reference's visiting order.
+  // l / 3, in the reference's visiting order (face index order).
   const int nev = go ? 3 * nv : 0;
   const int rank = hl / 3;
   const int e = hl - 3 * rank;
-  // the visible faces' triples by rank (their order among the faces)
-  if (go && vis) S.vis_abc[__popc(vm & (uint32_t)lanes_below(hl))] = fabc;
+  const uint32_t below = (uint32_t)lanes_below(hl);
+  if (go && f0.vis) S.vis_abc[__popc(vm0 & below)] = f0.abc;
+  if (go && f1.vis) S.vis_abc[nv0 + __popc(vm1 & below)] = f1.abc;
   wave_sync();
   const uint32_t eabc = (go && hl < nev) ? S.vis_abc[rank] : 0u;
   const uint32_t fa = eabc & 0xffu, fb = (eabc >> 8) & 0xffu, fc = (eabc >> 16) & 0xffu;
@@ -260,10 +299,9 @@ __device__ __forceinline__ int dual_expand(EpaHalf& S, int& nfaces, V3 np, int n
     go = false;
   }
   if (!__ballot(go)) return res;
-  // Replay of the pushes and swap-removes per half: lane p of `list` holds
-  // the event at list position p, lane v of `pos` the position of event v.
-  // The replay runs on the half's LDS lists: each step reads them at the
-  // same address in every lane of the half and one lane writes.
+  // Replay of the pushes and swap-removes on the half's LDS lists (list[p]
+  // = event at position p, pos[v] = position of event v): every lane of
+  // the half reads the same address, one lane writes.
   const uint32_t cancel = hballot(is_ev && match >= 0, h);
   int ne = 0;
   for (int l = 0; l < nevmax; ++l) {
@@ -305,28 +343,28 @@ __device__ __forceinline__ int dual_expand(EpaHalf& S, int& nfaces, V3 np, int n
     go = false;
   }
   // Commit: stable compaction of the faces that stay, then the fan.  A
-  // polytope that would pass 32 faces leaves the fast path instead.
-  const uint32_t keepm = hballot(go && live && !vis, h);
-  const int kept = __popc(keepm);
-  if (go && kept + ne > 32) {
+  // polytope that would pass the store leaves the fast path instead.
+  const bool k0 = go && f0.live && !f0.vis, k1 = TWO && go && f1.live && !f1.vis;
+  const uint32_t km0 = hballot(k0, h), km1 = TWO ? hballot(k1, h) : 0u;
+  const int kept0 = __popc(km0);
+  const int kept = kept0 + __popc(km1);
+  if (go && kept + ne > kDualF) {
     res = -2;
     go = false;
   }
   wave_sync();
-  if (go && live && !vis) {
-    int d = __popc(keepm & (uint32_t)lanes_below(hl));
-    S.nx[d] = fnx;
-    S.ny[d] = fny;
-    S.nz[d] = fnz;
-    S.dist[d] = fd;
-    S.pd[d] = fpd;
-    S.abc[d] = fabc;
-  }
+  if (go && k0) f0.store(S, __popc(km0 & below));
+  if (TWO && go && k1) f1.store(S, kept0 + __popc(km1 & below));
   wave_sync();
   if (go && is_h) dual_fan_face(S, kept + hl, hu, hw, vu, vw, np, nidx);
   if (go) nfaces = kept + ne;
   wave_sync();
   return res;
+}
+// The second face slot is only visited while a half holds more than 32 faces.
+__device__ __forceinline__ int dual_expand(EpaHalf& S, int& nfaces, V3 np, int nidx, int hl, int h, bool act) {
+  return __ballot(act && nfaces > 32) ? dual_expand_t<true>(S, nfaces, np, nidx, hl, h, act)
+                                      : dual_expand_t<false>(S, nfaces, np, nidx, hl, h, act);
 }
 
 // epa.rs:127-205 for the pair of each half.  `act` false: the half has no

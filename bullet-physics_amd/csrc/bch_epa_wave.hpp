@@ -736,9 +736,7 @@ __device__ __forceinline__ void epa_queue_run(const uint32_t* __restrict__ n_que
   }
 }
 
-// Five waves per SIMD (96 VGPRs, a few spills) measured faster than the
-// four that 114 VGPRs allow: the loop waits on LDS and cross-lane moves.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) void k_epa_wave(const uint32_t* __restrict__ n_queue,
+__global__ __launch_bounds__(64) void k_epa_wave(const uint32_t* __restrict__ n_queue,
                                                  const float4* __restrict__ epa_q, const uint32_t* __restrict__ pairs,
                                                  const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
                                                  uint32_t* __restrict__ ccount, float4* __restrict__ cslots,

@@ -53,7 +53,7 @@ struct DevBuf {
 };
 
 constexpr int kBlock = 256;
-constexpr uint32_t kEpaDeferGrid = 4096;  // one-pair EPA blocks for the deferred entries
+constexpr uint32_t kEpaDeferGrid = 16384;  // one-pair EPA blocks for the deferred entries (~4% of the queue)
 inline uint32_t blocks(uint64_t n, int b = kBlock) { return (uint32_t)((n + b - 1) / b); }
 
 }  // namespace
