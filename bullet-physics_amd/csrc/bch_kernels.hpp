@@ -5,17 +5,18 @@
 //   k_aabb        body record -> world AABB + margin (fused), scene bounds,
 //                 big / NaN classification
 //   k_morton      30-bit Morton code of the AABB centre (world id above it)
-//   radix sort    (rocPRIM) codes -> leaf order
+//   radix sort    (bch_sort.hpp) codes -> leaf order
 //   k_tree_*      implicit complete binary tree over the sorted leaves;
 //                 boxes unioned level by level in LDS
-//   k_traverse    one lane per leaf walks the tree over the leaves to its
-//                 right: exact pair set without duplicates
+//   k_traverse    each leaf walks the tree over the leaves to its right
+//                 (two leaves per lane): exact pair set without duplicates
 //   k_big         bodies with unbounded AABBs (planes) against their world
-//   radix sort    (rocPRIM) pair keys i << 32 | j -> lexicographic order
+//   radix sort    (bch_sort.hpp) pair keys i << 32 | j -> lexicographic order
 //   k_np<CLS>     one lane per pair, one kernel per shape class
 //   k_np_hull     a group of lanes per pair with a hull
-//   k_epa_wave    one wavefront per deep-overlap pair, polytope in LDS
-//                 (bch_epa_wave.hpp)
+//   k_epa_dual    EPA of the deep-overlap pairs, two per wavefront, polytopes
+//                 in LDS (bch_epa_dual.hpp); the few outside its fast path
+//                 rerun one per wavefront in k_epa_wave (bch_epa_wave.hpp)
 //   k_gather      per-pair contact slots -> dense CSR contacts
 #pragma once
 #include "bch_narrow.hpp"
@@ -468,52 +469,102 @@ __global__ void k_tree_up(float4* __restrict__ nodes, uint32_t first, int m) {
   }
 }
 
-// One lane per leaf s: walk the subtrees right of s in depth-first order;
+// Leaf s walks the subtrees right of s in depth-first order;
 // every overlapping leaf is a pair found exactly once.  Keys i << 32 | j go to
 // the leaf's K slots; counts beyond K are reported for a rerun with more
 // slots.  With `wend` (batches: leaves sorted world-major), the walk stops at
 // the first node whose first leaf lies past the query's world.
 // Counts of slots n_reg .. n_all - 1 are cleared, so a scan over all n_all
 // slots needs no host-side n_reg.
-__global__ void k_traverse(const Counters* __restrict__ cnt, const float4* __restrict__ nodes, uint32_t N, int L,
-                           const uint32_t* __restrict__ wend_of_leaf, uint32_t K, uint64_t* __restrict__ slots,
-                           uint32_t* __restrict__ counts, uint32_t n_all, Counters* cnt_rw) {
-  uint32_t n = cnt->n_reg;
-  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) {
-    if (s < n_all) counts[s] = 0u;
-    return;
-  }
-  float4 qlo = nodes[2 * (uint64_t)(N + s)], qhi = nodes[2 * (uint64_t)(N + s) + 1];
-  uint32_t body = f2u(qlo.w);
-  uint32_t wend = wend_of_leaf ? wend_of_leaf[s] : 0xffffffffu;
-  uint32_t c = 0;
-  uint64_t* my = slots + (uint64_t)s * K;
-  uint32_t k = tree_rope(N + s);
-  while (k) {
-    if (wend_of_leaf) {
-      int d = 31 - __clz(k);
-      uint32_t first = (k << (L - d)) - N;
-      if (first >= wend) break;
-    }
-    float4 lo = nodes[2 * (uint64_t)k], hi = nodes[2 * (uint64_t)k + 1];
-    bool ov = qlo.x <= hi.x && qlo.y <= hi.y && qlo.z <= hi.z && lo.x <= qhi.x && lo.y <= qhi.y && lo.z <= qhi.z;
-    if (ov) {
-      if (k >= N) {
-        uint32_t other = f2u(lo.w);
-        uint32_t a = body < other ? body : other, b = body < other ? other : body;
-        if (c < K) my[c] = ((uint64_t)a << 32) | b;
-        ++c;
-        k = tree_rope(k);
-      } else {
-        k <<= 1;
-      }
+// W leaves per lane (W t .. W t + W - 1), stepped together: each step
+// issues all node loads before any is used, so a lane keeps W dependent
+// chains in flight (W = 2 measured best; 3 and 4 slower).
+struct LeafWalk {
+  float4 qlo, qhi;
+  uint32_t body, wend, c, k, s;
+  uint64_t* my;
+};
+__device__ __forceinline__ void walk_begin(LeafWalk& w, uint32_t s, const float4* __restrict__ nodes, uint32_t N,
+                                           const uint32_t* __restrict__ wend_of_leaf, uint32_t K,
+                                           uint64_t* __restrict__ slots, bool valid) {
+  w.s = s;
+  w.c = 0;
+  w.k = 0;
+  if (!valid) return;
+  w.qlo = nodes[2 * (uint64_t)(N + s)];
+  w.qhi = nodes[2 * (uint64_t)(N + s) + 1];
+  w.body = f2u(w.qlo.w);
+  w.wend = wend_of_leaf ? wend_of_leaf[s] : 0xffffffffu;
+  w.my = slots + (uint64_t)s * K;
+  w.k = tree_rope(N + s);
+}
+__device__ __forceinline__ void walk_step(LeafWalk& w, float4 lo, float4 hi, uint32_t N, uint32_t K) {
+  bool ov = w.qlo.x <= hi.x && w.qlo.y <= hi.y && w.qlo.z <= hi.z && lo.x <= w.qhi.x && lo.y <= w.qhi.y &&
+            lo.z <= w.qhi.z;
+  if (ov) {
+    if (w.k >= N) {
+      uint32_t other = f2u(lo.w);
+      uint32_t a = w.body < other ? w.body : other, b = w.body < other ? other : w.body;
+      if (w.c < K) w.my[w.c] = ((uint64_t)a << 32) | b;
+      ++w.c;
+      w.k = tree_rope(w.k);
     } else {
-      k = tree_rope(k);
+      w.k <<= 1;
+    }
+  } else {
+    w.k = tree_rope(w.k);
+  }
+}
+__device__ __forceinline__ void walk_world_stop(LeafWalk& w, const uint32_t* __restrict__ wend_of_leaf, uint32_t N,
+                                                int L) {
+  if (wend_of_leaf && w.k) {
+    int d = 31 - __clz(w.k);
+    uint32_t first = (w.k << (L - d)) - N;
+    if (first >= w.wend) w.k = 0;
+  }
+}
+template <int W>
+__global__ void k_traverse(const Counters* __restrict__ cnt, const float4* __restrict__ nodes, uint32_t N, int L,
+                             const uint32_t* __restrict__ wend_of_leaf, uint32_t K, uint64_t* __restrict__ slots,
+                             uint32_t* __restrict__ counts, uint32_t n_all, Counters* cnt_rw) {
+  const uint32_t n = cnt->n_reg;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (W * t >= n_all) return;
+  LeafWalk w[W];
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    walk_begin(w[i], W * t + i, nodes, N, wend_of_leaf, K, slots, W * t + i < n);
+    walk_world_stop(w[i], wend_of_leaf, N, L);
+  }
+  for (;;) {
+    uint32_t any = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) any |= w[i].k;
+    if (!any) break;
+    float4 lo[W], hi[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      lo[i] = hi[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (w[i].k) {
+        lo[i] = nodes[2 * (uint64_t)w[i].k];
+        hi[i] = nodes[2 * (uint64_t)w[i].k + 1];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      if (w[i].k) {
+        walk_step(w[i], lo[i], hi[i], N, K);
+        walk_world_stop(w[i], wend_of_leaf, N, L);
+      }
     }
   }
-  counts[s] = c < K ? c : K;
-  if (c > K) atomicMax(&cnt_rw->max_leaf, c);
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    if (W * t + i < n_all) {
+      counts[W * t + i] = w[i].c < K ? w[i].c : K;
+      if (w[i].c > K) atomicMax(&cnt_rw->max_leaf, w[i].c);
+    }
+  }
 }
 
 // Leaf slots -> dense keys at the scanned offsets; keys past `cap` are

@@ -239,7 +239,7 @@ int broadphase_capacity(bch_context* c, uint32_t n, const uint32_t* d_world_offs
   Counters* cnt = c->counters.as<Counters>();
   uint64_t cap = c->P_cap;
   ENSURE(c, c->slots, (size_t)n * c->K * sizeof(uint64_t));
-  hipLaunchKernelGGL(k_traverse, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, nodes, N, L, wend, c->K,
+  hipLaunchKernelGGL(k_traverse<2>, dim3(blocks((n + 1) / 2)), dim3(kBlock), 0, s, cnt, nodes, N, L, wend, c->K,
                      c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), n, cnt);
   mark(c, 3);
   int rc = dev_scan(c, c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), n, c->scan_part, s);
@@ -318,7 +318,7 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   if (c->cap_mode) return broadphase_capacity(c, n, d_world_offsets, nodes, N, L, wend, wbits);
   for (int attempt = 0; attempt < 3; ++attempt) {
     ENSURE(c, c->slots, (size_t)n * c->K * sizeof(uint64_t));
-    hipLaunchKernelGGL(k_traverse, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, nodes, N, L, wend, c->K,
+    hipLaunchKernelGGL(k_traverse<2>, dim3(blocks((n + 1) / 2)), dim3(kBlock), 0, s, cnt, nodes, N, L, wend, c->K,
                        c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), n, cnt);
     mark(c, 3);
     HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
