@@ -108,6 +108,30 @@ def test_mixed_convex_pass(bpa, scenes, oracle, ctx):
     assert np.array_equal((routes & 8) != 0, path == 2), "EPA route differs from the oracle's convex_convex"
 
 
+def test_large_hull_pass_defers_epa_to_the_one_pair_kernel(bpa, scenes, oracle, ctx):
+    """Hulls of 65-160 vertices: their EPA pairs leave the two-pair kernel's
+    fast path (hulls of at most 64 vertices) and rerun in the one-pair
+    kernel, hulls past 128 vertices through its memory-resident support;
+    every contact must still match the oracle."""
+    _flush("case: mixed_convex n=4000 seed=21, hulls of 65-160 points")
+    sc = scenes.mixed_convex(4000, 21, mix=(0.25, 0.15, 0.1, 0.5), hull_pts=(65, 160))
+    ref_pairs, ref_cnt, ref_pts, path = _oracle_pass(oracle, sc, MARGIN, "sweep")
+    w = bpa.CollisionWorld(sc, ctx)
+    w.collide(MARGIN)
+    pairs, offs, cont = w.results()
+    assert np.array_equal(pairs, ref_pairs)
+    routes = w.routes()
+    hc = sc["hull_count"].astype(np.int64)
+    big = (hc[pairs[:, 0]] > 64) | (hc[pairs[:, 1]] > 64)
+    huge = (hc[pairs[:, 0]] > 128) | (hc[pairs[:, 1]] > 128)
+    epa = (routes & 8) != 0
+    _flush(f"EPA pairs {int(epa.sum())}, with a hull past 64 vertices {int((epa & big).sum())}, "
+           f"past 128 {int((epa & huge).sum())}")
+    assert int((epa & big).sum()) > 20 and int((epa & huge).sum()) > 5
+    assert np.array_equal(epa, path == 2)
+    compare_contacts(ref_cnt, ref_pts, offs, cont, label="large_hulls")
+
+
 def test_collide_batch_matches_oracle(bpa, scenes, oracle, ctx):
     """bch_collide_batch on caller-given pairs (unsorted, with repeats)."""
     _flush("case: collide_batch mixed 5000")
