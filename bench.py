@@ -402,6 +402,8 @@ def main():
     ap.add_argument("--only-pass", action="store_true",
                     help="skip the host-buffer, captured and manifold legs (for kernel traces of the pass alone)")
     ap.add_argument("--worlds", type=int, default=4096)
+    ap.add_argument("--tuning", default="",
+                    help="fixed launch knobs, e.g. hull_lanes=8,epa_grid=65536,np_grid=4096, instead of autotuning")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -425,8 +427,16 @@ def main():
     sc = bpa.scenes.mixed_convex(args.bodies, 2)
     log(f"scene: {args.bodies} bodies, cube side {sc['side']:.1f} m, generated in {time.time() - t0:.1f} s")
     w = bpa.CollisionWorld(sc)
-    tuned = w.autotune(args.margin, trials=3)
-    log(f"autotuned: {tuned}")
+    if args.tuning:
+        # fixed launch knobs instead of the autotuner (profiling runs that
+        # want every pass at the same launch shapes)
+        knobs = {k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))}
+        w.ctx.set_tuning(**knobs)
+        w.collide(args.margin)  # sizes the per-leaf pair slots
+        tuned = dict(w.ctx.tuning(), fixed=True)
+    else:
+        tuned = w.autotune(args.margin, trials=3)
+    log(f"tuning: {tuned}")
     for k in range(args.warmup):
         P, Cn = w.collide(args.margin)
         log(f"warmup {k}: {P} pairs, {Cn} contacts")
