@@ -395,7 +395,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--bodies", type=int, default=1_000_000)
     ap.add_argument("--margin", type=float, default=0.04)
-    ap.add_argument("--cpu-bodies", type=int, default=250_000)
+    ap.add_argument("--cpu-bodies", type=int, default=500_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-diff", action="store_true")
