@@ -75,6 +75,8 @@ def lib():
         L.oracle_brute_pairs.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         L.oracle_sweep_pairs.restype = C.c_int64
         L.oracle_sweep_pairs.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
+        L.oracle_gjk_exit_f32.argtypes = [sp, f32p, f32p, sp, f32p, f32p, C.c_float, C.POINTER(C.c_int),
+                                          C.POINTER(C.c_int)]
         L.oracle_dbvt_pairs.restype = C.c_int64
         L.oracle_dbvt_pairs.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64]
         L.oracle_collide_pairs_f32.argtypes = [scp, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -240,6 +242,15 @@ def gjk_contact(a: Shape, pa, qa, b: Shape, pb, qb, threshold, seed=None, f64=Fa
                                      None if seed is None else _f32(seed, 3), threshold, out, simp)
     o = np.array(out[:], np.float32)
     return k, o[0], o[1:4], o[4:7], np.array(simp[:], np.float32).reshape(4, 3)
+
+
+def gjk_exit(a: Shape, pa, qa, b: Shape, pb, qb, threshold):
+    """How the f32 gjk_contact walk ended -> (kind, iterations, capped)."""
+    ac, bc = a.c(), b.c()
+    it, cap = C.c_int(0), C.c_int(0)
+    k = lib().oracle_gjk_exit_f32(C.byref(ac), _f32(pa, 3), _f32(qa, 4), C.byref(bc), _f32(pb, 3), _f32(qb, 4),
+                                  threshold, C.byref(it), C.byref(cap))
+    return k, it.value, bool(cap.value)
 
 
 def gjk_intersect(a: Shape, pa, qa, b: Shape, pb, qb):

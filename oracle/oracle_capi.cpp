@@ -358,6 +358,19 @@ int oracle_gjk_contact_f32(const or_shape* a, const float* pa, const float* qa, 
   return g.kind;
 }
 
+// The walk's exit of gjk_contact in f32: returns the outcome kind and sets
+// *iterations and *capped (the 64-iteration cap, reported as separated).
+int oracle_gjk_exit_f32(const or_shape* a, const float* pa, const float* qa, const or_shape* b, const float* pb,
+                        const float* qb, float threshold, int* iterations, int* capped) {
+  std::vector<V3<float>> sa_, sb_;
+  Shape<float> sa = from_desc<float>(a, sa_), sb = from_desc<float>(b, sb_);
+  PairPose<float> pr{&sa, v3p<float>(pa), q4p<float>(qa), &sb, v3p<float>(pb), q4p<float>(qb)};
+  GjkOut<float> g = gjk_contact(pr, (const V3<float>*)nullptr, threshold);
+  *iterations = g.iterations;
+  *capped = g.capped ? 1 : 0;
+  return g.kind;
+}
+
 int oracle_gjk_contact_f64(const or_shape* a, const double* pa, const double* qa, const or_shape* b,
                            const double* pb, const double* qb, const double* seed, double threshold, double* out7) {
   std::vector<V3<double>> sa_, sb_;
