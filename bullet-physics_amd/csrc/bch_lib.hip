@@ -53,6 +53,7 @@ struct DevBuf {
 };
 
 constexpr int kBlock = 256;
+constexpr int kTraverseW = 1;  // tree leaves per lane in k_traverse
 constexpr uint32_t kEpaDeferGrid = 16384;  // one-pair EPA blocks for the deferred entries (~4% of the queue)
 inline uint32_t blocks(uint64_t n, int b = kBlock) { return (uint32_t)((n + b - 1) / b); }
 
@@ -178,7 +179,7 @@ int dev_scan(bch_context* c, const uint32_t* in, uint32_t* out, uint64_t n, DevB
 // and (k1, v1); *in_second tells which pair holds the result.
 template <class K>
 int dev_radix_sort(bch_context* c, K* k0, K* k1, uint32_t* v0, uint32_t* v1, uint64_t n, const int* shifts,
-                   int nshift, hipStream_t s, bool* in_second, const uint32_t* n_dev = nullptr) {
+                   int nshift, hipStream_t s, bool* in_second, const uint32_t* n_dev = nullptr, int pack = 0) {
   *in_second = false;
   if (n <= 1 || nshift == 0) return BCH_OK;
   if (n > 0xffffffffull) return fail(c, BCH_INVALID_ARG, "sort longer than 2^32 - 1");
@@ -190,16 +191,16 @@ int dev_radix_sort(bch_context* c, K* k0, K* k1, uint32_t* v0, uint32_t* v1, uin
   uint32_t* vin = v0;
   uint32_t* vout = v1;
   for (int p = 0; p < nshift; ++p) {
-    hipLaunchKernelGGL(k_rs_hist<K>, dim3(tiles), dim3(kRsThreads), 0, s, kin, (uint32_t)n, n_dev, shifts[p], tiles,
-                       c->rs_hist.as<uint32_t>());
+    hipLaunchKernelGGL(k_rs_hist<K>, dim3(tiles), dim3(kRsThreads), 0, s, kin, (uint32_t)n, n_dev, shifts[p], pack,
+                       tiles, c->rs_hist.as<uint32_t>());
     int rc = dev_scan(c, c->rs_hist.as<uint32_t>(), c->rs_offs.as<uint32_t>(), (uint64_t)256 * tiles, c->scan_part, s);
     if (rc) return rc;
     if (v0)
       hipLaunchKernelGGL((k_rs_scatter<K, true>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n, n_dev,
-                         shifts[p], tiles, c->rs_offs.as<uint32_t>(), kout, vout);
+                         shifts[p], pack, tiles, c->rs_offs.as<uint32_t>(), kout, vout);
     else
       hipLaunchKernelGGL((k_rs_scatter<K, false>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n, n_dev,
-                         shifts[p], tiles, c->rs_offs.as<uint32_t>(), kout, vout);
+                         shifts[p], pack, tiles, c->rs_offs.as<uint32_t>(), kout, vout);
     std::swap(kin, kout);
     std::swap(vin, vout);
   }
@@ -220,13 +221,12 @@ int bits_for(uint64_t v) {
   return b;
 }
 
-// The pair sort's digits: keys are i << 32 | j with i, j < n, so only the low
-// bits of each half vary.
-int pair_key_shifts(uint32_t n, int* ps) {
+// The pair sort's digits: keys are i << 32 | j with i, j < n < 2^b, sorted
+// as the packed 2b-bit number i << b | j (rs_digit's pack = b).
+int pair_key_shifts(uint32_t n, int* ps, int* pack) {
   int bn = bits_for(n);
-  int np = digit_shifts(0, bn, ps);
-  np += digit_shifts(32, 32 + bn, ps + np);
-  return np;
+  *pack = bn;
+  return digit_shifts(0, 2 * bn, ps);
 }
 
 // Capacity-mode tail of the broadphase: one traversal with the current slot
@@ -239,7 +239,7 @@ int broadphase_capacity(bch_context* c, uint32_t n, const uint32_t* d_world_offs
   Counters* cnt = c->counters.as<Counters>();
   uint64_t cap = c->P_cap;
   ENSURE(c, c->slots, (size_t)n * c->K * sizeof(uint64_t));
-  hipLaunchKernelGGL(k_traverse<2>, dim3(blocks((n + 1) / 2)), dim3(kBlock), 0, s, cnt, nodes, N, L, wend, c->K,
+  hipLaunchKernelGGL(k_traverse<kTraverseW>, dim3(blocks((n + kTraverseW - 1) / kTraverseW)), dim3(kBlock), 0, s, cnt, nodes, N, L, wend, c->K,
                      c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), n, cnt);
   mark(c, 3);
   int rc = dev_scan(c, c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), n, c->scan_part, s);
@@ -262,10 +262,11 @@ int broadphase_capacity(bch_context* c, uint32_t n, const uint32_t* d_world_offs
                      (uint32_t)cap, c->pair_keys.as<uint64_t>(), cnt);
   c->P = cap;
   int ps[16];
-  int np = pair_key_shifts(n, ps);
+  int pack = 0;
+  int np = pair_key_shifts(n, ps, &pack);
   bool palt = false;
   rc = dev_radix_sort<uint64_t>(c, c->pair_keys.as<uint64_t>(), c->pair_keys_alt.as<uint64_t>(), nullptr, nullptr, cap,
-                                ps, np, s, &palt, &cnt->n_pairs);
+                                ps, np, s, &palt, &cnt->n_pairs, pack);
   if (rc) return rc;
   if (palt) std::swap(c->pair_keys, c->pair_keys_alt);
   return BCH_OK;
@@ -318,7 +319,7 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   if (c->cap_mode) return broadphase_capacity(c, n, d_world_offsets, nodes, N, L, wend, wbits);
   for (int attempt = 0; attempt < 3; ++attempt) {
     ENSURE(c, c->slots, (size_t)n * c->K * sizeof(uint64_t));
-    hipLaunchKernelGGL(k_traverse<2>, dim3(blocks((n + 1) / 2)), dim3(kBlock), 0, s, cnt, nodes, N, L, wend, c->K,
+    hipLaunchKernelGGL(k_traverse<kTraverseW>, dim3(blocks((n + kTraverseW - 1) / kTraverseW)), dim3(kBlock), 0, s, cnt, nodes, N, L, wend, c->K,
                        c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), n, cnt);
     mark(c, 3);
     HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
@@ -365,10 +366,11 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
                               hipMemcpyDeviceToDevice, s));
   if (c->P > 1) {
     int ps[16];
-    int np = pair_key_shifts(n, ps);
+    int pack = 0;
+    int np = pair_key_shifts(n, ps, &pack);
     bool palt = false;
     rc = dev_radix_sort<uint64_t>(c, c->pair_keys.as<uint64_t>(), c->pair_keys_alt.as<uint64_t>(), nullptr, nullptr,
-                                  c->P, ps, np, s, &palt);
+                                  c->P, ps, np, s, &palt, nullptr, pack);
     if (rc) return rc;
     if (palt) std::swap(c->pair_keys, c->pair_keys_alt);
   }

@@ -122,12 +122,20 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __r
   }
 }
 
+// With pack = b > 0 the key i << 32 | j (i, j < 2^b) is read as the 2b-bit
+// number i << b | j, so a pair sort needs ceil(2b / 8) digits, not
+// 2 ceil(b / 8).
 template <class K>
-__device__ __forceinline__ uint32_t rs_digit(K key, int shift) { return (uint32_t)((uint64_t)key >> shift) & 0xffu; }
+__device__ __forceinline__ uint32_t rs_digit(K key, int shift, int pack) {
+  uint64_t k = (uint64_t)key;
+  if (pack) k = (k & ((1ull << pack) - 1ull)) | ((k >> 32) << pack);
+  return (uint32_t)(k >> shift) & 0xffu;
+}
 
 template <class K>
 __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const K* __restrict__ keys, uint32_t n,
-                                                        const uint32_t* __restrict__ n_dev, int shift, uint32_t tiles,
+                                                        const uint32_t* __restrict__ n_dev, int shift, int pack,
+                                                        uint32_t tiles,
                                                         uint32_t* __restrict__ hist) {
   n = capped_count(n, n_dev);
   __shared__ uint32_t h[256];
@@ -136,7 +144,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const K* __restrict__ ke
   uint64_t base = (uint64_t)blockIdx.x * kRsTile;
   for (int r = 0; r < kRsRounds; ++r) {
     uint64_t i = base + (uint64_t)r * kRsThreads + threadIdx.x;
-    if (i < n) atomicAdd(&h[rs_digit(keys[i], shift)], 1u);
+    if (i < n) atomicAdd(&h[rs_digit(keys[i], shift, pack)], 1u);
   }
   __syncthreads();
   hist[(uint64_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
@@ -145,7 +153,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const K* __restrict__ ke
 template <class K, bool VALS>
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                            uint32_t n, const uint32_t* __restrict__ n_dev, int shift,
-                                                           uint32_t tiles, const uint32_t* __restrict__ offs,
+                                                           int pack, uint32_t tiles, const uint32_t* __restrict__ offs,
                                                            K* __restrict__ kout, uint32_t* __restrict__ vout) {
   n = capped_count(n, n_dev);
   __shared__ uint32_t running[256];
@@ -163,7 +171,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const K* __restrict__
     K key = valid ? kin[i] : (K)0;
     uint32_t val = 0u;
     if (VALS && valid) val = vin[i];
-    uint32_t d = rs_digit(key, shift);
+    uint32_t d = rs_digit(key, shift, pack);
     // lanes of this wavefront holding the same digit
     uint64_t peers = __ballot(valid);
 #pragma unroll
