@@ -25,10 +25,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-STAGES = ["aabb", "tree_build", "tree_traverse", "pair_list", "narrow_closed_gjk", "narrow_hull_gjk", "epa",
+STAGES = ["aabb", "tree_build", "tree_traverse", "pair_list", "narrow_fork", "narrow_gjk", "epa",
           "contact_compaction"]
-# stages that are one kernel launch each: the roofline candidates
-SINGLE_KERNEL = {"tree_traverse": "k_traverse", "narrow_hull_gjk": "k_np_hull", "epa": "k_epa_dual"}
+# stages that are one kernel launch each (EPA: k_epa_dual plus the short one-pair launch over
+# the entries it defers): the roofline candidates.  narrow_gjk holds k_np_hull and the
+# k_np launches running beside it on a second stream, so it is no single kernel.
+SINGLE_KERNEL = {"tree_traverse": "k_traverse", "epa": "k_epa_dual"}
 
 
 def log(msg):

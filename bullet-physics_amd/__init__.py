@@ -552,7 +552,7 @@ class CollisionWorld:
         _check(self.ctx.h, self.L.bch_sd_backward(self.ctx.h, d_pairs, n_pairs, d_grad_out, d_grad_pose, d_status))
 
     # stage names of timings(), in order
-    STAGES = ("aabb", "tree_build", "tree_traverse", "pair_list", "narrow_closed_gjk", "narrow_hull_gjk", "epa",
+    STAGES = ("aabb", "tree_build", "tree_traverse", "pair_list", "narrow_fork", "narrow_gjk", "epa",
               "contact_compaction")
 
     def routes(self):

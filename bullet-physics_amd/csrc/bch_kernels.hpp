@@ -1,7 +1,8 @@
 // bch_kernels.hpp -- HIP kernels of the collision pass (gfx950).
 //
-// Pass structure (one stream, no host work between stages except one
-// count read-back before the pair sort):
+// Pass structure (one stream, forked once so the one-lane narrowphase
+// classes run beside k_np_hull; no host work between stages except one count
+// read-back before the pair sort):
 //   k_aabb        body record -> world AABB + margin (fused), scene bounds,
 //                 big / NaN classification
 //   k_morton      30-bit Morton code of the AABB centre (world id above it)
@@ -9,7 +10,8 @@
 //   k_tree_*      implicit complete binary tree over the sorted leaves;
 //                 boxes unioned level by level in LDS
 //   k_traverse    each leaf walks the tree over the leaves to its right
-//                 (two leaves per lane): exact pair set without duplicates
+//                 (one leaf per lane, XCD-contiguous leaf ranges): exact pair
+//                 set without duplicates
 //   k_big         bodies with unbounded AABBs (planes) against their world
 //   radix sort    (bch_sort.hpp) pair keys i << 32 | j -> lexicographic order
 //   k_np<CLS>     one lane per pair, one kernel per shape class
@@ -80,9 +82,10 @@ __global__ void k_pack_bodies(uint32_t n, const float* __restrict__ pos, const f
                               const uint8_t* __restrict__ type, const float* __restrict__ params,
                               const uint32_t* __restrict__ hull_first, const uint32_t* __restrict__ hull_count,
                               const float4* __restrict__ hull, const uint32_t* __restrict__ world_of,
-                              BodyRec* __restrict__ out) {
+                              BodyRec* __restrict__ out, uint8_t* __restrict__ type_out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  type_out[i] = type[i];
   BodyRec b;
   b.px = pos[3 * i];
   b.py = pos[3 * i + 1];
@@ -471,8 +474,9 @@ __global__ void k_tree_up(float4* __restrict__ nodes, uint32_t first, int m) {
 
 // Leaf s walks the subtrees right of s in depth-first order;
 // every overlapping leaf is a pair found exactly once.  Keys i << 32 | j go to
-// the leaf's K slots; counts beyond K are reported for a rerun with more
-// slots.  With `wend` (batches: leaves sorted world-major), the walk stops at
+// the leaf's K slots, stored slot-major (slot k of leaf s at k * n_all + s,
+// so neighbouring leaves write and k_compact_slots reads neighbouring
+// words); counts beyond K are reported for a rerun with more slots.  With `wend` (batches: leaves sorted world-major), the walk stops at
 // the first node whose first leaf lies past the query's world.
 // Counts of slots n_reg .. n_all - 1 are cleared, so a scan over all n_all
 // slots needs no host-side n_reg.
@@ -483,12 +487,12 @@ __global__ void k_tree_up(float4* __restrict__ nodes, uint32_t first, int m) {
 // 1M leaves against 0.73 ms for W = 2, which the remap did not speed up).
 struct LeafWalk {
   float4 qlo, qhi;
-  uint32_t body, wend, c, k, s;
+  uint32_t body, wend, c, k, s, stride;
   uint64_t* my;
 };
 __device__ __forceinline__ void walk_begin(LeafWalk& w, uint32_t s, const float4* __restrict__ nodes, uint32_t N,
                                            const uint32_t* __restrict__ wend_of_leaf, uint32_t K,
-                                           uint64_t* __restrict__ slots, bool valid) {
+                                           uint64_t* __restrict__ slots, uint32_t stride, bool valid) {
   w.s = s;
   w.c = 0;
   w.k = 0;
@@ -497,7 +501,8 @@ __device__ __forceinline__ void walk_begin(LeafWalk& w, uint32_t s, const float4
   w.qhi = nodes[2 * (uint64_t)(N + s) + 1];
   w.body = f2u(w.qlo.w);
   w.wend = wend_of_leaf ? wend_of_leaf[s] : 0xffffffffu;
-  w.my = slots + (uint64_t)s * K;
+  w.my = slots + s;
+  w.stride = stride;
   w.k = tree_rope(N + s);
 }
 __device__ __forceinline__ void walk_step(LeafWalk& w, float4 lo, float4 hi, uint32_t N, uint32_t K) {
@@ -507,7 +512,7 @@ __device__ __forceinline__ void walk_step(LeafWalk& w, float4 lo, float4 hi, uin
     if (w.k >= N) {
       uint32_t other = f2u(lo.w);
       uint32_t a = w.body < other ? w.body : other, b = w.body < other ? other : w.body;
-      if (w.c < K) w.my[w.c] = ((uint64_t)a << 32) | b;
+      if (w.c < K) w.my[(uint64_t)w.c * w.stride] = ((uint64_t)a << 32) | b;
       ++w.c;
       w.k = tree_rope(w.k);
     } else {
@@ -544,7 +549,7 @@ __global__ void k_traverse(const Counters* __restrict__ cnt, const float4* __res
   LeafWalk w[W];
 #pragma unroll
   for (int i = 0; i < W; ++i) {
-    walk_begin(w[i], W * t + i, nodes, N, wend_of_leaf, K, slots, W * t + i < n);
+    walk_begin(w[i], W * t + i, nodes, N, wend_of_leaf, K, slots, n_all, W * t + i < n);
     walk_world_stop(w[i], wend_of_leaf, N, L);
   }
   for (;;) {
@@ -578,9 +583,10 @@ __global__ void k_traverse(const Counters* __restrict__ cnt, const float4* __res
   }
 }
 
-// Leaf slots -> dense keys at the scanned offsets; keys past `cap` are
-// dropped (the capacity-mode pass reports them as an overflow).
-__global__ void k_compact_slots(const Counters* __restrict__ cnt, uint32_t K, const uint64_t* __restrict__ slots,
+// Leaf slots (slot-major, stride = the traversal's n_all) -> dense keys at
+// the scanned offsets; keys past `cap` are dropped (the capacity-mode pass
+// reports them as an overflow).
+__global__ void k_compact_slots(const Counters* __restrict__ cnt, uint32_t stride, const uint64_t* __restrict__ slots,
                                 const uint32_t* __restrict__ counts, const uint32_t* __restrict__ offsets,
                                 uint64_t cap, uint64_t* __restrict__ keys) {
   int n = (int)cnt->n_reg;
@@ -588,7 +594,7 @@ __global__ void k_compact_slots(const Counters* __restrict__ cnt, uint32_t K, co
   if (s >= n) return;
   uint32_t c = counts[s], o = offsets[s];
   for (uint32_t k = 0; k < c; ++k)
-    if ((uint64_t)o + k < cap) keys[o + k] = slots[(uint64_t)s * K + k];
+    if ((uint64_t)o + k < cap) keys[o + k] = slots[(uint64_t)k * stride + s];
 }
 
 // Capacity mode: big-body pairs follow the tree pairs (whose count the scan
@@ -667,7 +673,7 @@ __device__ __forceinline__ uint32_t dev_count(uint32_t n, const uint32_t* n_dev)
 }
 
 __global__ void k_split_pairs(uint32_t P, const uint32_t* __restrict__ P_dev, const uint64_t* __restrict__ keys,
-                              const BodyRec* __restrict__ bodies, uint32_t* __restrict__ pairs,
+                              const uint8_t* __restrict__ btype, uint32_t* __restrict__ pairs,
                               uint32_t* __restrict__ cls, uint32_t* __restrict__ iota) {
   P = dev_count(P, P_dev);
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -676,7 +682,7 @@ __global__ void k_split_pairs(uint32_t P, const uint32_t* __restrict__ P_dev, co
   uint32_t i = (uint32_t)(k >> 32), j = (uint32_t)k;
   pairs[2 * p] = i;
   pairs[2 * p + 1] = j;
-  cls[p] = (uint32_t)pair_class((int)(bodies[i].type & 0xffu), (int)(bodies[j].type & 0xffu));
+  cls[p] = (uint32_t)pair_class((int)btype[i], (int)btype[j]);
   iota[p] = p;
 }
 

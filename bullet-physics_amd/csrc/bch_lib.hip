@@ -63,6 +63,11 @@ struct bch_context {
   int device = 0;
   hipStream_t stream = nullptr;      // the stream every call is ordered on
   hipStream_t own_stream = nullptr;  // the context's own stream
+  // Second stream (default priority) for the closed-form and non-hull GJK
+  // launches, which run beside the hull GJK: forked from and joined back to
+  // `stream` with the two events.
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::string err;
   hipEvent_t ev[9] = {};
   float timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -71,6 +76,7 @@ struct bch_context {
   uint32_t n = 0, n_worlds = 1, max_world = 0;
   bool batched = false;
   DevBuf bodies, hull, world_offsets, world_of, staging_a, staging_b, staging_c, staging_d, staging_e, staging_f;
+  DevBuf btype;  // u8 shape type per body (the pair split reads it, not the 64-B records)
 
   // pass buffers
   DevBuf aabb, flags, big_ids, counters, partials, keys, keys_alt, vals, vals_alt, nodes, wend, hull_ids;
@@ -239,8 +245,8 @@ int broadphase_capacity(bch_context* c, uint32_t n, const uint32_t* d_world_offs
   Counters* cnt = c->counters.as<Counters>();
   uint64_t cap = c->P_cap;
   ENSURE(c, c->slots, (size_t)n * c->K * sizeof(uint64_t));
-  hipLaunchKernelGGL(k_traverse<kTraverseW>, dim3(blocks((n + kTraverseW - 1) / kTraverseW)), dim3(kBlock), 0, s, cnt, nodes, N, L, wend, c->K,
-                     c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), n, cnt);
+  hipLaunchKernelGGL(k_traverse<kTraverseW>, dim3(blocks((n + kTraverseW - 1) / kTraverseW)), dim3(kBlock), 0, s, cnt,
+                     nodes, N, L, wend, c->K, c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), n, cnt);
   mark(c, 3);
   int rc = dev_scan(c, c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), n, c->scan_part, s);
   if (rc) return rc;
@@ -255,7 +261,7 @@ int broadphase_capacity(bch_context* c, uint32_t n, const uint32_t* d_world_offs
                      big_cap, cnt);
   ENSURE(c, c->pair_keys, cap * sizeof(uint64_t));
   ENSURE(c, c->pair_keys_alt, cap * sizeof(uint64_t));
-  hipLaunchKernelGGL(k_compact_slots, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, c->K, c->slots.as<uint64_t>(),
+  hipLaunchKernelGGL(k_compact_slots, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, n, c->slots.as<uint64_t>(),
                      c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), cap, c->pair_keys.as<uint64_t>());
   hipLaunchKernelGGL(k_append_big, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(blocks(big_cap), 1024u))),
                      dim3(kBlock), 0, s, c->leaf_offsets.as<uint32_t>() + n, c->big_pairs.as<uint64_t>(), big_cap,
@@ -319,8 +325,8 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   if (c->cap_mode) return broadphase_capacity(c, n, d_world_offsets, nodes, N, L, wend, wbits);
   for (int attempt = 0; attempt < 3; ++attempt) {
     ENSURE(c, c->slots, (size_t)n * c->K * sizeof(uint64_t));
-    hipLaunchKernelGGL(k_traverse<kTraverseW>, dim3(blocks((n + kTraverseW - 1) / kTraverseW)), dim3(kBlock), 0, s, cnt, nodes, N, L, wend, c->K,
-                       c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), n, cnt);
+    hipLaunchKernelGGL(k_traverse<kTraverseW>, dim3(blocks((n + kTraverseW - 1) / kTraverseW)), dim3(kBlock), 0, s,
+                       cnt, nodes, N, L, wend, c->K, c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), n, cnt);
     mark(c, 3);
     HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
@@ -359,7 +365,7 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   c->P = (uint64_t)preg + pbig;
   ENSURE(c, c->pair_keys, std::max<uint64_t>(c->P, 1) * sizeof(uint64_t));
   ENSURE(c, c->pair_keys_alt, std::max<uint64_t>(c->P, 1) * sizeof(uint64_t));
-  hipLaunchKernelGGL(k_compact_slots, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, c->K, c->slots.as<uint64_t>(),
+  hipLaunchKernelGGL(k_compact_slots, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, n, c->slots.as<uint64_t>(),
                      c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), c->P, c->pair_keys.as<uint64_t>());
   if (pbig)
     HIP_TRY(c, hipMemcpyAsync(c->pair_keys.as<uint64_t>() + preg, c->big_pairs.p, pbig * sizeof(uint64_t),
@@ -418,7 +424,7 @@ int narrow_split(bch_context* c) {
   hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
   if (P)
     hipLaunchKernelGGL(k_split_pairs, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, P_dev,
-                       c->pair_keys.as<uint64_t>(), c->bodies.as<BodyRec>(), c->pairs.as<uint32_t>(),
+                       c->pair_keys.as<uint64_t>(), c->btype.as<uint8_t>(), c->pairs.as<uint32_t>(),
                        c->cls.as<uint32_t>(), c->perm.as<uint32_t>());
   return BCH_OK;
 }
@@ -445,10 +451,16 @@ int narrow_finish(bch_context* c, const float* d_seeds) {
     hipLaunchKernelGGL(k_class_bounds, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, P_dev,
                        c->cls_alt.as<uint32_t>(), cnt);
     mark(c, 4);
+    // The one-lane-per-pair classes run on the side stream beside k_np_hull
+    // (disjoint pairs; both append to the EPA queue through one atomic
+    // counter): each kernel fills the other's idle issue slots and tail.
+    // Stage 4 -> 5 is then only the fork; 5 -> 6 covers both streams.
+    HIP_TRY(c, hipEventRecord(c->fork_ev, s));
+    HIP_TRY(c, hipStreamWaitEvent(c->side, c->fork_ev, 0));
     const uint32_t* perm = c->perm_alt.as<uint32_t>();
     uint32_t g = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(P), c->np_grid));
 #define BCH_NP(CLS)                                                                                          \
-  hipLaunchKernelGGL(k_np<CLS>, dim3(g), dim3(kBlock), 0, s, cnt, perm, c->pairs.as<uint32_t>(),           \
+  hipLaunchKernelGGL(k_np<CLS>, dim3(g), dim3(kBlock), 0, c->side, cnt, perm, c->pairs.as<uint32_t>(),           \
                      c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_seeds, c->ccount.as<uint32_t>(),     \
                      c->cslots.as<float4>(), c->epa_q.as<float4>(), cnt)
     BCH_NP(CLS_NONE);
@@ -458,6 +470,7 @@ int narrow_finish(bch_context* c, const float* d_seeds) {
     BCH_NP(CLS_BOX_BOX);
     BCH_NP(CLS_GJK);
 #undef BCH_NP
+    HIP_TRY(c, hipEventRecord(c->join_ev, c->side));
     mark(c, 5);
 #define BCH_NP_HULL(G)                                                                                     \
   hipLaunchKernelGGL(k_np_hull<G>, dim3(g), dim3(kBlock), 0, s, cnt, perm, c->pairs.as<uint32_t>(),        \
@@ -470,6 +483,7 @@ int narrow_finish(bch_context* c, const float* d_seeds) {
     else
       BCH_NP_HULL(16);
 #undef BCH_NP_HULL
+    HIP_TRY(c, hipStreamWaitEvent(s, c->join_ev, 0));
     mark(c, 6);
     // two pairs per wavefront; the entries that leave its fast path run
     // through the one-pair kernel from the start
@@ -527,6 +541,7 @@ int upload_bodies(bch_context* c, const bch_bodies* b, const uint32_t* world_of_
   hipStream_t s = c->stream;
   uint32_t n = b->n;
   ENSURE(c, c->bodies, (size_t)std::max<uint32_t>(n, 1) * sizeof(BodyRec));
+  ENSURE(c, c->btype, (size_t)std::max<uint32_t>(n, 1));
   ENSURE(c, c->staging_a, (size_t)std::max<uint32_t>(n, 1) * 3 * sizeof(float));
   ENSURE(c, c->staging_b, (size_t)std::max<uint32_t>(n, 1) * 4 * sizeof(float));
   ENSURE(c, c->staging_c, (size_t)std::max<uint32_t>(n, 1));
@@ -575,7 +590,7 @@ int upload_bodies(bch_context* c, const bch_bodies* b, const uint32_t* world_of_
   if (n)
     hipLaunchKernelGGL(k_pack_bodies, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->staging_a.as<float>(),
                        c->staging_b.as<float>(), c->staging_c.as<uint8_t>(), c->staging_d.as<float>(), hf, hc,
-                       c->hull.as<float4>(), wo, c->bodies.as<BodyRec>());
+                       c->hull.as<float4>(), wo, c->bodies.as<BodyRec>(), c->btype.as<uint8_t>());
   HIP_TRY(c, hipGetLastError());
   HIP_TRY(c, hipStreamSynchronize(s));
   return BCH_OK;
@@ -784,7 +799,7 @@ int run_depen_step(bch_context* c, float aabb_margin, float lr, float* d_pos, fl
   if (rc) return rc;
   const uint32_t* P_dev = &cnt->n_pairs;
   hipLaunchKernelGGL(k_split_pairs, dim3(blocks(cap)), dim3(kBlock), 0, s, (uint32_t)cap, P_dev,
-                     c->pair_keys.as<uint64_t>(), c->bodies.as<BodyRec>(), c->pairs.as<uint32_t>(),
+                     c->pair_keys.as<uint64_t>(), c->btype.as<uint8_t>(), c->pairs.as<uint32_t>(),
                      c->cls.as<uint32_t>(), c->perm.as<uint32_t>());
   constexpr int G = 16;
   uint32_t g = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(cap, kBlock / G), 8192u));
@@ -859,6 +874,15 @@ int bch_create(int device, bch_context** out) {
     return BCH_HIP_ERROR;
   }
   c->stream = c->own_stream;
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess) {
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return BCH_HIP_ERROR;
+  }
   for (auto& e : c->ev) (void)hipEventCreate(&e);
   *out = c;
   return BCH_OK;
@@ -873,7 +897,7 @@ void bch_destroy(bch_context* c) {
   DevBuf* all[] = {&c->bodies, &c->hull, &c->world_offsets, &c->world_of, &c->staging_a, &c->staging_b,
                    &c->staging_c, &c->staging_d, &c->staging_e, &c->staging_f, &c->aabb, &c->flags, &c->big_ids,
                    &c->counters, &c->partials, &c->keys, &c->keys_alt, &c->vals, &c->vals_alt, &c->nodes, &c->hull_ids,
-                   &c->wend, &c->leaf_counts, &c->leaf_offsets, &c->slots, &c->pair_keys,
+                   &c->btype, &c->wend, &c->leaf_counts, &c->leaf_offsets, &c->slots, &c->pair_keys,
                    &c->pair_keys_alt, &c->big_pairs, &c->pairs, &c->cls, &c->cls_alt, &c->perm, &c->perm_alt,
                    &c->ccount, &c->coffsets, &c->cslots, &c->contacts, &c->epa_q, &c->epa_q2,
                    &c->scan_part, &c->rs_hist, &c->rs_offs, &c->sd_out, &c->sd_status, &c->pen_pairs,
@@ -887,6 +911,9 @@ void bch_destroy(bch_context* c) {
                    &c->sw_radius, &c->sw_pose, &c->sw_hit, &c->sw_out, &c->sw_keys, &c->sw_order};
   for (DevBuf* b : all) b->release();
   for (auto& e : c->ev) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(c->fork_ev);
+  (void)hipEventDestroy(c->join_ev);
+  (void)hipStreamDestroy(c->side);
   (void)hipStreamDestroy(c->own_stream);
   delete c;
 }

@@ -72,9 +72,10 @@ int bch_synchronize(bch_context* ctx);
 /* Device times (ms) of the last bch_world_collide pass, 8 floats:
  * [0] world AABBs, [1] tree build (Morton codes, sort, nodes, refit),
  * [2] tree traversal (pair search), [3] pair list (planes, compaction,
- * sort, shape-class split), [4] closed-form and GJK pairs without a hull
- * (k_np kernels), [5] GJK pairs with a hull (k_np_hull), [6] EPA
- * (k_epa_wave), [7] contact compaction. */
+ * sort, shape-class split), [4] the fork of the narrowphase onto a second
+ * stream (near zero), [5] closed-form and GJK pairs, with and without a
+ * hull (k_np on the second stream beside k_np_hull), [6] EPA (k_epa_dual
+ * and k_epa_wave), [7] contact compaction. */
 int bch_last_timings(bch_context* ctx, float* out_ms);
 
 /* No reference counterpart: launch-shape knobs of the device passes.  They
