@@ -323,24 +323,24 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   }
   mark(c, 2);
   if (c->cap_mode) return broadphase_capacity(c, n, d_world_offsets, nodes, N, L, wend, wbits);
+  // The prefix offsets of the per-leaf counts are scanned before the host
+  // looks at the slot check (leaves past n_reg count 0), so one read-back
+  // serves both; a leaf with more pairs than slots reruns the walk.
+  uint32_t preg = 0;
   for (int attempt = 0; attempt < 3; ++attempt) {
     ENSURE(c, c->slots, (size_t)n * c->K * sizeof(uint64_t));
     hipLaunchKernelGGL(k_traverse<kTraverseW>, dim3(blocks((n + kTraverseW - 1) / kTraverseW)), dim3(kBlock), 0, s,
                        cnt, nodes, N, L, wend, c->K, c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), n, cnt);
     mark(c, 3);
+    rc = dev_scan(c, c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), n, c->scan_part, s);
+    if (rc) return rc;
     HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(&preg, c->leaf_offsets.as<uint32_t>() + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
     if (c->host_cnt.max_leaf <= c->K) break;
     c->K = std::max<uint32_t>(c->K * 2, (c->host_cnt.max_leaf + 7) & ~7u);
   }
   if (c->host_cnt.max_leaf > c->K) return fail(c, BCH_HIP_ERROR, "per-leaf pair slots did not converge");
-  uint32_t nreg = c->host_cnt.n_reg;
-  // prefix offsets of the per-leaf counts
-  rc = dev_scan(c, c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), nreg, c->scan_part, s);
-  if (rc) return rc;
-  uint32_t preg = 0;
-  HIP_TRY(c, hipMemcpyAsync(&preg, c->leaf_offsets.as<uint32_t>() + nreg, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_TRY(c, hipStreamSynchronize(s));
   // big bodies (planes): brute force against their world
   uint32_t nbig = c->host_cnt.n_big;
   uint64_t big_cap = 0;
@@ -513,18 +513,28 @@ int narrow_finish(bch_context* c, const float* d_seeds) {
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, 1u, c->ccount.as<uint32_t>() + P, 0u);
   int rc = dev_scan(c, c->ccount.as<uint32_t>(), c->coffsets.as<uint32_t>(), P, c->scan_part, s);
   if (rc) return rc;
-  uint32_t total = 0;
-  HIP_TRY(c, hipMemcpyAsync(&total, c->coffsets.as<uint32_t>() + P, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_TRY(c, hipStreamSynchronize(s));
-  c->C = total;
-  ENSURE(c, c->contacts, std::max<uint64_t>(c->C, 1) * 7 * sizeof(float));
+  // Gather into the contact buffer as earlier passes sized it (at first one
+  // contact per pair), read the total with the counters, and gather again
+  // only when this pass has more contacts than that buffer holds.
+  if (c->contacts.bytes == 0) ENSURE(c, c->contacts, std::max<uint64_t>(P, 1) * 7 * sizeof(float));
+  uint64_t ccap = c->contacts.bytes / (7 * sizeof(float));
   if (P)
     hipLaunchKernelGGL(k_gather, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, (const uint32_t*)nullptr,
-                       c->ccount.as<uint32_t>(), c->coffsets.as<uint32_t>(), c->cslots.as<float4>(), c->C,
+                       c->ccount.as<uint32_t>(), c->coffsets.as<uint32_t>(), c->cslots.as<float4>(), ccap,
                        c->contacts.as<float>(), cnt);
   mark(c, 8);
+  uint32_t total = 0;
+  HIP_TRY(c, hipMemcpyAsync(&total, c->coffsets.as<uint32_t>() + P, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
   HIP_TRY(c, hipStreamSynchronize(s));
+  c->C = total;
+  if (total > ccap) {
+    ENSURE(c, c->contacts, (uint64_t)total * 7 * sizeof(float));
+    hipLaunchKernelGGL(k_gather, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, (const uint32_t*)nullptr,
+                       c->ccount.as<uint32_t>(), c->coffsets.as<uint32_t>(), c->cslots.as<float4>(), (uint64_t)total,
+                       c->contacts.as<float>(), cnt);
+    HIP_TRY(c, hipStreamSynchronize(s));
+  }
   if (c->host_cnt.errors & 2u) return fail(c, BCH_HIP_ERROR, "EPA polytope store overflow");
   return BCH_OK;
 }

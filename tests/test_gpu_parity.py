@@ -279,6 +279,48 @@ def test_empty_and_tiny_worlds(bpa, scenes, ctx):
     assert P in (0, 1)
 
 
+def _box_columns(cols=16, height=4, sink=0.01):
+    """cols x cols columns of `height` unit cubes resting face to face on the
+    plane y = 0, each sunk `sink` m into the one below: three face-face box
+    pairs of four points per column, more contacts than pairs."""
+    n = cols * cols * height + 1
+    sc = {"position": np.zeros((n, 3), np.float32),
+          "orientation": np.tile(np.float32([0, 0, 0, 1]), (n, 1)),
+          "shape_type": np.zeros(n, np.uint8), "shape_params": np.zeros((n, 4), np.float32),
+          "hull_first": np.zeros(n, np.uint32), "hull_count": np.zeros(n, np.uint32),
+          "hull_vertices": np.zeros((0, 3), np.float32)}
+    sc["shape_type"][0] = 4  # plane y = 0, normal +Y
+    sc["shape_params"][0] = (0.0, 1.0, 0.0, 0.0)
+    k = 1
+    for x in range(cols):
+        for z in range(cols):
+            for h in range(height):
+                sc["shape_type"][k] = 1  # box
+                sc["shape_params"][k, :3] = 0.5
+                sc["position"][k] = (1.5 * x, 0.5 - sink + h * (1.0 - sink), 1.5 * z)
+                k += 1
+    return sc
+
+
+def test_contact_buffer_regrows_when_contacts_outnumber_pairs(bpa, oracle, ctx):
+    """The eager pass gathers contacts into a buffer sized by earlier passes
+    (at first one contact per pair, with headroom) and gathers again once it
+    is short.  Face-resting boxes give about 1.9 contacts per pair, so the
+    first pass takes the regrow path and the second the sized buffer; both
+    must equal the oracle."""
+    _flush("case: resting box columns, contacts > pairs")
+    sc = _box_columns()
+    ref_pairs, ref_cnt, ref_pts, _ = _oracle_pass(oracle, sc, MARGIN)
+    assert ref_cnt.sum() > 1.5 * ref_pairs.shape[0]
+    w = bpa.CollisionWorld(sc, ctx)
+    for rep in range(2):
+        P, Cn = w.collide(MARGIN)
+        pairs, offs, cont = w.results()
+        assert np.array_equal(pairs, ref_pairs)
+        assert Cn == int(ref_cnt.sum())
+        compare_contacts(ref_cnt, ref_pts, offs, cont, label=f"box columns, pass {rep}")
+
+
 def test_batched_worlds_equal_plain_per_world(bpa, scenes, oracle, ctx):
     """Independent worlds in one pass == each world collided alone (plain path)."""
     _flush("case: batched 64 worlds")
