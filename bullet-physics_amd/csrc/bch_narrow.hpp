@@ -106,6 +106,9 @@ BCH_HD bool plane_convex(V3 pn, float pc, V3 pp, Q4 pq, const ShapeRef& cv, V3 c
 }
 
 // ---------------------------------------------------------------- box-box --
+// Every array of the clip and cull below is indexed by compile-time
+// constants only (unrolled loops; a store at a run-time position is a select
+// over the eight slots), so the arrays stay in registers instead of scratch.
 struct ClipPoly {
   float x[8], y[8];
   int n;
@@ -113,23 +116,30 @@ struct ClipPoly {
 };
 
 BCH_HD void clip_push(ClipPoly& o, float a, float b) {
-  if (o.n >= 8) {
-    o.overflow = true;
-    return;
+  // past eight vertices: flagged, never written past
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    bool at = k == o.n;
+    o.x[k] = at ? a : o.x[k];
+    o.y[k] = at ? b : o.y[k];
   }
-  o.x[o.n] = a;
-  o.y[o.n] = b;
-  ++o.n;
+  o.overflow = o.overflow || o.n >= 8;
+  o.n = o.n < 8 ? o.n + 1 : o.n;
 }
 
 // box_box.rs:92-114: keep `sign * coord[axis] < h`.
 BCH_HD void clip_poly(const ClipPoly& s, int axis, float sign, float h, ClipPoly& o) {
   o.n = 0;
   o.overflow = s.overflow;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) o.x[k] = o.y[k] = 0.0f;
+#pragma unroll
   for (int i = 0; i < 8; ++i) {
     if (i >= s.n) break;
-    int j = (i + 1 == s.n) ? 0 : i + 1;
-    float cx = s.x[i], cy = s.y[i], nx = s.x[j], ny = s.y[j];
+    // the next vertex wraps to 0 after the last (i = 7 only when n = 8)
+    bool wrap = i + 1 == s.n;
+    float cx = s.x[i], cy = s.y[i];
+    float nx = wrap ? s.x[0] : s.x[i < 7 ? i + 1 : 0], ny = wrap ? s.y[0] : s.y[i < 7 ? i + 1 : 0];
     float ca = axis == 0 ? cx : cy, na = axis == 0 ? nx : ny;
     float cb = axis == 0 ? cy : cx, nb = axis == 0 ? ny : nx;
     bool cur_in = sign * ca < h;
@@ -302,8 +312,14 @@ BCH_HD bool box_box(V3 ha, V3 p1, Q4 oa, V3 hb, V3 p2, Q4 ob, Manifold4& out) {
   float n11 = m11 * det1, n12 = m12 * det1, n21 = m21 * det1, n22 = m22 * det1;
   V3 pts[8];
   float deps[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    pts[k] = V3{0.0f, 0.0f, 0.0f};
+    deps[k] = 0.0f;
+  }
   int np = 0;
   float sdep = comp(sa, code_n);
+#pragma unroll
   for (int i = 0; i < 8; ++i) {
     if (i >= buf.n) break;
     float rx = buf.x[i], ry = buf.y[i];
@@ -312,22 +328,24 @@ BCH_HD bool box_box(V3 ha, V3 p1, Q4 oa, V3 hb, V3 p2, Q4 ob, Manifold4& out) {
     V3 point = (center + kk1 * rba1) + kk2 * rba2;
     float dep = sdep - dot(normal2, point);
     if (dep >= 0.0f) {
-      pts[np] = point;
-      deps[np] = dep;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        bool at = k == np;
+        pts[k] = V3{at ? point.x : pts[k].x, at ? point.y : pts[k].y, at ? point.z : pts[k].z};
+        deps[k] = at ? dep : deps[k];
+      }
       ++np;
     }
   }
   if (np == 0) return false;
   // cull_points (box_box.rs:131-174): deepest (last maximum), then the rest by
-  // descending squared distance to it (stable), keep 4.
-  int sel[4];
-  int nsel;
-  if (np <= 4) {
-    for (int i = 0; i < 4; ++i) sel[i] = i;
-    nsel = np;
-  } else {
+  // descending squared distance to it (stable insertion sort), keep 4.
+  int sel[4] = {0, 1, 2, 3};
+  int nsel = np;
+  if (np > 4) {
     int i0 = 0;
     float dd = deps[0];
+#pragma unroll
     for (int i = 1; i < 8; ++i) {
       if (i >= np) break;
       if (deps[i] >= dd) {
@@ -335,29 +353,53 @@ BCH_HD bool box_box(V3 ha, V3 p1, Q4 oa, V3 hb, V3 p2, Q4 ob, Manifold4& out) {
         dd = deps[i];
       }
     }
+    V3 p0 = pts[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+      bool at = k == i0;
+      p0 = V3{at ? pts[k].x : p0.x, at ? pts[k].y : p0.y, at ? pts[k].z : p0.z};
+    }
     int ri[8];
     float rd[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      ri[k] = 0;
+      rd[k] = 0.0f;
+    }
     int rn = 0;
+#pragma unroll
     for (int i = 0; i < 8; ++i) {
       if (i >= np) break;
       if (i != i0) {
-        ri[rn] = i;
-        rd[rn] = len2(pts[i] - pts[i0]);
+        float d2 = len2(pts[i] - p0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          bool at = k == rn;
+          ri[k] = at ? i : ri[k];
+          rd[k] = at ? d2 : rd[k];
+        }
         ++rn;
       }
     }
+    // The insertion sort step by step: item j moves down while the item
+    // before it compares `<` (the same comparisons in the same order, so
+    // equal and NaN keys land where the loop of the reference puts them).
+#pragma unroll
     for (int j = 1; j < 8; ++j) {
       if (j >= rn) break;
       int ki = ri[j];
       float kd = rd[j];
-      int k = j;
-      while (k > 0 && rd[k - 1] < kd) {
-        ri[k] = ri[k - 1];
-        rd[k] = rd[k - 1];
-        --k;
+      bool moving = true;
+#pragma unroll
+      for (int k = j; k > 0; --k) {
+        bool shift = moving && rd[k - 1] < kd;
+        bool place = moving && !shift;
+        ri[k] = shift ? ri[k - 1] : (place ? ki : ri[k]);
+        rd[k] = shift ? rd[k - 1] : (place ? kd : rd[k]);
+        moving = shift;
       }
-      ri[k] = ki;
-      rd[k] = kd;
+      ri[0] = moving ? ki : ri[0];
+      rd[0] = moving ? kd : rd[0];
     }
     sel[0] = i0;
     sel[1] = ri[0];
@@ -365,11 +407,20 @@ BCH_HD bool box_box(V3 ha, V3 p1, Q4 oa, V3 hb, V3 p2, Q4 ob, Manifold4& out) {
     sel[3] = ri[2];
     nsel = 4;
   }
+#pragma unroll
   for (int i = 0; i < 4; ++i) {
     if (i >= nsel) break;
-    V3 pos = pts[sel[i]] + pa_ref;
-    if (!a_ref) pos = pos + normal * deps[sel[i]];
-    out.push(pos, normal, deps[sel[i]]);
+    V3 pt = pts[0];
+    float dp = deps[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+      bool at = k == sel[i];
+      pt = V3{at ? pts[k].x : pt.x, at ? pts[k].y : pt.y, at ? pts[k].z : pt.z};
+      dp = at ? deps[k] : dp;
+    }
+    V3 pos = pt + pa_ref;
+    if (!a_ref) pos = pos + normal * dp;
+    out.push(pos, normal, dp);
   }
   if (buf.overflow) out.n = -1;  // more than 8 clip vertices: report, never write past
   return true;
