@@ -49,6 +49,9 @@ struct MfPoint {
   uint32_t life;
 };
 
+// The cache's point array is indexed by compile-time constants only
+// (unrolled loops; a store or load at a run-time position is a select over
+// the slots), so a manifold stays in registers instead of scratch.
 struct MfMan {
   MfPoint pt[kMfCache + 4];  // capture appends up to four before truncating
   int n;
@@ -66,7 +69,27 @@ __device__ __forceinline__ float rotation_angle_w(float w) {  // manifold.rs:154
   return 2.0f * acosf(w);
 }
 
+// dst = v where c (component-wise: no struct select)
+__device__ __forceinline__ void mf_put(MfPoint& dst, bool c, const MfPoint& v) {
+  dst.la = sel(c, v.la, dst.la);
+  dst.lb = sel(c, v.lb, dst.lb);
+  dst.n = sel(c, v.n, dst.n);
+  dst.pen = c ? v.pen : dst.pen;
+  dst.imp = c ? v.imp : dst.imp;
+  dst.life = c ? v.life : dst.life;
+}
+
+__device__ __forceinline__ void mf_clear_points(MfMan& m) {
+  MfPoint z;
+  z.la = z.lb = z.n = splat(0.0f);
+  z.pen = z.imp = 0.0f;
+  z.life = 0u;
+#pragma unroll
+  for (int k = 0; k < kMfCache + 4; ++k) m.pt[k] = z;
+}
+
 __device__ __forceinline__ void mf_empty(MfMan& m) {
+  mf_clear_points(m);
   m.n = 0;
   m.dpa = m.dpb = splat(0.0f);
   m.dqa = m.dqb = Q4{0.0f, 0.0f, 0.0f, 1.0f};
@@ -84,6 +107,8 @@ __device__ __forceinline__ void mf_load(const float4* __restrict__ e, MfMan& m) 
   m.dqa = Q4{b.x, b.y, b.z, b.w};
   m.dpb = V3{c.x, c.y, c.z};
   m.dqb = Q4{d.x, d.y, d.z, d.w};
+  mf_clear_points(m);
+#pragma unroll
   for (int k = 0; k < kMfCache; ++k) {
     if (k >= m.n) break;
     float4 p0 = e[3 * k], p1 = e[3 * k + 1], p2 = e[3 * k + 2];
@@ -97,6 +122,7 @@ __device__ __forceinline__ void mf_load(const float4* __restrict__ e, MfMan& m) 
 }
 
 __device__ __forceinline__ void mf_store(const MfMan& m, float4* __restrict__ e) {
+#pragma unroll
   for (int k = 0; k < kMfCache; ++k) {
     if (k < m.n) {
       const MfPoint& p = m.pt[k];
@@ -118,6 +144,7 @@ __device__ __forceinline__ int mf_refresh(MfMan& m, V3 pa, Q4 qa, V3 pb, Q4 qb) 
   float thr = m.thr;
   float thr2 = thr * thr;
   int w = 0;
+#pragma unroll
   for (int r = 0; r < kMfCache; ++r) {
     if (r >= m.n) break;
     MfPoint p = m.pt[r];
@@ -125,12 +152,14 @@ __device__ __forceinline__ int mf_refresh(MfMan& m, V3 pa, Q4 qa, V3 pb, Q4 qb) 
     V3 wb = pb + qrot(qb, p.lb);
     V3 d = wa - wb;
     float pen = dot(d, p.n);
-    if (pen < -thr) continue;
     V3 lat = d - p.n * pen;
-    if (len2(lat) > thr2) continue;
+    bool keep = !(pen < -thr) && !(len2(lat) > thr2);
     p.pen = pen;
     p.life += 1u;
-    m.pt[w++] = p;
+    // survivor r goes to slot w <= r (slots above r are read later, never written)
+#pragma unroll
+    for (int k = 0; k <= r; ++k) mf_put(m.pt[k], keep && k == w, p);
+    w += keep ? 1 : 0;
   }
   m.n = w;
   return w;
@@ -159,6 +188,7 @@ __device__ __forceinline__ void mf_capture(MfMan& m, const float4* __restrict__ 
   m.thr = threshold;
   Q4 ia = qconj(qa), ib = qconj(qb);
   float thr2 = threshold * threshold;
+#pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (k >= n) break;
     float4 s0 = slots[2 * k], s1 = slots[2 * k + 1];
@@ -173,33 +203,41 @@ __device__ __forceinline__ void mf_capture(MfMan& m, const float4* __restrict__ 
     pt.pen = depth;
     pt.imp = 0.0f;
     pt.life = 0u;
+    // first cached point within the threshold with a like normal (its
+    // impulse is inherited), else a new slot at the end
     int slot = -1;
+    float imp = 0.0f;
+#pragma unroll
     for (int o = 0; o < kMfCache + 4; ++o) {
-      if (o >= m.n) break;
-      if (len2(m.pt[o].la - pt.la) < thr2 && dot(m.pt[o].n, pt.n) >= 0.9f) {
-        slot = o;
-        break;
-      }
+      bool hit = slot < 0 && o < m.n && len2(m.pt[o].la - pt.la) < thr2 && dot(m.pt[o].n, pt.n) >= 0.9f;
+      imp = hit ? m.pt[o].imp : imp;
+      slot = hit ? o : slot;
     }
-    if (slot >= 0) {
-      pt.imp = m.pt[slot].imp;
-      m.pt[slot] = pt;
-    } else {
-      m.pt[m.n++] = pt;
-    }
+    pt.imp = imp;
+    int target = slot >= 0 ? slot : m.n;
+#pragma unroll
+    for (int o = 0; o < kMfCache + 4; ++o) mf_put(m.pt[o], o == target, pt);
+    m.n += slot >= 0 ? 0 : 1;
   }
   if (m.n > kMfCache) {
-    // stable insertion sort, deepest first (total order)
+    // stable insertion sort, deepest first (total order), step by step:
+    // item a moves down while the item before it has the smaller key
+#pragma unroll
     for (int a = 1; a < kMfCache + 4; ++a) {
       if (a >= m.n) break;
       MfPoint x = m.pt[a];
       int64_t kx = total_key(x.pen);
-      int b = a - 1;
-      while (b >= 0 && total_key(m.pt[b].pen) < kx) {
-        m.pt[b + 1] = m.pt[b];
-        --b;
+      bool moving = true;
+#pragma unroll
+      for (int b = a - 1; b >= 0; --b) {
+        bool shift = moving && total_key(m.pt[b].pen) < kx;
+        bool place = moving && !shift;
+        MfPoint y = m.pt[b];
+        mf_put(m.pt[b + 1], shift, y);
+        mf_put(m.pt[b + 1], place, x);
+        moving = shift;
       }
-      m.pt[b + 1] = x;
+      mf_put(m.pt[0], moving, x);
     }
     m.n = kMfCache;
   }
@@ -213,6 +251,7 @@ __device__ __forceinline__ void mf_capture(MfMan& m, const float4* __restrict__ 
 // Emit the refreshed points (step.rs:748-760, 818-830).
 __device__ __forceinline__ void mf_emit(const MfMan& m, V3 pa, Q4 qa, V3 pb, Q4 qb, float warm,
                                         float4* __restrict__ out) {
+#pragma unroll
   for (int k = 0; k < kMfCache; ++k) {
     if (k >= m.n) break;
     const MfPoint& p = m.pt[k];
@@ -236,7 +275,7 @@ __device__ __forceinline__ int64_t mf_lookup(const uint64_t* __restrict__ keys, 
 
 // Per pair, before the narrowphase.  src[p] = the previous step's entry
 // index of the pair, kMfNone when it had none.
-__global__ void k_mf_pre(uint32_t P, const uint64_t* __restrict__ keys, const BodyRec* __restrict__ bodies,
+__global__ __launch_bounds__(256) void k_mf_pre(uint32_t P, const uint64_t* __restrict__ keys, const BodyRec* __restrict__ bodies,
                          const uint8_t* __restrict__ bflags, const uint32_t* __restrict__ group_mask,
                          const uint64_t* __restrict__ okeys, const uint8_t* __restrict__ ovalid,
                          const float4* __restrict__ oent, uint32_t M, float slop, float warm,
@@ -279,6 +318,7 @@ __global__ void k_mf_pre(uint32_t P, const uint64_t* __restrict__ keys, const Bo
         if (!mf_needs_redetect(m, pa, qa, pb, qb)) {  // step.rs:731-776
           int n = mf_refresh(m, pa, qa, pb, qb);
           bool deep = false;
+#pragma unroll
           for (int k = 0; k < kMfCache; ++k)
             if (k < n) deep = deep || m.pt[k].pen > slop;
           if (!fast || !deep) {
@@ -316,7 +356,7 @@ __global__ void k_mf_pre(uint32_t P, const uint64_t* __restrict__ keys, const Bo
 
 // Per pair after the narrowphase: capture + refresh, or removal
 // (step.rs:787-852).
-__global__ void k_mf_post(uint32_t P, const uint64_t* __restrict__ keys, const BodyRec* __restrict__ bodies,
+__global__ __launch_bounds__(256) void k_mf_post(uint32_t P, const uint64_t* __restrict__ keys, const BodyRec* __restrict__ bodies,
                           const float4* __restrict__ oent, const uint32_t* __restrict__ ccount,
                           const float4* __restrict__ cslots, float warm, uint8_t* __restrict__ route,
                           const uint32_t* __restrict__ src, uint8_t* __restrict__ nvalid, float4* __restrict__ nent,
