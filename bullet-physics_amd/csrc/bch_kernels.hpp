@@ -208,28 +208,45 @@ __device__ __forceinline__ void reduce_bounds(float* cmin, float* cmax, uint32_t
 }
 
 // Fold the per-block partials of up to two kernels into the counters.
-__global__ void k_bounds_final(const float* __restrict__ pa, uint32_t na, const float* __restrict__ pb, uint32_t nb,
-                               Counters* cnt) {
-  __shared__ float sm[6][256];
-  __shared__ uint32_t sc[256];
+// One block of 1024 threads reads the partials (8 floats each: min xyz,
+// max xyz, count) with two float4 loads per partial, then wave shuffles
+// and one value per wave in LDS.  min / max of finite centres and an integer
+// count are exact in any order.
+__global__ __launch_bounds__(1024) void k_bounds_final(const float* __restrict__ pa, uint32_t na,
+                                                       const float* __restrict__ pb, uint32_t nb, Counters* cnt) {
+  __shared__ float sm[6][16];
+  __shared__ uint32_t sc[16];
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   uint32_t tot = 0;
   for (uint32_t i = threadIdx.x; i < na + nb; i += blockDim.x) {
-    const float* p = i < na ? pa + 8 * i : pb + 8 * (i - na);
+    const float4* p = reinterpret_cast<const float4*>(i < na ? pa + 8 * (uint64_t)i : pb + 8 * (uint64_t)(i - na));
+    float4 a = p[0], b = p[1];
+    lo[0] = mn(lo[0], a.x);
+    lo[1] = mn(lo[1], a.y);
+    lo[2] = mn(lo[2], a.z);
+    hi[0] = mx(hi[0], a.w);
+    hi[1] = mx(hi[1], b.x);
+    hi[2] = mx(hi[2], b.y);
+    tot += f2u(b.z);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
     for (int k = 0; k < 3; ++k) {
-      lo[k] = mn(lo[k], p[k]);
-      hi[k] = mx(hi[k], p[3 + k]);
+      lo[k] = mn(lo[k], __shfl_xor(lo[k], off));
+      hi[k] = mx(hi[k], __shfl_xor(hi[k], off));
     }
-    tot += f2u(p[6]);
+    tot += __shfl_xor(tot, off);
   }
-  for (int k = 0; k < 3; ++k) {
-    sm[k][threadIdx.x] = lo[k];
-    sm[3 + k][threadIdx.x] = hi[k];
+  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    for (int k = 0; k < 3; ++k) {
+      sm[k][w] = lo[k];
+      sm[3 + k][w] = hi[k];
+    }
+    sc[w] = tot;
   }
-  sc[threadIdx.x] = tot;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (uint32_t i = 1; i < blockDim.x; ++i) {
+    for (int i = 1; i < nw; ++i) {
       for (int k = 0; k < 3; ++k) {
         lo[k] = mn(lo[k], sm[k][i]);
         hi[k] = mx(hi[k], sm[3 + k][i]);

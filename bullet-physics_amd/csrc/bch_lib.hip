@@ -288,7 +288,7 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   if (given_boxes) {
     hipLaunchKernelGGL(k_aabb_given, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->aabb.as<float4>(),
                        c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(), cnt, c->partials.as<float>());
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, c->partials.as<float>(), blocks(n),
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(1024), 0, s, c->partials.as<float>(), blocks(n),
                        (const float*)nullptr, 0u, cnt);
   }
   hipLaunchKernelGGL(k_morton, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->aabb.as<float4>(), c->flags.as<uint8_t>(),
@@ -644,7 +644,7 @@ void run_aabbs(bch_context* c, float aabb_margin) {
     hipLaunchKernelGGL(k_hull_aabb, dim3(hb), dim3(kBlock), 0, s, c->n_hulls, c->hull_ids.as<uint32_t>(),
                        c->bodies.as<BodyRec>(), c->hull.as<float4>(), aabb_margin, c->aabb.as<float4>(),
                        c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(), c->counters.as<Counters>(), hpart, bf, keep);
-  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, c->partials.as<float>(), blocks(n), hpart, hb,
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(1024), 0, s, c->partials.as<float>(), blocks(n), hpart, hb,
                      c->counters.as<Counters>());
   c->aabb_valid = true;
 }
