@@ -490,6 +490,18 @@ def main():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "algorithmic_bytes": ab, "kernel_ms": round(dur * 1e3, 4), "dominant_stage": dom,
             "routes": {"epa_pairs": int(((routes & 8) != 0).sum()), "hull_pairs": int(((routes & 7) == 6).sum())}}
+    # The kernel's real bound is instruction issue: its VALU wave-instructions
+    # per launch (SQ counters, profiles/<kernel>_mixed_1m_valu.json) over
+    # 1024 SIMDs x the live stage time x 2.4 GHz, against one wave64 VALU
+    # instruction per 4 cycles per SIMD.
+    vf = os.path.join(ROOT, "profiles", f"{kern}_mixed_1m_valu.json")
+    if os.path.exists(vf):
+        vi = json.load(open(vf))["valu_instructions_per_dispatch"]
+        rate = vi / (1024 * dur * 2.4e9)
+        roof["valu"] = {"instructions": vi, "achieved_per_simd_cycle": round(rate, 4), "peak_per_simd_cycle": 0.25,
+                        "frac": round(rate / 0.25, 3),
+                        "note": "instruction count from the SQ counters of the same kernel and workload; time = "
+                                "this run's stage time (the kernel plus the short one-pair launch after it)"}
 
     # last: it moves the bodies
     manifold = None if args.only_pass else manifold_frames(w, sc, args.margin, 5)
