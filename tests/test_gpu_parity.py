@@ -302,23 +302,27 @@ def _box_columns(cols=16, height=4, sink=0.01):
     return sc
 
 
-def test_contact_buffer_regrows_when_contacts_outnumber_pairs(bpa, oracle, ctx):
+def test_contact_buffer_regrows_when_contacts_outnumber_pairs(bpa, oracle):
     """The eager pass gathers contacts into a buffer sized by earlier passes
     (at first one contact per pair, with headroom) and gathers again once it
-    is short.  Face-resting boxes give about 1.9 contacts per pair, so the
-    first pass takes the regrow path and the second the sized buffer; both
-    must equal the oracle."""
-    _flush("case: resting box columns, contacts > pairs")
+    is short.  Face-resting boxes give about 1.9 contacts per pair, so on a
+    fresh context the first pass takes the regrow path and the second the
+    sized buffer; both must equal the oracle."""
+    _flush("case: resting box columns, contacts > pairs, fresh context")
     sc = _box_columns()
     ref_pairs, ref_cnt, ref_pts, _ = _oracle_pass(oracle, sc, MARGIN)
     assert ref_cnt.sum() > 1.5 * ref_pairs.shape[0]
-    w = bpa.CollisionWorld(sc, ctx)
-    for rep in range(2):
-        P, Cn = w.collide(MARGIN)
-        pairs, offs, cont = w.results()
-        assert np.array_equal(pairs, ref_pairs)
-        assert Cn == int(ref_cnt.sum())
-        compare_contacts(ref_cnt, ref_pts, offs, cont, label=f"box columns, pass {rep}")
+    fresh = bpa._Context(0)
+    try:
+        w = bpa.CollisionWorld(sc, fresh)
+        for rep in range(2):
+            P, Cn = w.collide(MARGIN)
+            pairs, offs, cont = w.results()
+            assert np.array_equal(pairs, ref_pairs)
+            assert Cn == int(ref_cnt.sum())
+            compare_contacts(ref_cnt, ref_pts, offs, cont, label=f"box columns, pass {rep}")
+    finally:
+        fresh.close()
 
 
 def test_batched_worlds_equal_plain_per_world(bpa, scenes, oracle, ctx):
