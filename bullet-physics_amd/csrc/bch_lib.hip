@@ -458,25 +458,15 @@ int narrow_finish(bch_context* c, const float* d_seeds) {
     // The one-lane-per-pair classes run on the side stream beside k_np_hull
     // (disjoint pairs; both append to the EPA queue through one atomic
     // counter): each kernel fills the other's idle issue slots and tail.
-    // Stage 4 -> 5 is then only the fork; 5 -> 6 covers both streams.
+    // k_np_hull, the longer branch, is submitted first and the side classes
+    // longest first.  Stage 4 -> 5 is then only the fork; 5 -> 6 covers both
+    // streams.
     HIP_TRY(c, hipEventRecord(c->fork_ev, s));
     HIP_TRY(c, hipStreamWaitEvent(c->side, c->fork_ev, 0));
+    mark(c, 5);
     const uint32_t* perm = c->perm_alt.as<uint32_t>();
     uint32_t g = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(P), c->np_grid));
-#define BCH_NP(CLS)                                                                                          \
-  hipLaunchKernelGGL(k_np<CLS>, dim3(g), dim3(kBlock), 0, c->side, cnt, perm, c->pairs.as<uint32_t>(),           \
-                     c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_seeds, c->ccount.as<uint32_t>(),     \
-                     c->cslots.as<float4>(), c->epa_q.as<float4>(), cnt)
-    BCH_NP(CLS_NONE);
-    BCH_NP(CLS_SPHERE_SPHERE);
-    BCH_NP(CLS_SPHERE_BOX);
-    BCH_NP(CLS_PLANE);
-    BCH_NP(CLS_BOX_BOX);
-    BCH_NP(CLS_GJK);
-#undef BCH_NP
-    HIP_TRY(c, hipEventRecord(c->join_ev, c->side));
-    mark(c, 5);
-#define BCH_NP_HULL(G)                                                                                     \
+#define BCH_NP_HULL(G)                                                                                  \
   hipLaunchKernelGGL(k_np_hull<G>, dim3(g), dim3(kBlock), 0, s, cnt, perm, c->pairs.as<uint32_t>(),        \
                      c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_seeds, c->ccount.as<uint32_t>(),     \
                      c->cslots.as<float4>(), c->epa_q.as<float4>(), cnt)
@@ -487,6 +477,18 @@ int narrow_finish(bch_context* c, const float* d_seeds) {
     else
       BCH_NP_HULL(16);
 #undef BCH_NP_HULL
+#define BCH_NP(CLS)                                                                                     \
+  hipLaunchKernelGGL(k_np<CLS>, dim3(g), dim3(kBlock), 0, c->side, cnt, perm, c->pairs.as<uint32_t>(),   \
+                     c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_seeds, c->ccount.as<uint32_t>(),     \
+                     c->cslots.as<float4>(), c->epa_q.as<float4>(), cnt)
+    BCH_NP(CLS_GJK);
+    BCH_NP(CLS_BOX_BOX);
+    BCH_NP(CLS_SPHERE_BOX);
+    BCH_NP(CLS_SPHERE_SPHERE);
+    BCH_NP(CLS_PLANE);
+    BCH_NP(CLS_NONE);
+#undef BCH_NP
+    HIP_TRY(c, hipEventRecord(c->join_ev, c->side));
     HIP_TRY(c, hipStreamWaitEvent(s, c->join_ev, 0));
     mark(c, 6);
     // two pairs per wavefront; the entries that leave its fast path run
