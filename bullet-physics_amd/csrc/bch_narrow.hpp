@@ -871,7 +871,9 @@ BCH_HD bool edge_dist(V3 pa, V3 pb, V3 n, float& out) {
     else if (b_dot_ba < 0.0f) out = len(pb);
     else {
       float a_dot_b = dot(pa, pb);
-      out = fsqrt(mx((len2(pa) * len2(pb) - a_dot_b * a_dot_b) / ba_l2, 0.0f));
+      // f32::max(0.0) keeps the non-NaN operand: an overflowed quotient gives 0
+      float q = (len2(pa) * len2(pb) - a_dot_b * a_dot_b) / ba_l2;
+      out = fsqrt(q > 0.0f ? q : 0.0f);
     }
     return true;
   }

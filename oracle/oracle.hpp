@@ -14,10 +14,11 @@
 // never links it.
 //
 // Parity pinning (which checks fix which function) is listed in the test
-// files under tests/ (test_oracle_*.py).  Functions without a closed-form,
-// known-answer or brute-force check: the quaternion op order (glam boundary)
-// and EPA's exact output on generic hulls -- "parity unpinned" beyond the
-// restatement itself.  The CCD sweeps' entry fractions are pinned (the
+// files under tests/ (test_oracle_*.py); tests/plant_oracle.py plants single
+// mistakes here and checks that some test fails.  Without a closed-form,
+// known-answer or brute-force check: the quaternion op order (glam boundary),
+// and so the last f32 bits of GJK / EPA on generic pairs (geometry pins them
+// to the walks' own tolerances) -- "parity unpinned" beyond the restatement.  The CCD sweeps' entry fractions are pinned (the
 // reference's sweep tests, a dense scan, the slab entry of box casts, the
 // sphere cast against the sphere sweep); their hit normal is pinned only on
 // the reference's axis-aligned cases and is otherwise "parity unpinned": it
@@ -1184,7 +1185,10 @@ bool edge_distance(V3<T> pa, V3<T> pb, V3<T> normal, T& out) {
     else if (b_dot_ba < T(0)) out = len(pb);
     else {
       T a_dot_b = dot(pa, pb);
-      out = std::sqrt(fmax_((len2(pa) * len2(pb) - a_dot_b * a_dot_b) / ba_l2, T(0)));
+      // Rust's f32::max(0.0) returns the non-NaN operand: an overflowed
+      // (inf - inf) quotient gives 0, not NaN
+      T q = (len2(pa) * len2(pb) - a_dot_b * a_dot_b) / ba_l2;
+      out = std::sqrt(q > T(0) ? q : T(0));
     }
     return true;
   }

@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# BCH_ORACLE_LIB names another build of the same sources (tests/plant_oracle.py
+# points it at copies with one planted mistake each).
+_LIB_PATH = os.environ.get("BCH_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 SPHERE, BOX, CAPSULE, HULL, PLANE = 0, 1, 2, 3, 4
@@ -57,6 +59,19 @@ def lib():
         L.oracle_gjk_intersect_f32.argtypes = [sp, f32p, f32p, sp, f32p, f32p, f32p]
         L.oracle_epa_pair_f32.argtypes = [f32p, sp, f32p, f32p, sp, f32p, f32p, C.c_int, f32p]
         L.oracle_epa_points_f32.argtypes = [f32p, f32p, C.c_int, f32p]
+        i32p = C.POINTER(C.c_int)
+        L.oracle_closest_origin_f32.argtypes = [f32p, C.c_int, f32p, f32p]
+        L.oracle_reduce_f32.argtypes = [f32p, f32p, C.c_int, f32p, f32p, i32p]
+        L.oracle_perpendicular_f32.argtypes = [f32p, f32p]
+        L.oracle_solve_f32.argtypes = [f32p, C.c_int, f32p, i32p]
+        L.oracle_enclosing_walk_scripted_f32.argtypes = [f32p, C.c_int, f32p, f32p, C.c_int, i32p]
+        L.oracle_epa_find_best_f32.argtypes = [f32p, C.c_int]
+        L.oracle_epa_expand_f32.argtypes = [i32p, C.c_int, f32p, C.c_int, i32p, i32p]
+        L.oracle_cull_points_f32.argtypes = [f32p, f32p, C.c_int, C.c_int, f32p, f32p]
+        L.oracle_epa_points_trace_f32.argtypes = [f32p, f32p, C.c_int, f32p, f32p, C.c_int, i32p, i32p, i32p]
+        L.oracle_epa_face_distance_f32.restype = C.c_float
+        L.oracle_epa_pair_trace_f32.argtypes = [f32p, sp, f32p, f32p, sp, f32p, f32p, C.c_int, f32p, i32p, i32p]
+        L.oracle_epa_face_distance_f32.argtypes = [f32p, f32p, f32p]
         scp = C.POINTER(OrScene)
         L.oracle_scene_aabbs_f32.argtypes = [scp, C.c_float, C.c_void_p, C.c_void_p]
         L.oracle_scene_thresholds_f32.argtypes = [scp, C.c_void_p]
@@ -269,12 +284,117 @@ def epa_pair(simplex, a: Shape, pa, qa, b: Shape, pb, qb, core=False):
     return st, np.array(out[:3], np.float32), float(out[3])
 
 
+def epa_pair_trace(simplex, a: Shape, pa, qa, b: Shape, pb, qb, core=False):
+    """epa_pair -> (status, normal, depth, vertices, iterations)."""
+    ac, bc = a.c(), b.c()
+    out = (C.c_float * 4)()
+    nv, it = C.c_int(0), C.c_int(0)
+    st = lib().oracle_epa_pair_trace_f32(_f32(np.asarray(simplex).ravel(), 12), C.byref(ac), _f32(pa, 3),
+                                         _f32(qa, 4), C.byref(bc), _f32(pb, 3), _f32(qb, 4), int(core), out,
+                                         C.byref(nv), C.byref(it))
+    return st, np.array(out[:3], np.float32), float(out[3]), nv.value, it.value
+
+
 def epa_points(simplex, points):
     pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
     out = (C.c_float * 4)()
     st = lib().oracle_epa_points_f32(_f32(np.asarray(simplex).ravel(), 12),
                                      pts.ctypes.data_as(C.POINTER(C.c_float)), pts.shape[0], out)
     return st, np.array(out[:3], np.float32), float(out[3])
+
+
+# ------------------------------------------------------------- helpers --
+def _fa(a, n=None):
+    a = np.ascontiguousarray(a, np.float32).ravel()
+    return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def closest_origin(points):
+    """closest_origin (gjk.rs:864-1043) -> (q, kept points in keep order)."""
+    pts, pp = _fa(points)
+    q = (C.c_float * 3)()
+    keep = (C.c_float * 12)()
+    k = lib().oracle_closest_origin_f32(pp, pts.size // 3, q, keep)
+    return np.array(q[:], np.float32), np.array(keep[:3 * k], np.float32).reshape(k, 3)
+
+
+def reduce_support(w, p_b):
+    """The contact walk's reduce (gjk.rs:352-583) -> (inside, v, p_b, len)."""
+    wa, wp = _fa(w)
+    ba, bp = _fa(p_b)
+    v, pb = (C.c_float * 3)(), (C.c_float * 3)()
+    n = C.c_int(0)
+    inside = lib().oracle_reduce_f32(wp, bp, wa.size // 3, v, pb, C.byref(n))
+    return bool(inside), np.array(v[:], np.float32), np.array(pb[:], np.float32), n.value
+
+
+def perpendicular(v):
+    out = (C.c_float * 3)()
+    lib().oracle_perpendicular_f32(_f32(v, 3), out)
+    return np.array(out[:], np.float32)
+
+
+def solve(points):
+    """solve (gjk.rs:609-620) on Simplex::new(points[0]) + add(points[1:]) -> (enclosed, dir, len)."""
+    pa, pp = _fa(points)
+    d = (C.c_float * 3)()
+    n = C.c_int(0)
+    r = lib().oracle_solve_f32(pp, pa.size // 3, d, C.byref(n))
+    return bool(r), np.array(d[:], np.float32), n.value
+
+
+def enclosing_walk_scripted(script, fallback, max_q=512):
+    """enclosing_walk (gjk.rs:299-342) whose k-th support call returns
+    script[min(k, len - 1)] -> (found, queried directions)."""
+    sa, sp = _fa(script)
+    q = (C.c_float * (3 * max_q))()
+    nq = C.c_int(0)
+    ok = lib().oracle_enclosing_walk_scripted_f32(sp, sa.size // 3, _f32(fallback, 3), q, max_q, C.byref(nq))
+    n = min(nq.value, max_q)
+    return bool(ok), np.array(q[:3 * n], np.float32).reshape(n, 3)
+
+
+def epa_points_trace(simplex, points, max_q=1024):
+    """EPA against a point set's support, recording the support queries ->
+    (status, normal, depth, queried directions, vertices, iterations)."""
+    pa, pp = _fa(points)
+    out = (C.c_float * 4)()
+    q = (C.c_float * (3 * max_q))()
+    nq, nv, it = C.c_int(0), C.c_int(0), C.c_int(0)
+    st = lib().oracle_epa_points_trace_f32(_f32(np.asarray(simplex).ravel(), 12), pp, pa.size // 3, out, q, max_q,
+                                           C.byref(nq), C.byref(nv), C.byref(it))
+    n = min(nq.value, max_q)
+    return (st, np.array(out[:3], np.float32), float(out[3]), np.array(q[:3 * n], np.float32).reshape(n, 3),
+            nv.value, it.value)
+
+
+def epa_face_distance(a, b, c):
+    return lib().oracle_epa_face_distance_f32(_f32(a, 3), _f32(b, 3), _f32(c, 3))
+
+
+def epa_find_best(distances):
+    da, dp = _fa(distances)
+    return lib().oracle_epa_find_best_f32(dp, da.size)
+
+
+def epa_expand(faces, verts):
+    """expand (epa.rs:309-388) toward verts[-1] -> (accepted, faces after)."""
+    f = np.ascontiguousarray(faces, np.int32).ravel()
+    va, vp = _fa(verts)
+    out = (C.c_int * (3 * (f.size // 3 + 64)))()
+    nf = C.c_int(0)
+    ok = lib().oracle_epa_expand_f32(f.ctypes.data_as(C.POINTER(C.c_int)), f.size // 3, vp, va.size // 3, out,
+                                     C.byref(nf))
+    return bool(ok), np.array(out[:3 * nf.value], np.int32).reshape(-1, 3)
+
+
+def cull_points(points, depths, maxc=4):
+    """cull_points (box_box.rs:131-174) -> (points, depths) kept."""
+    pa, pp = _fa(points)
+    da, dp = _fa(depths)
+    op, od = (C.c_float * 12)(), (C.c_float * 4)()
+    m = lib().oracle_cull_points_f32(pp, dp, da.size, maxc, op, od)
+    return np.array(op[:3 * m], np.float32).reshape(m, 3), np.array(od[:m], np.float32)
 
 
 # ------------------------------------------------------------------ scenes --

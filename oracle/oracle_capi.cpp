@@ -967,4 +967,179 @@ void oracle_sweep_mesh_scene(const or_scene* sc, void* const* meshes, const uint
   }
 }
 
+// ------------------------------------------------------------ helpers --
+// Helper-level entry points for the known-answer tests of the reference's
+// own helper tests (gjk.rs:1233-1537, :1703-2093; epa.rs:480-738) and of the
+// tie rules the reference documents (epa.rs:212-224, :333-345;
+// box_box.rs:126-174).  All f32.
+
+// closest_origin (gjk.rs:864-1043) on a simplex of n points; returns the
+// number of kept points (written to keep12 in the reference's keep order).
+int oracle_closest_origin_f32(const float* pts, int n, float* q3, float* keep12) {
+  CoSimplex s;
+  s.n = n;
+  for (int i = 0; i < n; ++i) s.p[i] = v3p<float>(pts + 3 * i);
+  V3<float> q = closest_origin(s);
+  q3[0] = q.x; q3[1] = q.y; q3[2] = q.z;
+  for (int i = 0; i < s.n; ++i) { keep12[3 * i] = s.p[i].x; keep12[3 * i + 1] = s.p[i].y; keep12[3 * i + 2] = s.p[i].z; }
+  return s.n;
+}
+
+// The contact walk's reduce (gjk.rs:352-583) on n support vertices (w, p_b);
+// returns 1 for Inside, else writes v, p_b and the shrunk length.
+int oracle_reduce_f32(const float* w12, const float* pb12, int n, float* v3, float* pb3, int* n_out) {
+  SupportVertex<float> buf[4];
+  for (int i = 0; i < 4; ++i) {
+    buf[i].w = i < n ? v3p<float>(w12 + 3 * i) : splat(0.0f);
+    buf[i].p_b = i < n ? v3p<float>(pb12 + 3 * i) : splat(0.0f);
+  }
+  int len = n;
+  Reduce<float> r = reduce(buf, len);
+  *n_out = len;
+  if (r.inside) return 1;
+  v3[0] = r.v.x; v3[1] = r.v.y; v3[2] = r.v.z;
+  pb3[0] = r.p_b.x; pb3[1] = r.p_b.y; pb3[2] = r.p_b.z;
+  return 0;
+}
+
+// perpendicular (gjk.rs:668-680)
+void oracle_perpendicular_f32(const float* v, float* out3) {
+  V3<float> p = perpendicular(v3p<float>(v));
+  out3[0] = p.x; out3[1] = p.y; out3[2] = p.z;
+}
+
+// solve (gjk.rs:609-620) on the strict-walk simplex built as
+// Simplex::new(pts[0]) followed by add(pts[1]), add(pts[2]), ...
+int oracle_solve_f32(const float* pts, int n, float* dir3, int* n_out) {
+  Simplex<float> s;
+  s.init(v3p<float>(pts));
+  for (int i = 1; i < n; ++i) s.add(v3p<float>(pts + 3 * i));
+  V3<float> dir = splat(0.0f);
+  bool r = solve(s, dir);
+  dir3[0] = dir.x; dir3[1] = dir.y; dir3[2] = dir.z;
+  *n_out = s.n;
+  return r ? 1 : 0;
+}
+
+// enclosing_walk (gjk.rs:299-342) with a scripted support: call k returns
+// script[min(k, nscript - 1)]; every queried direction is recorded (up to
+// max_q).  Returns 1 when an enclosing simplex was found.
+int oracle_enclosing_walk_scripted_f32(const float* script, int nscript, const float* fallback, float* queried,
+                                       int max_q, int* n_queried) {
+  int calls = 0;
+  auto sup = [&](V3<float> d) {
+    if (calls < max_q) { queried[3 * calls] = d.x; queried[3 * calls + 1] = d.y; queried[3 * calls + 2] = d.z; }
+    int k = calls < nscript ? calls : nscript - 1;
+    ++calls;
+    return v3p<float>(script + 3 * k);
+  };
+  Simplex<float> s;
+  bool ok = enclosing_walk<float>(sup, v3p<float>(fallback), (const V3<float>*)nullptr, s);
+  *n_queried = calls;
+  return ok ? 1 : 0;
+}
+
+namespace {
+Face<float> make_face(const std::vector<V3<float>>& v, int a, int b, int c) {
+  V3<float> cr = cross(v[b] - v[a], v[c] - v[a]);
+  V3<float> n = cr / len(cr);
+  return Face<float>{a, b, c, n, face_distance(n, v[a], v[b], v[c]), dot(n, v[a])};
+}
+}  // namespace
+
+// find_best (epa.rs:212-224) over faces whose |distance| values are given.
+int oracle_epa_find_best_f32(const float* distance, int n) {
+  std::vector<Face<float>> f(n);
+  for (int i = 0; i < n; ++i) f[i] = Face<float>{0, 0, 0, splat(0.0f), distance[i], distance[i]};
+  return find_best(f);
+}
+
+// expand (epa.rs:309-388) of the polytope with faces abc (index triples
+// into verts, normals built as the stitch builds them) toward verts[nv - 1].
+// Writes the resulting face triples in list order; returns 1 when accepted,
+// 0 when the horizon was rejected (faces unchanged).
+int oracle_epa_expand_f32(const int* abc, int nf, const float* verts, int nv, int* out_abc, int* out_nf) {
+  std::vector<V3<float>> v(nv);
+  for (int i = 0; i < nv; ++i) v[i] = v3p<float>(verts + 3 * i);
+  std::vector<Face<float>> faces;
+  for (int i = 0; i < nf; ++i) faces.push_back(make_face(v, abc[3 * i], abc[3 * i + 1], abc[3 * i + 2]));
+  std::vector<std::pair<int, int>> boundary;
+  bool ok = expand(faces, v, boundary, v[nv - 1], nv - 1);
+  for (size_t i = 0; i < faces.size(); ++i) {
+    out_abc[3 * i] = faces[i].a; out_abc[3 * i + 1] = faces[i].b; out_abc[3 * i + 2] = faces[i].c;
+  }
+  *out_nf = (int)faces.size();
+  return ok ? 1 : 0;
+}
+
+// cull_points (box_box.rs:131-174): n <= 8 points with depths, keep maxc.
+int oracle_cull_points_f32(const float* pts, const float* deps, int n, int maxc, float* out_pts, float* out_deps) {
+  V3<float> p[8], op[4];
+  float d[8], od[4];
+  for (int i = 0; i < n; ++i) { p[i] = v3p<float>(pts + 3 * i); d[i] = deps[i]; }
+  int m = cull_points(p, d, n, maxc, op, od);
+  for (int i = 0; i < m; ++i) {
+    out_pts[3 * i] = op[i].x; out_pts[3 * i + 1] = op[i].y; out_pts[3 * i + 2] = op[i].z;
+    out_deps[i] = od[i];
+  }
+  return m;
+}
+
+// EPA (epa.rs:109-205) against a finite point set's support (first maximum
+// wins, as the reference's point_support helper), recording every support
+// query direction (up to max_q); also reports the status, the number of
+// polytope vertices and the main-loop iteration count at exit.
+int oracle_epa_points_trace_f32(const float* simplex12, const float* pts, int npts, float* out4, float* queries,
+                                int max_q, int* n_queries, int* n_vertices, int* iterations) {
+  std::vector<V3<float>> P(npts);
+  for (int i = 0; i < npts; ++i) P[i] = v3p<float>(pts + 3 * i);
+  V3<float> s[4];
+  for (int i = 0; i < 4; ++i) s[i] = v3p<float>(simplex12 + 3 * i);
+  int calls = 0;
+  auto sup = [&](V3<float> d) {
+    if (calls < max_q) { queries[3 * calls] = d.x; queries[3 * calls + 1] = d.y; queries[3 * calls + 2] = d.z; }
+    ++calls;
+    V3<float> best = P[0];
+    float bd = dot(best, d);
+    for (int i = 1; i < npts; ++i) {
+      float dd = dot(P[i], d);
+      if (dd > bd) { best = P[i]; bd = dd; }
+    }
+    return best;
+  };
+  EpaOut<float> e = epa_penetration<float>(s, sup);
+  out4[0] = e.normal.x; out4[1] = e.normal.y; out4[2] = e.normal.z; out4[3] = e.depth;
+  *n_queries = calls;
+  *n_vertices = e.vertices;
+  *iterations = e.iterations;
+  return e.status;
+}
+
+// face_distance (epa.rs:262-303) of the triangle a, b, c with the normal
+// the polytope gives it (normalized cross(b - a, c - a)).
+float oracle_epa_face_distance_f32(const float* a, const float* b, const float* c) {
+  V3<float> pa = v3p<float>(a), pb = v3p<float>(b), pc = v3p<float>(c);
+  V3<float> cr = cross(pb - pa, pc - pa);
+  return face_distance(cr / len(cr), pa, pb, pc);
+}
+
+// oracle_epa_pair_f32 that also reports the polytope's vertex count and the
+// main-loop iteration count at exit.
+int oracle_epa_pair_trace_f32(const float* simplex12, const or_shape* a, const float* pa, const float* qa,
+                              const or_shape* b, const float* pb, const float* qb, int core, float* out4,
+                              int* n_vertices, int* iterations) {
+  std::vector<V3<float>> sa_, sb_;
+  Shape<float> sa = from_desc<float>(a, sa_), sb = from_desc<float>(b, sb_);
+  PairPose<float> pr{&sa, v3p<float>(pa), q4p<float>(qa), &sb, v3p<float>(pb), q4p<float>(qb)};
+  V3<float> s[4];
+  for (int i = 0; i < 4; ++i) s[i] = v3p<float>(simplex12 + 3 * i);
+  EpaOut<float> e;
+  if (core) e = epa_penetration<float>(s, [&](V3<float> d) { return pr.core_support(d).w; });
+  else e = epa_penetration<float>(s, [&](V3<float> d) { return pr.full_support(d); });
+  out4[0] = e.normal.x; out4[1] = e.normal.y; out4[2] = e.normal.z; out4[3] = e.depth;
+  *n_vertices = e.vertices;
+  *iterations = e.iterations;
+  return e.status;
+}
+
 }  // extern "C"
