@@ -230,6 +230,23 @@ def collide_batch(scene: dict, pairs, seeds=None, ctx: _Context | None = None):
     return cnt[:P], pts[:P]
 
 
+def collide_batch_poses(position, orientation, pairs, seeds=None, ctx: _Context | None = None):
+    """collide_batch on the shapes of the last collide_batch call of ctx at new
+    poses (only the poses are uploaded): (count u8[P], points f32[P, 4, 7])."""
+    ctx = ctx or default_context()
+    pos = np.ascontiguousarray(position, np.float32).reshape(-1, 3)
+    quat = np.ascontiguousarray(orientation, np.float32).reshape(-1, 4)
+    pr = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+    P = pr.shape[0]
+    cnt = np.zeros(max(P, 1), np.uint8)
+    pts = np.zeros((max(P, 1), 4, 7), np.float32)
+    sd = None if seeds is None else np.ascontiguousarray(seeds, np.float32).reshape(P, 3)
+    _check(ctx.h, ctx.L.bch_collide_batch_poses(ctx.h, pos.ctypes.data, quat.ctypes.data, pr.ctypes.data, P,
+                                                None if sd is None else sd.ctypes.data, cnt.ctypes.data,
+                                                pts.ctypes.data))
+    return cnt[:P], pts[:P]
+
+
 def _shape_scene(shapes_poses):
     """Tiny scene from [(type, params, hull_pts|None, pos, quat), ...]."""
     n = len(shapes_poses)

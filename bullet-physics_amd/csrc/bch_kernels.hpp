@@ -703,6 +703,58 @@ __global__ void k_split_pairs(uint32_t P, const uint32_t* __restrict__ P_dev, co
   iota[p] = p;
 }
 
+// Proxy arena update: slot idx[k] takes the box (lo, hi) = upd[2k], upd[2k + 1].
+__global__ void k_arena_scatter(uint32_t m, const uint32_t* __restrict__ idx, const float4* __restrict__ upd,
+                                float4* __restrict__ arena) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  uint32_t i = idx[k];
+  arena[2 * (uint64_t)i] = upd[2 * (uint64_t)k];
+  arena[2 * (uint64_t)i + 1] = upd[2 * (uint64_t)k + 1];
+}
+
+// Sorted pair keys i << 32 | j -> (i, j) as u32 pairs.
+__global__ void k_keys_to_pairs(uint32_t P, const uint64_t* __restrict__ keys, uint2* __restrict__ out) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  uint64_t k = keys[p];
+  out[p] = make_uint2((uint32_t)(k >> 32), (uint32_t)k);
+}
+
+// Caller pairs (i, j) -> keys i << 32 | j, kept in caller order.
+__global__ void k_pairs_to_keys(uint32_t P, const uint2* __restrict__ pairs, uint64_t* __restrict__ keys) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  uint2 q = pairs[p];
+  keys[p] = ((uint64_t)q.x << 32) | q.y;
+}
+
+// Per-pair contact slots -> the batch layout: a u8 count and four points of
+// (position xyz, normal xyz, depth), zero past the count.
+__global__ void k_batch_out(uint32_t P, const uint32_t* __restrict__ ccount, const float4* __restrict__ cslots,
+                            uint8_t* __restrict__ out_count, float* __restrict__ out_pts) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  uint32_t n = ccount[p];
+  out_count[p] = (uint8_t)n;
+  float* o = out_pts + 28 * (uint64_t)p;
+#pragma unroll
+  for (uint32_t q = 0; q < 4; ++q) {
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f), b = a;
+    if (q < n) {
+      a = cslots[8 * (uint64_t)p + 2 * q];
+      b = cslots[8 * (uint64_t)p + 2 * q + 1];
+    }
+    o[7 * q] = a.x;
+    o[7 * q + 1] = a.y;
+    o[7 * q + 2] = a.z;
+    o[7 * q + 3] = b.x;
+    o[7 * q + 4] = b.y;
+    o[7 * q + 5] = b.z;
+    o[7 * q + 6] = a.w;
+  }
+}
+
 __device__ __forceinline__ void store_contacts(float4* __restrict__ cslots, uint32_t p, const Manifold4& m) {
 #pragma unroll
   for (int c = 0; c < 4; ++c) {

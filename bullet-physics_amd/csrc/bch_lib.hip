@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -28,8 +29,10 @@ namespace {
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool borrowed = false;  // a view into memory this buffer does not own: never grown or freed
   bool ensure(size_t need) {
     if (need <= bytes) return true;
+    if (borrowed) return false;
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
@@ -42,7 +45,42 @@ struct DevBuf {
     return true;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !borrowed) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    borrowed = false;
+  }
+  void view(void* q, size_t nb) {
+    release();
+    p = q;
+    bytes = nb;
+    borrowed = true;
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+// Page-locked host memory that grows by reallocation, keeping its contents.
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  bool ensure(size_t need) {
+    if (need <= bytes) return true;
+    size_t nb = std::max<size_t>(need + need / 2, 4096);
+    void* q = nullptr;
+    if (hipHostMalloc(&q, nb, hipHostMallocDefault) != hipSuccess) return false;
+    if (p) {
+      memcpy(q, p, bytes);
+      (void)hipHostFree(p);
+    }
+    p = q;
+    bytes = nb;
+    return true;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
     p = nullptr;
     bytes = 0;
   }
@@ -130,9 +168,33 @@ struct bch_context {
       mx_epa_q2, mx_out_count, mx_out_pts;
   DevBuf sw_target, sw_caster, sw_from, sw_to, sw_radius, sw_pose, sw_hit, sw_out, sw_keys, sw_order;
 
-  // proxy API state (host mirror of the broadphase slots)
-  std::vector<float> proxy_box;  // 6 per slot
-  std::vector<uint8_t> proxy_live;
+  // Proxy broadphase (held by the batch context below): the arena of slot
+  // boxes lives on the device; the host keeps a pinned mirror (lo, hi as two
+  // float4 per slot, NaN for a free slot, which overlaps nothing), the live
+  // flags, the free slots (lowest first) and the slots changed since the
+  // last upload.
+  PinnedBuf proxy_box;  // 2 float4 per slot
+  uint32_t n_proxy = 0;
+  std::vector<uint8_t> proxy_live, proxy_dirty;
+  std::set<uint32_t> proxy_free;
+  std::vector<uint32_t> dirty_list;
+  uint32_t arena_n = 0;  // slots the device arena holds
+  DevBuf arena, arena_upd, bp_pairs;
+  PinnedBuf arena_stage;
+  bool bp_pairs_valid = false;  // the last compute_pairs result still holds (no change since)
+  uint64_t bp_count = 0;
+
+  // collide_batch: the body count and hull-vertex count of the shapes
+  // uploaded by the last bch_collide_batch (for bch_collide_batch_poses)
+  bool batch_shapes = false;
+  uint32_t batch_n = 0;
+  DevBuf batch_pairs, batch_seeds, batch_out_count, batch_out_pts;
+
+  // The batch entry points (caller-given pairs, mesh and sweep batches, the
+  // proxy broadphase) run on this second context of the same device and
+  // stream, so they never touch the world-resident state above.
+  bch_context* aux = nullptr;
+  bool is_aux = false;
 };
 
 namespace {
@@ -435,8 +497,9 @@ int narrow_split(bch_context* c) {
 
 // Rest of the narrowphase: class sort, the per-class kernels (with per-pair
 // GJK seeds when d_seeds is given; NaN x = none), EPA, dense contacts.
-// Fills c->coffsets (P+1), c->contacts (7C), c->C.
-int narrow_finish(bch_context* c, const float* d_seeds) {
+// Fills c->coffsets (P+1), c->contacts (7C), c->C; without `dense` (eager
+// pass only) it stops at the per-pair slots (c->ccount, c->cslots).
+int narrow_finish(bch_context* c, const float* d_seeds, bool dense = true) {
   hipStream_t s = c->stream;
   Counters* cnt = c->counters.as<Counters>();
   uint64_t P = c->P;
@@ -503,6 +566,13 @@ int narrow_finish(bch_context* c, const float* d_seeds) {
     mark(c, 7);
   } else {
     for (int k = 4; k <= 7; ++k) mark(c, k);
+  }
+  if (!dense && !c->cap_mode) {
+    HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipStreamSynchronize(s));
+    c->C = 0;
+    if (c->host_cnt.errors & 2u) return fail(c, BCH_HIP_ERROR, "EPA polytope store overflow");
+    return BCH_OK;
   }
   if (c->cap_mode) {
     int rc = dev_scan(c, c->ccount.as<uint32_t>(), c->coffsets.as<uint32_t>(), P, c->scan_part, s, P_dev,
@@ -924,8 +994,13 @@ void bch_destroy(bch_context* c) {
                    &c->d_mesh_order, &c->mx_pairs, &c->mx_poses, &c->mx_seeds, &c->mx_first, &c->mx_count,
                    &c->mx_offs, &c->mx_sub, &c->mx_sub_cnt, &c->mx_sub_slots, &c->mx_epa_q, &c->mx_epa_q2,
                    &c->mx_out_count, &c->mx_out_pts, &c->sw_target, &c->sw_caster, &c->sw_from, &c->sw_to,
-                   &c->sw_radius, &c->sw_pose, &c->sw_hit, &c->sw_out, &c->sw_keys, &c->sw_order};
+                   &c->sw_radius, &c->sw_pose, &c->sw_hit, &c->sw_out, &c->sw_keys, &c->sw_order, &c->arena,
+                   &c->arena_upd, &c->bp_pairs, &c->batch_pairs, &c->batch_seeds, &c->batch_out_count,
+                   &c->batch_out_pts};
   for (DevBuf* b : all) b->release();
+  c->proxy_box.release();
+  c->arena_stage.release();
+  if (c->aux) bch_destroy(c->aux);
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   (void)hipEventDestroy(c->fork_ev);
   (void)hipEventDestroy(c->join_ev);
@@ -1076,108 +1151,210 @@ int bch_get_tuning(bch_context* c, int knob, int* value) {
 }
 
 // ------------------------------------------------------------ proxies --
-int bch_bp_create_proxy(bch_context* c, const float* mn, const float* mx, uint32_t* out) {
-  if (!c || !mn || !mx || !out) return fail(c, BCH_INVALID_ARG, "NULL argument");
-  size_t slot = c->proxy_live.size();
-  for (size_t i = 0; i < c->proxy_live.size(); ++i)
-    if (!c->proxy_live[i]) {
-      slot = i;
-      break;
-    }
-  if (slot == c->proxy_live.size()) {
-    c->proxy_live.push_back(0);
-    c->proxy_box.resize(c->proxy_box.size() + 6);
+// The proxy broadphase and the batch entry points run on the batch context
+// (bch_context::aux): created on first use, bound to the caller's stream and
+// launch knobs on every call.
+static int batch_ctx(bch_context* c, bch_context** out) {
+  if (!c) return BCH_INVALID_ARG;
+  if (!c->aux) {
+    bch_context* a = nullptr;
+    int rc = bch_create(c->device, &a);
+    if (rc) return fail(c, rc, "cannot create the batch context");
+    a->is_aux = true;
+    c->aux = a;
   }
-  c->proxy_live[slot] = 1;
-  memcpy(&c->proxy_box[6 * slot], mn, 3 * sizeof(float));
-  memcpy(&c->proxy_box[6 * slot + 3], mx, 3 * sizeof(float));
-  *out = (uint32_t)slot;
+  bch_context* a = c->aux;
+  a->stream = c->stream;
+  a->hull_lanes = c->hull_lanes;
+  a->epa_grid = c->epa_grid;
+  a->np_grid = c->np_grid;
+  *out = a;
   return BCH_OK;
 }
 
+static int from_batch(bch_context* c, const bch_context* a, int rc) {
+  if (rc) c->err = a->err;
+  return rc;
+}
+
+#define BCH_ON_BATCH(c, a)              \
+  bch_context* a = nullptr;             \
+  do {                                  \
+    int rc_ = batch_ctx(c, &a);         \
+    if (rc_) return rc_;                \
+  } while (0)
+
+static void proxy_put(bch_context* a, uint32_t slot, const float* mn, const float* mx) {
+  float4* b = a->proxy_box.as<float4>() + 2 * (size_t)slot;
+  b[0] = mn ? make_float4(mn[0], mn[1], mn[2], 0.0f) : make_float4(NAN, NAN, NAN, 0.0f);
+  b[1] = mx ? make_float4(mx[0], mx[1], mx[2], 0.0f) : make_float4(NAN, NAN, NAN, 0.0f);
+  if (slot < a->arena_n && !a->proxy_dirty[slot]) {
+    a->proxy_dirty[slot] = 1;
+    a->dirty_list.push_back(slot);
+  }
+  a->bp_pairs_valid = false;
+}
+
+// broadphase/mod.rs:94-130.  The lowest free slot is reused
+// (brute_force.rs:26-34, axis_sweep3.rs:398-421, dbvt.rs:679-697).
+int bch_bp_create_proxy(bch_context* c, const float* mn, const float* mx, uint32_t* out) {
+  if (!c || !mn || !mx || !out) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  BCH_ON_BATCH(c, a);
+  uint32_t slot;
+  if (!a->proxy_free.empty()) {
+    slot = *a->proxy_free.begin();
+    a->proxy_free.erase(a->proxy_free.begin());
+  } else {
+    if (a->n_proxy == 0x7fffffffu) return fail(c, BCH_INVALID_ARG, "too many proxies");
+    if (!a->proxy_box.ensure((size_t)(a->n_proxy + 1) * 2 * sizeof(float4)))
+      return fail(c, BCH_HIP_ERROR, "pinned host allocation failed");
+    slot = a->n_proxy++;
+    a->proxy_live.push_back(0);
+    a->proxy_dirty.push_back(0);
+  }
+  a->proxy_live[slot] = 1;
+  proxy_put(a, slot, mn, mx);
+  *out = slot;
+  return BCH_OK;
+}
+
+// A dead slot is a no-op (axis_sweep3.rs:424-426, dbvt.rs:700-702).
 int bch_bp_set_aabb(bch_context* c, uint32_t proxy, const float* mn, const float* mx) {
   if (!c || !mn || !mx) return fail(c, BCH_INVALID_ARG, "NULL argument");
-  if (proxy >= c->proxy_live.size() || !c->proxy_live[proxy]) return BCH_OK;
-  memcpy(&c->proxy_box[6 * proxy], mn, 3 * sizeof(float));
-  memcpy(&c->proxy_box[6 * proxy + 3], mx, 3 * sizeof(float));
+  BCH_ON_BATCH(c, a);
+  if (proxy >= a->n_proxy || !a->proxy_live[proxy]) return BCH_OK;
+  proxy_put(a, proxy, mn, mx);
   return BCH_OK;
 }
 
 int bch_bp_set_aabbs(bch_context* c, uint32_t count, const uint32_t* proxies, const float* mins, const float* maxs) {
   if (!c || (count && (!proxies || !mins || !maxs))) return fail(c, BCH_INVALID_ARG, "NULL argument");
-  for (uint32_t k = 0; k < count; ++k) bch_bp_set_aabb(c, proxies[k], mins + 3 * k, maxs + 3 * k);
+  BCH_ON_BATCH(c, a);
+  for (uint32_t k = 0; k < count; ++k) {
+    uint32_t p = proxies[k];
+    if (p < a->n_proxy && a->proxy_live[p]) proxy_put(a, p, mins + 3 * (size_t)k, maxs + 3 * (size_t)k);
+  }
   return BCH_OK;
 }
 
 int bch_bp_destroy_proxy(bch_context* c, uint32_t proxy) {
   if (!c) return BCH_INVALID_ARG;
-  if (proxy < c->proxy_live.size()) c->proxy_live[proxy] = 0;
+  BCH_ON_BATCH(c, a);
+  if (proxy < a->n_proxy && a->proxy_live[proxy]) {
+    a->proxy_live[proxy] = 0;
+    proxy_put(a, proxy, nullptr, nullptr);  // NaN boxes overlap nothing
+    a->proxy_free.insert(proxy);
+  }
   return BCH_OK;
 }
 
 int bch_bp_proxy_aabb(bch_context* c, uint32_t proxy, float* mn, float* mx, int* live) {
   if (!c || !live) return fail(c, BCH_INVALID_ARG, "NULL argument");
-  *live = proxy < c->proxy_live.size() && c->proxy_live[proxy];
+  BCH_ON_BATCH(c, a);
+  *live = proxy < a->n_proxy && a->proxy_live[proxy];
   if (*live) {
-    if (mn) memcpy(mn, &c->proxy_box[6 * proxy], 3 * sizeof(float));
-    if (mx) memcpy(mx, &c->proxy_box[6 * proxy + 3], 3 * sizeof(float));
+    const float4* b = a->proxy_box.as<float4>() + 2 * (size_t)proxy;
+    if (mn) { mn[0] = b[0].x; mn[1] = b[0].y; mn[2] = b[0].z; }
+    if (mx) { mx[0] = b[1].x; mx[1] = b[1].y; mx[2] = b[1].z; }
   }
   return BCH_OK;
 }
 
+// Bring the device arena up to the host mirror: a whole upload when it grew
+// past its buffer or most slots changed, else the new tail slots plus a
+// scatter of the changed ones.
+static int arena_sync(bch_context* a) {
+  hipStream_t s = a->stream;
+  uint32_t n = a->n_proxy;
+  const float4* mirror = a->proxy_box.as<float4>();
+  bool whole = a->arena.bytes < (size_t)n * 2 * sizeof(float4) || a->arena_n == 0 ||
+               a->dirty_list.size() > (size_t)n / 4;
+  if (whole) {
+    if (a->arena.bytes < (size_t)n * 2 * sizeof(float4)) ENSURE(a, a->arena, (size_t)n * 2 * sizeof(float4));
+    HIP_TRY(a, hipMemcpyAsync(a->arena.p, mirror, (size_t)n * 2 * sizeof(float4), hipMemcpyHostToDevice, s));
+  } else {
+    if (n > a->arena_n)
+      HIP_TRY(a, hipMemcpyAsync(a->arena.as<float4>() + 2 * (size_t)a->arena_n, mirror + 2 * (size_t)a->arena_n,
+                                (size_t)(n - a->arena_n) * 2 * sizeof(float4), hipMemcpyHostToDevice, s));
+    uint32_t m = (uint32_t)a->dirty_list.size();
+    if (m) {
+      size_t ib = ((size_t)m * sizeof(uint32_t) + 15) & ~(size_t)15;
+      size_t need = ib + (size_t)m * 2 * sizeof(float4);
+      if (!a->arena_stage.ensure(need)) return fail(a, BCH_HIP_ERROR, "pinned host allocation failed");
+      ENSURE(a, a->arena_upd, need);
+      uint32_t* idx = a->arena_stage.as<uint32_t>();
+      float4* box = reinterpret_cast<float4*>(a->arena_stage.as<char>() + ib);
+      for (uint32_t k = 0; k < m; ++k) {
+        uint32_t i = a->dirty_list[k];
+        idx[k] = i;
+        box[2 * k] = mirror[2 * (size_t)i];
+        box[2 * k + 1] = mirror[2 * (size_t)i + 1];
+      }
+      HIP_TRY(a, hipMemcpyAsync(a->arena_upd.p, a->arena_stage.p, need, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_arena_scatter, dim3(blocks(m)), dim3(kBlock), 0, s, m, a->arena_upd.as<uint32_t>(),
+                         reinterpret_cast<const float4*>(a->arena_upd.as<char>() + ib), a->arena.as<float4>());
+    }
+  }
+  for (uint32_t i : a->dirty_list) a->proxy_dirty[i] = 0;
+  a->dirty_list.clear();
+  a->arena_n = n;
+  return BCH_OK;
+}
+
+// compute_pairs_into (broadphase/mod.rs:103-118): every live closed overlap
+// (i < j), lexicographically sorted.  The result is kept until a proxy
+// changes, so the OVERFLOW re-call with a larger capacity only copies it.
 int bch_bp_compute_pairs(bch_context* c, uint32_t* out_pairs, uint64_t capacity, uint64_t* out_count) {
   if (!c || !out_count) return fail(c, BCH_INVALID_ARG, "NULL argument");
   HIP_TRY(c, hipSetDevice(c->device));
-  uint32_t n = (uint32_t)c->proxy_live.size();
+  BCH_ON_BATCH(c, a);
   *out_count = 0;
-  if (n == 0) return BCH_OK;
-  int rc = ensure_pass_buffers(c, n);
-  if (rc) return rc;
-  std::vector<float4> boxes(2 * (size_t)n);
-  for (uint32_t i = 0; i < n; ++i) {
-    const float* b = &c->proxy_box[6 * i];
-    if (c->proxy_live[i]) {
-      boxes[2 * i] = make_float4(b[0], b[1], b[2], 0.0f);
-      boxes[2 * i + 1] = make_float4(b[3], b[4], b[5], 0.0f);
-    } else {
-      boxes[2 * i] = boxes[2 * i + 1] = make_float4(NAN, NAN, NAN, 0.0f);
+  uint32_t n = a->n_proxy;
+  hipStream_t s = a->stream;
+  if (!a->bp_pairs_valid) {
+    a->bp_count = 0;
+    if (n) {
+      int rc = ensure_pass_buffers(a, n);
+      if (!rc) rc = arena_sync(a);
+      if (rc) return from_batch(c, a, rc);
+      hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, a->counters.as<Counters>());
+      HIP_TRY(c, hipMemcpyAsync(a->aabb.p, a->arena.p, (size_t)n * 2 * sizeof(float4), hipMemcpyDeviceToDevice, s));
+      a->batched = false;
+      a->cap_mode = false;
+      rc = run_broadphase(a, n, nullptr, nullptr, true);
+      if (rc) return from_batch(c, a, rc);
+      a->bp_count = a->P;
+      if (a->P) {
+        ENSURE(a, a->bp_pairs, a->P * sizeof(uint2));
+        hipLaunchKernelGGL(k_keys_to_pairs, dim3(blocks(a->P)), dim3(kBlock), 0, s, (uint32_t)a->P,
+                           a->pair_keys.as<uint64_t>(), a->bp_pairs.as<uint2>());
+      }
     }
+    a->bp_pairs_valid = true;
   }
-  hipStream_t s = c->stream;
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, c->counters.as<Counters>());
-  HIP_TRY(c, hipMemcpyAsync(c->aabb.p, boxes.data(), boxes.size() * sizeof(float4), hipMemcpyHostToDevice, s));
-  bool saved = c->batched;
-  c->batched = false;
-  rc = run_broadphase(c, n, nullptr, nullptr, true);
-  c->batched = saved;
-  if (rc) return rc;
-  *out_count = c->P;
-  if (c->P > capacity || (!out_pairs && c->P)) return BCH_OVERFLOW;
-  if (c->P) {
-    std::vector<uint64_t> keys(c->P);
-    HIP_TRY(c, hipMemcpyAsync(keys.data(), c->pair_keys.p, c->P * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    HIP_TRY(c, hipStreamSynchronize(s));
-    for (uint64_t k = 0; k < c->P; ++k) {
-      out_pairs[2 * k] = (uint32_t)(keys[k] >> 32);
-      out_pairs[2 * k + 1] = (uint32_t)keys[k];
-    }
-  }
+  *out_count = a->bp_count;
+  if (a->bp_count > capacity || (!out_pairs && a->bp_count)) return BCH_OVERFLOW;
+  if (a->bp_count)
+    HIP_TRY(c, hipMemcpyAsync(out_pairs, a->bp_pairs.p, a->bp_count * sizeof(uint2), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
   return BCH_OK;
 }
 
 int bch_bp_aabb_candidates(bch_context* c, const float* mn, const float* mx, uint32_t* out, uint64_t capacity,
                            uint64_t* out_count) {
   if (!c || !mn || !mx || !out_count) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  BCH_ON_BATCH(c, a);
   // Host scan over the proxy arena: the exact query of
   // broadphase/mod.rs:135-146 (one box against every live proxy).
   uint64_t k = 0;
   Box6 q{V3{mn[0], mn[1], mn[2]}, V3{mx[0], mx[1], mx[2]}};
-  for (size_t i = 0; i < c->proxy_live.size(); ++i) {
-    if (!c->proxy_live[i]) continue;
-    const float* b = &c->proxy_box[6 * i];
-    Box6 o{V3{b[0], b[1], b[2]}, V3{b[3], b[4], b[5]}};
+  const float4* bx = a->proxy_box.as<float4>();
+  for (uint32_t i = 0; i < a->n_proxy; ++i) {
+    if (!a->proxy_live[i]) continue;
+    float4 lo = bx[2 * (size_t)i], hi = bx[2 * (size_t)i + 1];
+    Box6 o{V3{lo.x, lo.y, lo.z}, V3{hi.x, hi.y, hi.z}};
     if (overlaps(o, q)) {
-      if (out && k < capacity) out[k] = (uint32_t)i;
+      if (out && k < capacity) out[k] = i;
       ++k;
     }
   }
@@ -1188,30 +1365,33 @@ int bch_bp_aabb_candidates(bch_context* c, const float* mn, const float* mx, uin
 int bch_bp_raycast_candidates(bch_context* c, const float* from, const float* to, uint32_t* out,
                               uint64_t capacity, uint64_t* out_count) {
   if (!c || !from || !to || !out_count) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  BCH_ON_BATCH(c, a);
   // Host scan over the proxy arena with the slab test of
   // broadphase/mod.rs:153-183 (t in [0, 1]; fmaxf / fminf keep the non-NaN
   // operand, as f32::max / f32::min do).
   uint64_t k = 0;
-  for (size_t i = 0; i < c->proxy_live.size(); ++i) {
-    if (!c->proxy_live[i]) continue;
-    const float* b = &c->proxy_box[6 * i];
+  const float4* bx = a->proxy_box.as<float4>();
+  for (uint32_t i = 0; i < a->n_proxy; ++i) {
+    if (!a->proxy_live[i]) continue;
+    float4 lo4 = bx[2 * (size_t)i], hi4 = bx[2 * (size_t)i + 1];
+    const float b[6] = {lo4.x, lo4.y, lo4.z, hi4.x, hi4.y, hi4.z};
     float t_min = 0.0f, t_max = 1.0f;
     bool hit = true;
-    for (int a = 0; a < 3 && hit; ++a) {
-      float d = to[a] - from[a];
-      float lo = d >= 0.0f ? b[a] : b[3 + a];
-      float hi = d >= 0.0f ? b[3 + a] : b[a];
+    for (int ax = 0; ax < 3 && hit; ++ax) {
+      float d = to[ax] - from[ax];
+      float lo = d >= 0.0f ? b[ax] : b[3 + ax];
+      float hi = d >= 0.0f ? b[3 + ax] : b[ax];
       if (d == 0.0f) {
-        if (from[a] < lo || from[a] > hi) hit = false;
+        if (from[ax] < lo || from[ax] > hi) hit = false;
         continue;
       }
       float inv = 1.0f / d;
-      t_min = fmaxf(t_min, (lo - from[a]) * inv);
-      t_max = fminf(t_max, (hi - from[a]) * inv);
+      t_min = fmaxf(t_min, (lo - from[ax]) * inv);
+      t_max = fminf(t_max, (hi - from[ax]) * inv);
       if (t_min > t_max) hit = false;
     }
     if (hit) {
-      if (out && k < capacity) out[k] = (uint32_t)i;
+      if (out && k < capacity) out[k] = i;
       ++k;
     }
   }
@@ -1220,56 +1400,90 @@ int bch_bp_raycast_candidates(bch_context* c, const float* from, const float* to
 }
 
 // ------------------------------------------------------- collide_batch --
+// The narrowphase over caller-given pairs on the batch context's bodies:
+// pairs to keys on the device, the per-pair contact slots back in the batch
+// layout (u8 count, four 7-float points).
+static int collide_pairs_run(bch_context* a, const uint32_t* pairs, uint64_t n_pairs, const float* seeds,
+                             uint8_t* out_count, float* out_points) {
+  if (!n_pairs) return BCH_OK;
+  hipStream_t s = a->stream;
+  ENSURE(a, a->counters, sizeof(Counters));
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, a->counters.as<Counters>());
+  a->P = n_pairs;
+  a->cap_mode = false;
+  ENSURE(a, a->pair_keys, n_pairs * sizeof(uint64_t));
+  ENSURE(a, a->batch_pairs, n_pairs * sizeof(uint2));
+  HIP_TRY(a, hipMemcpyAsync(a->batch_pairs.p, pairs, n_pairs * sizeof(uint2), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_pairs_to_keys, dim3(blocks(n_pairs)), dim3(kBlock), 0, s, (uint32_t)n_pairs,
+                     a->batch_pairs.as<uint2>(), a->pair_keys.as<uint64_t>());
+  float* d_seeds = nullptr;
+  if (seeds) {
+    ENSURE(a, a->batch_seeds, n_pairs * 3 * sizeof(float));
+    d_seeds = a->batch_seeds.as<float>();
+    HIP_TRY(a, hipMemcpyAsync(d_seeds, seeds, n_pairs * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+  }
+  int rc = narrow_split(a);
+  if (!rc) rc = narrow_finish(a, d_seeds, false);
+  if (rc) return rc;
+  if (a->host_cnt.errors & 1u) return fail(a, BCH_HIP_ERROR, "box-box clip polygon overflow");
+  ENSURE(a, a->batch_out_count, n_pairs);
+  ENSURE(a, a->batch_out_pts, n_pairs * 28 * sizeof(float));
+  hipLaunchKernelGGL(k_batch_out, dim3(blocks(n_pairs)), dim3(kBlock), 0, s, (uint32_t)n_pairs,
+                     a->ccount.as<uint32_t>(), a->cslots.as<float4>(), a->batch_out_count.as<uint8_t>(),
+                     a->batch_out_pts.as<float>());
+  HIP_TRY(a, hipMemcpyAsync(out_count, a->batch_out_count.p, n_pairs, hipMemcpyDeviceToHost, s));
+  HIP_TRY(a, hipMemcpyAsync(out_points, a->batch_out_pts.p, n_pairs * 28 * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_TRY(a, hipStreamSynchronize(s));
+  return BCH_OK;
+}
+
+static int check_pairs(bch_context* c, const uint32_t* pairs, uint64_t n_pairs, uint32_t n) {
+  if (n_pairs > 0x7fffffffull) return fail(c, BCH_INVALID_ARG, "too many pairs");
+  for (uint64_t k = 0; k < 2 * n_pairs; ++k)
+    if (pairs[k] >= n) return fail(c, BCH_INVALID_ARG, "pair index out of range");
+  return BCH_OK;
+}
+
 int bch_collide_batch(bch_context* c, const bch_bodies* bodies, const uint32_t* pairs, uint64_t n_pairs,
                       const float* seeds, uint8_t* out_count, float* out_points) {
   if (!c) return BCH_INVALID_ARG;
   int rc = validate_bodies(c, bodies);
   if (rc) return rc;
   if (n_pairs && (!pairs || !out_count || !out_points)) return fail(c, BCH_INVALID_ARG, "NULL argument");
-  for (uint64_t k = 0; k < 2 * n_pairs; ++k)
-    if (pairs[k] >= bodies->n) return fail(c, BCH_INVALID_ARG, "pair index out of range");
+  rc = check_pairs(c, pairs, n_pairs, bodies->n);
+  if (rc) return rc;
   HIP_TRY(c, hipSetDevice(c->device));
-  rc = upload_bodies(c, bodies, nullptr);
+  BCH_ON_BATCH(c, a);
+  a->batch_shapes = false;
+  rc = upload_bodies(a, bodies, nullptr);
+  if (rc) return from_batch(c, a, rc);
+  a->n = bodies->n;
+  a->batched = false;
+  a->batch_shapes = true;
+  a->batch_n = bodies->n;
+  return from_batch(c, a, collide_pairs_run(a, pairs, n_pairs, seeds, out_count, out_points));
+}
+
+int bch_collide_batch_poses(bch_context* c, const float* position, const float* orientation, const uint32_t* pairs,
+                            uint64_t n_pairs, const float* seeds, uint8_t* out_count, float* out_points) {
+  if (!c) return BCH_INVALID_ARG;
+  BCH_ON_BATCH(c, a);
+  if (!a->batch_shapes) return fail(c, BCH_INVALID_ARG, "no shapes: call bch_collide_batch first");
+  uint32_t n = a->batch_n;
+  if (n && (!position || !orientation)) return fail(c, BCH_INVALID_ARG, "NULL pose arrays");
+  if (n_pairs && (!pairs || !out_count || !out_points)) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  int rc = check_pairs(c, pairs, n_pairs, n);
   if (rc) return rc;
-  c->n = bodies->n;
-  c->batched = false;
-  if (!n_pairs) return BCH_OK;
-  hipStream_t s = c->stream;
-  ENSURE(c, c->counters, sizeof(Counters));
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, c->counters.as<Counters>());
-  // Keys in caller order: the narrowphase consumes them unsorted.
-  c->P = n_pairs;
-  ENSURE(c, c->pair_keys, n_pairs * sizeof(uint64_t));
-  std::vector<uint64_t> keys(n_pairs);
-  for (uint64_t k = 0; k < n_pairs; ++k) keys[k] = ((uint64_t)pairs[2 * k] << 32) | pairs[2 * k + 1];
-  HIP_TRY(c, hipMemcpyAsync(c->pair_keys.p, keys.data(), n_pairs * sizeof(uint64_t), hipMemcpyHostToDevice, s));
-  float* d_seeds = nullptr;
-  if (seeds) {
-    ENSURE(c, c->staging_a, std::max<size_t>(c->staging_a.bytes, n_pairs * 3 * sizeof(float)));
-    d_seeds = c->staging_a.as<float>();
-    HIP_TRY(c, hipMemcpyAsync(d_seeds, seeds, n_pairs * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+  HIP_TRY(c, hipSetDevice(c->device));
+  hipStream_t s = a->stream;
+  if (n) {
+    HIP_TRY(c, hipMemcpyAsync(a->staging_a.p, position, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipMemcpyAsync(a->staging_b.p, orientation, (size_t)n * 4 * sizeof(float), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_pack_poses, dim3(blocks(n)), dim3(kBlock), 0, s, n, a->staging_a.as<float>(),
+                       a->staging_b.as<float>(), a->bodies.as<BodyRec>(), (const float*)nullptr, 0.0f,
+                       (const uint8_t*)nullptr);
   }
-  rc = run_narrowphase(c, d_seeds);
-  if (rc) return rc;
-  if (c->host_cnt.errors & 1u) return fail(c, BCH_HIP_ERROR, "box-box clip polygon overflow");
-  std::vector<uint32_t> cnt(n_pairs);
-  std::vector<float4> slots(n_pairs * 8);
-  HIP_TRY(c, hipMemcpyAsync(cnt.data(), c->ccount.p, n_pairs * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_TRY(c, hipMemcpyAsync(slots.data(), c->cslots.p, n_pairs * 8 * sizeof(float4), hipMemcpyDeviceToHost, s));
-  HIP_TRY(c, hipStreamSynchronize(s));
-  for (uint64_t k = 0; k < n_pairs; ++k) {
-    out_count[k] = (uint8_t)cnt[k];
-    for (uint32_t q = 0; q < 4; ++q) {
-      float* o = out_points + 28 * k + 7 * q;
-      if (q < cnt[k]) {
-        float4 a = slots[8 * k + 2 * q], b = slots[8 * k + 2 * q + 1];
-        o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = b.x; o[4] = b.y; o[5] = b.z; o[6] = a.w;
-      } else {
-        for (int z = 0; z < 7; ++z) o[z] = 0.0f;
-      }
-    }
-  }
-  return BCH_OK;
+  return from_batch(c, a, collide_pairs_run(a, pairs, n_pairs, seeds, out_count, out_points));
 }
 
 // ------------------------------------------------------ world pipeline --
@@ -1589,7 +1803,7 @@ int bch_world_manifold_reset(bch_context* c) {
 }
 
 // ------------------------------------------------------ triangle meshes --
-int bch_mesh_add(bch_context* c, const float* vertices, uint32_t n_vertices, const uint32_t* indices,
+static int mesh_add_on(bch_context* c, const float* vertices, uint32_t n_vertices, const uint32_t* indices,
                  uint32_t n_triangles, uint32_t* out_mesh) {
   if (!c || !out_mesh || (n_vertices && !vertices) || (n_triangles && !indices))
     return fail(c, BCH_INVALID_ARG, "NULL argument");
@@ -1640,7 +1854,15 @@ int bch_mesh_add(bch_context* c, const float* vertices, uint32_t n_vertices, con
   return BCH_OK;
 }
 
-int bch_mesh_clear(bch_context* c) {
+int bch_mesh_add(bch_context* c, const float* vertices, uint32_t n_vertices, const uint32_t* indices,
+                 uint32_t n_triangles, uint32_t* out_mesh) {
+  if (!c) return BCH_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  BCH_ON_BATCH(c, a);
+  return from_batch(c, a, mesh_add_on(a, vertices, n_vertices, indices, n_triangles, out_mesh));
+}
+
+static int mesh_clear_on(bch_context* c) {
   if (!c) return BCH_INVALID_ARG;
   c->mesh_recs.clear();
   c->mesh_nodes.clear();
@@ -1649,6 +1871,13 @@ int bch_mesh_clear(bch_context* c) {
   c->mesh_order.clear();
   c->mesh_dirty = true;
   return BCH_OK;
+}
+
+int bch_mesh_clear(bch_context* c) {
+  if (!c) return BCH_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  BCH_ON_BATCH(c, a);
+  return from_batch(c, a, mesh_clear_on(a));
 }
 
 // Upload the host-side mesh tables when a mesh was added or cleared.
@@ -1679,7 +1908,7 @@ static int sync_meshes(bch_context* c, hipStream_t s) {
   return BCH_OK;
 }
 
-int bch_collide_mesh_batch(bch_context* c, const bch_bodies* bodies, const uint32_t* pairs, const float* mesh_poses,
+static int collide_mesh_batch_on(bch_context* c, const bch_bodies* bodies, const uint32_t* pairs, const float* mesh_poses,
                            const uint8_t* mesh_first, uint64_t n_pairs, const float* seeds, uint8_t* out_count,
                            float* out_points) {
   if (!c) return BCH_INVALID_ARG;
@@ -1765,6 +1994,15 @@ int bch_collide_mesh_batch(bch_context* c, const bch_bodies* bodies, const uint3
   return BCH_OK;
 }
 
+int bch_collide_mesh_batch(bch_context* c, const bch_bodies* bodies, const uint32_t* pairs, const float* mesh_poses,
+                           const uint8_t* mesh_first, uint64_t n_pairs, const float* seeds, uint8_t* out_count,
+                           float* out_points) {
+  if (!c) return BCH_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  BCH_ON_BATCH(c, a);
+  return from_batch(c, a, collide_mesh_batch_on(a, bodies, pairs, mesh_poses, mesh_first, n_pairs, seeds, out_count, out_points));
+}
+
 // --------------------------------------------------------- CCD sweeps --
 // Shared checks and uploads of the sweep batches: query arrays to the device.
 static int sweep_upload(bch_context* c, uint64_t n, const uint32_t* idx, const uint32_t* caster, const float* from,
@@ -1804,7 +2042,7 @@ static int sweep_download(bch_context* c, uint64_t n, uint8_t* out_hit, float* o
   return BCH_OK;
 }
 
-int bch_sweep_batch(bch_context* c, const bch_bodies* bodies, const uint32_t* target, const uint32_t* caster,
+static int sweep_batch_on(bch_context* c, const bch_bodies* bodies, const uint32_t* target, const uint32_t* caster,
                     const float* from, const float* to, const float* radius, uint64_t n, uint8_t* out_hit,
                     float* out_sweep) {
   if (!c) return BCH_INVALID_ARG;
@@ -1847,7 +2085,16 @@ int bch_sweep_batch(bch_context* c, const bch_bodies* bodies, const uint32_t* ta
   return sweep_download(c, n, out_hit, out_sweep, s);
 }
 
-int bch_sweep_mesh_batch(bch_context* c, const bch_bodies* bodies, const uint32_t* mesh_of, const float* mesh_poses,
+int bch_sweep_batch(bch_context* c, const bch_bodies* bodies, const uint32_t* target, const uint32_t* caster,
+                    const float* from, const float* to, const float* radius, uint64_t n, uint8_t* out_hit,
+                    float* out_sweep) {
+  if (!c) return BCH_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  BCH_ON_BATCH(c, a);
+  return from_batch(c, a, sweep_batch_on(a, bodies, target, caster, from, to, radius, n, out_hit, out_sweep));
+}
+
+static int sweep_mesh_batch_on(bch_context* c, const bch_bodies* bodies, const uint32_t* mesh_of, const float* mesh_poses,
                          const uint32_t* caster, const float* from, const float* to, const float* radius, uint64_t n,
                          uint8_t* out_hit, float* out_sweep) {
   if (!c) return BCH_INVALID_ARG;
@@ -1887,6 +2134,15 @@ int bch_sweep_mesh_batch(bch_context* c, const bch_bodies* bodies, const uint32_
                      c->d_mesh_nodes.as<float4>(), c->d_mesh_order.as<uint32_t>(), c->d_mesh_tris.as<float4>(),
                      c->sw_hit.as<uint8_t>(), c->sw_out.as<float>(), c->counters.as<Counters>());
   return sweep_download(c, n, out_hit, out_sweep, s);
+}
+
+int bch_sweep_mesh_batch(bch_context* c, const bch_bodies* bodies, const uint32_t* mesh_of, const float* mesh_poses,
+                         const uint32_t* caster, const float* from, const float* to, const float* radius, uint64_t n,
+                         uint8_t* out_hit, float* out_sweep) {
+  if (!c) return BCH_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  BCH_ON_BATCH(c, a);
+  return from_batch(c, a, sweep_mesh_batch_on(a, bodies, mesh_of, mesh_poses, caster, from, to, radius, n, out_hit, out_sweep));
 }
 
 // ---------------------------------------------- captured de-penetration --

@@ -8,6 +8,10 @@
  * Host pointers are caller-owned and only read/written during the call.
  * One context per host thread (the reference is single-threaded:
  * `&mut self` everywhere); calls on one context are stream-ordered.
+ * A context holds two independent states: the resident world (bch_world_*,
+ * bch_sd_*, bch_depen_*) and the batch state (the proxy broadphase
+ * bch_bp_*, bch_collide_batch*, the mesh and sweep batches).  Calls on one
+ * never change the other, so one context can serve a whole PhysicsWorld.
  *
  * Conventions (crates/bullet-collision/src/narrowphase/mod.rs:21-23):
  *   - a pair (i, j) always has i < j; pair lists are sorted lexicographically
@@ -133,6 +137,13 @@ int bch_bp_raycast_candidates(bch_context* ctx, const float ray_from[3], const f
  * None); out_points[28k + 7c ..] = contact c.  Host pointers. */
 int bch_collide_batch(bch_context* ctx, const bch_bodies* bodies, const uint32_t* pairs, uint64_t n_pairs,
                       const float* seeds, uint8_t* out_count, float* out_points);
+/* bch_collide_batch on the shapes of the last bch_collide_batch call with
+ * new poses (position[3n], orientation[4n], n = that call's body count):
+ * only the poses are uploaded.  The per-pair loop of step.rs:445-854 sees
+ * the same shapes every step, so only this call runs per step.  Host
+ * pointers; BCH_INVALID_ARG before any bch_collide_batch. */
+int bch_collide_batch_poses(bch_context* ctx, const float* position, const float* orientation, const uint32_t* pairs,
+                            uint64_t n_pairs, const float* seeds, uint8_t* out_count, float* out_points);
 
 /* ------------------------------------------------- triangle meshes -- */
 /* A static triangle mesh (TriangleMesh::new, shapes/triangle_mesh.rs:84-123):
