@@ -28,7 +28,7 @@ EXPORTS = [
     "bch_world_manifold_impulses", "bch_world_manifold_cache", "bch_world_manifold_reset",
     "bch_world_set_poses_predicted", "bch_world_set_poses_predicted_device",
     "bch_mesh_add", "bch_mesh_clear", "bch_collide_mesh_batch", "bch_sweep_batch", "bch_sweep_mesh_batch",
-    "bch_collide_batch_poses",
+    "bch_collide_batch_poses", "bch_collide_batch_static",
 ]
 
 # launch-shape knobs (bch_set_tuning)
@@ -95,6 +95,7 @@ def load():
     L.bch_bp_raycast_candidates.argtypes = [vp, vp, vp, vp, u64, C.POINTER(u64)]
     L.bch_collide_batch.argtypes = [vp, C.POINTER(BchBodies), vp, u64, vp, vp, vp]
     L.bch_collide_batch_poses.argtypes = [vp, vp, vp, vp, u64, vp, vp, vp]
+    L.bch_collide_batch_static.argtypes = [vp, C.POINTER(BchBodies), vp, u64, vp, vp, vp]
     L.bch_world_set_bodies.argtypes = [vp, C.POINTER(BchBodies), u32, vp]
     L.bch_world_set_poses.argtypes = [vp, vp, vp]
     L.bch_world_set_poses_device.argtypes = [vp, vp, vp]

@@ -703,6 +703,22 @@ __global__ void k_split_pairs(uint32_t P, const uint32_t* __restrict__ P_dev, co
   iota[p] = p;
 }
 
+// Input check of the static batch: shape types, hull vertex ranges and pair
+// indices; a bad input sets error bit 6 and nothing else runs.
+__global__ void k_static_check(uint32_t n, const uint8_t* __restrict__ type, const uint32_t* __restrict__ hull_first,
+                               const uint32_t* __restrict__ hull_count, uint32_t nv, uint32_t P,
+                               const uint32_t* __restrict__ pairs, Counters* cnt) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (i < n) {
+    uint8_t t = type[i];
+    if (t > PLANE) bad = true;
+    else if (t == HULL) bad = !hull_first || !hull_count || (uint64_t)hull_first[i] + hull_count[i] > nv;
+  }
+  if (i < P) bad = bad || pairs[2 * (uint64_t)i] >= n || pairs[2 * (uint64_t)i + 1] >= n;
+  if (bad) atomicOr(&cnt->errors, 64u);
+}
+
 // Proxy arena update: slot idx[k] takes the box (lo, hi) = upd[2k], upd[2k + 1].
 __global__ void k_arena_scatter(uint32_t m, const uint32_t* __restrict__ idx, const float4* __restrict__ upd,
                                 float4* __restrict__ arena) {

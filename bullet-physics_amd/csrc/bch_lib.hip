@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <set>
 #include <string>
 #include <vector>
@@ -95,6 +96,53 @@ constexpr int kTraverseW = 1;  // tree leaves per lane in k_traverse
 constexpr uint32_t kEpaDeferGrid = 16384;  // one-pair EPA blocks for the deferred entries (~4% of the queue)
 inline uint32_t blocks(uint64_t n, int b = kBlock) { return (uint32_t)((n + b - 1) / b); }
 
+// Working memory of bch_collide_batch_static, the entry point that allocates
+// nothing: one static device array per GPU, carved into the buffers the
+// narrowphase over caller pairs uses.
+constexpr size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t kStP = BCH_STATIC_MAX_PAIRS, kStN = BCH_STATIC_MAX_BODIES, kStV = BCH_STATIC_MAX_HULL_VERTICES;
+constexpr size_t kStTiles = (kStP + kRsTile - 1) / kRsTile;
+constexpr size_t kStBodies = 0;
+constexpr size_t kStBtype = kStBodies + al256(kStN * sizeof(BodyRec));
+constexpr size_t kStHull = kStBtype + al256(kStN);
+constexpr size_t kStKeys = kStHull + al256(kStV * sizeof(float4));
+constexpr size_t kStPairs = kStKeys + al256(kStP * sizeof(uint64_t));
+constexpr size_t kStU32 = kStPairs + al256(kStP * 2 * sizeof(uint32_t));     // cls, cls_alt, perm, perm_alt
+constexpr size_t kStCount = kStU32 + 4 * al256(kStP * sizeof(uint32_t));    // ccount, coffsets
+constexpr size_t kStSlots = kStCount + 2 * al256((kStP + 1) * sizeof(uint32_t));
+constexpr size_t kStEpa = kStSlots + al256(kStP * 8 * sizeof(float4));       // two EPA queues
+constexpr size_t kStCounters = kStEpa + 2 * al256(kStP * 4 * sizeof(float4));
+constexpr size_t kStHist = kStCounters + al256(sizeof(Counters));
+constexpr size_t kStOffs = kStHist + al256(256 * kStTiles * sizeof(uint32_t));
+constexpr size_t kStPart = kStOffs + al256((256 * kStTiles + 1) * sizeof(uint32_t));  // the sort's scan partials
+constexpr size_t kStParts = (256 * kStTiles + kScanTile - 1) / kScanTile;
+constexpr size_t kStBytes = kStPart + al256(kStParts * sizeof(uint32_t));
+static_assert(kStBytes == BCH_STATIC_BYTES, "BCH_STATIC_BYTES must equal the carved layout");
+
+}  // namespace
+
+// The array itself.  In host code `scratch` names a host-side stand-in, not
+// the device address: the address comes from hipGetSymbolAddress, once per
+// device (scratch_init), and only that address is ever handed to kernels.
+static __device__ __align__(256) unsigned char scratch[kStBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// Resolve the device address of `scratch` on `device` (current); once per device.
+int scratch_init(int device, void** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) return BCH_NO_DEVICE;
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return BCH_HIP_ERROR;
+    g_scratch[device] = p;
+  }
+  *out = g_scratch[device];
+  return BCH_OK;
+}
 }  // namespace
 
 struct bch_context {
@@ -195,6 +243,12 @@ struct bch_context {
   // stream, so they never touch the world-resident state above.
   bch_context* aux = nullptr;
   bool is_aux = false;
+
+  // bch_collide_batch_static runs on this context: its buffers are views into
+  // the GPU's static array, its streams and events are borrowed from the
+  // owner, and a buffer that would need to grow is an error.
+  bch_context* stat = nullptr;
+  bool no_alloc = false;
 };
 
 namespace {
@@ -215,6 +269,7 @@ int fail(bch_context* c, int code, const std::string& msg) {
   do {                                                                                       \
     size_t need_ = (want);                                                                   \
     if (need_ > (buf).bytes) {                                                               \
+      if ((ctx)->no_alloc) return fail(ctx, BCH_INVALID_ARG, "input exceeds the static scratch"); \
       if ((ctx)->capturing) return fail(ctx, BCH_HIP_ERROR, "buffer growth during capture"); \
       ++(ctx)->realloc_gen;                                                                  \
     }                                                                                        \
@@ -223,7 +278,7 @@ int fail(bch_context* c, int code, const std::string& msg) {
 
 // Stage boundary event (not while capturing: the graph is timed as a whole).
 void mark(bch_context* c, int k) {
-  if (!c->capturing) (void)hipEventRecord(c->ev[k], c->stream);
+  if (!c->capturing && !c->no_alloc) (void)hipEventRecord(c->ev[k], c->stream);
 }
 
 // Exclusive scan of n u32 values: out[0..n) offsets, out[n] the total.  With
@@ -970,6 +1025,34 @@ int bch_create(int device, bch_context** out) {
     return BCH_HIP_ERROR;
   }
   for (auto& e : c->ev) (void)hipEventCreate(&e);
+  // the static scratch of this device (bch_collide_batch_static): its
+  // address, and the views its context works on
+  void* sp = nullptr;
+  if (scratch_init(device, &sp) != BCH_OK) {
+    bch_destroy(c);
+    return BCH_HIP_ERROR;
+  }
+  bch_context* st = new bch_context();
+  st->device = device;
+  st->no_alloc = true;
+  char* b = static_cast<char*>(sp);
+  st->bodies.view(b + kStBodies, kStN * sizeof(BodyRec));
+  st->btype.view(b + kStBtype, kStN);
+  st->hull.view(b + kStHull, kStV * sizeof(float4));
+  st->pair_keys.view(b + kStKeys, kStP * sizeof(uint64_t));
+  st->pairs.view(b + kStPairs, kStP * 2 * sizeof(uint32_t));
+  DevBuf* u32s[4] = {&st->cls, &st->cls_alt, &st->perm, &st->perm_alt};
+  for (int k = 0; k < 4; ++k) u32s[k]->view(b + kStU32 + k * al256(kStP * sizeof(uint32_t)), kStP * sizeof(uint32_t));
+  st->ccount.view(b + kStCount, (kStP + 1) * sizeof(uint32_t));
+  st->coffsets.view(b + kStCount + al256((kStP + 1) * sizeof(uint32_t)), (kStP + 1) * sizeof(uint32_t));
+  st->cslots.view(b + kStSlots, kStP * 8 * sizeof(float4));
+  st->epa_q.view(b + kStEpa, kStP * 4 * sizeof(float4));
+  st->epa_q2.view(b + kStEpa + al256(kStP * 4 * sizeof(float4)), kStP * 4 * sizeof(float4));
+  st->counters.view(b + kStCounters, sizeof(Counters));
+  st->rs_hist.view(b + kStHist, 256 * kStTiles * sizeof(uint32_t));
+  st->rs_offs.view(b + kStOffs, (256 * kStTiles + 1) * sizeof(uint32_t));
+  st->scan_part.view(b + kStPart, kStParts * sizeof(uint32_t));
+  c->stat = st;
   *out = c;
   return BCH_OK;
 }
@@ -1001,6 +1084,7 @@ void bch_destroy(bch_context* c) {
   c->proxy_box.release();
   c->arena_stage.release();
   if (c->aux) bch_destroy(c->aux);
+  delete c->stat;  // views only: nothing of the static array is freed
   for (auto& e : c->ev) (void)hipEventDestroy(e);
   (void)hipEventDestroy(c->fork_ev);
   (void)hipEventDestroy(c->join_ev);
@@ -2322,6 +2406,66 @@ int bch_world_results_device(bch_context* c, const uint32_t** d_pairs, const uin
   if (d_pairs) *d_pairs = c->pairs.as<uint32_t>();
   if (d_offsets) *d_offsets = c->coffsets.as<uint32_t>();
   if (d_contacts) *d_contacts = c->contacts.as<float>();
+  return BCH_OK;
+}
+
+// ------------------------------------------------ no allocation (static) --
+// bch_collide_batch over device arrays with the static scratch as working
+// memory: no hipMalloc, hipFree or allocating library call.  The inputs are
+// checked on the device first (shape types, hull ranges, pair indices), so
+// no kernel reads past a caller array.
+int bch_collide_batch_static(bch_context* c, const bch_bodies* d_bodies, const uint32_t* d_pairs, uint64_t n_pairs,
+                             const float* d_seeds, uint8_t* d_out_count, float* d_out_points) {
+  if (!c) return BCH_INVALID_ARG;
+  if (!d_bodies) return fail(c, BCH_INVALID_ARG, "bodies is NULL");
+  const bch_bodies* b = d_bodies;
+  uint32_t n = b->n;
+  uint32_t nv = b->hull_vertices ? b->n_hull_vertices : 0;
+  if (n > BCH_STATIC_MAX_BODIES) return fail(c, BCH_INVALID_ARG, "more bodies than BCH_STATIC_MAX_BODIES");
+  if (nv > BCH_STATIC_MAX_HULL_VERTICES)
+    return fail(c, BCH_INVALID_ARG, "more hull vertices than BCH_STATIC_MAX_HULL_VERTICES");
+  if (n_pairs > BCH_STATIC_MAX_PAIRS) return fail(c, BCH_INVALID_ARG, "more pairs than BCH_STATIC_MAX_PAIRS");
+  if (n && (!b->position || !b->orientation || !b->shape_type || !b->shape_params))
+    return fail(c, BCH_INVALID_ARG, "body arrays must be non-NULL");
+  if (n_pairs && (!d_pairs || !d_out_count || !d_out_points)) return fail(c, BCH_INVALID_ARG, "NULL argument");
+  HIP_TRY(c, hipSetDevice(c->device));
+  bch_context* a = c->stat;
+  a->stream = c->stream;
+  a->side = c->side;
+  a->fork_ev = c->fork_ev;
+  a->join_ev = c->join_ev;
+  a->hull_lanes = c->hull_lanes;
+  a->epa_grid = c->epa_grid;
+  a->np_grid = c->np_grid;
+  a->cap_mode = false;
+  hipStream_t s = a->stream;
+  Counters* cnt = a->counters.as<Counters>();
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, cnt);
+  uint64_t span = std::max<uint64_t>(std::max<uint64_t>(n, n_pairs), 1);
+  hipLaunchKernelGGL(k_static_check, dim3(blocks(span)), dim3(kBlock), 0, s, n, b->shape_type, b->hull_first,
+                     b->hull_count, nv, (uint32_t)n_pairs, d_pairs, cnt);
+  HIP_TRY(c, hipMemcpyAsync(&a->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+  HIP_TRY(c, hipStreamSynchronize(s));
+  if (a->host_cnt.errors & 64u)
+    return fail(c, BCH_INVALID_ARG, "shape type, hull vertex range or pair index out of range");
+  if (!n_pairs) return BCH_OK;
+  if (nv)
+    hipLaunchKernelGGL(k_xyz_to_float4, dim3(blocks(nv)), dim3(kBlock), 0, s, nv, b->hull_vertices,
+                       a->hull.as<float4>());
+  if (n)
+    hipLaunchKernelGGL(k_pack_bodies, dim3(blocks(n)), dim3(kBlock), 0, s, n, b->position, b->orientation,
+                       b->shape_type, b->shape_params, b->hull_first, b->hull_count, a->hull.as<float4>(),
+                       (const uint32_t*)nullptr, a->bodies.as<BodyRec>(), a->btype.as<uint8_t>());
+  a->P = n_pairs;
+  hipLaunchKernelGGL(k_pairs_to_keys, dim3(blocks(n_pairs)), dim3(kBlock), 0, s, (uint32_t)n_pairs,
+                     reinterpret_cast<const uint2*>(d_pairs), a->pair_keys.as<uint64_t>());
+  int rc = narrow_split(a);
+  if (!rc) rc = narrow_finish(a, d_seeds, false);
+  if (rc) return from_batch(c, a, rc);
+  if (a->host_cnt.errors & 1u) return fail(c, BCH_HIP_ERROR, "box-box clip polygon overflow");
+  hipLaunchKernelGGL(k_batch_out, dim3(blocks(n_pairs)), dim3(kBlock), 0, s, (uint32_t)n_pairs,
+                     a->ccount.as<uint32_t>(), a->cslots.as<float4>(), d_out_count, d_out_points);
+  HIP_TRY(c, hipGetLastError());
   return BCH_OK;
 }
 

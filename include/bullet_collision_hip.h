@@ -393,6 +393,29 @@ int bch_depen_history(bch_context* ctx, float* loss, float* max_pen, uint32_t* n
  * bch_world_set_bodies and bch_destroy free it too). */
 int bch_depen_release(bch_context* ctx);
 
+/* --------------------------------------------- no allocation (static) -- */
+/* No counterpart in the reference: this is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  bch_collide_batch with device arrays:
+ * every pointer inside d_bodies and every array argument is a device pointer
+ * (d_bodies itself, with n and n_hull_vertices, is host memory); d_pairs
+ * u32[2 n_pairs], d_seeds f32[3 n_pairs] or NULL, d_out_count u8[n_pairs],
+ * d_out_points f32[28 n_pairs] in bch_collide_batch's layout.  The working
+ * memory is one static device array of BCH_STATIC_BYTES per GPU, held until
+ * the process ends; the call makes no hipMalloc, hipFree or allocating
+ * library call.  Inputs beyond BCH_STATIC_MAX_PAIRS pairs,
+ * BCH_STATIC_MAX_BODIES bodies or BCH_STATIC_MAX_HULL_VERTICES hull vertices,
+ * and bad shape types, hull ranges or pair indices (checked on the device
+ * first), return BCH_INVALID_ARG.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other threads, other contexts, other
+ * streams) race.  The outputs are written in stream order. */
+#define BCH_STATIC_MAX_PAIRS 32768u
+#define BCH_STATIC_MAX_BODIES 65536u
+#define BCH_STATIC_MAX_HULL_VERTICES 131072u
+#define BCH_STATIC_BYTES 16073984u
+int bch_collide_batch_static(bch_context* ctx, const bch_bodies* d_bodies, const uint32_t* d_pairs, uint64_t n_pairs,
+                             const float* d_seeds, uint8_t* d_out_count, float* d_out_points);
+
 #ifdef __cplusplus
 }
 #endif
