@@ -27,10 +27,6 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 STAGES = ["aabb", "tree_build", "tree_traverse", "pair_list", "narrow_fork", "narrow_gjk", "epa",
           "contact_compaction"]
-# stages that are one kernel launch each (EPA: k_epa_dual plus the short one-pair launch over
-# the entries it defers): the roofline candidates.  narrow_gjk holds k_np_hull and the
-# k_np launches running beside it on a second stream, so it is no single kernel.
-SINGLE_KERNEL = {"tree_traverse": "k_traverse", "epa": "k_epa_dual"}
 
 
 def log(msg):
@@ -38,32 +34,43 @@ def log(msg):
 
 
 def algorithmic_bytes(sc, pairs, offs, routes, kernel):
-    """Algorithmic HBM bytes one launch of `kernel` must move (each datum once).
+    """Algorithmic HBM bytes one launch of `kernel` must move, each datum once:
+    a body (and its hull) is charged once per launch however many of the
+    launch's pairs touch it.
 
     Body record = pose 28 B + shape 16 B = 44 B; hull vertex = 12 B; contact
     = position 12 + normal 12 + depth 4 = 28 B.
     k_traverse  per body: its AABB 24 B read; per pair: 8 B written.
-    k_np_hull   per pair with a hull: pair index 4 + pair 8 + two bodies 88
-                + both hulls' vertices + count 4 + its band contacts 28 each,
-                + 64 B queue entry for a pair sent on to EPA.
-    k_epa_dual  per queued pair: queue entry 64 + pair 8 + two bodies 88 +
-                both hulls' vertices + count 4 + route 8 (read, write) +
-                28 per contact (the stage includes the one-pair k_epa_wave
-                launch over the few entries the two-pair kernel defers).
+    k_np_hull   per pair with a hull: pair index 4 + pair 8 + count 4 + its
+                band contacts 28 each, + 64 B queue entry for a pair sent on
+                to EPA; per distinct body of those pairs: 44 + its vertices.
+    k_epa_dual  per queued pair: queue entry 64 + pair 8 + count 4 + route 8
+                (read, write) + 28 per contact; per distinct body of those
+                pairs: 44 + its vertices (the stage includes the one-pair
+                k_epa_wave launch over the few entries the two-pair kernel
+                defers).
+    pass        the whole pass: N bodies x 44 + 12 x every hull vertex +
+                P x (8 pair + 4 count) + 28 per contact.
     """
     P = pairs.shape[0]
     hc = sc["hull_count"].astype(np.int64)
-    hv = hc[pairs[:, 0]] + hc[pairs[:, 1]]
     ncon = np.diff(offs.astype(np.int64))
     cls = routes & 7
     epa = (routes & 8) != 0
+
+    def bodies(m):
+        ids = np.unique(pairs[m].reshape(-1))
+        return int(ids.shape[0] * 44 + 12 * hc[ids].sum())
+
     if kernel == "k_traverse":
         return int(sc["shape_type"].shape[0] * 24 + 8 * P)
     if kernel == "k_np_hull":
         m = cls == 6
-        return int(m.sum() * (4 + 8 + 88 + 4) + 12 * hv[m].sum() + 28 * ncon[m & ~epa].sum() + 64 * (m & epa).sum())
+        return int(m.sum() * (4 + 8 + 4) + 28 * ncon[m & ~epa].sum() + 64 * (m & epa).sum()) + bodies(m)
     if kernel == "k_epa_dual":
-        return int(epa.sum() * (64 + 8 + 88 + 4 + 8) + 12 * hv[epa].sum() + 28 * ncon[epa].sum())
+        return int(epa.sum() * (64 + 8 + 4 + 8) + 28 * ncon[epa].sum()) + bodies(epa)
+    if kernel == "pass":
+        return int(sc["shape_type"].shape[0] * 44 + 12 * hc.sum() + P * 12 + 28 * ncon.sum())
     return None
 
 
@@ -390,6 +397,222 @@ def depenetration(bpa, steps=100):
                                      "every 25 steps"}}
 
 
+def valu_calibration(blocks=8192, rounds=2000, reps=5):
+    """The VALU issue rate the hardware sustains: k_valu_probe (libbch_probe.so)
+    runs 8 independent v_fma_f32 chains per lane, 128 FMAs per round, at 8
+    blocks of 256 threads per CU.  Reported per SIMD-cycle at 2.4 GHz over
+    1024 SIMDs, the same convention as roofline.valu."""
+    import ctypes as C
+    import torch
+    sink = torch.empty(blocks * 256, device="cuda")
+    L = C.CDLL(os.path.join(ROOT, "bullet-physics_amd", "libbch_probe.so"))
+    L.bch_probe_valu.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_float)]
+    ms = C.c_float()
+    rc = L.bch_probe_valu(blocks, rounds, reps, C.c_void_p(sink.data_ptr()), C.byref(ms))
+    assert rc == 0, "probe launch failed"
+    fma = blocks * 4 * rounds * 128  # wave64 v_fma_f32 instructions per launch (4 waves per block)
+    rate = fma / (1024 * ms.value * 1e-3 * 2.4e9)
+    return {"probe": "k_valu_probe", "blocks": blocks, "rounds": rounds, "ms_per_launch": round(ms.value, 4),
+            "fma_wave_instructions": fma, "per_simd_cycle_at_2.4GHz": round(rate, 4)}
+
+
+def scene_boxes(sc, margin):
+    """World AABBs of a scene's convex bodies (numpy, float64 rounded to f32):
+    box corners |R| h, capsule segment ends +- r, hull vertex extremes,
+    sphere +- r; padded by the margin.  Proxy boxes for the drop-in legs."""
+    q = sc["orientation"].astype(np.float64)
+    x, y, z, w = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    R = np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], 1),
+                  np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], 1),
+                  np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], 1)], 1)
+    t = sc["shape_type"]
+    prm = sc["shape_params"].astype(np.float64)
+    half = np.zeros((t.shape[0], 3))
+    m = t == 1
+    half[m] = np.einsum("nij,nj->ni", np.abs(R[m]), prm[m, :3])
+    m = t == 0
+    half[m] = prm[m, :1]
+    m = t == 2
+    half[m] = np.abs(R[m][:, :, 1]) * prm[m, 1:2] + prm[m, :1]
+    pos = sc["position"].astype(np.float64)
+    lo, hi = pos - half, pos + half
+    hull = np.nonzero(t == 3)[0]
+    if hull.size:
+        first = sc["hull_first"][hull].astype(np.int64)
+        cnt = sc["hull_count"][hull].astype(np.int64)
+        owner = np.repeat(np.arange(hull.size), cnt)
+        vidx = np.concatenate([np.arange(f, f + c) for f, c in zip(first, cnt)])
+        v = np.einsum("nij,nj->ni", R[hull][owner], sc["hull_vertices"][vidx].astype(np.float64))
+        starts = np.concatenate(([0], np.cumsum(cnt)[:-1]))
+        lo[hull] = pos[hull] + np.minimum.reduceat(v, starts, axis=0)
+        hi[hull] = pos[hull] + np.maximum.reduceat(v, starts, axis=0)
+    return (lo - margin).astype(np.float32), (hi + margin).astype(np.float32)
+
+
+def dropin_broadphase(bpa, sc, margin, steps, moved=0.3):
+    """The drop-in BroadPhase (bch_bp_*) over the scene's bodies as proxies:
+    per step set_aabbs on a random 30 % of them (moved by ~5 mm), then
+    compute_pairs into a host list, as the Rust world's update_aabbs +
+    compute_pairs_into would call it.  Host wall time per step."""
+    lo, hi = scene_boxes(sc, margin)
+    n = lo.shape[0]
+    bp = bpa.BroadPhase()
+    t0 = time.perf_counter()
+    for i in range(n):
+        bp.create_proxy(lo[i], hi[i])
+    create_s = time.perf_counter() - t0
+    pairs = bp.compute_pairs()
+    rng = np.random.default_rng(3)
+    k = int(moved * n)
+    sets, tc, ts = [], [], []
+    for _ in range(max(steps, 1) + 1):
+        idx = rng.choice(n, k, replace=False).astype(np.uint32)
+        d = rng.normal(0.0, 0.005, (k, 3)).astype(np.float32)
+        sets.append((idx, lo[idx] + d, hi[idx] + d))
+    for s_ in range(len(sets)):
+        idx, a, b = sets[s_]
+        t1 = time.perf_counter()
+        bp.set_aabbs(idx, a, b)
+        t2 = time.perf_counter()
+        pairs = bp.compute_pairs()
+        t3 = time.perf_counter()
+        if s_:  # the first round is a warm-up
+            ts.append(t2 - t1)
+            tc.append(t3 - t2)
+    step = float(np.median(np.add(ts, tc)))
+    bp.ctx.close()
+    return {"workload": f"{n} proxies (the scene's AABBs), {k} moved by set_aabbs per step",
+            "pairs": int(pairs.shape[0]), "ms_per_step": round(step * 1e3, 3),
+            "set_aabbs_ms": round(float(np.median(ts)) * 1e3, 3),
+            "compute_pairs_ms": round(float(np.median(tc)) * 1e3, 3),
+            "pairs_per_s": round(pairs.shape[0] / step, 1), "create_proxies_s": round(create_s, 2),
+            "includes": "host mirror updates, upload and device scatter of the moved slots, broadphase, "
+                        "pairs copied into a host array"}
+
+
+def dropin_collide_batch(bpa, sc, pairs, steps):
+    """bch_collide_batch over the pass's pairs with host buffers both ways
+    (bodies + pairs in, 4 x 7-float points per pair out), and the pose-only
+    bch_collide_batch_poses that a world step would call after the first."""
+    import ctypes as C
+    ctx = bpa._Context(0)
+    L, h = ctx.L, ctx.h
+    b = bpa.Bodies(sc)
+    pr = np.ascontiguousarray(pairs, np.uint32)
+    P = pr.shape[0]
+    # output arrays reused across steps, as a caller's buffers would be
+    cnt, pts = np.zeros(P, np.uint8), np.zeros((P, 4, 7), np.float32)
+    cnt2, pts2 = np.zeros(P, np.uint8), np.zeros((P, 4, 7), np.float32)
+    full = lambda: L.bch_collide_batch(h, C.byref(b.c), pr.ctypes.data, P, None, cnt.ctypes.data, pts.ctypes.data)
+    poses = lambda: L.bch_collide_batch_poses(h, b.pos.ctypes.data, b.quat.ctypes.data, pr.ctypes.data, P, None,
+                                              cnt2.ctypes.data, pts2.ctypes.data)
+    assert full() == 0 and poses() == 0
+    tf, tp = [], []
+    for _ in range(max(min(steps, 5), 1)):
+        t1 = time.perf_counter()
+        assert full() == 0
+        tf.append(time.perf_counter() - t1)
+        t1 = time.perf_counter()
+        assert poses() == 0
+        tp.append(time.perf_counter() - t1)
+    ctx.close()
+    f, p = float(np.median(tf)), float(np.median(tp))
+    return {"workload": f"{P} caller pairs of the scene", "contacts": int(cnt.sum()),
+            "results_equal": bool(np.array_equal(cnt, cnt2) and np.array_equal(pts, pts2)),
+            "full_ms": round(f * 1e3, 3), "full_pairs_per_s": round(P / f, 1),
+            "poses_ms": round(p * 1e3, 3), "poses_pairs_per_s": round(P / p, 1),
+            "includes": "host arrays in (bodies or poses, pairs), 113 B per pair out over PCIe"}
+
+
+def gradient_check(bpa, oracle_mod, n_worlds=16, directions=16, h=1e-4):
+    """The de-penetration gradient (worlds of the configs[4] distributions) against
+    f64 central differences of the same penalty evaluated by the oracle (its own
+    broadphase and f64 signed distance), along directions that move three bodies."""
+    import torch
+    diff = importlib.import_module("bullet-physics_amd.diff")
+    sc = bpa.scenes.batched_boxes_hulls(n_worlds, 5)
+    w = bpa.CollisionWorld(sc)
+    pos = torch.from_numpy(sc["position"]).cuda().requires_grad_(True)
+    quat = torch.from_numpy(sc["orientation"]).cuda().requires_grad_(True)
+    loss, pen, pairs = diff.penetration_loss(w, pos, quat)
+    loss.backward()
+    gp = pos.grad.cpu().numpy().astype(np.float64)
+    gq = quat.grad.cpu().numpy().astype(np.float64)
+    with torch.no_grad():
+        _, n32, _, _, _ = diff.signed_distance(w, pos, quat, pairs)
+    n32 = n32.cpu().numpy().astype(np.float64)
+    pr32 = pairs.cpu().numpy().astype(np.int64)
+    pen32 = pen.detach().cpu().numpy()
+    w.ctx.set_stream(None)
+
+    def loss64(p, q):
+        s2 = dict(sc)
+        s2["position"] = p.astype(np.float32)
+        s2["orientation"] = q.astype(np.float32)
+        pr = oracle_mod.sweep_pairs(*oracle_mod.scene_aabbs(s2, 0.04))
+        i = pr.astype(np.int64)
+        pose = np.concatenate([p[i[:, 0]], q[i[:, 0]], p[i[:, 1]], q[i[:, 1]]], axis=1)
+        s2["pairs"] = pr
+        out, _ = oracle_mod.sd_pairs(s2, pr, pose, f64=True)
+        e = np.maximum(-out[:, 0], 0.0)
+        return float(np.sum(e * e))
+
+    p0, q0 = sc["position"].astype(np.float64), sc["orientation"].astype(np.float64)
+    l0 = loss64(p0, q0)
+    # the f64 normals of the same pairs: where EPA's f32 and f64 runs stop on
+    # different faces (two directions of near-equal depth, both within EPA's
+    # 1e-4 exit), each gradient is that of its own face
+    s0 = dict(sc)
+    s0["pairs"] = pr32.astype(np.uint32)
+    o64, _ = oracle_mod.sd_pairs(s0, s0["pairs"], np.concatenate(
+        [p0[pr32[:, 0]], q0[pr32[:, 0]], p0[pr32[:, 1]], q0[pr32[:, 1]]], axis=1), f64=True)
+    ang = np.arctan2(np.linalg.norm(np.cross(n32, o64[:, 1:4]), axis=1), np.sum(n32 * o64[:, 1:4], axis=1))
+    rng = np.random.default_rng(8)
+    rel, nonsmooth, face_tie = [], 0, []
+    for _ in range(directions):
+        vp, vq = np.zeros_like(p0), np.zeros_like(q0)
+        bs = rng.choice(p0.shape[0], 3, replace=False)
+        for b in bs:
+            vp[b] = rng.normal(size=3)
+            vq[b] = rng.normal(size=4)
+        touched = (np.isin(pr32[:, 0], bs) | np.isin(pr32[:, 1], bs)) & (pen32 > 0)
+        if touched.any() and ang[touched].max() > 1e-2:
+            face_tie.append(round(float(ang[touched].max()), 4))
+            continue
+        g = float(np.sum(gp * vp) + np.sum(gq * vq))
+        fd, jump = [], []
+        for hh in (h, h / 2):
+            lp, lm = loss64(p0 + hh * vp, q0 + hh * vq), loss64(p0 - hh * vp, q0 - hh * vq)
+            fd.append((lp - lm) / (2 * hh))
+            jump.append(abs(lp - 2 * l0 + lm) / hh)
+        scale = max(abs(fd[0]), 1e-3)
+        ratio = jump[1] / max(jump[0], 1e-300)
+        if jump[0] <= 1e-3 * scale or 0.3 <= ratio <= 0.7:
+            # the h difference, or its Richardson extrapolation with h / 2 where
+            # the difference's own truncation error dominates
+            rel.append(min(abs(g - fd[0]), abs(g - (4 * fd[1] - fd[0]) / 3)) / scale)
+        else:
+            nonsmooth += 1
+    return {"workload": f"{n_worlds} worlds of boxes and hulls (the de-penetration distributions), "
+                        f"{int(pairs.shape[0])} pairs, {int((pen > 0).sum())} penetrating",
+            "loss_f32": float(loss.detach()), "loss_f64": l0, "directions": directions, "smooth": len(rel),
+            "non_smooth_within_h": nonsmooth,
+            "epa_face_ties": {"directions": len(face_tie), "normal_angle_rad": face_tie,
+                              "pairs_over_1e-2_rad": int(((pen32 > 0) & (ang > 1e-2)).sum()),
+                              "penetrating_pairs": int((pen32 > 0).sum()),
+                              "note": "directions moving a body of a pair whose f32 (GPU) and f64 EPA normals differ "
+                                      "by more than 1e-2 rad: the two runs stopped on different faces of near-equal "
+                                      "depth, so their gradients are those of different faces; not held to the bar"},
+            "relative_error_median": float(np.median(rel)) if rel else None,
+            "relative_error_max": float(np.max(rel)) if rel else None,
+            "within_1e-3": int(sum(r <= 1e-3 for r in rel)),
+            "h": h, "note": "f64 central differences of the oracle's penalty along directions moving 3 bodies (the "
+                            "better of the h difference and its Richardson extrapolation with h/2); a direction "
+                            "is non-smooth when its second difference does not halve with h. The oracle's f64 "
+                            "adjoint meets these differences to ~1e-7 in every direction, so what remains is the "
+                            "f32 forward's rounding"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -408,7 +631,12 @@ def main():
                     help="fixed launch knobs, e.g. hull_lanes=8,epa_grid=65536,np_grid=4096, instead of autotuning")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device-to-device copy three times (HBM counter calibration)")
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="only run the VALU issue-rate probe (independent FMA chains, full occupancy)")
     args = ap.parse_args()
+    if args.valu_calibration:
+        print(json.dumps(valu_calibration()), flush=True)
+        return
     if args.copy_calibration:
         import torch
         n = 64 * 1024 * 1024
@@ -434,7 +662,7 @@ def main():
         # want every pass at the same launch shapes)
         knobs = {k: int(v) for k, v in (kv.split("=") for kv in args.tuning.split(","))}
         w.ctx.set_tuning(**knobs)
-        w.collide(args.margin)  # sizes the per-leaf pair slots
+        w.collide(args.margin)  # sizes the pair buffers
         tuned = dict(w.ctx.tuning(), fixed=True)
     else:
         tuned = w.autotune(args.margin, trials=3)
@@ -443,15 +671,19 @@ def main():
         P, Cn = w.collide(args.margin)
         log(f"warmup {k}: {P} pairs, {Cn} contacts")
     stage_ms = np.zeros(len(STAGES))
+    epa_dual_ms = 0.0
     w.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         P, Cn = w.collide(args.margin)
-        stage_ms += w.timings()
+        t_all = w.timings_all()
+        stage_ms += t_all[:8]
+        epa_dual_ms += float(t_all[8])
     w.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     stage_ms /= args.steps
+    epa_dual_ms /= args.steps
     log("stages (ms): " + ", ".join(f"{s} {v:.3f}" for s, v in zip(STAGES, stage_ms)))
     pairs, offs, cont = w.results()
     value = P / (ms * 1e-3)
@@ -476,11 +708,12 @@ def main():
     log(f"captured pass: {graph}")
     routes = w.routes()
     dom = STAGES[int(np.argmax(stage_ms))]
-    cand = [st for st in SINGLE_KERNEL]
-    roof_stage = max(cand, key=lambda st: stage_ms[STAGES.index(st)])
-    kern = SINGLE_KERNEL[roof_stage]
+    # single-kernel times from the pass's own HIP events: the traversal stage
+    # is k_traverse alone, and event 9 ends k_epa_dual inside the EPA stage
+    kernel_ms = {"k_traverse": float(stage_ms[STAGES.index("tree_traverse")]), "k_epa_dual": epa_dual_ms}
+    kern = max(kernel_ms, key=kernel_ms.get)
     ab = algorithmic_bytes(sc, pairs, offs, routes, kern)
-    dur = stage_ms[STAGES.index(roof_stage)] * 1e-3
+    dur = kernel_ms[kern] * 1e-3
     achieved = ab / dur / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"{kern}_mixed_1m_traffic.json")
@@ -490,18 +723,32 @@ def main():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
             "algorithmic_bytes": ab, "kernel_ms": round(dur * 1e3, 4), "dominant_stage": dom,
             "routes": {"epa_pairs": int(((routes & 8) != 0).sum()), "hull_pairs": int(((routes & 7) == 6).sum())}}
+    pb = algorithmic_bytes(sc, pairs, offs, routes, "pass")
+    roof["pass"] = {"algorithmic_bytes": pb, "ms": round(ms, 4), "achieved": round(pb / (ms * 1e-3) / 1e9, 2),
+                    "frac": round(pb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                    "note": "the whole pass's bytes (each body, hull vertex, pair, count and contact once) over "
+                            "the pass time"}
     # The kernel's real bound is instruction issue: its VALU wave-instructions
     # per launch (SQ counters, profiles/<kernel>_mixed_1m_valu.json) over
-    # 1024 SIMDs x the live stage time x 2.4 GHz, against one wave64 VALU
-    # instruction per 4 cycles per SIMD.
+    # 1024 SIMDs x its own event time x 2.4 GHz, against the rate the FMA
+    # probe sustains (profiles/valu_probe_calibration.json).
     vf = os.path.join(ROOT, "profiles", f"{kern}_mixed_1m_valu.json")
     if os.path.exists(vf):
         vi = json.load(open(vf))["valu_instructions_per_dispatch"]
         rate = vi / (1024 * dur * 2.4e9)
-        roof["valu"] = {"instructions": vi, "achieved_per_simd_cycle": round(rate, 4), "peak_per_simd_cycle": 0.25,
-                        "frac": round(rate / 0.25, 3),
-                        "note": "instruction count from the SQ counters of the same kernel and workload; time = "
-                                "this run's stage time (the kernel plus the short one-pair launch after it)"}
+        cal = os.path.join(ROOT, "profiles", "valu_probe_calibration.json")
+        ceil = json.load(open(cal))["per_simd_cycle_at_2.4GHz"] if os.path.exists(cal) else None
+        roof["valu"] = {"instructions": vi, "achieved_per_simd_cycle": round(rate, 4),
+                        "peak_per_simd_cycle": ceil, "frac": round(rate / ceil, 3) if ceil else None,
+                        "note": "instructions from the SQ counters of the same kernel and workload over this "
+                                "run's k_epa_dual time; the peak is the rate the FMA probe sustains "
+                                "(bench.py --valu-calibration), both per SIMD-cycle at 2.4 GHz"}
+
+    dropin = None
+    if not args.only_pass:
+        dropin = {"broadphase": dropin_broadphase(bpa, sc, args.margin, min(args.steps, 5)),
+                  "collide_batch": dropin_collide_batch(bpa, sc, pairs, args.steps)}
+        log(f"drop-in entry points: {dropin}")
 
     # last: it moves the bodies
     manifold = None if args.only_pass else manifold_frames(w, sc, args.margin, 5)
@@ -534,6 +781,10 @@ def main():
         except Exception:
             cpu["cpu_model"] = platform.processor()
         log(f"cpu baseline: {cpu['value']:.0f} pairs/s, all cores {cpu['all_cores']['value']:.0f} ({cpu['sample']})")
+        if diff_out is not None:
+            # the oracle as the checker of the de-penetration gradient
+            diff_out["gradient_check"] = gradient_check(bpa, O)
+            log(f"gradient check: {diff_out['gradient_check']}")
 
     out = {
         "metric": "contact pairs/sec (broad+narrow, f32) on 1 MI355X; batched vs plain speedup",
@@ -545,8 +796,10 @@ def main():
                    "bodies": args.bodies, "pairs": int(P), "contacts": int(Cn), "tuning": tuned},
         "contacts_per_s": round(Cn / (ms * 1e-3), 1),
         "stage_ms": {s: round(float(v), 4) for s, v in zip(STAGES, stage_ms)},
+        "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
         "roofline": roof,
         "host_buffers": pcie,
+        "dropin": dropin,
         "captured": graph,
         "manifold": manifold,
         "mesh": mesh,

@@ -49,10 +49,10 @@ class _Context:
         self.h = h
 
     _KNOBS = {"hull_lanes": _lib.TUNE_HULL_LANES,
-              "pair_slots": _lib.TUNE_PAIR_SLOTS, "epa_grid": _lib.TUNE_EPA_GRID, "np_grid": _lib.TUNE_NP_GRID}
+              "traverse_block": _lib.TUNE_TRAVERSE_BLOCK, "epa_grid": _lib.TUNE_EPA_GRID, "np_grid": _lib.TUNE_NP_GRID}
 
     def set_tuning(self, **knobs):
-        """Launch-shape knobs (hull_lanes, pair_slots, epa_grid,
+        """Launch-shape knobs (hull_lanes, traverse_block, epa_grid,
         np_grid); they never change results."""
         for k, v in knobs.items():
             if k not in self._KNOBS:
@@ -582,7 +582,11 @@ class CollisionWorld:
 
     def timings(self):
         """Device milliseconds of the last pass per stage (see STAGES)."""
-        out = np.zeros(8, np.float32)
+        return self.timings_all()[:8]
+
+    def timings_all(self):
+        """The eight stage times, then the two-pair EPA kernel alone (part of "epa")."""
+        out = np.zeros(9, np.float32)
         _check(self.ctx.h, self.L.bch_last_timings(self.ctx.h, out.ctypes.data))
         return out
 

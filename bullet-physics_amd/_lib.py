@@ -32,7 +32,7 @@ EXPORTS = [
 ]
 
 # launch-shape knobs (bch_set_tuning)
-TUNE_HULL_LANES, TUNE_PAIR_SLOTS, TUNE_EPA_GRID, TUNE_NP_GRID = 1, 2, 3, 4
+TUNE_HULL_LANES, TUNE_TRAVERSE_BLOCK, TUNE_EPA_GRID, TUNE_NP_GRID = 1, 2, 3, 4
 
 
 class BchBodies(C.Structure):

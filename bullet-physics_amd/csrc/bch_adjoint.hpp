@@ -29,7 +29,7 @@ namespace bch {
 enum : uint32_t {
   SD_WALK = 0,             // Band arm of the walk (separated, or inside the margins)
   SD_EPA = 1,              // EPA converged
-  SD_CAPPED = 2,           // walk stopped at its iteration cap (last iterate)
+  SD_CAPPED = 2,           // walk stopped at its iteration cap (closest iterate)
   SD_TOUCH = 3,            // cores touch and no enclosing tetrahedron: sd = -(margins), no normal
   SD_EPA_UNCONVERGED = 4,  // EPA ended without converging (reported face)
   SD_FLAG_EDGE = 8,        // EPA face distance came from an edge or vertex, not the plane

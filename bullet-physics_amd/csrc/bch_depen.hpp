@@ -18,7 +18,7 @@ struct DepenRecord {
   float max_pen;      // largest max(0, -sd)
   uint32_t n_pairs;   // broadphase pairs found
   uint32_t n_pen;     // penetrating pairs
-  uint32_t flags;     // Counters::errors bits; 32 = per-leaf pair slots overflowed
+  uint32_t flags;     // Counters::errors bits (8 = pair capacity exceeded)
   uint32_t pad[3];
 };
 
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kDepenBlock) void k_depen_terms(uint32_t P, const u
 // One block: the step's loss and record; hist[*step] when it fits, then
 // ++*step.
 __global__ __launch_bounds__(kDepenBlock) void k_depen_record(const float2* __restrict__ partials, uint32_t nb,
-                                                              const Counters* __restrict__ cnt, uint32_t K,
+                                                              const Counters* __restrict__ cnt,
                                                               DepenRecord* __restrict__ hist, uint32_t hist_cap,
                                                               uint32_t* __restrict__ step) {
   __shared__ float ss[kDepenBlock], sm[kDepenBlock];
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kDepenBlock) void k_depen_record(const float2* __re
       r.max_pen = sm[0];
       r.n_pairs = cnt->n_pairs_total;
       r.n_pen = cnt->n_pen;
-      r.flags = cnt->errors | (cnt->max_leaf > K ? 32u : 0u);
+      r.flags = cnt->errors;
       r.pad[0] = r.pad[1] = r.pad[2] = 0u;
       hist[k] = r;
     }

@@ -559,8 +559,16 @@ __device__ __forceinline__ void epa_dual_run(const uint32_t* __restrict__ n_queu
   }
 }
 
+// Waves per SIMD the two-pair EPA is compiled for (its register budget).
+// At 5 (96 VGPRs, 88 B spilled per lane) the 1M-body mixed scene ran the
+// kernel 2 % faster (1.547 against 1.582 ms), but the spills raised its HBM
+// traffic from 0.25 to 0.91 GB per launch, so 4 (no spill) is the default.
+#ifndef BCH_EPA_WAVES
+#define BCH_EPA_WAVES 4
+#endif
+
 // The world pass's EPA queue (pairs of the pass, 8 contact slots per pair).
-__global__ __launch_bounds__(64) void k_epa_dual(const uint32_t* __restrict__ n_queue,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BCH_EPA_WAVES))) void k_epa_dual(const uint32_t* __restrict__ n_queue,
                                                  const float4* __restrict__ epa_q, const uint32_t* __restrict__ pairs,
                                                  const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
                                                  uint32_t* __restrict__ ccount, float4* __restrict__ cslots,

@@ -29,11 +29,12 @@ struct Counters {
   uint32_t n_reg;       // bodies in the tree
   uint32_t n_big;       // bodies with unbounded AABBs
   uint32_t bounds[6];   // orderable-encoded min xyz, max xyz of regular centres
-  uint32_t max_leaf;    // largest per-leaf pair count seen
+  uint32_t wave_over;   // largest tree-walk wavefront pair count past its region (0: none)
   uint32_t n_big_pairs;
   uint32_t n_epa;
-  uint32_t errors;      // bit 0 clip overflow, bit 1 epa store overflow, bit 2 big-pair overflow,
-                        // bit 3 pair capacity overflow, bit 4 contact capacity overflow
+  uint32_t errors;      // bit 0 clip overflow, bit 1 epa store overflow, bit 3 pair capacity overflow,
+                        // bit 4 contact capacity overflow, bit 5 mesh BVH stack, bit 6 bad static-batch
+                        // input, bit 7 a tree-walk wavefront's pair region overflowed
   uint32_t cls_start[8];  // pair range of each shape class after the class sort
   uint32_t cls_end[8];
   // capacity-mode pass (counts stay on the device)
@@ -64,7 +65,7 @@ __global__ void k_reset(Counters* c) {
     c->n_big = 0;
     c->bounds[0] = c->bounds[1] = c->bounds[2] = 0xffffffffu;
     c->bounds[3] = c->bounds[4] = c->bounds[5] = 0u;
-    c->max_leaf = 0;
+    c->wave_over = 0;
     c->n_big_pairs = 0;
     c->n_epa = 0;
     c->errors = 0;
@@ -489,55 +490,41 @@ __global__ void k_tree_up(float4* __restrict__ nodes, uint32_t first, int m) {
   }
 }
 
-// Leaf s walks the subtrees right of s in depth-first order;
-// every overlapping leaf is a pair found exactly once.  Keys i << 32 | j go to
-// the leaf's K slots, stored slot-major (slot k of leaf s at k * n_all + s,
-// so neighbouring leaves write and k_compact_slots reads neighbouring
-// words); counts beyond K are reported for a rerun with more slots.  With `wend` (batches: leaves sorted world-major), the walk stops at
-// the first node whose first leaf lies past the query's world.
-// Counts of slots n_reg .. n_all - 1 are cleared, so a scan over all n_all
-// slots needs no host-side n_reg.
-// W leaves per lane (W t .. W t + W - 1), stepped together: each step
-// issues all node loads before any is used, so a lane keeps W dependent
-// chains in flight.  Blocks are remapped with xcd_block so each XCD walks a
-// contiguous run of leaves; with that remap W = 1 measured best (0.56 ms on
-// 1M leaves against 0.73 ms for W = 2, which the remap did not speed up).
+// Leaf s walks the subtrees right of s in depth-first order; every
+// overlapping leaf is a pair found exactly once, as the key i << 32 | j
+// (i < j).  With `wend` (batches: leaves sorted world-major), the walk stops
+// at the first node whose first leaf lies past the query's world.
 struct LeafWalk {
   float4 qlo, qhi;
-  uint32_t body, wend, c, k, s, stride;
-  uint64_t* my;
+  uint32_t body, wend, k;
 };
 __device__ __forceinline__ void walk_begin(LeafWalk& w, uint32_t s, const float4* __restrict__ nodes, uint32_t N,
-                                           const uint32_t* __restrict__ wend_of_leaf, uint32_t K,
-                                           uint64_t* __restrict__ slots, uint32_t stride, bool valid) {
-  w.s = s;
-  w.c = 0;
+                                           const uint32_t* __restrict__ wend_of_leaf, bool valid) {
   w.k = 0;
   if (!valid) return;
   w.qlo = nodes[2 * (uint64_t)(N + s)];
   w.qhi = nodes[2 * (uint64_t)(N + s) + 1];
   w.body = f2u(w.qlo.w);
   w.wend = wend_of_leaf ? wend_of_leaf[s] : 0xffffffffu;
-  w.my = slots + s;
-  w.stride = stride;
   w.k = tree_rope(N + s);
 }
-__device__ __forceinline__ void walk_step(LeafWalk& w, float4 lo, float4 hi, uint32_t N, uint32_t K) {
+// One node of the walk; returns whether it completed a pair (*key).
+__device__ __forceinline__ bool walk_step(LeafWalk& w, float4 lo, float4 hi, uint32_t N, uint64_t* key) {
   bool ov = w.qlo.x <= hi.x && w.qlo.y <= hi.y && w.qlo.z <= hi.z && lo.x <= w.qhi.x && lo.y <= w.qhi.y &&
             lo.z <= w.qhi.z;
-  if (ov) {
-    if (w.k >= N) {
+  bool pair = false;
+  if (ov && w.k < N) {
+    w.k <<= 1;
+  } else {
+    if (ov) {
       uint32_t other = f2u(lo.w);
       uint32_t a = w.body < other ? w.body : other, b = w.body < other ? other : w.body;
-      if (w.c < K) w.my[(uint64_t)w.c * w.stride] = ((uint64_t)a << 32) | b;
-      ++w.c;
-      w.k = tree_rope(w.k);
-    } else {
-      w.k <<= 1;
+      *key = ((uint64_t)a << 32) | b;
+      pair = true;
     }
-  } else {
     w.k = tree_rope(w.k);
   }
+  return pair;
 }
 __device__ __forceinline__ void walk_world_stop(LeafWalk& w, const uint32_t* __restrict__ wend_of_leaf, uint32_t N,
                                                 int L) {
@@ -556,90 +543,107 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
   return x * q + (x < r ? x : r) + k;
 }
 
-template <int W>
-__global__ void k_traverse(const Counters* __restrict__ cnt, const float4* __restrict__ nodes, uint32_t N, int L,
-                             const uint32_t* __restrict__ wend_of_leaf, uint32_t K, uint64_t* __restrict__ slots,
-                             uint32_t* __restrict__ counts, uint32_t n_all, Counters* cnt_rw) {
+// Per-wavefront pair compaction.  Each step's pairs are ranked by a ballot
+// and appended to the wavefront's LDS list; every 64 of them go out as one
+// coalesced 512-B store into the wavefront's own region of R slots (no
+// atomics).  Pairs past R are counted but not written.
+struct PairStage {
+  uint64_t* st;       // this wavefront's 128 LDS slots
+  uint64_t* region;   // its R output slots
+  uint32_t fill, lane, written, R;
+};
+__device__ __forceinline__ void stage_flush(PairStage& ps, uint32_t m) {
+  if (ps.lane < m && ps.written + ps.lane < ps.R) ps.region[ps.written + ps.lane] = ps.st[ps.lane];
+  ps.written += m;
+}
+__device__ __forceinline__ void stage_push(PairStage& ps, bool emit, uint64_t key) {
+  uint64_t m = __ballot(emit);
+  if (!m) return;
+  uint32_t r = __popcll(m & ((1ull << ps.lane) - 1ull));
+  if (emit) ps.st[ps.fill + r] = key;
+  ps.fill += (uint32_t)__popcll(m);
+  if (ps.fill >= 64) {
+    stage_flush(ps, 64);
+    uint32_t rest = ps.fill - 64;
+    uint64_t v = ps.lane < rest ? ps.st[64 + ps.lane] : 0ull;
+    if (ps.lane < rest) ps.st[ps.lane] = v;
+    ps.fill = rest;
+  }
+}
+
+// The tree walk of every leaf (one leaf per lane, lanes of a wavefront kept
+// together until the last walk ends).  Wavefront w (logical, after the XCD
+// remap: leaves 64 w .. 64 w + 63) writes its pairs to regions[w R ..] and
+// its count to wave_counts[w]; a count past R raises wave_over to it.
+// Blocks are remapped with xcd_block so each XCD walks a contiguous run of
+// leaves.
+template <int B>
+__global__ __launch_bounds__(B) void k_traverse(const Counters* __restrict__ cnt, const float4* __restrict__ nodes,
+                                                uint32_t N, int L, const uint32_t* __restrict__ wend_of_leaf,
+                                                uint64_t* __restrict__ regions, uint32_t R,
+                                                uint32_t* __restrict__ wave_counts, Counters* cnt_rw) {
+  __shared__ uint64_t stage[B / 64][128];
   const uint32_t n = cnt->n_reg;
-  const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (W * t >= n_all) return;
-  LeafWalk w[W];
-#pragma unroll
-  for (int i = 0; i < W; ++i) {
-    walk_begin(w[i], W * t + i, nodes, N, wend_of_leaf, K, slots, n_all, W * t + i < n);
-    walk_world_stop(w[i], wend_of_leaf, N, L);
+  const uint32_t t = xcd_block(blockIdx.x, gridDim.x) * B + threadIdx.x;
+  const uint32_t wave = t >> 6;
+  PairStage ps{stage[threadIdx.x >> 6], regions + (uint64_t)wave * R, 0u, threadIdx.x & 63u, 0u, R};
+  LeafWalk w;
+  walk_begin(w, t, nodes, N, wend_of_leaf, t < n);
+  walk_world_stop(w, wend_of_leaf, N, L);
+  while (__any(w.k != 0)) {
+    bool emit = false;
+    uint64_t key = 0;
+    if (w.k) {
+      float4 lo = nodes[2 * (uint64_t)w.k], hi = nodes[2 * (uint64_t)w.k + 1];
+      emit = walk_step(w, lo, hi, N, &key);
+      walk_world_stop(w, wend_of_leaf, N, L);
+    }
+    stage_push(ps, emit, key);
   }
-  for (;;) {
-    uint32_t any = 0;
-#pragma unroll
-    for (int i = 0; i < W; ++i) any |= w[i].k;
-    if (!any) break;
-    float4 lo[W], hi[W];
-#pragma unroll
-    for (int i = 0; i < W; ++i) {
-      lo[i] = hi[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (w[i].k) {
-        lo[i] = nodes[2 * (uint64_t)w[i].k];
-        hi[i] = nodes[2 * (uint64_t)w[i].k + 1];
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < W; ++i) {
-      if (w[i].k) {
-        walk_step(w[i], lo[i], hi[i], N, K);
-        walk_world_stop(w[i], wend_of_leaf, N, L);
-      }
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < W; ++i) {
-    if (W * t + i < n_all) {
-      counts[W * t + i] = w[i].c < K ? w[i].c : K;
-      if (w[i].c > K) atomicMax(&cnt_rw->max_leaf, w[i].c);
-    }
+  if (ps.fill) stage_flush(ps, ps.fill);
+  if (ps.lane == 0) {
+    wave_counts[wave] = ps.written;
+    if (ps.written > R) atomicMax(&cnt_rw->wave_over, ps.written);
   }
 }
 
-// Leaf slots (slot-major, stride = the traversal's n_all) -> dense keys at
-// the scanned offsets; keys past `cap` are dropped (the capacity-mode pass
-// reports them as an overflow).
-__global__ void k_compact_slots(const Counters* __restrict__ cnt, uint32_t stride, const uint64_t* __restrict__ slots,
-                                const uint32_t* __restrict__ counts, const uint32_t* __restrict__ offsets,
-                                uint64_t cap, uint64_t* __restrict__ keys) {
-  int n = (int)cnt->n_reg;
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  uint32_t c = counts[s], o = offsets[s];
-  for (uint32_t k = 0; k < c; ++k)
-    if ((uint64_t)o + k < cap) keys[o + k] = slots[(uint64_t)k * stride + s];
+// The wavefronts' regions -> one dense list at their scanned offsets (one
+// block per wavefront), up to `cap`.
+__global__ __launch_bounds__(64) void k_concat_regions(const uint64_t* __restrict__ regions, uint32_t R,
+                                                       const uint32_t* __restrict__ wave_counts,
+                                                       const uint32_t* __restrict__ wave_offs, uint64_t cap,
+                                                       uint64_t* __restrict__ out) {
+  uint32_t w = blockIdx.x;
+  uint32_t c = wave_counts[w];
+  c = c < R ? c : R;
+  uint64_t o = wave_offs[w];
+  const uint64_t* src = regions + (uint64_t)w * R;
+  for (uint32_t k = threadIdx.x; k < c; k += 64)
+    if (o + k < cap) out[o + k] = src[k];
 }
 
-// Capacity mode: big-body pairs follow the tree pairs (whose count the scan
-// left at tree_total); sets the held and found pair counts.
-__global__ void k_append_big(const uint32_t* __restrict__ tree_total, const uint64_t* __restrict__ big_pairs,
-                             uint32_t big_cap, uint32_t cap, uint64_t* __restrict__ keys, Counters* cnt) {
-  uint32_t preg = *tree_total;
-  uint32_t found = cnt->n_big_pairs;
-  uint32_t nb = found < big_cap ? found : big_cap;
-  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nb; t += gridDim.x * blockDim.x) {
-    uint64_t idx = (uint64_t)preg + t;
-    if (idx < cap) keys[idx] = big_pairs[t];
-  }
+// Capacity mode: the held and found pair counts (tree pairs at
+// *tree_total, then the big bodies' pairs) and the overflow flags: bit 3
+// pair capacity, bit 7 a wavefront region.
+__global__ void k_pair_counts(Counters* cnt, const uint32_t* __restrict__ tree_total, uint64_t cap, uint32_t R) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    uint64_t held = (uint64_t)preg + nb;
-    uint64_t total = (uint64_t)preg + found;
-    cnt->n_pairs = (uint32_t)(held < cap ? held : cap);
+    uint64_t total = (uint64_t)*tree_total + cnt->n_big_pairs;
+    cnt->n_pairs = (uint32_t)(total < cap ? total : cap);
     cnt->n_pairs_total = (uint32_t)(total < 0xffffffffull ? total : 0xffffffffull);
     if (total > cap) atomicOr(&cnt->errors, 8u);
+    if (cnt->wave_over > R) atomicOr(&cnt->errors, 128u);
   }
 }
 
-// Bodies with unbounded AABBs: tested against every other live body of their
-// world (big-big pairs once, from the lower id).  Wave-aggregated append.
+// Bodies with unbounded boxes (planes) against every body of their world,
+// brute force; pairs appended after the tree walk's *tree_total pairs (the
+// count, Counters::n_big_pairs, still grows past `cap`).
 __global__ void k_big(const Counters* __restrict__ cnt, const uint32_t* __restrict__ big_ids,
                       const float4* __restrict__ aabb, const uint8_t* __restrict__ flags,
                       const BodyRec* __restrict__ bodies, const uint32_t* __restrict__ world_offsets, uint32_t n,
-                      uint64_t* __restrict__ out, uint32_t cap, Counters* cnt_rw) {
+                      uint64_t* __restrict__ out, uint64_t cap, const uint32_t* __restrict__ tree_total,
+                      Counters* cnt_rw) {
+  const uint64_t t0 = *tree_total;
   uint32_t nbig = cnt->n_big;
   for (uint32_t bi = blockIdx.y; bi < nbig; bi += gridDim.y) {
     uint32_t b = big_ids[bi];
@@ -671,10 +675,9 @@ __global__ void k_big(const Counters* __restrict__ cnt, const uint32_t* __restri
         base_idx = __shfl(base_idx, leader);
         if (emit) {
           uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
-          uint32_t idx = base_idx + rank;
+          uint64_t idx = t0 + base_idx + rank;
           uint32_t lo = b < j ? b : j, hi = b < j ? j : b;
           if (idx < cap) out[idx] = ((uint64_t)lo << 32) | hi;
-          else atomicOr(&cnt_rw->errors, 4u);
         }
       }
     }
@@ -1118,8 +1121,11 @@ __global__ void k_world_end(const Counters* __restrict__ cnt, const uint64_t* __
   wend[s] = (uint32_t)lo;
 }
 
-__global__ void k_reset_big(Counters* c) {
-  if (threadIdx.x == 0) c->n_big_pairs = 0;
+__global__ void k_reset_tree(Counters* c) {
+  if (threadIdx.x == 0) {
+    c->wave_over = 0;
+    c->n_big_pairs = 0;
+  }
 }
 __global__ void k_reset_epa(Counters* c) {
   if (threadIdx.x == 0) {

@@ -92,7 +92,6 @@ struct PinnedBuf {
 };
 
 constexpr int kBlock = 256;
-constexpr int kTraverseW = 1;  // tree leaves per lane in k_traverse
 constexpr uint32_t kEpaDeferGrid = 16384;  // one-pair EPA blocks for the deferred entries (~4% of the queue)
 inline uint32_t blocks(uint64_t n, int b = kBlock) { return (uint32_t)((n + b - 1) / b); }
 
@@ -155,8 +154,8 @@ struct bch_context {
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::string err;
-  hipEvent_t ev[9] = {};
-  float timings[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  hipEvent_t ev[10] = {};  // stage boundaries 0-8; 9 ends the two-pair EPA kernel
+  float timings[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
   // world-resident bodies
   uint32_t n = 0, n_worlds = 1, max_world = 0;
@@ -167,12 +166,16 @@ struct bch_context {
   // pass buffers
   DevBuf aabb, flags, big_ids, counters, partials, keys, keys_alt, vals, vals_alt, nodes, wend, hull_ids;
   uint32_t n_hulls = 0;
-  DevBuf leaf_counts, leaf_offsets, slots, pair_keys, pair_keys_alt, big_pairs, pairs, cls, cls_alt, perm,
+  DevBuf walk_regions, wave_counts, wave_offs, pair_keys, pair_keys_alt, pairs, cls, cls_alt, perm,
       perm_alt, ccount, coffsets, cslots, contacts, epa_q, epa_q2, scan_part, rs_hist, rs_offs;
   int hull_lanes = 16;      // lanes per hull GJK pair (4, 8 or 16)
   uint32_t epa_grid = 32768; // EPA blocks (one wavefront each)
   uint32_t np_grid = 2048;  // narrowphase blocks per shape class
-  uint32_t K = 16;
+  uint32_t trav_block = 256;  // threads per tree-walk block (64, 128, 256, 512)
+  uint64_t tree_cap = 0;      // pair buffer of the eager pass, sized by earlier passes
+  uint32_t walk_R = 1024;     // pair slots of one tree-walk wavefront's region
+  uint32_t walk_nw = 0;       // wavefronts of the last walk
+  bool walk_grow = false;     // a replayed walk overflowed a region: the next capture doubles walk_R
   uint64_t P = 0, C = 0;
   Counters host_cnt{};
   uint32_t n_planes = 0;
@@ -227,6 +230,7 @@ struct bch_context {
   std::set<uint32_t> proxy_free;
   std::vector<uint32_t> dirty_list;
   uint32_t arena_n = 0;  // slots the device arena holds
+  bool arena_whole = false;  // so many slots changed that the next sync uploads the whole arena
   DevBuf arena, arena_upd, bp_pairs;
   PinnedBuf arena_stage;
   bool bp_pairs_valid = false;  // the last compute_pairs result still holds (no change since)
@@ -275,6 +279,13 @@ int fail(bch_context* c, int code, const std::string& msg) {
     }                                                                                        \
     if (!(buf).ensure(need_)) return fail(ctx, BCH_HIP_ERROR, "device allocation failed");  \
   } while (0)
+
+// Stage times of the last eager pass from its boundary events; the ninth is
+// the two-pair EPA kernel alone (stage 6 minus the one-pair launch after it).
+void stage_times(bch_context* c) {
+  for (int k = 0; k < 8; ++k) (void)hipEventElapsedTime(&c->timings[k], c->ev[k], c->ev[k + 1]);
+  (void)hipEventElapsedTime(&c->timings[8], c->ev[6], c->ev[9]);
+}
 
 // Stage boundary event (not while capturing: the graph is timed as a whole).
 void mark(bch_context* c, int k) {
@@ -352,37 +363,64 @@ int pair_key_shifts(uint32_t n, int* ps, int* pack) {
   return digit_shifts(0, 2 * bn, ps);
 }
 
-// Capacity-mode tail of the broadphase: one traversal with the current slot
-// count, a scan over every leaf slot, big-body pairs appended on the device,
-// and the sort over the device-held count.  Nothing is read back.
+// The tree walk of every leaf into its wavefront's region (stage event 3
+// follows it), the regions scanned and concatenated into c->pair_keys, then
+// the big bodies' brute force appended after them, all up to `cap` keys.
+// The tree total lands at c->wave_offs[nw], the big count in
+// Counters::n_big_pairs, and a region overflow in Counters::wave_over.
+int walk_pairs(bch_context* c, uint32_t n, const uint32_t* d_world_offsets, const float4* nodes, uint32_t N, int L,
+               const uint32_t* wend, uint64_t cap) {
+  hipStream_t s = c->stream;
+  Counters* cnt = c->counters.as<Counters>();
+  const uint32_t B = c->trav_block;
+  const uint32_t nb = blocks(std::max<uint32_t>(n, 1), B);
+  const uint32_t nw = nb * (B / 64);
+  ENSURE(c, c->walk_regions, (size_t)nw * c->walk_R * sizeof(uint64_t));
+  ENSURE(c, c->wave_counts, ((size_t)nw + 1) * sizeof(uint32_t));
+  ENSURE(c, c->wave_offs, ((size_t)nw + 1) * sizeof(uint32_t));
+  hipLaunchKernelGGL(k_reset_tree, dim3(1), dim3(64), 0, s, cnt);
+  uint64_t* reg = c->walk_regions.as<uint64_t>();
+  uint32_t* wc = c->wave_counts.as<uint32_t>();
+  const uint32_t R = c->walk_R;
+#define BCH_WALK(BS)                                                                                          \
+  hipLaunchKernelGGL(k_traverse<BS>, dim3(nb), dim3(BS), 0, s, cnt, nodes, N, L, wend, reg, R, wc, cnt)
+  switch (B) {
+    case 64: BCH_WALK(64); break;
+    case 128: BCH_WALK(128); break;
+    case 512: BCH_WALK(512); break;
+    default: BCH_WALK(256);
+  }
+#undef BCH_WALK
+  mark(c, 3);
+  int rc = dev_scan(c, wc, c->wave_offs.as<uint32_t>(), nw, c->scan_part, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_concat_regions, dim3(nw), dim3(64), 0, s, reg, R, wc, c->wave_offs.as<uint32_t>(), cap,
+                     c->pair_keys.as<uint64_t>());
+  // k_big reads the big-body count on the device; an empty list exits at once
+  uint32_t span = c->batched ? c->max_world : n;
+  uint32_t gx = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(span), 1024u));
+  uint32_t gy = std::max<uint32_t>(1u, std::min<uint32_t>(c->n_planes, 65535u));
+  hipLaunchKernelGGL(k_big, dim3(gx, gy), dim3(kBlock), 0, s, cnt, c->big_ids.as<uint32_t>(), c->aabb.as<float4>(),
+                     c->flags.as<uint8_t>(), c->bodies.as<BodyRec>(), d_world_offsets, n, c->pair_keys.as<uint64_t>(),
+                     cap, c->wave_offs.as<uint32_t>() + nw, cnt);
+  c->walk_nw = nw;
+  return BCH_OK;
+}
+
+// Capacity-mode tail of the broadphase: the walk into the fixed pair buffer,
+// the held / found counts on the device, and the sort over the held count.
+// Nothing is read back.
 int broadphase_capacity(bch_context* c, uint32_t n, const uint32_t* d_world_offsets, const float4* nodes, uint32_t N,
-                        int L, const uint32_t* wend, int wbits) {
-  (void)wbits;
+                        int L, const uint32_t* wend) {
   hipStream_t s = c->stream;
   Counters* cnt = c->counters.as<Counters>();
   uint64_t cap = c->P_cap;
-  ENSURE(c, c->slots, (size_t)n * c->K * sizeof(uint64_t));
-  hipLaunchKernelGGL(k_traverse<kTraverseW>, dim3(blocks((n + kTraverseW - 1) / kTraverseW)), dim3(kBlock), 0, s, cnt,
-                     nodes, N, L, wend, c->K, c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), n, cnt);
-  mark(c, 3);
-  int rc = dev_scan(c, c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), n, c->scan_part, s);
-  if (rc) return rc;
-  uint32_t big_cap = (uint32_t)std::min<uint64_t>(cap, 0xffffffffull);
-  ENSURE(c, c->big_pairs, (size_t)big_cap * sizeof(uint64_t));
-  hipLaunchKernelGGL(k_reset_big, dim3(1), dim3(64), 0, s, cnt);
-  uint32_t span = c->batched ? c->max_world : n;
-  uint32_t gx = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(span), 1024u));
-  uint32_t gy = std::max<uint32_t>(1u, std::min<uint32_t>(std::max(c->n_planes, c->host_cnt.n_big), 65535u));
-  hipLaunchKernelGGL(k_big, dim3(gx, gy), dim3(kBlock), 0, s, cnt, c->big_ids.as<uint32_t>(), c->aabb.as<float4>(),
-                     c->flags.as<uint8_t>(), c->bodies.as<BodyRec>(), d_world_offsets, n, c->big_pairs.as<uint64_t>(),
-                     big_cap, cnt);
   ENSURE(c, c->pair_keys, cap * sizeof(uint64_t));
   ENSURE(c, c->pair_keys_alt, cap * sizeof(uint64_t));
-  hipLaunchKernelGGL(k_compact_slots, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, n, c->slots.as<uint64_t>(),
-                     c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), cap, c->pair_keys.as<uint64_t>());
-  hipLaunchKernelGGL(k_append_big, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(blocks(big_cap), 1024u))),
-                     dim3(kBlock), 0, s, c->leaf_offsets.as<uint32_t>() + n, c->big_pairs.as<uint64_t>(), big_cap,
-                     (uint32_t)cap, c->pair_keys.as<uint64_t>(), cnt);
+  int rc = walk_pairs(c, n, d_world_offsets, nodes, N, L, wend, cap);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pair_counts, dim3(1), dim3(64), 0, s, cnt, c->wave_offs.as<uint32_t>() + c->walk_nw, cap,
+                     c->walk_R);
   c->P = cap;
   int ps[16];
   int pack = 0;
@@ -443,54 +481,36 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
     hipLaunchKernelGGL(k_world_end, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, skeys, wend);
   }
   mark(c, 2);
-  if (c->cap_mode) return broadphase_capacity(c, n, d_world_offsets, nodes, N, L, wend, wbits);
-  // The prefix offsets of the per-leaf counts are scanned before the host
-  // looks at the slot check (leaves past n_reg count 0), so one read-back
-  // serves both; a leaf with more pairs than slots reruns the walk.
-  uint32_t preg = 0;
+  if (c->cap_mode) return broadphase_capacity(c, n, d_world_offsets, nodes, N, L, wend);
+  // The pair buffer and the wavefronts' regions are sized by earlier passes
+  // (5 pairs per body, 1024 per wavefront at first); a walk whose
+  // wavefront overflowed its region runs again with a region that holds
+  // it, and a list longer than the buffer is concatenated again.
+  uint64_t cap = std::max<uint64_t>(c->tree_cap, std::max<uint64_t>(5ull * n, 1024));
+  uint32_t tree_total = 0;
   for (int attempt = 0; attempt < 3; ++attempt) {
-    ENSURE(c, c->slots, (size_t)n * c->K * sizeof(uint64_t));
-    hipLaunchKernelGGL(k_traverse<kTraverseW>, dim3(blocks((n + kTraverseW - 1) / kTraverseW)), dim3(kBlock), 0, s,
-                       cnt, nodes, N, L, wend, c->K, c->slots.as<uint64_t>(), c->leaf_counts.as<uint32_t>(), n, cnt);
-    mark(c, 3);
-    rc = dev_scan(c, c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), n, c->scan_part, s);
-    if (rc) return rc;
+    ENSURE(c, c->pair_keys, cap * sizeof(uint64_t));
+    ENSURE(c, c->pair_keys_alt, cap * sizeof(uint64_t));
+    int rc2 = walk_pairs(c, n, d_world_offsets, nodes, N, L, wend, cap);
+    if (rc2) return rc2;
     HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
-    HIP_TRY(c, hipMemcpyAsync(&preg, c->leaf_offsets.as<uint32_t>() + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(c, hipMemcpyAsync(&tree_total, c->wave_offs.as<uint32_t>() + c->walk_nw, sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, s));
     HIP_TRY(c, hipStreamSynchronize(s));
-    if (c->host_cnt.max_leaf <= c->K) break;
-    c->K = std::max<uint32_t>(c->K * 2, (c->host_cnt.max_leaf + 7) & ~7u);
-  }
-  if (c->host_cnt.max_leaf > c->K) return fail(c, BCH_HIP_ERROR, "per-leaf pair slots did not converge");
-  // big bodies (planes): brute force against their world
-  uint32_t nbig = c->host_cnt.n_big;
-  uint64_t big_cap = 0;
-  if (nbig) {
-    big_cap = std::max<uint64_t>((uint64_t)nbig * (c->batched ? c->max_world : n), 1024);
-    for (int attempt = 0; attempt < 2; ++attempt) {
-      ENSURE(c, c->big_pairs, big_cap * sizeof(uint64_t));
-      hipLaunchKernelGGL(k_reset_big, dim3(1), dim3(64), 0, s, cnt);
-      uint32_t gy = std::min<uint32_t>(nbig, 65535u);
-      uint32_t span = c->batched ? c->max_world : n;
-      uint32_t gx = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(span), 1024u));
-      hipLaunchKernelGGL(k_big, dim3(gx, gy), dim3(kBlock), 0, s, cnt, c->big_ids.as<uint32_t>(), c->aabb.as<float4>(),
-                         c->flags.as<uint8_t>(), c->bodies.as<BodyRec>(), d_world_offsets, n,
-                         c->big_pairs.as<uint64_t>(), (uint32_t)std::min<uint64_t>(big_cap, 0xffffffffu), cnt);
-      HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
-      HIP_TRY(c, hipStreamSynchronize(s));
-      if (c->host_cnt.n_big_pairs <= big_cap) break;
-      big_cap = c->host_cnt.n_big_pairs;
+    uint64_t total = (uint64_t)tree_total + c->host_cnt.n_big_pairs;
+    bool region_short = c->host_cnt.wave_over > c->walk_R;
+    if (!region_short && total <= cap) break;
+    if (region_short) {
+      uint32_t r = c->walk_R;
+      while (r < c->host_cnt.wave_over) r *= 2;
+      c->walk_R = r;
     }
+    if (total > cap) cap = total + total / 4;
   }
-  uint64_t pbig = c->host_cnt.n_big_pairs;
-  c->P = (uint64_t)preg + pbig;
-  ENSURE(c, c->pair_keys, std::max<uint64_t>(c->P, 1) * sizeof(uint64_t));
-  ENSURE(c, c->pair_keys_alt, std::max<uint64_t>(c->P, 1) * sizeof(uint64_t));
-  hipLaunchKernelGGL(k_compact_slots, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, n, c->slots.as<uint64_t>(),
-                     c->leaf_counts.as<uint32_t>(), c->leaf_offsets.as<uint32_t>(), c->P, c->pair_keys.as<uint64_t>());
-  if (pbig)
-    HIP_TRY(c, hipMemcpyAsync(c->pair_keys.as<uint64_t>() + preg, c->big_pairs.p, pbig * sizeof(uint64_t),
-                              hipMemcpyDeviceToDevice, s));
+  uint64_t total = (uint64_t)tree_total + c->host_cnt.n_big_pairs;
+  if (c->host_cnt.wave_over > c->walk_R || total > cap) return fail(c, BCH_HIP_ERROR, "pair buffers did not converge");
+  c->tree_cap = cap;
+  c->P = total;
   if (c->P > 1) {
     int ps[16];
     int pack = 0;
@@ -518,8 +538,6 @@ int ensure_pass_buffers(bch_context* c, uint32_t n) {
   while (N < n) N <<= 1;
   ENSURE(c, c->nodes, (size_t)N * 4 * sizeof(float4));
   ENSURE(c, c->wend, (size_t)(n + 1) * sizeof(uint32_t));
-  ENSURE(c, c->leaf_counts, (size_t)(n + 1) * sizeof(uint32_t));
-  ENSURE(c, c->leaf_offsets, (size_t)(n + 1) * sizeof(uint32_t));
   return BCH_OK;
 }
 
@@ -615,12 +633,14 @@ int narrow_finish(bch_context* c, const float* d_seeds, bool dense = true) {
                        c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                        c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(),
                        c->epa_q2.as<float4>(), cnt);
+    mark(c, 9);
     hipLaunchKernelGGL(k_epa_wave, dim3(kEpaDeferGrid), dim3(64), 0, s, &cnt->n_epa_defer, c->epa_q2.as<float4>(),
                        c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                        c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(), cnt);
     mark(c, 7);
   } else {
     for (int k = 4; k <= 7; ++k) mark(c, k);
+    mark(c, 9);
   }
   if (!dense && !c->cap_mode) {
     HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
@@ -954,7 +974,7 @@ int run_depen_step(bch_context* c, float aabb_margin, float lr, float* d_pos, fl
   hipLaunchKernelGGL(k_depen_terms, dim3(nb), dim3(kDepenBlock), 0, s, (uint32_t)cap, P_dev, c->sd_out.as<float>(),
                      c->pairs.as<uint32_t>(), c->pen_pairs.as<uint32_t>(), c->pen_g.as<float>(), &cnt->n_pen,
                      c->pen_part.as<float2>());
-  hipLaunchKernelGGL(k_depen_record, dim3(1), dim3(kDepenBlock), 0, s, c->pen_part.as<float2>(), nb, cnt, c->K,
+  hipLaunchKernelGGL(k_depen_record, dim3(1), dim3(kDepenBlock), 0, s, c->pen_part.as<float2>(), nb, cnt,
                      c->dhist.as<DepenRecord>(), c->dhist_cap, c->dstep.as<uint32_t>());
   if (!update) return BCH_OK;
   const uint32_t* pen_dev = &cnt->n_pen;
@@ -978,6 +998,29 @@ int run_depen_step(bch_context* c, float aabb_margin, float lr, float* d_pos, fl
 // Counts of the last capacity-mode pass (waits for the stream): held (what
 // the buffers hold, also left in c->P and c->C for the result getters) and
 // found.
+// The uncaptured sizing pass of a capture: run `pass` (capacity mode) and
+// read its counters; while a tree-walk wavefront overflowed its region,
+// give the regions twice that wavefront's count and run it again.  A region
+// overflow reported by an earlier replay doubles the regions first.
+template <class F>
+int size_walk(bch_context* c, F pass) {
+  if (c->walk_grow) {
+    c->walk_R *= 2;
+    c->walk_grow = false;
+  }
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    int rc = pass();
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->host_cnt.wave_over <= c->walk_R) return BCH_OK;
+    uint32_t r = c->walk_R;
+    while (r < 2 * c->host_cnt.wave_over) r *= 2;
+    c->walk_R = r;
+  }
+  return fail(c, BCH_HIP_ERROR, "tree-walk regions did not converge");
+}
+
 int read_capacity_counts(bch_context* c, uint64_t* n_pairs, uint64_t* n_contacts, uint64_t* found_pairs,
                          uint64_t* found_contacts) {
   HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
@@ -991,8 +1034,10 @@ int read_capacity_counts(bch_context* c, uint64_t* n_pairs, uint64_t* n_contacts
   if (found_contacts) *found_contacts = h.n_contacts;
   if (h.errors & 1u) return fail(c, BCH_HIP_ERROR, "box-box clip polygon overflow");
   if (h.errors & 2u) return fail(c, BCH_HIP_ERROR, "EPA polytope store overflow");
-  if (h.max_leaf > c->K || (h.errors & 4u))
-    return fail(c, BCH_OVERFLOW, "pair slots overflowed: capture again at these poses");
+  if (h.errors & 128u) {
+    c->walk_grow = true;
+    return fail(c, BCH_OVERFLOW, "a tree-walk wavefront's pair region overflowed: capture again");
+  }
   if (h.errors & 8u) return fail(c, BCH_OVERFLOW, "pair capacity exceeded");
   if (h.errors & 16u) return fail(c, BCH_OVERFLOW, "contact capacity exceeded");
   return BCH_OK;
@@ -1066,8 +1111,8 @@ void bch_destroy(bch_context* c) {
   DevBuf* all[] = {&c->bodies, &c->hull, &c->world_offsets, &c->world_of, &c->staging_a, &c->staging_b,
                    &c->staging_c, &c->staging_d, &c->staging_e, &c->staging_f, &c->aabb, &c->flags, &c->big_ids,
                    &c->counters, &c->partials, &c->keys, &c->keys_alt, &c->vals, &c->vals_alt, &c->nodes, &c->hull_ids,
-                   &c->btype, &c->wend, &c->leaf_counts, &c->leaf_offsets, &c->slots, &c->pair_keys,
-                   &c->pair_keys_alt, &c->big_pairs, &c->pairs, &c->cls, &c->cls_alt, &c->perm, &c->perm_alt,
+                   &c->btype, &c->wend, &c->walk_regions,
+                   &c->wave_counts, &c->wave_offs, &c->pair_keys, &c->pair_keys_alt, &c->pairs, &c->cls, &c->cls_alt, &c->perm, &c->perm_alt,
                    &c->ccount, &c->coffsets, &c->cslots, &c->contacts, &c->epa_q, &c->epa_q2,
                    &c->scan_part, &c->rs_hist, &c->rs_offs, &c->sd_out, &c->sd_status, &c->pen_pairs,
                    &c->pen_g, &c->g14, &c->body_grad, &c->pen_part, &c->dhist, &c->dstep, &c->mkeys[0],
@@ -1103,7 +1148,7 @@ int bch_synchronize(bch_context* c) {
 
 int bch_last_timings(bch_context* c, float* out_ms) {
   if (!c || !out_ms) return BCH_INVALID_ARG;
-  for (int i = 0; i < 8; ++i) out_ms[i] = c->timings[i];
+  for (int i = 0; i < 9; ++i) out_ms[i] = c->timings[i];
   return BCH_OK;
 }
 
@@ -1130,9 +1175,10 @@ int bch_set_tuning(bch_context* c, int knob, int value) {
       if (value != 4 && value != 8 && value != 16) return fail(c, BCH_INVALID_ARG, "hull lanes must be 4, 8 or 16");
       c->hull_lanes = value;
       return BCH_OK;
-    case BCH_TUNE_PAIR_SLOTS:
-      if (value < 1 || value > 4096) return fail(c, BCH_INVALID_ARG, "pair slots per leaf must be in [1, 4096]");
-      c->K = (uint32_t)value;
+    case BCH_TUNE_TRAVERSE_BLOCK:
+      if (value != 64 && value != 128 && value != 256 && value != 512)
+        return fail(c, BCH_INVALID_ARG, "tree-walk block must be 64, 128, 256 or 512 threads");
+      c->trav_block = (uint32_t)value;
       return BCH_OK;
     case BCH_TUNE_EPA_GRID:
       if (value < 1 || value > (1 << 20)) return fail(c, BCH_INVALID_ARG, "EPA grid must be in [1, 2^20]");
@@ -1169,7 +1215,7 @@ int bch_autotune(bch_context* c, float aabb_margin, int trials, float* out_ms) {
   if (!c || trials < 1) return BCH_INVALID_ARG;
   if (!c->n) return fail(c, BCH_INVALID_ARG, "no bodies: call bch_world_set_bodies first");
   float best = 0.0f;
-  int rc = time_pass(c, aabb_margin, trials, &best);  // also grows the pair slots to fit
+  int rc = time_pass(c, aabb_margin, trials, &best);  // also sizes the pair buffer
   if (rc) return rc;
   // one knob at a time, in the order of their weight in the pass
   const int hull_opts[] = {8, 16};
@@ -1206,18 +1252,16 @@ int bch_autotune(bch_context* c, float aabb_margin, int trials, float* out_ms) {
     c->np_grid = keep;
   }
   {
-    // per-leaf pair slots: the smallest power of two that held every leaf
-    // (fewer slots, less to compact), or twice that
-    uint32_t fit = 8;
-    while (fit < c->host_cnt.max_leaf) fit *= 2;
-    uint32_t keep = c->K;
-    for (uint32_t v : {fit, 2 * fit}) {
-      c->K = v;
+    // threads per tree-walk block (the walk's tile: leaves per block, and so
+    // the run of leaves each XCD's blocks cover)
+    uint32_t keep = c->trav_block;
+    for (uint32_t v : {128u, 256u, 512u}) {
+      c->trav_block = v;
       float ms;
       if ((rc = time_pass(c, aabb_margin, trials, &ms))) return rc;
       if (ms < best) { best = ms; keep = v; }
     }
-    c->K = keep;
+    c->trav_block = keep;
   }
   if (out_ms) *out_ms = best;
   return BCH_OK;
@@ -1227,7 +1271,7 @@ int bch_get_tuning(bch_context* c, int knob, int* value) {
   if (!c || !value) return BCH_INVALID_ARG;
   switch (knob) {
     case BCH_TUNE_HULL_LANES: *value = c->hull_lanes; return BCH_OK;
-    case BCH_TUNE_PAIR_SLOTS: *value = (int)c->K; return BCH_OK;
+    case BCH_TUNE_TRAVERSE_BLOCK: *value = (int)c->trav_block; return BCH_OK;
     case BCH_TUNE_EPA_GRID: *value = (int)c->epa_grid; return BCH_OK;
     case BCH_TUNE_NP_GRID: *value = (int)c->np_grid; return BCH_OK;
     default: return fail(c, BCH_INVALID_ARG, "unknown tuning knob");
@@ -1272,9 +1316,10 @@ static void proxy_put(bch_context* a, uint32_t slot, const float* mn, const floa
   float4* b = a->proxy_box.as<float4>() + 2 * (size_t)slot;
   b[0] = mn ? make_float4(mn[0], mn[1], mn[2], 0.0f) : make_float4(NAN, NAN, NAN, 0.0f);
   b[1] = mx ? make_float4(mx[0], mx[1], mx[2], 0.0f) : make_float4(NAN, NAN, NAN, 0.0f);
-  if (slot < a->arena_n && !a->proxy_dirty[slot]) {
+  if (!a->arena_whole && slot < a->arena_n && !a->proxy_dirty[slot]) {
     a->proxy_dirty[slot] = 1;
     a->dirty_list.push_back(slot);
+    if (a->dirty_list.size() > (size_t)a->n_proxy / 4) a->arena_whole = true;
   }
   a->bp_pairs_valid = false;
 }
@@ -1314,6 +1359,7 @@ int bch_bp_set_aabb(bch_context* c, uint32_t proxy, const float* mn, const float
 int bch_bp_set_aabbs(bch_context* c, uint32_t count, const uint32_t* proxies, const float* mins, const float* maxs) {
   if (!c || (count && (!proxies || !mins || !maxs))) return fail(c, BCH_INVALID_ARG, "NULL argument");
   BCH_ON_BATCH(c, a);
+  if (count > a->n_proxy / 4) a->arena_whole = true;  // the next sync uploads everything: skip the bookkeeping
   for (uint32_t k = 0; k < count; ++k) {
     uint32_t p = proxies[k];
     if (p < a->n_proxy && a->proxy_live[p]) proxy_put(a, p, mins + 3 * (size_t)k, maxs + 3 * (size_t)k);
@@ -1351,7 +1397,7 @@ static int arena_sync(bch_context* a) {
   hipStream_t s = a->stream;
   uint32_t n = a->n_proxy;
   const float4* mirror = a->proxy_box.as<float4>();
-  bool whole = a->arena.bytes < (size_t)n * 2 * sizeof(float4) || a->arena_n == 0 ||
+  bool whole = a->arena.bytes < (size_t)n * 2 * sizeof(float4) || a->arena_n == 0 || a->arena_whole ||
                a->dirty_list.size() > (size_t)n / 4;
   if (whole) {
     if (a->arena.bytes < (size_t)n * 2 * sizeof(float4)) ENSURE(a, a->arena, (size_t)n * 2 * sizeof(float4));
@@ -1381,6 +1427,7 @@ static int arena_sync(bch_context* a) {
   }
   for (uint32_t i : a->dirty_list) a->proxy_dirty[i] = 0;
   a->dirty_list.clear();
+  a->arena_whole = false;
   a->arena_n = n;
   return BCH_OK;
 }
@@ -1670,7 +1717,7 @@ int bch_world_collide(bch_context* c, float aabb_margin, uint64_t* out_pairs, ui
   HIP_TRY(c, hipSetDevice(c->device));
   int rc = run_pass(c, aabb_margin);
   if (rc) return rc;
-  for (int k = 0; k < 8; ++k) (void)hipEventElapsedTime(&c->timings[k], c->ev[k], c->ev[k + 1]);
+  stage_times(c);
   if (c->host_cnt.errors & 1u) return fail(c, BCH_HIP_ERROR, "box-box clip polygon overflow");
   if (out_pairs) *out_pairs = c->P;
   if (out_contacts) *out_contacts = c->C;
@@ -1687,20 +1734,10 @@ int bch_world_capture(bch_context* c, float aabb_margin, uint64_t pair_capacity,
   release_graph(c);
   c->P_cap = pair_capacity;
   c->C_cap = contact_capacity;
-  // One uncaptured pass in capacity mode sizes every buffer and settles the
-  // per-leaf slot count for these poses; its counts are what the capture
-  // reports.
+  // One uncaptured pass in capacity mode sizes every buffer for these
+  // poses; its counts are what the capture reports.
   c->cap_mode = true;
-  int rc = BCH_OK;
-  for (int attempt = 0; attempt < 3; ++attempt) {
-    rc = run_pass(c, aabb_margin);
-    if (rc) break;
-    HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (c->host_cnt.max_leaf <= c->K) break;
-    c->K = std::max<uint32_t>(c->K * 2, (c->host_cnt.max_leaf + 7) & ~7u);
-  }
-  if (!rc && c->host_cnt.max_leaf > c->K) rc = fail(c, BCH_HIP_ERROR, "per-leaf pair slots did not converge");
+  int rc = size_walk(c, [&] { return run_pass(c, aabb_margin); });
   if (rc) {
     c->cap_mode = false;
     return rc;
@@ -1824,7 +1861,7 @@ int bch_world_manifold_pass(bch_context* c, float aabb_margin, float linear_slop
   c->mcur = nw;
   c->mM = P;
   c->mNP = total;
-  for (int k = 0; k < 8; ++k) (void)hipEventElapsedTime(&c->timings[k], c->ev[k], c->ev[k + 1]);
+  stage_times(c);
   if (c->host_cnt.errors & 1u) return fail(c, BCH_HIP_ERROR, "box-box clip polygon overflow");
   if (out_pairs) *out_pairs = P;
   if (out_points) *out_points = total;
@@ -2244,21 +2281,12 @@ int bch_depen_capture(bch_context* c, float aabb_margin, float lr, float* d_posi
   c->dhist_cap = std::max<uint32_t>(history_capacity, 1u);
   ENSURE(c, c->dhist, (size_t)c->dhist_cap * sizeof(DepenRecord));
   ENSURE(c, c->dstep, sizeof(uint32_t));
-  // An uncaptured step without the update sizes the buffers and settles the
-  // per-leaf pair slots at the current poses (which it leaves unchanged).
+  // An uncaptured step without the update sizes the buffers at the current
+  // poses (which it leaves unchanged).
   c->cap_mode = true;
-  int rc = BCH_OK;
-  for (int attempt = 0; attempt < 3; ++attempt) {
-    rc = run_depen_step(c, aabb_margin, lr, d_position, d_orientation, false);
-    if (rc) break;
-    HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (c->host_cnt.max_leaf <= c->K) break;
-    c->K = std::max<uint32_t>(c->K * 2, (c->host_cnt.max_leaf + 7) & ~7u);
-  }
-  if (!rc && c->host_cnt.max_leaf > c->K) rc = fail(c, BCH_HIP_ERROR, "per-leaf pair slots did not converge");
+  int rc = size_walk(c, [&] { return run_depen_step(c, aabb_margin, lr, d_position, d_orientation, false); });
   if (!rc && found_pairs) *found_pairs = c->host_cnt.n_pairs_total;
-  if (!rc && (c->host_cnt.n_pairs_total > pair_capacity || (c->host_cnt.errors & 4u)))
+  if (!rc && c->host_cnt.n_pairs_total > pair_capacity)
     rc = fail(c, BCH_OVERFLOW, "pair capacity exceeded at the current poses");
   if (rc) {
     c->cap_mode = false;
@@ -2310,7 +2338,8 @@ int bch_depen_history(bch_context* c, float* loss, float* max_pen, uint32_t* n_p
       if (n_pairs) n_pairs[k] = rec[k].n_pairs;
       if (n_pen) n_pen[k] = rec[k].n_pen;
     }
-    over = over || (rec[k].flags & (4u | 8u | 32u)) != 0;
+    over = over || (rec[k].flags & (8u | 128u)) != 0;
+    if (rec[k].flags & 128u) c->walk_grow = true;
   }
   if (over) return fail(c, BCH_OVERFLOW, "a step's pairs exceeded the capacity: capture again with a larger one");
   if (steps > c->dhist_cap || steps > capacity) return fail(c, BCH_OVERFLOW, "more steps than history capacity");

@@ -736,7 +736,9 @@ struct GjkResult {
   V3 normal, point_on_b;
   V3 t0, t1, t2, t3;
   // Signed-distance walks only (SD = true): the active simplex at a Band
-  // exit or at the iteration cap, and the last iterate at the cap.
+  // exit; at the iteration cap, the closest iterate (least |v| of all
+  // iterations: an f32 walk that misses the 1 - 1e-6 exit by rounding cycles
+  // through worse simplices afterwards) with its active simplex.
   bool capped;
   V3 last_v, last_pb;
   Simp act;
@@ -762,6 +764,7 @@ BCH_HD GjkResult gjk_contact_t(const Pair& pr, bool has_seed, V3 seed, float thr
   S.n = 1;
   V3 dir = -first.w;
   float touch2 = 1e-12f * fmax_nan(len2(first.w), 1.0f);
+  float best_v2 = INFINITY;
   for (int it = 0; it < 64; ++it) {
     Reduce r = reduce(S);
     if (r.inside) {
@@ -808,7 +811,8 @@ BCH_HD GjkResult gjk_contact_t(const Pair& pr, bool has_seed, V3 seed, float thr
       }
       return R;
     }
-    if (SD) {
+    if (SD && !(v2 >= best_v2)) {  // the first of equal iterates; NaN passes
+      best_v2 = v2;
       R.act = S;
       R.last_v = v;
       R.last_pb = r.p_b;

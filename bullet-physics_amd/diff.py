@@ -124,7 +124,8 @@ def depenetrate(world, position, orientation, steps: int = 100, lr: float = 0.2,
 
 
 def depenetrate_graph(world, position, orientation, steps: int = 100, lr: float = 0.2, aabb_margin: float = 0.04,
-                      headroom: float = 1.5, chunk: int = 25):
+                      headroom: float = 1.5, chunk: int = 25, pair_capacity: int | None = None,
+                      info: dict | None = None):
     """:func:`depenetrate` with each step one captured HIP graph
     (bch_depen_capture): broadphase, signed distance, the adjoint of the
     penetrating pairs, the per-body gradient and the pose update all run on
@@ -132,7 +133,9 @@ def depenetrate_graph(world, position, orientation, steps: int = 100, lr: float 
     history is read back every `chunk` steps; a chunk in which the pairs
     overflowed the capacity is rolled back and rerun after a capture with a
     larger one.  Returns the final poses and per step (loss, max
-    penetration, pairs), like :func:`depenetrate`."""
+    penetration, pairs), like :func:`depenetrate`.  `pair_capacity` fixes the
+    first capture's capacity (default: the first pass's pairs x headroom);
+    `info`, when given, receives the number of recaptures."""
     import ctypes as C
 
     import numpy as np
@@ -145,8 +148,9 @@ def depenetrate_graph(world, position, orientation, steps: int = 100, lr: float 
     _bind_stream(world)
     world.set_poses_device(pos.data_ptr(), quat.data_ptr())
     P0, _ = world.collide(aabb_margin)
-    cap = max(1024, int(P0 * headroom) + 1024)
+    cap = pair_capacity if pair_capacity is not None else max(1024, int(P0 * headroom) + 1024)
     found = C.c_uint64()
+    recaptures = 0
 
     def capture():
         nonlocal cap
@@ -178,9 +182,12 @@ def depenetrate_graph(world, position, orientation, steps: int = 100, lr: float 
                 quat.copy_(saved[1])
                 cap = max(2 * cap, int(npairs[:k].max() * headroom) + 1024)
                 capture()
+                recaptures += 1
                 continue
             _check(h, rc)
             history += [(float(loss[s]), float(maxpen[s]), int(npairs[s])) for s in range(k)]
     finally:
         L.bch_depen_release(h)
+    if info is not None:
+        info["recaptures"] = recaptures
     return pos, quat, history

@@ -73,24 +73,25 @@ int bch_create(int device, bch_context** out);
 void bch_destroy(bch_context* ctx);
 const char* bch_last_error(const bch_context* ctx);
 int bch_synchronize(bch_context* ctx);
-/* Device times (ms) of the last bch_world_collide pass, 8 floats:
+/* Device times (ms) of the last bch_world_collide pass:
  * [0] world AABBs, [1] tree build (Morton codes, sort, nodes, refit),
  * [2] tree traversal (pair search), [3] pair list (planes, compaction,
  * sort, shape-class split), [4] the fork of the narrowphase onto a second
  * stream (near zero), [5] closed-form and GJK pairs, with and without a
  * hull (k_np on the second stream beside k_np_hull), [6] EPA (k_epa_dual
- * and k_epa_wave), [7] contact compaction. */
+ * and k_epa_wave), [7] contact compaction; then [8] the two-pair EPA kernel
+ * k_epa_dual alone (part of [6]).  out_ms holds 9 floats. */
 int bch_last_timings(bch_context* ctx, float* out_ms);
 
 /* No reference counterpart: launch-shape knobs of the device passes.  They
  * change how work is spread over the GPU, never the results.
  *   BCH_TUNE_HULL_LANES  lanes per GJK pair with a hull: 4, 8, 16 (16)
- *   BCH_TUNE_PAIR_SLOTS  initial pair slots per tree leaf (16; grows itself)
+ *   BCH_TUNE_TRAVERSE_BLOCK  threads per tree-walk block: 64, 128, 256, 512 (256)
  *   BCH_TUNE_EPA_GRID    EPA blocks of one wavefront (32768)
  *   BCH_TUNE_NP_GRID     narrowphase blocks per shape class (2048) */
 enum {
   BCH_TUNE_HULL_LANES = 1,
-  BCH_TUNE_PAIR_SLOTS = 2,
+  BCH_TUNE_TRAVERSE_BLOCK = 2,
   BCH_TUNE_EPA_GRID = 3,
   BCH_TUNE_NP_GRID = 4
 };
@@ -98,7 +99,7 @@ int bch_set_tuning(bch_context* ctx, int knob, int value);
 int bch_get_tuning(bch_context* ctx, int knob, int* value);
 /* Pick the knobs for the bodies loaded by bch_world_set_bodies: times the
  * pass (median of `trials`, device time) over hull lanes, EPA grid,
- * narrowphase grid and per-leaf pair slots, one knob at a time, and keeps
+ * narrowphase grid and tree-walk block, one knob at a time, and keeps
  * the fastest; *out_ms (may be NULL) receives the best pass time. */
 int bch_autotune(bch_context* ctx, float aabb_margin, int trials, float* out_ms);
 
@@ -239,7 +240,7 @@ int bch_world_routes(bch_context* ctx, uint8_t* routes);
  * (narrowphase/gjk.rs:171-288) with threshold = +inf, so a separated pair of
  * any distance ends in its Band arm, and EPA plus the contact of
  * convex_convex.rs:117-154 for overlapping cores.  A walk stopped by its
- * iteration cap reports its last iterate like the Band arm.  All pointers
+ * iteration cap reports its closest iterate like the Band arm.  All pointers
  * are DEVICE pointers; the calls are ordered on the context stream and do
  * not synchronise.
  *   d_pairs  [2P] body indices (i, j): shape A = body i, shape B = body j
@@ -275,7 +276,7 @@ int bch_world_results_device(bch_context* ctx, const uint32_t** d_pairs, const u
  * trip.  The pass is run in capacity mode, with buffers for pair_capacity
  * pairs and contact_capacity contacts, counts and overflow flags kept on the
  * device, and captured in a HIP graph.  Before capturing, one uncaptured
- * pass at the current poses sizes the buffers and the per-leaf pair slots;
+ * pass at the current poses sizes the buffers;
  * its counts are returned like bch_world_replay_counts (BCH_OVERFLOW if that
  * pass already overflowed: capture again with larger capacities).  Results
  * are identical to bch_world_collide's. */
@@ -290,7 +291,7 @@ int bch_world_replay(bch_context* ctx);
  * *n_pairs / *n_contacts are what the buffers hold, the sizes
  * bch_world_results then writes; *found_pairs / *found_contacts what the
  * pass found (any pointer may be NULL).  The two agree unless the result is
- * BCH_OVERFLOW: a capacity (or the per-leaf pair slots) was exceeded, the
+ * BCH_OVERFLOW: a capacity was exceeded, the
  * held results are incomplete, capture again with capacities above the
  * found counts. */
 int bch_world_replay_counts(bch_context* ctx, uint64_t* n_pairs, uint64_t* n_contacts, uint64_t* found_pairs,

@@ -752,11 +752,13 @@ struct GjkOut {
   V3<T> normal{0, 0, 0}, point_on_b{0, 0, 0};
   V3<T> simplex[4];
   int iterations = 0;
-  // Not in the reference (signed-distance callers only): the walk's last
-  // closest point v and B witness when it stopped at the 64-iteration cap.
+  // Not in the reference (signed-distance callers only): when the walk
+  // stopped at the 64-iteration cap, its closest iterate v (the least |v| of
+  // all iterations; an f32 walk that misses the 1 - 1e-6 exit by rounding
+  // cycles through worse simplices afterwards) and B witness.
   bool capped = false;
   V3<T> last_v{0, 0, 0}, last_pb{0, 0, 0};
-  // the active simplex (w, b) at a Band exit or at the cap
+  // the active simplex (w, b) at a Band exit, or of that iterate at the cap
   SupportVertex<T> active[4];
   int n_active = 0;
 };
@@ -1095,6 +1097,7 @@ GjkOut<T> gjk_contact(const PairPose<T>& pr, const V3<T>* seed, T threshold) {
   V3<T> dir = -first.w;
   T touch2 = ORL(1e-12) * std::fmax(len2(first.w), T(1));
   const T REL_ERROR2 = ORL(1e-6);
+  T best_v2 = std::numeric_limits<T>::infinity();
   for (int it = 0; it < 64; ++it) {
     out.iterations = it + 1;
     Reduce<T> r = reduce(buf, n);
@@ -1141,12 +1144,15 @@ GjkOut<T> gjk_contact(const PairPose<T>& pr, const V3<T>* seed, T threshold) {
       out.kind = 0;
       return out;
     }
-    for (int i = 0; i < n; ++i) out.active[i] = buf[i];
-    out.n_active = n;
+    if (!(v2 >= best_v2)) {  // the closest iterate so far: the first of equals, and NaN passes
+      best_v2 = v2;
+      for (int i = 0; i < n; ++i) out.active[i] = buf[i];
+      out.n_active = n;
+      out.last_v = v;
+      out.last_pb = r.p_b;
+    }
     buf[n] = w;
     ++n;
-    out.last_v = v;
-    out.last_pb = r.p_b;
   }
   out.kind = 0;
   out.capped = true;

@@ -111,6 +111,10 @@ MUTATIONS = [
      "capsule.rs:59-69 core tip picks +hh for dir.y >= 0",
      "        return mk(T(0), dir.y >= T(0) ? half_height : -half_height, T(0));",
      "        return mk(T(0), dir.y > T(0) ? half_height : -half_height, T(0));"),
+    ("capped_last_iterate",
+     "signed-distance extension: a capped walk reports its closest iterate, not the last",
+     "    if (!(v2 >= best_v2)) {",
+     "    if ((true)) {"),
 ]
 
 TESTS = ["tests/test_oracle_kat.py", "tests/test_oracle_math.py", "tests/test_oracle_pins.py",

@@ -370,3 +370,50 @@ def test_captured_depenetration_reduces_penetration(torch, bpa, scenes, ctx, dif
     assert hist[-1][1] < 0.5 * hist[0][1]
     n = q.norm(dim=1)
     assert torch.all((n - 1).abs() < 1e-5)
+
+
+def test_captured_depenetration_recovers_from_a_mid_chunk_overflow(torch, bpa, scenes, ctx, diff):
+    """A capacity of exactly the first pass's pairs, and the descent run
+    backwards (lr < 0 drives penetrating bodies together, so later steps find
+    more pairs): a step inside a chunk overflows, the chunk is rolled back to
+    its starting poses, captured again with a larger capacity and rerun, and
+    the run still records every step."""
+    _flush("case: captured depenetration, mid-chunk pair overflow, 16 worlds")
+    sc = scenes.batched_boxes_hulls(16, 21, 64, 256)
+    w = bpa.CollisionWorld(sc, ctx)
+    pos = torch.from_numpy(sc["position"]).cuda()
+    quat = torch.from_numpy(sc["orientation"]).cuda()
+    P0, _ = w.collide(0.04)
+    info = {}
+    p, q, hist = diff.depenetrate_graph(w, pos, quat, steps=24, lr=-0.05, chunk=8, pair_capacity=P0, info=info)
+    pairs = [h[2] for h in hist]
+    _flush(f"pairs {P0} -> {max(pairs)}, recaptures {info['recaptures']}")
+    assert len(hist) == 24
+    assert max(pairs) > P0 and info["recaptures"] >= 1
+    assert hist[-1][1] > hist[0][1]  # run backwards, the penetration grows
+    n = q.norm(dim=1)
+    assert torch.all((n - 1).abs() < 1e-5)
+
+
+def test_both_descent_loops_end_near_zero_penetration(torch, bpa, scenes, ctx, diff):
+    """100 steps of the autograd loop and of the captured loop from the same
+    start: the two trajectories part after a few steps (float-atomic gradient
+    sums, then EPA's 1e-4 tolerance), but both end with the loss below 1e-6
+    of its start.  Pushing one pair apart can drive a body into a neighbour:
+    a fresh overlap of up to a few cm then appears and decays by about 5x per
+    step, so the end state depends on when the last one came.  Both loops
+    must therefore also reach, at some step of the last 10, a loss below 1e-9
+    of the start with no pair deeper than 0.1 mm (the f32 floor is ~0.5 um)."""
+    _flush("case: autograd and captured descent, 100 steps, 16 worlds")
+    sc = scenes.batched_boxes_hulls(16, 21, 64, 256)
+    w = bpa.CollisionWorld(sc, ctx)
+    pos = torch.from_numpy(sc["position"]).cuda()
+    quat = torch.from_numpy(sc["orientation"]).cuda()
+    _, _, ha = diff.depenetrate(w, pos, quat, steps=100)
+    _, _, hg = diff.depenetrate_graph(w, pos, quat, steps=100, chunk=25)
+    _flush(f"autograd: loss {ha[0][0]:.4g} -> {ha[-1][0]:.3g}, max pen {ha[-1][1]:.3g}; "
+           f"captured: loss {hg[0][0]:.4g} -> {hg[-1][0]:.3g}, max pen {hg[-1][1]:.3g}")
+    for h in (ha, hg):
+        assert h[-1][0] < 1e-6 * h[0][0]
+        best = min(h[-10:], key=lambda e: e[0])
+        assert best[0] < 1e-9 * h[0][0] and best[1] < 1e-4, best

@@ -375,3 +375,24 @@ def test_signed_distance_reference_adjoint_matches_f64_differences(oracle, scene
     assert smooth.sum() >= 0.9 * P
     assert err[smooth].max() < 1e-3
     assert np.median(err[smooth]) < 1e-5
+
+
+def test_capped_walk_reports_its_closest_iterate(oracle, scenes):
+    """An f32 walk that misses the 1 - 1e-6 exit by rounding cycles through
+    worse simplices until the 64-iteration cap.  The signed distance then
+    reports the closest iterate, which meets the f64 walk (converged on the
+    same pairs) to 1e-4 m on at least 90 % of them and to 1 cm on all (a walk
+    that starts cycling before it nears the minimum keeps a coarser best);
+    the last iterate of the cycle was up to 9 cm off (box-hull pairs of the
+    de-penetration worlds)."""
+    sc = scenes.batched_boxes_hulls(16, 5)
+    pairs = oracle.sweep_pairs(*oracle.scene_aabbs(sc, 0.04))
+    o32, st32 = oracle.sd_pairs(sc, pairs)
+    o64, st64 = oracle.sd_pairs(sc, pairs, None, f64=True)
+    capped = st32 == oracle.SD_CAPPED
+    both = capped & (st64 == oracle.SD_WALK)
+    assert capped.sum() >= 20 and both.sum() >= 0.9 * capped.sum()
+    err = np.abs(o32[both, 0] - o64[both, 0])
+    close = err < 1e-4
+    assert close.mean() >= 0.9 and err.max() < 1e-2
+    assert np.abs(o32[both, 1:4] - o64[both, 1:4])[close].max() < 1e-3
