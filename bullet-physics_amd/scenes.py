@@ -321,7 +321,81 @@ def hull_pairs(n_pairs: int = 262_144, seed: int = 4, npts: int = 32, semi_lo: f
     sc["position"] = pos
     sc["pairs"] = np.stack([2 * k, 2 * k + 1], axis=1).astype(np.uint32)
     sc["target"] = target.astype(np.float32)
+    # B at s along u is b_base + s u (s = target above); exact_penetration
+    # moves the overlapping half along this line
+    sc["direction"] = u
+    sc["b_base"] = pa + sa - sb
     return sc
+
+
+def exact_penetration(sc: dict, signed_distance, iters: int = 40, reach: float = 2.5, tol: float = 1e-4) -> dict:
+    """Place B of every overlapping pair of a :func:`hull_pairs` scene so that
+    its signed distance equals its target -U[0, gap], by bisection along the
+    pair's direction u.
+
+    ``signed_distance(position)`` returns the signed distance of every pair
+    of ``sc["pairs"]`` with the bodies at ``position`` [2P, 3] (float32).
+    Along the line B = b_base + s u the signed distance is convex in s (the
+    signed distance to the convex Minkowski difference) and equals s for
+    s >= 0, so for a target t < 0 the set {s : sd(s) <= t} is an interval
+    whose right end is the one crossing in [-reach, 0].  A pair whose line
+    never reaches its target depth keeps the deepest point found on a
+    coarse scan instead, and its target becomes the signed distance there;
+    their number is returned in ``sc["unreached"]``.  A pair whose computed
+    signed distance jumps across the crossing (f32 walks near the iteration
+    cap) ends more than ``tol`` from its target and takes the signed
+    distance it ends at as its target too (``sc["retargeted"]``).
+    ``sc["overlap"]`` marks the pairs of the overlapping half.  Separated
+    pairs are left as built (exact by construction)."""
+    tgt = sc["target"].astype(np.float64)
+    pen = tgt < 0
+    u = sc["direction"]
+    base = sc["b_base"]
+    pos = sc["position"].copy()
+
+    b_rows = 2 * np.nonzero(pen)[0] + 1
+
+    def place(s):
+        pos[b_rows] = (base[pen] + u[pen] * s[pen, None]).astype(np.float32)
+
+    def sd_at(s):
+        place(s)
+        return np.asarray(signed_distance(pos), np.float64)
+
+    # a coarse scan for a start at or below the target (the deepest sample
+    # where none is)
+    n = tgt.shape[0]
+    lo = np.full(n, -reach)
+    best_s, best_sd = np.zeros(n), np.zeros(n)
+    found = ~pen
+    for s_try in np.linspace(-reach / 10, -reach, 10):
+        sv = np.full(n, s_try)
+        sd = sd_at(sv)
+        deeper = pen & (sd < best_sd)
+        best_s = np.where(deeper, s_try, best_s)
+        best_sd = np.where(deeper, sd, best_sd)
+        hit = pen & ~found & (sd <= tgt)
+        lo = np.where(hit, s_try, lo)
+        found |= hit
+    unreached = pen & ~found
+    tgt = np.where(unreached, best_sd, tgt)
+    lo = np.where(unreached, best_s, lo)
+    hi = np.zeros(n)
+    for _ in range(iters):
+        mid = 0.5 * (lo + hi)
+        below = sd_at(mid) <= tgt
+        lo = np.where(pen & below, mid, lo)
+        hi = np.where(pen & ~below, mid, hi)
+    sd = sd_at(np.where(unreached, best_s, 0.5 * (lo + hi)))
+    off = pen & (np.abs(sd - tgt) > tol)
+    tgt = np.where(off | unreached, sd, tgt)
+    out = dict(sc)
+    out["position"] = pos
+    out["target"] = tgt.astype(np.float32)
+    out["overlap"] = pen
+    out["unreached"] = int(unreached.sum())
+    out["retargeted"] = int((off & ~unreached).sum())
+    return out
 
 
 def batched_boxes_hulls(n_worlds: int = 1024, seed: int = 5, lo_count: int = 64, hi_count: int = 1024,

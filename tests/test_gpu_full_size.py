@@ -1,0 +1,189 @@
+"""The benchmark configurations at their full size on the GPU.
+
+- 1,000,000 mixed convex bodies: the whole pair set against the oracle's
+  DbvtBroadPhase pairs (values and order), the partner sets of 256 sampled
+  bodies against brute force over every body, and the contacts of 50,000
+  sampled pairs against the oracle's collide_seeded.
+- 4,096 worlds of 16-2,048 bodies: the batched pass equals the plain pass
+  world by world for every world, and the oracle's pairs and contacts for 16
+  worlds including the largest.
+- 262,144 hull pairs at signed distances +-U[0, 0.2] m: every separated pair
+  whose walk converges at its target (the few that cycle to the iteration
+  cap are counted), 16,384 sampled pairs against the oracle's f32 signed
+  distance, and 4,096 sampled overlapping pairs at their target depth by the
+  oracle's f64 signed distance.
+- 1,024 worlds of 64-1,024 boxes and hulls: the captured de-penetration step
+  against the autograd step from the same poses.
+"""
+import numpy as np
+import pytest
+
+from test_gpu_parity import compare_contacts
+
+pytestmark = pytest.mark.gpu
+
+MARGIN = 0.04
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    assert t.cuda.is_available()
+    return t
+
+
+def test_full_mixed_scene(bpa, scenes, oracle, ctx):
+    _flush("case: mixed convex scene, 1000000 bodies, seed 2")
+    sc = scenes.mixed_convex(1_000_000, 2)
+    w = bpa.CollisionWorld(sc, ctx)
+    P, Cn = w.collide(MARGIN)
+    pairs, offs, cont = w.results()
+    _flush(f"  GPU pass: {P} pairs, {Cn} contacts")
+    mn, mx = oracle.scene_aabbs(sc, MARGIN)
+    ref = oracle.dbvt_pairs(mn, mx)
+    assert np.array_equal(pairs, ref), "pair set differs from the oracle's DBVT pairs"
+    _flush("  pair set equal to the oracle's DBVT pairs, order included")
+    rng = np.random.default_rng(5)
+    n = sc["shape_type"].shape[0]
+    for i in rng.choice(n, 256, replace=False):
+        ov = np.nonzero(np.all((mn[i] <= mx) & (mn <= mx[i]), axis=1))[0]
+        ov = ov[ov != i]
+        got = np.union1d(pairs[pairs[:, 0] == i, 1], pairs[pairs[:, 1] == i, 0])
+        assert np.array_equal(got, ov), f"partner set of body {i}"
+    _flush("  256 partner sets equal to brute force")
+    k = np.sort(rng.choice(P, 50_000, replace=False))
+    rc, rp, _ = oracle.collide_pairs(sc, pairs[k])
+    o = offs.astype(np.int64)
+    sub_offs = np.concatenate(([0], np.cumsum(o[k + 1] - o[k])))
+    sub = np.concatenate([cont[o[j]:o[j + 1]] for j in k])
+    compare_contacts(rc, rp, sub_offs, sub, label="1M scene, 50000 sampled pairs")
+
+
+def test_full_batched_worlds(bpa, scenes, oracle, ctx):
+    _flush("case: 4096 batched worlds, 16-2048 bodies, seed 3")
+    sc = scenes.batched_worlds(4096, 3)
+    w = bpa.CollisionWorld(sc, ctx)
+    P, Cn = w.collide(MARGIN)
+    pairs, offs, cont = w.results()
+    wo = sc["world_offsets"].astype(np.int64)
+    nw = wo.shape[0] - 1
+    _flush(f"  batched pass: {sc['shape_type'].shape[0]} bodies, {P} pairs, {Cn} contacts")
+    world_of = np.searchsorted(wo, pairs[:, 0], side="right") - 1
+    assert np.all(pairs[:, 1] < wo[world_of + 1]), "pair crosses worlds"
+    first = np.searchsorted(world_of, np.arange(nw + 1))
+    sizes = np.diff(wo)
+    check = set(np.random.default_rng(6).choice(nw, 15, replace=False).tolist()) | {int(np.argmax(sizes))}
+    o = offs.astype(np.int64)
+    for wi in range(nw):
+        sub = scenes.world_slice(sc, wi)
+        pw = bpa.CollisionWorld(sub, ctx, batched=False)
+        pw.collide(MARGIN)
+        pp, po, pc = pw.results()
+        a, b = first[wi], first[wi + 1]
+        assert np.array_equal(pairs[a:b] - wo[wi], pp), f"world {wi} pairs differ"
+        assert np.array_equal(cont[o[a]:o[b]], pc), f"world {wi} contacts differ"
+        if wi in check:
+            smn, smx = oracle.scene_aabbs(sub, MARGIN)
+            assert np.array_equal(pp, oracle.brute_pairs(smn, smx)), f"world {wi} pairs differ from brute force"
+            rc, rp, _ = oracle.collide_pairs(sub, pp)
+            compare_contacts(rc, rp, po, pc, label=f"world {wi} ({sizes[wi]} bodies)")
+        if wi % 512 == 0:
+            _flush(f"  world {wi}: equal to the plain pass")
+    _flush(f"  all {nw} worlds equal to the plain pass; {len(check)} against the oracle")
+
+
+def test_full_hull_pairs(torch, bpa, scenes, oracle, ctx):
+    _flush("case: 262144 hull pairs, signed distance +-U[0, 0.2] m, seed 4")
+    sc = scenes.hull_pairs(262_144, 4)
+    P = sc["pairs"].shape[0]
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    try:
+        w = bpa.CollisionWorld(sc, ctx, batched=False)
+        pairs = torch.from_numpy(sc["pairs"].astype(np.int32)).cuda()
+        out = torch.empty((P, 10), device="cuda")
+        st = torch.empty(P, dtype=torch.uint8, device="cuda")
+
+        def sd_at(position):
+            w.set_poses(position, sc["orientation"])
+            w.sd_forward(pairs.data_ptr(), P, out.data_ptr(), st.data_ptr())
+            return out[:, 0].cpu().numpy()
+
+        sc = scenes.exact_penetration(sc, sd_at)
+        w.set_poses(sc["position"], sc["orientation"])
+        w.sd_forward(pairs.data_ptr(), P, out.data_ptr(), st.data_ptr())
+        got = out.cpu().numpy()
+        gst = st.cpu().numpy() & 7
+    finally:
+        ctx.set_stream(None)
+    pen = sc["overlap"]
+    err = np.abs(got[:, 0].astype(np.float64) - sc["target"])
+    _flush(f"  placement: {sc['unreached']} overlapping pairs never reach their depth on their line, "
+           f"{sc['retargeted']} retargeted; max |sd - target| {err[~pen].max():.3g} separated, "
+           f"{err[pen].max():.3g} overlapping")
+    assert sc["unreached"] + sc["retargeted"] <= 0.02 * pen.sum()
+    assert err[pen].max() <= 1e-4
+    # The reference's reduce_tetrahedron keeps the first face whose plane
+    # separates the origin (gjk.rs:547-566), not the closest one, so on a few
+    # pairs |v| grows and the walk cycles to the 64-iteration cap, in f64 as
+    # in f32; the oracle's restatement does the same.  Those are counted, the
+    # rest held to the target.  A pair within f32 resolution of touching
+    # (target < 1e-4 m) may find its cores overlapping and go to EPA.
+    walk = gst == 0
+    capped = (~pen) & (gst == 2)
+    touch = (~pen) & (gst == 1)
+    _flush(f"  separated: {int((~pen & walk).sum())} walks within {err[~pen & walk].max():.3g} of the target, "
+           f"{int(touch.sum())} near-touching pairs through EPA, {int(capped.sum())} capped walks "
+           f"(max |sd - target| {err[capped].max() if capped.any() else 0:.3g})")
+    assert np.all(walk | touch | capped | pen)
+    assert err[~pen & ~capped].max() < 1e-4 and (sc["target"][touch] < 1e-4).all()
+    assert capped.sum() <= 1e-4 * P
+    rng = np.random.default_rng(7)
+    k = np.sort(rng.choice(P, 16_384, replace=False))
+    ref, rst = oracle.sd_pairs(sc, sc["pairs"][k])
+    assert np.array_equal(gst[k], rst)
+    exact = int(np.all(got[k] == ref, axis=1).sum())
+    d = np.abs(got[k, 0].astype(np.float64) - ref[:, 0])
+    _flush(f"  16384 sampled pairs: {exact} bit-identical to the oracle's f32 signed distance, max |dsd| {d.max():.3g}")
+    assert d.max() <= 1e-5
+    kp = np.sort(rng.choice(np.nonzero(pen & (sc["target"] < 0))[0], 4096, replace=False))
+    r64, s64 = oracle.sd_pairs(sc, sc["pairs"][kp], None, f64=True)
+    e64 = np.abs(r64[:, 0] - sc["target"][kp])
+    _flush(f"  4096 sampled overlapping pairs: max |sd_f64 - target| {e64.max():.3g}")
+    assert (s64 == oracle.SD_EPA).all() and e64.max() < 1e-4
+
+
+def test_full_depenetration_step(torch, bpa, scenes, ctx):
+    import importlib
+    diff = importlib.import_module("bullet-physics_amd.diff")
+    _flush("case: de-penetration, 1024 worlds of 64-1024 boxes and hulls, seed 5")
+    sc = scenes.batched_boxes_hulls(1024, 5)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    try:
+        w = bpa.CollisionWorld(sc, ctx)
+        pos = torch.from_numpy(sc["position"]).cuda()
+        quat = torch.from_numpy(sc["orientation"]).cuda()
+        p_a, q_a, h_a = diff.depenetrate(w, pos, quat, steps=1)
+        p_g, q_g, h_g = diff.depenetrate_graph(w, pos, quat, steps=1)
+    finally:
+        ctx.set_stream(None)
+    (l_a, m_a, n_a), (l_b, m_b, n_b) = h_a[0], h_g[0]
+    _flush(f"  step 0: loss autograd {l_a:.8g} captured {l_b:.8g}; max pen {m_a:.7g} / {m_b:.7g}; "
+           f"pairs {n_a} / {n_b}")
+    assert n_a == n_b and m_a == m_b
+    assert abs(l_a - l_b) <= 1e-6 * l_a
+    dp = (p_g - p_a).abs().max().item()
+    dq = (q_g - q_a).abs().max().item()
+    # both sides sum the per-body gradients with float atomics in their own
+    # order; each updated coordinate then differs by the rounding of its step
+    # plus an ulp or two of the coordinate itself (the worlds sit side by side,
+    # so coordinates reach thousands of metres)
+    pa_np = p_a.cpu().numpy()
+    d_np = np.abs(p_g.cpu().numpy() - pa_np)
+    excess = (d_np - 2 * np.spacing(np.abs(pa_np))).max()
+    _flush(f"  after 1 step: max |dpos| {dp:.3g}, beyond 2 ulp of its coordinate by at most {excess:.3g}; "
+           f"max |dquat| {dq:.3g}")
+    assert excess < 1e-5 and dq < 1e-5

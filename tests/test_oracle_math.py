@@ -396,3 +396,30 @@ def test_capped_walk_reports_its_closest_iterate(oracle, scenes):
     close = err < 1e-4
     assert close.mean() >= 0.9 and err.max() < 1e-2
     assert np.abs(o32[both, 1:4] - o64[both, 1:4])[close].max() < 1e-3
+
+
+def test_exact_penetration_places_overlapping_pairs_at_their_depth(oracle, scenes):
+    """scenes.exact_penetration bisects B along the pair's direction until
+    the signed distance (here the oracle's f32 one, as the library's is on
+    the GPU) meets the target -U[0, 0.2] m.  The f64 signed distance of the
+    placed pairs then meets the target to 1e-4 m (EPA's exit tolerance), the
+    separated half is untouched, and at most 2 % of the overlapping pairs
+    keep a lowered target because their line never reaches the drawn
+    depth."""
+    sc = scenes.hull_pairs(2000, 4)
+
+    def sd32(pos):
+        s2 = dict(sc)
+        s2["position"] = pos
+        return oracle.sd_pairs(s2, sc["pairs"])[0][:, 0]
+
+    ex = scenes.exact_penetration(sc, sd32)
+    pen = sc["target"] < 0
+    assert np.array_equal(ex["overlap"], pen)
+    assert np.array_equal(ex["position"][0::2], sc["position"][0::2])
+    assert np.array_equal(ex["position"][1::2][~pen], sc["position"][1::2][~pen])
+    assert ex["unreached"] + ex["retargeted"] <= 0.02 * pen.sum()
+    out, st = oracle.sd_pairs(ex, ex["pairs"], None, f64=True)
+    assert np.abs(out[:, 0] - ex["target"]).max() < 1e-4
+    assert (st[pen & (ex["target"] < 0)] == oracle.SD_EPA).all()
+    assert (ex["target"][pen] == sc["target"][pen]).sum() >= 0.98 * pen.sum()
