@@ -49,11 +49,12 @@ class _Context:
         self.h = h
 
     _KNOBS = {"hull_lanes": _lib.TUNE_HULL_LANES,
-              "traverse_block": _lib.TUNE_TRAVERSE_BLOCK, "epa_grid": _lib.TUNE_EPA_GRID, "np_grid": _lib.TUNE_NP_GRID}
+              "traverse_block": _lib.TUNE_TRAVERSE_BLOCK, "epa_grid": _lib.TUNE_EPA_GRID, "np_grid": _lib.TUNE_NP_GRID,
+              "epa_wave_faces": _lib.TUNE_EPA_WAVE_FACES}
 
     def set_tuning(self, **knobs):
         """Launch-shape knobs (hull_lanes, traverse_block, epa_grid,
-        np_grid); they never change results."""
+        np_grid, epa_wave_faces); they never change results."""
         for k, v in knobs.items():
             if k not in self._KNOBS:
                 raise KeyError(f"unknown tuning knob {k!r}")
@@ -589,6 +590,13 @@ class CollisionWorld:
         out = np.zeros(9, np.float32)
         _check(self.ctx.h, self.L.bch_last_timings(self.ctx.h, out.ctypes.data))
         return out
+
+    def epa_routes(self):
+        """EPA entries of the last pass: (queued, handed to the one-pair
+        kernel, rerun with the 2048-face store)."""
+        out = np.zeros(3, np.uint32)
+        _check(self.ctx.h, self.L.bch_last_epa_routes(self.ctx.h, out.ctypes.data))
+        return tuple(int(x) for x in out)
 
     def synchronize(self):
         _check(self.ctx.h, self.L.bch_synchronize(self.ctx.h))

@@ -16,7 +16,7 @@ BCH_OK, BCH_OVERFLOW, BCH_INVALID_ARG, BCH_HIP_ERROR, BCH_NO_DEVICE = 0, 1, 2, 3
 
 # Every symbol include/bullet_collision_hip.h declares.
 EXPORTS = [
-    "bch_create", "bch_destroy", "bch_last_error", "bch_synchronize", "bch_last_timings",
+    "bch_create", "bch_destroy", "bch_last_error", "bch_synchronize", "bch_last_timings", "bch_last_epa_routes",
     "bch_bp_create_proxy", "bch_bp_set_aabb", "bch_bp_destroy_proxy", "bch_bp_proxy_aabb", "bch_bp_set_aabbs",
     "bch_bp_compute_pairs", "bch_bp_aabb_candidates", "bch_bp_raycast_candidates", "bch_collide_batch",
     "bch_world_set_bodies", "bch_world_set_poses", "bch_world_set_poses_device", "bch_world_collide",
@@ -32,7 +32,7 @@ EXPORTS = [
 ]
 
 # launch-shape knobs (bch_set_tuning)
-TUNE_HULL_LANES, TUNE_TRAVERSE_BLOCK, TUNE_EPA_GRID, TUNE_NP_GRID = 1, 2, 3, 4
+TUNE_HULL_LANES, TUNE_TRAVERSE_BLOCK, TUNE_EPA_GRID, TUNE_NP_GRID, TUNE_EPA_WAVE_FACES = 1, 2, 3, 4, 5
 
 
 class BchBodies(C.Structure):
@@ -85,6 +85,7 @@ def load():
     L.bch_last_error.restype = C.c_char_p
     L.bch_synchronize.argtypes = [vp]
     L.bch_last_timings.argtypes = [vp, vp]
+    L.bch_last_epa_routes.argtypes = [vp, vp]
     L.bch_bp_create_proxy.argtypes = [vp, vp, vp, C.POINTER(u32)]
     L.bch_bp_set_aabb.argtypes = [vp, u32, vp, vp]
     L.bch_bp_destroy_proxy.argtypes = [vp, u32]

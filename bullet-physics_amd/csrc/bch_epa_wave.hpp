@@ -22,6 +22,13 @@
 // polytope outside that fast path (more than 64 faces, more than 21 visible
 // faces, or a directed edge shared by two visible faces) runs the general
 // stitch, which searches the list with one ballot per 64 entries.
+//
+// The polytope store is a template: the one-pair kernels keep 256 faces in
+// LDS (8 KB per wavefront).  A convex polytope of <= 128 vertices has at most
+// 2 * 128 - 4 = 252 faces, so only a numerically non-convex expansion (a
+// visible set that is not one disk) can outgrow it; such an entry goes to an
+// overflow queue and k_epa_big reruns it from the start with a 2048-face
+// store (51 KB of LDS), so one pathological pair never fails a pass.
 #pragma once
 #include "bch_kernels.hpp"
 
@@ -29,13 +36,19 @@ namespace bch {
 
 #define BCH_WEPA_V 128
 #define BCH_WEPA_F 256
-#define BCH_WEPA_R 4  // horizon registers per lane in the general stitch (256 edges)
+#define BCH_WEPA_BIG_F 2048
 
-struct EpaLds {
+// F faces; the general stitch holds up to F horizon edges, F / 64 registers
+// per lane.
+template <int F>
+struct WaveEpaStore {
+  static constexpr int kF = F;
   float4 v[BCH_WEPA_V];
-  float nx[BCH_WEPA_F], ny[BCH_WEPA_F], nz[BCH_WEPA_F], dist[BCH_WEPA_F], pd[BCH_WEPA_F];
-  uint32_t abc[BCH_WEPA_F];
+  float nx[F], ny[F], nz[F], dist[F], pd[F];
+  uint32_t abc[F];
 };
+using EpaLds = WaveEpaStore<BCH_WEPA_F>;
+using EpaBig = WaveEpaStore<BCH_WEPA_BIG_F>;
 
 // Order LDS traffic between lanes of one wavefront.
 __device__ __forceinline__ void wave_sync() {
@@ -269,14 +282,16 @@ struct SeqPair {
   }
 };
 
-__device__ __forceinline__ V3 lv(const EpaLds& S, int i) {
+template <class St>
+__device__ __forceinline__ V3 lv(const St& S, int i) {
   float4 v = S.v[i];
   return V3{v.x, v.y, v.z};
 }
 
 // epa.rs:212-224: index of the first face with the smallest |distance|
 // (NaN distances are skipped; a NaN first face is never replaced).
-__device__ __forceinline__ int wave_best(const EpaLds& S, int nfaces, int lane) {
+template <class St>
+__device__ __forceinline__ int wave_best(const St& S, int nfaces, int lane) {
   if (is_nan(S.dist[0])) return 0;
   if (nfaces <= 64) {
     float v = INFINITY;
@@ -302,13 +317,14 @@ __device__ __forceinline__ int wave_best(const EpaLds& S, int nfaces, int lane) 
   return wave_min_i(bv == m ? bi : 0x7fffffff);
 }
 
-__device__ __forceinline__ bool face_vis_lds(const EpaLds& S, int f, V3 np) {
+template <class St>
+__device__ __forceinline__ bool face_vis_lds(const St& S, int f, V3 np) {
   return dot(V3{S.nx[f], S.ny[f], S.nz[f]}, np) - S.pd[f] > 1e-5f;
 }
 
 // The fan face of horizon edge (u, w) and the new vertex (epa.rs:372-386).
-__device__ __forceinline__ void fan_face(EpaLds& S, int d, uint32_t u, uint32_t w, V3 vu, V3 vw, V3 np,
-                                         int nidx) {
+template <class St>
+__device__ __forceinline__ void fan_face(St& S, int d, uint32_t u, uint32_t w, V3 vu, V3 vw, V3 np, int nidx) {
   V3 cr = cross(vw - vu, np - vu);
   float l = len(cr);
   V3 n = cr / l;
@@ -321,10 +337,11 @@ __device__ __forceinline__ void fan_face(EpaLds& S, int d, uint32_t u, uint32_t 
 }
 
 // epa.rs:309-388, general form (any face count).  Returns 1 ok (or nothing
-// visible), 0 degenerate horizon (hull untouched), -1 store overflow (hull
-// untouched).
-__device__ __noinline__ int wave_expand_general(EpaLds& S, int& nfaces, V3 np, int nidx, int lane) {
-  constexpr int R = BCH_WEPA_R;
+// visible), 0 degenerate horizon (hull untouched), -1 store overflow: more
+// than `flimit` (<= St::kF) faces or St::kF horizon edges (hull untouched).
+template <class St>
+__device__ __noinline__ int wave_expand_general(St& S, int& nfaces, V3 np, int nidx, int lane, int flimit) {
+  constexpr int R = St::kF / 64;
   const int nchunks = (nfaces + 63) / 64;
   // Horizon in visible-face order with swap-remove cancellation; entry k of
   // the edge list is held by lane k % 64 in register bl[k / 64].
@@ -402,7 +419,7 @@ __device__ __noinline__ int wave_expand_general(EpaLds& S, int& nfaces, V3 np, i
     bool live = f < nfaces;
     kept += __popcll(__ballot(live && !face_vis_lds(S, live ? f : 0, np)));
   }
-  if (kept + ne > BCH_WEPA_F) return -1;
+  if (kept + ne > flimit) return -1;
   // Commit: stable compaction of the faces that stay (visibility re-evaluated
   // on the same values, as the reference does) ...
   int outp = 0;
@@ -449,8 +466,9 @@ __device__ __noinline__ int wave_expand_general(EpaLds& S, int& nfaces, V3 np, i
 }
 
 // epa.rs:309-388.  Same contract as wave_expand_general.
-__device__ __forceinline__ int wave_expand(EpaLds& S, int& nfaces, V3 np, int nidx, int lane) {
-  if (nfaces > 64) return wave_expand_general(S, nfaces, np, nidx, lane);
+template <class St>
+__device__ __forceinline__ int wave_expand(St& S, int& nfaces, V3 np, int nidx, int lane, int flimit) {
+  if (nfaces > 64) return wave_expand_general(S, nfaces, np, nidx, lane, flimit);
   const bool live = lane < nfaces;
   float fnx = 0.0f, fny = 0.0f, fnz = 0.0f, fd = 0.0f, fpd = 0.0f;
   uint32_t fabc = 0u;
@@ -466,7 +484,7 @@ __device__ __forceinline__ int wave_expand(EpaLds& S, int& nfaces, V3 np, int ni
   const uint64_t vm = __ballot(vis);
   if (!vm) return 1;
   const int nv = __popcll(vm);
-  if (nv > 21) return wave_expand_general(S, nfaces, np, nidx, lane);
+  if (nv > 21) return wave_expand_general(S, nfaces, np, nidx, lane, flimit);
   // Edge events: lane l < 3 nv holds edge l % 3 of the visible face of rank
   // l / 3, in the reference's visiting order.
   const int nev = 3 * nv;
@@ -497,7 +515,7 @@ __device__ __forceinline__ int wave_expand(EpaLds& S, int& nfaces, V3 np, int ni
     if (ek == rev && k < lane) match = k;
     if (ek == edge && k != lane) dup = true;
   }
-  if (__ballot(is_ev && dup)) return wave_expand_general(S, nfaces, np, nidx, lane);
+  if (__ballot(is_ev && dup)) return wave_expand_general(S, nfaces, np, nidx, lane, flimit);
   // Replay of the pushes and swap-removes: lane p of `list` holds the event
   // at list position p, lane v of `pos` the position of event v.
   const uint64_t cancel = __ballot(is_ev && match >= 0);
@@ -528,9 +546,10 @@ __device__ __forceinline__ int wave_expand(EpaLds& S, int& nfaces, V3 np, int ni
   }
   if (__ballot(bad)) return 0;
   // Commit: stable compaction of the faces that stay, then the fan (at most
-  // 64 + 63 faces, within the store).
+  // 64 + 63 faces, within the store unless the caller set a lower limit).
   const uint64_t keepm = __ballot(live && !vis);
   const int kept = __popcll(keepm);
+  if (kept + ne > flimit) return -1;
   wave_sync();
   if (live && !vis) {
     int d = __popcll(keepm & lanes_below(lane));
@@ -550,8 +569,9 @@ __device__ __forceinline__ int wave_expand(EpaLds& S, int& nfaces, V3 np, int ni
 
 // epa.rs:127-205 for one pair; every lane returns the result.  `mink(d,
 // lane)` is the wavefront-uniform Minkowski support.
-template <class Mink>
-__device__ __forceinline__ EpaResult wave_epa(const Mink& mink, V3 t0, V3 t1, V3 t2, V3 t3, EpaLds& S, int lane) {
+template <class Mink, class St>
+__device__ __forceinline__ EpaResult wave_epa(const Mink& mink, V3 t0, V3 t1, V3 t2, V3 t3, St& S, int lane,
+                                              int flimit = St::kF) {
   EpaResult R;
   if (lane < 4) {
     V3 w = sel(lane == 0, t0, sel(lane == 1, t1, sel(lane == 2, t2, t3)));
@@ -623,7 +643,7 @@ __device__ __forceinline__ EpaResult wave_epa(const Mink& mink, V3 t0, V3 t1, V3
     if (lane == 0) S.v[idx] = make_float4(p.x, p.y, p.z, 0.0f);
     ++nverts;
     wave_sync();
-    int e = wave_expand(S, nfaces, p, idx, lane);
+    int e = wave_expand(S, nfaces, p, idx, lane, flimit);
     if (e <= 0) {
       status = e == 0 ? 1 : 4;
       stop = e < 0;
@@ -652,7 +672,7 @@ __device__ __forceinline__ EpaResult wave_epa(const Mink& mink, V3 t0, V3 t1, V3
       if (lane == 0) S.v[nidx] = make_float4(np.x, np.y, np.z, 0.0f);
       ++nverts;
       wave_sync();
-      int e = wave_expand(S, nfaces, np, nidx, lane);
+      int e = wave_expand(S, nfaces, np, nidx, lane, flimit);
       if (e <= 0) {
         status = e == 0 ? 1 : 4;
         break;
@@ -689,12 +709,16 @@ struct WorldEpaSrc {
 // Grid-stride over the EPA queue, one wavefront per entry and one LDS
 // polytope per wavefront; the contact of entry p goes to ccount[p] and
 // cslots[stride * p ..] (convex_convex.rs:117-154: margins added back,
-// witness = supportA(n) - n depth).  `route` (may be NULL) gets bit 3.
-template <class Src>
+// witness = supportA(n) - n depth).  `route` (may be NULL) gets bit 3.  An
+// entry whose polytope outgrows `flimit` faces is appended to `over_q`
+// (counted in cnt->n_epa_over, its outputs left to the rerun) when that
+// queue is given, and flags errors bit 1 otherwise.
+template <class Src, class St>
 __device__ __forceinline__ void epa_queue_run(const uint32_t* __restrict__ n_queue, const float4* __restrict__ epa_q,
                                               const Src& src, uint32_t* __restrict__ ccount,
                                               float4* __restrict__ cslots, uint32_t stride,
-                                              uint32_t* __restrict__ route, Counters* cnt, EpaLds& S) {
+                                              uint32_t* __restrict__ route, Counters* cnt, St& S,
+                                              float4* __restrict__ over_q = nullptr, int flimit = St::kF) {
   const int lane = threadIdx.x & 63;
   const uint32_t nq = *n_queue;
   for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
@@ -708,7 +732,7 @@ __device__ __forceinline__ void epa_queue_run(const uint32_t* __restrict__ n_que
     if (SplitPair::fits(pr)) {
       SplitPair sp;
       sp.load(pr, lane);
-      e = wave_epa(sp, t0, t1, t2, t3, S, lane);
+      e = wave_epa(sp, t0, t1, t2, t3, S, lane, flimit);
       if (!(e.depth <= 0.0f)) point_a = sp.support_a(e.n, lane);
     } else {
       EpaPair ep;
@@ -716,12 +740,23 @@ __device__ __forceinline__ void epa_queue_run(const uint32_t* __restrict__ n_que
       ep.ha.load(pr.a, lane);
       ep.hb.load(pr.b, lane);
       SeqPair sq{&ep};
-      e = wave_epa(sq, t0, t1, t2, t3, S, lane);
+      e = wave_epa(sq, t0, t1, t2, t3, S, lane, flimit);
       if (!(e.depth <= 0.0f)) point_a = sq.support_a(e.n, lane);
     }
     bool hit = !(e.depth <= 0.0f);
     float depth = 0.0f;
     if (hit) depth = (e.depth + pr.a.margin) + pr.b.margin;
+    if (e.status == 4 && over_q) {
+      if (lane == 0) {
+        uint32_t k = atomicAdd(&cnt->n_epa_over, 1u);
+        over_q[4 * (uint64_t)k] = a;
+        over_q[4 * (uint64_t)k + 1] = b;
+        over_q[4 * (uint64_t)k + 2] = c;
+        over_q[4 * (uint64_t)k + 3] = d;
+      }
+      wave_sync();
+      continue;
+    }
     if (lane == 0) {
       if (e.status == 4) atomicOr(&cnt->errors, 2u);
       ccount[p] = hit ? 1u : 0u;
@@ -736,13 +771,27 @@ __device__ __forceinline__ void epa_queue_run(const uint32_t* __restrict__ n_que
   }
 }
 
+// `over_q` receives the entries that outgrow `flimit` faces (256 unless a
+// test lowers it) for k_epa_big.
 __global__ __launch_bounds__(64) void k_epa_wave(const uint32_t* __restrict__ n_queue,
                                                  const float4* __restrict__ epa_q, const uint32_t* __restrict__ pairs,
                                                  const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
                                                  uint32_t* __restrict__ ccount, float4* __restrict__ cslots,
-                                                 uint32_t* __restrict__ route, Counters* cnt) {
+                                                 uint32_t* __restrict__ route, Counters* cnt,
+                                                 float4* __restrict__ over_q, int flimit) {
   __shared__ EpaLds S;
-  epa_queue_run(n_queue, epa_q, WorldEpaSrc{pairs, bodies, hull}, ccount, cslots, 8u, route, cnt, S);
+  epa_queue_run(n_queue, epa_q, WorldEpaSrc{pairs, bodies, hull}, ccount, cslots, 8u, route, cnt, S, over_q, flimit);
+}
+
+// The overflow queue of k_epa_wave, rerun from the start with the 2048-face
+// store; an entry that outgrows even that flags errors bit 1.
+__global__ __launch_bounds__(64) void k_epa_big(const uint32_t* __restrict__ n_over,
+                                                const float4* __restrict__ over_q, const uint32_t* __restrict__ pairs,
+                                                const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
+                                                uint32_t* __restrict__ ccount, float4* __restrict__ cslots,
+                                                uint32_t* __restrict__ route, Counters* cnt) {
+  __shared__ EpaBig S;
+  epa_queue_run(n_over, over_q, WorldEpaSrc{pairs, bodies, hull}, ccount, cslots, 8u, route, cnt, S);
 }
 
 }  // namespace bch

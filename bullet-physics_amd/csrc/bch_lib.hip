@@ -93,6 +93,7 @@ struct PinnedBuf {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kEpaDeferGrid = 16384;  // one-pair EPA blocks for the deferred entries (~4% of the queue)
+constexpr uint32_t kEpaBigGrid = 256;      // blocks for the entries that outgrow the one-pair store
 inline uint32_t blocks(uint64_t n, int b = kBlock) { return (uint32_t)((n + b - 1) / b); }
 
 // Working memory of bch_collide_batch_static, the entry point that allocates
@@ -171,6 +172,7 @@ struct bch_context {
   int hull_lanes = 16;      // lanes per hull GJK pair (4, 8 or 16)
   uint32_t epa_grid = 32768; // EPA blocks (one wavefront each)
   uint32_t np_grid = 2048;  // narrowphase blocks per shape class
+  int epa_wave_faces = 256;  // face store the one-pair EPA may fill before the overflow route
   uint32_t trav_block = 256;  // threads per tree-walk block (64, 128, 256, 512)
   uint64_t tree_cap = 0;      // pair buffer of the eager pass, sized by earlier passes
   uint32_t walk_R = 1024;     // pair slots of one tree-walk wavefront's region
@@ -634,7 +636,13 @@ int narrow_finish(bch_context* c, const float* d_seeds, bool dense = true) {
                        c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(),
                        c->epa_q2.as<float4>(), cnt);
     mark(c, 9);
+    // its entries that outgrow 256 faces go to epa_q (consumed by now) and
+    // rerun with a 2048-face store
     hipLaunchKernelGGL(k_epa_wave, dim3(kEpaDeferGrid), dim3(64), 0, s, &cnt->n_epa_defer, c->epa_q2.as<float4>(),
+                       c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
+                       c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(), cnt,
+                       c->epa_q.as<float4>(), c->epa_wave_faces);
+    hipLaunchKernelGGL(k_epa_big, dim3(kEpaBigGrid), dim3(64), 0, s, &cnt->n_epa_over, c->epa_q.as<float4>(),
                        c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                        c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(), cnt);
     mark(c, 7);
@@ -1152,6 +1160,22 @@ int bch_last_timings(bch_context* c, float* out_ms) {
   return BCH_OK;
 }
 
+int bch_last_epa_routes(bch_context* c, uint32_t* out3) {
+  if (!c || !out3) return BCH_INVALID_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  if (!c->counters.p) {
+    out3[0] = out3[1] = out3[2] = 0;
+    return BCH_OK;
+  }
+  Counters h;
+  HIP_TRY(c, hipMemcpyAsync(&h, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  out3[0] = h.n_epa;
+  out3[1] = h.n_epa_defer;
+  out3[2] = h.n_epa_over;
+  return BCH_OK;
+}
+
 int bch_set_stream(bch_context* c, void* stream) {
   if (!c) return BCH_INVALID_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
@@ -1187,6 +1211,10 @@ int bch_set_tuning(bch_context* c, int knob, int value) {
     case BCH_TUNE_NP_GRID:
       if (value < 1 || value > (1 << 20)) return fail(c, BCH_INVALID_ARG, "narrowphase grid must be in [1, 2^20]");
       c->np_grid = (uint32_t)value;
+      return BCH_OK;
+    case BCH_TUNE_EPA_WAVE_FACES:
+      if (value < 16 || value > 256) return fail(c, BCH_INVALID_ARG, "one-pair EPA face store must be in [16, 256]");
+      c->epa_wave_faces = value;
       return BCH_OK;
     default:
       return fail(c, BCH_INVALID_ARG, "unknown tuning knob");
@@ -1274,6 +1302,7 @@ int bch_get_tuning(bch_context* c, int knob, int* value) {
     case BCH_TUNE_TRAVERSE_BLOCK: *value = (int)c->trav_block; return BCH_OK;
     case BCH_TUNE_EPA_GRID: *value = (int)c->epa_grid; return BCH_OK;
     case BCH_TUNE_NP_GRID: *value = (int)c->np_grid; return BCH_OK;
+    case BCH_TUNE_EPA_WAVE_FACES: *value = c->epa_wave_faces; return BCH_OK;
     default: return fail(c, BCH_INVALID_ARG, "unknown tuning knob");
   }
 }
@@ -1296,6 +1325,7 @@ static int batch_ctx(bch_context* c, bch_context** out) {
   a->hull_lanes = c->hull_lanes;
   a->epa_grid = c->epa_grid;
   a->np_grid = c->np_grid;
+  a->epa_wave_faces = c->epa_wave_faces;
   *out = a;
   return BCH_OK;
 }
@@ -2099,6 +2129,9 @@ static int collide_mesh_batch_on(bch_context* c, const bch_bodies* bodies, const
                        cnt);
     hipLaunchKernelGGL(k_epa_mesh, dim3(kEpaDeferGrid), dim3(64), 0, s, &cnt->n_epa_defer,
                        c->mx_epa_q2.as<float4>(), src, c->mx_sub_cnt.as<uint32_t>(), c->mx_sub_slots.as<float4>(),
+                       cnt, c->mx_epa_q.as<float4>(), c->epa_wave_faces);
+    hipLaunchKernelGGL(k_epa_mesh_big, dim3(kEpaBigGrid), dim3(64), 0, s, &cnt->n_epa_over,
+                       c->mx_epa_q.as<float4>(), src, c->mx_sub_cnt.as<uint32_t>(), c->mx_sub_slots.as<float4>(),
                        cnt);
   }
   hipLaunchKernelGGL(k_mesh_reduce, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, c->mx_offs.as<uint32_t>(),
@@ -2466,6 +2499,7 @@ int bch_collide_batch_static(bch_context* c, const bch_bodies* d_bodies, const u
   a->hull_lanes = c->hull_lanes;
   a->epa_grid = c->epa_grid;
   a->np_grid = c->np_grid;
+  a->epa_wave_faces = c->epa_wave_faces;
   a->cap_mode = false;
   hipStream_t s = a->stream;
   Counters* cnt = a->counters.as<Counters>();

@@ -298,9 +298,18 @@ struct MeshEpaSrc {
 __global__ __launch_bounds__(64) void k_epa_mesh(const uint32_t* __restrict__ n_queue,
                                                  const float4* __restrict__ epa_q, MeshEpaSrc src,
                                                  uint32_t* __restrict__ sub_cnt, float4* __restrict__ sub_slots,
-                                                 Counters* cnt) {
+                                                 Counters* cnt, float4* __restrict__ over_q, int flimit) {
   __shared__ EpaLds S;
-  epa_queue_run(n_queue, epa_q, src, sub_cnt, sub_slots, 2u, (uint32_t*)nullptr, cnt, S);
+  epa_queue_run(n_queue, epa_q, src, sub_cnt, sub_slots, 2u, (uint32_t*)nullptr, cnt, S, over_q, flimit);
+}
+
+// k_epa_mesh's overflow queue with the 2048-face store (as k_epa_big).
+__global__ __launch_bounds__(64) void k_epa_mesh_big(const uint32_t* __restrict__ n_over,
+                                                     const float4* __restrict__ over_q, MeshEpaSrc src,
+                                                     uint32_t* __restrict__ sub_cnt, float4* __restrict__ sub_slots,
+                                                     Counters* cnt) {
+  __shared__ EpaBig S;
+  epa_queue_run(n_over, over_q, src, sub_cnt, sub_slots, 2u, (uint32_t*)nullptr, cnt, S);
 }
 
 // The same queue two sub-pairs per wavefront (bch_epa_dual.hpp); entries

@@ -83,17 +83,30 @@ int bch_synchronize(bch_context* ctx);
  * k_epa_dual alone (part of [6]).  out_ms holds 9 floats. */
 int bch_last_timings(bch_context* ctx, float* out_ms);
 
+/* No reference counterpart: how the EPA entries of the last world pass on
+ * this context were routed, read from the device (waits for the context
+ * stream): out3[0] entries queued (pairs whose cores overlap), [1] entries
+ * the two-pair kernel handed to the one-pair kernel, [2] entries whose
+ * polytope outgrew the one-pair kernel's store and reran with the
+ * 2048-face store.  Results never depend on the route. */
+int bch_last_epa_routes(bch_context* ctx, uint32_t* out3);
+
 /* No reference counterpart: launch-shape knobs of the device passes.  They
  * change how work is spread over the GPU, never the results.
  *   BCH_TUNE_HULL_LANES  lanes per GJK pair with a hull: 4, 8, 16 (16)
  *   BCH_TUNE_TRAVERSE_BLOCK  threads per tree-walk block: 64, 128, 256, 512 (256)
  *   BCH_TUNE_EPA_GRID    EPA blocks of one wavefront (32768)
- *   BCH_TUNE_NP_GRID     narrowphase blocks per shape class (2048) */
+ *   BCH_TUNE_NP_GRID     narrowphase blocks per shape class (2048)
+ *   BCH_TUNE_EPA_WAVE_FACES  faces the one-pair EPA keeps before it hands an
+ *                        entry to the 2048-face overflow kernel: 16..256
+ *                        (256).  Results do not depend on it; lowering it
+ *                        only exercises the overflow route (tests). */
 enum {
   BCH_TUNE_HULL_LANES = 1,
   BCH_TUNE_TRAVERSE_BLOCK = 2,
   BCH_TUNE_EPA_GRID = 3,
-  BCH_TUNE_NP_GRID = 4
+  BCH_TUNE_NP_GRID = 4,
+  BCH_TUNE_EPA_WAVE_FACES = 5
 };
 int bch_set_tuning(bch_context* ctx, int knob, int value);
 int bch_get_tuning(bch_context* ctx, int knob, int* value);

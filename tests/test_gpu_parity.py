@@ -132,6 +132,35 @@ def test_large_hull_pass_defers_epa_to_the_one_pair_kernel(bpa, scenes, oracle, 
     compare_contacts(ref_cnt, ref_pts, offs, cont, label="large_hulls")
 
 
+def test_epa_store_overflow_reruns_with_the_big_store(bpa, scenes, oracle, ctx):
+    """A polytope that outgrows the one-pair kernel's store goes to the
+    2048-face kernel instead of failing the pass.  No natural input is known
+    to do that (a convex polytope of <= 128 vertices has <= 252 faces), so the
+    one-pair store is cut to 16 faces here: nearly every deferred entry then
+    overflows, and the pass must equal the default one bit for bit, and the
+    oracle."""
+    _flush("case: EPA overflow route, hulls of 65-160 points, one-pair store cut to 16 faces")
+    sc = scenes.mixed_convex(4000, 21, mix=(0.25, 0.15, 0.1, 0.5), hull_pts=(65, 160))
+    w = bpa.CollisionWorld(sc, ctx)
+    w.collide(MARGIN)
+    ref = w.results()
+    n_q, n_def, n_over = w.epa_routes()
+    assert n_def > 20 and n_over == 0
+    try:
+        ctx.set_tuning(epa_wave_faces=16)
+        w.collide(MARGIN)
+        got = w.results()
+        routes = w.epa_routes()
+    finally:
+        ctx.set_tuning(epa_wave_faces=256)
+    _flush(f"EPA entries {n_q}, deferred {n_def}; with 16 faces: {routes[2]} through the 2048-face store")
+    assert routes[:2] == (n_q, n_def) and routes[2] >= 0.9 * n_def
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+    ref_pairs, ref_cnt, ref_pts, _ = _oracle_pass(oracle, sc, MARGIN, "sweep")
+    compare_contacts(ref_cnt, ref_pts, got[1], got[2], label="overflow route")
+
+
 def test_collide_batch_matches_oracle(bpa, scenes, oracle, ctx):
     """bch_collide_batch on caller-given pairs (unsorted, with repeats)."""
     _flush("case: collide_batch mixed 5000")
