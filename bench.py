@@ -44,11 +44,11 @@ def algorithmic_bytes(sc, pairs, offs, routes, kernel):
     k_np_hull   per pair with a hull: pair index 4 + pair 8 + count 4 + its
                 band contacts 28 each, + 64 B queue entry for a pair sent on
                 to EPA; per distinct body of those pairs: 44 + its vertices.
-    k_epa_dual  per queued pair: queue entry 64 + pair 8 + count 4 + route 8
+    k_epa_quad  per queued pair: queue entry 64 + pair 8 + count 4 + route 8
                 (read, write) + 28 per contact; per distinct body of those
-                pairs: 44 + its vertices (the stage includes the one-pair
-                k_epa_wave launch over the few entries the two-pair kernel
-                defers).
+                pairs: 44 + its vertices (the ~2 % of entries it hands on to
+                the two-pair and one-pair kernels are counted as if finished
+                there: a 64-B queue entry written instead of the outputs).
     pass        the whole pass: N bodies x 44 + 12 x every hull vertex +
                 P x (8 pair + 4 count) + 28 per contact.
     """
@@ -67,7 +67,7 @@ def algorithmic_bytes(sc, pairs, offs, routes, kernel):
     if kernel == "k_np_hull":
         m = cls == 6
         return int(m.sum() * (4 + 8 + 4) + 28 * ncon[m & ~epa].sum() + 64 * (m & epa).sum()) + bodies(m)
-    if kernel == "k_epa_dual":
+    if kernel == "k_epa_quad":
         return int(epa.sum() * (64 + 8 + 4 + 8) + 28 * ncon[epa].sum()) + bodies(epa)
     if kernel == "pass":
         return int(sc["shape_type"].shape[0] * 44 + 12 * hc.sum() + P * 12 + 28 * ncon.sum())
@@ -692,19 +692,19 @@ def main():
         P, Cn = w.collide(args.margin)
         log(f"warmup {k}: {P} pairs, {Cn} contacts")
     stage_ms = np.zeros(len(STAGES))
-    epa_dual_ms = 0.0
+    epa_quad_ms = 0.0
     w.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         P, Cn = w.collide(args.margin)
         t_all = w.timings_all()
         stage_ms += t_all[:8]
-        epa_dual_ms += float(t_all[8])
+        epa_quad_ms += float(t_all[8])
     w.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     stage_ms /= args.steps
-    epa_dual_ms /= args.steps
+    epa_quad_ms /= args.steps
     log("stages (ms): " + ", ".join(f"{s} {v:.3f}" for s, v in zip(STAGES, stage_ms)))
     pairs, offs, cont = w.results()
     value = P / (ms * 1e-3)
@@ -730,8 +730,8 @@ def main():
     routes = w.routes()
     dom = STAGES[int(np.argmax(stage_ms))]
     # single-kernel times from the pass's own HIP events: the traversal stage
-    # is k_traverse alone, and event 9 ends k_epa_dual inside the EPA stage
-    kernel_ms = {"k_traverse": float(stage_ms[STAGES.index("tree_traverse")]), "k_epa_dual": epa_dual_ms}
+    # is k_traverse alone, and event 9 ends k_epa_quad inside the EPA stage
+    kernel_ms = {"k_traverse": float(stage_ms[STAGES.index("tree_traverse")]), "k_epa_quad": epa_quad_ms}
     kern = max(kernel_ms, key=kernel_ms.get)
     ab = algorithmic_bytes(sc, pairs, offs, routes, kern)
     dur = kernel_ms[kern] * 1e-3
@@ -762,7 +762,7 @@ def main():
         roof["valu"] = {"instructions": vi, "achieved_per_simd_cycle": round(rate, 4),
                         "peak_per_simd_cycle": ceil, "frac": round(rate / ceil, 3) if ceil else None,
                         "note": "instructions from the SQ counters of the same kernel and workload over this "
-                                "run's k_epa_dual time; the peak is the rate the FMA probe sustains "
+                                "run's k_epa_quad time; the peak is the rate the FMA probe sustains "
                                 "(bench.py --valu-calibration), both per SIMD-cycle at 2.4 GHz"}
 
     dropin = None

@@ -592,9 +592,10 @@ class CollisionWorld:
         return out
 
     def epa_routes(self):
-        """EPA entries of the last pass: (queued, handed to the one-pair
-        kernel, rerun with the 2048-face store)."""
-        out = np.zeros(3, np.uint32)
+        """EPA entries of the last pass: (queued for the four-pair kernel,
+        handed to the two-pair kernel, handed to the one-pair kernel, rerun
+        with the 2048-face store)."""
+        out = np.zeros(4, np.uint32)
         _check(self.ctx.h, self.L.bch_last_epa_routes(self.ctx.h, out.ctypes.data))
         return tuple(int(x) for x in out)
 

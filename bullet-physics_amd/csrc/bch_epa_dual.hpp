@@ -505,14 +505,13 @@ __device__ __forceinline__ void dual_epa(const DualShapes& ds, V3 t0, V3 t1, V3 
 
 // Grid-stride over an EPA queue, two entries per wavefront (Src as in
 // epa_queue_run).  Outputs as epa_queue_run; an entry that leaves the fast
-// path is appended to defer_q (count in cnt->n_epa_defer) for the one-pair
-// kernel.
+// path is appended to defer_q (count in *n_defer) for the one-pair kernel.
 template <class Src>
 __device__ __forceinline__ void epa_dual_run(const uint32_t* __restrict__ n_queue, const float4* __restrict__ epa_q,
                                              const Src& src, uint32_t* __restrict__ ccount,
                                              float4* __restrict__ cslots, uint32_t stride,
                                              uint32_t* __restrict__ route, float4* __restrict__ defer_q,
-                                             Counters* cnt, EpaHalf* S) {
+                                             uint32_t* n_defer, Counters* cnt, EpaHalf* S) {
   const int lane = threadIdx.x & 63;
   const int h = lane >> 5;
   const int hl = lane & 31;
@@ -539,7 +538,7 @@ __device__ __forceinline__ void epa_dual_run(const uint32_t* __restrict__ n_queu
     if (hit) depth = (e.depth + pr.a.margin) + pr.b.margin;
     if (hl == 0 && has) {
       if (deferred) {
-        uint32_t slot = atomicAdd(&cnt->n_epa_defer, 1u);
+        uint32_t slot = atomicAdd(n_defer, 1u);
         defer_q[4 * (uint64_t)slot] = a;
         defer_q[4 * (uint64_t)slot + 1] = b;
         defer_q[4 * (uint64_t)slot + 2] = c;
@@ -567,15 +566,17 @@ __device__ __forceinline__ void epa_dual_run(const uint32_t* __restrict__ n_queu
 #define BCH_EPA_WAVES 4
 #endif
 
-// The world pass's EPA queue (pairs of the pass, 8 contact slots per pair).
+// An EPA queue of the world pass (pairs of the pass, 8 contact slots per
+// pair); the entries that leave the fast path go to defer_q (count in
+// *n_defer) for k_epa_wave.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BCH_EPA_WAVES))) void k_epa_dual(const uint32_t* __restrict__ n_queue,
                                                  const float4* __restrict__ epa_q, const uint32_t* __restrict__ pairs,
                                                  const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
                                                  uint32_t* __restrict__ ccount, float4* __restrict__ cslots,
                                                  uint32_t* __restrict__ route, float4* __restrict__ defer_q,
-                                                 Counters* cnt) {
+                                                 uint32_t* n_defer, Counters* cnt) {
   __shared__ EpaHalf S[2];
-  epa_dual_run(n_queue, epa_q, WorldEpaSrc{pairs, bodies, hull}, ccount, cslots, 8u, route, defer_q, cnt, S);
+  epa_dual_run(n_queue, epa_q, WorldEpaSrc{pairs, bodies, hull}, ccount, cslots, 8u, route, defer_q, n_defer, cnt, S);
 }
 
 }  // namespace bch

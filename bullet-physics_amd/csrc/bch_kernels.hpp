@@ -42,7 +42,9 @@ struct Counters {
   uint32_t n_pairs_total;  // pairs found
   uint32_t n_contacts;     // contacts found
   uint32_t n_pen;          // penetrating pairs (de-penetration step)
-  uint32_t n_epa_defer;    // EPA entries handed from the two-pair kernel to the one-pair kernel
+  uint32_t n_epa_defer;    // EPA entries the four-pair kernel hands to the two-pair kernel (mesh batch:
+                           // the two-pair kernel to the one-pair kernel)
+  uint32_t n_epa_defer2;   // entries the two-pair kernel hands to the one-pair kernel (world pass)
   uint32_t n_epa_over;     // entries whose polytope outgrew the one-pair kernel's store (rerun by k_epa_big)
 };
 
@@ -75,6 +77,7 @@ __global__ void k_reset(Counters* c) {
     c->n_contacts = 0;
     c->n_pen = 0;
     c->n_epa_defer = 0;
+    c->n_epa_defer2 = 0;
     c->n_epa_over = 0;
   }
 }
@@ -1133,6 +1136,7 @@ __global__ void k_reset_epa(Counters* c) {
   if (threadIdx.x == 0) {
     c->n_epa = 0;
     c->n_epa_defer = 0;
+    c->n_epa_defer2 = 0;
     c->n_epa_over = 0;
     c->errors &= ~2u;
     for (int k = 0; k < 8; ++k) c->cls_start[k] = c->cls_end[k] = 0u;

@@ -16,6 +16,7 @@
 #include "bch_kernels.hpp"
 #include "bch_epa_wave.hpp"
 #include "bch_epa_dual.hpp"
+#include "bch_epa_quad.hpp"
 #include "bch_adjoint.hpp"
 #include "bch_sort.hpp"
 #include "bch_depen.hpp"
@@ -629,20 +630,24 @@ int narrow_finish(bch_context* c, const float* d_seeds, bool dense = true) {
     HIP_TRY(c, hipEventRecord(c->join_ev, c->side));
     HIP_TRY(c, hipStreamWaitEvent(s, c->join_ev, 0));
     mark(c, 6);
-    // two pairs per wavefront; the entries that leave its fast path run
-    // through the one-pair kernel from the start
-    hipLaunchKernelGGL(k_epa_dual, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
+    // four pairs per wavefront; an entry outside its budget reruns from the
+    // start two pairs per wavefront, then one pair per wavefront, then with
+    // the 2048-face store.  The queues alternate between epa_q and epa_q2,
+    // each consumed by the kernel before the one that refills it.
+    hipLaunchKernelGGL(k_epa_quad, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
                        c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                        c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(),
-                       c->epa_q2.as<float4>(), cnt);
+                       c->epa_q2.as<float4>(), &cnt->n_epa_defer, cnt);
     mark(c, 9);
-    // its entries that outgrow 256 faces go to epa_q (consumed by now) and
-    // rerun with a 2048-face store
-    hipLaunchKernelGGL(k_epa_wave, dim3(kEpaDeferGrid), dim3(64), 0, s, &cnt->n_epa_defer, c->epa_q2.as<float4>(),
+    hipLaunchKernelGGL(k_epa_dual, dim3(kEpaDeferGrid), dim3(64), 0, s, &cnt->n_epa_defer, c->epa_q2.as<float4>(),
+                       c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
+                       c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(),
+                       c->epa_q.as<float4>(), &cnt->n_epa_defer2, cnt);
+    hipLaunchKernelGGL(k_epa_wave, dim3(kEpaDeferGrid), dim3(64), 0, s, &cnt->n_epa_defer2, c->epa_q.as<float4>(),
                        c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                        c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(), cnt,
-                       c->epa_q.as<float4>(), c->epa_wave_faces);
-    hipLaunchKernelGGL(k_epa_big, dim3(kEpaBigGrid), dim3(64), 0, s, &cnt->n_epa_over, c->epa_q.as<float4>(),
+                       c->epa_q2.as<float4>(), c->epa_wave_faces);
+    hipLaunchKernelGGL(k_epa_big, dim3(kEpaBigGrid), dim3(64), 0, s, &cnt->n_epa_over, c->epa_q2.as<float4>(),
                        c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                        c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(), cnt);
     mark(c, 7);
@@ -1160,19 +1165,20 @@ int bch_last_timings(bch_context* c, float* out_ms) {
   return BCH_OK;
 }
 
-int bch_last_epa_routes(bch_context* c, uint32_t* out3) {
-  if (!c || !out3) return BCH_INVALID_ARG;
+int bch_last_epa_routes(bch_context* c, uint32_t* out4) {
+  if (!c || !out4) return BCH_INVALID_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
   if (!c->counters.p) {
-    out3[0] = out3[1] = out3[2] = 0;
+    out4[0] = out4[1] = out4[2] = out4[3] = 0;
     return BCH_OK;
   }
   Counters h;
   HIP_TRY(c, hipMemcpyAsync(&h, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  out3[0] = h.n_epa;
-  out3[1] = h.n_epa_defer;
-  out3[2] = h.n_epa_over;
+  out4[0] = h.n_epa;
+  out4[1] = h.n_epa_defer;
+  out4[2] = h.n_epa_defer2;
+  out4[3] = h.n_epa_over;
   return BCH_OK;
 }
 

@@ -319,7 +319,7 @@ __global__ __launch_bounds__(64) void k_epa_mesh_dual(const uint32_t* __restrict
                                                       uint32_t* __restrict__ sub_cnt, float4* __restrict__ sub_slots,
                                                       float4* __restrict__ defer_q, Counters* cnt) {
   __shared__ EpaHalf S[2];
-  epa_dual_run(n_queue, epa_q, src, sub_cnt, sub_slots, 2u, (uint32_t*)nullptr, defer_q, cnt, S);
+  epa_dual_run(n_queue, epa_q, src, sub_cnt, sub_slots, 2u, (uint32_t*)nullptr, defer_q, &cnt->n_epa_defer, cnt, S);
 }
 
 // ContactManifold::reduce (manifold.rs:76-110) over the pair's sub-pair
