@@ -693,6 +693,7 @@ def main():
         log(f"warmup {k}: {P} pairs, {Cn} contacts")
     stage_ms = np.zeros(len(STAGES))
     epa_quad_ms = 0.0
+    np_hull_ms = 0.0
     w.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -700,11 +701,13 @@ def main():
         t_all = w.timings_all()
         stage_ms += t_all[:8]
         epa_quad_ms += float(t_all[8])
+        np_hull_ms += float(t_all[9])
     w.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     stage_ms /= args.steps
     epa_quad_ms /= args.steps
+    np_hull_ms /= args.steps
     log("stages (ms): " + ", ".join(f"{s} {v:.3f}" for s, v in zip(STAGES, stage_ms)))
     pairs, offs, cont = w.results()
     value = P / (ms * 1e-3)
@@ -730,8 +733,11 @@ def main():
     routes = w.routes()
     dom = STAGES[int(np.argmax(stage_ms))]
     # single-kernel times from the pass's own HIP events: the traversal stage
-    # is k_traverse alone, and event 9 ends k_epa_quad inside the EPA stage
-    kernel_ms = {"k_traverse": float(stage_ms[STAGES.index("tree_traverse")]), "k_epa_quad": epa_quad_ms}
+    # is k_traverse alone, event 9 ends k_epa_quad inside the EPA stage and
+    # event 10 ends k_np_hull (on the pass's stream, beside the side stream's
+    # k_np launches) inside the narrowphase stage
+    kernel_ms = {"k_traverse": float(stage_ms[STAGES.index("tree_traverse")]), "k_epa_quad": epa_quad_ms,
+                 "k_np_hull": np_hull_ms}
     kern = max(kernel_ms, key=kernel_ms.get)
     ab = algorithmic_bytes(sc, pairs, offs, routes, kern)
     dur = kernel_ms[kern] * 1e-3
@@ -762,7 +768,7 @@ def main():
         roof["valu"] = {"instructions": vi, "achieved_per_simd_cycle": round(rate, 4),
                         "peak_per_simd_cycle": ceil, "frac": round(rate / ceil, 3) if ceil else None,
                         "note": "instructions from the SQ counters of the same kernel and workload over this "
-                                "run's k_epa_quad time; the peak is the rate the FMA probe sustains "
+                                "run's own event time of that kernel; the peak is the rate the FMA probe sustains "
                                 "(bench.py --valu-calibration), both per SIMD-cycle at 2.4 GHz"}
 
     dropin = None

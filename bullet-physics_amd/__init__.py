@@ -586,16 +586,16 @@ class CollisionWorld:
         return self.timings_all()[:8]
 
     def timings_all(self):
-        """The eight stage times, then the two-pair EPA kernel alone (part of "epa")."""
-        out = np.zeros(9, np.float32)
+        """The eight stage times, then the four-pair EPA kernel alone (part
+        of "epa") and the hull GJK kernel alone (part of "narrow_gjk")."""
+        out = np.zeros(10, np.float32)
         _check(self.ctx.h, self.L.bch_last_timings(self.ctx.h, out.ctypes.data))
         return out
 
     def epa_routes(self):
         """EPA entries of the last pass: (queued for the four-pair kernel,
-        handed to the two-pair kernel, handed to the one-pair kernel, rerun
-        with the 2048-face store)."""
-        out = np.zeros(4, np.uint32)
+        handed to the one-pair kernel, rerun with the 2048-face store)."""
+        out = np.zeros(3, np.uint32)
         _check(self.ctx.h, self.L.bch_last_epa_routes(self.ctx.h, out.ctypes.data))
         return tuple(int(x) for x in out)
 

@@ -1,4 +1,6 @@
-// bch_epa_dual.hpp -- EPA for two pairs per wavefront, one per 32-lane half.
+// bch_epa_dual.hpp -- EPA for two pairs per wavefront, one per 32-lane half
+// (the mesh batch's sub-pairs; the world pass runs four pairs per wavefront,
+// bch_epa_quad.hpp, whose helpers build on these).
 //
 // The polytopes of the world pass are small: the closest-face search and
 // the expansions see at most 64 faces for all but a few pairs in 10^5, and
@@ -556,27 +558,6 @@ __device__ __forceinline__ void epa_dual_run(const uint32_t* __restrict__ n_queu
     }
     wave_sync();
   }
-}
-
-// Waves per SIMD the two-pair EPA is compiled for (its register budget).
-// At 5 (96 VGPRs, 88 B spilled per lane) the 1M-body mixed scene ran the
-// kernel 2 % faster (1.547 against 1.582 ms), but the spills raised its HBM
-// traffic from 0.25 to 0.91 GB per launch, so 4 (no spill) is the default.
-#ifndef BCH_EPA_WAVES
-#define BCH_EPA_WAVES 4
-#endif
-
-// An EPA queue of the world pass (pairs of the pass, 8 contact slots per
-// pair); the entries that leave the fast path go to defer_q (count in
-// *n_defer) for k_epa_wave.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BCH_EPA_WAVES))) void k_epa_dual(const uint32_t* __restrict__ n_queue,
-                                                 const float4* __restrict__ epa_q, const uint32_t* __restrict__ pairs,
-                                                 const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
-                                                 uint32_t* __restrict__ ccount, float4* __restrict__ cslots,
-                                                 uint32_t* __restrict__ route, float4* __restrict__ defer_q,
-                                                 uint32_t* n_defer, Counters* cnt) {
-  __shared__ EpaHalf S[2];
-  epa_dual_run(n_queue, epa_q, WorldEpaSrc{pairs, bodies, hull}, ccount, cslots, 8u, route, defer_q, n_defer, cnt, S);
 }
 
 }  // namespace bch

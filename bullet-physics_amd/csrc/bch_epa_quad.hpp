@@ -1,8 +1,8 @@
 // bch_epa_quad.hpp -- EPA for four pairs per wavefront, one per 16-lane
 // quarter (one DPP row).
 //
-// Same algorithm, statement order and tie rules as k_epa_wave and k_epa_dual
-// (narrowphase/epa.rs:127-398), on a smaller budget per pair: faces l and
+// Same algorithm, statement order and tie rules as k_epa_wave and the mesh
+// batch's two-pair kernel (narrowphase/epa.rs:127-398), on a smaller budget per pair: faces l and
 // l + 16 of a quarter's polytope in lane l (32 faces), at most 5 visible
 // faces per expansion (15 edge events, one per lane), at most 64 vertices,
 // the shapes' supports on 8 lanes each with up to 8 hull vertices per lane
@@ -14,9 +14,11 @@
 //
 // A pair outside that budget (a hull of more than 64 vertices, more than 32
 // faces, more than 5 visible faces, more than 64 vertices, a directed edge
-// shared by two visible faces) is not finished here: its entry goes to the
-// next queue, which k_epa_dual runs from the start, so every result is the
-// same as the one-pair kernel's.
+// shared by two visible faces) is not finished here: its entry goes to a
+// second queue, which k_epa_wave runs from the start, so every result is the
+// one-pair kernel's.  (Routing these through the two-pair kernel first made
+// the EPA stage 0.1 ms longer on the 1M-body scene: they are the largest
+// polytopes, and two per wavefront ran them more slowly than one.)
 //
 // As in the dual kernel, every cross-lane operation runs with all 64 lanes
 // active; a quarter's state lives in quarter-uniform registers and its
@@ -591,7 +593,7 @@ __device__ __forceinline__ void epa_quad_run(const uint32_t* __restrict__ n_queu
 #endif
 
 // The world pass's EPA queue, four entries per wavefront; the entries that
-// leave the budget go to defer_q for k_epa_dual.
+// leave the budget go to defer_q for k_epa_wave.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BCH_EPA_QUAD_WAVES))) void k_epa_quad(const uint32_t* __restrict__ n_queue,
                                                  const float4* __restrict__ epa_q, const uint32_t* __restrict__ pairs,
                                                  const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,

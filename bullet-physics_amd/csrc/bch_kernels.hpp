@@ -16,8 +16,8 @@
 //   radix sort    (bch_sort.hpp) pair keys i << 32 | j -> lexicographic order
 //   k_np<CLS>     one lane per pair, one kernel per shape class
 //   k_np_hull     a group of lanes per pair with a hull
-//   k_epa_dual    EPA of the deep-overlap pairs, two per wavefront, polytopes
-//                 in LDS (bch_epa_dual.hpp); the few outside its fast path
+//   k_epa_quad    EPA of the deep-overlap pairs, four per wavefront, polytopes
+//                 in LDS (bch_epa_quad.hpp); the few outside its budget
 //                 rerun one per wavefront in k_epa_wave (bch_epa_wave.hpp)
 //   k_gather      per-pair contact slots -> dense CSR contacts
 #pragma once
@@ -42,9 +42,8 @@ struct Counters {
   uint32_t n_pairs_total;  // pairs found
   uint32_t n_contacts;     // contacts found
   uint32_t n_pen;          // penetrating pairs (de-penetration step)
-  uint32_t n_epa_defer;    // EPA entries the four-pair kernel hands to the two-pair kernel (mesh batch:
-                           // the two-pair kernel to the one-pair kernel)
-  uint32_t n_epa_defer2;   // entries the two-pair kernel hands to the one-pair kernel (world pass)
+  uint32_t n_epa_defer;    // EPA entries handed to the one-pair kernel (world pass: by the four-pair
+                           // kernel; mesh batch: by the two-pair kernel)
   uint32_t n_epa_over;     // entries whose polytope outgrew the one-pair kernel's store (rerun by k_epa_big)
 };
 
@@ -77,7 +76,6 @@ __global__ void k_reset(Counters* c) {
     c->n_contacts = 0;
     c->n_pen = 0;
     c->n_epa_defer = 0;
-    c->n_epa_defer2 = 0;
     c->n_epa_over = 0;
   }
 }
@@ -1136,7 +1134,6 @@ __global__ void k_reset_epa(Counters* c) {
   if (threadIdx.x == 0) {
     c->n_epa = 0;
     c->n_epa_defer = 0;
-    c->n_epa_defer2 = 0;
     c->n_epa_over = 0;
     c->errors &= ~2u;
     for (int k = 0; k < 8; ++k) c->cls_start[k] = c->cls_end[k] = 0u;

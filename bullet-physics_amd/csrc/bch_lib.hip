@@ -156,8 +156,8 @@ struct bch_context {
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::string err;
-  hipEvent_t ev[10] = {};  // stage boundaries 0-8; 9 ends the two-pair EPA kernel
-  float timings[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  hipEvent_t ev[11] = {};  // stage boundaries 0-8; 9 ends the four-pair EPA kernel, 10 the hull GJK kernel
+  float timings[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
   // world-resident bodies
   uint32_t n = 0, n_worlds = 1, max_world = 0;
@@ -288,6 +288,7 @@ int fail(bch_context* c, int code, const std::string& msg) {
 void stage_times(bch_context* c) {
   for (int k = 0; k < 8; ++k) (void)hipEventElapsedTime(&c->timings[k], c->ev[k], c->ev[k + 1]);
   (void)hipEventElapsedTime(&c->timings[8], c->ev[6], c->ev[9]);
+  (void)hipEventElapsedTime(&c->timings[9], c->ev[5], c->ev[10]);
 }
 
 // Stage boundary event (not while capturing: the graph is timed as a whole).
@@ -616,6 +617,7 @@ int narrow_finish(bch_context* c, const float* d_seeds, bool dense = true) {
     else
       BCH_NP_HULL(16);
 #undef BCH_NP_HULL
+    mark(c, 10);
 #define BCH_NP(CLS)                                                                                     \
   hipLaunchKernelGGL(k_np<CLS>, dim3(g), dim3(kBlock), 0, c->side, cnt, perm, c->pairs.as<uint32_t>(),   \
                      c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_seeds, c->ccount.as<uint32_t>(),     \
@@ -631,29 +633,26 @@ int narrow_finish(bch_context* c, const float* d_seeds, bool dense = true) {
     HIP_TRY(c, hipStreamWaitEvent(s, c->join_ev, 0));
     mark(c, 6);
     // four pairs per wavefront; an entry outside its budget reruns from the
-    // start two pairs per wavefront, then one pair per wavefront, then with
-    // the 2048-face store.  The queues alternate between epa_q and epa_q2,
-    // each consumed by the kernel before the one that refills it.
+    // start one pair per wavefront (epa_q2, then epa_q: each queue is
+    // consumed by the kernel before the one that refills it), and an entry
+    // that outgrows the one-pair store once more with the 2048-face store
     hipLaunchKernelGGL(k_epa_quad, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
                        c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                        c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(),
                        c->epa_q2.as<float4>(), &cnt->n_epa_defer, cnt);
     mark(c, 9);
-    hipLaunchKernelGGL(k_epa_dual, dim3(kEpaDeferGrid), dim3(64), 0, s, &cnt->n_epa_defer, c->epa_q2.as<float4>(),
-                       c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
-                       c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(),
-                       c->epa_q.as<float4>(), &cnt->n_epa_defer2, cnt);
-    hipLaunchKernelGGL(k_epa_wave, dim3(kEpaDeferGrid), dim3(64), 0, s, &cnt->n_epa_defer2, c->epa_q.as<float4>(),
+    hipLaunchKernelGGL(k_epa_wave, dim3(kEpaDeferGrid), dim3(64), 0, s, &cnt->n_epa_defer, c->epa_q2.as<float4>(),
                        c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                        c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(), cnt,
-                       c->epa_q2.as<float4>(), c->epa_wave_faces);
-    hipLaunchKernelGGL(k_epa_big, dim3(kEpaBigGrid), dim3(64), 0, s, &cnt->n_epa_over, c->epa_q2.as<float4>(),
+                       c->epa_q.as<float4>(), c->epa_wave_faces);
+    hipLaunchKernelGGL(k_epa_big, dim3(kEpaBigGrid), dim3(64), 0, s, &cnt->n_epa_over, c->epa_q.as<float4>(),
                        c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                        c->ccount.as<uint32_t>(), c->cslots.as<float4>(), c->cls.as<uint32_t>(), cnt);
     mark(c, 7);
   } else {
     for (int k = 4; k <= 7; ++k) mark(c, k);
     mark(c, 9);
+    mark(c, 10);
   }
   if (!dense && !c->cap_mode) {
     HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
@@ -1161,24 +1160,23 @@ int bch_synchronize(bch_context* c) {
 
 int bch_last_timings(bch_context* c, float* out_ms) {
   if (!c || !out_ms) return BCH_INVALID_ARG;
-  for (int i = 0; i < 9; ++i) out_ms[i] = c->timings[i];
+  for (int i = 0; i < 10; ++i) out_ms[i] = c->timings[i];
   return BCH_OK;
 }
 
-int bch_last_epa_routes(bch_context* c, uint32_t* out4) {
-  if (!c || !out4) return BCH_INVALID_ARG;
+int bch_last_epa_routes(bch_context* c, uint32_t* out3) {
+  if (!c || !out3) return BCH_INVALID_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
   if (!c->counters.p) {
-    out4[0] = out4[1] = out4[2] = out4[3] = 0;
+    out3[0] = out3[1] = out3[2] = 0;
     return BCH_OK;
   }
   Counters h;
   HIP_TRY(c, hipMemcpyAsync(&h, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  out4[0] = h.n_epa;
-  out4[1] = h.n_epa_defer;
-  out4[2] = h.n_epa_defer2;
-  out4[3] = h.n_epa_over;
+  out3[0] = h.n_epa;
+  out3[1] = h.n_epa_defer;
+  out3[2] = h.n_epa_over;
   return BCH_OK;
 }
 

@@ -80,17 +80,17 @@ int bch_synchronize(bch_context* ctx);
  * stream (near zero), [5] closed-form and GJK pairs, with and without a
  * hull (k_np on the second stream beside k_np_hull), [6] EPA (all its
  * kernels), [7] contact compaction; then [8] the four-pair EPA kernel
- * k_epa_quad alone (part of [6]).  out_ms holds 9 floats. */
+ * k_epa_quad alone (part of [6]) and [9] the hull GJK kernel k_np_hull alone
+ * (part of [5], beside the second stream).  out_ms holds 10 floats. */
 int bch_last_timings(bch_context* ctx, float* out_ms);
 
 /* No reference counterpart: how the EPA entries of the last world pass on
  * this context were routed, read from the device (waits for the context
- * stream): out4[0] entries queued (pairs whose cores overlap) for the
- * four-pair kernel, [1] entries it handed to the two-pair kernel, [2]
- * entries that one handed to the one-pair kernel, [3] entries whose polytope
- * outgrew the one-pair kernel's store and reran with the 2048-face store.
- * Results never depend on the route. */
-int bch_last_epa_routes(bch_context* ctx, uint32_t* out4);
+ * stream): out3[0] entries queued (pairs whose cores overlap) for the
+ * four-pair kernel, [1] entries it handed to the one-pair kernel, [2]
+ * entries whose polytope outgrew the one-pair kernel's store and reran with
+ * the 2048-face store.  Results never depend on the route. */
+int bch_last_epa_routes(bch_context* ctx, uint32_t* out3);
 
 /* No reference counterpart: launch-shape knobs of the device passes.  They
  * change how work is spread over the GPU, never the results.

@@ -144,7 +144,7 @@ def test_epa_store_overflow_reruns_with_the_big_store(bpa, scenes, oracle, ctx):
     w = bpa.CollisionWorld(sc, ctx)
     w.collide(MARGIN)
     ref = w.results()
-    n_q, n_dual, n_def, n_over = w.epa_routes()
+    n_q, n_def, n_over = w.epa_routes()
     assert n_def > 20 and n_over == 0
     try:
         ctx.set_tuning(epa_wave_faces=16)
@@ -153,9 +153,9 @@ def test_epa_store_overflow_reruns_with_the_big_store(bpa, scenes, oracle, ctx):
         routes = w.epa_routes()
     finally:
         ctx.set_tuning(epa_wave_faces=256)
-    _flush(f"EPA entries {n_q}, to the two-pair kernel {n_dual}, to the one-pair kernel {n_def}; with 16 faces: "
-           f"{routes[3]} through the 2048-face store")
-    assert routes[:3] == (n_q, n_dual, n_def) and routes[3] >= 0.9 * n_def
+    _flush(f"EPA entries {n_q}, to the one-pair kernel {n_def}; with 16 faces: {routes[2]} through the 2048-face "
+           f"store")
+    assert routes[:2] == (n_q, n_def) and routes[2] >= 0.9 * n_def
     for a, b in zip(got, ref):
         assert np.array_equal(a, b)
     ref_pairs, ref_cnt, ref_pts, _ = _oracle_pass(oracle, sc, MARGIN, "sweep")
