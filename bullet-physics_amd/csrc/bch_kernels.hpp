@@ -89,7 +89,9 @@ __global__ void k_pack_bodies(uint32_t n, const float* __restrict__ pos, const f
                               BodyRec* __restrict__ out, uint8_t* __restrict__ type_out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  type_out[i] = type[i];
+  // the pair split's type byte: shape type, bit 4 set for a hull of more
+  // than 32 vertices (its supports take twice the scan)
+  type_out[i] = (uint8_t)(type[i] | (type[i] == HULL && hull_count[i] > 32u ? 0x10u : 0u));
   BodyRec b;
   b.px = pos[3 * i];
   b.py = pos[3 * i + 1];
@@ -695,6 +697,12 @@ __device__ __forceinline__ uint32_t dev_count(uint32_t n, const uint32_t* n_dev)
   return m < n ? m : n;
 }
 
+// Per-pair class word: bits 0-2 the shape class (the route readback's low
+// byte; EPA sets bit 3), bits 8-15 the sort digit class << 5 | sub, which
+// groups the pairs by class and, within a class, by `sub`.
+__device__ __forceinline__ uint32_t cls_key(uint32_t c, uint32_t sub) { return c | (((c << 5) | sub) << 8); }
+__device__ __forceinline__ uint32_t cls_of_key(uint32_t k) { return (k >> 13) & 7u; }
+
 __global__ void k_split_pairs(uint32_t P, const uint32_t* __restrict__ P_dev, const uint64_t* __restrict__ keys,
                               const uint8_t* __restrict__ btype, uint32_t* __restrict__ pairs,
                               uint32_t* __restrict__ cls, uint32_t* __restrict__ iota) {
@@ -705,7 +713,13 @@ __global__ void k_split_pairs(uint32_t P, const uint32_t* __restrict__ P_dev, co
   uint32_t i = (uint32_t)(k >> 32), j = (uint32_t)k;
   pairs[2 * p] = i;
   pairs[2 * p + 1] = j;
-  cls[p] = (uint32_t)pair_class((int)btype[i], (int)btype[j]);
+  const uint32_t ta = btype[i], tb = btype[j];
+  const uint32_t c = (uint32_t)pair_class((int)(ta & 7u), (int)(tb & 7u));
+  // within the walk classes the pairs are grouped by their shape types and
+  // hull size, so that the lanes of a wavefront take the same support code
+  const uint32_t sub =
+      (c == CLS_GJK_HULL || c == CLS_GJK) ? (((ta & 3u) << 2) | (tb & 3u) | ((ta | tb) & 0x10u)) : 0u;
+  cls[p] = cls_key(c, sub);
   iota[p] = p;
 }
 
@@ -803,9 +817,9 @@ __global__ void k_class_bounds(uint32_t P, const uint32_t* __restrict__ P_dev, c
   P = dev_count(P, P_dev);
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= P) return;
-  uint32_t c = cls_sorted[t];
-  if (t == 0 || cls_sorted[t - 1] != c) cnt->cls_start[c] = t;
-  if (t == P - 1 || cls_sorted[t + 1] != c) cnt->cls_end[c] = t + 1;
+  uint32_t c = cls_of_key(cls_sorted[t]);
+  if (t == 0 || cls_of_key(cls_sorted[t - 1]) != c) cnt->cls_start[c] = t;
+  if (t == P - 1 || cls_of_key(cls_sorted[t + 1]) != c) cnt->cls_end[c] = t + 1;
 }
 
 // One lane per pair of shape class CLS (grid-stride over the class range):

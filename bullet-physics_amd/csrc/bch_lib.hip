@@ -582,8 +582,9 @@ int narrow_finish(bch_context* c, const float* d_seeds, bool dense = true) {
   uint64_t P = c->P;
   const uint32_t* P_dev = c->cap_mode ? &cnt->n_pairs : nullptr;
   if (P) {
-    // one 8-bit digit holds the class; one pass leaves the result in the alt pair
-    int cs[1] = {0};
+    // one 8-bit digit (bits 8-15) holds the class and its sub-group; one pass
+    // leaves the result in the alt pair
+    int cs[1] = {8};
     bool calt = false;
     int rc = dev_radix_sort<uint32_t>(c, c->cls.as<uint32_t>(), c->cls_alt.as<uint32_t>(), c->perm.as<uint32_t>(),
                                       c->perm_alt.as<uint32_t>(), P, cs, 1, s, &calt, P_dev);

@@ -349,7 +349,7 @@ __global__ __launch_bounds__(256) void k_mf_pre(uint32_t P, const uint64_t* __re
   }
   route[p] = r;
   if (r != MR_PENDING && r != MR_PENDING_REFRESHED) {
-    cls[p] = kMfNoDetect;
+    cls[p] = cls_key(kMfNoDetect, 0u);
     ccount[p] = 0u;
   }
 }
