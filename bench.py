@@ -369,13 +369,14 @@ def differentiation(bpa, steps, n_pairs=262_144):
     t3 = time.perf_counter()
     fwd = (t2 - t1) / k
     bwd = (t3 - t2) / k
-    status = np.bincount(st.cpu().numpy() & 7, minlength=5)
+    status = np.bincount(st.cpu().numpy() & 7, minlength=6)
     w.ctx.set_stream(None)
     return {"workload": f"{P} convex-hull pairs, 32 points each, half separated by U[0,0.2] m, half penetrating "
                         f"by U[0,0.2] m", "placement": placed,
             "forward_ms": round(fwd * 1e3, 3), "backward_ms": round(bwd * 1e3, 3),
             "pairs_per_s_forward": round(P / fwd, 1), "pairs_per_s_forward_backward": round(P / (fwd + bwd), 1),
-            "routes": {"walk": int(status[0]), "epa": int(status[1]), "capped": int(status[2])}}
+            "routes": {"walk": int(status[0]), "epa": int(status[1]), "capped": int(status[2]),
+                       "separated_after_rounding": int(status[5])}}
 
 
 def depenetration(bpa, steps=100):
@@ -570,7 +571,7 @@ def gradient_check(bpa, oracle_mod, n_worlds=16, directions=16, h=1e-4):
         s2 = dict(sc)
         s2["position"] = p.astype(np.float32)
         s2["orientation"] = q.astype(np.float32)
-        pr = oracle_mod.sweep_pairs(*oracle_mod.scene_aabbs(s2, 0.04))
+        pr = oracle_mod.world_pairs(s2, 0.04)
         i = pr.astype(np.int64)
         pose = np.concatenate([p[i[:, 0]], q[i[:, 0]], p[i[:, 1]], q[i[:, 1]]], axis=1)
         s2["pairs"] = pr

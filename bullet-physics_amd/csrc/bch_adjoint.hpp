@@ -32,9 +32,15 @@ enum : uint32_t {
   SD_CAPPED = 2,           // walk stopped at its iteration cap (closest iterate)
   SD_TOUCH = 3,            // cores touch and no enclosing tetrahedron: sd = -(margins), no normal
   SD_EPA_UNCONVERGED = 4,  // EPA ended without converging (reported face)
+  SD_SEPARATED = 5,        // walk proved a core gap, then enclosed the origin by rounding (closest iterate)
+  SD_EPA_CONTRADICTED = 6, // EPA depth beyond the walk's depth bound (closest iterate)
   SD_FLAG_EDGE = 8,        // EPA face distance came from an edge or vertex, not the plane
   SD_FLAG_SPLIT = 16       // a face vertex could not be split into support points
 };
+
+// float4 per queued signed-distance EPA entry: the walk's enclosing
+// simplex with the pair index, then its depth bound and closest iterate.
+constexpr uint32_t kSdEpaEntry = 8;
 
 // Pose gradient accumulator for one pair: position A, quaternion A,
 // position B, quaternion B.
@@ -350,7 +356,7 @@ __global__ __launch_bounds__(256) void k_sd_walk(uint32_t P, const uint32_t* __r
       need_epa = g.kind == 2;
       if (sub == 0 && !need_epa) {
         bool band = g.kind == 1 || g.capped;
-        uint32_t st = band ? (g.kind == 1 ? SD_WALK : SD_CAPPED) : SD_TOUCH;
+        uint32_t st = band ? (g.kind == 1 ? SD_WALK : (g.separated ? SD_SEPARATED : SD_CAPPED)) : SD_TOUCH;
         SdOut o;
         if (band) {
           if (g.kind == 1) {
@@ -387,11 +393,20 @@ __global__ __launch_bounds__(256) void k_sd_walk(uint32_t P, const uint32_t* __r
       if (lane == leader) qb = atomicAdd(&cnt->n_epa, (uint32_t)__popcll(mask));
       qb = __shfl(qb, leader);
       if (enq) {
-        uint32_t q = qb + __popcll(mask & ((1ull << lane) - 1ull));
-        epa_q[4 * (uint64_t)q] = make_float4(g.t0.x, g.t0.y, g.t0.z, u2f(t));
-        epa_q[4 * (uint64_t)q + 1] = make_float4(g.t1.x, g.t1.y, g.t1.z, 0.0f);
-        epa_q[4 * (uint64_t)q + 2] = make_float4(g.t2.x, g.t2.y, g.t2.z, 0.0f);
-        epa_q[4 * (uint64_t)q + 3] = make_float4(g.t3.x, g.t3.y, g.t3.z, 0.0f);
+        // entry of kSdEpaEntry float4: the enclosing simplex and the pair,
+        // then the walk's depth bound and closest iterate (for an EPA
+        // result the bound contradicts)
+        uint64_t e = (uint64_t)kSdEpaEntry * (qb + __popcll(mask & ((1ull << lane) - 1ull)));
+        epa_q[e] = make_float4(g.t0.x, g.t0.y, g.t0.z, u2f(t));
+        epa_q[e + 1] = make_float4(g.t1.x, g.t1.y, g.t1.z, 0.0f);
+        epa_q[e + 2] = make_float4(g.t2.x, g.t2.y, g.t2.z, 0.0f);
+        epa_q[e + 3] = make_float4(g.t3.x, g.t3.y, g.t3.z, 0.0f);
+        epa_q[e + 4] = make_float4(g.last_v.x, g.last_v.y, g.last_v.z, g.depth_bound);
+        epa_q[e + 5] = make_float4(g.last_pb.x, g.last_pb.y, g.last_pb.z, u2f((uint32_t)g.act.n));
+        epa_q[e + 6] = make_float4(u2f((uint32_t)g.act.s0.ia), u2f((uint32_t)g.act.s1.ia),
+                                   u2f((uint32_t)g.act.s2.ia), 0.0f);
+        epa_q[e + 7] = make_float4(u2f((uint32_t)g.act.s0.ib), u2f((uint32_t)g.act.s1.ib),
+                                   u2f((uint32_t)g.act.s2.ib), 0.0f);
       }
     }
   }
@@ -434,8 +449,8 @@ __global__ __launch_bounds__(64) void k_sd_epa(const uint32_t* __restrict__ n_qu
   const int lane = threadIdx.x & 63;
   const uint32_t nq = *n_queue;
   for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
-    float4 a = epa_q[4 * (uint64_t)q], b = epa_q[4 * (uint64_t)q + 1], c = epa_q[4 * (uint64_t)q + 2],
-           d = epa_q[4 * (uint64_t)q + 3];
+    const float4* ent = epa_q + (uint64_t)kSdEpaEntry * q;
+    float4 a = ent[0], b = ent[1], c = ent[2], d = ent[3];
     uint32_t t = f2u(a.w);
     uint32_t i = pairs[2 * t], j = pairs[2 * t + 1];
     Pair pr = make_pair(bodies[i], bodies[j], hull);
@@ -463,8 +478,27 @@ __global__ __launch_bounds__(64) void k_sd_epa(const uint32_t* __restrict__ n_qu
     o.pa = wa;
     o.pb = o.pa - e.n * depth;
     uint32_t st = e.status == 0 ? SD_EPA : SD_EPA_UNCONVERGED;
+    // A core depth beyond the walk's own bound (plus EPA's exit tolerance)
+    // comes from a polytope corrupted by rounding: the closest iterate of
+    // the walk stands, as at the iteration cap.
+    float4 it_v = ent[4], it_pb = ent[5];
+    bool contradicted = e.depth > it_v.w + 1e-4f;
+    if (contradicted) {
+      st = SD_EPA_CONTRADICTED;
+      o = band_outputs(pr, V3{it_v.x, it_v.y, it_v.z}, V3{it_pb.x, it_pb.y, it_pb.z});
+    }
     if (!BWD) {
       if (lane == 0) store_out(out + 10 * (uint64_t)t, o);
+    } else if (contradicted) {
+      float4 ia = ent[6], ib = ent[7];
+      Simp act;
+      act.n = (int)f2u(it_pb.w);
+      act.s0.ia = (int)f2u(ia.x); act.s1.ia = (int)f2u(ia.y); act.s2.ia = (int)f2u(ia.z);
+      act.s0.ib = (int)f2u(ib.x); act.s1.ib = (int)f2u(ib.y); act.s2.ib = (int)f2u(ib.z);
+      PoseGrad Gr;
+      Gr.zero();
+      band_vjp(pr, act, gin + 10 * (uint64_t)t, Gr);
+      if (lane == 0) store_grad(out + 14 * (uint64_t)t, Gr);
     } else {
       PoseGrad Gr;
       Gr.zero();

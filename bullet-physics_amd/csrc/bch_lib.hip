@@ -955,7 +955,7 @@ int run_depen_step(bch_context* c, float aabb_margin, float lr, float* d_pos, fl
   ENSURE(c, c->pairs, cap * 2 * sizeof(uint32_t));
   ENSURE(c, c->cls, cap * sizeof(uint32_t));
   ENSURE(c, c->perm, cap * sizeof(uint32_t));
-  ENSURE(c, c->epa_q, cap * 4 * sizeof(float4));
+  ENSURE(c, c->epa_q, cap * kSdEpaEntry * sizeof(float4));
   ENSURE(c, c->sd_out, cap * 10 * sizeof(float));
   ENSURE(c, c->sd_status, cap);
   ENSURE(c, c->pen_pairs, cap * 2 * sizeof(uint32_t));
@@ -2406,7 +2406,7 @@ static int run_sd(bch_context* c, const uint32_t* d_pairs, uint64_t P, const flo
   if (P > 0xffffffffull) return fail(c, BCH_INVALID_ARG, "more than 2^32 - 1 pairs");
   if (P && !c->n) return fail(c, BCH_INVALID_ARG, "no bodies: call bch_world_set_bodies first");
   HIP_TRY(c, hipSetDevice(c->device));
-  ENSURE(c, c->epa_q, std::max<uint64_t>(P, 1) * 4 * sizeof(float4));
+  ENSURE(c, c->epa_q, std::max<uint64_t>(P, 1) * kSdEpaEntry * sizeof(float4));
   hipStream_t s = c->stream;
   Counters* cnt = c->counters.as<Counters>();
   hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);

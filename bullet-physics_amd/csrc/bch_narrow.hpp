@@ -740,6 +740,14 @@ struct GjkResult {
   // iterations: an f32 walk that misses the 1 - 1e-6 exit by rounding cycles
   // through worse simplices afterwards) with its active simplex.
   bool capped;
+  // Set with capped: the walk had proven a core gap (an iterate with
+  // delta / |v| > 1e-4 m) and later ended on a simplex enclosing or touching
+  // the origin, which only rounding can produce (a flat simplex, or
+  // coordinates far from the origin); the closest iterate stands.
+  bool separated;
+  // The least support-plane bound -delta / |v| over the iterations: the
+  // core penetration depth is at most this (EPA beyond it is contradicted).
+  float depth_bound;
   V3 last_v, last_pb;
   Simp act;
 };
@@ -752,7 +760,12 @@ template <class SupF, bool SD = false>
 BCH_HD GjkResult gjk_contact_t(const Pair& pr, bool has_seed, V3 seed, float threshold, SupF sup) {
   GjkResult R;
   R.kind = 0;
-  if (SD) R.capped = false;
+  if (SD) {
+    R.capped = R.separated = false;
+    R.depth_bound = INFINITY;
+    R.last_v = R.last_pb = V3{0.0f, 0.0f, 0.0f};
+    R.act.n = 0;
+  }
   float margin_a = pr.a.margin, margin_b = pr.b.margin;
   float margin_sum = margin_a + margin_b;
   V3 center_dir = pr.pb - pr.pa;
@@ -765,8 +778,15 @@ BCH_HD GjkResult gjk_contact_t(const Pair& pr, bool has_seed, V3 seed, float thr
   V3 dir = -first.w;
   float touch2 = 1e-12f * fmax_nan(len2(first.w), 1.0f);
   float best_v2 = INFINITY;
+  bool proven_gap = false;
   for (int it = 0; it < 64; ++it) {
     Reduce r = reduce(S);
+    V3 v = r.v;
+    float v2 = len2(v);
+    if (SD && proven_gap && (r.inside || v2 < touch2)) {
+      R.capped = R.separated = true;
+      return R;
+    }
     if (r.inside) {
       R.kind = 2;
       R.t0 = S.s0.w;
@@ -775,8 +795,6 @@ BCH_HD GjkResult gjk_contact_t(const Pair& pr, bool has_seed, V3 seed, float thr
       R.t3 = S.s3.w;
       return R;
     }
-    V3 v = r.v;
-    float v2 = len2(v);
     if (v2 < touch2) {
       WSimp W;
       bool ok = enclosing_walk_t(sup, fallback, dir, W);
@@ -794,6 +812,11 @@ BCH_HD GjkResult gjk_contact_t(const Pair& pr, bool has_seed, V3 seed, float thr
     float delta = dot(v, w.w);
     float band = margin_sum + threshold;
     if (delta > 0.0f && delta * delta > v2 * band * band) return R;
+    if (SD && delta > 0.0f && delta * delta > v2 * (1e-4f * 1e-4f)) proven_gap = true;
+    if (SD) {
+      float bound = -delta / fsqrt(v2);
+      if (bound < R.depth_bound) R.depth_bound = bound;
+    }
     bool dup = near_eq(w.w, S.s0.w, 1e-6f);
     if (S.n > 1) dup = dup || near_eq(w.w, S.s1.w, 1e-6f);
     if (S.n > 2) dup = dup || near_eq(w.w, S.s2.w, 1e-6f);

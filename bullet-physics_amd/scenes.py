@@ -399,15 +399,20 @@ def exact_penetration(sc: dict, signed_distance, iters: int = 40, reach: float =
 
 
 def batched_boxes_hulls(n_worlds: int = 1024, seed: int = 5, lo_count: int = 64, hi_count: int = 1024,
-                        fill: float = 0.35) -> dict:
+                        fill: float = 0.35, gap: float | None = None) -> dict:
     """``n_worlds`` independent worlds of boxes and hulls (50/50), body count
     log-uniform in [lo_count, hi_count], for the differentiable batch.
 
     Boxes: half-extents U[0.25, 1.0] per axis; hulls: 16-64 points on
     ellipsoids with semi-axes U[0.3, 1.2]; orientations uniform.  World w
-    fills its own cube (side chosen so that the bodies' bounding volume is
-    ``fill`` of it, which leaves many pairs interpenetrating) and the worlds
-    sit side by side along x, 10 m apart."""
+    fills its own cube [0, side_w]^3 (side chosen so that the bodies'
+    bounding volume is ``fill`` of it, which leaves many pairs
+    interpenetrating), in its own coordinates: the cubes of all worlds share
+    the origin, as independent worlds do (the batched broadphase keeps them
+    apart by world id).  With ``gap`` the cubes instead sit side by side
+    along x, ``gap`` metres apart; 1,024 worlds then reach x = 21 km, where
+    an f32 ulp is 2 mm and the f32 walks of the reference's arithmetic report
+    false overlaps hundreds of millimetres deep."""
     rng = Lcg(seed)
     u = rng.unit(n_worlds).astype(np.float64)
     counts = np.floor(np.exp(math.log(lo_count) + u * (math.log(hi_count) - math.log(lo_count)))).astype(np.int64)
@@ -434,7 +439,7 @@ def batched_boxes_hulls(n_worlds: int = 1024, seed: int = 5, lo_count: int = 64,
     sc["orientation"] = random_quaternions(rng, N)
     offs = np.concatenate(([0], np.cumsum(counts))).astype(np.uint32)
     side = (counts * 8 * 0.6 ** 3 / fill) ** (1.0 / 3.0)
-    x0 = np.concatenate(([0.0], np.cumsum(side + 10.0)[:-1]))
+    x0 = np.concatenate(([0.0], np.cumsum(side + gap)[:-1])) if gap is not None else np.zeros(n_worlds)
     wid = np.repeat(np.arange(n_worlds), counts)
     u3 = rng.unit(3 * N).astype(np.float64).reshape(N, 3)
     pos = u3 * side[wid, None]

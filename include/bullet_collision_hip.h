@@ -254,14 +254,24 @@ int bch_world_routes(bch_context* ctx, uint8_t* routes);
  * (narrowphase/gjk.rs:171-288) with threshold = +inf, so a separated pair of
  * any distance ends in its Band arm, and EPA plus the contact of
  * convex_convex.rs:117-154 for overlapping cores.  A walk stopped by its
- * iteration cap reports its closest iterate like the Band arm.  All pointers
+ * iteration cap reports its closest iterate like the Band arm, and so does a
+ * walk that proved a core gap above 1e-4 m (one iterate's support plane) and
+ * then ended on a simplex enclosing the origin, which only rounding can
+ * produce (the reference's f32 walk would go on to EPA with it and report a
+ * false, often deep, penetration), and an EPA whose core depth exceeds the
+ * walk's own bound (the least support-plane distance -delta / |v| of its
+ * iterates) by more than 1e-4 m, which only a polytope corrupted by rounding
+ * can give.  All pointers
  * are DEVICE pointers; the calls are ordered on the context stream and do
  * not synchronise.
  *   d_pairs  [2P] body indices (i, j): shape A = body i, shape B = body j
  *   d_out    [10P] per pair: sd (> 0 separated, < 0 penetrating), unit normal
  *            A -> B, witness on A, witness on B (pa = pb - n sd)
  *   d_status [P] or NULL: 0 walk, 1 EPA, 2 capped walk, 3 touching cores (no
- *            normal), 4 EPA unconverged; +8 EPA depth from an edge, +16 a
+ *            normal), 4 EPA unconverged, 5 walk ended by rounding after it
+ *            proved a gap (closest iterate), 6 EPA contradicted by the
+ *            walk's depth bound (closest iterate); +8 EPA depth from an
+ *            edge, +16 a
  *            face vertex not split (backward only); 255 pair index out of
  *            range (outputs NaN).
  * No reference counterpart beyond the walks it reuses. */

@@ -757,6 +757,16 @@ struct GjkOut {
   // all iterations; an f32 walk that misses the 1 - 1e-6 exit by rounding
   // cycles through worse simplices afterwards) and B witness.
   bool capped = false;
+  // Also not in the reference: the walk had proven a core gap (an iterate
+  // with delta / |v| > SD_GAP_PROOF) and later ended on a simplex that
+  // enclosed or touched the origin, which rounding alone can produce (a
+  // flat simplex, or coordinates far from the origin).  The closest iterate
+  // is reported as for the cap, with capped and separated both set.
+  bool separated = false;
+  // Signed-distance walks: the least support-plane bound -delta / |v| over
+  // the iterations.  Each iterate's support plane along -v lies that far
+  // beyond the origin, so the core penetration depth is at most this.
+  T depth_bound = std::numeric_limits<T>::infinity();
   V3<T> last_v{0, 0, 0}, last_pb{0, 0, 0};
   // the active simplex (w, b) at a Band exit, or of that iterate at the cap
   SupportVertex<T> active[4];
@@ -1082,9 +1092,19 @@ struct PairPose {
   }
 };
 
-// gjk.rs:171-288 -- the margin-core distance walk.
+// Signed-distance walks only: a core gap larger than this (metres) proven
+// by one iterate (support plane delta / |v|) outranks a later enclosure.
+constexpr double SD_GAP_PROOF = 1e-4;
+// Signed-distance EPA only: a core depth more than this (EPA's own exit
+// tolerance, epa.rs:35) beyond the walk's depth bound is contradicted.
+constexpr double SD_EPA_SLACK = 1e-4;
+
+// gjk.rs:171-288 -- the margin-core distance walk.  sd_walk (not in the
+// reference; signed-distance callers, threshold = +inf) keeps the closest
+// iterate and reports it when the walk hits the cap or ends enclosing the
+// origin after it proved a gap.
 template <class T>
-GjkOut<T> gjk_contact(const PairPose<T>& pr, const V3<T>* seed, T threshold) {
+GjkOut<T> gjk_contact(const PairPose<T>& pr, const V3<T>* seed, T threshold, bool sd_walk = false) {
   GjkOut<T> out;
   T margin_a = pr.a->margin(), margin_b = pr.b->margin();
   T margin_sum = margin_a + margin_b;
@@ -1098,16 +1118,24 @@ GjkOut<T> gjk_contact(const PairPose<T>& pr, const V3<T>* seed, T threshold) {
   T touch2 = ORL(1e-12) * std::fmax(len2(first.w), T(1));
   const T REL_ERROR2 = ORL(1e-6);
   T best_v2 = std::numeric_limits<T>::infinity();
+  const T gap2 = T(SD_GAP_PROOF) * T(SD_GAP_PROOF);
+  bool proven_gap = false;
   for (int it = 0; it < 64; ++it) {
     out.iterations = it + 1;
     Reduce<T> r = reduce(buf, n);
+    V3<T> v = r.v;
+    T v2 = len2(v);
+    if (sd_walk && proven_gap && (r.inside || v2 < touch2)) {
+      out.kind = 0;
+      out.capped = true;
+      out.separated = true;
+      return out;
+    }
     if (r.inside) {
       out.kind = 2;
       for (int i = 0; i < 4; ++i) out.simplex[i] = buf[i].w;
       return out;
     }
-    V3<T> v = r.v;
-    T v2 = len2(v);
     if (v2 < touch2) {
       Simplex<T> s;
       auto sup = [&](V3<T> d) { return pr.core_support(d).w; };
@@ -1125,6 +1153,11 @@ GjkOut<T> gjk_contact(const PairPose<T>& pr, const V3<T>* seed, T threshold) {
     T delta = dot(v, w.w);
     T band = margin_sum + threshold;
     if (delta > T(0) && delta * delta > v2 * band * band) { out.kind = 0; return out; }
+    if (delta > T(0) && delta * delta > v2 * gap2) proven_gap = true;
+    if (sd_walk) {
+      T bound = -delta / std::sqrt(v2);
+      if (bound < out.depth_bound) out.depth_bound = bound;
+    }
     bool duplicate = false;
     for (int i = 0; i < n; ++i)
       if (abs_diff_eq(w.w, buf[i].w, ORL(1e-6))) duplicate = true;
@@ -1371,11 +1404,24 @@ bool convex_convex(const PairPose<T>& pr, const V3<T>* seed, T threshold, Manifo
 // Not a reference function: the signed distance between two convex shapes
 // built from the reference's own walks.  The margin-core walk (gjk.rs:171-288)
 // runs with threshold = +inf, so a separated pair ends in its Band arm at any
-// distance; overlapping cores go to EPA as convex_convex.rs:117-154 does.  A
-// walk stopped by its 64-iteration cap reports its last iterate the way the
-// Band arm would.  Outputs: sd (> 0 separated), unit normal A -> B, witness
-// points on A and on B with pa = pb - n sd.
-enum SdStatus { SD_WALK = 0, SD_EPA = 1, SD_CAPPED = 2, SD_TOUCH = 3, SD_EPA_UNCONVERGED = 4 };
+// distance; overlapping cores go to EPA as convex_convex.rs:117-154 does.
+// Three exits report the walk's closest iterate the way the Band arm would:
+// the 64-iteration cap; an enclosure after the walk proved a core gap
+// (SD_GAP_PROOF); and an EPA depth beyond the walk's own depth bound
+// (SD_EPA_SLACK), which only a polytope corrupted by rounding can give (the
+// reference's f32 EPA, started from the nearly flat simplex of just-touching
+// hulls, has converged on faces 0.02-0.18 m deep).  Outputs: sd (> 0
+// separated), unit normal A -> B, witness points on A and on B with
+// pa = pb - n sd.
+enum SdStatus {
+  SD_WALK = 0,
+  SD_EPA = 1,
+  SD_CAPPED = 2,
+  SD_TOUCH = 3,
+  SD_EPA_UNCONVERGED = 4,
+  SD_SEPARATED = 5,        // the walk proved a core gap, then enclosed the origin by rounding
+  SD_EPA_CONTRADICTED = 6  // EPA's depth exceeds the walk's depth bound
+};
 template <class T>
 struct SdOut {
   int status = SD_TOUCH;
@@ -1386,8 +1432,15 @@ template <class T>
 SdOut<T> signed_distance(const PairPose<T>& pr) {
   SdOut<T> o;
   T ma = pr.a->margin(), mb = pr.b->margin();
-  GjkOut<T> g = gjk_contact(pr, (const V3<T>*)nullptr, std::numeric_limits<T>::infinity());
-  if (g.kind == 1 || (g.kind == 0 && g.capped)) {
+  GjkOut<T> g = gjk_contact(pr, (const V3<T>*)nullptr, std::numeric_limits<T>::infinity(), true);
+  EpaOut<T> e{};
+  bool contradicted = false;
+  if (g.kind == 2) {
+    auto sup = [&](V3<T> d) { return pr.core_support(d).w; };
+    e = epa_penetration<T>(g.simplex, sup);
+    contradicted = e.depth > g.depth_bound + T(SD_EPA_SLACK);
+  }
+  if (g.kind == 1 || (g.kind == 0 && g.capped) || contradicted) {
     if (g.kind == 1) {
       o.status = SD_WALK;
       o.sd = g.distance;
@@ -1395,7 +1448,7 @@ SdOut<T> signed_distance(const PairPose<T>& pr) {
       o.pb = g.point_on_b;
     } else {
       T core_distance = std::sqrt(len2(g.last_v));
-      o.status = SD_CAPPED;
+      o.status = contradicted ? SD_EPA_CONTRADICTED : (g.separated ? SD_SEPARATED : SD_CAPPED);
       o.sd = core_distance - (ma + mb);
       o.n = (-g.last_v) / core_distance;
       o.pb = g.last_pb - o.n * mb;
@@ -1404,8 +1457,6 @@ SdOut<T> signed_distance(const PairPose<T>& pr) {
     return o;
   }
   if (g.kind == 2) {
-    auto sup = [&](V3<T> d) { return pr.core_support(d).w; };
-    EpaOut<T> e = epa_penetration<T>(g.simplex, sup);
     T depth = (e.depth + ma) + mb;
     o.status = e.status == EPA_CONVERGED ? SD_EPA : SD_EPA_UNCONVERGED;
     o.sd = -depth;
@@ -1484,9 +1535,16 @@ bool sd_vjp(const PairPose<T>& pr, const T* g10, T* g14) {
   V3<T> gpa_pos = splat(T(0)), gpb_pos = splat(T(0));
   T gqa[4] = {0, 0, 0, 0}, gqb[4] = {0, 0, 0, 0};
   V3<T> gn = mk(g10[1], g10[2], g10[3]), gpa = mk(g10[4], g10[5], g10[6]), gpb = mk(g10[7], g10[8], g10[9]);
-  GjkOut<T> g = gjk_contact(pr, (const V3<T>*)nullptr, std::numeric_limits<T>::infinity());
+  GjkOut<T> g = gjk_contact(pr, (const V3<T>*)nullptr, std::numeric_limits<T>::infinity(), true);
   bool ok = true;
-  if (g.kind == 1 || (g.kind == 0 && g.capped)) {
+  EpaOut<T> e{};
+  bool contradicted = false;
+  if (g.kind == 2) {
+    auto sup = [&](V3<T> d) { return pr.core_support(d).w; };
+    e = epa_penetration<T>(g.simplex, sup);
+    contradicted = e.depth > g.depth_bound + T(SD_EPA_SLACK);
+  }
+  if (g.kind == 1 || (g.kind == 0 && g.capped) || contradicted) {
     int k = g.n_active;
     V3<T> w[3], b[3];
     for (int i = 0; i < k && i < 3; ++i) {
@@ -1551,8 +1609,6 @@ bool sd_vjp(const PairPose<T>& pr, const T* g10, T* g14) {
       qrot_vjp(pr.qb, sd_to_local(pr.qb, pr.pb, b[i]), gbi, gqb);
     }
   } else if (g.kind == 2) {
-    auto sup = [&](V3<T> d) { return pr.core_support(d).w; };
-    EpaOut<T> e = epa_penetration<T>(g.simplex, sup);
     T depth = (e.depth + ma) + mb;
     V3<T> n = e.normal;
     V3<T> g_pat = gpa + gpb;

@@ -487,6 +487,16 @@ def sweep_pairs(mn, mx):
     return out[:cnt]
 
 
+def world_pairs(scene, margin):
+    """Sorted pair set of a batch of independent worlds (``world_offsets``),
+    world by world, in the batch's body indices: worlds may share space."""
+    wo = np.asarray(scene["world_offsets"], np.int64)
+    mn, mx = scene_aabbs(scene, margin)
+    out = [sweep_pairs(mn[wo[w]:wo[w + 1]], mx[wo[w]:wo[w + 1]]).astype(np.int64) + wo[w]
+           for w in range(wo.shape[0] - 1)]
+    return np.concatenate(out).astype(np.uint32) if out else np.zeros((0, 2), np.uint32)
+
+
 def dbvt_pairs(mn, mx):
     """DbvtBroadPhase's pair set on a fresh scene (dbvt.rs), sorted."""
     mn = np.ascontiguousarray(mn, np.float32)
@@ -516,7 +526,7 @@ def collide_pairs(scene, pairs, seeds=None, f64=False):
     return cnt, pts, path
 
 
-SD_WALK, SD_EPA, SD_CAPPED, SD_TOUCH, SD_EPA_UNCONVERGED = 0, 1, 2, 3, 4
+SD_WALK, SD_EPA, SD_CAPPED, SD_TOUCH, SD_EPA_UNCONVERGED, SD_SEPARATED, SD_EPA_CONTRADICTED = 0, 1, 2, 3, 4, 5, 6
 
 
 def signed_distance(a: Shape, pa, qa, b: Shape, pb, qb, f64=False):

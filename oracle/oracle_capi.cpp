@@ -594,7 +594,7 @@ int sd_active(const or_scene* sc, uint32_t i, uint32_t j, double* w, double* b) 
   SceneShapes<T> ss;
   ss.build(sc);
   PairPose<T> pr{&ss.shapes[i], ss.pos[i], ss.quat[i], &ss.shapes[j], ss.pos[j], ss.quat[j]};
-  GjkOut<T> g = gjk_contact(pr, (const V3<T>*)nullptr, std::numeric_limits<T>::infinity());
+  GjkOut<T> g = gjk_contact(pr, (const V3<T>*)nullptr, std::numeric_limits<T>::infinity(), true);
   if (!(g.kind == 1 || (g.kind == 0 && g.capped))) return 0;
   for (int k = 0; k < g.n_active; ++k) {
     w[3 * k] = g.active[k].w.x; w[3 * k + 1] = g.active[k].w.y; w[3 * k + 2] = g.active[k].w.z;

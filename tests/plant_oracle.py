@@ -111,6 +111,14 @@ MUTATIONS = [
      "capsule.rs:59-69 core tip picks +hh for dir.y >= 0",
      "        return mk(T(0), dir.y >= T(0) ? half_height : -half_height, T(0));",
      "        return mk(T(0), dir.y > T(0) ? half_height : -half_height, T(0));"),
+    ("sd_gap_proof_off",
+     "signed-distance extension: a walk that proved a core gap never reports an enclosure",
+     "    if (delta > T(0) && delta * delta > v2 * gap2) proven_gap = true;",
+     "    if ((false)) proven_gap = true;"),
+    ("sd_epa_bound_off",
+     "signed-distance extension: an EPA depth beyond the walk's depth bound is not reported",
+     "    contradicted = e.depth > g.depth_bound + T(SD_EPA_SLACK);\n  }\n  if (g.kind == 1 || (g.kind == 0 && g.capped) || contradicted) {\n    if (g.kind == 1) {",
+     "    contradicted = false && e.depth > g.depth_bound;\n  }\n  if (g.kind == 1 || (g.kind == 0 && g.capped) || contradicted) {\n    if (g.kind == 1) {"),
     ("capped_last_iterate",
      "signed-distance extension: a capped walk reports its closest iterate, not the last",
      "    if (!(v2 >= best_v2)) {",

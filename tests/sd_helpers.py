@@ -53,3 +53,34 @@ def witness_margin(sc, n):
         elif t[k] == 2:
             out[k] = 2 * abs(nl[k, 1]) * par[k, 1]
     return out
+
+
+def rounding_exit_scene():
+    """The recorded rounding-exit pairs (tests/golden/sd_rounding_exits.json)
+    as a scene of 2k bodies with pairs (2k, 2k + 1), plus the case list."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "sd_rounding_exits.json")
+    with open(path) as f:
+        cases = json.load(f)["cases"]
+    n = 2 * len(cases)
+    sc = dict(position=np.zeros((n, 3), np.float32), orientation=np.zeros((n, 4), np.float32),
+              shape_type=np.zeros(n, np.uint8), shape_params=np.zeros((n, 4), np.float32),
+              hull_first=np.zeros(n, np.uint32), hull_count=np.zeros(n, np.uint32))
+    verts = []
+    for k, c in enumerate(cases):
+        for side, body in (("a", 2 * k), ("b", 2 * k + 1)):
+            s = c["shape_" + side]
+            sc["position"][body] = np.float32(c["pos_" + side])
+            sc["orientation"][body] = np.float32(c["quat_" + side])
+            if s["type"] == "box":
+                sc["shape_type"][body] = 1
+                sc["shape_params"][body, :3] = np.float32(s["half_extents"])
+            else:
+                sc["shape_type"][body] = 3
+                sc["hull_first"][body] = sum(len(v) for v in verts)
+                sc["hull_count"][body] = len(s["points"])
+                verts.append(np.float32(s["points"]))
+    sc["hull_vertices"] = np.concatenate(verts).reshape(-1, 3) if verts else np.zeros((0, 3), np.float32)
+    sc["pairs"] = np.arange(n, dtype=np.uint32).reshape(-1, 2)
+    return sc, cases

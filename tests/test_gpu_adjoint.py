@@ -81,7 +81,7 @@ def test_sd_forward_matches_oracle(torch, bpa, scenes, oracle, ctx, mix):
     dpb = np.linalg.norm(out[ok, 7:10].astype(np.float64) - ref[ok, 7:10], axis=1)
     ang = _angle(out[ok, 1:4], ref[ok, 1:4])
     exact = int(np.sum(np.all(out == ref, axis=1)))
-    _flush(f"sd forward: {out.shape[0]} pairs ({np.bincount(rst, minlength=5)} walk/epa/capped/touch/unconv), "
+    _flush(f"sd forward: {out.shape[0]} pairs ({np.bincount(rst, minlength=6)} walk/epa/capped/touch/unconv/separated), "
            f"{exact} bit-identical, max |dsd| {dsd.max():.3g}, |dpa| {dpa.max():.3g}, |dpb| {dpb.max():.3g}, "
            f"angle {ang.max():.3g}")
     assert dsd.max() <= TOL_M and dpa.max() <= TOL_M and dpb.max() <= TOL_M and ang.max() <= TOL_RAD
@@ -177,6 +177,39 @@ def test_sd_gradient_wrt_positions_is_the_normal(torch, bpa, scenes, oracle, ctx
     assert np.abs(gp[p[epa, 0]] + nn[epa]).max() < TOL_RAD
 
 
+def test_sd_rounding_exits_match_oracle_and_differentiate(torch, bpa, oracle, ctx, diff):
+    """Pairs recorded from the penalty descent on which the f32 walk takes a
+    rounding exit (status 5: an enclosure after the walk proved a gap; 6:
+    EPA's depth beyond the walk's own depth bound), tests/golden.  Forward:
+    status and all ten outputs bit-identical to the oracle.  Backward: the
+    Band arm's adjoint at the closest iterate, so d sd / d p_B is a unit
+    vector, d sd / d p_A its negative, and it points along the reported
+    normal (within 0.05 rad: the adjoint rebuilds the iterate's simplex in
+    double, and some of these pairs are within 1e-5 m of touching)."""
+    from sd_helpers import rounding_exit_scene
+    _flush("case: recorded rounding-exit pairs")
+    sc, cases = rounding_exit_scene()
+    _, _, out, st = _gpu_sd(torch, bpa, sc, ctx)
+    ref, rst = oracle.sd_pairs(sc, sc["pairs"])
+    assert np.array_equal(st & 7, rst) and set(rst.tolist()) == {oracle.SD_SEPARATED, oracle.SD_EPA_CONTRADICTED}
+    assert np.array_equal(out, ref)
+    w = bpa.CollisionWorld(sc, ctx)
+    pos = torch.from_numpy(sc["position"]).cuda().requires_grad_(True)
+    quat = torch.from_numpy(sc["orientation"]).cuda().requires_grad_(True)
+    pairs = torch.from_numpy(sc["pairs"].astype(np.int32)).cuda().contiguous()
+    sd, n, _, _, _ = diff.signed_distance(w, pos, quat, pairs)
+    sd.sum().backward()
+    gp = pos.grad.cpu().numpy().astype(np.float64)
+    p = sc["pairs"].astype(np.int64)
+    gb, ga = gp[p[:, 1]], gp[p[:, 0]]
+    ang = _angle(gb.astype(np.float32), n.detach().cpu().numpy())
+    _flush(f"  {len(cases)} pairs, forward bit-identical; |grad p_B| in [{np.linalg.norm(gb, axis=1).min():.6f}, "
+           f"{np.linalg.norm(gb, axis=1).max():.6f}], max angle to n {ang.max():.3g} rad")
+    assert np.isfinite(gp).all() and np.isfinite(quat.grad.cpu().numpy()).all()
+    assert np.abs(np.linalg.norm(gb, axis=1) - 1.0).max() < 1e-5 and np.abs(ga + gb).max() < 1e-6
+    assert ang.max() < 0.05
+
+
 def test_autograd_function_matches_raw_adjoint(torch, bpa, scenes, ctx, diff):
     """The torch.autograd.Function sums the per-pair adjoint over bodies."""
     _flush("case: autograd function")
@@ -217,12 +250,12 @@ def test_autograd_function_matches_raw_adjoint(torch, bpa, scenes, ctx, diff):
 
 
 def _loss64(oracle, sc, pos, quat, margin=0.04):
-    """sum max(0, -sd)^2 over the oracle's own pairs at these poses (f64 sd)."""
+    """sum max(0, -sd)^2 over the oracle's own pairs at these poses (f64 sd),
+    world by world."""
     s2 = dict(sc)
     s2["position"] = pos.astype(np.float32)
     s2["orientation"] = quat.astype(np.float32)
-    mn, mx = oracle.scene_aabbs(s2, margin)
-    pairs = oracle.sweep_pairs(mn, mx)
+    pairs = oracle.world_pairs(s2, margin)
     p = pairs.astype(np.int64)
     pose = np.concatenate([pos[p[:, 0]], quat[p[:, 0]], pos[p[:, 1]], quat[p[:, 1]]], axis=1)
     s2["pairs"] = pairs

@@ -386,7 +386,7 @@ def test_capped_walk_reports_its_closest_iterate(oracle, scenes):
     the last iterate of the cycle was up to 9 cm off (box-hull pairs of the
     de-penetration worlds)."""
     sc = scenes.batched_boxes_hulls(16, 5)
-    pairs = oracle.sweep_pairs(*oracle.scene_aabbs(sc, 0.04))
+    pairs = oracle.world_pairs(sc, 0.04)
     o32, st32 = oracle.sd_pairs(sc, pairs)
     o64, st64 = oracle.sd_pairs(sc, pairs, None, f64=True)
     capped = st32 == oracle.SD_CAPPED
@@ -396,6 +396,57 @@ def test_capped_walk_reports_its_closest_iterate(oracle, scenes):
     close = err < 1e-4
     assert close.mean() >= 0.9 and err.max() < 1e-2
     assert np.abs(o32[both, 1:4] - o64[both, 1:4])[close].max() < 1e-3
+
+
+@pytest.mark.parametrize("shift", [0.0, 20000.0])
+def test_walk_that_proved_a_gap_never_reports_an_overlap(oracle, scenes, shift):
+    """The signed-distance walk runs with threshold +inf, so it goes on past
+    the point where the reference's band exit (gjk.rs:241-244) would have
+    stopped a separated pair.  An iterate whose support plane lies beyond the
+    origin (delta = v . w > 0) bounds the core distance from below by
+    delta / |v|; once that bound exceeds 1e-4 m the cores are apart, and a
+    later simplex that encloses or touches the origin is rounding (a flat
+    simplex, or large coordinates).  Without the rule such pairs went on to
+    EPA and reported penetrations of 0.2-0.6 m for cores 0.1 m apart (the
+    de-penetration worlds, x up to 21 km).  Held, on 16 worlds of boxes and
+    hulls (at the origin, and moved to x = 20 km where an f32 ulp is 2 mm):
+    no pair whose f64 walk ends more than 1 mm apart goes to EPA in f32; the
+    pairs that take the new exit (status SD_SEPARATED) meet the f64 distance
+    to 1e-5 m at the origin and within a few ulps of their coordinates at
+    20 km; and the exit is taken (so this test exercises it)."""
+    sc = scenes.batched_boxes_hulls(16, 5)
+    sc["position"] = (sc["position"].astype(np.float64) + np.array([shift, 0.0, 0.0])).astype(np.float32)
+    pairs = oracle.world_pairs(sc, 0.04)
+    o32, st32 = oracle.sd_pairs(sc, pairs)
+    o64, st64 = oracle.sd_pairs(sc, pairs, None, f64=True)
+    false_epa = (st32 == oracle.SD_EPA) & (o64[:, 0] > 1e-3)
+    assert not false_epa.any(), f"{int(false_epa.sum())} separated pairs reported as overlapping"
+    sep = st32 == oracle.SD_SEPARATED
+    assert sep.sum() >= 10
+    err = np.abs(o32[sep, 0] - o64[sep, 0])
+    assert err.max() <= (1e-5 if shift == 0.0 else 4 * np.spacing(np.float32(shift)))
+
+
+def test_signed_distance_rounding_exits_on_recorded_pairs(oracle):
+    """Pairs recorded from the penalty descent (tests/golden, written by
+    make_sd_rounding_exits.py on the GPU) on which the f32 walk takes a
+    rounding exit: status 5 (an enclosure after the walk proved a gap) or 6
+    (EPA's core depth beyond the walk's own bound, the least support-plane
+    distance -delta / |v| of its iterates, by more than 1e-4 m).  The oracle
+    takes the same exits and lands bit for bit on the GPU's recorded signed
+    distance; the reported closest iterate meets the f64 signed distance to
+    2e-4 m, where EPA on the same simplex had reported depths of 0.02-0.18 m
+    (caught by the planted mistakes sd_gap_proof_off and sd_epa_bound_off)."""
+    from sd_helpers import rounding_exit_scene
+    sc, cases = rounding_exit_scene()
+    o32, st32 = oracle.sd_pairs(sc, sc["pairs"])
+    want = np.array([c["status"] for c in cases])
+    assert np.array_equal(st32, want)
+    assert (want == oracle.SD_SEPARATED).sum() >= 3 and (want == oracle.SD_EPA_CONTRADICTED).sum() >= 3
+    assert np.array_equal(o32[:, 0], np.float32([c["sd_f32_gpu"] for c in cases]))
+    o64, _ = oracle.sd_pairs(sc, sc["pairs"], None, f64=True)
+    assert np.allclose(o64[:, 0], [c["sd_f64"] for c in cases], rtol=0, atol=1e-9)
+    assert np.abs(o32[:, 0] - o64[:, 0]).max() <= 2e-4
 
 
 def test_exact_penetration_places_overlapping_pairs_at_their_depth(oracle, scenes):
