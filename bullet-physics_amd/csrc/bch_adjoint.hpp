@@ -30,7 +30,7 @@ enum : uint32_t {
   SD_WALK = 0,             // Band arm of the walk (separated, or inside the margins)
   SD_EPA = 1,              // EPA converged
   SD_CAPPED = 2,           // walk stopped at its iteration cap (closest iterate)
-  SD_TOUCH = 3,            // cores touch and no enclosing tetrahedron: sd = -(margins), no normal
+  SD_TOUCH = 3,            // cores touch and no enclosing tetrahedron: sd = -(margins), normal from the touching simplex
   SD_EPA_UNCONVERGED = 4,  // EPA ended without converging (reported face)
   SD_SEPARATED = 5,        // walk proved a core gap, then enclosed the origin by rounding (closest iterate)
   SD_EPA_CONTRADICTED = 6, // EPA depth beyond the walk's depth bound (closest iterate)
@@ -68,6 +68,28 @@ BCH_HD SdOut band_outputs(const Pair& pr, V3 v, V3 p_b) {
   o.pb = p_b - o.n * pr.b.margin;
   o.pa = o.pb - o.n * o.sd;
   return o;
+}
+
+// The normal reported for touching cores: the walk's last support direction
+// s with its components along the touching simplex removed (a triangle's
+// face normal, a segment's perpendicular; a single point keeps s).
+BCH_HD V3 sd_touch_normal(const Simp& S, V3 s) {
+  V3 n = s;
+  if (S.n == 3) {
+    V3 c = cross(S.s1.w - S.s0.w, S.s2.w - S.s0.w);
+    float c2 = len2(c);
+    if (c2 > 0.0f) n = c * (dot(c, s) / c2);
+  } else if (S.n == 2) {
+    V3 e = S.s1.w - S.s0.w;
+    float e2 = len2(e);
+    if (e2 > 0.0f) n = s - e * (dot(s, e) / e2);
+  }
+  float n2 = len2(n);
+  if (!(n2 > 0.0f)) {
+    n = s;
+    n2 = len2(s);
+  }
+  return n / fsqrt(n2);
 }
 
 // Core support point `idx` of a shape in local coordinates (the finite set
@@ -153,7 +175,12 @@ BCH_HD AffineD affine_weights_d(const D3* w, int k) {
   return A;
 }
 
-BCH_HD void band_vjp(const Pair& pr, const Simp& act, const float* g10, PoseGrad& G) {
+// touch_dir (touching cores, SD_TOUCH): the witnesses move with their bodies
+// and the reported normal n is held fixed, d sd = n . (d b - d a) over the
+// touching simplex's weights; n is rebuilt in double from the simplex and
+// the walk's last support direction as the forward builds it in f32.
+BCH_HD void band_vjp(const Pair& pr, const Simp& act, const float* g10, PoseGrad& G,
+                     const V3* touch_dir = nullptr) {
   int k = act.n;
   const SV* sv[3] = {&act.s0, &act.s1, &act.s2};
   D3 xa[3], xb[3], w[3], b[3];
@@ -167,7 +194,7 @@ BCH_HD void band_vjp(const Pair& pr, const Simp& act, const float* g10, PoseGrad
   // the least-norm point's feature: drop vertices whose exact weight is not
   // positive (the f32 walk can keep one at rounding level)
   AffineD A = affine_weights_d(w, k);
-  for (int pass = 0; pass < 2 && k > 1; ++pass) {
+  for (int pass = 0; pass < 2 && k > 1 && !touch_dir; ++pass) {
     int drop = -1;
     double low = 0.0;
     for (int i = 0; i < 3; ++i)
@@ -193,11 +220,31 @@ BCH_HD void band_vjp(const Pair& pr, const Simp& act, const float* g10, PoseGrad
   D3 nv = v * (1.0 / core);
   D3 n = -nv;
   double sd = core - (ma + mb);
+  if (touch_dir) {
+    D3 s = d3(*touch_dir);
+    n = s;
+    if (k == 3) {
+      D3 c = crossd(w[1] - w[0], w[2] - w[0]);
+      double c2 = dotd(c, c);
+      if (c2 > 0.0) n = c * (dotd(c, s) / c2);
+    } else if (k == 2) {
+      D3 e = w[1] - w[0];
+      double e2 = dotd(e, e);
+      if (e2 > 0.0) n = s - e * (dotd(s, e) / e2);
+    }
+    double n2 = dotd(n, n);
+    if (!(n2 > 0.0)) {
+      n = s;
+      n2 = dotd(s, s);
+    }
+    n = n * (1.0 / sqrt(n2));
+    sd = -(ma + mb);
+  }
   D3 gn{g10[1], g10[2], g10[3]}, gpa{g10[4], g10[5], g10[6]}, gpb{g10[7], g10[8], g10[9]};
   D3 g_pbt = gpb + gpa;
   D3 g_nt = (gn - gpa * sd) - g_pbt * mb;
   double g_sdt = (double)g10[0] - dotd(n, gpa);
-  D3 g_v = nv * g_sdt - (g_nt - n * dotd(n, g_nt)) * (1.0 / core);
+  D3 g_v = touch_dir ? n * (-g_sdt) : nv * g_sdt - (g_nt - n * dotd(n, g_nt)) * (1.0 / core);
   D3 gw[3], gb[3];
   double glam[3];
   for (int i = 0; i < 3; ++i) {
@@ -205,7 +252,7 @@ BCH_HD void band_vjp(const Pair& pr, const Simp& act, const float* g10, PoseGrad
     gb[i] = g_pbt * A.lam[i];
     glam[i] = dotd(w[i], g_v) + dotd(b[i], g_pbt);
   }
-  if (k >= 2) {
+  if (k >= 2 && !touch_dir) {
     // g_mu_j = g_lam_j - g_lam_0; u = M^-1 g_mu;
     // g_D = -(v u^T + (D u) mu^T), g_w0 -= D u
     double gm0 = glam[1] - glam[0], gm1 = k == 3 ? glam[2] - glam[0] : 0.0;
@@ -370,6 +417,11 @@ __global__ __launch_bounds__(256) void k_sd_walk(uint32_t P, const uint32_t* __r
         } else {
           o.sd = -(pr.a.margin + pr.b.margin);
           o.n = o.pa = o.pb = splat(0.0f);
+          if (g.touch) {
+            o.n = sd_touch_normal(g.act, g.last_v);
+            o.pb = g.last_pb - o.n * pr.b.margin;
+            o.pa = o.pb - o.n * o.sd;
+          }
         }
         if (status) status[t] = (uint8_t)st;
         if (!BWD) {
@@ -377,7 +429,7 @@ __global__ __launch_bounds__(256) void k_sd_walk(uint32_t P, const uint32_t* __r
         } else {
           PoseGrad Gr;
           Gr.zero();
-          if (band) band_vjp(pr, g.act, gin + 10 * (uint64_t)t, Gr);
+          if (band || g.touch) band_vjp(pr, g.act, gin + 10 * (uint64_t)t, Gr, band ? nullptr : &g.last_v);
           store_grad(out + 14 * (uint64_t)t, Gr);
         }
       }

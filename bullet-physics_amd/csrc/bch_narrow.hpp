@@ -750,6 +750,12 @@ struct GjkResult {
   float depth_bound;
   V3 last_v, last_pb;
   Simp act;
+  // Signed-distance walks: the cores touched (|v|^2 below the touch gate)
+  // and the enclosing walk found no tetrahedron, where the reference reports
+  // Separated.  act then holds the touching simplex, last_pb its B witness
+  // and last_v the walk's last support direction (the initial direction on
+  // the first iteration), from which the reported normal is taken.
+  bool touch;
 };
 
 // gjk.rs:171-288 -- the margin-core distance walk.
@@ -761,7 +767,7 @@ BCH_HD GjkResult gjk_contact_t(const Pair& pr, bool has_seed, V3 seed, float thr
   GjkResult R;
   R.kind = 0;
   if (SD) {
-    R.capped = R.separated = false;
+    R.capped = R.separated = R.touch = false;
     R.depth_bound = INFINITY;
     R.last_v = R.last_pb = V3{0.0f, 0.0f, 0.0f};
     R.act.n = 0;
@@ -779,6 +785,7 @@ BCH_HD GjkResult gjk_contact_t(const Pair& pr, bool has_seed, V3 seed, float thr
   float touch2 = 1e-12f * fmax_nan(len2(first.w), 1.0f);
   float best_v2 = INFINITY;
   bool proven_gap = false;
+  V3 search = initial;  // the direction of the latest support query (SD only)
   for (int it = 0; it < 64; ++it) {
     Reduce r = reduce(S);
     V3 v = r.v;
@@ -804,10 +811,16 @@ BCH_HD GjkResult gjk_contact_t(const Pair& pr, bool has_seed, V3 seed, float thr
         R.t1 = W.p1;
         R.t2 = W.p2;
         R.t3 = W.p3;
+      } else if (SD) {
+        R.touch = true;
+        R.act = S;
+        R.last_v = search;
+        R.last_pb = r.p_b;
       }
       return R;
     }
     dir = -v;
+    if (SD) search = dir;
     SV w = sup(dir);
     float delta = dot(v, w.w);
     float band = margin_sum + threshold;

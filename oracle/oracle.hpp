@@ -771,6 +771,14 @@ struct GjkOut {
   // the active simplex (w, b) at a Band exit, or of that iterate at the cap
   SupportVertex<T> active[4];
   int n_active = 0;
+  // Also not in the reference (signed-distance walks): the cores touched
+  // (|v|^2 below the touch gate) and the enclosing walk found no
+  // tetrahedron, where the reference reports Separated.  The touching
+  // simplex is kept in active[] with its B witness in last_pb, and
+  // touch_dir is the walk's last support direction (the initial direction
+  // on the first iteration), the normal reported for the touch.
+  bool touch = false;
+  V3<T> touch_dir{0, 0, 0};
 };
 template <class T>
 struct Reduce {
@@ -1116,6 +1124,7 @@ GjkOut<T> gjk_contact(const PairPose<T>& pr, const V3<T>* seed, T threshold, boo
   int n = 1;
   V3<T> dir = -first.w;
   T touch2 = ORL(1e-12) * std::fmax(len2(first.w), T(1));
+  V3<T> search = initial;  // the direction of the latest support query
   const T REL_ERROR2 = ORL(1e-6);
   T best_v2 = std::numeric_limits<T>::infinity();
   const T gap2 = T(SD_GAP_PROOF) * T(SD_GAP_PROOF);
@@ -1145,10 +1154,18 @@ GjkOut<T> gjk_contact(const PairPose<T>& pr, const V3<T>* seed, T threshold, boo
         for (int i = 0; i < 4; ++i) out.simplex[i] = s.pts[i];
       } else {
         out.kind = 0;
+        if (sd_walk) {
+          out.touch = true;
+          out.touch_dir = search;
+          for (int i = 0; i < n; ++i) out.active[i] = buf[i];
+          out.n_active = n;
+          out.last_pb = r.p_b;
+        }
       }
       return out;
     }
     dir = -v;
+    search = dir;
     SupportVertex<T> w = pr.core_support(dir);
     T delta = dot(v, w.w);
     T band = margin_sum + threshold;
@@ -1417,7 +1434,7 @@ enum SdStatus {
   SD_WALK = 0,
   SD_EPA = 1,
   SD_CAPPED = 2,
-  SD_TOUCH = 3,
+  SD_TOUCH = 3,            // cores touch, no enclosing tetrahedron: sd = -(margins), normal = last search direction
   SD_EPA_UNCONVERGED = 4,
   SD_SEPARATED = 5,        // the walk proved a core gap, then enclosed the origin by rounding
   SD_EPA_CONTRADICTED = 6  // EPA's depth exceeds the walk's depth bound
@@ -1428,6 +1445,25 @@ struct SdOut {
   T sd = 0;
   V3<T> n{0, 0, 0}, pa{0, 0, 0}, pb{0, 0, 0};
 };
+// The normal reported for touching cores: the walk's last support direction
+// with its components along the touching simplex removed (a triangle's face
+// normal, a segment's perpendicular; a single point keeps the direction).
+template <class T>
+V3<T> sd_touch_normal(const GjkOut<T>& g) {
+  V3<T> s = g.touch_dir, n = s;
+  if (g.n_active == 3) {
+    V3<T> c = cross(g.active[1].w - g.active[0].w, g.active[2].w - g.active[0].w);
+    T c2 = len2(c);
+    if (c2 > T(0)) n = c * (dot(c, s) / c2);
+  } else if (g.n_active == 2) {
+    V3<T> e = g.active[1].w - g.active[0].w;
+    T e2 = len2(e);
+    if (e2 > T(0)) n = s - e * (dot(s, e) / e2);
+  }
+  T n2 = len2(n);
+  if (!(n2 > T(0))) n = s, n2 = len2(s);
+  return n / std::sqrt(n2);
+}
 template <class T>
 SdOut<T> signed_distance(const PairPose<T>& pr) {
   SdOut<T> o;
@@ -1467,6 +1503,13 @@ SdOut<T> signed_distance(const PairPose<T>& pr) {
   }
   o.status = SD_TOUCH;
   o.sd = -(ma + mb);
+  if (g.touch) {
+    // the cores touch: core distance 0, normal along the walk's last support
+    // direction, witnesses from the touching simplex
+    o.n = sd_touch_normal(g);
+    o.pb = g.last_pb - o.n * mb;
+    o.pa = o.pb - o.n * o.sd;
+  }
   return o;
 }
 
@@ -1544,7 +1587,11 @@ bool sd_vjp(const PairPose<T>& pr, const T* g10, T* g14) {
     e = epa_penetration<T>(g.simplex, sup);
     contradicted = e.depth > g.depth_bound + T(SD_EPA_SLACK);
   }
-  if (g.kind == 1 || (g.kind == 0 && g.capped) || contradicted) {
+  // touching cores (SD_TOUCH): the witnesses move with their bodies and the
+  // reported normal is held fixed, d sd = n . (d b - d a) over the touching
+  // simplex's weights
+  bool touch = g.kind == 0 && g.touch;
+  if (g.kind == 1 || (g.kind == 0 && g.capped) || contradicted || touch) {
     int k = g.n_active;
     V3<T> w[3], b[3];
     for (int i = 0; i < k && i < 3; ++i) {
@@ -1577,12 +1624,12 @@ bool sd_vjp(const PairPose<T>& pr, const T* g10, T* g14) {
     V3<T> v = splat(T(0));
     for (int i = 0; i < k; ++i) v = v + w[i] * lam[i];
     T core = std::sqrt(len2(v));
-    V3<T> n = (-v) / core;
-    T sd = core - (ma + mb);
+    V3<T> n = touch ? sd_touch_normal(g) : (-v) / core;
+    T sd = touch ? -(ma + mb) : core - (ma + mb);
     V3<T> g_pbt = gpb + gpa;
     V3<T> g_nt = (gn - gpa * sd) - g_pbt * mb;
     T g_sdt = g10[0] - dot(n, gpa);
-    V3<T> g_v = (v / core) * g_sdt - (g_nt - n * dot(n, g_nt)) / core;
+    V3<T> g_v = touch ? n * (-g_sdt) : (v / core) * g_sdt - (g_nt - n * dot(n, g_nt)) / core;
     V3<T> gw[3], gb[3];
     T glam[3];
     for (int i = 0; i < 3; ++i) {
@@ -1590,7 +1637,7 @@ bool sd_vjp(const PairPose<T>& pr, const T* g10, T* g14) {
       gb[i] = g_pbt * lam[i];
       glam[i] = i < k ? dot(w[i], g_v) + dot(b[i], g_pbt) : T(0);
     }
-    if (k >= 2) {
+    if (k >= 2 && !touch) {
       T gm0 = glam[1] - glam[0], gm1 = k == 3 ? glam[2] - glam[0] : T(0);
       T u0 = k == 3 ? i00 * gm0 + i01 * gm1 : i00 * gm0;
       T u1 = k == 3 ? i01 * gm0 + i11 * gm1 : T(0);

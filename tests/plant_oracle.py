@@ -119,6 +119,14 @@ MUTATIONS = [
      "signed-distance extension: an EPA depth beyond the walk's depth bound is not reported",
      "    contradicted = e.depth > g.depth_bound + T(SD_EPA_SLACK);\n  }\n  if (g.kind == 1 || (g.kind == 0 && g.capped) || contradicted) {\n    if (g.kind == 1) {",
      "    contradicted = false && e.depth > g.depth_bound;\n  }\n  if (g.kind == 1 || (g.kind == 0 && g.capped) || contradicted) {\n    if (g.kind == 1) {"),
+    ("sd_touch_no_normal",
+     "signed-distance extension: touching cores report a normal and witnesses",
+     "  if (g.touch) {\n    // the cores touch",
+     "  if ((false)) {\n    // the cores touch"),
+    ("sd_touch_raw_direction",
+     "signed-distance extension: the touch normal drops the simplex's in-plane components",
+     "  V3<T> s = g.touch_dir, n = s;\n  if (g.n_active == 3) {",
+     "  V3<T> s = g.touch_dir, n = s;\n  if ((false)) {"),
     ("capped_last_iterate",
      "signed-distance extension: a capped walk reports its closest iterate, not the last",
      "    if (!(v2 >= best_v2)) {",

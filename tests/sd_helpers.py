@@ -84,3 +84,55 @@ def rounding_exit_scene():
     sc["hull_vertices"] = np.concatenate(verts).reshape(-1, 3) if verts else np.zeros((0, 3), np.float32)
     sc["pairs"] = np.arange(n, dtype=np.uint32).reshape(-1, 2)
     return sc, cases
+
+
+def touch_scene(oracle, trials=300, seed=1):
+    """Box-hull pairs whose cores touch: a box (half-extents U[0.25, 1] m,
+    margin 0.04) at the origin and a hull of 24 points on an ellipsoid
+    (semi-axes U[0.3, 1.2] m, margin 0) moved along a random direction u to
+    where the f64 core distance crosses zero (80 bisection steps), both at
+    random orientations.  Kept: the pairs whose f32 walk ends on the touch
+    gate with no enclosing tetrahedron (status SD_TOUCH).  Returns a scene
+    with pairs (2k, 2k + 1) and sc["direction"] = u."""
+    rng = np.random.default_rng(seed)
+    keep = []
+    for _ in range(trials):
+        he = rng.uniform(0.25, 1.0, 3).astype(np.float32)
+        d = rng.normal(size=(24, 3))
+        pts = (d / np.linalg.norm(d, axis=1, keepdims=True) * rng.uniform(0.3, 1.2, 3)).astype(np.float32)
+        qa = rng.normal(size=4)
+        qb = rng.normal(size=4)
+        qa = (qa / np.linalg.norm(qa)).astype(np.float32)
+        qb = (qb / np.linalg.norm(qb)).astype(np.float32)
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        a, b = oracle.box(*he), oracle.hull(pts)
+        lo, hi = 0.0, 4.0
+        for _ in range(80):
+            m = 0.5 * (lo + hi)
+            out, _ = oracle.signed_distance(a, np.zeros(3), qa, b, m * u, qb, f64=True)
+            if out[0] + 0.04 < 0.0:
+                lo = m
+            else:
+                hi = m
+        pb = (hi * u).astype(np.float32)
+        _, st = oracle.signed_distance(a, np.zeros(3), qa, b, pb, qb)
+        if st == oracle.SD_TOUCH:
+            keep.append((he, pts, qa, qb, pb, u))
+    n = 2 * len(keep)
+    sc = dict(position=np.zeros((n, 3), np.float32), orientation=np.zeros((n, 4), np.float32),
+              shape_type=np.zeros(n, np.uint8), shape_params=np.zeros((n, 4), np.float32),
+              hull_first=np.zeros(n, np.uint32), hull_count=np.zeros(n, np.uint32))
+    for k, (he, pts, qa, qb, pb, u) in enumerate(keep):
+        sc["orientation"][2 * k] = qa
+        sc["orientation"][2 * k + 1] = qb
+        sc["position"][2 * k + 1] = pb
+        sc["shape_type"][2 * k] = 1
+        sc["shape_params"][2 * k, :3] = he
+        sc["shape_type"][2 * k + 1] = 3
+        sc["hull_first"][2 * k + 1] = 24 * k
+        sc["hull_count"][2 * k + 1] = 24
+    sc["hull_vertices"] = (np.concatenate([k[1] for k in keep]) if keep else np.zeros((0, 3))).astype(np.float32)
+    sc["pairs"] = np.arange(n, dtype=np.uint32).reshape(-1, 2)
+    sc["direction"] = np.array([k[5] for k in keep]).reshape(-1, 3)
+    return sc
