@@ -119,7 +119,7 @@ __global__ void k_pack_bodies(uint32_t n, const float* __restrict__ pos, const f
   ShapeRef s = shape_of(b, hull);
   b.threshold = breaking_threshold(s);
   b.world = world_of ? world_of[i] : 0u;
-  b.pad = 0u;
+  b.bound = bound_radius(s) + b.threshold;
   out[i] = b;
 }
 
@@ -279,12 +279,19 @@ __device__ __forceinline__ bool keeps_aabb(const uint8_t* bflags, uint32_t i, in
 __global__ void k_aabb(uint32_t n, const BodyRec* __restrict__ bodies, const float4* __restrict__ hull, float margin,
                        float4* __restrict__ aabb, uint8_t* __restrict__ flags, uint32_t* __restrict__ big_ids,
                        Counters* cnt, float* __restrict__ partial, const uint8_t* __restrict__ bflags = nullptr,
-                       int keep = 0) {
+                       int keep = 0, float4* __restrict__ bsph = nullptr) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
   uint32_t reg = 0;
   BodyRec b;
   if (i < n) b = bodies[i];
+  if (i < n && bsph) {
+    // the body's bounding ball at its current pose (every body, also one
+    // that keeps its previous AABB); NaN radius for a non-finite pose
+    float s = ((b.px + b.py) + b.pz) + (((b.qx + b.qy) + b.qz) + b.qw);
+    bool finite = s - s == 0.0f;
+    bsph[i] = make_float4(b.px, b.py, b.pz, finite ? b.bound : NAN);
+  }
   if (i < n && (b.type & 0xffu) != HULL) {  // hull boxes: k_hull_aabb
     Box6 bx;
     if (keeps_aabb(bflags, i, keep)) {
@@ -703,9 +710,24 @@ __device__ __forceinline__ uint32_t dev_count(uint32_t n, const uint32_t* n_dev)
 __device__ __forceinline__ uint32_t cls_key(uint32_t c, uint32_t sub) { return c | (((c << 5) | sub) << 8); }
 __device__ __forceinline__ uint32_t cls_of_key(uint32_t k) { return (k >> 13) & 7u; }
 
+// The sort class of walk-class pairs whose bounding balls are apart by more
+// than the band: no kernel runs for them (their count is written here).
+constexpr uint32_t kClsNoKernel = 7;
+
+// With bsph (the bodies' bounding balls of this pass), a walk-class pair
+// (GJK, GJK_HULL) whose centres are farther apart than the two radii
+// (each grown by its contact_breaking_threshold) plus a rounding allowance
+// cannot end in the reference's Band or Overlap arms: the walk's closest
+// point v is a point of the core Minkowski difference, so |v| exceeds the
+// margins plus the band (gjk.rs:236-244, 261-275).  Its result is None
+// (count 0) and it is sorted into kClsNoKernel; its route keeps its class.
+// The rest of a walk class is grouped by shape types, hull size and whether
+// the centres lie within 0.85 of the radii (deeper pairs walk longer), so the
+// lanes of a wavefront take the same support code and similar walk lengths.
 __global__ void k_split_pairs(uint32_t P, const uint32_t* __restrict__ P_dev, const uint64_t* __restrict__ keys,
                               const uint8_t* __restrict__ btype, uint32_t* __restrict__ pairs,
-                              uint32_t* __restrict__ cls, uint32_t* __restrict__ iota) {
+                              uint32_t* __restrict__ cls, uint32_t* __restrict__ iota,
+                              const float4* __restrict__ bsph = nullptr, uint32_t* __restrict__ ccount = nullptr) {
   P = dev_count(P, P_dev);
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
@@ -715,11 +737,33 @@ __global__ void k_split_pairs(uint32_t P, const uint32_t* __restrict__ P_dev, co
   pairs[2 * p + 1] = j;
   const uint32_t ta = btype[i], tb = btype[j];
   const uint32_t c = (uint32_t)pair_class((int)(ta & 7u), (int)(tb & 7u));
-  // within the walk classes the pairs are grouped by their shape types and
-  // hull size, so that the lanes of a wavefront take the same support code
-  const uint32_t sub =
-      (c == CLS_GJK_HULL || c == CLS_GJK) ? (((ta & 3u) << 2) | (tb & 3u) | ((ta | tb) & 0x10u)) : 0u;
-  cls[p] = cls_key(c, sub);
+  uint32_t sort_c = c, sub = 0u;
+  if (c == CLS_GJK_HULL || c == CLS_GJK) {
+    bool deep = false;
+    if (bsph) {
+      float4 A = bsph[i], B = bsph[j];
+      float dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+      float d2 = (dx * dx + dy * dy) + dz * dz;
+      float lim = A.w + B.w;
+      float mag = fmaxf(fmaxf(fmaxf(fabsf(A.x), fabsf(A.y)), fmaxf(fabsf(A.z), fabsf(B.x))),
+                        fmaxf(fabsf(B.y), fabsf(B.z)));
+      float far = lim * (1.0f + 1e-5f) + 1e-5f * (1.0f + mag);
+      if (d2 > far * far) {  // NaN never passes
+        sort_c = kClsNoKernel;
+        if (ccount) ccount[p] = 0u;
+      }
+      deep = d2 < (0.85f * lim) * (0.85f * lim);
+    }
+    const uint32_t ta3 = ta & 3u, tb3 = tb & 3u;
+    if (c == CLS_GJK_HULL) {
+      // 7 type combinations (one shape is a hull), the large-hull bit, deep
+      const uint32_t combo = ta3 == (uint32_t)HULL ? tb3 : 4u + ta3;
+      sub = (combo << 2) | (((ta | tb) & 0x10u) >> 3) | (deep ? 1u : 0u);
+    } else {
+      sub = (ta3 << 2) | tb3 | (deep ? 0x10u : 0u);
+    }
+  }
+  cls[p] = c | (((sort_c << 5) | sub) << 8);
   iota[p] = p;
 }
 

@@ -167,6 +167,7 @@ struct bch_context {
 
   // pass buffers
   DevBuf aabb, flags, big_ids, counters, partials, keys, keys_alt, vals, vals_alt, nodes, wend, hull_ids;
+  DevBuf bsph;  // per body: position and bound (bounding ball), written by k_aabb each pass
   uint32_t n_hulls = 0;
   DevBuf walk_regions, wave_counts, wave_offs, pair_keys, pair_keys_alt, pairs, cls, cls_alt, perm,
       perm_alt, ccount, coffsets, cslots, contacts, epa_q, epa_q2, scan_part, rs_hist, rs_offs;
@@ -542,12 +543,13 @@ int ensure_pass_buffers(bch_context* c, uint32_t n) {
   while (N < n) N <<= 1;
   ENSURE(c, c->nodes, (size_t)N * 4 * sizeof(float4));
   ENSURE(c, c->wend, (size_t)(n + 1) * sizeof(uint32_t));
+  ENSURE(c, c->bsph, (size_t)n * sizeof(float4));
   return BCH_OK;
 }
 
 // First part of the narrowphase: buffers, and the pair split of
 // c->pair_keys into (i, j), shape classes and the identity permutation.
-int narrow_split(bch_context* c) {
+int narrow_split(bch_context* c, const float4* bsph = nullptr) {
   hipStream_t s = c->stream;
   Counters* cnt = c->counters.as<Counters>();
   uint64_t P = c->P;
@@ -568,7 +570,7 @@ int narrow_split(bch_context* c) {
   if (P)
     hipLaunchKernelGGL(k_split_pairs, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, P_dev,
                        c->pair_keys.as<uint64_t>(), c->btype.as<uint8_t>(), c->pairs.as<uint32_t>(),
-                       c->cls.as<uint32_t>(), c->perm.as<uint32_t>());
+                       c->cls.as<uint32_t>(), c->perm.as<uint32_t>(), bsph, c->ccount.as<uint32_t>());
   return BCH_OK;
 }
 
@@ -706,7 +708,7 @@ int narrow_finish(bch_context* c, const float* d_seeds, bool dense = true) {
 // Narrowphase over c->pair_keys (sorted, c->P pairs) with the world bodies.
 // Fills c->pairs (2P), c->coffsets (P+1), c->contacts (7C), c->C.
 int run_narrowphase(bch_context* c, const float* d_seeds) {
-  int rc = narrow_split(c);
+  int rc = narrow_split(c, c->bsph.as<float4>());
   if (rc) return rc;
   return narrow_finish(c, d_seeds);
 }
@@ -797,7 +799,7 @@ void run_aabbs(bch_context* c, float aabb_margin) {
   int keep = c->aabb_valid ? 1 : 0;
   hipLaunchKernelGGL(k_aabb, dim3(blocks(n)), dim3(kBlock), 0, s, n, c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                      aabb_margin, c->aabb.as<float4>(), c->flags.as<uint8_t>(), c->big_ids.as<uint32_t>(),
-                     c->counters.as<Counters>(), c->partials.as<float>(), bf, keep);
+                     c->counters.as<Counters>(), c->partials.as<float>(), bf, keep, c->bsph.as<float4>());
   uint32_t hb = c->n_hulls ? blocks(c->n_hulls, kBlock / 16) : 0;
   float* hpart = c->partials.as<float>() + 8 * (size_t)blocks(n);
   if (c->n_hulls)
@@ -1861,7 +1863,7 @@ int bch_world_manifold_pass(bch_context* c, float aabb_margin, float linear_slop
   ENSURE(c, c->moff, (Pm + 1) * sizeof(uint32_t));
   ENSURE(c, c->mslots, Pm * 12 * sizeof(float4));
   ENSURE(c, c->mseeds, Pm * 3 * sizeof(float));
-  rc = narrow_split(c);
+  rc = narrow_split(c, c->bsph.as<float4>());
   if (rc) return rc;
   int cur = c->mcur;
   if (P)

@@ -18,14 +18,17 @@ enum : int { SPHERE = 0, BOX = 1, CAPSULE = 2, HULL = 3, PLANE = 4 };
 // 64-byte body record kept resident in HBM (one cache-line half per body):
 //   r0 = position.xyz, type   r1 = orientation (x, y, z, w)
 //   r2 = shape params (hull: bits(first vertex), bits(vertex count))
-//   r3 = contact_breaking_threshold, margin, bits(world), 0
+//   r3 = contact_breaking_threshold, margin, bits(world), bound
+// bound = bound_radius (a ball about the position holding the shape and its
+// margin-grown core) + contact_breaking_threshold.
 struct alignas(16) BodyRec {
   float px, py, pz;
   uint32_t type;
   float qx, qy, qz, qw;
   float a0, a1, a2, a3;
   float threshold, margin;
-  uint32_t world, pad;
+  uint32_t world;
+  float bound;
 };
 
 struct ShapeRef {
@@ -193,6 +196,38 @@ BCH_HD Box6 world_aabb(const ShapeRef& s, V3 p, Q4 q) {
       return Box6{-b, b};
     }
   }
+}
+
+// Radius of a ball about the body's position that holds both the full shape
+// and the core grown by the margin (the rounded shape the margin-core walk
+// measures): sphere r, capsule hh + r, box max(|h|, |core h| + 0.04), hull
+// the farthest vertex (margin 0), plane +inf.  Not a reference quantity: the
+// pair split uses it to skip walks that cannot end within the band.  NaN
+// when a parameter is not finite (no pair of the body is skipped then).
+BCH_HD float bound_radius(const ShapeRef& s) {
+  float r;
+  if (s.type == SPHERE) {
+    r = s.a0;
+  } else if (s.type == CAPSULE) {
+    r = s.a1 + s.a0;
+  } else if (s.type == BOX) {
+    V3 h{s.a0, s.a1, s.a2};
+    V3 hc{fmaxf(s.a0 - 0.04f, 0.0f), fmaxf(s.a1 - 0.04f, 0.0f), fmaxf(s.a2 - 0.04f, 0.0f)};
+    float rh = sqrtf(len2(h)), rc = sqrtf(len2(hc)) + 0.04f;
+    r = rh > rc ? rh : rc;
+  } else if (s.type == HULL) {
+    float r2 = 0.0f;
+    bool finite = true;
+    for (int v = 0; v < s.nhull; ++v) {
+      float q = len2(V3{s.hull[v].x, s.hull[v].y, s.hull[v].z});
+      finite = finite && q <= 3.0e38f;
+      r2 = q > r2 ? q : r2;
+    }
+    r = finite ? sqrtf(r2) : NAN;
+  } else {
+    return INFINITY;
+  }
+  return r >= 0.0f && r <= 3.0e38f ? r : NAN;
 }
 
 // Local AABB and contact_breaking_threshold (shapes/mod.rs:254-270, 388-392).
