@@ -485,3 +485,80 @@ def test_predicted_poses_and_sleeping_aabbs(bpa, scenes, oracle, ctx):
     _, mn2, mx2 = oracle.scene_aabbs_predicted(moved, vel, dt, MARGIN, flags=flags, prev=None)
     assert not np.array_equal(oracle.sweep_pairs(mn2, mx2), ref_pairs)
     w.ctx.close()
+
+
+def _facing_quat(v, u):
+    """Unit quaternion (x, y, z, w) turning direction v onto direction u."""
+    v = v / np.linalg.norm(v)
+    u = u / np.linalg.norm(u)
+    c = np.cross(v, u)
+    q = np.array([c[0], c[1], c[2], 1.0 + float(v @ u)])
+    return q / np.linalg.norm(q)
+
+
+def test_walk_pairs_at_the_bounding_ball_edge(bpa, oracle, ctx):
+    """Walk-class pairs (hull, box and capsule combinations) turned so that
+    each body's farthest point faces the other, placed at gaps of -0.5 to 3
+    times the pair's contact_breaking_threshold and at the split's
+    bounding-ball cut itself (the radii plus both thresholds, +-0.2 %): the
+    pairs the split skips as farther apart than the band, and the ones it
+    keeps, give the oracle's pair set and contacts (a wrongly skipped pair
+    would lose its band contact)."""
+    _flush("case: walk-class pairs at the bounding-ball edge")
+    rng = np.random.default_rng(11)
+    kinds = [(3, 3), (1, 3), (3, 1), (2, 3), (3, 2), (1, 2), (2, 2)]
+    fracs = [-0.5, 0.25, 0.5, 0.9, 0.999, 1.001, 1.5, 3.0]
+    bodies = []
+    for k in range(224):
+        ta, tb = kinds[k % len(kinds)]
+        pair = []
+        for t in (ta, tb):
+            if t == 3:
+                d = rng.normal(size=(24, 3))
+                pts = (d / np.linalg.norm(d, axis=1, keepdims=True) * rng.uniform(0.3, 1.2, 3)).astype(np.float32)
+                far = pts[np.argmax(np.linalg.norm(pts.astype(np.float64), axis=1))].astype(np.float64)
+                pair.append(dict(t=3, pts=pts, far=far, R=float(np.linalg.norm(far)), s=oracle.hull(pts)))
+            elif t == 1:
+                h = rng.uniform(0.25, 1.0, 3).astype(np.float32)
+                pair.append(dict(t=1, h=h, far=h.astype(np.float64), R=float(np.linalg.norm(h)), s=oracle.box(*h)))
+            else:
+                r, hh = np.float32(rng.uniform(0.1, 0.5)), np.float32(rng.uniform(0.2, 1.0))
+                pair.append(dict(t=2, r=r, hh=hh, far=np.array([0.0, 1.0, 0.0]), R=float(hh + r),
+                                 s=oracle.capsule(r, hh)))
+        A, B = pair
+        ta_thr, tb_thr = oracle.threshold(A["s"]), oracle.threshold(B["s"])
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        pa = 6.0 * np.array([k % 8, (k // 8) % 8, k // 64]) - 21.0  # near the origin: a small rounding allowance
+        if k % 4 == 3:  # at the cut: radii plus both thresholds, +-0.2 %
+            d = (A["R"] + B["R"] + ta_thr + tb_thr) * (1.0 + rng.uniform(-2e-3, 2e-3))
+        else:
+            d = A["R"] + B["R"] + fracs[(k // 4) % len(fracs)] * min(ta_thr, tb_thr)
+        A["p"], A["q"] = pa, _facing_quat(A["far"], u)
+        B["p"], B["q"] = pa + d * u, _facing_quat(B["far"], -u)
+        bodies += [A, B]
+    n = len(bodies)
+    sc = dict(position=np.zeros((n, 3), np.float32), orientation=np.zeros((n, 4), np.float32),
+              shape_type=np.zeros(n, np.uint8), shape_params=np.zeros((n, 4), np.float32),
+              hull_first=np.zeros(n, np.uint32), hull_count=np.zeros(n, np.uint32))
+    verts = []
+    for i, b in enumerate(bodies):
+        sc["position"][i] = b["p"]
+        sc["orientation"][i] = b["q"]
+        sc["shape_type"][i] = b["t"]
+        if b["t"] == 3:
+            sc["hull_first"][i] = 24 * len(verts)
+            sc["hull_count"][i] = 24
+            verts.append(b["pts"])
+        elif b["t"] == 1:
+            sc["shape_params"][i, :3] = b["h"]
+        else:
+            sc["shape_params"][i, :2] = (b["r"], b["hh"])
+    sc["hull_vertices"] = np.concatenate(verts).astype(np.float32)
+    ref_pairs, ref_cnt, ref_pts, _ = _oracle_pass(oracle, sc, MARGIN)
+    w = bpa.CollisionWorld(sc, ctx)
+    w.collide(MARGIN)
+    pairs, offs, cont = w.results()
+    assert np.array_equal(pairs, ref_pairs)
+    assert ref_pairs.shape[0] >= 150 and (ref_cnt > 0).sum() >= 60 and (ref_cnt == 0).sum() >= 40
+    compare_contacts(ref_cnt, ref_pts, offs, cont, label="bounding-ball edge")
