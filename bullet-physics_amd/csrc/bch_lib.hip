@@ -306,10 +306,12 @@ int dev_scan(bch_context* c, const uint32_t* in, uint32_t* out, uint64_t n, DevB
   uint32_t nb = (uint32_t)std::max<uint64_t>(1, (n + kScanTile - 1) / kScanTile);
   ENSURE(c, part, (size_t)nb * sizeof(uint32_t));
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanThreads), 0, s, in, (uint32_t)n, n_dev, part.as<uint32_t>());
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, part.as<uint32_t>(), nb, out, (uint32_t)n, n_dev,
-                     total_copy);
+  const int prefixed = nb > kScanSelfTiles ? 1 : 0;
+  if (prefixed)
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, s, part.as<uint32_t>(), nb, out, (uint32_t)n, n_dev,
+                       total_copy);
   hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanThreads), 0, s, in, (uint32_t)n, n_dev, part.as<uint32_t>(),
-                     out);
+                     out, total_copy, prefixed);
   return BCH_OK;
 }
 

@@ -4,9 +4,12 @@
 //
 // Scan (u32, n items -> n + 1 offsets, the last one the total):
 //   k_scan_reduce   per-block sums of 2048-item tiles
-//   k_scan_top      one block: exclusive scan of the tile sums (any count),
-//                   writes the total at out[n]
-//   k_scan_apply    per-tile exclusive scan plus the tile's offset
+//   k_scan_apply    per-tile exclusive scan plus the tile's offset (the sum
+//                   of the earlier tiles' sums, which every block adds up
+//                   itself); the last tile writes the total at out[n]
+//   k_scan_top      for more than kScanSelfTiles tiles only: one block scans
+//                   the tile sums in place first (and writes the total), and
+//                   k_scan_apply reads its offset from there
 // An optional device count n_dev caps n (a count an earlier kernel of the
 // same sequence wrote), so a fixed launch shape serves any count up to n.
 // Radix sort (u64 keys, optional u32 values, 8-bit digits, stable):
@@ -100,11 +103,24 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_top(uint32_t* __restrict_
   }
 }
 
+// Launched with the same grid as k_scan_reduce (nb blocks).  With
+// `prefixed` the partials already hold the tiles' offsets (k_scan_top ran).
+constexpr uint32_t kScanSelfTiles = 4096;
 __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t n,
                                                              const uint32_t* __restrict__ n_dev,
                                                              const uint32_t* __restrict__ partials,
-                                                             uint32_t* __restrict__ out) {
+                                                             uint32_t* __restrict__ out,
+                                                             uint32_t* __restrict__ total_copy, int prefixed) {
   n = capped_count(n, n_dev);
+  // this tile's offset: the sum of the earlier tiles' sums
+  uint32_t off = 0;
+  if (prefixed) {
+    off = partials[blockIdx.x];
+  } else {
+    uint32_t before = 0;
+    for (uint32_t b = threadIdx.x; b < blockIdx.x; b += kScanThreads) before += partials[b];
+    (void)block_exclusive_scan(before, off);
+  }
   uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint32_t s = 0;
@@ -114,11 +130,15 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __r
     s += v[k];
   }
   uint32_t tot;
-  uint32_t run = block_exclusive_scan(s, tot) + partials[blockIdx.x];
+  uint32_t run = block_exclusive_scan(s, tot) + off;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     if (base + k < n) out[base + k] = run;
     run += v[k];
+  }
+  if (!prefixed && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    out[n] = off + tot;
+    if (total_copy) total_copy[0] = off + tot;
   }
 }
 
