@@ -649,49 +649,70 @@ __global__ void k_pair_counts(Counters* cnt, const uint32_t* __restrict__ tree_t
 
 // Bodies with unbounded boxes (planes) against every body of their world,
 // brute force; pairs appended after the tree walk's *tree_total pairs (the
-// count, Counters::n_big_pairs, still grows past `cap`).
-__global__ void k_big(const Counters* __restrict__ cnt, const uint32_t* __restrict__ big_ids,
-                      const float4* __restrict__ aabb, const uint8_t* __restrict__ flags,
-                      const BodyRec* __restrict__ bodies, const uint32_t* __restrict__ world_offsets, uint32_t n,
-                      uint64_t* __restrict__ out, uint64_t cap, const uint32_t* __restrict__ tree_total,
-                      Counters* cnt_rw) {
+// count, Counters::n_big_pairs, still grows past `cap`).  One block takes
+// a big body and a chunk of kBigChunk bodies of its world: its pairs are
+// ranked by ballots within the wavefronts and by wavefront counts in LDS,
+// staged there, and claimed with one atomic per block (one counter shared
+// by every block: an atomic per wavefront serialised 27k same-address
+// atomics over 4096 planes).
+constexpr uint32_t kBigChunk = 2048;
+__global__ __launch_bounds__(256) void k_big(const Counters* __restrict__ cnt, const uint32_t* __restrict__ big_ids,
+                                             const float4* __restrict__ aabb, const uint8_t* __restrict__ flags,
+                                             const BodyRec* __restrict__ bodies,
+                                             const uint32_t* __restrict__ world_offsets, uint32_t n,
+                                             uint64_t* __restrict__ out, uint64_t cap,
+                                             const uint32_t* __restrict__ tree_total, Counters* cnt_rw) {
+  __shared__ uint64_t stage[kBigChunk];
+  __shared__ uint32_t wcnt[4];
+  __shared__ uint32_t base_s;
   const uint64_t t0 = *tree_total;
-  uint32_t nbig = cnt->n_big;
+  const uint32_t nbig = cnt->n_big;
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (uint32_t bi = blockIdx.y; bi < nbig; bi += gridDim.y) {
-    uint32_t b = big_ids[bi];
+    const uint32_t b = big_ids[bi];
     uint32_t w0 = 0, w1 = n;
     if (world_offsets && bodies) {
       uint32_t w = bodies[b].world;
       w0 = world_offsets[w];
       w1 = world_offsets[w + 1];
     }
-    float4 blo = aabb[2 * b], bhi = aabb[2 * b + 1];
-    Box6 bb{V3{blo.x, blo.y, blo.z}, V3{bhi.x, bhi.y, bhi.z}};
-    for (uint32_t base = w0 + blockIdx.x * blockDim.x; base < w1; base += gridDim.x * blockDim.x) {
-      uint32_t j = base + threadIdx.x;
-      bool emit = false;
-      if (j < w1 && j != b) {
-        uint8_t f = flags[j];
-        if (f == FLAG_REG || (f == FLAG_BIG && j > b)) {
-          float4 lo = aabb[2 * j], hi = aabb[2 * j + 1];
-          Box6 o{V3{lo.x, lo.y, lo.z}, V3{hi.x, hi.y, hi.z}};
-          emit = overlaps(bb, o);
+    const float4 blo = aabb[2 * b], bhi = aabb[2 * b + 1];
+    const Box6 bb{V3{blo.x, blo.y, blo.z}, V3{bhi.x, bhi.y, bhi.z}};
+    for (uint32_t c0 = w0 + blockIdx.x * kBigChunk; c0 < w1; c0 += gridDim.x * kBigChunk) {
+      const uint32_t c1 = c0 + kBigChunk < w1 ? c0 + kBigChunk : w1;
+      uint32_t fill = 0;  // block-uniform
+      for (uint32_t base = c0; base < c1; base += blockDim.x) {
+        const uint32_t j = base + threadIdx.x;
+        bool emit = false;
+        if (j < c1 && j != b) {
+          const uint8_t f = flags[j];
+          if (f == FLAG_REG || (f == FLAG_BIG && j > b)) {
+            const float4 lo = aabb[2 * j], hi = aabb[2 * j + 1];
+            emit = overlaps(bb, Box6{V3{lo.x, lo.y, lo.z}, V3{hi.x, hi.y, hi.z}});
+          }
         }
-      }
-      uint64_t mask = __ballot(emit);
-      if (mask) {
-        uint32_t lane = threadIdx.x & 63;
-        uint32_t leader = __ffsll((unsigned long long)mask) - 1;
-        uint32_t base_idx = 0;
-        if (lane == leader) base_idx = atomicAdd(&cnt_rw->n_big_pairs, (uint32_t)__popcll(mask));
-        base_idx = __shfl(base_idx, leader);
+        const uint64_t mask = __ballot(emit);
+        if (lane == 0) wcnt[wv] = (uint32_t)__popcll(mask);
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (uint32_t k = 0; k < blockDim.x / 64; ++k) {
+          before += k < wv ? wcnt[k] : 0u;
+          total += wcnt[k];
+        }
         if (emit) {
-          uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
-          uint64_t idx = t0 + base_idx + rank;
-          uint32_t lo = b < j ? b : j, hi = b < j ? j : b;
-          if (idx < cap) out[idx] = ((uint64_t)lo << 32) | hi;
+          const uint32_t lo_id = b < j ? b : j, hi_id = b < j ? j : b;
+          stage[fill + before + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] =
+              ((uint64_t)lo_id << 32) | hi_id;
         }
+        fill += total;
+        __syncthreads();
       }
+      if (threadIdx.x == 0) base_s = fill ? atomicAdd(&cnt_rw->n_big_pairs, fill) : 0u;
+      __syncthreads();
+      const uint64_t at = t0 + base_s;
+      for (uint32_t k = threadIdx.x; k < fill; k += blockDim.x)
+        if (at + k < cap) out[at + k] = stage[k];
+      __syncthreads();
     }
   }
 }

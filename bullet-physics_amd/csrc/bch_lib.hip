@@ -405,7 +405,7 @@ int walk_pairs(bch_context* c, uint32_t n, const uint32_t* d_world_offsets, cons
                      c->pair_keys.as<uint64_t>());
   // k_big reads the big-body count on the device; an empty list exits at once
   uint32_t span = c->batched ? c->max_world : n;
-  uint32_t gx = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(span), 1024u));
+  uint32_t gx = std::max<uint32_t>(1u, std::min<uint32_t>((span + kBigChunk - 1) / kBigChunk, 1024u));
   uint32_t gy = std::max<uint32_t>(1u, std::min<uint32_t>(c->n_planes, 65535u));
   hipLaunchKernelGGL(k_big, dim3(gx, gy), dim3(kBlock), 0, s, cnt, c->big_ids.as<uint32_t>(), c->aabb.as<float4>(),
                      c->flags.as<uint8_t>(), c->bodies.as<BodyRec>(), d_world_offsets, n, c->pair_keys.as<uint64_t>(),
@@ -458,12 +458,18 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   int wbits = d_world_of ? bits_for(c->n_worlds) : 0;
   int end_bit = std::min(64, 31 + wbits);
   // all-ones keys of excluded bodies still sort last within end_bit bits.
-  // The lowest 7 code bits are not sorted: a cell of 2^7 finest Morton
-  // cells holds ~0.1 bodies at 1M bodies, so the leaf order (and the tree)
-  // barely changes, and the pair set is exact for any leaf order.  At 1M
-  // bodies that is 3 digit passes instead of 4.
+  // The lowest code bits are not sorted: they leave 2^(30 - drop) >= 8 m
+  // sorted cells for the m bodies of the largest world (drop = 7 at 1M
+  // bodies in one world, 16 for worlds of at most 2048 bodies), so a cell
+  // holds ~0.1 bodies of a world whose bodies spread over the scene bounds,
+  // the leaf order (and the tree) barely changes, and the pair set is exact
+  // for any leaf order.  At 1M bodies that is 3 digit passes instead of 4,
+  // and 4 instead of 5 for 4096 worlds.
+  uint32_t m = d_world_of ? std::max<uint32_t>(c->max_world, 1u) : std::max<uint32_t>(n, 1u);
+  int drop = 30 - bits_for(8ull * m - 1);
+  drop = std::max(0, std::min(drop, 24));
   int shifts[8];
-  int ns = digit_shifts(7, end_bit, shifts);
+  int ns = digit_shifts(drop, end_bit, shifts);
   bool alt = false;
   int rc = dev_radix_sort<uint64_t>(c, c->keys.as<uint64_t>(), c->keys_alt.as<uint64_t>(), c->vals.as<uint32_t>(),
                                     c->vals_alt.as<uint32_t>(), n, shifts, ns, s, &alt);
