@@ -363,9 +363,20 @@ int bits_for(uint64_t v) {
 }
 
 // The pair sort's digits: keys are i << 32 | j with i, j < n < 2^b, sorted
-// as the packed 2b-bit number i << b | j (rs_digit's pack = b).
-int pair_key_shifts(uint32_t n, int* ps, int* pack) {
+// as the packed 2b-bit number i << b | j (rs_digit's pack = b).  In a batch
+// every pair lies inside one world, so j - i < max_world <= 2^w, and the
+// key is sorted as the (b + w)-bit number i << w | (j - i) when that is
+// shorter (pack = w | kPackDelta): 4 digit passes instead of 6 for 4096
+// worlds of at most 2048 bodies.
+int pair_key_shifts(const bch_context* c, uint32_t n, int* ps, int* pack) {
   int bn = bits_for(n);
+  if (c->batched && c->max_world > 0) {
+    int bw = bits_for(c->max_world - 1);
+    if (bw < bn) {
+      *pack = bw | kPackDelta;
+      return digit_shifts(0, bn + bw, ps);
+    }
+  }
   *pack = bn;
   return digit_shifts(0, 2 * bn, ps);
 }
@@ -431,7 +442,7 @@ int broadphase_capacity(bch_context* c, uint32_t n, const uint32_t* d_world_offs
   c->P = cap;
   int ps[16];
   int pack = 0;
-  int np = pair_key_shifts(n, ps, &pack);
+  int np = pair_key_shifts(c, n, ps, &pack);
   bool palt = false;
   rc = dev_radix_sort<uint64_t>(c, c->pair_keys.as<uint64_t>(), c->pair_keys_alt.as<uint64_t>(), nullptr, nullptr, cap,
                                 ps, np, s, &palt, &cnt->n_pairs, pack);
@@ -527,7 +538,7 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   if (c->P > 1) {
     int ps[16];
     int pack = 0;
-    int np = pair_key_shifts(n, ps, &pack);
+    int np = pair_key_shifts(c, n, ps, &pack);
     bool palt = false;
     rc = dev_radix_sort<uint64_t>(c, c->pair_keys.as<uint64_t>(), c->pair_keys_alt.as<uint64_t>(), nullptr, nullptr,
                                   c->P, ps, np, s, &palt, nullptr, pack);

@@ -144,11 +144,19 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(const uint32_t* __r
 
 // With pack = b > 0 the key i << 32 | j (i, j < 2^b) is read as the 2b-bit
 // number i << b | j, so a pair sort needs ceil(2b / 8) digits, not
-// 2 ceil(b / 8).
+// 2 ceil(b / 8).  With pack = b | kPackDelta the key is read as
+// i << b | (j - i), which orders the keys the same way when every j - i is
+// below 2^b (pairs inside worlds of at most 2^b bodies).
+constexpr int kPackDelta = 0x100;
 template <class K>
 __device__ __forceinline__ uint32_t rs_digit(K key, int shift, int pack) {
   uint64_t k = (uint64_t)key;
-  if (pack) k = (k & ((1ull << pack) - 1ull)) | ((k >> 32) << pack);
+  if (pack & kPackDelta) {
+    const int b = pack & 0xff;
+    k = ((k >> 32) << b) | (((k & 0xffffffffull) - (k >> 32)) & ((1ull << b) - 1ull));
+  } else if (pack) {
+    k = (k & ((1ull << pack) - 1ull)) | ((k >> 32) << pack);
+  }
   return (uint32_t)(k >> shift) & 0xffu;
 }
 
