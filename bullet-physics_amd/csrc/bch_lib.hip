@@ -338,10 +338,10 @@ int dev_radix_sort(bch_context* c, K* k0, K* k1, uint32_t* v0, uint32_t* v1, uin
     if (rc) return rc;
     if (v0)
       hipLaunchKernelGGL((k_rs_scatter<K, true>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n, n_dev,
-                         shifts[p], pack, tiles, c->rs_offs.as<uint32_t>(), kout, vout);
+                         shifts[p], pack, tiles, c->rs_hist.as<uint32_t>(), c->rs_offs.as<uint32_t>(), kout, vout);
     else
       hipLaunchKernelGGL((k_rs_scatter<K, false>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n, n_dev,
-                         shifts[p], pack, tiles, c->rs_offs.as<uint32_t>(), kout, vout);
+                         shifts[p], pack, tiles, c->rs_hist.as<uint32_t>(), c->rs_offs.as<uint32_t>(), kout, vout);
     std::swap(kin, kout);
     std::swap(vin, vout);
   }

@@ -178,21 +178,35 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const K* __restrict__ ke
   hist[(uint64_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// counts = the per-tile digit counts k_rs_hist wrote ([256][tiles]), offs =
+// their scan (the global position of each tile's first item of a digit).
+// The tile's items are first placed in LDS in digit order (stable), then
+// written out from there: consecutive LDS slots of one digit go to
+// consecutive global positions, so the stores come in runs (a wavefront's
+// direct stores hit up to 64 separate lines otherwise).
 template <class K, bool VALS>
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                            uint32_t n, const uint32_t* __restrict__ n_dev, int shift,
-                                                           int pack, uint32_t tiles, const uint32_t* __restrict__ offs,
-                                                           K* __restrict__ kout, uint32_t* __restrict__ vout) {
+                                                           int pack, uint32_t tiles, const uint32_t* __restrict__ counts,
+                                                           const uint32_t* __restrict__ offs, K* __restrict__ kout,
+                                                           uint32_t* __restrict__ vout) {
   n = capped_count(n, n_dev);
-  __shared__ uint32_t running[256];
+  __shared__ K skey[kRsTile];
+  __shared__ uint32_t sval[VALS ? kRsTile : 1];
+  __shared__ uint32_t running[256], lstart[256], gstart[256];
   __shared__ uint32_t wcount[kRsThreads / 64][256];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  running[threadIdx.x] = offs[(uint64_t)threadIdx.x * tiles + blockIdx.x];
+  const uint64_t ti = (uint64_t)threadIdx.x * tiles + blockIdx.x;
+  uint32_t total;
+  const uint32_t ls = block_exclusive_scan(counts[ti], total);
+  lstart[threadIdx.x] = ls;
+  running[threadIdx.x] = ls;
+  gstart[threadIdx.x] = offs[ti];
 #pragma unroll
   for (int k = 0; k < kRsThreads / 64; ++k) wcount[k][threadIdx.x] = 0u;
   __syncthreads();
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  uint64_t base = (uint64_t)blockIdx.x * kRsTile;
+  const uint64_t base = (uint64_t)blockIdx.x * kRsTile;
   for (int r = 0; r < kRsRounds; ++r) {
     uint64_t i = base + (uint64_t)r * kRsThreads + threadIdx.x;
     bool valid = i < n;
@@ -215,8 +229,8 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const K* __restrict__
       uint32_t pos = running[d] + rank;
       for (int k = 0; k < kRsThreads / 64; ++k)
         if (k < w) pos += wcount[k][d];
-      kout[pos] = key;
-      if (VALS) vout[pos] = val;
+      skey[pos] = key;
+      if (VALS) sval[pos] = val;
     }
     __syncthreads();
     uint32_t add = 0u;
@@ -227,6 +241,14 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const K* __restrict__
     }
     running[threadIdx.x] += add;
     __syncthreads();
+  }
+  // the tile's `total` items, in digit order
+  for (uint32_t k = threadIdx.x; k < total; k += kRsThreads) {
+    K key = skey[k];
+    uint32_t d = rs_digit(key, shift, pack);
+    uint32_t pos = gstart[d] + (k - lstart[d]);
+    kout[pos] = key;
+    if (VALS) vout[pos] = sval[k];
   }
 }
 
