@@ -50,11 +50,11 @@ class _Context:
 
     _KNOBS = {"hull_lanes": _lib.TUNE_HULL_LANES,
               "traverse_block": _lib.TUNE_TRAVERSE_BLOCK, "epa_grid": _lib.TUNE_EPA_GRID, "np_grid": _lib.TUNE_NP_GRID,
-              "epa_wave_faces": _lib.TUNE_EPA_WAVE_FACES}
+              "epa_wave_faces": _lib.TUNE_EPA_WAVE_FACES, "sort_tile": _lib.TUNE_SORT_TILE}
 
     def set_tuning(self, **knobs):
         """Launch-shape knobs (hull_lanes, traverse_block, epa_grid,
-        np_grid, epa_wave_faces); they never change results."""
+        np_grid, epa_wave_faces, sort_tile); they never change results."""
         for k, v in knobs.items():
             if k not in self._KNOBS:
                 raise KeyError(f"unknown tuning knob {k!r}")

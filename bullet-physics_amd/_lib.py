@@ -32,7 +32,7 @@ EXPORTS = [
 ]
 
 # launch-shape knobs (bch_set_tuning)
-TUNE_HULL_LANES, TUNE_TRAVERSE_BLOCK, TUNE_EPA_GRID, TUNE_NP_GRID, TUNE_EPA_WAVE_FACES = 1, 2, 3, 4, 5
+TUNE_HULL_LANES, TUNE_TRAVERSE_BLOCK, TUNE_EPA_GRID, TUNE_NP_GRID, TUNE_EPA_WAVE_FACES, TUNE_SORT_TILE = 1, 2, 3, 4, 5, 6
 
 
 class BchBodies(C.Structure):

@@ -176,6 +176,7 @@ struct bch_context {
   uint32_t np_grid = 2048;  // narrowphase blocks per shape class
   int epa_wave_faces = 256;  // face store the one-pair EPA may fill before the overflow route
   uint32_t trav_block = 256;  // threads per tree-walk block (64, 128, 256, 512)
+  uint32_t sort_tile = 2048;  // items per radix-sort tile (2048 or 4096; the static batch keeps 4096)
   uint64_t tree_cap = 0;      // pair buffer of the eager pass, sized by earlier passes
   uint32_t walk_R = 1024;     // pair slots of one tree-walk wavefront's region
   uint32_t walk_nw = 0;       // wavefronts of the last walk
@@ -318,13 +319,34 @@ int dev_scan(bch_context* c, const uint32_t* in, uint32_t* out, uint64_t n, DevB
 // Stable LSD radix sort of n keys (with values when v0 != nullptr) over the
 // 8-bit digits starting at the given bit shifts.  Ping-pongs between (k0, v0)
 // and (k1, v1); *in_second tells which pair holds the result.
+template <class K, int ROUNDS>
+int radix_pass(bch_context* c, K* kin, K* kout, uint32_t* vin, uint32_t* vout, bool vals, uint64_t n,
+               const uint32_t* n_dev, int shift, int pack, uint32_t tiles, hipStream_t s) {
+  hipLaunchKernelGGL((k_rs_hist<K, ROUNDS>), dim3(tiles), dim3(kRsThreads), 0, s, kin, (uint32_t)n, n_dev, shift, pack,
+                     tiles, c->rs_hist.as<uint32_t>());
+  int rc = dev_scan(c, c->rs_hist.as<uint32_t>(), c->rs_offs.as<uint32_t>(), (uint64_t)256 * tiles, c->scan_part, s);
+  if (rc) return rc;
+  if (vals)
+    hipLaunchKernelGGL((k_rs_scatter<K, true, ROUNDS>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n,
+                       n_dev, shift, pack, tiles, c->rs_hist.as<uint32_t>(), c->rs_offs.as<uint32_t>(), kout, vout);
+  else
+    hipLaunchKernelGGL((k_rs_scatter<K, false, ROUNDS>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n,
+                       n_dev, shift, pack, tiles, c->rs_hist.as<uint32_t>(), c->rs_offs.as<uint32_t>(), kout, vout);
+  return BCH_OK;
+}
+
+// Stable LSD radix sort of n keys (with values when v0 != nullptr) over the
+// 8-bit digits starting at the given bit shifts.  Ping-pongs between (k0, v0)
+// and (k1, v1); *in_second tells which pair holds the result.  The tile
+// (items per histogram and scatter block) is the context's sort_tile knob.
 template <class K>
 int dev_radix_sort(bch_context* c, K* k0, K* k1, uint32_t* v0, uint32_t* v1, uint64_t n, const int* shifts,
                    int nshift, hipStream_t s, bool* in_second, const uint32_t* n_dev = nullptr, int pack = 0) {
   *in_second = false;
   if (n <= 1 || nshift == 0) return BCH_OK;
   if (n > 0xffffffffull) return fail(c, BCH_INVALID_ARG, "sort longer than 2^32 - 1");
-  uint32_t tiles = (uint32_t)((n + kRsTile - 1) / kRsTile);
+  const uint32_t tile = c->sort_tile == 2048 ? 2048u : (uint32_t)kRsTile;
+  uint32_t tiles = (uint32_t)((n + tile - 1) / tile);
   ENSURE(c, c->rs_hist, (size_t)256 * tiles * sizeof(uint32_t));
   ENSURE(c, c->rs_offs, ((size_t)256 * tiles + 1) * sizeof(uint32_t));
   K* kin = k0;
@@ -332,16 +354,10 @@ int dev_radix_sort(bch_context* c, K* k0, K* k1, uint32_t* v0, uint32_t* v1, uin
   uint32_t* vin = v0;
   uint32_t* vout = v1;
   for (int p = 0; p < nshift; ++p) {
-    hipLaunchKernelGGL(k_rs_hist<K>, dim3(tiles), dim3(kRsThreads), 0, s, kin, (uint32_t)n, n_dev, shifts[p], pack,
-                       tiles, c->rs_hist.as<uint32_t>());
-    int rc = dev_scan(c, c->rs_hist.as<uint32_t>(), c->rs_offs.as<uint32_t>(), (uint64_t)256 * tiles, c->scan_part, s);
+    int rc = tile == 2048
+                 ? radix_pass<K, 8>(c, kin, kout, vin, vout, v0 != nullptr, n, n_dev, shifts[p], pack, tiles, s)
+                 : radix_pass<K, kRsRounds>(c, kin, kout, vin, vout, v0 != nullptr, n, n_dev, shifts[p], pack, tiles, s);
     if (rc) return rc;
-    if (v0)
-      hipLaunchKernelGGL((k_rs_scatter<K, true>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n, n_dev,
-                         shifts[p], pack, tiles, c->rs_hist.as<uint32_t>(), c->rs_offs.as<uint32_t>(), kout, vout);
-    else
-      hipLaunchKernelGGL((k_rs_scatter<K, false>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n, n_dev,
-                         shifts[p], pack, tiles, c->rs_hist.as<uint32_t>(), c->rs_offs.as<uint32_t>(), kout, vout);
     std::swap(kin, kout);
     std::swap(vin, vout);
   }
@@ -1114,6 +1130,7 @@ int bch_create(int device, bch_context** out) {
   bch_context* st = new bch_context();
   st->device = device;
   st->no_alloc = true;
+  st->sort_tile = (uint32_t)kRsTile;  // the carved layout holds kStTiles tiles
   char* b = static_cast<char*>(sp);
   st->bodies.view(b + kStBodies, kStN * sizeof(BodyRec));
   st->btype.view(b + kStBtype, kStN);
@@ -1242,6 +1259,10 @@ int bch_set_tuning(bch_context* c, int knob, int value) {
       if (value < 16 || value > 256) return fail(c, BCH_INVALID_ARG, "one-pair EPA face store must be in [16, 256]");
       c->epa_wave_faces = value;
       return BCH_OK;
+    case BCH_TUNE_SORT_TILE:
+      if (value != 2048 && value != 4096) return fail(c, BCH_INVALID_ARG, "sort tile must be 2048 or 4096 items");
+      c->sort_tile = (uint32_t)value;
+      return BCH_OK;
     default:
       return fail(c, BCH_INVALID_ARG, "unknown tuning knob");
   }
@@ -1317,6 +1338,18 @@ int bch_autotune(bch_context* c, float aabb_margin, int trials, float* out_ms) {
     }
     c->trav_block = keep;
   }
+  {
+    // items per radix-sort tile (the histogram and scatter blocks of every
+    // sort in the pass: Morton codes, pair keys, classes)
+    uint32_t keep = c->sort_tile;
+    for (uint32_t v : {2048u, 4096u}) {
+      c->sort_tile = v;
+      float ms;
+      if ((rc = time_pass(c, aabb_margin, trials, &ms))) return rc;
+      if (ms < best) { best = ms; keep = v; }
+    }
+    c->sort_tile = keep;
+  }
   if (out_ms) *out_ms = best;
   return BCH_OK;
 }
@@ -1329,6 +1362,7 @@ int bch_get_tuning(bch_context* c, int knob, int* value) {
     case BCH_TUNE_EPA_GRID: *value = (int)c->epa_grid; return BCH_OK;
     case BCH_TUNE_NP_GRID: *value = (int)c->np_grid; return BCH_OK;
     case BCH_TUNE_EPA_WAVE_FACES: *value = c->epa_wave_faces; return BCH_OK;
+    case BCH_TUNE_SORT_TILE: *value = (int)c->sort_tile; return BCH_OK;
     default: return fail(c, BCH_INVALID_ARG, "unknown tuning knob");
   }
 }
@@ -1352,6 +1386,7 @@ static int batch_ctx(bch_context* c, bch_context** out) {
   a->epa_grid = c->epa_grid;
   a->np_grid = c->np_grid;
   a->epa_wave_faces = c->epa_wave_faces;
+  a->sort_tile = c->sort_tile;
   *out = a;
   return BCH_OK;
 }
@@ -2525,7 +2560,7 @@ int bch_collide_batch_static(bch_context* c, const bch_bodies* d_bodies, const u
   a->hull_lanes = c->hull_lanes;
   a->epa_grid = c->epa_grid;
   a->np_grid = c->np_grid;
-  a->epa_wave_faces = c->epa_wave_faces;
+  a->epa_wave_faces = c->epa_wave_faces;  // sort_tile stays the carved layout's
   a->cap_mode = false;
   hipStream_t s = a->stream;
   Counters* cnt = a->counters.as<Counters>();

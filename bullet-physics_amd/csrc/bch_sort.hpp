@@ -160,7 +160,9 @@ __device__ __forceinline__ uint32_t rs_digit(K key, int shift, int pack) {
   return (uint32_t)(k >> shift) & 0xffu;
 }
 
-template <class K>
+// ROUNDS rounds of kRsThreads items per tile (the tile size is a tuning
+// knob: 8 or 16 rounds, 2048 or 4096 items).
+template <class K, int ROUNDS = kRsRounds>
 __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const K* __restrict__ keys, uint32_t n,
                                                         const uint32_t* __restrict__ n_dev, int shift, int pack,
                                                         uint32_t tiles,
@@ -169,8 +171,8 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const K* __restrict__ ke
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0u;
   __syncthreads();
-  uint64_t base = (uint64_t)blockIdx.x * kRsTile;
-  for (int r = 0; r < kRsRounds; ++r) {
+  uint64_t base = (uint64_t)blockIdx.x * (kRsThreads * ROUNDS);
+  for (int r = 0; r < ROUNDS; ++r) {
     uint64_t i = base + (uint64_t)r * kRsThreads + threadIdx.x;
     if (i < n) atomicAdd(&h[rs_digit(keys[i], shift, pack)], 1u);
   }
@@ -184,15 +186,15 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const K* __restrict__ ke
 // written out from there: consecutive LDS slots of one digit go to
 // consecutive global positions, so the stores come in runs (a wavefront's
 // direct stores hit up to 64 separate lines otherwise).
-template <class K, bool VALS>
+template <class K, bool VALS, int ROUNDS = kRsRounds>
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                            uint32_t n, const uint32_t* __restrict__ n_dev, int shift,
                                                            int pack, uint32_t tiles, const uint32_t* __restrict__ counts,
                                                            const uint32_t* __restrict__ offs, K* __restrict__ kout,
                                                            uint32_t* __restrict__ vout) {
   n = capped_count(n, n_dev);
-  __shared__ K skey[kRsTile];
-  __shared__ uint32_t sval[VALS ? kRsTile : 1];
+  __shared__ K skey[kRsThreads * ROUNDS];
+  __shared__ uint32_t sval[VALS ? kRsThreads * ROUNDS : 1];
   __shared__ uint32_t running[256], lstart[256], gstart[256];
   __shared__ uint32_t wcount[kRsThreads / 64][256];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -206,8 +208,8 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const K* __restrict__
   for (int k = 0; k < kRsThreads / 64; ++k) wcount[k][threadIdx.x] = 0u;
   __syncthreads();
   const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  const uint64_t base = (uint64_t)blockIdx.x * kRsTile;
-  for (int r = 0; r < kRsRounds; ++r) {
+  const uint64_t base = (uint64_t)blockIdx.x * (kRsThreads * ROUNDS);
+  for (int r = 0; r < ROUNDS; ++r) {
     uint64_t i = base + (uint64_t)r * kRsThreads + threadIdx.x;
     bool valid = i < n;
     K key = valid ? kin[i] : (K)0;

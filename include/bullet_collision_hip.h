@@ -101,13 +101,16 @@ int bch_last_epa_routes(bch_context* ctx, uint32_t* out3);
  *   BCH_TUNE_EPA_WAVE_FACES  faces the one-pair EPA keeps before it hands an
  *                        entry to the 2048-face overflow kernel: 16..256
  *                        (256).  Results do not depend on it; lowering it
- *                        only exercises the overflow route (tests). */
+ *                        only exercises the overflow route (tests).
+ *   BCH_TUNE_SORT_TILE   items per radix-sort tile (histogram and scatter
+ *                        block of every sort): 2048 or 4096 (2048) */
 enum {
   BCH_TUNE_HULL_LANES = 1,
   BCH_TUNE_TRAVERSE_BLOCK = 2,
   BCH_TUNE_EPA_GRID = 3,
   BCH_TUNE_NP_GRID = 4,
-  BCH_TUNE_EPA_WAVE_FACES = 5
+  BCH_TUNE_EPA_WAVE_FACES = 5,
+  BCH_TUNE_SORT_TILE = 6
 };
 int bch_set_tuning(bch_context* ctx, int knob, int value);
 int bch_get_tuning(bch_context* ctx, int knob, int* value);

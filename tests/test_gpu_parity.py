@@ -562,3 +562,24 @@ def test_walk_pairs_at_the_bounding_ball_edge(bpa, oracle, ctx):
     assert np.array_equal(pairs, ref_pairs)
     assert ref_pairs.shape[0] >= 150 and (ref_cnt > 0).sum() >= 60 and (ref_cnt == 0).sum() >= 40
     compare_contacts(ref_cnt, ref_pts, offs, cont, label="bounding-ball edge")
+
+
+def test_sort_tile_keeps_results(bpa, scenes, ctx):
+    """Every sort of the pass (Morton codes, pair keys, shape classes) with
+    2048- and 4096-item tiles (the sort_tile knob): identical pairs, offsets
+    and contacts, on one world and on a batch of worlds."""
+    _flush("case: sort tile 2048 vs 4096")
+    keep = ctx.tuning()["sort_tile"]
+    try:
+        for name, sc in (("mixed", scenes.mixed_convex(30000, 13)), ("batched", scenes.batched_worlds(48, 5))):
+            w = bpa.CollisionWorld(sc, ctx)
+            out = []
+            for tile in (4096, 2048):
+                ctx.set_tuning(sort_tile=tile)
+                w.collide(MARGIN)
+                out.append(w.results())
+            _flush(f"  {name}: {out[0][0].shape[0]} pairs, {out[0][2].shape[0]} contacts")
+            for a, b in zip(*out):
+                assert np.array_equal(a, b)
+    finally:
+        ctx.set_tuning(sort_tile=keep)
