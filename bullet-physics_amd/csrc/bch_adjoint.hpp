@@ -22,6 +22,7 @@
 // difference of the two support sets and matching it exactly (the same
 // arithmetic produced them).
 #pragma once
+#include "bch_epa_quad.hpp"
 #include "bch_epa_wave.hpp"
 
 namespace bch {
@@ -569,6 +570,65 @@ __global__ __launch_bounds__(64) void k_sd_epa(const uint32_t* __restrict__ n_qu
       if (lane == 0) store_grad(out + 14 * (uint64_t)t, Gr);
     }
     if (lane == 0 && status) status[t] = (uint8_t)st;
+    wave_sync();
+  }
+}
+
+// The forward's EPA, four pairs per wavefront: the world pass's four-pair
+// EPA (same algorithm, statement order and tie rules, same budget) on the
+// signed-distance queue.  An entry outside the budget goes whole to defer_q
+// (the same kSdEpaEntry layout), which k_sd_epa<false> then runs from the
+// start, so every result is the one-pair kernel's.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BCH_EPA_QUAD_WAVES))) void k_sd_epa_quad(
+    const uint32_t* __restrict__ n_queue, const float4* __restrict__ epa_q, const uint32_t* __restrict__ pairs,
+    const BodyRec* __restrict__ bodies, const float4* __restrict__ hull, float* __restrict__ out,
+    uint8_t* __restrict__ status, float4* __restrict__ defer_q, uint32_t* n_defer) {
+  __shared__ EpaQuarter S[4];
+  const int lane = threadIdx.x & 63;
+  const int qi = lane >> 4;
+  const int ql = lane & 15;
+  const uint32_t nq = *n_queue;
+  const WorldEpaSrc src{pairs, bodies, hull};
+  for (uint32_t base = 4u * blockIdx.x; base < nq; base += 4u * gridDim.x) {
+    const uint32_t q = base + (uint32_t)qi;
+    const bool has = q < nq;
+    const uint32_t qq = has ? q : base;  // an empty quarter reads the first entry, unused
+    const float4* ent = epa_q + (uint64_t)kSdEpaEntry * qq;
+    const float4 a = ent[0], b = ent[1], c = ent[2], d = ent[3];
+    const uint32_t t = f2u(a.w);
+    const Pair pr = src.pair(t);
+    const bool fits = QuadShapes::fits(pr);
+    QuadShapes qs;
+    qs.load(pr, lane);
+    EpaResult e;
+    bool deferred;
+    quad_epa(qs, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, V3{d.x, d.y, d.z}, S[qi], lane, qi,
+             has && fits, e, deferred);
+    deferred = has && (deferred || !fits);
+    const V3 wa = qs.support_a(e.n, lane);
+    if (ql == 0 && has) {
+      if (deferred) {
+        uint32_t slot = atomicAdd(n_defer, 1u);
+        for (uint32_t k = 0; k < kSdEpaEntry; ++k) defer_q[(uint64_t)kSdEpaEntry * slot + k] = ent[k];
+      } else {
+        const float depth = (e.depth + pr.a.margin) + pr.b.margin;
+        SdOut o;
+        o.sd = -depth;
+        o.n = e.n;
+        o.pa = wa;
+        o.pb = o.pa - e.n * depth;
+        uint32_t st = e.status == 0 ? SD_EPA : SD_EPA_UNCONVERGED;
+        // as in k_sd_epa: a core depth beyond the walk's own bound stands
+        // down for the walk's closest iterate
+        const float4 it_v = ent[4], it_pb = ent[5];
+        if (e.depth > it_v.w + 1e-4f) {
+          st = SD_EPA_CONTRADICTED;
+          o = band_outputs(pr, V3{it_v.x, it_v.y, it_v.z}, V3{it_pb.x, it_pb.y, it_pb.z});
+        }
+        store_out(out + 10 * (uint64_t)t, o);
+        if (status) status[t] = (uint8_t)st;
+      }
+    }
     wave_sync();
   }
 }

@@ -168,6 +168,7 @@ struct bch_context {
   // pass buffers
   DevBuf aabb, flags, big_ids, counters, partials, keys, keys_alt, vals, vals_alt, nodes, wend, hull_ids;
   DevBuf bsph;  // per body: position and bound (bounding ball), written by k_aabb each pass
+  DevBuf sd_defer;  // signed-distance EPA entries the four-pair kernel hands on
   uint32_t n_hulls = 0;
   DevBuf walk_regions, wave_counts, wave_offs, pair_keys, pair_keys_alt, pairs, cls, cls_alt, perm,
       perm_alt, ccount, coffsets, cslots, contacts, epa_q, epa_q2, scan_part, rs_hist, rs_offs;
@@ -978,6 +979,20 @@ int capture_graph(bch_context* c, F body, hipGraph_t* out_g, hipGraphExec_t* out
   return BCH_OK;
 }
 
+// The forward signed distance's EPA over the walk's queue (c->epa_q, count
+// Counters::n_epa): four pairs per wavefront, then the entries it hands on
+// one pair per wavefront (c->sd_defer, count Counters::n_epa_defer).  The
+// queues hold one entry per pair at most, sized by the caller.
+void sd_forward_epa(bch_context* c, const uint32_t* d_pairs, float* d_out, uint8_t* d_status, hipStream_t s) {
+  Counters* cnt = c->counters.as<Counters>();
+  hipLaunchKernelGGL(k_sd_epa_quad, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(), d_pairs,
+                     c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_out, d_status, c->sd_defer.as<float4>(),
+                     &cnt->n_epa_defer);
+  hipLaunchKernelGGL(k_sd_epa<false>, dim3(kEpaDeferGrid), dim3(64), 0, s, &cnt->n_epa_defer,
+                     c->sd_defer.as<float4>(), d_pairs, c->bodies.as<BodyRec>(), c->hull.as<float4>(),
+                     (const float*)nullptr, d_out, d_status);
+}
+
 // One de-penetration step in capacity mode (c->P_cap pairs): poses packed
 // from the caller's arrays, AABBs, broadphase, signed distance of every pair,
 // loss record, then (update) the adjoint of the penetrating pairs, the
@@ -993,6 +1008,7 @@ int run_depen_step(bch_context* c, float aabb_margin, float lr, float* d_pos, fl
   ENSURE(c, c->cls, cap * sizeof(uint32_t));
   ENSURE(c, c->perm, cap * sizeof(uint32_t));
   ENSURE(c, c->epa_q, cap * kSdEpaEntry * sizeof(float4));
+  ENSURE(c, c->sd_defer, cap * kSdEpaEntry * sizeof(float4));
   ENSURE(c, c->sd_out, cap * 10 * sizeof(float));
   ENSURE(c, c->sd_status, cap);
   ENSURE(c, c->pen_pairs, cap * 2 * sizeof(uint32_t));
@@ -1018,9 +1034,7 @@ int run_depen_step(bch_context* c, float aabb_margin, float lr, float* d_pos, fl
   hipLaunchKernelGGL((k_sd_walk<G, false>), dim3(g), dim3(kBlock), 0, s, (uint32_t)cap, P_dev, n,
                      c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(), (const float*)nullptr,
                      c->sd_out.as<float>(), c->sd_status.as<uint8_t>(), c->epa_q.as<float4>(), cnt);
-  hipLaunchKernelGGL(k_sd_epa<false>, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
-                     c->pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(), (const float*)nullptr,
-                     c->sd_out.as<float>(), c->sd_status.as<uint8_t>());
+  sd_forward_epa(c, c->pairs.as<uint32_t>(), c->sd_out.as<float>(), c->sd_status.as<uint8_t>(), s);
   hipLaunchKernelGGL(k_depen_terms, dim3(nb), dim3(kDepenBlock), 0, s, (uint32_t)cap, P_dev, c->sd_out.as<float>(),
                      c->pairs.as<uint32_t>(), c->pen_pairs.as<uint32_t>(), c->pen_g.as<float>(), &cnt->n_pen,
                      c->pen_part.as<float2>());
@@ -2463,6 +2477,7 @@ static int run_sd(bch_context* c, const uint32_t* d_pairs, uint64_t P, const flo
   if (P && !c->n) return fail(c, BCH_INVALID_ARG, "no bodies: call bch_world_set_bodies first");
   HIP_TRY(c, hipSetDevice(c->device));
   ENSURE(c, c->epa_q, std::max<uint64_t>(P, 1) * kSdEpaEntry * sizeof(float4));
+  if (!bwd) ENSURE(c, c->sd_defer, std::max<uint64_t>(P, 1) * kSdEpaEntry * sizeof(float4));
   hipStream_t s = c->stream;
   Counters* cnt = c->counters.as<Counters>();
   hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
@@ -2481,8 +2496,7 @@ static int run_sd(bch_context* c, const uint32_t* d_pairs, uint64_t P, const flo
                          c->n, d_pairs,
                          c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_gin, d_out, d_status,
                          c->epa_q.as<float4>(), cnt);
-      hipLaunchKernelGGL(k_sd_epa<false>, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
-                         d_pairs, c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_gin, d_out, d_status);
+      sd_forward_epa(c, d_pairs, d_out, d_status, s);
     }
   }
   HIP_TRY(c, hipGetLastError());
