@@ -491,6 +491,43 @@ __device__ __forceinline__ bool wave_split(const Pair& pr, V3 w, V3& xa, V3& xb,
   return false;
 }
 
+// The EPA arm of the VJP for one queue entry `ent` of pair t, after EPA
+// reported (n, depth, status) and, with has_face, the face (w0, w1, w2) of
+// Minkowski vertices; the whole wavefront takes part (the vertex split),
+// lane 0 stores.  A depth beyond the walk's own bound takes the Band arm at
+// the walk's closest iterate instead, as the forward did.
+__device__ __forceinline__ void sd_epa_vjp(const Pair& pr, const float4* ent, uint32_t t, V3 en, float edepth,
+                                           int estatus, bool has_face, V3 w0, V3 w1, V3 w2,
+                                           const float* __restrict__ gin, float* __restrict__ out,
+                                           uint8_t* __restrict__ status, int lane) {
+  uint32_t st = estatus == 0 ? SD_EPA : SD_EPA_UNCONVERGED;
+  const float4 it_v = ent[4], it_pb = ent[5];
+  PoseGrad Gr;
+  Gr.zero();
+  if (edepth > it_v.w + 1e-4f) {
+    st = SD_EPA_CONTRADICTED;
+    float4 ia = ent[6], ib = ent[7];
+    Simp act;
+    act.n = (int)f2u(it_pb.w);
+    act.s0.ia = (int)f2u(ia.x); act.s1.ia = (int)f2u(ia.y); act.s2.ia = (int)f2u(ia.z);
+    act.s0.ib = (int)f2u(ib.x); act.s1.ib = (int)f2u(ib.y); act.s2.ib = (int)f2u(ib.z);
+    band_vjp(pr, act, gin + 10 * (uint64_t)t, Gr);
+  } else if (has_face) {
+    float fd_plane = dot(w0, en);
+    if (!(fd_plane == edepth)) st |= SD_FLAG_EDGE;
+    V3 xa[3], xb[3];
+    bool ok = wave_split(pr, w0, xa[0], xb[0], lane);
+    ok = wave_split(pr, w1, xa[1], xb[1], lane) && ok;
+    ok = wave_split(pr, w2, xa[2], xb[2], lane) && ok;
+    if (!ok) st |= SD_FLAG_SPLIT;
+    if (ok) epa_vjp(pr, en, xa, xb, gin + 10 * (uint64_t)t, Gr);
+  }
+  if (lane == 0) {
+    store_grad(out + 14 * (uint64_t)t, Gr);
+    if (status) status[t] = (uint8_t)st;
+  }
+}
+
 // EPA for the queued pairs (one wavefront each) and the EPA arm of the
 // outputs (BWD = false) or of the VJP (BWD = true).
 template <bool BWD>
@@ -524,65 +561,53 @@ __global__ __launch_bounds__(64) void k_sd_epa(const uint32_t* __restrict__ n_qu
       e = wave_epa(sq, t0, t1, t2, t3, S, lane);
       wa = sq.support_a(e.n, lane);
     }
-    float depth = (e.depth + pr.a.margin) + pr.b.margin;
-    SdOut o;
-    o.sd = -depth;
-    o.n = e.n;
-    o.pa = wa;
-    o.pb = o.pa - e.n * depth;
-    uint32_t st = e.status == 0 ? SD_EPA : SD_EPA_UNCONVERGED;
-    // A core depth beyond the walk's own bound (plus EPA's exit tolerance)
-    // comes from a polytope corrupted by rounding: the closest iterate of
-    // the walk stands, as at the iteration cap.
-    float4 it_v = ent[4], it_pb = ent[5];
-    bool contradicted = e.depth > it_v.w + 1e-4f;
-    if (contradicted) {
-      st = SD_EPA_CONTRADICTED;
-      o = band_outputs(pr, V3{it_v.x, it_v.y, it_v.z}, V3{it_pb.x, it_pb.y, it_pb.z});
-    }
     if (!BWD) {
-      if (lane == 0) store_out(out + 10 * (uint64_t)t, o);
-    } else if (contradicted) {
-      float4 ia = ent[6], ib = ent[7];
-      Simp act;
-      act.n = (int)f2u(it_pb.w);
-      act.s0.ia = (int)f2u(ia.x); act.s1.ia = (int)f2u(ia.y); act.s2.ia = (int)f2u(ia.z);
-      act.s0.ib = (int)f2u(ib.x); act.s1.ib = (int)f2u(ib.y); act.s2.ib = (int)f2u(ib.z);
-      PoseGrad Gr;
-      Gr.zero();
-      band_vjp(pr, act, gin + 10 * (uint64_t)t, Gr);
-      if (lane == 0) store_grad(out + 14 * (uint64_t)t, Gr);
+      float depth = (e.depth + pr.a.margin) + pr.b.margin;
+      SdOut o;
+      o.sd = -depth;
+      o.n = e.n;
+      o.pa = wa;
+      o.pb = o.pa - e.n * depth;
+      uint32_t st = e.status == 0 ? SD_EPA : SD_EPA_UNCONVERGED;
+      // A core depth beyond the walk's own bound (plus EPA's exit tolerance)
+      // comes from a polytope corrupted by rounding: the closest iterate of
+      // the walk stands, as at the iteration cap.
+      float4 it_v = ent[4], it_pb = ent[5];
+      if (e.depth > it_v.w + 1e-4f) {
+        st = SD_EPA_CONTRADICTED;
+        o = band_outputs(pr, V3{it_v.x, it_v.y, it_v.z}, V3{it_pb.x, it_pb.y, it_pb.z});
+      }
+      if (lane == 0) {
+        store_out(out + 10 * (uint64_t)t, o);
+        if (status) status[t] = (uint8_t)st;
+      }
     } else {
-      PoseGrad Gr;
-      Gr.zero();
+      V3 w0 = splat(0.0f), w1 = w0, w2 = w0;
       if (e.face >= 0) {
         uint32_t abc = S.abc[e.face];
-        V3 w0 = lv(S, abc & 0xffu), w1 = lv(S, (abc >> 8) & 0xffu), w2 = lv(S, (abc >> 16) & 0xffu);
-        float fd_plane = dot(w0, e.n);
-        if (!(fd_plane == e.depth)) st |= SD_FLAG_EDGE;
-        V3 xa[3], xb[3];
-        bool ok = wave_split(pr, w0, xa[0], xb[0], lane);
-        ok = wave_split(pr, w1, xa[1], xb[1], lane) && ok;
-        ok = wave_split(pr, w2, xa[2], xb[2], lane) && ok;
-        if (!ok) st |= SD_FLAG_SPLIT;
-        if (ok) epa_vjp(pr, e.n, xa, xb, gin + 10 * (uint64_t)t, Gr);
+        w0 = lv(S, abc & 0xffu);
+        w1 = lv(S, (abc >> 8) & 0xffu);
+        w2 = lv(S, (abc >> 16) & 0xffu);
       }
-      if (lane == 0) store_grad(out + 14 * (uint64_t)t, Gr);
+      sd_epa_vjp(pr, ent, t, e.n, e.depth, e.status, e.face >= 0, w0, w1, w2, gin, out, status, lane);
     }
-    if (lane == 0 && status) status[t] = (uint8_t)st;
     wave_sync();
   }
 }
 
-// The forward's EPA, four pairs per wavefront: the world pass's four-pair
-// EPA (same algorithm, statement order and tie rules, same budget) on the
-// signed-distance queue.  An entry outside the budget goes whole to defer_q
-// (the same kSdEpaEntry layout), which k_sd_epa<false> then runs from the
-// start, so every result is the one-pair kernel's.
+// The signed distance's EPA, four pairs per wavefront: the world pass's
+// four-pair EPA (same algorithm, statement order and tie rules, same budget)
+// on the signed-distance queue.  An entry outside the budget goes whole to
+// defer_q (the same kSdEpaEntry layout), which k_sd_epa then runs from the
+// start, so every result is the one-pair kernel's.  BWD = false writes the
+// outputs; BWD = true writes, per queue entry, the reported normal, depth,
+// status and face (rec, 4 float4) for k_sd_epa_vjp.
+constexpr uint32_t kSdRecFace = 0x100u, kSdRecDeferred = 0x200u;
+template <bool BWD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BCH_EPA_QUAD_WAVES))) void k_sd_epa_quad(
     const uint32_t* __restrict__ n_queue, const float4* __restrict__ epa_q, const uint32_t* __restrict__ pairs,
     const BodyRec* __restrict__ bodies, const float4* __restrict__ hull, float* __restrict__ out,
-    uint8_t* __restrict__ status, float4* __restrict__ defer_q, uint32_t* n_defer) {
+    uint8_t* __restrict__ status, float4* __restrict__ rec, float4* __restrict__ defer_q, uint32_t* n_defer) {
   __shared__ EpaQuarter S[4];
   const int lane = threadIdx.x & 63;
   const int qi = lane >> 4;
@@ -610,7 +635,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BCH_EPA_QUAD
       if (deferred) {
         uint32_t slot = atomicAdd(n_defer, 1u);
         for (uint32_t k = 0; k < kSdEpaEntry; ++k) defer_q[(uint64_t)kSdEpaEntry * slot + k] = ent[k];
-      } else {
+        if (BWD) rec[4 * (uint64_t)q + 1] = make_float4(0.0f, 0.0f, 0.0f, u2f(kSdRecDeferred));
+      } else if (!BWD) {
         const float depth = (e.depth + pr.a.margin) + pr.b.margin;
         SdOut o;
         o.sd = -depth;
@@ -627,8 +653,46 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BCH_EPA_QUAD
         }
         store_out(out + 10 * (uint64_t)t, o);
         if (status) status[t] = (uint8_t)st;
+      } else {
+        V3 w0 = splat(0.0f), w1 = w0, w2 = w0;
+        const bool face = e.face >= 0;
+        if (face) {
+          uint32_t abc = S[qi].abc[e.face];
+          w0 = qv(S[qi], abc & 0xffu);
+          w1 = qv(S[qi], (abc >> 8) & 0xffu);
+          w2 = qv(S[qi], (abc >> 16) & 0xffu);
+        }
+        float4* r = rec + 4 * (uint64_t)q;
+        r[0] = make_float4(e.n.x, e.n.y, e.n.z, e.depth);
+        r[1] = make_float4(w0.x, w0.y, w0.z, u2f((uint32_t)(e.status & 0xff) | (face ? kSdRecFace : 0u)));
+        r[2] = make_float4(w1.x, w1.y, w1.z, 0.0f);
+        r[3] = make_float4(w2.x, w2.y, w2.z, 0.0f);
       }
     }
+    wave_sync();
+  }
+}
+
+// The EPA arm of the VJP for the entries k_sd_epa_quad<true> finished, one
+// wavefront per entry (the face's vertex split runs over its 64 lanes).
+__global__ __launch_bounds__(64) void k_sd_epa_vjp(const uint32_t* __restrict__ n_queue,
+                                                   const float4* __restrict__ epa_q, const float4* __restrict__ rec,
+                                                   const uint32_t* __restrict__ pairs,
+                                                   const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
+                                                   const float* __restrict__ gin, float* __restrict__ out,
+                                                   uint8_t* __restrict__ status) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t nq = *n_queue;
+  for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+    const float4* r = rec + 4 * (uint64_t)q;
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+    const uint32_t flags = f2u(r1.w);
+    if (flags & kSdRecDeferred) continue;  // wavefront-uniform
+    const float4* ent = epa_q + (uint64_t)kSdEpaEntry * q;
+    const uint32_t t = f2u(ent[0].w);
+    const Pair pr = make_pair(bodies[pairs[2 * t]], bodies[pairs[2 * t + 1]], hull);
+    sd_epa_vjp(pr, ent, t, V3{r0.x, r0.y, r0.z}, r0.w, (int)(flags & 0xffu), (flags & kSdRecFace) != 0,
+               V3{r1.x, r1.y, r1.z}, V3{r2.x, r2.y, r2.z}, V3{r3.x, r3.y, r3.z}, gin, out, status, lane);
     wave_sync();
   }
 }

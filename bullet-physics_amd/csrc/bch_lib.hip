@@ -169,6 +169,7 @@ struct bch_context {
   DevBuf aabb, flags, big_ids, counters, partials, keys, keys_alt, vals, vals_alt, nodes, wend, hull_ids;
   DevBuf bsph;  // per body: position and bound (bounding ball), written by k_aabb each pass
   DevBuf sd_defer;  // signed-distance EPA entries the four-pair kernel hands on
+  DevBuf sd_rec;    // the adjoint's EPA faces, one record (4 float4) per queue entry
   uint32_t n_hulls = 0;
   DevBuf walk_regions, wave_counts, wave_offs, pair_keys, pair_keys_alt, pairs, cls, cls_alt, perm,
       perm_alt, ccount, coffsets, cslots, contacts, epa_q, epa_q2, scan_part, rs_hist, rs_offs;
@@ -985,12 +986,30 @@ int capture_graph(bch_context* c, F body, hipGraph_t* out_g, hipGraphExec_t* out
 // queues hold one entry per pair at most, sized by the caller.
 void sd_forward_epa(bch_context* c, const uint32_t* d_pairs, float* d_out, uint8_t* d_status, hipStream_t s) {
   Counters* cnt = c->counters.as<Counters>();
-  hipLaunchKernelGGL(k_sd_epa_quad, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(), d_pairs,
-                     c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_out, d_status, c->sd_defer.as<float4>(),
-                     &cnt->n_epa_defer);
+  hipLaunchKernelGGL(k_sd_epa_quad<false>, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
+                     d_pairs, c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_out, d_status, (float4*)nullptr,
+                     c->sd_defer.as<float4>(), &cnt->n_epa_defer);
   hipLaunchKernelGGL(k_sd_epa<false>, dim3(kEpaDeferGrid), dim3(64), 0, s, &cnt->n_epa_defer,
                      c->sd_defer.as<float4>(), d_pairs, c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                      (const float*)nullptr, d_out, d_status);
+}
+
+// The adjoint's EPA arm over the backward walk's queue: EPA again, four
+// pairs per wavefront, recording each entry's face (c->sd_rec); the VJP of
+// those entries one wavefront each; the entries handed on, whole, one pair
+// per wavefront.  Needs c->sd_rec and c->sd_defer for the queue's length.
+void sd_backward_epa(bch_context* c, const uint32_t* d_pairs, const float* d_gin, float* d_grad, uint8_t* d_status,
+                     hipStream_t s) {
+  Counters* cnt = c->counters.as<Counters>();
+  hipLaunchKernelGGL(k_sd_epa_quad<true>, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
+                     d_pairs, c->bodies.as<BodyRec>(), c->hull.as<float4>(), (float*)nullptr, (uint8_t*)nullptr,
+                     c->sd_rec.as<float4>(), c->sd_defer.as<float4>(), &cnt->n_epa_defer);
+  hipLaunchKernelGGL(k_sd_epa_vjp, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
+                     c->sd_rec.as<float4>(), d_pairs, c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_gin, d_grad,
+                     d_status);
+  hipLaunchKernelGGL(k_sd_epa<true>, dim3(kEpaDeferGrid), dim3(64), 0, s, &cnt->n_epa_defer,
+                     c->sd_defer.as<float4>(), d_pairs, c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_gin, d_grad,
+                     d_status);
 }
 
 // One de-penetration step in capacity mode (c->P_cap pairs): poses packed
@@ -1009,6 +1028,7 @@ int run_depen_step(bch_context* c, float aabb_margin, float lr, float* d_pos, fl
   ENSURE(c, c->perm, cap * sizeof(uint32_t));
   ENSURE(c, c->epa_q, cap * kSdEpaEntry * sizeof(float4));
   ENSURE(c, c->sd_defer, cap * kSdEpaEntry * sizeof(float4));
+  ENSURE(c, c->sd_rec, cap * 4 * sizeof(float4));
   ENSURE(c, c->sd_out, cap * 10 * sizeof(float));
   ENSURE(c, c->sd_status, cap);
   ENSURE(c, c->pen_pairs, cap * 2 * sizeof(uint32_t));
@@ -1046,9 +1066,7 @@ int run_depen_step(bch_context* c, float aabb_margin, float lr, float* d_pos, fl
   hipLaunchKernelGGL((k_sd_walk<G, true>), dim3(g), dim3(kBlock), 0, s, (uint32_t)cap, pen_dev, n,
                      c->pen_pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
                      c->pen_g.as<float>(), c->g14.as<float>(), (uint8_t*)nullptr, c->epa_q.as<float4>(), cnt);
-  hipLaunchKernelGGL(k_sd_epa<true>, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
-                     c->pen_pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
-                     c->pen_g.as<float>(), c->g14.as<float>(), (uint8_t*)nullptr);
+  sd_backward_epa(c, c->pen_pairs.as<uint32_t>(), c->pen_g.as<float>(), c->g14.as<float>(), (uint8_t*)nullptr, s);
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks((uint64_t)n * 7)), dim3(kBlock), 0, s, n * 7,
                      c->body_grad.as<uint32_t>(), 0u);
   float* gpos = c->body_grad.as<float>();
@@ -2477,7 +2495,8 @@ static int run_sd(bch_context* c, const uint32_t* d_pairs, uint64_t P, const flo
   if (P && !c->n) return fail(c, BCH_INVALID_ARG, "no bodies: call bch_world_set_bodies first");
   HIP_TRY(c, hipSetDevice(c->device));
   ENSURE(c, c->epa_q, std::max<uint64_t>(P, 1) * kSdEpaEntry * sizeof(float4));
-  if (!bwd) ENSURE(c, c->sd_defer, std::max<uint64_t>(P, 1) * kSdEpaEntry * sizeof(float4));
+  ENSURE(c, c->sd_defer, std::max<uint64_t>(P, 1) * kSdEpaEntry * sizeof(float4));
+  if (bwd) ENSURE(c, c->sd_rec, std::max<uint64_t>(P, 1) * 4 * sizeof(float4));
   hipStream_t s = c->stream;
   Counters* cnt = c->counters.as<Counters>();
   hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
@@ -2489,8 +2508,7 @@ static int run_sd(bch_context* c, const uint32_t* d_pairs, uint64_t P, const flo
                          c->n, d_pairs,
                          c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_gin, d_out, d_status,
                          c->epa_q.as<float4>(), cnt);
-      hipLaunchKernelGGL(k_sd_epa<true>, dim3(c->epa_grid), dim3(64), 0, s, &cnt->n_epa, c->epa_q.as<float4>(),
-                         d_pairs, c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_gin, d_out, d_status);
+      sd_backward_epa(c, d_pairs, d_gin, d_out, d_status, s);
     } else {
       hipLaunchKernelGGL((k_sd_walk<G, false>), dim3(g), dim3(kBlock), 0, s, (uint32_t)P, (const uint32_t*)nullptr,
                          c->n, d_pairs,
