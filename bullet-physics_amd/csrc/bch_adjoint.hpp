@@ -362,13 +362,19 @@ BCH_HD void store_out(float* o10, const SdOut& o) {
 // the group).  BWD = false writes out[10P]; BWD = true reads gin[10P] and
 // writes grad[14P].  Overlapping cores are queued for the EPA kernel.  P_dev
 // (when given) caps P with a count computed on the device.
+// BWD = true leaves the Band and touch arms' VJP to k_sd_band_vjp: the walk
+// writes, per pair, the final active simplex's local support indices (and
+// the touch direction) to brec (kSdBandRec float4, kind 0 = nothing there:
+// an EPA pair or a zero gradient already stored), so the double-precision
+// adjoint runs one lane per pair instead of on one lane of each group.
+constexpr uint32_t kSdBandRec = 3;
 template <int G, bool BWD>
 __global__ __launch_bounds__(256) void k_sd_walk(uint32_t P, const uint32_t* __restrict__ P_dev, uint32_t nbodies,
                                                  const uint32_t* __restrict__ pairs,
                                                  const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
                                                  const float* __restrict__ gin, float* __restrict__ out,
                                                  uint8_t* __restrict__ status, float4* __restrict__ epa_q,
-                                                 Counters* cnt) {
+                                                 Counters* cnt, float4* __restrict__ brec = nullptr) {
   P = dev_count(P, P_dev);
   __shared__ float4 hstage[256 / G][2][64];
   const uint32_t per_block = blockDim.x / G;
@@ -387,6 +393,7 @@ __global__ __launch_bounds__(256) void k_sd_walk(uint32_t P, const uint32_t* __r
       // a pair index outside the bodies: NaN outputs, error status
       if (status) status[t] = 0xffu;
       for (int k = 0; k < (BWD ? 14 : 10); ++k) out[(BWD ? 14 : 10) * (uint64_t)t + k] = NAN;
+      if (BWD) brec[kSdBandRec * (uint64_t)t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     if (t < P && !bad) {
       Pair pr = make_pair(bodies[i], bodies[j], hull);
@@ -427,13 +434,20 @@ __global__ __launch_bounds__(256) void k_sd_walk(uint32_t P, const uint32_t* __r
         if (status) status[t] = (uint8_t)st;
         if (!BWD) {
           store_out(out + 10 * (uint64_t)t, o);
+        } else if (band || g.touch) {
+          float4* r = brec + kSdBandRec * (uint64_t)t;
+          r[0] = make_float4(u2f((band ? 1u : 2u) | ((uint32_t)g.act.n << 8)), u2f((uint32_t)g.act.s0.ia),
+                             u2f((uint32_t)g.act.s1.ia), u2f((uint32_t)g.act.s2.ia));
+          r[1] = make_float4(u2f((uint32_t)g.act.s0.ib), u2f((uint32_t)g.act.s1.ib), u2f((uint32_t)g.act.s2.ib), 0.0f);
+          r[2] = make_float4(g.last_v.x, g.last_v.y, g.last_v.z, 0.0f);
         } else {
           PoseGrad Gr;
           Gr.zero();
-          if (band || g.touch) band_vjp(pr, g.act, gin + 10 * (uint64_t)t, Gr, band ? nullptr : &g.last_v);
           store_grad(out + 14 * (uint64_t)t, Gr);
+          brec[kSdBandRec * (uint64_t)t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
       }
+      if (BWD && sub == 0 && need_epa) brec[kSdBandRec * (uint64_t)t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -462,6 +476,39 @@ __global__ __launch_bounds__(256) void k_sd_walk(uint32_t P, const uint32_t* __r
                                    u2f((uint32_t)g.act.s2.ib), 0.0f);
       }
     }
+  }
+}
+
+// The Band and touch arms' VJP of the pairs k_sd_walk<G, true> recorded, one
+// lane per pair (the same band_vjp, on the recorded simplex indices).
+__global__ __launch_bounds__(256) void k_sd_band_vjp(uint32_t P, const uint32_t* __restrict__ P_dev,
+                                                     const float4* __restrict__ brec,
+                                                     const uint32_t* __restrict__ pairs,
+                                                     const BodyRec* __restrict__ bodies,
+                                                     const float4* __restrict__ hull, const float* __restrict__ gin,
+                                                     float* __restrict__ out) {
+  P = dev_count(P, P_dev);
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < P; t += gridDim.x * blockDim.x) {
+    const float4* r = brec + kSdBandRec * (uint64_t)t;
+    const float4 r0 = r[0];
+    const uint32_t h = f2u(r0.x);
+    const uint32_t kind = h & 3u;
+    if (kind == 0u) continue;
+    const float4 r1 = r[1], r2 = r[2];
+    Simp act;
+    act.n = (int)(h >> 8);
+    act.s0.ia = (int)f2u(r0.y);
+    act.s1.ia = (int)f2u(r0.z);
+    act.s2.ia = (int)f2u(r0.w);
+    act.s0.ib = (int)f2u(r1.x);
+    act.s1.ib = (int)f2u(r1.y);
+    act.s2.ib = (int)f2u(r1.z);
+    const Pair pr = make_pair(bodies[pairs[2 * (uint64_t)t]], bodies[pairs[2 * (uint64_t)t + 1]], hull);
+    const V3 dir{r2.x, r2.y, r2.z};
+    PoseGrad Gr;
+    Gr.zero();
+    band_vjp(pr, act, gin + 10 * (uint64_t)t, Gr, kind == 2u ? &dir : nullptr);
+    store_grad(out + 14 * (uint64_t)t, Gr);
   }
 }
 

@@ -170,6 +170,7 @@ struct bch_context {
   DevBuf bsph;  // per body: position and bound (bounding ball), written by k_aabb each pass
   DevBuf sd_defer;  // signed-distance EPA entries the four-pair kernel hands on
   DevBuf sd_rec;    // the adjoint's EPA faces, one record (4 float4) per queue entry
+  DevBuf sd_brec;   // the adjoint walk's Band / touch simplex per pair (kSdBandRec float4)
   uint32_t n_hulls = 0;
   DevBuf walk_regions, wave_counts, wave_offs, pair_keys, pair_keys_alt, pairs, cls, cls_alt, perm,
       perm_alt, ccount, coffsets, cslots, contacts, epa_q, epa_q2, scan_part, rs_hist, rs_offs;
@@ -1029,6 +1030,7 @@ int run_depen_step(bch_context* c, float aabb_margin, float lr, float* d_pos, fl
   ENSURE(c, c->epa_q, cap * kSdEpaEntry * sizeof(float4));
   ENSURE(c, c->sd_defer, cap * kSdEpaEntry * sizeof(float4));
   ENSURE(c, c->sd_rec, cap * 4 * sizeof(float4));
+  ENSURE(c, c->sd_brec, cap * kSdBandRec * sizeof(float4));
   ENSURE(c, c->sd_out, cap * 10 * sizeof(float));
   ENSURE(c, c->sd_status, cap);
   ENSURE(c, c->pen_pairs, cap * 2 * sizeof(uint32_t));
@@ -1065,7 +1067,11 @@ int run_depen_step(bch_context* c, float aabb_margin, float lr, float* d_pos, fl
   hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
   hipLaunchKernelGGL((k_sd_walk<G, true>), dim3(g), dim3(kBlock), 0, s, (uint32_t)cap, pen_dev, n,
                      c->pen_pairs.as<uint32_t>(), c->bodies.as<BodyRec>(), c->hull.as<float4>(),
-                     c->pen_g.as<float>(), c->g14.as<float>(), (uint8_t*)nullptr, c->epa_q.as<float4>(), cnt);
+                     c->pen_g.as<float>(), c->g14.as<float>(), (uint8_t*)nullptr, c->epa_q.as<float4>(), cnt,
+                     c->sd_brec.as<float4>());
+  hipLaunchKernelGGL(k_sd_band_vjp, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(blocks(cap), 8192u))),
+                     dim3(kBlock), 0, s, (uint32_t)cap, pen_dev, c->sd_brec.as<float4>(), c->pen_pairs.as<uint32_t>(),
+                     c->bodies.as<BodyRec>(), c->hull.as<float4>(), c->pen_g.as<float>(), c->g14.as<float>());
   sd_backward_epa(c, c->pen_pairs.as<uint32_t>(), c->pen_g.as<float>(), c->g14.as<float>(), (uint8_t*)nullptr, s);
   hipLaunchKernelGGL(k_fill_u32, dim3(blocks((uint64_t)n * 7)), dim3(kBlock), 0, s, n * 7,
                      c->body_grad.as<uint32_t>(), 0u);
@@ -2497,6 +2503,7 @@ static int run_sd(bch_context* c, const uint32_t* d_pairs, uint64_t P, const flo
   ENSURE(c, c->epa_q, std::max<uint64_t>(P, 1) * kSdEpaEntry * sizeof(float4));
   ENSURE(c, c->sd_defer, std::max<uint64_t>(P, 1) * kSdEpaEntry * sizeof(float4));
   if (bwd) ENSURE(c, c->sd_rec, std::max<uint64_t>(P, 1) * 4 * sizeof(float4));
+  if (bwd) ENSURE(c, c->sd_brec, std::max<uint64_t>(P, 1) * kSdBandRec * sizeof(float4));
   hipStream_t s = c->stream;
   Counters* cnt = c->counters.as<Counters>();
   hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
@@ -2507,7 +2514,10 @@ static int run_sd(bch_context* c, const uint32_t* d_pairs, uint64_t P, const flo
       hipLaunchKernelGGL((k_sd_walk<G, true>), dim3(g), dim3(kBlock), 0, s, (uint32_t)P, (const uint32_t*)nullptr,
                          c->n, d_pairs,
                          c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_gin, d_out, d_status,
-                         c->epa_q.as<float4>(), cnt);
+                         c->epa_q.as<float4>(), cnt, c->sd_brec.as<float4>());
+      hipLaunchKernelGGL(k_sd_band_vjp, dim3(std::max<uint32_t>(1u, std::min<uint32_t>(blocks(P), 8192u))),
+                         dim3(kBlock), 0, s, (uint32_t)P, (const uint32_t*)nullptr, c->sd_brec.as<float4>(), d_pairs,
+                         c->bodies.as<BodyRec>(), c->hull.as<float4>(), d_gin, d_out);
       sd_backward_epa(c, d_pairs, d_gin, d_out, d_status, s);
     } else {
       hipLaunchKernelGGL((k_sd_walk<G, false>), dim3(g), dim3(kBlock), 0, s, (uint32_t)P, (const uint32_t*)nullptr,
