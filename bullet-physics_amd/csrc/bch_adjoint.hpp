@@ -538,6 +538,49 @@ __device__ __forceinline__ bool wave_split(const Pair& pr, V3 w, V3& xa, V3& xb,
   return false;
 }
 
+// wave_split for the three vertices of a face at once, with the same result
+// per vertex (first B candidate in index order, then the lowest A lane,
+// A candidates in chunks of 64): B's world core points are computed once,
+// one per lane, and read back by index (the same arithmetic as wave_split's
+// per-candidate recomputation, so the same bits), instead of once per A
+// chunk, B candidate and vertex.  B sets of more than 64 points take
+// wave_split.
+__device__ __forceinline__ bool wave_split3(const Pair& pr, const V3* w, V3* xa, V3* xb, int lane) {
+  const int na = core_set_size(pr.a), nb = core_set_size(pr.b);
+  if (nb > 64) {
+    bool ok = wave_split(pr, w[0], xa[0], xb[0], lane);
+    ok = wave_split(pr, w[1], xa[1], xb[1], lane) && ok;
+    return wave_split(pr, w[2], xa[2], xb[2], lane) && ok;
+  }
+  const V3 lbl = lane < nb ? core_set_point(pr.b, lane) : splat(0.0f);
+  const V3 wbl = qrot(pr.qb, lbl) + pr.pb;
+  bool f0 = false, f1 = false, f2 = false;
+  for (int ca = 0; ca < na && !(f0 && f1 && f2); ca += 64) {
+    const int ia = ca + lane;
+    const bool va = ia < na;
+    const V3 la = va ? core_set_point(pr.a, ia) : splat(0.0f);
+    const V3 wa = qrot(pr.qa, la) + pr.pa;
+    for (int ib = 0; ib < nb && !(f0 && f1 && f2); ++ib) {
+      const V3 d = wa - V3{rdl_f(wbl.x, ib), rdl_f(wbl.y, ib), rdl_f(wbl.z, ib)};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const bool done = k == 0 ? f0 : (k == 1 ? f1 : f2);
+        if (done) continue;
+        const uint64_t hit = __ballot(va && d.x == w[k].x && d.y == w[k].y && d.z == w[k].z);
+        if (hit) {
+          const int l = ctz64(hit);
+          xa[k] = V3{rdl_f(la.x, l), rdl_f(la.y, l), rdl_f(la.z, l)};
+          xb[k] = V3{rdl_f(lbl.x, ib), rdl_f(lbl.y, ib), rdl_f(lbl.z, ib)};
+          if (k == 0) f0 = true;
+          else if (k == 1) f1 = true;
+          else f2 = true;
+        }
+      }
+    }
+  }
+  return f0 && f1 && f2;
+}
+
 // The EPA arm of the VJP for one queue entry `ent` of pair t, after EPA
 // reported (n, depth, status) and, with has_face, the face (w0, w1, w2) of
 // Minkowski vertices; the whole wavefront takes part (the vertex split),
@@ -563,9 +606,8 @@ __device__ __forceinline__ void sd_epa_vjp(const Pair& pr, const float4* ent, ui
     float fd_plane = dot(w0, en);
     if (!(fd_plane == edepth)) st |= SD_FLAG_EDGE;
     V3 xa[3], xb[3];
-    bool ok = wave_split(pr, w0, xa[0], xb[0], lane);
-    ok = wave_split(pr, w1, xa[1], xb[1], lane) && ok;
-    ok = wave_split(pr, w2, xa[2], xb[2], lane) && ok;
+    const V3 wf[3] = {w0, w1, w2};
+    bool ok = wave_split3(pr, wf, xa, xb, lane);
     if (!ok) st |= SD_FLAG_SPLIT;
     if (ok) epa_vjp(pr, en, xa, xb, gin + 10 * (uint64_t)t, Gr);
   }
