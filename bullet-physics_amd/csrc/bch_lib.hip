@@ -1050,7 +1050,7 @@ int run_depen_step(bch_context* c, float aabb_margin, float lr, float* d_pos, fl
   hipLaunchKernelGGL(k_split_pairs, dim3(blocks(cap)), dim3(kBlock), 0, s, (uint32_t)cap, P_dev,
                      c->pair_keys.as<uint64_t>(), c->btype.as<uint8_t>(), c->pairs.as<uint32_t>(),
                      c->cls.as<uint32_t>(), c->perm.as<uint32_t>());
-  constexpr int G = 16;
+  constexpr int G = 8;
   uint32_t g = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(cap, kBlock / G), 8192u));
   hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
   hipLaunchKernelGGL((k_sd_walk<G, false>), dim3(g), dim3(kBlock), 0, s, (uint32_t)cap, P_dev, n,
@@ -2508,7 +2508,7 @@ static int run_sd(bch_context* c, const uint32_t* d_pairs, uint64_t P, const flo
   Counters* cnt = c->counters.as<Counters>();
   hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
   if (P) {
-    constexpr int G = 16;
+    constexpr int G = 8;
     uint32_t g = std::max<uint32_t>(1u, std::min<uint32_t>(blocks(P, kBlock / G), 8192u));
     if (bwd) {
       hipLaunchKernelGGL((k_sd_walk<G, true>), dim3(g), dim3(kBlock), 0, s, (uint32_t)P, (const uint32_t*)nullptr,
