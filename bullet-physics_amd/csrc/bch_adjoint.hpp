@@ -368,15 +368,22 @@ BCH_HD void store_out(float* o10, const SdOut& o) {
 // an EPA pair or a zero gradient already stored), so the double-precision
 // adjoint runs one lane per pair instead of on one lane of each group.
 constexpr uint32_t kSdBandRec = 3;
+// Three waves per SIMD (168 VGPRs, a few spills; 224 uncapped, two waves):
+// the 1024-world de-penetration step runs 9.4 -> 8.1 ms, 262,144 hull pairs
+// about as before (forward 1.85 -> 1.87 ms, backward 2.87 -> 2.81 ms); at four
+// waves (128 VGPRs) the spills cost more than the occupancy gains.
+#ifndef BCH_SD_WALK_WAVES
+#define BCH_SD_WALK_WAVES 3
+#endif
 template <int G, bool BWD>
-__global__ __launch_bounds__(256) void k_sd_walk(uint32_t P, const uint32_t* __restrict__ P_dev, uint32_t nbodies,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BCH_SD_WALK_WAVES))) void k_sd_walk(uint32_t P, const uint32_t* __restrict__ P_dev, uint32_t nbodies,
                                                  const uint32_t* __restrict__ pairs,
                                                  const BodyRec* __restrict__ bodies, const float4* __restrict__ hull,
                                                  const float* __restrict__ gin, float* __restrict__ out,
                                                  uint8_t* __restrict__ status, float4* __restrict__ epa_q,
                                                  Counters* cnt, float4* __restrict__ brec = nullptr) {
   P = dev_count(P, P_dev);
-  __shared__ float4 hstage[256 / G][2][64];
+  __shared__ float4 hstage[256 / G][64];
   const uint32_t per_block = blockDim.x / G;
   int grp = threadIdx.x / G, sub = threadIdx.x % G;
   for (uint32_t base = blockIdx.x * per_block; base < P; base += gridDim.x * per_block) {
@@ -397,13 +404,19 @@ __global__ __launch_bounds__(256) void k_sd_walk(uint32_t P, const uint32_t* __r
     }
     if (t < P && !bad) {
       Pair pr = make_pair(bodies[i], bodies[j], hull);
-      if (pr.a.type == HULL && pr.a.nhull <= 64) {
-        for (int v = sub; v < pr.a.nhull; v += G) hstage[grp][0][v] = pr.a.hull[v];
-        pr.a.hull = hstage[grp][0];
+      // one 64-vertex slot per group: both hulls when they fit together,
+      // else one hull of at most 64 vertices (A first), the other read
+      // through the cache
+      const int na = pr.a.type == HULL ? (int)pr.a.nhull : 65, nb = pr.b.type == HULL ? (int)pr.b.nhull : 65;
+      int used = 0;
+      if (na <= 64) {
+        for (int v = sub; v < na; v += G) hstage[grp][v] = pr.a.hull[v];
+        pr.a.hull = hstage[grp];
+        used = na;
       }
-      if (pr.b.type == HULL && pr.b.nhull <= 64) {
-        for (int v = sub; v < pr.b.nhull; v += G) hstage[grp][1][v] = pr.b.hull[v];
-        pr.b.hull = hstage[grp][1];
+      if (nb <= 64 - used) {
+        for (int v = sub; v < nb; v += G) hstage[grp][used + v] = pr.b.hull[v];
+        pr.b.hull = hstage[grp] + used;
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();

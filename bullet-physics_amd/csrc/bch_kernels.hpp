@@ -1095,8 +1095,16 @@ struct GroupSupport {
 
 // Pairs with a hull (class CLS_GJK_HULL), G lanes per pair: every lane of a
 // group runs the same walk; the hull support scans are split over the group.
+// The walk is latency-bound, so occupancy pays: capped at 128 VGPRs, G >= 8
+// runs four waves per SIMD (three uncapped, at 158 VGPRs), with a few
+// spills: 678,321 hull pairs of the 1M-body mixed scene 1.03 -> 0.92 ms (at
+// five waves, 96 VGPRs, the spills make it 1.13 ms).  G = 4 stages 64 KB per
+// block, two blocks per CU, so it stays uncapped.
+#ifndef BCH_NP_HULL_WAVES
+#define BCH_NP_HULL_WAVES 4
+#endif
 template <int G>
-__global__ __launch_bounds__(256) void k_np_hull(const Counters* __restrict__ cnt_ro, const uint32_t* __restrict__ perm,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(G >= 8 ? BCH_NP_HULL_WAVES : 1))) void k_np_hull(const Counters* __restrict__ cnt_ro, const uint32_t* __restrict__ perm,
                                                  const uint32_t* __restrict__ pairs, const BodyRec* __restrict__ bodies,
                                                  const float4* __restrict__ hull, const float* __restrict__ seeds,
                                                  uint32_t* __restrict__ ccount, float4* __restrict__ cslots,

@@ -90,6 +90,61 @@ def test_sd_forward_matches_oracle(torch, bpa, scenes, oracle, ctx, mix):
     assert np.abs(out[sep, 0] - sc["target"][sep]).max() < 1e-4
 
 
+def _pad_hulls(sc, count_a, count_b):
+    """The scene with A's hulls padded to count_a vertices and B's to count_b
+    by appending copies of their own vertices (in order, cycling)."""
+    nb = sc["shape_type"].shape[0]
+    verts, first, count = [], np.zeros(nb, np.uint32), np.zeros(nb, np.uint32)
+    at = 0
+    for b in range(nb):
+        v = sc["hull_vertices"][sc["hull_first"][b]: sc["hull_first"][b] + sc["hull_count"][b]]
+        n = max(count_a if b % 2 == 0 else count_b, v.shape[0])
+        v = v[np.arange(n) % v.shape[0]]
+        first[b], count[b] = at, n
+        verts.append(v)
+        at += n
+    out = dict(sc)
+    out["hull_vertices"] = np.concatenate(verts).astype(np.float32)
+    out["hull_first"], out["hull_count"] = first, count
+    return out
+
+
+def test_sd_hull_staging_leaves_results_unchanged(torch, bpa, scenes, oracle, ctx):
+    """The walk stages hull vertices in one 64-vertex LDS slot per pair: both
+    hulls when they fit together, else A alone or B alone (<= 64 vertices),
+    else neither (both read through the cache).  Padding a hull with copies
+    of its own vertices changes which of these happens but no support point
+    (of equal projections the first vertex wins, and the copies come after
+    the originals), so status, all ten forward outputs and the fourteen pose
+    gradients of the backward must be bit-identical across the four
+    stagings.  The unpadded scene meets the oracle."""
+    _flush("case: sd hull staging, 24-point hulls padded to 24/48/70")
+    base = scenes.hull_pairs(256, 9, npts=24, spacing=0.0)
+    P = base["pairs"].shape[0]
+    g10 = torch.from_numpy(np.random.default_rng(11).normal(size=(P, 10)).astype(np.float32)).cuda().contiguous()
+    results = {}
+    for ca, cb in ((24, 24), (24, 48), (70, 24), (70, 70)):
+        sc = _pad_hulls(base, ca, cb)
+        w, pairs, out, st = _gpu_sd(torch, bpa, sc, ctx)
+        g14 = torch.empty((P, 14), dtype=torch.float32, device="cuda")
+        stb = torch.empty(P, dtype=torch.uint8, device="cuda")
+        w.sd_backward(pairs.data_ptr(), P, g10.data_ptr(), g14.data_ptr(), stb.data_ptr())
+        w.synchronize()
+        results[(ca, cb)] = (out, st, g14.cpu().numpy(), stb.cpu().numpy())
+    out0, st0, g0, stb0 = results[(24, 24)]
+    ref, rst = oracle.sd_pairs(base, base["pairs"])
+    assert np.array_equal(st0 & 7, rst)
+    ok = rst != oracle.SD_TOUCH
+    assert np.abs(out0[ok, 0].astype(np.float64) - ref[ok, 0]).max() <= TOL_M
+    assert np.isfinite(g0).all()
+    for key, (out, st, g14, stb) in results.items():
+        _flush(f"  A {key[0]} / B {key[1]} vertices: {np.bincount(st & 7, minlength=6)} walk/epa/capped/touch/"
+               f"unconv/separated, forward equal {np.array_equal(out, out0)}, gradients equal "
+               f"{np.array_equal(g14, g0)}")
+        assert np.array_equal(st, st0) and np.array_equal(stb, stb0)
+        assert np.array_equal(out, out0) and np.array_equal(g14, g0)
+
+
 @pytest.mark.parametrize("mix", [False, True])
 def test_sd_adjoint_vs_f64_finite_differences(torch, bpa, scenes, oracle, ctx, mix):
     """Every pair is held to 1e-3 relative against f64 central differences
