@@ -115,7 +115,7 @@ constexpr size_t kStEpa = kStSlots + al256(kStP * 8 * sizeof(float4));       // 
 constexpr size_t kStCounters = kStEpa + 2 * al256(kStP * 4 * sizeof(float4));
 constexpr size_t kStHist = kStCounters + al256(sizeof(Counters));
 constexpr size_t kStOffs = kStHist + al256(256 * kStTiles * sizeof(uint32_t));
-constexpr size_t kStPart = kStOffs + al256((256 * kStTiles + 1) * sizeof(uint32_t));  // the sort's scan partials
+constexpr size_t kStPart = kStOffs + al256((256 * kStTiles + 256) * sizeof(uint32_t));  // scan partials
 constexpr size_t kStParts = (256 * kStTiles + kScanTile - 1) / kScanTile;
 constexpr size_t kStBytes = kStPart + al256(kStParts * sizeof(uint32_t));
 static_assert(kStBytes == BCH_STATIC_BYTES, "BCH_STATIC_BYTES must equal the carved layout");
@@ -327,14 +327,16 @@ int radix_pass(bch_context* c, K* kin, K* kout, uint32_t* vin, uint32_t* vout, b
                const uint32_t* n_dev, int shift, int pack, uint32_t tiles, hipStream_t s) {
   hipLaunchKernelGGL((k_rs_hist<K, ROUNDS>), dim3(tiles), dim3(kRsThreads), 0, s, kin, (uint32_t)n, n_dev, shift, pack,
                      tiles, c->rs_hist.as<uint32_t>());
-  int rc = dev_scan(c, c->rs_hist.as<uint32_t>(), c->rs_offs.as<uint32_t>(), (uint64_t)256 * tiles, c->scan_part, s);
-  if (rc) return rc;
+  uint32_t* offs = c->rs_offs.as<uint32_t>();
+  uint32_t* rowtot = offs + (size_t)256 * tiles;
+  hipLaunchKernelGGL(k_rs_row_scan, dim3(256), dim3(kScanThreads), 0, s, c->rs_hist.as<uint32_t>(), tiles, offs,
+                     rowtot);
   if (vals)
     hipLaunchKernelGGL((k_rs_scatter<K, true, ROUNDS>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n,
-                       n_dev, shift, pack, tiles, c->rs_hist.as<uint32_t>(), c->rs_offs.as<uint32_t>(), kout, vout);
+                       n_dev, shift, pack, tiles, c->rs_hist.as<uint32_t>(), offs, rowtot, kout, vout);
   else
     hipLaunchKernelGGL((k_rs_scatter<K, false, ROUNDS>), dim3(tiles), dim3(kRsThreads), 0, s, kin, vin, (uint32_t)n,
-                       n_dev, shift, pack, tiles, c->rs_hist.as<uint32_t>(), c->rs_offs.as<uint32_t>(), kout, vout);
+                       n_dev, shift, pack, tiles, c->rs_hist.as<uint32_t>(), offs, rowtot, kout, vout);
   return BCH_OK;
 }
 
@@ -351,7 +353,7 @@ int dev_radix_sort(bch_context* c, K* k0, K* k1, uint32_t* v0, uint32_t* v1, uin
   const uint32_t tile = c->sort_tile == 2048 ? 2048u : (uint32_t)kRsTile;
   uint32_t tiles = (uint32_t)((n + tile - 1) / tile);
   ENSURE(c, c->rs_hist, (size_t)256 * tiles * sizeof(uint32_t));
-  ENSURE(c, c->rs_offs, ((size_t)256 * tiles + 1) * sizeof(uint32_t));
+  ENSURE(c, c->rs_offs, ((size_t)256 * tiles + 256) * sizeof(uint32_t));  // offsets, then the digit totals
   K* kin = k0;
   K* kout = k1;
   uint32_t* vin = v0;
@@ -1184,7 +1186,7 @@ int bch_create(int device, bch_context** out) {
   st->epa_q2.view(b + kStEpa + al256(kStP * 4 * sizeof(float4)), kStP * 4 * sizeof(float4));
   st->counters.view(b + kStCounters, sizeof(Counters));
   st->rs_hist.view(b + kStHist, 256 * kStTiles * sizeof(uint32_t));
-  st->rs_offs.view(b + kStOffs, (256 * kStTiles + 1) * sizeof(uint32_t));
+  st->rs_offs.view(b + kStOffs, (256 * kStTiles + 256) * sizeof(uint32_t));
   st->scan_part.view(b + kStPart, kStParts * sizeof(uint32_t));
   c->stat = st;
   *out = c;

@@ -14,8 +14,12 @@
 // same sequence wrote), so a fixed launch shape serves any count up to n.
 // Radix sort (u64 keys, optional u32 values, 8-bit digits, stable):
 //   k_rs_hist       per-tile digit histograms, digit-major [256][tiles]
-//   scan            over the 256 x tiles counts -> global digit offsets
-//   k_rs_scatter    per tile, in item order: each wavefront ranks its items
+//   k_rs_row_scan   one block per digit scans its row of tile counts (the
+//                   tile's offset among the digit's items) and writes the
+//                   digit's total
+//   k_rs_scatter    adds the digit's start (the exclusive scan of the 256
+//                   totals, done by every block) to its row offsets; then
+//                   per tile, in item order: each wavefront ranks its items
 //                   among equal digits with eight ballots, wavefront counts
 //                   meet in LDS, a running per-digit base keeps the order
 //                   across the tile's rounds (stable, as LSD needs)
@@ -180,8 +184,38 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(const K* __restrict__ ke
   hist[(uint64_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// counts = the per-tile digit counts k_rs_hist wrote ([256][tiles]), offs =
-// their scan (the global position of each tile's first item of a digit).
+// Row d of counts ([256][tiles]) scanned into offs (exclusive, per digit:
+// how many items of digit d the earlier tiles hold); rowtot[d] = the row's
+// sum.  One block per digit.
+__global__ __launch_bounds__(kScanThreads) void k_rs_row_scan(const uint32_t* __restrict__ counts, uint32_t tiles,
+                                                              uint32_t* __restrict__ offs,
+                                                              uint32_t* __restrict__ rowtot) {
+  const uint64_t row = (uint64_t)blockIdx.x * tiles;
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < tiles; b0 += kScanTile) {
+    const uint32_t base = b0 + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      v[k] = base + k < tiles ? counts[row + base + k] : 0u;
+      s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan(s, tot) + carry;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      if (base + k < tiles) offs[row + base + k] = run;
+      run += v[k];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) rowtot[blockIdx.x] = carry;
+}
+
+// counts = the per-tile digit counts k_rs_hist wrote ([256][tiles]), offs and
+// rowtot = what k_rs_row_scan made of them: a tile's first item of digit d
+// goes to (rowtot[0] + ... + rowtot[d - 1]) + offs[d][tile].
 // The tile's items are first placed in LDS in digit order (stable), then
 // written out from there: consecutive LDS slots of one digit go to
 // consecutive global positions, so the stores come in runs (a wavefront's
@@ -190,7 +224,8 @@ template <class K, bool VALS, int ROUNDS = kRsRounds>
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                            uint32_t n, const uint32_t* __restrict__ n_dev, int shift,
                                                            int pack, uint32_t tiles, const uint32_t* __restrict__ counts,
-                                                           const uint32_t* __restrict__ offs, K* __restrict__ kout,
+                                                           const uint32_t* __restrict__ offs,
+                                                           const uint32_t* __restrict__ rowtot, K* __restrict__ kout,
                                                            uint32_t* __restrict__ vout) {
   n = capped_count(n, n_dev);
   __shared__ K skey[kRsThreads * ROUNDS];
@@ -199,11 +234,13 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const K* __restrict__
   __shared__ uint32_t wcount[kRsThreads / 64][256];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint64_t ti = (uint64_t)threadIdx.x * tiles + blockIdx.x;
+  uint32_t all;
+  const uint32_t dstart = block_exclusive_scan(rowtot[threadIdx.x], all);
   uint32_t total;
   const uint32_t ls = block_exclusive_scan(counts[ti], total);
   lstart[threadIdx.x] = ls;
   running[threadIdx.x] = ls;
-  gstart[threadIdx.x] = offs[ti];
+  gstart[threadIdx.x] = dstart + offs[ti];
 #pragma unroll
   for (int k = 0; k < kRsThreads / 64; ++k) wcount[k][threadIdx.x] = 0u;
   __syncthreads();

@@ -440,7 +440,7 @@ int bch_depen_release(bch_context* ctx);
 #define BCH_STATIC_MAX_PAIRS 32768u
 #define BCH_STATIC_MAX_BODIES 65536u
 #define BCH_STATIC_MAX_HULL_VERTICES 131072u
-#define BCH_STATIC_BYTES 16073984u
+#define BCH_STATIC_BYTES 16074752u
 int bch_collide_batch_static(bch_context* ctx, const bch_bodies* d_bodies, const uint32_t* d_pairs, uint64_t n_pairs,
                              const float* d_seeds, uint8_t* d_out_count, float* d_out_points);
 
