@@ -181,6 +181,8 @@ struct bch_context {
   uint32_t trav_block = 256;  // threads per tree-walk block (64, 128, 256, 512)
   uint32_t sort_tile = 2048;  // items per radix-sort tile (2048 or 4096; the static batch keeps 4096)
   uint64_t tree_cap = 0;      // pair buffer of the eager pass, sized by earlier passes
+  // capacities of the next eager pass's device-count run (0: none yet)
+  uint64_t eager_P_cap = 0, eager_C_cap = 0;
   uint32_t walk_R = 1024;     // pair slots of one tree-walk wavefront's region
   uint32_t walk_nw = 0;       // wavefronts of the last walk
   bool walk_grow = false;     // a replayed walk overflowed a region: the next capture doubles walk_R
@@ -1770,6 +1772,7 @@ int bch_world_set_bodies(bch_context* c, const bch_bodies* bodies, uint32_t n_wo
     return fail(c, BCH_INVALID_ARG, "n_worlds > 1 needs world_offsets");
   }
   release_graph(c);  // the captured graphs are shaped for the old bodies
+  c->eager_P_cap = c->eager_C_cap = 0;
   release_depen_graph(c);
   c->mM = 0;  // cached manifolds name bodies by index
   c->has_bflags = c->has_bgm = false;
@@ -1846,10 +1849,44 @@ int bch_world_set_poses_device(bch_context* c, const float* d_position, const fl
 int bch_world_collide(bch_context* c, float aabb_margin, uint64_t* out_pairs, uint64_t* out_contacts) {
   if (!c) return BCH_INVALID_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
-  int rc = run_pass(c, aabb_margin);
-  if (rc) return rc;
+  // Once a pass has counted this world's pairs and contacts, the next one
+  // runs the captured pass's kernels (counts kept on the device, no host
+  // round trip inside the pass) against 1.25x those counts and reads the
+  // counts back at its end.  A run that overflows a capacity or a walk
+  // region goes again as the first pass did, with its round trips.
+  bool done = false;
+  if (c->eager_P_cap && c->n) {
+    const uint64_t pc = c->P_cap, cc = c->C_cap;
+    c->P_cap = c->eager_P_cap;
+    c->C_cap = c->eager_C_cap;
+    c->cap_mode = true;
+    int rc = run_pass(c, aabb_margin);
+    c->cap_mode = false;
+    c->P_cap = pc;
+    c->C_cap = cc;
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(&c->host_cnt, c->counters.p, sizeof(Counters), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const Counters& h = c->host_cnt;
+    if (!(h.errors & (8u | 16u | 128u))) {
+      c->P = h.n_pairs;
+      c->C = h.n_contacts;
+      done = true;
+    } else if (h.errors & 128u) {
+      uint32_t r = c->walk_R;
+      while (r < 2 * h.wave_over) r *= 2;
+      c->walk_R = r;
+    }
+  }
+  if (!done) {
+    int rc = run_pass(c, aabb_margin);
+    if (rc) return rc;
+  }
+  c->eager_P_cap = std::max<uint64_t>(c->P + c->P / 4, 1024);
+  c->eager_C_cap = std::max<uint64_t>(c->C + c->C / 4, 1024);
   stage_times(c);
   if (c->host_cnt.errors & 1u) return fail(c, BCH_HIP_ERROR, "box-box clip polygon overflow");
+  if (c->host_cnt.errors & 2u) return fail(c, BCH_HIP_ERROR, "EPA polytope store overflow");
   if (out_pairs) *out_pairs = c->P;
   if (out_contacts) *out_contacts = c->C;
   return BCH_OK;

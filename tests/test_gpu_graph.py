@@ -88,6 +88,41 @@ def test_replays_match_eager_batched_worlds(bpa, scenes, ctx):
     _replays_match(bpa, w, poses, refs, [2, 1, 0, 2], "batched")
 
 
+@pytest.mark.parametrize("kind", ["mixed", "batched"])
+def test_later_eager_passes_match_a_first_pass(bpa, scenes, ctx, kind):
+    """After its first pass, a world's eager pass runs on device-held counts
+    against 1.25x the last pass's counts and falls back to the first pass's
+    host-sized route when a count outgrows that.  At every pose of a
+    sequence (jittered, spread out: fewer pairs, packed closer: about twice
+    the pairs, so the fallback runs) the results equal those of a first pass
+    over the same poses on a fresh context."""
+    _flush(f"case: later eager passes vs first passes, {kind}")
+    sc = scenes.mixed_convex(20000, 2) if kind == "mixed" else scenes.batched_worlds(64, 3, 16, 512)
+    planes = sc["shape_type"] == 4
+    base = (sc["position"], sc["orientation"])
+    spread = sc["position"].copy()
+    spread[~planes] *= 1.3
+    packed = sc["position"].copy()
+    packed[~planes] *= 0.75
+    poses = [base, _jittered(sc, 1, 0.05), (spread, sc["orientation"]), (packed, sc["orientation"])]
+    w = bpa.CollisionWorld(sc, ctx)
+    ref_ctx = bpa._Context(0)
+    refs = {}
+    counts = []
+    for k in (0, 1, 0, 2, 3, 3, 1):
+        w.set_poses(*poses[k])
+        w.collide(MARGIN)
+        got = w.results()
+        if k not in refs:
+            fresh = bpa.CollisionWorld(dict(sc, position=poses[k][0], orientation=poses[k][1]), ref_ctx)
+            fresh.collide(MARGIN)
+            refs[k] = fresh.results()
+        counts.append(got[0].shape[0])
+        _same(got, refs[k], f"{kind} eager pass at pose {k}")
+    _flush(f"  pairs per pass {counts}")
+    assert counts[4] > 1.25 * counts[3]  # the packed pose outgrew the device-count capacities
+
+
 def test_capacity_overflow_is_reported(bpa, scenes, ctx):
     """Capacities below the counts: the capture's own pass and a later replay
     report BCH_OVERFLOW (held results stay within the buffers); an eager
