@@ -172,6 +172,7 @@ struct bch_context {
   DevBuf sd_rec;    // the adjoint's EPA faces, one record (4 float4) per queue entry
   DevBuf sd_brec;   // the adjoint walk's Band / touch simplex per pair (kSdBandRec float4)
   uint32_t n_hulls = 0;
+  DevBuf world_reg;  // per world: tree bodies, then their exclusive scan (the per-world Morton sort)
   DevBuf walk_regions, wave_counts, wave_offs, pair_keys, pair_keys_alt, pairs, cls, cls_alt, perm,
       perm_alt, ccount, coffsets, cslots, contacts, epa_q, epa_q2, scan_part, rs_hist, rs_offs;
   int hull_lanes = 16;      // lanes per hull GJK pair (4, 8 or 16)
@@ -505,9 +506,23 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   int shifts[8];
   int ns = digit_shifts(drop, end_bit, shifts);
   bool alt = false;
-  int rc = dev_radix_sort<uint64_t>(c, c->keys.as<uint64_t>(), c->keys_alt.as<uint64_t>(), c->vals.as<uint32_t>(),
-                                    c->vals_alt.as<uint32_t>(), n, shifts, ns, s, &alt);
-  if (rc) return rc;
+  int rc = BCH_OK;
+  if (d_world_of && d_world_offsets && c->max_world <= kSegMax) {
+    // worlds of at most kSegMax bodies: one block sorts each world (same order)
+    const uint32_t nw = c->n_worlds;
+    ENSURE(c, c->world_reg, (size_t)2 * (nw + 1) * sizeof(uint32_t));
+    uint32_t* cntw = c->world_reg.as<uint32_t>();
+    hipLaunchKernelGGL(k_seg_count, dim3(nw), dim3(256), 0, s, c->keys.as<uint64_t>(), d_world_offsets, cntw);
+    rc = dev_scan(c, cntw, cntw + nw + 1, nw, c->scan_part, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_seg_sort, dim3(nw), dim3(256), 0, s, c->keys.as<uint64_t>(), d_world_offsets, cntw + nw + 1,
+                       nw, drop, c->keys_alt.as<uint64_t>(), c->vals_alt.as<uint32_t>());
+    alt = true;
+  } else {
+    rc = dev_radix_sort<uint64_t>(c, c->keys.as<uint64_t>(), c->keys_alt.as<uint64_t>(), c->vals.as<uint32_t>(),
+                                  c->vals_alt.as<uint32_t>(), n, shifts, ns, s, &alt);
+    if (rc) return rc;
+  }
   const uint64_t* skeys = alt ? c->keys_alt.as<uint64_t>() : c->keys.as<uint64_t>();
   const uint32_t* order = alt ? c->vals_alt.as<uint32_t>() : c->vals.as<uint32_t>();
   int L = 0;
@@ -1204,7 +1219,7 @@ void bch_destroy(bch_context* c) {
   DevBuf* all[] = {&c->bodies, &c->hull, &c->world_offsets, &c->world_of, &c->staging_a, &c->staging_b,
                    &c->staging_c, &c->staging_d, &c->staging_e, &c->staging_f, &c->aabb, &c->flags, &c->big_ids,
                    &c->counters, &c->partials, &c->keys, &c->keys_alt, &c->vals, &c->vals_alt, &c->nodes, &c->hull_ids,
-                   &c->btype, &c->wend, &c->walk_regions,
+                   &c->btype, &c->wend, &c->world_reg, &c->walk_regions,
                    &c->wave_counts, &c->wave_offs, &c->pair_keys, &c->pair_keys_alt, &c->pairs, &c->cls, &c->cls_alt, &c->perm, &c->perm_alt,
                    &c->ccount, &c->coffsets, &c->cslots, &c->contacts, &c->epa_q, &c->epa_q2,
                    &c->scan_part, &c->rs_hist, &c->rs_offs, &c->sd_out, &c->sd_status, &c->pen_pairs,

@@ -291,4 +291,98 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const K* __restrict__
   }
 }
 
+// ------------------------------------------------ per-world Morton sort --
+// The Morton keys of a batch of worlds, where world w's bodies are the index
+// range [off[w], off[w + 1]) of at most kSegMax bodies and a body outside
+// the tree carries the key ~0: the same order as the stable radix sort of
+// all keys over bits [drop, 31 + world bits), i.e. the tree's bodies world
+// by world in (code >> drop) order, ties in body order, then every excluded
+// body in body order, with one block per world and no pass over the whole
+// array per digit.  k_seg_count counts each world's tree bodies; their
+// exclusive scan gives each world's first output slot.
+constexpr uint32_t kSegMax = 2048;
+__global__ __launch_bounds__(256) void k_seg_count(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ off,
+                                                   uint32_t* __restrict__ count) {
+  const uint32_t w = blockIdx.x, o0 = off[w], o1 = off[w + 1];
+  uint32_t c = 0;
+  for (uint32_t i = o0 + threadIdx.x; i < o1; i += 256) c += keys[i] != ~0ull ? 1u : 0u;
+  uint32_t tot;
+  (void)block_exclusive_scan(c, tot);
+  if (threadIdx.x == 0) count[w] = tot;
+}
+
+// One block per world: the world's keys as (sort key << 11 | local index),
+// sort key = code bits [drop, 30) below a top bit set for an excluded body,
+// LSD-sorted in LDS by 8-bit digits (ballot ranking, as k_rs_scatter), then
+// written to their global slots: tree body k of world w to first[w] + k,
+// excluded body k to n_tree + (off[w] - first[w]) + k, where first holds
+// the scan of k_seg_count's counts and first[n_worlds] = n_tree.
+__global__ __launch_bounds__(256) void k_seg_sort(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ off,
+                                                  const uint32_t* __restrict__ first, uint32_t n_worlds, int drop,
+                                                  uint64_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+  __shared__ uint32_t buf[2][kSegMax];
+  __shared__ uint32_t running[256];
+  __shared__ uint32_t wcount[4][256];
+  const uint32_t w = blockIdx.x, o0 = off[w], m = off[w + 1] - o0;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const int cb = 30 - drop;  // code bits kept
+  for (uint32_t k = threadIdx.x; k < m; k += 256) {
+    const uint64_t key = keys[o0 + k];
+    const uint32_t sk = key == ~0ull ? (1u << cb) : (uint32_t)((key >> drop) & ((1ull << cb) - 1ull));
+    buf[0][k] = (sk << 11) | k;
+  }
+  int cur = 0;
+  for (int sh = 11; sh < 11 + cb + 1; sh += 8) {
+    running[threadIdx.x] = 0u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wcount[k][threadIdx.x] = 0u;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < m; k += 256) atomicAdd(&running[(buf[cur][k] >> sh) & 0xffu], 1u);
+    __syncthreads();
+    uint32_t all;
+    const uint32_t base = block_exclusive_scan(running[threadIdx.x], all);
+    running[threadIdx.x] = base;
+    __syncthreads();
+    for (uint32_t r0 = 0; r0 < m; r0 += 256) {
+      const uint32_t k = r0 + threadIdx.x;
+      const bool valid = k < m;
+      const uint32_t item = valid ? buf[cur][k] : 0u;
+      const uint32_t d = (item >> sh) & 0xffu;
+      uint64_t peers = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const uint64_t mb = __ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? mb : ~mb;
+      }
+      const uint32_t rank = (uint32_t)__popcll(peers & lt), cnt = (uint32_t)__popcll(peers);
+      if (valid && rank + 1 == cnt) wcount[wv][d] = cnt;
+      __syncthreads();
+      if (valid) {
+        uint32_t pos = running[d] + rank;
+        for (int k2 = 0; k2 < 4; ++k2)
+          if (k2 < wv) pos += wcount[k2][d];
+        buf[cur ^ 1][pos] = item;
+      }
+      __syncthreads();
+      uint32_t add = 0u;
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) {
+        add += wcount[k2][threadIdx.x];
+        wcount[k2][threadIdx.x] = 0u;
+      }
+      running[threadIdx.x] += add;
+      __syncthreads();
+    }
+    cur ^= 1;
+  }
+  const uint32_t f = first[w], nt = first[n_worlds], rw = first[w + 1] - f;
+  for (uint32_t k = threadIdx.x; k < m; k += 256) {
+    const uint32_t body = o0 + (buf[cur][k] & 0x7ffu);
+    const uint32_t dest = k < rw ? f + k : nt + (o0 - f) + (k - rw);
+    kout[dest] = keys[body];
+    vout[dest] = body;
+  }
+}
+
 }  // namespace bch

@@ -355,10 +355,15 @@ def test_contact_buffer_regrows_when_contacts_outnumber_pairs(bpa, oracle):
         fresh.close()
 
 
-def test_batched_worlds_equal_plain_per_world(bpa, scenes, oracle, ctx):
-    """Independent worlds in one pass == each world collided alone (plain path)."""
-    _flush("case: batched 64 worlds")
-    sc = scenes.batched_worlds(64, 3, 16, 512)
+@pytest.mark.parametrize("worlds", ["64 of 16-512", "8 of 16-4096"])
+def test_batched_worlds_equal_plain_per_world(bpa, scenes, oracle, ctx, worlds):
+    """Independent worlds in one pass == each world collided alone (plain
+    path).  Worlds of at most 2048 bodies have their Morton keys sorted one
+    block per world; a batch with a larger world sorts them all at once."""
+    _flush(f"case: batched {worlds} bodies")
+    sc = scenes.batched_worlds(64, 3, 16, 512) if worlds.startswith("64") else scenes.batched_worlds(8, 7, 16, 4096)
+    if worlds.startswith("8"):
+        assert np.diff(sc["world_offsets"]).max() > 2048
     w = bpa.CollisionWorld(sc, ctx)
     w.collide(MARGIN)
     pairs, offs, cont = w.results()
