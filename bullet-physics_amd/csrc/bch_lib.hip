@@ -505,7 +505,7 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   drop = std::max(0, std::min(drop, 24));
   int shifts[8];
   int ns = digit_shifts(drop, end_bit, shifts);
-  bool alt = false;
+  bool alt = false, seg = false;
   int rc = BCH_OK;
   if (d_world_of && d_world_offsets && c->max_world <= kSegMax) {
     // worlds of at most kSegMax bodies: one block sorts each world (same order)
@@ -516,8 +516,9 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
     rc = dev_scan(c, cntw, cntw + nw + 1, nw, c->scan_part, s);
     if (rc) return rc;
     hipLaunchKernelGGL(k_seg_sort, dim3(nw), dim3(256), 0, s, c->keys.as<uint64_t>(), d_world_offsets, cntw + nw + 1,
-                       nw, drop, c->keys_alt.as<uint64_t>(), c->vals_alt.as<uint32_t>());
+                       nw, drop, c->keys_alt.as<uint64_t>(), c->vals_alt.as<uint32_t>(), c->wend.as<uint32_t>());
     alt = true;
+    seg = true;
   } else {
     rc = dev_radix_sort<uint64_t>(c, c->keys.as<uint64_t>(), c->keys_alt.as<uint64_t>(), c->vals.as<uint32_t>(),
                                   c->vals_alt.as<uint32_t>(), n, shifts, ns, s, &alt);
@@ -540,7 +541,7 @@ int run_broadphase(bch_context* c, uint32_t n, const uint32_t* d_world_of, const
   uint32_t* wend = nullptr;
   if (d_world_of) {
     wend = c->wend.as<uint32_t>();
-    hipLaunchKernelGGL(k_world_end, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, skeys, wend);
+    if (!seg) hipLaunchKernelGGL(k_world_end, dim3(blocks(n)), dim3(kBlock), 0, s, cnt, skeys, wend);
   }
   mark(c, 2);
   if (c->cap_mode) return broadphase_capacity(c, n, d_world_offsets, nodes, N, L, wend);

@@ -316,10 +316,12 @@ __global__ __launch_bounds__(256) void k_seg_count(const uint64_t* __restrict__ 
 // LSD-sorted in LDS by 8-bit digits (ballot ranking, as k_rs_scatter), then
 // written to their global slots: tree body k of world w to first[w] + k,
 // excluded body k to n_tree + (off[w] - first[w]) + k, where first holds
-// the scan of k_seg_count's counts and first[n_worlds] = n_tree.
+// the scan of k_seg_count's counts and first[n_worlds] = n_tree.  wend[s]
+// of a tree slot s = the first slot of a later world (first[w + 1]).
 __global__ __launch_bounds__(256) void k_seg_sort(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ off,
                                                   const uint32_t* __restrict__ first, uint32_t n_worlds, int drop,
-                                                  uint64_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+                                                  uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                  uint32_t* __restrict__ wend) {
   __shared__ uint32_t buf[2][kSegMax];
   __shared__ uint32_t running[256];
   __shared__ uint32_t wcount[4][256];
@@ -382,6 +384,7 @@ __global__ __launch_bounds__(256) void k_seg_sort(const uint64_t* __restrict__ k
     const uint32_t dest = k < rw ? f + k : nt + (o0 - f) + (k - rw);
     kout[dest] = keys[body];
     vout[dest] = body;
+    if (k < rw) wend[dest] = f + rw;
   }
 }
 
