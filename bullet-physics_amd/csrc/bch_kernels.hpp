@@ -745,13 +745,12 @@ constexpr uint32_t kClsNoKernel = 7;
 // The rest of a walk class is grouped by shape types, hull size and whether
 // the centres lie within 0.85 of the radii (deeper pairs walk longer), so the
 // lanes of a wavefront take the same support code and similar walk lengths.
-__global__ void k_split_pairs(uint32_t P, const uint32_t* __restrict__ P_dev, const uint64_t* __restrict__ keys,
-                              const uint8_t* __restrict__ btype, uint32_t* __restrict__ pairs,
-                              uint32_t* __restrict__ cls, uint32_t* __restrict__ iota,
-                              const float4* __restrict__ bsph = nullptr, uint32_t* __restrict__ ccount = nullptr) {
-  P = dev_count(P, P_dev);
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
+// One pair of the split: writes pairs[2p..], cls[p] (and ccount[p] = 0 for
+// a skipped walk pair); returns the class word.
+__device__ __forceinline__ uint32_t split_pair(uint32_t p, const uint64_t* __restrict__ keys,
+                                               const uint8_t* __restrict__ btype, uint32_t* __restrict__ pairs,
+                                               uint32_t* __restrict__ cls, const float4* __restrict__ bsph,
+                                               uint32_t* __restrict__ ccount) {
   uint64_t k = keys[p];
   uint32_t i = (uint32_t)(k >> 32), j = (uint32_t)k;
   pairs[2 * p] = i;
@@ -784,8 +783,45 @@ __global__ void k_split_pairs(uint32_t P, const uint32_t* __restrict__ P_dev, co
       sub = (ta3 << 2) | tb3 | (deep ? 0x10u : 0u);
     }
   }
-  cls[p] = c | (((sort_c << 5) | sub) << 8);
+  const uint32_t w = c | (((sort_c << 5) | sub) << 8);
+  cls[p] = w;
+  return w;
+}
+
+__global__ void k_split_pairs(uint32_t P, const uint32_t* __restrict__ P_dev, const uint64_t* __restrict__ keys,
+                              const uint8_t* __restrict__ btype, uint32_t* __restrict__ pairs,
+                              uint32_t* __restrict__ cls, uint32_t* __restrict__ iota,
+                              const float4* __restrict__ bsph = nullptr, uint32_t* __restrict__ ccount = nullptr) {
+  P = dev_count(P, P_dev);
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  (void)split_pair(p, keys, btype, pairs, cls, bsph, ccount);
   iota[p] = p;
+}
+
+// The split of one class-sort tile of ROUNDS x 256 pairs per block, with the
+// tile's histogram of the class digit (bits 8-15 of the class word) written
+// as k_rs_hist writes it ([256][tiles]), so the class sort skips its own
+// histogram pass.  The sort takes the pair index itself as the value.
+template <int ROUNDS>
+__global__ __launch_bounds__(256) void k_split_pairs_hist(uint32_t P, const uint32_t* __restrict__ P_dev,
+                                                         const uint64_t* __restrict__ keys,
+                                                         const uint8_t* __restrict__ btype,
+                                                         uint32_t* __restrict__ pairs, uint32_t* __restrict__ cls,
+                                                         const float4* __restrict__ bsph,
+                                                         uint32_t* __restrict__ ccount, uint32_t tiles,
+                                                         uint32_t* __restrict__ hist) {
+  P = dev_count(P, P_dev);
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0u;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * (256 * ROUNDS);
+  for (int r = 0; r < ROUNDS; ++r) {
+    const uint64_t p = base + (uint64_t)r * 256 + threadIdx.x;
+    if (p < P) atomicAdd(&h[(split_pair((uint32_t)p, keys, btype, pairs, cls, bsph, ccount) >> 8) & 0xffu], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
 }
 
 // Input check of the static batch: shape types, hull vertex ranges and pair

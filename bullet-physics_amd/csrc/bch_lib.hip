@@ -181,6 +181,7 @@ struct bch_context {
   int epa_wave_faces = 256;  // face store the one-pair EPA may fill before the overflow route
   uint32_t trav_block = 256;  // threads per tree-walk block (64, 128, 256, 512)
   uint32_t sort_tile = 2048;  // items per radix-sort tile (2048 or 4096; the static batch keeps 4096)
+  bool cls_hist_ready = false;  // the split has written the class sort's tile histograms
   uint64_t tree_cap = 0;      // pair buffer of the eager pass, sized by earlier passes
   // capacities of the next eager pass's device-count run (0: none yet)
   uint64_t eager_P_cap = 0, eager_C_cap = 0;
@@ -327,9 +328,10 @@ int dev_scan(bch_context* c, const uint32_t* in, uint32_t* out, uint64_t n, DevB
 // and (k1, v1); *in_second tells which pair holds the result.
 template <class K, int ROUNDS>
 int radix_pass(bch_context* c, K* kin, K* kout, uint32_t* vin, uint32_t* vout, bool vals, uint64_t n,
-               const uint32_t* n_dev, int shift, int pack, uint32_t tiles, hipStream_t s) {
-  hipLaunchKernelGGL((k_rs_hist<K, ROUNDS>), dim3(tiles), dim3(kRsThreads), 0, s, kin, (uint32_t)n, n_dev, shift, pack,
-                     tiles, c->rs_hist.as<uint32_t>());
+               const uint32_t* n_dev, int shift, int pack, uint32_t tiles, hipStream_t s, bool hist_ready = false) {
+  if (!hist_ready)
+    hipLaunchKernelGGL((k_rs_hist<K, ROUNDS>), dim3(tiles), dim3(kRsThreads), 0, s, kin, (uint32_t)n, n_dev, shift,
+                       pack, tiles, c->rs_hist.as<uint32_t>());
   uint32_t* offs = c->rs_offs.as<uint32_t>();
   uint32_t* rowtot = offs + (size_t)256 * tiles;
   hipLaunchKernelGGL(k_rs_row_scan, dim3(256), dim3(kScanThreads), 0, s, c->rs_hist.as<uint32_t>(), tiles, offs,
@@ -347,9 +349,12 @@ int radix_pass(bch_context* c, K* kin, K* kout, uint32_t* vin, uint32_t* vout, b
 // 8-bit digits starting at the given bit shifts.  Ping-pongs between (k0, v0)
 // and (k1, v1); *in_second tells which pair holds the result.  The tile
 // (items per histogram and scatter block) is the context's sort_tile knob.
+// With index_vals the values start as the items' indices (v0 is not read);
+// with hist_ready the first pass's tile histograms are already in rs_hist.
 template <class K>
 int dev_radix_sort(bch_context* c, K* k0, K* k1, uint32_t* v0, uint32_t* v1, uint64_t n, const int* shifts,
-                   int nshift, hipStream_t s, bool* in_second, const uint32_t* n_dev = nullptr, int pack = 0) {
+                   int nshift, hipStream_t s, bool* in_second, const uint32_t* n_dev = nullptr, int pack = 0,
+                   bool index_vals = false, bool hist_ready = false) {
   *in_second = false;
   if (n <= 1 || nshift == 0) return BCH_OK;
   if (n > 0xffffffffull) return fail(c, BCH_INVALID_ARG, "sort longer than 2^32 - 1");
@@ -359,15 +364,18 @@ int dev_radix_sort(bch_context* c, K* k0, K* k1, uint32_t* v0, uint32_t* v1, uin
   ENSURE(c, c->rs_offs, ((size_t)256 * tiles + 256) * sizeof(uint32_t));  // offsets, then the digit totals
   K* kin = k0;
   K* kout = k1;
-  uint32_t* vin = v0;
+  uint32_t* vin = index_vals ? nullptr : v0;
   uint32_t* vout = v1;
+  const bool vals = v0 != nullptr || index_vals;
   for (int p = 0; p < nshift; ++p) {
+    const bool hr = hist_ready && p == 0;
     int rc = tile == 2048
-                 ? radix_pass<K, 8>(c, kin, kout, vin, vout, v0 != nullptr, n, n_dev, shifts[p], pack, tiles, s)
-                 : radix_pass<K, kRsRounds>(c, kin, kout, vin, vout, v0 != nullptr, n, n_dev, shifts[p], pack, tiles, s);
+                 ? radix_pass<K, 8>(c, kin, kout, vin, vout, vals, n, n_dev, shifts[p], pack, tiles, s, hr)
+                 : radix_pass<K, kRsRounds>(c, kin, kout, vin, vout, vals, n, n_dev, shifts[p], pack, tiles, s, hr);
     if (rc) return rc;
     std::swap(kin, kout);
-    std::swap(vin, vout);
+    vin = vout;
+    vout = vin == v1 ? v0 : v1;
   }
   *in_second = (nshift & 1) != 0;
   return BCH_OK;
@@ -625,10 +633,26 @@ int narrow_split(bch_context* c, const float4* bsph = nullptr) {
   // capacity mode: P is the capacity, the device holds the count
   const uint32_t* P_dev = c->cap_mode ? &cnt->n_pairs : nullptr;
   hipLaunchKernelGGL(k_reset_epa, dim3(1), dim3(64), 0, s, cnt);
-  if (P)
+  c->cls_hist_ready = false;
+  if (P > 1) {
+    // the split also counts the class sort's first (only) digit per tile
+    const uint32_t tile = c->sort_tile == 2048 ? 2048u : (uint32_t)kRsTile;
+    const uint32_t tiles = (uint32_t)((P + tile - 1) / tile);
+    ENSURE(c, c->rs_hist, (size_t)256 * tiles * sizeof(uint32_t));
+    if (tile == 2048)
+      hipLaunchKernelGGL(k_split_pairs_hist<8>, dim3(tiles), dim3(256), 0, s, (uint32_t)P, P_dev,
+                         c->pair_keys.as<uint64_t>(), c->btype.as<uint8_t>(), c->pairs.as<uint32_t>(),
+                         c->cls.as<uint32_t>(), bsph, c->ccount.as<uint32_t>(), tiles, c->rs_hist.as<uint32_t>());
+    else
+      hipLaunchKernelGGL(k_split_pairs_hist<kRsRounds>, dim3(tiles), dim3(256), 0, s, (uint32_t)P, P_dev,
+                         c->pair_keys.as<uint64_t>(), c->btype.as<uint8_t>(), c->pairs.as<uint32_t>(),
+                         c->cls.as<uint32_t>(), bsph, c->ccount.as<uint32_t>(), tiles, c->rs_hist.as<uint32_t>());
+    c->cls_hist_ready = true;
+  } else if (P) {
     hipLaunchKernelGGL(k_split_pairs, dim3(blocks(P)), dim3(kBlock), 0, s, (uint32_t)P, P_dev,
                        c->pair_keys.as<uint64_t>(), c->btype.as<uint8_t>(), c->pairs.as<uint32_t>(),
                        c->cls.as<uint32_t>(), c->perm.as<uint32_t>(), bsph, c->ccount.as<uint32_t>());
+  }
   return BCH_OK;
 }
 
@@ -646,8 +670,13 @@ int narrow_finish(bch_context* c, const float* d_seeds, bool dense = true) {
     // leaves the result in the alt pair
     int cs[1] = {8};
     bool calt = false;
+    // the values are the pair indices (the split writes no identity
+    // permutation), and the split has counted the digit unless something
+    // rewrote the classes since
     int rc = dev_radix_sort<uint32_t>(c, c->cls.as<uint32_t>(), c->cls_alt.as<uint32_t>(), c->perm.as<uint32_t>(),
-                                      c->perm_alt.as<uint32_t>(), P, cs, 1, s, &calt, P_dev);
+                                      c->perm_alt.as<uint32_t>(), P, cs, 1, s, &calt, P_dev, 0, true,
+                                      c->cls_hist_ready);
+    c->cls_hist_ready = false;
     if (rc) return rc;
     if (!calt) {  // P <= 1 (eager pass only): nothing moved
       HIP_TRY(c, hipMemcpyAsync(c->cls_alt.p, c->cls.p, P * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
@@ -2021,6 +2050,7 @@ int bch_world_manifold_pass(bch_context* c, float aabb_margin, float linear_slop
                        warm_start, c->cls.as<uint32_t>(), c->ccount.as<uint32_t>(), c->mseeds.as<float>(),
                        c->mroute.as<uint8_t>(), c->msrc.as<uint32_t>(), c->mvalid[nw].as<uint8_t>(),
                        c->ment[nw].as<float4>(), c->mcnt.as<uint32_t>(), c->mslots.as<float4>());
+  c->cls_hist_ready = false;  // k_mf_pre rewrote classes: the class sort counts them itself
   rc = narrow_finish(c, c->mseeds.as<float>());
   if (rc) return rc;
   if (P)

@@ -251,7 +251,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(const K* __restrict__
     bool valid = i < n;
     K key = valid ? kin[i] : (K)0;
     uint32_t val = 0u;
-    if (VALS && valid) val = vin[i];
+    if (VALS && valid) val = vin ? vin[i] : (uint32_t)i;  // no vin: the item's index
     uint32_t d = rs_digit(key, shift, pack);
     // lanes of this wavefront holding the same digit
     uint64_t peers = __ballot(valid);
