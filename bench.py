@@ -547,7 +547,7 @@ def dropin_collide_batch(bpa, sc, pairs, steps):
 
 
 def gradient_check(bpa, oracle_mod, n_worlds=16, directions=16, h=1e-4):
-    """The de-penetration gradient (worlds of the configs[4] distributions) against
+    """The de-penetration gradient (worlds of the de-penetration scene's distributions) against
     f64 central differences of the same penalty evaluated by the oracle (its own
     broadphase and f64 signed distance), along directions that move three bodies."""
     import torch
